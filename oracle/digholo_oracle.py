@@ -1,0 +1,456 @@
+"""CPU restatement of digHolo's batch-processing path (FP64 oracle).
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker for the HIP path: only
+``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import it.  It is never the thing measured or shipped, and the
+product library has no code path that reaches it.
+
+What it restates (all citations are digHolo.cpp unless noted):
+
+* FFT-window placement ............................ :7267-7391
+* Fourier/camera axes ............................. :2841-2883, :7726-7731
+* IFFT-window bounds (low-res, r2c) ............... :7868-8048
+* Fourier-window copy (mask, fill-factor, phase) .. :1808-2122, :8558-8700
+* inverse FFT of the window ....................... :7533-7582, :8717-8735
+* reference wave (tilt, defocus, centre) .......... :10201-10504
+* tilt removal (V = -conj(E * ref)) ............... :2701-2837
+* HG indices and FP64 Hermite basis ............... :1245-1484, :10044-10645
+* separable overlap c = sum_y HGy sum_x HGx V ...... :2149-2341, :9348-9467
+* power-meter metrics (IL, MDL, diag, SNR) ........ :10714-11129
+
+Numerical contract
+------------------
+Discrete decisions (window sizes and corners, the circular Fourier mask) are
+evaluated in float32 exactly as the reference evaluates them, so that the HIP
+path and this oracle pick the same pixels.  Every continuous quantity (DFTs,
+phases, Hermite functions, sinc envelope, normalisation) is evaluated in
+float64 from the float32 user inputs.  The reference's int16 storage of fields,
+reference wave and modes (:2820-2832, :10490-10497, :1444-1471) is *not*
+reproduced: the output here is the exact value those int16 paths approximate
+(their own error is ~6e-5 relative L2).  ``emulate_int16=True`` adds the int16
+field quantisation of ``applyTilt16`` for error-budget tests.
+
+Pinning: every function here is checked in ``tests/test_oracle.py`` against
+closed forms or brute force (direct DFT, Hermite polynomials, discrete
+orthonormality, known-answer hologram recovery, Parseval, numpy.linalg SVD).
+The reference publishes no golden vectors and cannot be compiled here (its
+FFTW/MKL headers are absent), so nothing is pinned against reference output.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List, Tuple
+
+import numpy as np
+
+F32 = np.float32
+PI_F32 = F32(3.14159265358979323846)          # digHolo.cpp:93 (float pi)
+UNIT_ANGLE_F32 = F32(PI_F32 / F32(180.0))     # digHolo.h:192 (pi/180.0f)
+PIXEL_QUANTA = 16                              # digHolo.cpp:99
+ALIGN_FLOATS = 16                              # digHolo.cpp:117 (ALIGN 64 / sizeof(float))
+
+TILTX, TILTY, DEFOCUS = 0, 1, 2
+
+
+@dataclass
+class Config:
+    """Subset of ``digHoloConfig`` (digHolo.cpp:3323-3425) that the path reads.
+
+    Defaults follow the reference constructor, except ``resolution_mode`` which
+    callers set to 1 (low-res IFFT window) for this path.
+    """
+    frame_width: int = 320
+    frame_height: int = 256
+    pixel_size: float = 20e-6
+    pol_count: int = 1
+    fft_window_x: int = 0
+    fft_window_y: int = 0
+    aperture_deg: float = 0.4
+    resolution_mode: int = 1
+    wavelength: float = 1565e-9
+    beam_centre_x: List[float] = field(default_factory=lambda: [0.0, 0.0])
+    beam_centre_y: List[float] = field(default_factory=lambda: [0.0, 0.0])
+    tilt_x: List[float] = field(default_factory=lambda: [-0.75, -0.75])
+    tilt_y: List[float] = field(default_factory=lambda: [0.75, 0.75])
+    defocus: List[float] = field(default_factory=lambda: [0.0, 0.0])
+    waist: List[float] = field(default_factory=lambda: [510.0, 510.0])
+    group_count: int = 0
+    fill_factor: int = 1
+    pol_lock_tilt: int = 0
+    pol_lock_defocus: int = 0
+    pol_lock_waist: int = 1
+
+
+# ---------------------------------------------------------------------------
+# Geometry (float32 mimicry of the reference's discrete decisions)
+# ---------------------------------------------------------------------------
+
+def _round_f32(v) -> int:
+    """C ``round`` on a float: half away from zero."""
+    v = float(v)
+    return int(math.floor(v + 0.5)) if v >= 0 else -int(math.floor(-v + 0.5))
+
+
+def fft_window(cfg: Config) -> Tuple[int, int, List[int], List[int]]:
+    """FFT window size and per-pol corner.  digHolo.cpp:7283-7377."""
+    fw = cfg.fft_window_x if cfg.fft_window_x > 0 else cfg.frame_width // cfg.pol_count
+    fh = cfg.fft_window_y if cfg.fft_window_y > 0 else cfg.frame_height
+    W = PIXEL_QUANTA * (fw // PIXEL_QUANTA)
+    H = PIXEL_QUANTA * (fh // PIXEL_QUANTA)
+    W = max(W, 16)
+    H = max(H, 16)
+    W = min(W, cfg.frame_width)
+    H = min(H, cfg.frame_height)
+    px = F32(cfg.pixel_size)
+    cxs, cys = [], []
+    for p in range(cfg.pol_count):
+        bcx = F32(cfg.beam_centre_x[p])
+        bcy = F32(cfg.beam_centre_y[p])
+        CX = F32(bcx / px) + F32(cfg.frame_width // 2)
+        CY = F32(bcy / px) + F32(cfg.frame_height // 2)
+        cx = _round_f32(F32(F32(CX - F32(W // 2)) / F32(ALIGN_FLOATS))) * ALIGN_FLOATS
+        cy = _round_f32(F32(CY - F32(H // 2)))
+        cx = max(cx, 0)
+        cy = max(cy, 0)
+        if cx + W > cfg.frame_width:
+            cx = cfg.frame_width - W
+        if cy + H > cfg.frame_height:
+            cy = cfg.frame_height - H
+        cxs.append(cx)
+        cys.append(cy)
+    return W, H, cxs, cys
+
+
+def k_axis_f32(n: int, pixel_size: float) -> np.ndarray:
+    """Fourier axis in FFT order as the reference stores it (float32).
+
+    kFactor = (float)(2*(2*pi/(2*px))/n) (digHolo.cpp:7726-7729);
+    k[i] = kFactor*i for i <= n/2 else kFactor*(i-n) (:2843-2867).
+    """
+    px = F32(pixel_size)
+    pixel_period = F32(2.0 * float(px))
+    kf = F32(2.0 * (2.0 * math.pi / float(pixel_period)) / n)
+    i = np.arange(n)
+    idx = np.where(i <= n // 2, i, i - n).astype(np.float32)
+    return (kf * idx).astype(np.float32)
+
+
+def tilt_rad_f32(deg: float) -> np.float32:
+    return F32(F32(deg) * UNIT_ANGLE_F32)
+
+
+def ifft_window(cfg: Config, W: int, H: int) -> Tuple[int, int, List[int], List[int]]:
+    """IFFT window size and per-pol corner (low-res, r2c).  digHolo.cpp:7868-8048."""
+    lam = F32(cfg.wavelength)
+    k0max = F32(F32(2.0) * PI_F32 / lam)
+    k0min = k0max
+    kx = k_axis_f32(W, cfg.pixel_size)
+    ky = k_axis_f32(H, cfg.pixel_size)
+    dkx = F32(kx[1] - kx[0])
+    dky = F32(ky[1] - ky[0])
+    r0 = F32(F32(cfg.aperture_deg) * UNIT_ANGLE_F32)
+    wc, hc = [], []
+    w = 0
+    h = 0
+    for p in range(cfg.pol_count):
+        pp = 0 if cfg.pol_lock_tilt else p
+        tx = tilt_rad_f32(cfg.tilt_x[pp])
+        ty = tilt_rad_f32(cfg.tilt_y[pp])
+        maxEX = F32(float(F32(F32(tx + r0) * k0max) / dkx) + W / 2.0)
+        maxEY = F32(float(F32(F32(ty + r0) * k0max) / dky) + H / 2.0)
+        minSX = F32(float(F32(F32(tx - r0) * k0min) / dkx) + W / 2.0)
+        minSY = F32(float(F32(F32(ty - r0) * k0min) / dky) + H / 2.0)
+        wc.append(_round_f32(float(F32(maxEX + minSX)) / 4.0) * 2)
+        hc.append(_round_f32(float(F32(maxEY + minSY)) / 4.0) * 2)
+        wp = int(F32(maxEX - minSX + F32(1.0)))
+        hp = int(F32(maxEY - minSY + F32(1.0)))
+        wp = int(16 * math.ceil(float(F32(wp)) / 16))
+        hp = int(16 * math.ceil(float(F32(hp)) / 16))
+        w = max(w, wp)
+        h = max(h, hp)
+    w = max(w, 16)
+    h = max(h, 16)
+    SX, SY = [], []
+    for p in range(cfg.pol_count):
+        sx = int(wc[p] - w / 2.0)
+        if sx < W // 2:
+            sx = W // 2
+        if sx < 0:
+            sx = 0
+        if sx + w > W:
+            sx = W - w
+        sy = int(hc[p] - h / 2.0)
+        SX.append(sx)
+        SY.append(sy)
+    return w, h, SX, SY
+
+
+@dataclass
+class Plan:
+    W: int
+    H: int
+    cx: List[int]
+    cy: List[int]
+    w: int
+    h: int
+    SX: List[int]
+    SY: List[int]
+
+
+def apply_pol_locks(cfg: Config) -> Config:
+    """Tilt/defocus locks copy pol 0's values into pol 1 (digHolo.cpp:10296-10318)."""
+    import copy
+    c = copy.deepcopy(cfg)
+    if c.pol_lock_tilt:
+        c.tilt_x[1] = c.tilt_x[0]
+        c.tilt_y[1] = c.tilt_y[0]
+    if c.pol_lock_defocus:
+        c.defocus[1] = c.defocus[0]
+    return c
+
+
+def make_plan(cfg: Config) -> Plan:
+    W, H, cx, cy = fft_window(cfg)
+    w, h, SX, SY = ifft_window(cfg, W, H)
+    return Plan(W, H, cx, cy, w, h, SX, SY)
+
+
+# ---------------------------------------------------------------------------
+# Hermite-Gaussian basis (FP64)
+# ---------------------------------------------------------------------------
+
+def hg_indices(G: int) -> Tuple[np.ndarray, np.ndarray]:
+    """Mode enumeration: group g, k=0..g -> (m=g-k, n=k).  digHolo.cpp:1245-1262."""
+    M, N = [], []
+    for g in range(G):
+        for k in range(g + 1):
+            M.append(g - k)
+            N.append(k)
+    return np.array(M, dtype=np.int64), np.array(N, dtype=np.int64)
+
+
+def hermite_functions(nmax: int, u: np.ndarray) -> np.ndarray:
+    """Normalised Hermite functions psi_n(u), n < nmax (FP64, three-term recurrence).
+
+    psi_0 = pi^-1/4 exp(-u^2/2); psi_1 = sqrt(2) u psi_0;
+    psi_n = sqrt(2/n) u psi_{n-1} - sqrt((n-1)/n) psi_{n-2}   (digHolo.cpp:1327-1376).
+    Mantissa/exponent are tracked separately (the reference's trick, :1378-1415)
+    so that nothing under- or overflows at high order.
+    """
+    u = np.asarray(u, dtype=np.float64)
+    out = np.zeros((nmax,) + u.shape)
+    if nmax == 0:
+        return out
+    # log-magnitude bookkeeping: psi = mant * 2**expo * exp(-u^2/2)
+    g = -0.5 * u * u
+    m0 = np.full(u.shape, math.pi ** -0.25)
+    e0 = np.zeros(u.shape)
+    out[0] = m0 * np.exp(g)
+    if nmax == 1:
+        return out
+    m1 = math.sqrt(2.0) * math.pi ** -0.25 * u
+    e1 = np.zeros(u.shape)
+    out[1] = m1 * np.exp(g)
+    for n in range(2, nmax):
+        a = math.sqrt(2.0 / n)
+        b = math.sqrt((n - 1.0) / n)
+        # bring the older term to the exponent of the newer one
+        mn = a * u * m1 - b * m0 * np.exp2(e0 - e1)
+        en = e1.copy()
+        fm, fe = np.frexp(mn)
+        mn = fm
+        en = en + fe
+        m0, e0 = m1, e1
+        m1, e1 = mn, en
+        with np.errstate(over="ignore", under="ignore"):
+            out[n] = mn * np.exp(g + en * math.log(2.0))
+    return out
+
+
+def hg_modes_1d(nmax: int, x: np.ndarray, waist: float, dx: float) -> np.ndarray:
+    """HG mode profile psi_n(sqrt(2) x / w) * sqrt(dx * sqrt(2) / w).  digHolo.cpp:1272-1451."""
+    u = math.sqrt(2.0) * np.asarray(x, dtype=np.float64) / waist
+    return hermite_functions(nmax, u) * math.sqrt(dx * math.sqrt(2.0) / waist)
+
+
+# ---------------------------------------------------------------------------
+# The processing path
+# ---------------------------------------------------------------------------
+
+@dataclass
+class PolTables:
+    """Per-polarisation quantities that do not depend on the frame."""
+    col: np.ndarray        # [w] column of the r2c plane each window column reads
+    row: np.ndarray        # [h] row of the r2c plane each window row reads
+    factor: np.ndarray     # [h, w] complex mask * sinc * phase * norm
+    ref: np.ndarray        # [h, w] complex reference wave exp(i(phix+phiy))
+    hgx: np.ndarray        # [G, w]
+    hgy: np.ndarray        # [G, h]
+
+
+def _pol_index(cfg: Config, p: int) -> Tuple[int, int]:
+    """Which pol's tilt/defocus the reference wave uses (digHolo.cpp:10289-10325).
+
+    With PolLockDefocus the reference overwrites ``polIDX`` with 0 and then
+    reads tilt, tilt offset and defocus through it; with PolLockTilt only the
+    tilt is copied from pol 0.
+    """
+    tilt_p = 0 if (cfg.pol_lock_tilt or cfg.pol_lock_defocus) else p
+    idx_p = 0 if cfg.pol_lock_defocus else p
+    return tilt_p, idx_p
+
+
+def pol_tables(cfg: Config, plan: Plan, p: int) -> PolTables:
+    W, H, w, h = plan.W, plan.H, plan.w, plan.h
+    px32 = F32(cfg.pixel_size)
+    px = float(px32)
+    lam32 = F32(cfg.wavelength)
+    lam = float(lam32)
+
+    # --- Fourier-window copy, low-res r2c branch (digHolo.cpp:1845-2065) ---
+    kxa = k_axis_f32(W, cfg.pixel_size)
+    kya = k_axis_f32(H, cfg.pixel_size)
+    SX0 = plan.SX[p] - W // 2
+    col = SX0 + np.arange(w)
+    j = np.arange(h)
+    jSY = j + plan.SY[p]
+    row = np.mod(jSY + H // 2, H)
+    # float32 mask exactly as the reference evaluates it (:1950-2003, :8647-8656).
+    # The mask centre uses this pol's own tilt (zernCoefs[polIdx], :8620-8621).
+    k0_32 = F32(F32(2.0) * PI_F32 / lam32)
+    tx32 = tilt_rad_f32(cfg.tilt_x[p])
+    ty32 = tilt_rad_f32(cfg.tilt_y[p])
+    r32 = F32(F32(cfg.aperture_deg) * UNIT_ANGLE_F32)
+    kx0 = F32(k0_32 * F32(math.sin(float(tx32))))
+    ky0 = F32(k0_32 * F32(math.sin(float(ty32))))
+    kr = F32(k0_32 * F32(math.sin(float(r32))))
+    r02 = F32(kr * kr)
+    full_shift = F32(abs(F32(F32(kya[1] - kya[0]) * F32(H))))
+    shift = np.where(jSY < 0, full_shift, np.where(jSY > H, -full_shift, F32(0))).astype(np.float32)
+    kx32 = kxa[col]
+    ky32 = kya[row]
+    dkx = (kx32 - kx0).astype(np.float32)
+    dky = (ky32 - (ky0 + shift).astype(np.float32)).astype(np.float32)
+    dr2 = (dkx * dkx)[None, :] + (dky * dky)[:, None]
+    inside = (dr2.astype(np.float32) <= r02)
+
+    # continuous parts in FP64
+    dk_x = 2.0 * math.pi / (px * W)
+    dk_y = 2.0 * math.pi / (px * H)
+    kx = col.astype(np.float64) * dk_x                      # col in [0, W/2)
+    ky = np.where(row <= H // 2, row, row - H).astype(np.float64) * dk_y
+    cx0 = (plan.cx[p] + W // 2) - cfg.frame_width // 2
+    cy0 = (plan.cy[p] + H // 2) - cfg.frame_height // 2
+    bcx = float(F32(cfg.beam_centre_x[p])) - cx0 * px
+    bcy = float(F32(cfg.beam_centre_y[p])) - cy0 * px
+    if cfg.fill_factor:
+        def xcsc(t):
+            t = np.asarray(t, dtype=np.float64)
+            out = np.ones_like(t)
+            nz = t != 0
+            out[nz] = t[nz] / np.sin(t[nz])
+            return out
+        sx = xcsc(kx * px / 2.0)
+        sy = xcsc(ky * px / 2.0)
+    else:
+        sx = np.ones(w)
+        sy = np.ones(h)
+    norm = 1.0 / math.sqrt(float(W) * H * w * h)
+    fx = sx * np.exp(1j * kx * bcx)
+    fy = sy * np.exp(1j * ky * bcy)
+    factor = inside * (fy[:, None] * fx[None, :]) * norm
+
+    # --- reference wave (digHolo.cpp:10289-10497), FP64 phase ---
+    tp, ip = _pol_index(cfg, p)
+    k0 = 2.0 * math.pi / lam
+    thx = float(F32(cfg.tilt_x[tp])) * math.pi / 180.0
+    thy = float(F32(cfg.tilt_y[tp])) * math.pi / 180.0
+    thoff_x = (plan.SX[ip] - W / 2.0) * dk_x
+    thoff_y = (plan.SY[ip] - H / 2.0) * dk_y
+    D = float(F32(cfg.defocus[ip]))
+    pxx = px * W / w
+    pxy = px * H / h
+    x = (np.arange(w) - w // 2) * pxx
+    y = (np.arange(h) - h // 2) * pxy
+    kxr = -k0 * thx + thoff_x
+    kyr = -k0 * thy + thoff_y
+    focal = -math.pi * D / lam
+    phix = kxr * x + float(F32(cfg.beam_centre_x[p])) * (-k0 * thx) + focal * x * x
+    phiy = kyr * y + float(F32(cfg.beam_centre_y[p])) * (-k0 * thy) + focal * y * y
+    ref = np.exp(1j * phiy)[:, None] * np.exp(1j * phix)[None, :]
+
+    # --- HG basis (digHolo.cpp:10506-10645) ---
+    G = cfg.group_count
+    wp = float(F32(cfg.waist[0 if cfg.pol_lock_waist else p]))
+    if wp == 0:
+        wp = float(F32(400e-6))
+    hgx = hg_modes_1d(G, x, wp, pxx)
+    hgy = hg_modes_1d(G, y, wp, pxy)
+    return PolTables(col=col, row=row, factor=factor, ref=ref, hgx=hgx, hgy=hgy)
+
+
+def fft_plane(window: np.ndarray) -> np.ndarray:
+    """Unnormalised forward 2-D R2C DFT (FFTW r2c convention, digHolo.cpp:7502)."""
+    return np.fft.rfft2(np.asarray(window, dtype=np.float64))
+
+
+def process_batch(frames: np.ndarray, cfg: Config, emulate_int16: bool = False):
+    """Full ``digHoloProcessBatch`` path.
+
+    frames: [B, frameH, frameW] (uint16 or float).  Returns (fields, coefs, plan)
+    with fields [B, pol, h, w] complex128 and coefs [B, pol, M] complex128.
+    """
+    frames = np.asarray(frames)
+    B = frames.shape[0]
+    cfg = apply_pol_locks(cfg)
+    plan = make_plan(cfg)
+    P = cfg.pol_count
+    G = cfg.group_count
+    Mi, Ni = hg_indices(G)
+    tabs = [pol_tables(cfg, plan, p) for p in range(P)]
+    fields = np.zeros((B, P, plan.h, plan.w), dtype=np.complex128)
+    coefs = np.zeros((B, P, len(Mi)), dtype=np.complex128)
+    for p in range(P):
+        t = tabs[p]
+        win = frames[:, plan.cy[p]:plan.cy[p] + plan.H, plan.cx[p]:plan.cx[p] + plan.W]
+        F = np.fft.rfft2(win.astype(np.float64), axes=(-2, -1))
+        Fw = F[:, t.row[:, None], t.col[None, :]] * t.factor[None]
+        E = np.fft.ifft2(Fw, axes=(-2, -1)) * (plan.w * plan.h)      # unnormalised e^{+i}
+        V = -np.conj(E * t.ref[None])
+        if emulate_int16:
+            V = quantise_int16(V)
+        fields[:, p] = V
+        if G > 0:
+            T = np.einsum("mx,byx->bmy", t.hgx, V)       # stage 1: x contraction
+            C = np.einsum("ny,bmy->bmn", t.hgy, T)       # stage 2: y contraction
+            coefs[:, p] = C[:, Mi, Ni]
+    return fields, coefs, plan
+
+
+def quantise_int16(V: np.ndarray) -> np.ndarray:
+    """Per-frame int16 storage of ``applyTilt16`` (digHolo.cpp:2820-2832, :9174, :4954-4976)."""
+    out = np.empty_like(V)
+    for b in range(V.shape[0]):
+        mx = np.max(np.abs(V[b]))
+        scale = np.float32(32767.0 / mx) if mx > 0 else np.float32(0)
+        qr = np.clip(np.rint(scale * V[b].real), -32768, 32767)
+        qi = np.clip(np.rint(scale * V[b].imag), -32768, 32767)
+        inv = (1.0 / scale) if scale != 0 else 0.0
+        out[b] = (qr + 1j * qi) * inv
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Power-meter metrics (digHolo.cpp:10714-11129, :12183-12191)
+# ---------------------------------------------------------------------------
+
+METRIC_IL, METRIC_MDL, METRIC_DIAG, METRIC_SNRAVG, METRIC_DIAGBEST, METRIC_DIAGWORST, \
+    METRIC_SNRBEST, METRIC_SNRWORST, METRIC_SNRMG = range(9)
+
+
+def relative_l2(a, b) -> float:
+    a = np.asarray(a)
+    b = np.asarray(b)
+    den = np.linalg.norm(b.ravel())
+    return float(np.linalg.norm((a - b).ravel()) / den) if den > 0 else float(np.linalg.norm(a.ravel()))

@@ -1,0 +1,203 @@
+"""Pins the FP64 oracle against closed forms, brute force and known answers.
+
+The reference ships no tests or golden vectors and cannot be compiled here, so
+each oracle function is pinned by mathematics chosen so that a plausible
+mistake (dropped term, wrong sign, swapped index, wrong normalisation) fails.
+The IFFT-window sizes are the ones the survey measured by running the
+reference itself (320x256/128 -> 48, 640x512/192 -> 64, /256 -> 80,
+1024x1024/512 -> 160).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import digholo_oracle as O
+from digholo_amd import synth
+
+
+def small_cfg(**kw):
+    c = O.Config(frame_width=320, frame_height=256, pol_count=1, fft_window_x=128, fft_window_y=128,
+                 aperture_deg=0.7, tilt_x=[1.5, 1.5], tilt_y=[1.5, 1.5], group_count=10,
+                 waist=[200e-6, 200e-6], fill_factor=0)
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def recover(spec, cfg, B=2, seed=1):
+    """Fit coefs = alpha * c_true per (pol, frame); return worst residual and alpha/prediction."""
+    frames, c, s, A = synth.make_batch(spec, B, seed=seed, quantise=False)
+    fields, coefs, plan = O.process_batch(frames, cfg)
+    errs, ratios = [], []
+    for p in range(cfg.pol_count):
+        c0x = plan.SX[p] - plan.W // 2
+        c0y = plan.SY[p] - plan.H // 2
+        # closed form: V = -(-1)^(c0x+c0y) s A E_sig / px  in the HG-normalised overlap
+        pred = -((-1) ** (c0x + c0y)) * s * A[p] / spec.pixel_size
+        for b in range(B):
+            ct, cr = c[b, p], coefs[b, p]
+            alpha = np.vdot(ct, cr) / np.vdot(ct, ct)
+            errs.append(np.linalg.norm(cr - alpha * ct) / np.linalg.norm(cr))
+            ratios.append(alpha / pred)
+    return max(errs), np.array(ratios), plan
+
+
+def test_rfft2_matches_direct_dft():
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((6, 8))
+    H, W = x.shape
+    ky = np.arange(H)[:, None, None, None]
+    kx = np.arange(W // 2 + 1)[None, :, None, None]
+    yy = np.arange(H)[None, None, :, None]
+    xx = np.arange(W)[None, None, None, :]
+    direct = np.sum(x[None, None] * np.exp(-2j * np.pi * (ky * yy / H + kx * xx / W)), axis=(2, 3))
+    assert np.allclose(O.fft_plane(x), direct, atol=1e-12)
+
+
+def test_hermite_functions_closed_form():
+    u = np.linspace(-6, 6, 241)
+    psi = O.hermite_functions(21, u)
+    for n in range(21):
+        c = np.zeros(n + 1)
+        c[n] = 1.0
+        Hn = np.polynomial.hermite.hermval(u, c)
+        ref = Hn * np.exp(-u * u / 2) / math.sqrt(2.0 ** n * math.factorial(n) * math.sqrt(math.pi))
+        assert np.allclose(psi[n], ref, rtol=1e-10, atol=1e-13), n
+
+
+def test_hermite_functions_orthonormal_high_order():
+    # 80 orders on a fine grid: sum psi_m psi_n du = delta
+    u = np.linspace(-20, 20, 8001)
+    du = u[1] - u[0]
+    psi = O.hermite_functions(80, u)
+    gram = psi @ psi.T * du
+    assert np.max(np.abs(gram - np.eye(80))) < 1e-9
+    # and agrees with the plain recurrence used by the generator
+    assert np.allclose(psi, synth.hermite_functions(80, u), rtol=1e-9, atol=1e-300)
+
+
+def test_hg_indices_order():
+    M, N = O.hg_indices(4)
+    assert list(zip(M, N)) == [(0, 0), (1, 0), (0, 1), (2, 0), (1, 1), (0, 2),
+                               (3, 0), (2, 1), (1, 2), (0, 3)]
+
+
+@pytest.mark.parametrize("frame,pols,win,G,w_ifft", [
+    ((320, 256), 1, 128, 10, 48),
+    ((640, 512), 2, 192, 30, 64),
+    ((640, 512), 2, 256, 45, 80),
+    ((1024, 1024), 2, 512, 80, 160),
+])
+def test_ifft_window_sizes_match_reference_runs(frame, pols, win, G, w_ifft):
+    cfg = O.Config(frame_width=frame[0], frame_height=frame[1], pol_count=pols, fft_window_x=win,
+                   fft_window_y=win, aperture_deg=0.7, tilt_x=[1.5, 1.5], tilt_y=[1.5, 1.5],
+                   group_count=G, beam_centre_x=[-1.6e-3, 1.6e-3] if pols == 2 else [0, 0])
+    plan = O.make_plan(cfg)
+    assert (plan.w, plan.h) == (w_ifft, w_ifft)
+    assert plan.W == win and plan.H == win
+    for p in range(pols):
+        # window column range stays inside the r2c half plane (no skipped columns)
+        assert plan.SX[p] >= plan.W // 2 and plan.SX[p] + plan.w <= plan.W
+
+
+def test_fft_window_corner_rounding():
+    # corner x snaps to 16 px, y rounds to the nearest pixel, both clamp to the frame
+    cfg = small_cfg(beam_centre_x=[37e-6, 0], beam_centre_y=[-55e-6, 0])
+    W, H, cx, cy = O.fft_window(cfg)
+    assert (W, H) == (128, 128)
+    assert cx == [96]          # round((160 + 1.85 - 64)/16)*16
+    assert cy == [61]          # round(128 - 2.75 - 64)
+    cfg = small_cfg(beam_centre_x=[-5e-3, 0], beam_centre_y=[9e-3, 0])
+    W, H, cx, cy = O.fft_window(cfg)
+    assert cx == [0] and cy == [256 - 128]
+
+
+def test_known_answer_single_pol():
+    spec = synth.spec_small()
+    spec.beam_centre_x = [37e-6, 0]
+    spec.beam_centre_y = [-55e-6, 0]
+    cfg = small_cfg(beam_centre_x=[37e-6, 0], beam_centre_y=[-55e-6, 0])
+    err, ratio, _ = recover(spec, cfg, B=3)
+    assert err < 1e-5
+    assert np.allclose(ratio, 1.0, atol=1e-6)
+
+
+def test_known_answer_fill_factor_correction():
+    # pixel-integrated (100% fill factor) frames are recovered exactly only with
+    # the inverse-sinc envelope on both axes
+    spec = synth.spec_small()
+    spec.pixel_quadrature = 4
+    spec.beam_centre_x = [37e-6, 0]
+    cfg = small_cfg(beam_centre_x=[37e-6, 0], fill_factor=1)
+    err, ratio, _ = recover(spec, cfg, B=2)
+    assert err < 1e-5
+    assert np.allclose(ratio, 1.0, atol=1e-5)
+    cfg.fill_factor = 0
+    err0, ratio0, _ = recover(spec, cfg, B=1)
+    assert abs(ratio0[0]) < 0.9                    # uncorrected amplitude droops
+
+
+def test_known_answer_dual_pol_defocus_wrapped_window():
+    spec = synth.spec_dual_640()
+    spec.group_count = 12
+    spec.defocus = [0.3, -0.2]
+    spec.beam_centre_y = [40e-6, -70e-6]
+    spec.tilt_x_deg = [1.5, 1.4]
+    spec.tilt_y_deg = [1.5, -1.6]
+    cfg = O.Config(frame_width=640, frame_height=512, pol_count=2, fft_window_x=256, fft_window_y=256,
+                   aperture_deg=0.7, tilt_x=[1.5, 1.4], tilt_y=[1.5, -1.6], defocus=[0.3, -0.2],
+                   group_count=12, waist=[300e-6, 300e-6], beam_centre_x=[-1.6e-3, 1.6e-3],
+                   beam_centre_y=[40e-6, -70e-6], fill_factor=0)
+    err, ratio, plan = recover(spec, cfg, B=2)
+    assert plan.SY[1] < 0                          # pol 1 window wraps past ky = -pi/px
+    assert err < 1e-5
+    assert np.allclose(ratio, 1.0, atol=1e-4)
+
+
+def test_planted_mistakes_fail_known_answer(monkeypatch):
+    spec = synth.spec_small()
+    cfg = small_cfg()
+    # swapping the x/y mode orders breaks recovery
+    real = O.hg_indices
+    monkeypatch.setattr(O, "hg_indices", lambda G: real(G)[::-1])
+    err, _, _ = recover(spec, cfg, B=1)
+    assert err > 0.1
+    monkeypatch.setattr(O, "hg_indices", real)
+    # using the wrong sign on the window's beam-centre ramp breaks recovery
+    spec.beam_centre_x = [150e-6, 0]
+    cfg = small_cfg(beam_centre_x=[150e-6, 0])
+    base, _, _ = recover(spec, cfg, B=1)
+    assert base < 1e-5
+    cfg_bad = small_cfg(beam_centre_x=[-150e-6, 0])
+    frames, c, s, A = synth.make_batch(spec, 1, seed=1, quantise=False)
+    _, coefs, _ = O.process_batch(frames, cfg_bad)
+    ct, cr = c[0, 0], coefs[0, 0]
+    alpha = np.vdot(ct, cr) / np.vdot(ct, ct)
+    assert np.linalg.norm(cr - alpha * ct) / np.linalg.norm(cr) > 0.1
+
+
+def test_int16_emulation_error_budget():
+    # the reference stores fields as int16 with a per-frame scale; that storage
+    # alone moves fields by ~1e-5..1e-4 relative L2 on realistic frames
+    spec = synth.spec_small()
+    frames, c, s, A = synth.make_batch(spec, 4, seed=2, quantise=True)
+    cfg = small_cfg(fill_factor=1)
+    f0, c0, _ = O.process_batch(frames, cfg)
+    f1, c1, _ = O.process_batch(frames, cfg, emulate_int16=True)
+    e_f = O.relative_l2(f1, f0)
+    e_c = O.relative_l2(c1, c0)
+    assert 1e-6 < e_f < 2e-4
+    assert e_c < 2e-4
+
+
+def test_linearity():
+    spec = synth.spec_small()
+    f1, _, _, _ = synth.make_batch(spec, 2, seed=4, quantise=False)
+    f2, _, _, _ = synth.make_batch(spec, 2, seed=5, quantise=False)
+    cfg = small_cfg(fill_factor=1)
+    a, b = 0.7, -1.3
+    _, cA, _ = O.process_batch(a * f1 + b * f2, cfg)
+    _, c1, _ = O.process_batch(f1, cfg)
+    _, c2, _ = O.process_batch(f2, cfg)
+    assert O.relative_l2(cA, a * c1 + b * c2) < 1e-12
