@@ -1,0 +1,1207 @@
+// digHolo-compatible C API over the HIP batch-processing kernels.
+//
+// Host responsibilities: handle registry and configuration (validation and
+// return conventions of the reference's EXT_C wrappers, digHolo.cpp:13413-15415),
+// window geometry (digHolo.cpp:7267-7391, :7868-8048, evaluated in float32 like
+// the reference so the same pixels are chosen), the Fourier-window mask
+// (float32, :1950-2003), FFT plans and twiddle tables, device buffers, and the
+// launch sequence on the handle's HIP stream.  There is no CPU compute path.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/digHolo.h"
+#include "dh_internal.h"
+
+namespace {
+
+constexpr float PI_F = 3.14159265358979323846f;        // digHolo.cpp:93
+constexpr double PI_D = 3.14159265358979323846;
+const float UNIT_ANGLE_F = PI_F / 180.0f;               // digHolo.h:192
+constexpr int PIXEL_QUANTA = 16;                         // digHolo.cpp:99
+constexpr int POLCOUNTMAX = 2;                           // digHolo.cpp:101
+
+// Defaults of digHoloConfig (digHolo.cpp:3327-3425, :3440-3478).
+struct Config {
+  int frameWidth = 320, frameHeight = 256;
+  float pixelSize = 20e-6f;
+  float apertureSize = 0.4f;
+  int resolutionIdx = 0;
+  float waist[2] = {510.f, 510.f};
+  float tiltX[2] = {-0.75f, -0.75f};
+  float tiltY[2] = {0.75f, 0.75f};
+  float defocus[2] = {0.f, 0.f};
+  float beamCX[2] = {0.f, 0.f}, beamCY[2] = {0.f, 0.f};
+  int batchCount = 0, avgCount = 1, avgMode = 0;
+  float wavelengthCentre = 1565e-9f;
+  int polLockTilt = 0, polLockWaist = 1, polLockDefocus = 0;
+  int polCount = 1;
+  int aaTilt = 1, aaDefocus = 1, aaCentre = 1, aaWaist = 1, aaFourierRadius = 1;
+  int aaPolIndependence = 0, aaMulConjTrans = 0;
+  float aaTol = 0.f;
+  int aaMode = 0, aaGoalIdx = 0;
+  int fillFactor = 1;
+  int fftWindowSizeX = 0, fftWindowSizeY = 0;
+  int maxMG = 0;
+  int basisType = 0;
+  int verbosity = 0;
+  int threadCount = (int)std::thread::hardware_concurrency();
+  int precision = DIGHOLO_PRECISION_FP32;
+};
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t ensure(size_t count) {
+    if (count <= n) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) n = count;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+template <typename T>
+struct PinnedBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t ensure(size_t count) {
+    if (count <= n) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+    hipError_t e = hipHostMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T), hipHostMallocDefault);
+    if (e == hipSuccess) n = count;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+struct FFTEntry {
+  DevBuf<float2> tw;
+  dh::FFTPlan plan{};
+};
+
+// Everything derived from the configuration that the kernels consume.
+struct Derived {
+  dh::Geometry g{};
+  dh::TableArgs tab{};
+  int SX[2]{}, SY[2]{};
+  std::vector<uint8_t> mask;
+  float pxx = 0.f, pxy = 0.f;  // float32 reconstructed pixel pitch (axes export)
+};
+
+struct Handle {
+  Config cfg;
+  const void* frames = nullptr;
+  int frameU16 = 0;
+  int transpose = 0;
+
+  bool devInit = false;
+  hipStream_t ownStream = nullptr;
+  hipStream_t stream = nullptr;
+
+  DevBuf<char> staging;
+  DevBuf<float2> rowspec, win, fields, coefs, factor, ref, twW;
+  DevBuf<float> hgx, hgy;
+  DevBuf<uint8_t> mask;
+  DevBuf<int2> modes;
+  std::map<int, std::unique_ptr<FFTEntry>> ffts;
+  int twWn = 0;
+
+  bool tablesValid = false;
+  Derived tabKey;  // derived state the device tables were built from
+  Derived cur;     // derived state of the last processed batch
+  bool haveBatch = false;
+
+  PinnedBuf<float2> hCoefs, hFields;
+  bool coefsHostValid = false, fieldsHostValid = false;
+  std::vector<float> xAxis, yAxis;
+
+  bool stageTiming = false;
+  hipEvent_t ev[6] = {};
+  float stageMs[5] = {};
+};
+
+std::vector<Handle*> g_handles;     // index = handle; nullptr once destroyed
+std::mutex g_registry;
+
+Handle* get(int idx) {
+  std::lock_guard<std::mutex> lk(g_registry);
+  if (idx < 0 || idx >= (int)g_handles.size()) return nullptr;
+  return g_handles[idx];
+}
+
+void log_err(const Handle*, const char* what, hipError_t e) {
+  fprintf(stderr, "digHolo: %s failed: %s\n", what, hipGetErrorString(e));
+}
+
+// ---------------------------------------------------------------------------
+// geometry (float32 exactly as the reference evaluates it)
+// ---------------------------------------------------------------------------
+float k_factor(int n, float px) {
+  const float pixelPeriod = (float)(2.0 * (double)px);
+  return (float)(2 * (2.0 * PI_D / pixelPeriod) / n);     // digHolo.cpp:7726-7729
+}
+float k_axis(int i, int n, float kf) {                    // digHolo.cpp:2843-2867
+  return i <= n / 2 ? kf * (float)i : kf * (float)(i - n);
+}
+
+int imod(int a, int n) {
+  int r = a % n;
+  return r < 0 ? r + n : r;
+}
+
+// Returns false (and an error code) if the configuration cannot be processed.
+bool derive(const Config& c, int B, Derived& d, int& err) {
+  err = DIGHOLO_ERROR_SUCCESS;
+  const int P = c.polCount;
+  // --- FFT window, digHolo.cpp:7283-7377 ---
+  int fwx = c.fftWindowSizeX > 0 ? c.fftWindowSizeX : c.frameWidth / P;
+  int fwy = c.fftWindowSizeY > 0 ? c.fftWindowSizeY : c.frameHeight;
+  int W = PIXEL_QUANTA * (fwx / PIXEL_QUANTA);
+  int H = PIXEL_QUANTA * (fwy / PIXEL_QUANTA);
+  W = std::max(W, 16);
+  H = std::max(H, 16);
+  W = std::min(W, c.frameWidth);
+  H = std::min(H, c.frameHeight);
+  if (W < 16 || H < 16) {
+    err = DIGHOLO_ERROR_INVALIDDIMENSION;
+    return false;
+  }
+  const float px = c.pixelSize;
+  int cx[2] = {0, 0}, cy[2] = {0, 0};
+  for (int p = 0; p < P; ++p) {
+    const float CX = (c.beamCX[p] / px) + (float)(c.frameWidth / 2);
+    const float CY = (c.beamCY[p] / px) + (float)(c.frameHeight / 2);
+    int x = (int)std::round((CX - (float)(W / 2)) / 16.0f) * 16;
+    int y = (int)std::round(CY - (float)(H / 2));
+    x = std::max(x, 0);
+    y = std::max(y, 0);
+    if (x + W > c.frameWidth) x = c.frameWidth - W;
+    if (y + H > c.frameHeight) y = c.frameHeight - H;
+    cx[p] = x;
+    cy[p] = y;
+  }
+  // --- IFFT window, digHolo.cpp:7868-8048 ---
+  const float lam = c.wavelengthCentre;
+  const float k0max = 2.0f * PI_F / lam;
+  const float k0min = k0max;
+  const float kfx = k_factor(W, px), kfy = k_factor(H, px);
+  const float dkx = k_axis(1, W, kfx) - k_axis(0, W, kfx);
+  const float dky = k_axis(1, H, kfy) - k_axis(0, H, kfy);
+  const float r0 = c.apertureSize * UNIT_ANGLE_F;
+  int w = 0, h = 0, wc[2] = {0, 0}, hc[2] = {0, 0};
+  for (int p = 0; p < P; ++p) {
+    const int pp = c.polLockTilt ? 0 : p;
+    const float tx = c.tiltX[pp] * UNIT_ANGLE_F;
+    const float ty = c.tiltY[pp] * UNIT_ANGLE_F;
+    const float maxEX = (float)((double)((tx + r0) * k0max / dkx) + W / 2.0);
+    const float maxEY = (float)((double)((ty + r0) * k0max / dky) + H / 2.0);
+    const float minSX = (float)((double)((tx - r0) * k0min / dkx) + W / 2.0);
+    const float minSY = (float)((double)((ty - r0) * k0min / dky) + H / 2.0);
+    wc[p] = (int)std::round((double)(maxEX + minSX) / 4.0) * 2;
+    hc[p] = (int)std::round((double)(maxEY + minSY) / 4.0) * 2;
+    int wp = (int)(maxEX - minSX + 1.0f);
+    int hp = (int)(maxEY - minSY + 1.0f);
+    wp = (int)(16 * std::ceil((1.0f * wp) / 16));
+    hp = (int)(16 * std::ceil((1.0f * hp) / 16));
+    w = std::max(w, wp);
+    h = std::max(h, hp);
+  }
+  w = std::max(w, 16);
+  h = std::max(h, 16);
+  if (w > W / 2 || h > H) {
+    err = DIGHOLO_ERROR_INVALIDDIMENSION;   // Fourier window larger than the r2c half plane
+    return false;
+  }
+  for (int p = 0; p < P; ++p) {
+    int sx = (int)(wc[p] - w / 2.0);
+    if (sx < W / 2) sx = W / 2;
+    if (sx < 0) sx = 0;
+    if (sx + w > W) sx = W - w;
+    d.SX[p] = sx;
+    d.SY[p] = (int)(hc[p] - h / 2.0);   // not clamped: rows wrap around ky (digHolo.cpp:8025-8036)
+  }
+  dh::Geometry& g = d.g;
+  g = dh::Geometry{};
+  g.B = B;
+  g.P = P;
+  g.frameW = c.frameWidth;
+  g.frameH = c.frameHeight;
+  g.W = W;
+  g.H = H;
+  g.w = w;
+  g.h = h;
+  for (int p = 0; p < P; ++p) {
+    g.cx[p] = cx[p];
+    g.cy[p] = cy[p];
+    g.c0[p] = d.SX[p] - W / 2;
+    g.SY[p] = d.SY[p];
+  }
+  g.G = c.maxMG;
+  g.M = g.G * (g.G + 1) / 2;
+
+  // --- Fourier-window mask in float32 (digHolo.cpp:1950-2003, :8647-8656) ---
+  d.mask.assign((size_t)P * h * w, 0);
+  const float fullShift = std::fabs((k_axis(1, H, kfy) - k_axis(0, H, kfy)) * (float)H);
+  for (int p = 0; p < P; ++p) {
+    const float k0 = 2.0f * PI_F / lam;
+    const float tx = c.tiltX[p] * UNIT_ANGLE_F;
+    const float ty = c.tiltY[p] * UNIT_ANGLE_F;
+    const float kx0 = k0 * (float)std::sin((double)tx);
+    const float ky0 = k0 * (float)std::sin((double)ty);
+    const float kr = k0 * (float)std::sin((double)r0);
+    const float r02 = kr * kr;
+    for (int j = 0; j < h; ++j) {
+      const int jSY = j + d.SY[p];
+      const int row = imod(jSY + H / 2, H);
+      const float shift = jSY < 0 ? fullShift : (jSY > H ? -fullShift : 0.0f);
+      const float dk_y = k_axis(row, H, kfy) - (ky0 + shift);
+      const float dky2 = dk_y * dk_y;
+      for (int i = 0; i < w; ++i) {
+        const int col = g.c0[p] + i;
+        const float dk_x = k_axis(col, W, kfx) - kx0;
+        const float dkx2 = dk_x * dk_x;
+        const float dr2 = dkx2 + dky2;
+        d.mask[((size_t)p * h + j) * w + i] = dr2 <= r02 ? 1 : 0;
+      }
+    }
+  }
+
+  // --- FP64 parameters of the window factor, reference wave and basis ---
+  dh::TableArgs& t = d.tab;
+  t = dh::TableArgs{};
+  t.g = g;
+  const double pxd = (double)px;
+  const double lamd = (double)lam;
+  const double k0d = 2.0 * PI_D / lamd;
+  for (int p = 0; p < P; ++p) {
+    dh::PolParams& q = t.pp[p];
+    q.dkx = 2.0 * PI_D / (pxd * W);
+    q.dky = 2.0 * PI_D / (pxd * H);
+    const int cx0 = (cx[p] + W / 2) - c.frameWidth / 2;
+    const int cy0 = (cy[p] + H / 2) - c.frameHeight / 2;
+    q.bcx = (double)c.beamCX[p] - cx0 * pxd;
+    q.bcy = (double)c.beamCY[p] - cy0 * pxd;
+    q.px = pxd;
+    q.norm = 1.0 / std::sqrt((double)W * H * w * h);
+    q.fill = c.fillFactor ? 1 : 0;
+    // reference wave: pol-lock index rules of digHolo.cpp:10289-10325
+    const int tp = (c.polLockTilt || c.polLockDefocus) ? 0 : p;
+    const int ip = c.polLockDefocus ? 0 : p;
+    const double thx = (double)c.tiltX[tp] * PI_D / 180.0;
+    const double thy = (double)c.tiltY[tp] * PI_D / 180.0;
+    const double thoffx = (d.SX[ip] - W / 2.0) * q.dkx;
+    const double thoffy = (d.SY[ip] - H / 2.0) * q.dky;
+    q.kxr = -k0d * thx + thoffx;
+    q.kyr = -k0d * thy + thoffy;
+    q.pistx = (double)c.beamCX[p] * (-k0d * thx);
+    q.pisty = (double)c.beamCY[p] * (-k0d * thy);
+    q.focal = -PI_D * (double)c.defocus[ip] / lamd;
+    q.pxx = pxd * W / w;
+    q.pxy = pxd * H / h;
+    float wst = c.waist[c.polLockWaist ? 0 : p];
+    if (wst == 0.f) wst = 400e-6f;
+    q.waist = (double)wst;
+  }
+  d.pxx = (px * W) / w;                 // digHolo.cpp:8184-8185 (float)
+  d.pxy = (px * H) / h;
+  return true;
+}
+
+bool same_tables(const Derived& a, const Derived& b) {
+  const dh::Geometry &x = a.g, &y = b.g;
+  if (x.P != y.P || x.W != y.W || x.H != y.H || x.w != y.w || x.h != y.h || x.G != y.G) return false;
+  for (int p = 0; p < x.P; ++p)
+    if (x.c0[p] != y.c0[p] || x.SY[p] != y.SY[p]) return false;
+  if (std::memcmp(a.tab.pp, b.tab.pp, sizeof(a.tab.pp)) != 0) return false;
+  return a.mask == b.mask;
+}
+
+// ---------------------------------------------------------------------------
+// FFT plans
+// ---------------------------------------------------------------------------
+bool factorise(int n, dh::FFTPlan& pl) {
+  pl.n = n;
+  pl.nstages = 0;
+  int m = n;
+  const int pref[] = {8, 4, 2, 3, 5};
+  for (int r : pref) {
+    while (m % r == 0) {
+      if (pl.nstages >= dh::kMaxStages) return false;
+      pl.radix[pl.nstages++] = r;
+      m /= r;
+    }
+  }
+  for (int r = 7; m > 1; r += 2) {
+    while (m % r == 0) {
+      if (pl.nstages >= dh::kMaxStages) return false;
+      pl.radix[pl.nstages++] = r;
+      m /= r;
+    }
+  }
+  return true;
+}
+
+const dh::FFTPlan* fft_plan(Handle& h, int n) {
+  auto it = h.ffts.find(n);
+  if (it != h.ffts.end()) return &it->second->plan;
+  auto e = std::make_unique<FFTEntry>();
+  if (!factorise(n, e->plan)) return nullptr;
+  std::vector<float2> tw(n);
+  for (int k = 0; k < n; ++k) {
+    const double a = -2.0 * PI_D * (double)k / n;
+    tw[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  if (e->tw.ensure(n) != hipSuccess) return nullptr;
+  if (hipMemcpy(e->tw.p, tw.data(), n * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+  e->plan.tw = e->tw.p;
+  const dh::FFTPlan* r = &e->plan;
+  h.ffts[n] = std::move(e);
+  return r;
+}
+
+bool is_device_pointer(const void* ptr) {
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, ptr) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.isManaged;
+}
+
+int ensure_device(Handle& h) {
+  if (h.devInit) return DIGHOLO_ERROR_SUCCESS;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+    (void)hipGetLastError();
+    fprintf(stderr, "digHolo: no HIP device available; this library has no CPU path\n");
+    return DIGHOLO_ERROR_ERROR;
+  }
+  if (hipStreamCreateWithFlags(&h.ownStream, hipStreamNonBlocking) != hipSuccess) return DIGHOLO_ERROR_ERROR;
+  if (!h.stream) h.stream = h.ownStream;
+  for (auto& e : h.ev)
+    if (hipEventCreate(&e) != hipSuccess) return DIGHOLO_ERROR_ERROR;
+  h.devInit = true;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+#define DH_TRY(expr, what)                 \
+  do {                                     \
+    hipError_t _e = (expr);                \
+    if (_e != hipSuccess) {                \
+      log_err(&h, what, _e);               \
+      return DIGHOLO_ERROR_ERROR;          \
+    }                                      \
+  } while (0)
+
+// Queues one full batch on the handle's stream.  On return the coefficients and
+// fields are (or will be, once the stream drains) in device memory.
+int enqueue_batch(Handle& h) {
+  Config& c = h.cfg;
+  if (!h.frames) return DIGHOLO_ERROR_NULLPOINTER;
+  if (c.batchCount < 1) c.batchCount = 1;
+  if (c.avgCount > 1) {
+    fprintf(stderr, "digHolo: frame averaging (avgCount > 1) is not implemented in this build\n");
+    return DIGHOLO_ERROR_INVALIDARGUMENT;
+  }
+  if (c.resolutionIdx != 1) {
+    fprintf(stderr, "digHolo: IFFTResolutionMode 0 (full resolution) is not implemented in this build; "
+                    "set digHoloConfigSetIFFTResolutionMode(handle, 1)\n");
+    return DIGHOLO_ERROR_INVALIDARGUMENT;
+  }
+  if (h.transpose) {
+    fprintf(stderr, "digHolo: transposed uint16 frames are not implemented in this build\n");
+    return DIGHOLO_ERROR_INVALIDARGUMENT;
+  }
+  if (c.basisType != DIGHOLO_BASISTYPE_HG && c.maxMG > 0) {
+    fprintf(stderr, "digHolo: only the HG basis is implemented in this build\n");
+    return DIGHOLO_ERROR_INVALIDARGUMENT;
+  }
+  // pol locks write pol 0's values into pol 1 (digHolo.cpp:10296-10318, :10527-10531)
+  if (c.polLockTilt) {
+    c.tiltX[1] = c.tiltX[0];
+    c.tiltY[1] = c.tiltY[0];
+  }
+  if (c.polLockDefocus) c.defocus[1] = c.defocus[0];
+  if (c.polLockWaist) c.waist[1] = c.waist[0];
+
+  Derived d;
+  int err;
+  if (!derive(c, c.batchCount, d, err)) {
+    fprintf(stderr, "digHolo: window configuration is invalid for this frame (error %d)\n", err);
+    return err;
+  }
+  const dh::Geometry& g = d.g;
+  int e0 = ensure_device(h);
+  if (e0) return e0;
+
+  const size_t B = g.B, P = g.P;
+  const size_t frameElems = B * (size_t)g.frameW * g.frameH;
+  const size_t elemBytes = h.frameU16 ? 2 : 4;
+  const void* dframes = h.frames;
+  if (!is_device_pointer(h.frames) || (reinterpret_cast<uintptr_t>(h.frames) % 8) != 0) {
+    DH_TRY(h.staging.ensure(frameElems * elemBytes), "frame staging alloc");
+    DH_TRY(hipMemcpyAsync(h.staging.p, h.frames, frameElems * elemBytes, hipMemcpyDefault, h.stream),
+           "frame upload");
+    dframes = h.staging.p;
+  }
+  DH_TRY(h.rowspec.ensure(B * P * g.H * g.w), "alloc");
+  DH_TRY(h.win.ensure(B * P * g.h * g.w), "alloc");
+  DH_TRY(h.fields.ensure(B * P * g.h * g.w), "alloc");
+  DH_TRY(h.coefs.ensure(std::max<size_t>(B * P * g.M, 1)), "alloc");
+
+  const dh::FFTPlan* pRow = fft_plan(h, g.W / 2);
+  const dh::FFTPlan* pCol = fft_plan(h, g.H);
+  const dh::FFTPlan* pIx = fft_plan(h, g.w);
+  const dh::FFTPlan* pIy = fft_plan(h, g.h);
+  if (!pRow || !pCol || !pIx || !pIy) {
+    fprintf(stderr, "digHolo: FFT plan setup failed\n");
+    return DIGHOLO_ERROR_MEMORYALLOCATION;
+  }
+  if (h.twWn != g.W) {
+    std::vector<float2> tw(g.W / 2 + 1);
+    for (int k = 0; k <= g.W / 2; ++k) {
+      const double a = -2.0 * PI_D * (double)k / g.W;
+      tw[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    DH_TRY(h.twW.ensure(tw.size()), "alloc");
+    DH_TRY(hipMemcpy(h.twW.p, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice), "twiddles");
+    h.twWn = g.W;
+  }
+
+  const bool timing = h.stageTiming;
+  if (timing) DH_TRY(hipEventRecord(h.ev[0], h.stream), "event");
+  if (!h.tablesValid || !same_tables(d, h.tabKey)) {
+    const size_t T = P * g.h * g.w;
+    DH_TRY(h.mask.ensure(T), "alloc");
+    DH_TRY(h.factor.ensure(T), "alloc");
+    DH_TRY(h.ref.ensure(T), "alloc");
+    DH_TRY(h.hgx.ensure(std::max<size_t>(P * g.G * g.w, 1)), "alloc");
+    DH_TRY(h.hgy.ensure(std::max<size_t>(P * g.G * g.h, 1)), "alloc");
+    DH_TRY(hipMemcpyAsync(h.mask.p, d.mask.data(), T, hipMemcpyHostToDevice, h.stream), "mask upload");
+    std::vector<int2> modes;
+    for (int gg = 0; gg < g.G; ++gg)
+      for (int k = 0; k <= gg; ++k) modes.push_back(make_int2(gg - k, k));   // digHolo.cpp:1252-1262
+    DH_TRY(h.modes.ensure(std::max<size_t>(modes.size(), 1)), "alloc");
+    if (!modes.empty())
+      DH_TRY(hipMemcpyAsync(h.modes.p, modes.data(), modes.size() * sizeof(int2), hipMemcpyHostToDevice,
+                            h.stream), "modes");
+    DH_TRY(dh::launch_tables(d.tab, h.mask.p, h.factor.p, h.ref.p, h.hgx.p, h.hgy.p, h.stream), "tables");
+    DH_TRY(hipStreamSynchronize(h.stream), "tables sync");   // the host vectors above go out of scope
+    h.tabKey = d;
+    h.tablesValid = true;
+  }
+  if (timing) DH_TRY(hipEventRecord(h.ev[1], h.stream), "event");
+  DH_TRY(dh::launch_row_fft(g, *pRow, h.twW.p, dframes, h.frameU16, h.rowspec.p, h.stream), "row FFT");
+  if (timing) DH_TRY(hipEventRecord(h.ev[2], h.stream), "event");
+  DH_TRY(dh::launch_col_fft_window(g, *pCol, h.rowspec.p, h.factor.p, h.win.p, h.stream), "column FFT");
+  if (timing) DH_TRY(hipEventRecord(h.ev[3], h.stream), "event");
+  DH_TRY(dh::launch_ifft_rows(g, *pIx, h.win.p, h.stream), "IFFT rows");
+  DH_TRY(dh::launch_ifft_cols_tilt(g, *pIy, h.win.p, h.ref.p, h.fields.p, h.stream), "IFFT columns");
+  if (timing) DH_TRY(hipEventRecord(h.ev[4], h.stream), "event");
+  if (g.G > 0) {
+    DH_TRY(dh::launch_overlap_fp32(g, h.fields.p, h.hgx.p, h.hgy.p, h.modes.p, h.coefs.p, h.stream), "overlap");
+  }
+  if (timing) DH_TRY(hipEventRecord(h.ev[5], h.stream), "event");
+
+  h.cur = d;
+  h.haveBatch = true;
+  h.coefsHostValid = false;
+  h.fieldsHostValid = false;
+  // reconstructed-plane axes, float32 like digHolo.cpp:8184-8189 / :2875-2880
+  h.xAxis.resize(g.w);
+  h.yAxis.resize(g.h);
+  for (int i = 0; i < g.w; ++i) h.xAxis[i] = (float)(i - g.w / 2) * d.pxx;
+  for (int j = 0; j < g.h; ++j) h.yAxis[j] = (float)(j - g.h / 2) * d.pxy;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+void collect_stage_times(Handle& h) {
+  if (!h.stageTiming) return;
+  (void)hipEventSynchronize(h.ev[5]);
+  (void)hipEventElapsedTime(&h.stageMs[0], h.ev[1], h.ev[2]);   // row FFT
+  (void)hipEventElapsedTime(&h.stageMs[1], h.ev[2], h.ev[3]);   // column FFT + window
+  (void)hipEventElapsedTime(&h.stageMs[2], h.ev[3], h.ev[4]);   // IFFT + tilt
+  (void)hipEventElapsedTime(&h.stageMs[3], h.ev[0], h.ev[1]);   // tables (0 when cached)
+  (void)hipEventElapsedTime(&h.stageMs[4], h.ev[4], h.ev[5]);   // overlap
+}
+
+int download_coefs(Handle& h) {
+  if (h.coefsHostValid) return DIGHOLO_ERROR_SUCCESS;
+  const dh::Geometry& g = h.cur.g;
+  const size_t n = (size_t)g.B * g.P * g.M;
+  DH_TRY(h.hCoefs.ensure(std::max<size_t>(n, 1)), "pinned alloc");
+  if (n) DH_TRY(hipMemcpyAsync(h.hCoefs.p, h.coefs.p, n * sizeof(float2), hipMemcpyDeviceToHost, h.stream), "D2H");
+  DH_TRY(hipStreamSynchronize(h.stream), "sync");
+  h.coefsHostValid = true;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+int download_fields(Handle& h) {
+  if (h.fieldsHostValid) return DIGHOLO_ERROR_SUCCESS;
+  const dh::Geometry& g = h.cur.g;
+  const size_t n = (size_t)g.B * g.P * g.h * g.w;
+  DH_TRY(h.hFields.ensure(n), "pinned alloc");
+  DH_TRY(hipMemcpyAsync(h.hFields.p, h.fields.p, n * sizeof(float2), hipMemcpyDeviceToHost, h.stream), "D2H");
+  DH_TRY(hipStreamSynchronize(h.stream), "sync");
+  h.fieldsHostValid = true;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+void destroy_handle(Handle* h) {
+  if (!h) return;
+  if (h->devInit) {
+    (void)hipStreamSynchronize(h->stream);
+    h->staging.release();
+    h->rowspec.release();
+    h->win.release();
+    h->fields.release();
+    h->coefs.release();
+    h->factor.release();
+    h->ref.release();
+    h->twW.release();
+    h->hgx.release();
+    h->hgy.release();
+    h->mask.release();
+    h->modes.release();
+    for (auto& kv : h->ffts) kv.second->tw.release();
+    h->hCoefs.release();
+    h->hFields.release();
+    for (auto& e : h->ev) (void)hipEventDestroy(e);
+    if (h->ownStream) (void)hipStreamDestroy(h->ownStream);
+  }
+  delete h;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C API
+// ===========================================================================
+#define H_OR(ret)        \
+  Handle* h = get(handleIdx); \
+  if (!h) return ret;
+#define CFG (h->cfg)
+
+EXT_C int digHoloCreate(void) {
+  std::lock_guard<std::mutex> lk(g_registry);
+  g_handles.push_back(new Handle());
+  return (int)g_handles.size() - 1;           // handles are never reused (digHolo.cpp:13434-13446)
+}
+
+EXT_C int digHoloDestroy(int handleIdx) {
+  Handle* h = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_registry);
+    if (handleIdx < 0 || handleIdx >= (int)g_handles.size() || !g_handles[handleIdx])
+      return DIGHOLO_ERROR_INVALIDHANDLE;
+    h = g_handles[handleIdx];
+    g_handles[handleIdx] = nullptr;
+  }
+  destroy_handle(h);
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+// ---- frame buffer (digHolo.cpp:4235-4350) ----
+EXT_C int digHoloSetFrameBuffer(int handleIdx, float* buffer) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (!buffer) return DIGHOLO_ERROR_NULLPOINTER;
+  h->frames = buffer;
+  h->frameU16 = 0;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C float* digHoloGetFrameBuffer(int handleIdx) {
+  H_OR(nullptr);
+  return h->frameU16 ? nullptr : (float*)h->frames;
+}
+EXT_C int digHoloSetFrameBufferUint16(int handleIdx, unsigned short* buffer, int transposeMode) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (!buffer) return DIGHOLO_ERROR_NULLPOINTER;
+  h->frames = buffer;
+  h->frameU16 = 1;
+  h->transpose = transposeMode ? 1 : 0;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C unsigned short* digHoloGetFrameBufferUint16(int handleIdx, int* transposeMode) {
+  H_OR(nullptr);
+  if (transposeMode) *transposeMode = h->transpose;
+  return h->frameU16 ? (unsigned short*)h->frames : nullptr;
+}
+
+// ---- frame config ----
+EXT_C int digHoloConfigSetFrameDimensions(int handleIdx, int width, int height) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (width % PIXEL_QUANTA || height % PIXEL_QUANTA || width < 0 || height < 0)   // digHolo.cpp:4154-4170
+    return DIGHOLO_ERROR_INVALIDDIMENSION;
+  CFG.frameWidth = width;
+  CFG.frameHeight = height;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C int digHoloConfigGetFrameDimensions(int handleIdx, int* width, int* height) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  *width = CFG.frameWidth;
+  *height = CFG.frameHeight;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C int digHoloConfigSetFramePixelSize(int handleIdx, float pixelSize) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (!(pixelSize > 0 && std::isfinite(pixelSize))) return DIGHOLO_ERROR_INVALIDDIMENSION;
+  CFG.pixelSize = pixelSize;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C float digHoloConfigGetFramePixelSize(int handleIdx) {
+  H_OR(0.f);
+  return CFG.pixelSize;
+}
+EXT_C int digHoloConfigSetPolCount(int handleIdx, int polCount) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (polCount > 0 && polCount <= POLCOUNTMAX) {
+    CFG.polCount = polCount;
+    return DIGHOLO_ERROR_SUCCESS;
+  }
+  return DIGHOLO_ERROR_INVALIDPOLARISATION;
+}
+EXT_C int digHoloConfigGetPolCount(int handleIdx) {
+  H_OR(0);
+  return CFG.polCount;
+}
+EXT_C int digHoloConfigSetFillFactorCorrectionEnabled(int handleIdx, int enabled) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  CFG.fillFactor = !(enabled == 0);
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C int digHoloConfigGetFillFactorCorrectionEnabled(int handleIdx) {
+  H_OR(0);
+  return CFG.fillFactor;
+}
+
+// ---- batch (digHolo.cpp:4401-4467, :15073-15121) ----
+static int set_batch(Handle* h, int batchCount, const void* buf, int u16, int avgCount, int avgMode, int transpose) {
+  if (!buf) return DIGHOLO_ERROR_NULLPOINTER;
+  h->frames = buf;
+  h->frameU16 = u16;
+  h->transpose = transpose ? 1 : 0;
+  if (batchCount < 1) batchCount = 1;
+  if (avgCount < 1) avgCount = 1;
+  if (!u16 && (avgMode < 0 || avgMode > 1)) avgMode = 0;
+  h->cfg.batchCount = batchCount;
+  h->cfg.avgCount = avgCount;
+  h->cfg.avgMode = avgMode;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C int digHoloSetBatch(int handleIdx, int batchCount, float* frameBuffer) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  return set_batch(h, batchCount, frameBuffer, 0, 1, 0, 0);
+}
+EXT_C int digHoloSetBatchAvg(int handleIdx, int batchCount, float* frameBuffer, int avgCount, int avgMode) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  return set_batch(h, batchCount, frameBuffer, 0, avgCount, avgMode, 0);
+}
+EXT_C int digHoloSetBatchUint16(int handleIdx, int batchCount, unsigned short* frameBuffer, int transpose) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  return set_batch(h, batchCount, frameBuffer, 1, 1, 0, transpose);
+}
+EXT_C int digHoloSetBatchAvgUint16(int handleIdx, int batchCount, unsigned short* frameBuffer, int avgCount,
+                                   int avgMode, int transpose) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  return set_batch(h, batchCount, frameBuffer, 1, avgCount, avgMode, transpose);
+}
+EXT_C int digHoloConfigSetBatchCount(int handleIdx, int batchCount) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (batchCount < 1) return DIGHOLO_ERROR_INVALIDARGUMENT;
+  CFG.batchCount = batchCount;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C int digHoloConfigGetBatchCount(int handleIdx) {
+  H_OR(0);
+  return CFG.batchCount;
+}
+EXT_C int digHoloConfigSetBatchAvgCount(int handleIdx, int avgCount) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (avgCount < 1) return DIGHOLO_ERROR_INVALIDARGUMENT;
+  CFG.avgCount = avgCount;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C int digHoloConfigGetBatchAvgCount(int handleIdx) {
+  H_OR(0);
+  return CFG.avgCount;
+}
+EXT_C int digHoloConfigSetBatchAvgMode(int handleIdx, int avgMode) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (avgMode < 0 || avgMode >= DIGHOLO_AVGMODE_COUNT) return DIGHOLO_ERROR_INVALIDARGUMENT;
+  CFG.avgMode = avgMode;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C int digHoloConfigGetBatchAvgMode(int handleIdx) {
+  H_OR(0);
+  return CFG.avgMode;
+}
+
+// ---- windows (digHolo.cpp:14234-14330, :13518-13540, :14414-14436, :14589-14611) ----
+EXT_C int digHoloConfigSetfftWindowSize(int handleIdx, int width, int height) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (width < 0 || height < 0) return DIGHOLO_ERROR_INVALIDDIMENSION;
+  CFG.fftWindowSizeX = PIXEL_QUANTA * (width / PIXEL_QUANTA);
+  CFG.fftWindowSizeY = PIXEL_QUANTA * (height / PIXEL_QUANTA);
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C int digHoloConfigGetfftWindowSize(int handleIdx, int* width, int* height) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  *width = CFG.fftWindowSizeX;
+  *height = CFG.fftWindowSizeY;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C int digHoloConfigSetfftWindowSizeX(int handleIdx, int width) {
+  H_OR(0);
+  if (width < 0) return 0;
+  CFG.fftWindowSizeX = PIXEL_QUANTA * (width / PIXEL_QUANTA);
+  return CFG.fftWindowSizeX;                  // returns the value set (digHolo.cpp:14234-14248)
+}
+EXT_C int digHoloConfigGetfftWindowSizeX(int handleIdx) {
+  H_OR(0);
+  return CFG.fftWindowSizeX;
+}
+EXT_C int digHoloConfigSetfftWindowSizeY(int handleIdx, int height) {
+  H_OR(0);
+  CFG.fftWindowSizeY = PIXEL_QUANTA * (height / PIXEL_QUANTA);
+  return CFG.fftWindowSizeY;
+}
+EXT_C int digHoloConfigGetfftWindowSizeY(int handleIdx) {
+  H_OR(0);
+  return CFG.fftWindowSizeY;
+}
+EXT_C int digHoloConfigSetFourierWindowRadius(int handleIdx, float windowRadius) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (windowRadius > 0) {
+    CFG.apertureSize = windowRadius;
+    return DIGHOLO_ERROR_SUCCESS;
+  }
+  return DIGHOLO_ERROR_INVALIDDIMENSION;
+}
+EXT_C float digHoloConfigGetFourierWindowRadius(int handleIdx) {
+  H_OR(0.f);
+  return CFG.apertureSize;
+}
+EXT_C int digHoloConfigSetIFFTResolutionMode(int handleIdx, int resolutionModeIdx) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (resolutionModeIdx == 0 || resolutionModeIdx == 1) {
+    CFG.resolutionIdx = resolutionModeIdx;
+    return DIGHOLO_ERROR_SUCCESS;
+  }
+  return DIGHOLO_ERROR_INVALIDARGUMENT;
+}
+EXT_C int digHoloConfigGetIFFTResolutionMode(int handleIdx) {
+  H_OR(0);
+  return CFG.resolutionIdx;
+}
+EXT_C int digHoloConfigSetWavelengthCentre(int handleIdx, float lambda0) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (lambda0 > 0 && std::isfinite(lambda0)) {
+    CFG.wavelengthCentre = lambda0;
+    return DIGHOLO_ERROR_SUCCESS;
+  }
+  return DIGHOLO_ERROR_INVALIDARGUMENT;
+}
+EXT_C float digHoloConfigGetWavelengthCentre(int handleIdx) {
+  H_OR(0.f);
+  return CFG.wavelengthCentre;
+}
+
+// ---- AutoAlign switches (digHolo.cpp:13542-13735): non-zero stored as 1 ----
+#define AA_FLAG(Name, field)                                         \
+  EXT_C int digHoloConfigSet##Name(int handleIdx, int v) {           \
+    H_OR(DIGHOLO_ERROR_INVALIDHANDLE);                               \
+    CFG.field = v ? 1 : 0;                                           \
+    return DIGHOLO_ERROR_SUCCESS;                                    \
+  }                                                                  \
+  EXT_C int digHoloConfigGet##Name(int handleIdx) {                  \
+    H_OR(0);                                                         \
+    return CFG.field;                                                \
+  }
+AA_FLAG(AutoAlignBeamCentre, aaCentre)
+AA_FLAG(AutoAlignDefocus, aaDefocus)
+AA_FLAG(AutoAlignTilt, aaTilt)
+AA_FLAG(AutoAlignBasisWaist, aaWaist)
+AA_FLAG(AutoAlignFourierWindowRadius, aaFourierRadius)
+AA_FLAG(AutoAlignPolIndependence, aaPolIndependence)
+AA_FLAG(AutoAlignBasisMulConjTrans, aaMulConjTrans)
+#undef AA_FLAG
+
+EXT_C int digHoloConfigSetAutoAlignTol(int handleIdx, float tolerance) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (!std::isfinite(tolerance)) return DIGHOLO_ERROR_INVALIDARGUMENT;
+  CFG.aaTol = tolerance;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C float digHoloConfigGetAutoAlignTol(int handleIdx) {
+  H_OR(0.f);
+  return CFG.aaTol;
+}
+EXT_C int digHoloConfigSetAutoAlignMode(int handleIdx, int alignMode) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (alignMode < 0 || alignMode >= DIGHOLO_AUTOALIGNMODE_COUNT) return DIGHOLO_ERROR_INVALIDARGUMENT;
+  CFG.aaMode = alignMode;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C int digHoloConfigGetAutoAlignMode(int handleIdx) {
+  H_OR(0);
+  return CFG.aaMode;
+}
+EXT_C int digHoloConfigSetAutoAlignGoalIdx(int handleIdx, int goalIdx) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (goalIdx < 0 || goalIdx >= DIGHOLO_METRIC_COUNT) return DIGHOLO_ERROR_INVALIDARGUMENT;
+  CFG.aaGoalIdx = goalIdx;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C int digHoloConfigGetAutoAlignGoalIdx(int handleIdx) {
+  H_OR(0);
+  return CFG.aaGoalIdx;
+}
+
+// ---- alignment parameters (digHolo.cpp:13736-13810, :14114-14230) ----
+EXT_C int digHoloConfigSetBeamCentre(int handleIdx, int axisIdx, int polIdx, float value) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (!std::isfinite(value)) return DIGHOLO_ERROR_INVALIDARGUMENT;
+  if (!(polIdx < CFG.polCount && polIdx >= 0)) return DIGHOLO_ERROR_INVALIDPOLARISATION;
+  if (axisIdx == 0) CFG.beamCX[polIdx] = value;
+  else if (axisIdx == 1) CFG.beamCY[polIdx] = value;
+  else return DIGHOLO_ERROR_INVALIDAXIS;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C float digHoloConfigGetBeamCentre(int handleIdx, int axisIdx, int polIdx) {
+  H_OR(0.f);
+  if (!(polIdx < CFG.polCount && polIdx >= 0)) return 0.f;
+  if (axisIdx == 0) return CFG.beamCX[polIdx];
+  if (axisIdx == 1) return CFG.beamCY[polIdx];
+  return 0.f;
+}
+EXT_C int digHoloConfigSetTilt(int handleIdx, int axisIdx, int polIdx, float value) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (!std::isfinite(value)) return DIGHOLO_ERROR_INVALIDARGUMENT;
+  if (!(polIdx < POLCOUNTMAX && polIdx >= 0)) return DIGHOLO_ERROR_INVALIDPOLARISATION;
+  if (axisIdx == 0) CFG.tiltX[polIdx] = value;
+  else if (axisIdx == 1) CFG.tiltY[polIdx] = value;
+  else return DIGHOLO_ERROR_INVALIDAXIS;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C float digHoloConfigGetTilt(int handleIdx, int axisIdx, int polIdx) {
+  H_OR(0.f);
+  if (!(polIdx < POLCOUNTMAX && polIdx >= 0)) return 0.f;
+  if (axisIdx == 0) return CFG.tiltX[polIdx];
+  if (axisIdx == 1) return CFG.tiltY[polIdx];
+  return 0.f;
+}
+EXT_C int digHoloConfigSetDefocus(int handleIdx, int polIdx, float value) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (!(polIdx < POLCOUNTMAX && polIdx >= 0)) return DIGHOLO_ERROR_INVALIDPOLARISATION;
+  CFG.defocus[polIdx] = value;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C float digHoloConfigGetDefocus(int handleIdx, int polIdx) {
+  H_OR(0.f);
+  if (!(polIdx < POLCOUNTMAX && polIdx >= 0)) return 0.f;
+  return CFG.defocus[polIdx];
+}
+#define POL_LOCK(Name, field)                                        \
+  EXT_C int digHoloConfigSet##Name(int handleIdx, int v) {           \
+    H_OR(DIGHOLO_ERROR_INVALIDHANDLE);                               \
+    CFG.field = v ? 1 : 0;                                           \
+    return DIGHOLO_ERROR_SUCCESS;                                    \
+  }                                                                  \
+  EXT_C int digHoloConfigGet##Name(int handleIdx) {                  \
+    H_OR(0);                                                         \
+    return CFG.field;                                                \
+  }
+POL_LOCK(PolLockTilt, polLockTilt)
+POL_LOCK(PolLockDefocus, polLockDefocus)
+POL_LOCK(PolLockBasisWaist, polLockWaist)
+#undef POL_LOCK
+
+// ---- basis (digHolo.cpp:14023-14071, :14717-14790) ----
+EXT_C int digHoloConfigSetBasisGroupCount(int handleIdx, int groupCount) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  CFG.maxMG = std::abs(groupCount);
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C int digHoloConfigGetBasisGroupCount(int handleIdx) {
+  H_OR(0);
+  return CFG.maxMG;
+}
+EXT_C int digHoloConfigSetBasisWaist(int handleIdx, int polIdx, float value) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (!(value > 0 && std::isfinite(value))) return DIGHOLO_ERROR_INVALIDDIMENSION;
+  if (!(polIdx < POLCOUNTMAX && polIdx >= 0)) return DIGHOLO_ERROR_INVALIDPOLARISATION;
+  for (int p = 0; p < CFG.polCount; ++p) CFG.waist[p] = value;   // all pols, as digHolo.cpp:14058-14063
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C float digHoloConfigGetBasisWaist(int handleIdx, int polIdx) {
+  H_OR(0.f);
+  if (!(polIdx < POLCOUNTMAX && polIdx >= 0)) return 0.f;
+  return CFG.waist[polIdx];
+}
+EXT_C int digHoloConfigSetBasisTypeHG(int handleIdx) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  CFG.basisType = DIGHOLO_BASISTYPE_HG;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C int digHoloConfigSetBasisType(int handleIdx, int basisType) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (basisType < 0 || basisType > DIGHOLO_BASISTYPE_CUSTOM) return DIGHOLO_ERROR_INVALIDARGUMENT;
+  CFG.basisType = basisType;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C int digHoloConfigGetBasisType(int handleIdx) {
+  H_OR(0);
+  return CFG.basisType;
+}
+
+// ---- processing ----
+EXT_C complex64* digHoloProcessBatch(int handleIdx, int* batchCount, int* modeCount, int* polCount) {
+  Handle* h = get(handleIdx);
+  if (!h) {
+    *batchCount = 0;
+    *modeCount = 0;
+    *polCount = 0;
+    return nullptr;
+  }
+  int e = enqueue_batch(*h);
+  if (!e) e = download_coefs(*h);
+  collect_stage_times(*h);
+  if (e) {
+    *batchCount = 0;
+    *modeCount = 0;
+    *polCount = 0;
+    return nullptr;
+  }
+  const dh::Geometry& g = h->cur.g;
+  *batchCount = g.B;
+  *polCount = g.P;
+  *modeCount = g.M;
+  return g.M > 0 ? reinterpret_cast<complex64*>(h->hCoefs.p) : nullptr;   // G=0: no decomposition
+}
+
+EXT_C complex64* digHoloBasisGetCoefs(int handleIdx, int* batchCount, int* modeCount, int* polCount) {
+  Handle* h = get(handleIdx);
+  if (!h || !h->haveBatch || h->cur.g.M == 0 || download_coefs(*h)) {
+    *batchCount = 0;
+    *modeCount = 0;
+    *polCount = 0;
+    return nullptr;
+  }
+  const dh::Geometry& g = h->cur.g;
+  *batchCount = g.B;
+  *polCount = g.P;
+  *modeCount = g.M;
+  return reinterpret_cast<complex64*>(h->hCoefs.p);
+}
+
+EXT_C complex64* digHoloGetFields(int handleIdx, int* batchCount, int* polCount, float** x, float** y, int* width,
+                                  int* height) {
+  Handle* h = get(handleIdx);
+  if (!h || !h->haveBatch || download_fields(*h)) {
+    *batchCount = 0;
+    *polCount = 0;
+    *x = nullptr;
+    *y = nullptr;
+    *width = 0;
+    *height = 0;
+    return nullptr;
+  }
+  const dh::Geometry& g = h->cur.g;
+  *batchCount = g.B;
+  *polCount = g.P;
+  *width = g.w;
+  *height = g.h;
+  *x = h->xAxis.data();
+  *y = h->yAxis.data();
+  return reinterpret_cast<complex64*>(h->hFields.p);
+}
+
+// ---- threads / console ----
+EXT_C int digHoloConfigSetThreadCount(int handleIdx, int threadCount) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (threadCount < 1) threadCount = 1;      // accepted and stored; GPU work ignores it
+  CFG.threadCount = threadCount;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C int digHoloConfigGetThreadCount(int handleIdx) {
+  H_OR(0);
+  return CFG.threadCount;
+}
+EXT_C int digHoloConfigSetVerbosity(int handleIdx, int verbosity) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  CFG.verbosity = verbosity;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C int digHoloConfigGetVerbosity(int handleIdx) {
+  H_OR(0);
+  return CFG.verbosity;
+}
+
+// Per-stage rates in the reference's info layout (digHolo.h:1783-1788): calls
+// per second of each stage and frames per second end to end, measured with
+// HIP events on repeated batches for about goalDuration seconds.
+EXT_C float digHoloBenchmark(int handleIdx, float goalDuration, float* info) {
+  Handle* h = get(handleIdx);
+  if (!h) return 0.f;
+  const bool prevTiming = h->stageTiming;
+  h->stageTiming = true;
+  if (enqueue_batch(*h) || hipStreamSynchronize(h->stream) != hipSuccess) {
+    h->stageTiming = prevTiming;
+    return 0.f;
+  }
+  collect_stage_times(*h);
+  double acc[5] = {0, 0, 0, 0, 0};
+  int iters = 0;
+  hipEvent_t a, b;
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
+  double wall = 0;
+  while (wall < goalDuration || iters < 3) {
+    (void)hipEventRecord(a, h->stream);
+    if (enqueue_batch(*h)) break;
+    (void)hipEventRecord(b, h->stream);
+    (void)hipEventSynchronize(b);
+    collect_stage_times(*h);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, a, b);
+    wall += ms * 1e-3;
+    for (int k = 0; k < 5; ++k) acc[k] += h->stageMs[k];
+    ++iters;
+    if (iters > 100000) break;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  h->stageTiming = prevTiming;
+  const double B = h->cur.g.B;
+  const double fps = wall > 0 ? B * iters / wall : 0;
+  if (info) {
+    const double fft = acc[0] + acc[1];
+    info[DIGHOLO_BENCHMARK_FFT] = fft > 0 ? (float)(iters * 1e3 / fft) : 0.f;
+    info[DIGHOLO_BENCHMARK_IFFT] = acc[2] > 0 ? (float)(iters * 1e3 / acc[2]) : 0.f;
+    info[DIGHOLO_BENCHMARK_APPLYTILT] = acc[2] > 0 ? (float)(iters * 1e3 / acc[2]) : 0.f;   // fused with IFFT
+    info[DIGHOLO_BENCHMARK_BASIS] = acc[3] > 0 ? (float)(iters * 1e3 / acc[3]) : 0.f;
+    info[DIGHOLO_BENCHMARK_OVERLAP] = acc[4] > 0 ? (float)(iters * 1e3 / acc[4]) : 0.f;
+    info[DIGHOLO_BENCHMARK_TOTAL] = (float)fps;
+  }
+  return (float)fps;
+}
+
+// ---- extensions ----
+EXT_C int digHoloConfigSetOverlapPrecision(int handleIdx, int precision) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (precision != DIGHOLO_PRECISION_FP32 && precision != DIGHOLO_PRECISION_SPLIT16)
+    return DIGHOLO_ERROR_INVALIDARGUMENT;
+  CFG.precision = precision;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C int digHoloConfigGetOverlapPrecision(int handleIdx) {
+  H_OR(0);
+  return CFG.precision;
+}
+
+EXT_C complex64* digHoloProcessBatchDevice(int handleIdx, int* batchCount, int* modeCount, int* polCount) {
+  Handle* h = get(handleIdx);
+  if (!h || enqueue_batch(*h)) {
+    *batchCount = 0;
+    *modeCount = 0;
+    *polCount = 0;
+    return nullptr;
+  }
+  const dh::Geometry& g = h->cur.g;
+  *batchCount = g.B;
+  *polCount = g.P;
+  *modeCount = g.M;
+  return reinterpret_cast<complex64*>(h->coefs.p);
+}
+
+EXT_C complex64* digHoloGetFieldsDevice(int handleIdx, int* batchCount, int* polCount, int* width, int* height) {
+  Handle* h = get(handleIdx);
+  if (!h || !h->haveBatch) {
+    *batchCount = 0;
+    *polCount = 0;
+    *width = 0;
+    *height = 0;
+    return nullptr;
+  }
+  const dh::Geometry& g = h->cur.g;
+  *batchCount = g.B;
+  *polCount = g.P;
+  *width = g.w;
+  *height = g.h;
+  return reinterpret_cast<complex64*>(h->fields.p);
+}
+
+EXT_C int digHoloSynchronize(int handleIdx) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (!h->devInit) return DIGHOLO_ERROR_SUCCESS;
+  hipError_t e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) {
+    log_err(h, "synchronize", e);
+    return DIGHOLO_ERROR_ERROR;
+  }
+  collect_stage_times(*h);
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+EXT_C int digHoloSetStream(int handleIdx, void* stream) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (h->devInit) (void)hipStreamSynchronize(h->stream);
+  h->stream = stream ? static_cast<hipStream_t>(stream) : h->ownStream;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C void* digHoloGetStream(int handleIdx) {
+  H_OR(nullptr);
+  return h->stream;
+}
+
+EXT_C int digHoloGetWindowGeometry(int handleIdx, int* out) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (!out) return DIGHOLO_ERROR_NULLPOINTER;
+  Config c = h->cfg;
+  if (c.polLockTilt) {
+    c.tiltX[1] = c.tiltX[0];
+    c.tiltY[1] = c.tiltY[0];
+  }
+  Derived d;
+  int err;
+  if (!derive(c, std::max(c.batchCount, 1), d, err)) return err;
+  const dh::Geometry& g = d.g;
+  const int vals[12] = {g.W, g.H, g.w, g.h, g.cx[0], g.cy[0], g.cx[1], g.cy[1], d.SX[0], d.SY[0], d.SX[1], d.SY[1]};
+  std::memcpy(out, vals, sizeof(vals));
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+EXT_C int digHoloConfigSetStageTiming(int handleIdx, int enabled) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  h->stageTiming = enabled != 0;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C int digHoloGetStageTimes(int handleIdx, float* ms) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (!ms) return DIGHOLO_ERROR_NULLPOINTER;
+  std::memcpy(ms, h->stageMs, sizeof(h->stageMs));
+  return DIGHOLO_ERROR_SUCCESS;
+}

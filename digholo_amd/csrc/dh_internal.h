@@ -1,0 +1,63 @@
+// Shared declarations between the C API host code (dh_api.cpp) and the HIP
+// kernels (dh_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace dh {
+
+constexpr int kMaxPol = 2;
+constexpr int kMaxStages = 12;
+
+// One 1-D FFT length factored for the Stockham kernels.
+struct FFTPlan {
+  int n;                      // transform length (complex points)
+  int nstages;
+  int radix[kMaxStages];
+  const float2* tw;           // n twiddles exp(-2 pi i k / n), device memory
+};
+
+// Everything the kernels need about the windows of one batch.
+struct Geometry {
+  int B, P;                   // batch, polarisations
+  int frameW, frameH;
+  int W, H;                   // FFT window
+  int w, h;                   // IFFT window
+  int cx[kMaxPol], cy[kMaxPol];   // FFT window corners in the frame
+  int c0[kMaxPol];            // first r2c column read by the IFFT window (SX - W/2)
+  int SY[kMaxPol];            // IFFT window row start (shifted-spectrum coordinates)
+  int G, M;                   // HG groups and modes per pol
+};
+
+// FP64 per-pol parameters of the Fourier-window factor and the reference wave.
+struct PolParams {
+  double dkx, dky;            // 2 pi / (px W), 2 pi / (px H)
+  double bcx, bcy;            // beam centre relative to the FFT window centre (m)
+  double px;                  // camera pixel (m)
+  double norm;                // 1/sqrt(W H w h)
+  int fill;                   // fill-factor correction on/off
+  // reference wave: phi(x) = kx x + piston_x + focal x^2 (same for y)
+  double kxr, kyr, pistx, pisty, focal;
+  double pxx, pxy;            // reconstructed-field pixel pitch
+  double waist;               // HG waist
+};
+
+struct TableArgs {
+  Geometry g;
+  PolParams pp[kMaxPol];
+};
+
+// Launchers (defined in dh_kernels.hip).  All enqueue on `s`.
+hipError_t launch_tables(const TableArgs& a, const uint8_t* d_mask, float2* d_factor, float2* d_ref,
+                         float* d_hgx, float* d_hgy, hipStream_t s);
+hipError_t launch_row_fft(const Geometry& g, const FFTPlan& pl, const float2* twW, const void* frames,
+                          int frame_is_u16, float2* rowspec, hipStream_t s);
+hipError_t launch_col_fft_window(const Geometry& g, const FFTPlan& pl, const float2* rowspec,
+                                 const float2* factor, float2* win, hipStream_t s);
+hipError_t launch_ifft_rows(const Geometry& g, const FFTPlan& pl, float2* win, hipStream_t s);
+hipError_t launch_ifft_cols_tilt(const Geometry& g, const FFTPlan& pl, const float2* win,
+                                 const float2* ref, float2* fields, hipStream_t s);
+hipError_t launch_overlap_fp32(const Geometry& g, const float2* fields, const float* hgx,
+                               const float* hgy, const int2* modes, float2* coefs, hipStream_t s);
+
+}  // namespace dh

@@ -1,0 +1,566 @@
+// HIP kernels of the batch-processing path (gfx950).
+//
+// Data flow of one batch (B frames, P polarisations):
+//   frames (u16/f32) --k_row_fft--> rowspec[B][P][H][w]   r2c rows, only the w
+//                                                          columns the Fourier
+//                                                          window reads
+//   rowspec --k_col_fft_window--> win[B][P][h][w]         column FFT, keep the h
+//                                                          window rows, times the
+//                                                          window factor
+//   win --k_ifft_rows--> win (in place)                    inverse FFT along x
+//   win --k_ifft_cols_tilt--> fields[B][P][h][w]           inverse FFT along y,
+//                                                          times reference wave,
+//                                                          V = -conj(E ref)
+//   fields --k_overlap_fp32--> coefs[B][P][M]              separable HG overlap
+//
+// All transforms are batched mixed-radix Stockham FFTs staged in LDS
+// (radix 8/4/2/3/5 butterflies, any other prime by a direct DFT).
+#include <algorithm>
+
+#include "dh_internal.h"
+
+namespace dh {
+
+// ---------------------------------------------------------------------------
+// complex helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ __forceinline__ float2 mul_mi(float2 a) {
+  return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+template <bool INV>
+__device__ __forceinline__ float2 twiddle(const float2* tw, int k) {
+  float2 w = tw[k];
+  if (INV) w.y = -w.y;
+  return w;
+}
+
+// ---------------------------------------------------------------------------
+// small DFT butterflies, X[k] = sum_r v[r] exp(-/+ 2 pi i r k / R)
+// ---------------------------------------------------------------------------
+template <bool INV>
+__device__ __forceinline__ void dft2(float2* v) {
+  float2 a = v[0];
+  v[0] = cadd(a, v[1]);
+  v[1] = csub(a, v[1]);
+}
+template <bool INV>
+__device__ __forceinline__ void dft4(float2* v) {
+  float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+  float2 t2 = cadd(v[1], v[3]), t3 = mul_mi<INV>(csub(v[1], v[3]));
+  v[0] = cadd(t0, t2);
+  v[2] = csub(t0, t2);
+  v[1] = cadd(t1, t3);
+  v[3] = csub(t1, t3);
+}
+template <bool INV>
+__device__ __forceinline__ void dft8(float2* v) {
+  const float s = 0.70710678118654752440f;
+  float2 a[4] = {v[0], v[2], v[4], v[6]};
+  float2 b[4] = {v[1], v[3], v[5], v[7]};
+  dft4<INV>(a);
+  dft4<INV>(b);
+  // b[k] *= exp(-/+ i pi k / 4)
+  float2 b1 = INV ? make_float2(s * (b[1].x - b[1].y), s * (b[1].x + b[1].y))
+                  : make_float2(s * (b[1].x + b[1].y), s * (b[1].y - b[1].x));
+  float2 b2 = mul_mi<INV>(b[2]);
+  float2 b3 = INV ? make_float2(-s * (b[3].x + b[3].y), s * (b[3].x - b[3].y))
+                  : make_float2(s * (b[3].y - b[3].x), -s * (b[3].x + b[3].y));
+  v[0] = cadd(a[0], b[0]);
+  v[4] = csub(a[0], b[0]);
+  v[1] = cadd(a[1], b1);
+  v[5] = csub(a[1], b1);
+  v[2] = cadd(a[2], b2);
+  v[6] = csub(a[2], b2);
+  v[3] = cadd(a[3], b3);
+  v[7] = csub(a[3], b3);
+}
+template <bool INV>
+__device__ __forceinline__ void dft3(float2* v) {
+  const float s3 = 0.86602540378443864676f;
+  float2 t = cadd(v[1], v[2]);
+  float2 u = mul_mi<INV>(cscale(csub(v[1], v[2]), s3));
+  float2 m = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
+  v[0] = cadd(v[0], t);
+  v[1] = cadd(m, u);
+  v[2] = csub(m, u);
+}
+template <bool INV>
+__device__ __forceinline__ void dft5(float2* v) {
+  const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+  const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+  float2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+  float2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+  float2 a1 = make_float2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+  float2 a2 = make_float2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+  float2 b1 = mul_mi<INV>(make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y));
+  float2 b2 = mul_mi<INV>(make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y));
+  v[0] = make_float2(v[0].x + t1.x + t2.x, v[0].y + t1.y + t2.y);
+  v[1] = cadd(a1, b1);
+  v[4] = csub(a1, b1);
+  v[2] = cadd(a2, b2);
+  v[3] = csub(a2, b2);
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dftR(float2* v) {
+  if constexpr (R == 2) dft2<INV>(v);
+  else if constexpr (R == 3) dft3<INV>(v);
+  else if constexpr (R == 4) dft4<INV>(v);
+  else if constexpr (R == 5) dft5<INV>(v);
+  else dft8<INV>(v);
+}
+
+// One radix-R Stockham butterfly: reads src[j + r n/R], applies the stage
+// twiddles, writes dst[(j/Ns) Ns R + j%Ns + r Ns].
+template <int R, bool INV>
+__device__ __forceinline__ void stage_bf(const float2* S, float2* D, int j, int jm, int nbf, int Ns,
+                                         int step, const float2* tw) {
+  float2 v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = S[j + r * nbf];
+  if (jm) {
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twiddle<INV>(tw, jm * r * step));
+  }
+  dftR<R, INV>(v);
+  const int idxD = (j - jm) * R + jm;
+#pragma unroll
+  for (int r = 0; r < R; ++r) D[idxD + r * Ns] = v[r];
+}
+
+// Radix without a dedicated butterfly: direct DFT from LDS.
+template <bool INV>
+__device__ void stage_bf_generic(const float2* S, float2* D, int R, int n, int j, int jm, int nbf, int Ns,
+                                 int step, const float2* tw) {
+  const int nR = n / R;
+  const int idxD = (j - jm) * R + jm;
+  for (int k = 0; k < R; ++k) {
+    float2 acc = make_float2(0.f, 0.f);
+    for (int r = 0; r < R; ++r) {
+      float2 v = S[j + r * nbf];
+      if (r && jm) v = cmul(v, twiddle<INV>(tw, jm * r * step));
+      acc = cadd(acc, cmul(v, twiddle<INV>(tw, ((r * k) % R) * nR)));
+    }
+    D[idxD + k * Ns] = acc;
+  }
+}
+
+// nb transforms of length pl.n, transform t at a[t*n .. t*n+n).  Uses b as the
+// ping-pong buffer; returns whichever buffer holds the result.  Must be called
+// by the whole block; ends with a barrier.
+template <bool INV>
+__device__ float2* lds_fft(float2* a, float2* b, int nb, const FFTPlan& pl) {
+  const int n = pl.n;
+  int Ns = 1;
+  float2* src = a;
+  float2* dst = b;
+  for (int s = 0; s < pl.nstages; ++s) {
+    const int R = pl.radix[s];
+    const int nbf = n / R;
+    const int total = nbf * nb;
+    const int step = n / (Ns * R);
+    for (int q = threadIdx.x; q < total; q += blockDim.x) {
+      const int t = q / nbf;
+      const int j = q - t * nbf;
+      const int jm = j % Ns;
+      const float2* S = src + t * n;
+      float2* D = dst + t * n;
+      switch (R) {
+        case 8: stage_bf<8, INV>(S, D, j, jm, nbf, Ns, step, pl.tw); break;
+        case 4: stage_bf<4, INV>(S, D, j, jm, nbf, Ns, step, pl.tw); break;
+        case 2: stage_bf<2, INV>(S, D, j, jm, nbf, Ns, step, pl.tw); break;
+        case 3: stage_bf<3, INV>(S, D, j, jm, nbf, Ns, step, pl.tw); break;
+        case 5: stage_bf<5, INV>(S, D, j, jm, nbf, Ns, step, pl.tw); break;
+        default: stage_bf_generic<INV>(S, D, R, n, j, jm, nbf, Ns, step, pl.tw); break;
+      }
+    }
+    __syncthreads();
+    float2* t = src;
+    src = dst;
+    dst = t;
+    Ns *= R;
+  }
+  return src;
+}
+
+constexpr int kThreads = 256;
+
+// ---------------------------------------------------------------------------
+// 1. frame ingest + r2c FFT along x, keeping the Fourier-window columns
+// ---------------------------------------------------------------------------
+// grid (ceil(H/RB), P, B).  Row y of window p of frame b is packed as
+// z[k] = x[2k] + i x[2k+1] (k < W/2), transformed, and unpacked with
+// X[k] = (Z[k] + conj Z[n-k])/2 - i/2 exp(-2 pi i k/W) (Z[k] - conj Z[n-k]).
+__global__ __launch_bounds__(kThreads) void k_row_fft(Geometry g, FFTPlan pl, const float2* __restrict__ twW,
+                                                      const void* __restrict__ frames, int u16,
+                                                      float2* __restrict__ rowspec, int RB) {
+  extern __shared__ float2 smem[];
+  const int n = pl.n;
+  float2* A = smem;
+  float2* Bf = smem + RB * n;
+  const int b = blockIdx.z, p = blockIdx.y, y0 = blockIdx.x * RB;
+  const int rows = min(RB, g.H - y0);
+  const size_t fbase = (size_t)b * g.frameH * g.frameW + (size_t)(g.cy[p] + y0) * g.frameW + g.cx[p];
+  for (int e = threadIdx.x; e < rows * n; e += blockDim.x) {
+    const int r = e / n, k = e - r * n;
+    const size_t off = fbase + (size_t)r * g.frameW + 2 * k;
+    float2 z;
+    if (u16) {
+      const ushort2 v = *reinterpret_cast<const ushort2*>(static_cast<const unsigned short*>(frames) + off);
+      z = make_float2((float)v.x, (float)v.y);
+    } else {
+      z = *reinterpret_cast<const float2*>(static_cast<const float*>(frames) + off);
+    }
+    A[e] = z;
+  }
+  __syncthreads();
+  const float2* Z = lds_fft<false>(A, Bf, rows, pl);
+  const int c0 = g.c0[p];
+  float2* out = rowspec + ((size_t)(b * g.P + p) * g.H + y0) * g.w;
+  for (int e = threadIdx.x; e < rows * g.w; e += blockDim.x) {
+    const int r = e / g.w, kk = e - r * g.w;
+    const int k = c0 + kk;
+    const float2 zk = Z[r * n + k];
+    float2 zc = Z[r * n + (k == 0 ? 0 : n - k)];
+    zc.y = -zc.y;
+    const float2 ev = cscale(cadd(zk, zc), 0.5f);
+    const float2 od = mul_mi<false>(cscale(csub(zk, zc), 0.5f));   // (zk - zc)/(2i)
+    out[e] = cadd(ev, cmul(twW[k], od));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 2. FFT along y of the w kept columns, pick the h window rows, apply the
+//    Fourier-window factor (mask * fill-factor envelope * centring ramp * norm)
+// ---------------------------------------------------------------------------
+// grid (ceil(w/CB), P, B)
+__global__ __launch_bounds__(kThreads) void k_col_fft_window(Geometry g, FFTPlan pl,
+                                                             const float2* __restrict__ rowspec,
+                                                             const float2* __restrict__ factor,
+                                                             float2* __restrict__ win, int CB) {
+  extern __shared__ float2 smem[];
+  const int n = pl.n;
+  float2* A = smem;
+  float2* Bf = smem + CB * n;
+  const int b = blockIdx.z, p = blockIdx.y, i0 = blockIdx.x * CB;
+  const int cols = min(CB, g.w - i0);
+  const float2* in = rowspec + (size_t)(b * g.P + p) * g.H * g.w + i0;
+  for (int e = threadIdx.x; e < n * cols; e += blockDim.x) {
+    const int y = e / cols, c = e - y * cols;
+    A[c * n + y] = in[(size_t)y * g.w + c];
+  }
+  __syncthreads();
+  const float2* F = lds_fft<false>(A, Bf, cols, pl);
+  const float2* fac = factor + (size_t)p * g.h * g.w + i0;
+  float2* out = win + (size_t)(b * g.P + p) * g.h * g.w + i0;
+  const int sy = g.SY[p] + n / 2;
+  for (int e = threadIdx.x; e < g.h * cols; e += blockDim.x) {
+    const int j = e / cols, c = e - j * cols;
+    int row = (j + sy) % n;
+    if (row < 0) row += n;
+    out[(size_t)j * g.w + c] = cmul(F[c * n + row], fac[(size_t)j * g.w + c]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3. inverse FFT along x, in place.  grid (ceil(h/RB), P, B)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_ifft_rows(Geometry g, FFTPlan pl, float2* __restrict__ win,
+                                                        int RB) {
+  extern __shared__ float2 smem[];
+  const int n = pl.n;
+  float2* A = smem;
+  float2* Bf = smem + RB * n;
+  const int b = blockIdx.z, p = blockIdx.y, y0 = blockIdx.x * RB;
+  const int rows = min(RB, g.h - y0);
+  float2* base = win + ((size_t)(b * g.P + p) * g.h + y0) * g.w;
+  for (int e = threadIdx.x; e < rows * n; e += blockDim.x) A[e] = base[e];
+  __syncthreads();
+  const float2* X = lds_fft<true>(A, Bf, rows, pl);
+  for (int e = threadIdx.x; e < rows * n; e += blockDim.x) base[e] = X[e];
+}
+
+// ---------------------------------------------------------------------------
+// 4. inverse FFT along y, remove the reference wave: V = -conj(E * ref).
+//    grid (ceil(w/CB), P, B)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_ifft_cols_tilt(Geometry g, FFTPlan pl,
+                                                             const float2* __restrict__ win,
+                                                             const float2* __restrict__ ref,
+                                                             float2* __restrict__ fields, int CB) {
+  extern __shared__ float2 smem[];
+  const int n = pl.n;
+  float2* A = smem;
+  float2* Bf = smem + CB * n;
+  const int b = blockIdx.z, p = blockIdx.y, i0 = blockIdx.x * CB;
+  const int cols = min(CB, g.w - i0);
+  const size_t base = (size_t)(b * g.P + p) * g.h * g.w + i0;
+  for (int e = threadIdx.x; e < n * cols; e += blockDim.x) {
+    const int y = e / cols, c = e - y * cols;
+    A[c * n + y] = win[base + (size_t)y * g.w + c];
+  }
+  __syncthreads();
+  const float2* E = lds_fft<true>(A, Bf, cols, pl);
+  const float2* rp = ref + (size_t)p * g.h * g.w + i0;
+  for (int e = threadIdx.x; e < n * cols; e += blockDim.x) {
+    const int j = e / cols, c = e - j * cols;
+    const float2 t = cmul(E[c * n + j], rp[(size_t)j * g.w + c]);
+    fields[base + (size_t)j * g.w + c] = make_float2(-t.x, t.y);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 5. separable HG overlap, plain FP32:
+//    c[m,n] = sum_y HGy[n][y] T[m][y],  T[m][y] = sum_x HGx[m][x] V[y][x]
+//    grid (P, B); each thread owns modes q = tid + k*256.
+// ---------------------------------------------------------------------------
+constexpr int kOverlapRows = 16;
+
+template <int MPT>
+__global__ __launch_bounds__(kThreads) void k_overlap_fp32(Geometry g, const float2* __restrict__ fields,
+                                                           const float* __restrict__ hgx,
+                                                           const float* __restrict__ hgy,
+                                                           const int2* __restrict__ modes,
+                                                           float2* __restrict__ coefs) {
+  extern __shared__ float2 smem[];
+  const int p = blockIdx.x, b = blockIdx.y;
+  const int G = g.G, w = g.w, h = g.h, M = g.M;
+  float* hx = reinterpret_cast<float*>(smem);
+  float* hy = hx + G * w;
+  float2* Vs = reinterpret_cast<float2*>(hy + ((G * h + 1) & ~1));
+  float2* Ts = Vs + kOverlapRows * w;
+  for (int e = threadIdx.x; e < G * w; e += blockDim.x) hx[e] = hgx[(size_t)p * G * w + e];
+  for (int e = threadIdx.x; e < G * h; e += blockDim.x) hy[e] = hgy[(size_t)p * G * h + e];
+  int2 mn[MPT];
+  float2 acc[MPT];
+#pragma unroll
+  for (int k = 0; k < MPT; ++k) {
+    const int q = threadIdx.x + k * kThreads;
+    mn[k] = q < M ? modes[q] : make_int2(0, 0);
+    acc[k] = make_float2(0.f, 0.f);
+  }
+  const float2* V = fields + (size_t)(b * g.P + p) * h * w;
+  for (int y0 = 0; y0 < h; y0 += kOverlapRows) {
+    const int rows = min(kOverlapRows, h - y0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < rows * w; e += blockDim.x) Vs[e] = V[(size_t)y0 * w + e];
+    __syncthreads();
+    for (int e = threadIdx.x; e < G * rows; e += blockDim.x) {
+      const int m = e / rows, yy = e - m * rows;
+      const float* hm = hx + m * w;
+      const float2* vr = Vs + yy * w;
+      float2 t = make_float2(0.f, 0.f);
+      for (int x = 0; x < w; ++x) {
+        t.x += hm[x] * vr[x].x;
+        t.y += hm[x] * vr[x].y;
+      }
+      Ts[m * kOverlapRows + yy] = t;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < MPT; ++k) {
+      const int q = threadIdx.x + k * kThreads;
+      if (q < M) {
+        const float* hn = hy + mn[k].y * h + y0;
+        const float2* tm = Ts + mn[k].x * kOverlapRows;
+        for (int yy = 0; yy < rows; ++yy) {
+          acc[k].x += hn[yy] * tm[yy].x;
+          acc[k].y += hn[yy] * tm[yy].y;
+        }
+      }
+    }
+  }
+  float2* out = coefs + (size_t)(b * g.P + p) * M;
+#pragma unroll
+  for (int k = 0; k < MPT; ++k) {
+    const int q = threadIdx.x + k * kThreads;
+    if (q < M) out[q] = acc[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 tables: Fourier-window factor, reference wave, HG basis
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double xcsc(double t) { return t == 0.0 ? 1.0 : t / sin(t); }
+
+// factor[p][j][i] = mask * S(kx) S(ky) exp(i(kx bcx + ky bcy)) * norm, with the
+// FFT-order frequency of the r2c row/column each window pixel reads;
+// ref[p][j][i] = exp(i(phi_x(i) + phi_y(j))).
+__global__ void k_tables(TableArgs a, const uint8_t* __restrict__ mask, float2* __restrict__ factor,
+                         float2* __restrict__ ref) {
+  const Geometry& g = a.g;
+  const int total = g.P * g.h * g.w;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += gridDim.x * blockDim.x) {
+    const int p = idx / (g.h * g.w);
+    const int rem = idx - p * g.h * g.w;
+    const int j = rem / g.w, i = rem - j * g.w;
+    const PolParams& pp = a.pp[p];
+    const int col = g.c0[p] + i;
+    int row = (j + g.SY[p] + g.H / 2) % g.H;
+    if (row < 0) row += g.H;
+    const double kx = col * pp.dkx;
+    const double ky = (row <= g.H / 2 ? row : row - g.H) * pp.dky;
+    double env = pp.norm;
+    if (pp.fill) env *= xcsc(0.5 * kx * pp.px) * xcsc(0.5 * ky * pp.px);
+    double sn, cs;
+    sincos(kx * pp.bcx + ky * pp.bcy, &sn, &cs);
+    const double m = mask[idx] ? env : 0.0;
+    factor[idx] = make_float2((float)(m * cs), (float)(m * sn));
+    const double x = (i - g.w / 2) * pp.pxx;
+    const double y = (j - g.h / 2) * pp.pxy;
+    const double phi = pp.kxr * x + pp.pistx + pp.focal * x * x + pp.kyr * y + pp.pisty + pp.focal * y * y;
+    sincos(phi, &sn, &cs);
+    ref[idx] = make_float2((float)cs, (float)sn);
+  }
+}
+
+// HG mode n at coordinate t: psi_n(sqrt2 t / waist) sqrt(dt sqrt2 / waist), by
+// the three-term recurrence with the mantissa renormalised every step and the
+// binary exponent carried separately (no under/overflow at high order).
+__global__ void k_hg_basis(TableArgs a, float* __restrict__ hgx, float* __restrict__ hgy) {
+  const Geometry& g = a.g;
+  const int per = g.w + g.h;
+  const int total = g.P * per;
+  const double rt2 = 1.41421356237309504880;
+  const double pim4 = 0.75112554446494248286;   // pi^-1/4
+  const double ln2 = 0.69314718055994530942;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += gridDim.x * blockDim.x) {
+    const int p = idx / per;
+    const int k = idx - p * per;
+    const PolParams& pp = a.pp[p];
+    const bool isx = k < g.w;
+    const int i = isx ? k : k - g.w;
+    const int N = isx ? g.w : g.h;
+    const double dt = isx ? pp.pxx : pp.pxy;
+    const double t = (i - N / 2) * dt;
+    const double u = rt2 * t / pp.waist;
+    const double nrm = sqrt(dt * rt2 / pp.waist);
+    const double gexp = -0.5 * u * u;
+    float* out = isx ? (hgx + (size_t)p * g.G * g.w + i) : (hgy + (size_t)p * g.G * g.h + i);
+    const size_t stride = N;
+    double m0 = pim4, e0 = 0.0;
+    double m1 = rt2 * pim4 * u, e1 = 0.0;
+    if (g.G > 0) out[0] = (float)(nrm * m0 * exp(gexp));
+    if (g.G > 1) out[stride] = (float)(nrm * m1 * exp(gexp));
+    for (int n = 2; n < g.G; ++n) {
+      const double c1 = sqrt(2.0 / n), c2 = sqrt((n - 1.0) / n);
+      double mn = c1 * u * m1 - c2 * m0 * exp2(e0 - e1);
+      int fe;
+      mn = frexp(mn, &fe);
+      const double en = e1 + fe;
+      m0 = m1;
+      e0 = e1;
+      m1 = mn;
+      e1 = en;
+      out[n * stride] = (float)(nrm * mn * exp(gexp + en * ln2));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline int lds_limit() { return 64 * 1024; }
+
+template <typename K>
+static hipError_t set_lds(K kern, size_t bytes) {
+  if (bytes > 64 * 1024)
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes);
+  return hipSuccess;
+}
+
+// transforms per block so that two ping-pong buffers fit comfortably
+static inline int batch_per_block(int n, int cap_elems) {
+  int nb = cap_elems / n;
+  return nb < 1 ? 1 : nb;
+}
+
+hipError_t launch_tables(const TableArgs& a, const uint8_t* d_mask, float2* d_factor, float2* d_ref,
+                         float* d_hgx, float* d_hgy, hipStream_t s) {
+  const int total = a.g.P * a.g.h * a.g.w;
+  k_tables<<<(total + 255) / 256, 256, 0, s>>>(a, d_mask, d_factor, d_ref);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || a.g.G == 0) return e;
+  const int t2 = a.g.P * (a.g.w + a.g.h);
+  k_hg_basis<<<(t2 + 63) / 64, 64, 0, s>>>(a, d_hgx, d_hgy);
+  return hipGetLastError();
+}
+
+hipError_t launch_row_fft(const Geometry& g, const FFTPlan& pl, const float2* twW, const void* frames,
+                          int frame_is_u16, float2* rowspec, hipStream_t s) {
+  const int RB = std::min(g.H, batch_per_block(pl.n, 2048));
+  const size_t lds = 2 * (size_t)RB * pl.n * sizeof(float2);
+  hipError_t e = set_lds(k_row_fft, lds);
+  if (e != hipSuccess) return e;
+  dim3 grid((g.H + RB - 1) / RB, g.P, g.B);
+  k_row_fft<<<grid, kThreads, lds, s>>>(g, pl, twW, frames, frame_is_u16, rowspec, RB);
+  return hipGetLastError();
+}
+
+hipError_t launch_col_fft_window(const Geometry& g, const FFTPlan& pl, const float2* rowspec,
+                                 const float2* factor, float2* win, hipStream_t s) {
+  const int CB = std::min(16, std::min(g.w, batch_per_block(pl.n, 4096)));
+  const size_t lds = 2 * (size_t)CB * pl.n * sizeof(float2);
+  hipError_t e = set_lds(k_col_fft_window, lds);
+  if (e != hipSuccess) return e;
+  dim3 grid((g.w + CB - 1) / CB, g.P, g.B);
+  k_col_fft_window<<<grid, kThreads, lds, s>>>(g, pl, rowspec, factor, win, CB);
+  return hipGetLastError();
+}
+
+hipError_t launch_ifft_rows(const Geometry& g, const FFTPlan& pl, float2* win, hipStream_t s) {
+  const int RB = std::min(g.h, batch_per_block(pl.n, 2048));
+  const size_t lds = 2 * (size_t)RB * pl.n * sizeof(float2);
+  hipError_t e = set_lds(k_ifft_rows, lds);
+  if (e != hipSuccess) return e;
+  dim3 grid((g.h + RB - 1) / RB, g.P, g.B);
+  k_ifft_rows<<<grid, kThreads, lds, s>>>(g, pl, win, RB);
+  return hipGetLastError();
+}
+
+hipError_t launch_ifft_cols_tilt(const Geometry& g, const FFTPlan& pl, const float2* win, const float2* ref,
+                                 float2* fields, hipStream_t s) {
+  const int CB = std::min(16, std::min(g.w, batch_per_block(pl.n, 4096)));
+  const size_t lds = 2 * (size_t)CB * pl.n * sizeof(float2);
+  hipError_t e = set_lds(k_ifft_cols_tilt, lds);
+  if (e != hipSuccess) return e;
+  dim3 grid((g.w + CB - 1) / CB, g.P, g.B);
+  k_ifft_cols_tilt<<<grid, kThreads, lds, s>>>(g, pl, win, ref, fields, CB);
+  return hipGetLastError();
+}
+
+template <int MPT>
+static hipError_t overlap_mpt(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
+                              const int2* modes, float2* coefs, size_t lds, hipStream_t s) {
+  hipError_t e = set_lds(k_overlap_fp32<MPT>, lds);
+  if (e != hipSuccess) return e;
+  dim3 grid(g.P, g.B);
+  k_overlap_fp32<MPT><<<grid, kThreads, lds, s>>>(g, fields, hgx, hgy, modes, coefs);
+  return hipGetLastError();
+}
+
+hipError_t launch_overlap_fp32(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
+                               const int2* modes, float2* coefs, hipStream_t s) {
+  const size_t lds = sizeof(float) * ((size_t)g.G * g.w + (((size_t)g.G * g.h + 1) & ~(size_t)1)) +
+                     sizeof(float2) * ((size_t)kOverlapRows * g.w + (size_t)g.G * kOverlapRows);
+  if (lds > 160 * 1024) return hipErrorInvalidValue;
+  const int mpt = (g.M + kThreads - 1) / kThreads;
+  if (mpt <= 1) return overlap_mpt<1>(g, fields, hgx, hgy, modes, coefs, lds, s);
+  if (mpt <= 2) return overlap_mpt<2>(g, fields, hgx, hgy, modes, coefs, lds, s);
+  if (mpt <= 4) return overlap_mpt<4>(g, fields, hgx, hgy, modes, coefs, lds, s);
+  if (mpt <= 8) return overlap_mpt<8>(g, fields, hgx, hgy, modes, coefs, lds, s);
+  if (mpt <= 16) return overlap_mpt<16>(g, fields, hgx, hgy, modes, coefs, lds, s);
+  if (mpt <= 32) return overlap_mpt<32>(g, fields, hgx, hgy, modes, coefs, lds, s);
+  return hipErrorInvalidValue;
+}
+
+}  // namespace dh

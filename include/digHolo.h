@@ -1,0 +1,219 @@
+/*
+ * digHolo-compatible C API of the MI355X batch-processing library
+ * (libdigholo.so, built from digholo_amd/csrc).
+ *
+ * Every declaration below keeps the signature and argument meaning of the
+ * reference declaration it replaces; the citation after each one is
+ * reference digHolo.h:line.  Unlike the reference on Linux (whose EXT_C is
+ * empty, digHolo.h:148-156, so its .so exports C++-mangled names), these are
+ * exported with C linkage.
+ *
+ * Extensions with no reference counterpart are grouped at the end and say so.
+ */
+#ifndef DIGHOLO_MI355X_H
+#define DIGHOLO_MI355X_H
+
+#ifdef __cplusplus
+#define EXT_C extern "C"
+#else
+#define EXT_C
+#endif
+
+typedef float complex64[2];                       /* digHolo.h:163 */
+
+/* Error codes: digHolo.h:171-182 */
+#define DIGHOLO_ERROR_SUCCESS 0
+#define DIGHOLO_ERROR_ERROR 1
+#define DIGHOLO_ERROR_INVALIDHANDLE 2
+#define DIGHOLO_ERROR_NULLPOINTER 3
+#define DIGHOLO_ERROR_SETFRAMEBUFFERDISABLED 4
+#define DIGHOLO_ERROR_INVALIDDIMENSION 5
+#define DIGHOLO_ERROR_INVALIDPOLARISATION 6
+#define DIGHOLO_ERROR_INVALIDAXIS 7
+#define DIGHOLO_ERROR_INVALIDARGUMENT 8
+#define DIGHOLO_ERROR_MEMORYALLOCATION 9
+#define DIGHOLO_ERROR_FILENOTCREATED 10
+#define DIGHOLO_ERROR_FILENOTFOUND 11
+
+/* Averaging modes: digHolo.h:655-658 */
+#define DIGHOLO_AVGMODE_SEQUENTIAL 0
+#define DIGHOLO_AVGMODE_INTERLACED 1
+#define DIGHOLO_AVGMODE_SEQUENTIALSWEEP 2
+#define DIGHOLO_AVGMODE_COUNT 3
+
+/* AutoAlign modes: digHolo.h:925-928 */
+#define DIGHOLO_AUTOALIGNMODE_FULL 0
+#define DIGHOLO_AUTOALIGNMODE_TWEAK 1
+#define DIGHOLO_AUTOALIGNMODE_ESTIMATE 2
+#define DIGHOLO_AUTOALIGNMODE_COUNT 3
+
+/* Basis types: digHolo.h:1348-1350 */
+#define DIGHOLO_BASISTYPE_HG 0
+#define DIGHOLO_BASISTYPE_LG 1
+#define DIGHOLO_BASISTYPE_CUSTOM 2
+
+/* Power-meter metrics: digHolo.h:1363-1372 */
+#define DIGHOLO_METRIC_IL 0
+#define DIGHOLO_METRIC_MDL 1
+#define DIGHOLO_METRIC_DIAG 2
+#define DIGHOLO_METRIC_SNRAVG 3
+#define DIGHOLO_METRIC_DIAGBEST 4
+#define DIGHOLO_METRIC_DIAGWORST 5
+#define DIGHOLO_METRIC_SNRBEST 6
+#define DIGHOLO_METRIC_SNRWORST 7
+#define DIGHOLO_METRIC_SNRMG 8
+#define DIGHOLO_METRIC_COUNT 9
+
+/* Benchmark info indices: digHolo.h:1783-1788 */
+#define DIGHOLO_BENCHMARK_FFT 0
+#define DIGHOLO_BENCHMARK_IFFT 1
+#define DIGHOLO_BENCHMARK_APPLYTILT 2
+#define DIGHOLO_BENCHMARK_BASIS 3
+#define DIGHOLO_BENCHMARK_OVERLAP 4
+#define DIGHOLO_BENCHMARK_TOTAL 5
+
+/* Verbosity: digHolo.h:1906-1909 */
+#define DIGHOLO_VERBOSITY_OFF 0
+#define DIGHOLO_VERBOSITY_BASIC 1
+#define DIGHOLO_VERBOSITY_DEBUG 2
+#define DIGHOLO_VERBOSITY_COOKED 3
+
+/* ---- lifecycle ---- */
+EXT_C int digHoloCreate(void);                                                    /* digHolo.h:209 */
+EXT_C int digHoloDestroy(int handleIdx);                                          /* digHolo.h:216 */
+
+/* ---- frame buffer ---- */
+EXT_C int digHoloSetFrameBuffer(int handleIdx, float* buffer);                    /* digHolo.h:245 */
+EXT_C float* digHoloGetFrameBuffer(int handleIdx);                                /* digHolo.h:255 */
+EXT_C int digHoloSetFrameBufferUint16(int handleIdx, unsigned short* buffer, int transposeMode); /* digHolo.h:268 */
+EXT_C unsigned short* digHoloGetFrameBufferUint16(int handleIdx, int* transposeMode);            /* digHolo.h:277 */
+
+/* ---- camera / frame configuration ---- */
+EXT_C int digHoloConfigSetFrameDimensions(int handleIdx, int width, int height);  /* digHolo.h:298 */
+EXT_C int digHoloConfigGetFrameDimensions(int handleIdx, int* width, int* height);/* digHolo.h:308 */
+EXT_C int digHoloConfigSetFramePixelSize(int handleIdx, float pixelSize);         /* digHolo.h:317 */
+EXT_C float digHoloConfigGetFramePixelSize(int handleIdx);                        /* digHolo.h:325 */
+EXT_C int digHoloConfigSetPolCount(int handleIdx, int polCount);                  /* digHolo.h:334 */
+EXT_C int digHoloConfigGetPolCount(int handleIdx);                                /* digHolo.h:344 */
+EXT_C int digHoloConfigSetFillFactorCorrectionEnabled(int handleIdx, int enabled);/* digHolo.h:449 */
+EXT_C int digHoloConfigGetFillFactorCorrectionEnabled(int handleIdx);             /* digHolo.h:460 */
+
+/* ---- batch ---- */
+EXT_C int digHoloSetBatch(int handleIdx, int batchCount, float* frameBuffer);     /* digHolo.h:483 */
+EXT_C int digHoloSetBatchAvg(int handleIdx, int batchCount, float* frameBuffer, int avgCount, int avgMode); /* digHolo.h:502 */
+EXT_C int digHoloSetBatchUint16(int handleIdx, int batchCount, unsigned short* frameBuffer, int transpose); /* digHolo.h:519 */
+EXT_C int digHoloSetBatchAvgUint16(int handleIdx, int batchCount, unsigned short* frameBuffer, int avgCount, int avgMode, int transpose); /* digHolo.h:540 */
+EXT_C int digHoloConfigSetBatchCount(int handleIdx, int batchCount);              /* digHolo.h:551 */
+EXT_C int digHoloConfigGetBatchCount(int handleIdx);                              /* digHolo.h:559 */
+EXT_C int digHoloConfigSetBatchAvgCount(int handleIdx, int avgCount);             /* digHolo.h:570 */
+EXT_C int digHoloConfigGetBatchAvgCount(int handleIdx);                           /* digHolo.h:580 */
+EXT_C int digHoloConfigSetBatchAvgMode(int handleIdx, int avgMode);               /* digHolo.h:670 */
+EXT_C int digHoloConfigGetBatchAvgMode(int handleIdx);                            /* digHolo.h:680 */
+
+/* ---- FFT / IFFT windows ---- */
+EXT_C int digHoloConfigSetfftWindowSize(int handleIdx, int width, int height);    /* digHolo.h:701 */
+EXT_C int digHoloConfigGetfftWindowSize(int handleIdx, int* width, int* height);  /* digHolo.h:712 */
+EXT_C int digHoloConfigSetfftWindowSizeX(int handleIdx, int width);               /* digHolo.h:722 */
+EXT_C int digHoloConfigGetfftWindowSizeX(int handleIdx);                          /* digHolo.h:730 */
+EXT_C int digHoloConfigGetfftWindowSizeY(int handleIdx);                          /* digHolo.h:739 */
+EXT_C int digHoloConfigSetfftWindowSizeY(int handleIdx, int height);              /* digHolo.h:749 */
+EXT_C int digHoloConfigSetFourierWindowRadius(int handleIdx, float windowRadius); /* digHolo.h:766 */
+EXT_C float digHoloConfigGetFourierWindowRadius(int handleIdx);                   /* digHolo.h:774 */
+EXT_C int digHoloConfigSetIFFTResolutionMode(int handleIdx, int resolutionModeIdx);/* digHolo.h:786 */
+EXT_C int digHoloConfigGetIFFTResolutionMode(int handleIdx);                      /* digHolo.h:797 */
+EXT_C int digHoloConfigSetWavelengthCentre(int handleIdx, float lambda0);         /* digHolo.h:814 */
+EXT_C float digHoloConfigGetWavelengthCentre(int handleIdx);                      /* digHolo.h:822 */
+
+/* ---- AutoAlign switches ---- */
+EXT_C int digHoloConfigSetAutoAlignBeamCentre(int handleIdx, int enable);         /* digHolo.h:938 */
+EXT_C int digHoloConfigGetAutoAlignBeamCentre(int handleIdx);                     /* digHolo.h:946 */
+EXT_C int digHoloConfigSetAutoAlignDefocus(int handleIdx, int enable);            /* digHolo.h:955 */
+EXT_C int digHoloConfigGetAutoAlignDefocus(int handleIdx);                        /* digHolo.h:963 */
+EXT_C int digHoloConfigSetAutoAlignTilt(int handleIdx, int enable);               /* digHolo.h:972 */
+EXT_C int digHoloConfigGetAutoAlignTilt(int handleIdx);                           /* digHolo.h:980 */
+EXT_C int digHoloConfigSetAutoAlignBasisWaist(int handleIdx, int enable);         /* digHolo.h:989 */
+EXT_C int digHoloConfigGetAutoAlignBasisWaist(int handleIdx);                     /* digHolo.h:997 */
+EXT_C int digHoloConfigSetAutoAlignFourierWindowRadius(int handleIdx, int enable);/* digHolo.h:1008 */
+EXT_C int digHoloConfigGetAutoAlignFourierWindowRadius(int handleIdx);            /* digHolo.h:1016 */
+EXT_C int digHoloConfigSetAutoAlignTol(int handleIdx, float tolerance);           /* digHolo.h:1025 */
+EXT_C float digHoloConfigGetAutoAlignTol(int handleIdx);                          /* digHolo.h:1033 */
+EXT_C int digHoloConfigSetAutoAlignPolIndependence(int handleIdx, int polIndependence); /* digHolo.h:1046 */
+EXT_C int digHoloConfigGetAutoAlignPolIndependence(int handleIdx);                /* digHolo.h:1058 */
+EXT_C int digHoloConfigSetAutoAlignBasisMulConjTrans(int handleIdx, int mulConjTranspose); /* digHolo.h:1071 */
+EXT_C int digHoloConfigGetAutoAlignBasisMulConjTrans(int handleIdx);              /* digHolo.h:1083 */
+EXT_C int digHoloConfigSetAutoAlignMode(int handleIdx, int alignMode);            /* digHolo.h:1101 */
+EXT_C int digHoloConfigGetAutoAlignMode(int handleIdx);                           /* digHolo.h:1109 */
+EXT_C int digHoloConfigSetAutoAlignGoalIdx(int handleIdx, int goalIdx);           /* digHolo.h:1128 */
+EXT_C int digHoloConfigGetAutoAlignGoalIdx(int handleIdx);                        /* digHolo.h:1136 */
+
+/* ---- alignment parameters ---- */
+EXT_C int digHoloConfigSetBeamCentre(int handleIdx, int axisIdx, int polIdx, float centre); /* digHolo.h:1205 */
+EXT_C float digHoloConfigGetBeamCentre(int handleIdx, int axisIdx, int polIdx);   /* digHolo.h:1215 */
+EXT_C int digHoloConfigSetTilt(int handleIdx, int axisIdx, int polIdx, float tilt);/* digHolo.h:1226 */
+EXT_C float digHoloConfigGetTilt(int handleIdx, int axisIdx, int polIdx);         /* digHolo.h:1236 */
+EXT_C int digHoloConfigSetDefocus(int handleIdx, int polIdx, float defocus);      /* digHolo.h:1246 */
+EXT_C float digHoloConfigGetDefocus(int handleIdx, int polIdx);                   /* digHolo.h:1255 */
+EXT_C int digHoloConfigSetPolLockTilt(int handleIdx, int polLock);                /* digHolo.h:1270 */
+EXT_C int digHoloConfigGetPolLockTilt(int handleIdx);                             /* digHolo.h:1282 */
+EXT_C int digHoloConfigSetPolLockDefocus(int handleIdx, int polLock);             /* digHolo.h:1297 */
+EXT_C int digHoloConfigGetPolLockDefocus(int handleIdx);                          /* digHolo.h:1309 */
+EXT_C int digHoloConfigSetPolLockBasisWaist(int handleIdx, int polLock);          /* digHolo.h:1318 */
+EXT_C int digHoloConfigGetPolLockBasisWaist(int handleIdx);                       /* digHolo.h:1326 */
+
+/* ---- basis ---- */
+EXT_C int digHoloConfigSetBasisGroupCount(int handleIdx, int groupCount);         /* digHolo.h:1384 */
+EXT_C int digHoloConfigGetBasisGroupCount(int handleIdx);                         /* digHolo.h:1393 */
+EXT_C int digHoloConfigSetBasisWaist(int handleIdx, int polIdx, float waist);     /* digHolo.h:1404 */
+EXT_C float digHoloConfigGetBasisWaist(int handleIdx, int polIdx);                /* digHolo.h:1416 */
+EXT_C int digHoloConfigSetBasisTypeHG(int handleIdx);                             /* digHolo.h:1424 */
+EXT_C int digHoloConfigSetBasisType(int handleIdx, int basisType);                /* digHolo.h:1464 */
+EXT_C int digHoloConfigGetBasisType(int handleIdx);                               /* digHolo.h:1472 */
+
+/* ---- processing ---- */
+EXT_C complex64* digHoloProcessBatch(int handleIdx, int* batchCount, int* modeCount, int* polCount); /* digHolo.h:1494 */
+EXT_C complex64* digHoloGetFields(int handleIdx, int* batchCount, int* polCount, float** x, float** y, int* width, int* height); /* digHolo.h:1549 */
+EXT_C complex64* digHoloBasisGetCoefs(int handleIdx, int* batchCount, int* modeCount, int* polCount); /* digHolo.h:1603 */
+
+/* ---- threads / console ---- */
+EXT_C int digHoloConfigSetThreadCount(int handleIdx, int threadCount);            /* digHolo.h:1749 */
+EXT_C int digHoloConfigGetThreadCount(int handleIdx);                             /* digHolo.h:1757 */
+EXT_C float digHoloBenchmark(int handleIdx, float goalDuration, float* info);     /* digHolo.h:1776 */
+EXT_C int digHoloConfigSetVerbosity(int handleIdx, int verbosity);                /* digHolo.h:1924 */
+EXT_C int digHoloConfigGetVerbosity(int handleIdx);                               /* digHolo.h:1937 */
+
+/* =====================================================================
+ * Extensions (no counterpart in the reference)
+ * ===================================================================== */
+
+/* Overlap precision: 0 = plain FP32, 1 = split-FP16 MFMA (hi+lo operands,
+ * FP32 accumulate).  Default 0. */
+#define DIGHOLO_PRECISION_FP32 0
+#define DIGHOLO_PRECISION_SPLIT16 1
+EXT_C int digHoloConfigSetOverlapPrecision(int handleIdx, int precision);
+EXT_C int digHoloConfigGetOverlapPrecision(int handleIdx);
+
+/* Same work as digHoloProcessBatch, but asynchronous on the handle's stream and
+ * returning a DEVICE pointer to the coefficients [batch][pol][mode]; the
+ * caller synchronises (digHoloSynchronize) before reading it.  Frames set
+ * with SetBatch* may be host or device pointers. */
+EXT_C complex64* digHoloProcessBatchDevice(int handleIdx, int* batchCount, int* modeCount, int* polCount);
+/* Device pointer to the reconstructed fields [batch][pol][height][width]. */
+EXT_C complex64* digHoloGetFieldsDevice(int handleIdx, int* batchCount, int* polCount, int* width, int* height);
+/* Blocks until all work queued on the handle's stream is done. */
+EXT_C int digHoloSynchronize(int handleIdx);
+/* Runs the work of the handle on an existing HIP stream (hipStream_t passed as
+ * void*); NULL reverts to the handle's own stream. */
+EXT_C int digHoloSetStream(int handleIdx, void* stream);
+EXT_C void* digHoloGetStream(int handleIdx);
+
+/* Window geometry derived from the current configuration, without touching
+ * the GPU: out[0..11] = W, H, w, h, cx0, cy0, cx1, cy1, SX0, SY0, SX1, SY1. */
+EXT_C int digHoloGetWindowGeometry(int handleIdx, int* out);
+
+/* Per-stage GPU time of the most recent processing call, measured with HIP
+ * events on the handle's stream when enabled:  ms[0..4] = row FFT, column FFT
+ * + Fourier window, IFFT + tilt, HG basis/tables, overlap. */
+EXT_C int digHoloConfigSetStageTiming(int handleIdx, int enabled);
+EXT_C int digHoloGetStageTimes(int handleIdx, float* ms);
+
+#endif

@@ -1,0 +1,186 @@
+"""Parity of the HIP path (through the C-ABI) with the FP64 oracle.
+
+Bar: relative L2 error of fields and coefficients <= 1e-5 for the FP32 path
+(the tolerance the batch path is specified to), on seeded frames of every
+window-size family the path meets (radix 2, 3, 5 transforms), single and dual
+polarisation, uint16 and float32 input, host and device frame pointers, a
+window that wraps around ky, defocus, pol locks, G = 0, and a second call on
+the same handle with a smaller batch.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from digholo_amd import api, synth
+from oracle import digholo_oracle as O
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("needs a GPU")
+    return api.load()
+
+
+def make_case(name, B, seed, quantise=True, **over):
+    if name == "small":
+        spec = synth.spec_small()
+        cfg = dict(frame_width=320, frame_height=256, pol_count=1, fft_window=(128, 128), waist=200e-6,
+                   group_count=10)
+    elif name == "dual640":
+        spec = synth.spec_dual_640()
+        cfg = dict(frame_width=640, frame_height=512, pol_count=2, fft_window=(256, 256), waist=300e-6,
+                   group_count=45, beam_centre_x=(-1.6e-3, 1.6e-3))
+    elif name == "dual640w192":
+        spec = synth.spec_dual_640()
+        spec.group_count = 30
+        cfg = dict(frame_width=640, frame_height=512, pol_count=2, fft_window=(192, 192), waist=300e-6,
+                   group_count=30, beam_centre_x=(-1.6e-3, 1.6e-3))
+    elif name == "radix5":
+        spec = synth.HologramSpec(frame_width=320, frame_height=320, pol_count=1, waist=150e-6, group_count=12)
+        cfg = dict(frame_width=320, frame_height=320, pol_count=1, fft_window=(160, 160), waist=150e-6,
+                   group_count=12)
+    elif name == "large1024":
+        spec = synth.HologramSpec(frame_width=1024, frame_height=1024, pol_count=2, waist=300e-6,
+                                  group_count=80, beam_centre_x=[-2.56e-3, 2.56e-3], group_decay=20.0)
+        cfg = dict(frame_width=1024, frame_height=1024, pol_count=2, fft_window=(512, 512), waist=300e-6,
+                   group_count=80, beam_centre_x=(-2.56e-3, 2.56e-3))
+    else:
+        raise KeyError(name)
+    for k, v in over.pop("spec", {}).items():
+        setattr(spec, k, v)
+    cfg.update(over)
+    frames, _, _, _ = synth.make_batch(spec, B, seed=seed, quantise=quantise)
+    if not quantise:
+        frames = frames.astype(np.float32)
+    return frames, cfg
+
+
+def oracle_for(cfg):
+    P = cfg.get("pol_count", 1)
+    two = lambda k, d: list(cfg.get(k, d))
+    return O.Config(frame_width=cfg["frame_width"], frame_height=cfg["frame_height"], pol_count=P,
+                    pixel_size=cfg.get("pixel_size", 20e-6), fft_window_x=cfg["fft_window"][0],
+                    fft_window_y=cfg["fft_window"][1], aperture_deg=cfg.get("aperture_deg", 0.7),
+                    wavelength=cfg.get("wavelength", 1565e-9),
+                    beam_centre_x=two("beam_centre_x", (0.0, 0.0)), beam_centre_y=two("beam_centre_y", (0.0, 0.0)),
+                    tilt_x=two("tilt_x", (1.5, 1.5)), tilt_y=two("tilt_y", (1.5, 1.5)),
+                    defocus=two("defocus", (0.0, 0.0)), waist=[cfg["waist"]] * 2,
+                    group_count=cfg["group_count"], fill_factor=cfg.get("fill_factor", 1),
+                    pol_lock_tilt=cfg.get("pol_lock_tilt", 0), pol_lock_defocus=cfg.get("pol_lock_defocus", 0),
+                    pol_lock_waist=cfg.get("pol_lock_waist", 1))
+
+
+def run_gpu(lib, frames, cfg, h=None):
+    if h is None:
+        h = lib.digHoloCreate()
+        api.configure(lib, h, **cfg)
+    coefs, fields, x, y = api.process_batch(lib, h, frames)
+    return coefs, fields, x, y, h
+
+
+def check(frames, cfg, coefs, fields, tol=TOL):
+    f_ref, c_ref, plan = O.process_batch(frames, oracle_for(cfg))
+    assert fields.shape == f_ref.shape
+    ef = O.relative_l2(fields, f_ref)
+    print(f"fields rel-L2 {ef:.3e}", flush=True)
+    assert ef <= tol
+    if cfg["group_count"] > 0:
+        assert coefs.shape == c_ref.shape
+        ec = O.relative_l2(coefs, c_ref)
+        print(f"coefs rel-L2 {ec:.3e}", flush=True)
+        assert ec <= tol
+        # per-frame, per-pol too: no frame may hide behind the batch norm
+        for b in range(coefs.shape[0]):
+            for p in range(coefs.shape[1]):
+                assert O.relative_l2(coefs[b, p], c_ref[b, p]) <= 5 * tol
+    else:
+        assert coefs is None
+    return plan
+
+
+@pytest.mark.parametrize("name,B", [("small", 8), ("dual640", 3), ("dual640w192", 3), ("radix5", 4)])
+def test_parity_uint16(lib, name, B):
+    frames, cfg = make_case(name, B, seed=11)
+    print(f"case {name} B={B} frames {frames.shape}", flush=True)
+    coefs, fields, x, y, h = run_gpu(lib, frames, cfg)
+    plan = check(frames, cfg, coefs, fields)
+    # reconstructed axes: (i - w/2) * px W / w (digHolo.cpp:8184-8189)
+    px = np.float32(20e-6)
+    assert np.allclose(x, (np.arange(plan.w) - plan.w // 2) * np.float32(px * plan.W / plan.w), rtol=1e-6)
+
+
+def test_parity_float_frames_wrapped_window_defocus(lib):
+    frames, cfg = make_case("dual640", 2, seed=5, quantise=False,
+                            tilt_x=(1.5, 1.4), tilt_y=(1.5, -1.6), defocus=(0.3, -0.2),
+                            beam_centre_y=(40e-6, -70e-6), fill_factor=0,
+                            spec=dict(tilt_x_deg=[1.5, 1.4], tilt_y_deg=[1.5, -1.6], defocus=[0.3, -0.2],
+                                      beam_centre_y=[40e-6, -70e-6], group_count=45))
+    assert frames.dtype == np.float32
+    coefs, fields, _, _, _ = run_gpu(lib, frames, cfg)
+    plan = check(frames, cfg, coefs, fields)
+    assert plan.SY[1] < 0
+
+
+def test_parity_pol_locks(lib):
+    frames, cfg = make_case("dual640", 2, seed=6, tilt_x=(1.5, 1.2), tilt_y=(1.5, 1.1), defocus=(0.4, -0.3),
+                            pol_lock_tilt=1, pol_lock_defocus=1)
+    coefs, fields, _, _, h = run_gpu(lib, frames, cfg)
+    check(frames, cfg, coefs, fields)
+    # the lock writes pol 0's values into pol 1 of the configuration
+    assert abs(lib.digHoloConfigGetTilt(h, 0, 1) - 1.5) < 1e-7
+    assert abs(lib.digHoloConfigGetDefocus(h, 1) - 0.4) < 1e-7
+
+
+def test_group_count_zero_gives_fields_only(lib):
+    frames, cfg = make_case("small", 3, seed=7, group_count=0)
+    coefs, fields, _, _, _ = run_gpu(lib, frames, cfg)
+    check(frames, cfg, coefs, fields)
+
+
+def test_second_call_smaller_batch_and_new_config(lib):
+    frames, cfg = make_case("small", 6, seed=8)
+    coefs, fields, _, _, h = run_gpu(lib, frames, cfg)
+    check(frames, cfg, coefs, fields)
+    frames2 = frames[:2].copy()
+    cfg2 = dict(cfg, tilt_x=(1.45, 1.45), beam_centre_x=(30e-6, 0.0))
+    api.configure(lib, h, **cfg2)
+    coefs2, fields2, _, _, _ = run_gpu(lib, frames2, cfg2, h=h)
+    check(frames2, cfg2, coefs2, fields2)
+
+
+def test_device_pointer_frames_and_device_outputs(lib):
+    import torch
+    frames, cfg = make_case("dual640", 3, seed=9)
+    h = lib.digHoloCreate()
+    api.configure(lib, h, **cfg)
+    dev = torch.from_numpy(frames.astype(np.int32)).to(torch.int16).cuda()   # same bits as uint16 < 32768
+    torch.cuda.synchronize()
+    assert lib.digHoloSetBatchUint16(h, frames.shape[0], C.c_void_p(dev.data_ptr()), 0) == 0
+    b, m, p = C.c_int(), C.c_int(), C.c_int()
+    ptr = lib.digHoloProcessBatchDevice(h, C.byref(b), C.byref(m), C.byref(p))
+    assert ptr and (b.value, p.value, m.value) == (3, 2, 1035)
+    assert lib.digHoloSynchronize(h) == 0
+    nbytes = b.value * p.value * m.value * 8
+    host = np.empty(nbytes // 8, dtype=np.complex64)
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    assert hip.hipMemcpy(host.ctypes.data, ptr, nbytes, 2) == 0          # hipMemcpyDeviceToHost
+    # the host API path on the same handle returns the same numbers
+    hb, hm, hp = C.c_int(), C.c_int(), C.c_int()
+    hptr = lib.digHoloBasisGetCoefs(h, C.byref(hb), C.byref(hm), C.byref(hp))
+    again = np.ctypeslib.as_array(C.cast(hptr, C.POINTER(C.c_float)), shape=(nbytes // 4,)).view(np.complex64)
+    assert np.array_equal(again, host)
+    _, c_ref, _ = O.process_batch(frames, oracle_for(cfg))
+    assert O.relative_l2(host.reshape(c_ref.shape), c_ref) <= TOL
+
+
+def test_large_window_high_order(lib):
+    frames, cfg = make_case("large1024", 1, seed=12)
+    coefs, fields, _, _, _ = run_gpu(lib, frames, cfg)
+    check(frames, cfg, coefs, fields)
