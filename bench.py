@@ -1,0 +1,199 @@
+"""Benchmark of the digHolo batch path on one MI355X.
+
+Workload: 640x512 uint16 frames, dual polarisation, batch 1000, FFT window
+256x256 per pol (IFFT window 80x80), 45 HG groups (1035 modes per pol).  One
+step = digHoloProcessBatchDevice over the whole batch with the frames already
+resident in HBM, fields and coefficients left in HBM.
+
+Prints one JSON line (see DESIGN.md for the definitions of the roofline and
+CPU-baseline fields).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "frames/s end-to-end 640x512 dual-pol, 45 HG groups: split-BF16 MFMA vs FP32"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+B = 1000
+FRAME_W, FRAME_H, POLS, WIN, GROUPS, WAIST = 640, 512, 2, 256, 45, 300e-6
+BEAM_X = (-1.6e-3, 1.6e-3)
+
+
+def algorithmic_bytes_per_frame(P, W, H, w, h, M):
+    """Window input (uint16) + coefficients + fields out, per frame (all pols)."""
+    return P * W * H * 2 + P * M * 8 + P * w * h * 8
+
+
+def fft_stage_bytes_per_frame(P, W, H, w, h):
+    """Window input (uint16) + the pruned spectrum window, per frame."""
+    return P * W * H * 2 + P * w * h * 8
+
+
+def overlap_flops_per_frame(P, G, w, h):
+    """Separable overlap, real HG x complex field = 4 flop/MAC, triangle in stage 2."""
+    M = G * (G + 1) // 2
+    return P * 4 * (G * w * h + M * h)
+
+
+def make_frames(n_unique=100, seed=1):
+    """n_unique rendered holograms; every one of the B frames gets its own read noise."""
+    import numpy as np
+    import torch
+    from digholo_amd import synth
+    spec = synth.spec_dual_640()
+    rng = np.random.default_rng(seed)
+    coefs = synth.random_coefs(spec, n_unique, rng)
+    clean, _, _ = synth.render(spec, coefs, quantise=False)
+    base = torch.from_numpy(clean.astype(np.float32)).cuda()
+    out = torch.empty((B, FRAME_H, FRAME_W), dtype=torch.int16, device="cuda")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    for b0 in range(0, B, n_unique):
+        n = min(n_unique, B - b0)
+        noisy = base[:n] + spec.noise_counts * torch.randn(base[:n].shape, generator=g, device="cuda")
+        out[b0:b0 + n] = noisy.round().clamp(0, 4095).to(torch.int16)
+    torch.cuda.synchronize()
+    return out
+
+
+def cpu_baseline(min_s=10.0, max_s=30.0):
+    """FP64 oracle timed on this host on a bounded sample of the same workload."""
+    import numpy as np
+    from digholo_amd import synth
+    from oracle import digholo_oracle as O
+    spec = synth.spec_dual_640()
+    frames, _, _, _ = synth.make_batch(spec, 20, seed=7)
+    cfg = O.Config(frame_width=FRAME_W, frame_height=FRAME_H, pol_count=POLS, fft_window_x=WIN,
+                   fft_window_y=WIN, aperture_deg=0.7, tilt_x=[1.5, 1.5], tilt_y=[1.5, 1.5],
+                   group_count=GROUPS, waist=[WAIST, WAIST], beam_centre_x=list(BEAM_X))
+    done, t0 = 0, time.perf_counter()
+    while True:
+        O.process_batch(frames, cfg)
+        done += frames.shape[0]
+        el = time.perf_counter() - t0
+        if el >= min_s or el >= max_s:
+            break
+    return {"value": done / el, "unit": "frames/s", "cores": 1, "kind": "port",
+            "sample": f"{done} frames of the same workload through the FP64 numpy oracle "
+                      f"(single-threaded pocketfft + einsum), {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", choices=["fp32", "split16"], default="split16")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from digholo_amd import api
+
+    lib = api.load()
+    torch.cuda.set_device(0)
+    frames = make_frames()
+    h = lib.digHoloCreate()
+    api.configure(lib, h, frame_width=FRAME_W, frame_height=FRAME_H, pol_count=POLS, fft_window=(WIN, WIN),
+                  waist=WAIST, group_count=GROUPS, beam_centre_x=BEAM_X)
+    assert lib.digHoloSetBatchUint16(h, B, C.c_void_p(frames.data_ptr()), 0) == 0
+    geo = api.window_geometry(lib, h)
+    b, m, p = C.c_int(), C.c_int(), C.c_int()
+
+    def step():
+        ptr = lib.digHoloProcessBatchDevice(h, C.byref(b), C.byref(m), C.byref(p))
+        if not ptr:
+            raise RuntimeError("digHoloProcessBatchDevice failed")
+
+    step()
+    assert lib.digHoloSynchronize(h) == 0
+    # time on the stream the kernels run on (the handle's own HIP stream)
+    stream = torch.cuda.ExternalStream(lib.digHoloGetStream(h))
+
+    def timed(precision):
+        assert lib.digHoloConfigSetOverlapPrecision(h, precision) == 0
+        for _ in range(args.warmup):
+            step()
+        lib.digHoloSynchronize(h)
+        lib.digHoloConfigSetStageTiming(h, 1)
+        stage = (C.c_float * 5)()
+        lib.digHoloGetStageTimes(h, stage)          # drop anything collected so far
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        end.record(stream)
+        end.synchronize()
+        wall = time.perf_counter() - t0
+        lib.digHoloGetStageTimes(h, stage)
+        lib.digHoloConfigSetStageTiming(h, 0)
+        return start.elapsed_time(end) / args.steps, [float(x) for x in stage], wall
+
+    ms32, st32, _ = timed(api.DIGHOLO_PRECISION_FP32)
+    ms16, st16, wall = timed(api.DIGHOLO_PRECISION_SPLIT16)
+    headline = api.DIGHOLO_PRECISION_SPLIT16 if args.precision == "split16" else api.DIGHOLO_PRECISION_FP32
+    ms_per_step, st = (ms16, st16) if headline == api.DIGHOLO_PRECISION_SPLIT16 else (ms32, st32)
+
+    Wd, Hd, w, hh, M = geo["W"], geo["H"], geo["w"], geo["h"], m.value
+    frames_s = B / (ms_per_step * 1e-3)
+    fft_ms = st[0] + st[1]
+    fft_bytes = fft_stage_bytes_per_frame(POLS, Wd, Hd, w, hh) * B
+    achieved = fft_bytes / (fft_ms * 1e-3) / 1e9
+    ov_flops = overlap_flops_per_frame(POLS, GROUPS, w, hh) * B
+    out = {
+        "metric": METRIC,
+        "value": round(frames_s, 1),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded HG holograms, 12-bit uint16 with read noise)",
+        "config": {"workload": "640x512 uint16 dual-pol, batch 1000, FFT 256x256 -> IFFT 80x80, "
+                               "45 HG groups (1035 modes/pol), frames resident in HBM",
+                   "global_batch": B, "overlap_precision": "split16" if headline else "fp32",
+                   "fft_window": Wd, "ifft_window": w, "modes_per_pol": M, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "k_row_fft + k_col_fft_window (FFT stage)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": fft_bytes, "avg_launch_ms": round(fft_ms, 4)},
+        "fp32_overlap": {"frames_s": round(B / (ms32 * 1e-3), 1), "ms_per_step": round(ms32, 4),
+                         "overlap_ms": round(st32[4], 4)},
+        "split16_overlap": {"frames_s": round(B / (ms16 * 1e-3), 1), "ms_per_step": round(ms16, 4),
+                            "overlap_ms": round(st16[4], 4)},
+        "split_vs_fp32_end_to_end": round(ms32 / ms16, 3),
+        "split_vs_fp32_overlap_kernel": round(st32[4] / st16[4], 3) if st16[4] > 0 else None,
+        "stage_ms": {"row_fft": st[0], "col_fft_window": st[1], "ifft_tilt": st[2], "tables": st[3],
+                     "overlap": st[4]},
+        "overlap_tflops": round(ov_flops / (st[4] * 1e-3) / 1e12, 3) if st[4] > 0 else None,
+        "end_to_end_hbm_frac": round(algorithmic_bytes_per_frame(POLS, Wd, Hd, w, hh, M) * frames_s / 1e9
+                                     / HBM_PEAK_GBS, 4),
+        "host_wall_s": round(wall, 3),
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

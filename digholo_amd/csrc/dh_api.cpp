@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -125,7 +126,6 @@ struct Handle {
   DevBuf<float2> rowspec, win, fields, coefs, factor, ref, twW;
   DevBuf<float> hgx, hgy;
   DevBuf<uint8_t> mask;
-  DevBuf<int2> modes;
   std::map<int, std::unique_ptr<FFTEntry>> ffts;
   int twWn = 0;
 
@@ -138,9 +138,13 @@ struct Handle {
   bool coefsHostValid = false, fieldsHostValid = false;
   std::vector<float> xAxis, yAxis;
 
+  // Stage timing: one set of 6 events per batch enqueued since the last
+  // collection, so back-to-back batches need no host synchronisation.
   bool stageTiming = false;
-  hipEvent_t ev[6] = {};
-  float stageMs[5] = {};
+  std::vector<std::array<hipEvent_t, 6>> evPool;
+  size_t evUsed = 0;
+  float stageMs[5] = {};   // average over the collected batches
+  int stageCount = 0;
 };
 
 std::vector<Handle*> g_handles;     // index = handle; nullptr once destroyed
@@ -401,8 +405,6 @@ int ensure_device(Handle& h) {
   }
   if (hipStreamCreateWithFlags(&h.ownStream, hipStreamNonBlocking) != hipSuccess) return DIGHOLO_ERROR_ERROR;
   if (!h.stream) h.stream = h.ownStream;
-  for (auto& e : h.ev)
-    if (hipEventCreate(&e) != hipSuccess) return DIGHOLO_ERROR_ERROR;
   h.devInit = true;
   return DIGHOLO_ERROR_SUCCESS;
 }
@@ -492,7 +494,16 @@ int enqueue_batch(Handle& h) {
   }
 
   const bool timing = h.stageTiming;
-  if (timing) DH_TRY(hipEventRecord(h.ev[0], h.stream), "event");
+  hipEvent_t* ev = nullptr;
+  if (timing) {
+    if (h.evUsed == h.evPool.size()) {
+      std::array<hipEvent_t, 6> set{};
+      for (auto& e : set) DH_TRY(hipEventCreate(&e), "event create");
+      h.evPool.push_back(set);
+    }
+    ev = h.evPool[h.evUsed++].data();
+    DH_TRY(hipEventRecord(ev[0], h.stream), "event");
+  }
   if (!h.tablesValid || !same_tables(d, h.tabKey)) {
     const size_t T = P * g.h * g.w;
     DH_TRY(h.mask.ensure(T), "alloc");
@@ -501,30 +512,24 @@ int enqueue_batch(Handle& h) {
     DH_TRY(h.hgx.ensure(std::max<size_t>(P * g.G * g.w, 1)), "alloc");
     DH_TRY(h.hgy.ensure(std::max<size_t>(P * g.G * g.h, 1)), "alloc");
     DH_TRY(hipMemcpyAsync(h.mask.p, d.mask.data(), T, hipMemcpyHostToDevice, h.stream), "mask upload");
-    std::vector<int2> modes;
-    for (int gg = 0; gg < g.G; ++gg)
-      for (int k = 0; k <= gg; ++k) modes.push_back(make_int2(gg - k, k));   // digHolo.cpp:1252-1262
-    DH_TRY(h.modes.ensure(std::max<size_t>(modes.size(), 1)), "alloc");
-    if (!modes.empty())
-      DH_TRY(hipMemcpyAsync(h.modes.p, modes.data(), modes.size() * sizeof(int2), hipMemcpyHostToDevice,
-                            h.stream), "modes");
     DH_TRY(dh::launch_tables(d.tab, h.mask.p, h.factor.p, h.ref.p, h.hgx.p, h.hgy.p, h.stream), "tables");
-    DH_TRY(hipStreamSynchronize(h.stream), "tables sync");   // the host vectors above go out of scope
+    DH_TRY(hipStreamSynchronize(h.stream), "tables sync");   // the host mask vector goes out of scope
     h.tabKey = d;
     h.tablesValid = true;
   }
-  if (timing) DH_TRY(hipEventRecord(h.ev[1], h.stream), "event");
+  if (timing) DH_TRY(hipEventRecord(ev[1], h.stream), "event");
   DH_TRY(dh::launch_row_fft(g, *pRow, h.twW.p, dframes, h.frameU16, h.rowspec.p, h.stream), "row FFT");
-  if (timing) DH_TRY(hipEventRecord(h.ev[2], h.stream), "event");
+  if (timing) DH_TRY(hipEventRecord(ev[2], h.stream), "event");
   DH_TRY(dh::launch_col_fft_window(g, *pCol, h.rowspec.p, h.factor.p, h.win.p, h.stream), "column FFT");
-  if (timing) DH_TRY(hipEventRecord(h.ev[3], h.stream), "event");
+  if (timing) DH_TRY(hipEventRecord(ev[3], h.stream), "event");
   DH_TRY(dh::launch_ifft_rows(g, *pIx, h.win.p, h.stream), "IFFT rows");
   DH_TRY(dh::launch_ifft_cols_tilt(g, *pIy, h.win.p, h.ref.p, h.fields.p, h.stream), "IFFT columns");
-  if (timing) DH_TRY(hipEventRecord(h.ev[4], h.stream), "event");
+  if (timing) DH_TRY(hipEventRecord(ev[4], h.stream), "event");
   if (g.G > 0) {
-    DH_TRY(dh::launch_overlap_fp32(g, h.fields.p, h.hgx.p, h.hgy.p, h.modes.p, h.coefs.p, h.stream), "overlap");
+    DH_TRY(dh::launch_overlap_mfma(g, h.fields.p, h.hgx.p, h.hgy.p, h.coefs.p,
+                                   c.precision == DIGHOLO_PRECISION_SPLIT16, h.stream), "overlap");
   }
-  if (timing) DH_TRY(hipEventRecord(h.ev[5], h.stream), "event");
+  if (timing) DH_TRY(hipEventRecord(ev[5], h.stream), "event");
 
   h.cur = d;
   h.haveBatch = true;
@@ -538,14 +543,25 @@ int enqueue_batch(Handle& h) {
   return DIGHOLO_ERROR_SUCCESS;
 }
 
+// Averages the stage times of every batch enqueued with timing on since the
+// last collection (waits for the last one to finish).
 void collect_stage_times(Handle& h) {
-  if (!h.stageTiming) return;
-  (void)hipEventSynchronize(h.ev[5]);
-  (void)hipEventElapsedTime(&h.stageMs[0], h.ev[1], h.ev[2]);   // row FFT
-  (void)hipEventElapsedTime(&h.stageMs[1], h.ev[2], h.ev[3]);   // column FFT + window
-  (void)hipEventElapsedTime(&h.stageMs[2], h.ev[3], h.ev[4]);   // IFFT + tilt
-  (void)hipEventElapsedTime(&h.stageMs[3], h.ev[0], h.ev[1]);   // tables (0 when cached)
-  (void)hipEventElapsedTime(&h.stageMs[4], h.ev[4], h.ev[5]);   // overlap
+  if (h.evUsed == 0) return;
+  (void)hipEventSynchronize(h.evPool[h.evUsed - 1][5]);
+  double acc[5] = {0, 0, 0, 0, 0};
+  // stage k spans events (from[k], to[k]): row FFT, column FFT + window,
+  // IFFT + tilt, tables (0 when cached), overlap
+  const int from[5] = {1, 2, 3, 0, 4}, to[5] = {2, 3, 4, 1, 5};
+  for (size_t i = 0; i < h.evUsed; ++i) {
+    for (int k = 0; k < 5; ++k) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, h.evPool[i][from[k]], h.evPool[i][to[k]]);
+      acc[k] += ms;
+    }
+  }
+  for (int k = 0; k < 5; ++k) h.stageMs[k] = (float)(acc[k] / h.evUsed);
+  h.stageCount = (int)h.evUsed;
+  h.evUsed = 0;
 }
 
 int download_coefs(Handle& h) {
@@ -585,11 +601,11 @@ void destroy_handle(Handle* h) {
     h->hgx.release();
     h->hgy.release();
     h->mask.release();
-    h->modes.release();
     for (auto& kv : h->ffts) kv.second->tw.release();
     h->hCoefs.release();
     h->hFields.release();
-    for (auto& e : h->ev) (void)hipEventDestroy(e);
+    for (auto& set : h->evPool)
+      for (auto& e : set) (void)hipEventDestroy(e);
     if (h->ownStream) (void)hipStreamDestroy(h->ownStream);
   }
   delete h;
@@ -1162,7 +1178,6 @@ EXT_C int digHoloSynchronize(int handleIdx) {
     log_err(h, "synchronize", e);
     return DIGHOLO_ERROR_ERROR;
   }
-  collect_stage_times(*h);
   return DIGHOLO_ERROR_SUCCESS;
 }
 
@@ -1202,6 +1217,7 @@ EXT_C int digHoloConfigSetStageTiming(int handleIdx, int enabled) {
 EXT_C int digHoloGetStageTimes(int handleIdx, float* ms) {
   H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
   if (!ms) return DIGHOLO_ERROR_NULLPOINTER;
+  collect_stage_times(*h);
   std::memcpy(ms, h->stageMs, sizeof(h->stageMs));
   return DIGHOLO_ERROR_SUCCESS;
 }

@@ -57,7 +57,8 @@ hipError_t launch_col_fft_window(const Geometry& g, const FFTPlan& pl, const flo
 hipError_t launch_ifft_rows(const Geometry& g, const FFTPlan& pl, float2* win, hipStream_t s);
 hipError_t launch_ifft_cols_tilt(const Geometry& g, const FFTPlan& pl, const float2* win,
                                  const float2* ref, float2* fields, hipStream_t s);
-hipError_t launch_overlap_fp32(const Geometry& g, const float2* fields, const float* hgx,
-                               const float* hgy, const int2* modes, float2* coefs, hipStream_t s);
+// Separable HG overlap on MFMA: split = 0 plain FP32, 1 split-FP16 (hi+lo).
+hipError_t launch_overlap_mfma(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
+                               float2* coefs, int split, hipStream_t s);
 
 }  // namespace dh

@@ -11,7 +11,7 @@
 //   win --k_ifft_cols_tilt--> fields[B][P][h][w]           inverse FFT along y,
 //                                                          times reference wave,
 //                                                          V = -conj(E ref)
-//   fields --k_overlap_fp32--> coefs[B][P][M]              separable HG overlap
+//   fields --k_overlap_mfma (dh_overlap.hip)--> coefs[B][P][M]   separable HG overlap
 //
 // All transforms are batched mixed-radix Stockham FFTs staged in LDS
 // (radix 8/4/2/3/5 butterflies, any other prime by a direct DFT).
@@ -318,75 +318,6 @@ __global__ __launch_bounds__(kThreads) void k_ifft_cols_tilt(Geometry g, FFTPlan
 }
 
 // ---------------------------------------------------------------------------
-// 5. separable HG overlap, plain FP32:
-//    c[m,n] = sum_y HGy[n][y] T[m][y],  T[m][y] = sum_x HGx[m][x] V[y][x]
-//    grid (P, B); each thread owns modes q = tid + k*256.
-// ---------------------------------------------------------------------------
-constexpr int kOverlapRows = 16;
-
-template <int MPT>
-__global__ __launch_bounds__(kThreads) void k_overlap_fp32(Geometry g, const float2* __restrict__ fields,
-                                                           const float* __restrict__ hgx,
-                                                           const float* __restrict__ hgy,
-                                                           const int2* __restrict__ modes,
-                                                           float2* __restrict__ coefs) {
-  extern __shared__ float2 smem[];
-  const int p = blockIdx.x, b = blockIdx.y;
-  const int G = g.G, w = g.w, h = g.h, M = g.M;
-  float* hx = reinterpret_cast<float*>(smem);
-  float* hy = hx + G * w;
-  float2* Vs = reinterpret_cast<float2*>(hy + ((G * h + 1) & ~1));
-  float2* Ts = Vs + kOverlapRows * w;
-  for (int e = threadIdx.x; e < G * w; e += blockDim.x) hx[e] = hgx[(size_t)p * G * w + e];
-  for (int e = threadIdx.x; e < G * h; e += blockDim.x) hy[e] = hgy[(size_t)p * G * h + e];
-  int2 mn[MPT];
-  float2 acc[MPT];
-#pragma unroll
-  for (int k = 0; k < MPT; ++k) {
-    const int q = threadIdx.x + k * kThreads;
-    mn[k] = q < M ? modes[q] : make_int2(0, 0);
-    acc[k] = make_float2(0.f, 0.f);
-  }
-  const float2* V = fields + (size_t)(b * g.P + p) * h * w;
-  for (int y0 = 0; y0 < h; y0 += kOverlapRows) {
-    const int rows = min(kOverlapRows, h - y0);
-    __syncthreads();
-    for (int e = threadIdx.x; e < rows * w; e += blockDim.x) Vs[e] = V[(size_t)y0 * w + e];
-    __syncthreads();
-    for (int e = threadIdx.x; e < G * rows; e += blockDim.x) {
-      const int m = e / rows, yy = e - m * rows;
-      const float* hm = hx + m * w;
-      const float2* vr = Vs + yy * w;
-      float2 t = make_float2(0.f, 0.f);
-      for (int x = 0; x < w; ++x) {
-        t.x += hm[x] * vr[x].x;
-        t.y += hm[x] * vr[x].y;
-      }
-      Ts[m * kOverlapRows + yy] = t;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < MPT; ++k) {
-      const int q = threadIdx.x + k * kThreads;
-      if (q < M) {
-        const float* hn = hy + mn[k].y * h + y0;
-        const float2* tm = Ts + mn[k].x * kOverlapRows;
-        for (int yy = 0; yy < rows; ++yy) {
-          acc[k].x += hn[yy] * tm[yy].x;
-          acc[k].y += hn[yy] * tm[yy].y;
-        }
-      }
-    }
-  }
-  float2* out = coefs + (size_t)(b * g.P + p) * M;
-#pragma unroll
-  for (int k = 0; k < MPT; ++k) {
-    const int q = threadIdx.x + k * kThreads;
-    if (q < M) out[q] = acc[k];
-  }
-}
-
-// ---------------------------------------------------------------------------
 // FP64 tables: Fourier-window factor, reference wave, HG basis
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double xcsc(double t) { return t == 0.0 ? 1.0 : t / sin(t); }
@@ -536,31 +467,6 @@ hipError_t launch_ifft_cols_tilt(const Geometry& g, const FFTPlan& pl, const flo
   dim3 grid((g.w + CB - 1) / CB, g.P, g.B);
   k_ifft_cols_tilt<<<grid, kThreads, lds, s>>>(g, pl, win, ref, fields, CB);
   return hipGetLastError();
-}
-
-template <int MPT>
-static hipError_t overlap_mpt(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
-                              const int2* modes, float2* coefs, size_t lds, hipStream_t s) {
-  hipError_t e = set_lds(k_overlap_fp32<MPT>, lds);
-  if (e != hipSuccess) return e;
-  dim3 grid(g.P, g.B);
-  k_overlap_fp32<MPT><<<grid, kThreads, lds, s>>>(g, fields, hgx, hgy, modes, coefs);
-  return hipGetLastError();
-}
-
-hipError_t launch_overlap_fp32(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
-                               const int2* modes, float2* coefs, hipStream_t s) {
-  const size_t lds = sizeof(float) * ((size_t)g.G * g.w + (((size_t)g.G * g.h + 1) & ~(size_t)1)) +
-                     sizeof(float2) * ((size_t)kOverlapRows * g.w + (size_t)g.G * kOverlapRows);
-  if (lds > 160 * 1024) return hipErrorInvalidValue;
-  const int mpt = (g.M + kThreads - 1) / kThreads;
-  if (mpt <= 1) return overlap_mpt<1>(g, fields, hgx, hgy, modes, coefs, lds, s);
-  if (mpt <= 2) return overlap_mpt<2>(g, fields, hgx, hgy, modes, coefs, lds, s);
-  if (mpt <= 4) return overlap_mpt<4>(g, fields, hgx, hgy, modes, coefs, lds, s);
-  if (mpt <= 8) return overlap_mpt<8>(g, fields, hgx, hgy, modes, coefs, lds, s);
-  if (mpt <= 16) return overlap_mpt<16>(g, fields, hgx, hgy, modes, coefs, lds, s);
-  if (mpt <= 32) return overlap_mpt<32>(g, fields, hgx, hgy, modes, coefs, lds, s);
-  return hipErrorInvalidValue;
 }
 
 }  // namespace dh

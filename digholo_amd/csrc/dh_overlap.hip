@@ -1,0 +1,310 @@
+// Separable Hermite-Gaussian overlap on MFMA (gfx950).
+//
+//   stage 1:  T[(y,c)][m] = sum_x V_c[y][x] HGx[m][x]        (c = re/im)
+//   stage 2:  C[(m,c)][n] += sum_y T[(y,c)][m] HGy[n][y]      -> coef(m,n), m+n < G
+//
+// One workgroup (4 waves) owns one (frame, pol) field and walks it in chunks
+// of 16 field rows.  Per chunk, stage 1 is a (32 x Gp x w) product whose
+// output is re-laid out in LDS as the A operand of stage 2, a (2Gp x Gp x 16)
+// product accumulated in registers across chunks.  Tiles of stage 2 that hold
+// no mode with m+n < G are skipped.
+//
+// Two arithmetic forms:
+//  * FP32:   v_mfma_f32_16x16x4_f32 on float operands;
+//  * SPLIT:  each operand x = hi + lo with hi = fp16(x), lo = fp16(x - hi);
+//            hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_f16, FP32 accumulate.
+//            The field is scaled by a power of two per (frame, pol) so that its
+//            largest magnitude lies in (128, 256]; every partial sum then stays
+//            far inside the fp16 range, and the scale is removed at the end.
+//
+// MFMA operand layout used (16x16 output tile, 64 lanes, q = lane/16):
+//   A: lane holds row lane%16, k-slots kk(q, j);  B: column lane%16, same slots;
+//   D: lane holds column lane%16, rows 4q + i (i = 0..3).
+// A and B use the same k-slot mapping, so the contraction is independent of how
+// the hardware orders k inside an instruction.
+#include <algorithm>
+
+#include "dh_internal.h"
+
+namespace dh {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int kOvThreads = 256;
+constexpr int kChunk = 16;          // field rows per chunk (K of stage 2)
+
+template <bool SPLIT>
+struct OvT {
+  using E = typename std::conditional<SPLIT, _Float16, float>::type;
+};
+
+__device__ __forceinline__ void split16(float x, _Float16& hi, _Float16& lo) {
+  hi = (_Float16)x;
+  lo = (_Float16)(x - (float)hi);
+}
+
+// row pitch (elements) of an LDS matrix with K columns: pad by 8 elements to
+// spread 16 consecutive rows over the banks
+__host__ __device__ constexpr int pitch_of(int K) { return K + 8; }
+
+template <bool SPLIT, int TPW>
+__global__ __launch_bounds__(kOvThreads) void k_overlap_mfma(Geometry g, const float2* __restrict__ fields,
+                                                             const float* __restrict__ hgx,
+                                                             const float* __restrict__ hgy,
+                                                             float2* __restrict__ coefs) {
+  using E = typename OvT<SPLIT>::E;
+  extern __shared__ __align__(16) unsigned char ov_smem[];
+  const int p = blockIdx.x, b = blockIdx.y;
+  const int G = g.G, w = g.w, h = g.h, M = g.M;
+  const int Gp = (G + 15) & ~15;
+  const int pw = pitch_of(w), ph = pitch_of(h), pk = pitch_of(kChunk);
+  const int nL = SPLIT ? 2 : 1;   // hi (+ lo) planes
+  E* Bx = reinterpret_cast<E*>(ov_smem);          // [nL][Gp][pw]   HGx[m][x]
+  E* By = Bx + nL * Gp * pw;                      // [nL][Gp][ph]   HGy[n][y]
+  E* A1 = By + nL * Gp * ph;                      // [nL][32][pw]   V chunk, rows (yy, c)
+  E* A2 = A1 + nL * 32 * pw;                      // [nL][2Gp][pk]  T chunk, rows (m, c)
+  __shared__ float s_red[kOvThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, q = lane >> 4;
+
+  const float2* V = fields + (size_t)(b * g.P + p) * h * w;
+
+  // --- per-field power-of-two scale (split form only) ---
+  float scale = 1.f;
+  if (SPLIT) {
+    float mx = 0.f;
+    for (int e = tid; e < h * w; e += kOvThreads) {
+      const float2 v = V[e];
+      mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)));
+    }
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    if (lane == 0) s_red[wave] = mx;
+    __syncthreads();
+    mx = fmaxf(fmaxf(s_red[0], s_red[1]), fmaxf(s_red[2], s_red[3]));
+    if (mx > 0.f) {
+      int ex;
+      frexpf(mx, &ex);                 // mx in [2^(ex-1), 2^ex)
+      scale = ldexpf(1.f, 8 - ex);     // max * scale in [128, 256)
+    }
+  }
+
+  // --- basis into LDS (rows m >= G are zero) ---
+  for (int e = tid; e < Gp * w; e += kOvThreads) {
+    const int m = e / w, x = e - m * w;
+    const float v = m < G ? hgx[((size_t)p * G + m) * w + x] : 0.f;
+    if (SPLIT) {
+      _Float16 hi, lo;
+      split16(v, hi, lo);
+      Bx[m * pw + x] = hi;
+      Bx[Gp * pw + m * pw + x] = lo;
+    } else {
+      Bx[m * pw + x] = v;
+    }
+  }
+  for (int e = tid; e < Gp * h; e += kOvThreads) {
+    const int n = e / h, y = e - n * h;
+    const float v = n < G ? hgy[((size_t)p * G + n) * h + y] : 0.f;
+    if (SPLIT) {
+      _Float16 hi, lo;
+      split16(v, hi, lo);
+      By[n * ph + y] = hi;
+      By[Gp * ph + n * ph + y] = lo;
+    } else {
+      By[n * ph + y] = v;
+    }
+  }
+
+  // --- stage-2 tiles owned by this wave ---
+  const int rt2 = (2 * Gp) / 16, ct2 = Gp / 16;
+  int myTile[TPW];
+  int nMine = 0;
+  {
+    int cnt = 0;
+    for (int t = 0; t < rt2 * ct2; ++t) {
+      const int mt = t / ct2, nt = t - mt * ct2;
+      if (8 * mt + 16 * nt >= G) continue;          // every (m, n) in the tile has m + n >= G
+      if (cnt % 4 == wave && nMine < TPW) myTile[nMine++] = t;
+      ++cnt;
+    }
+  }
+  floatx4 acc[TPW];
+#pragma unroll
+  for (int i = 0; i < TPW; ++i) acc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  const int rt1 = 2, ct1 = Gp / 16;
+  for (int y0 = 0; y0 < h; y0 += kChunk) {
+    __syncthreads();   // previous chunk's A1/A2 readers are done (and basis/scale ready)
+    // V chunk -> A1 rows (yy, c)
+    for (int e = tid; e < kChunk * w; e += kOvThreads) {
+      const int yy = e / w, x = e - yy * w;
+      const float2 v = V[(size_t)(y0 + yy) * w + x];
+      if (SPLIT) {
+        _Float16 hi, lo;
+        split16(v.x * scale, hi, lo);
+        A1[(2 * yy) * pw + x] = hi;
+        A1[32 * pw + (2 * yy) * pw + x] = lo;
+        split16(v.y * scale, hi, lo);
+        A1[(2 * yy + 1) * pw + x] = hi;
+        A1[32 * pw + (2 * yy + 1) * pw + x] = lo;
+      } else {
+        A1[(2 * yy) * pw + x] = v.x;
+        A1[(2 * yy + 1) * pw + x] = v.y;
+      }
+    }
+    __syncthreads();
+    // stage 1: tiles (rt in 0..1) x (ct in 0..Gp/16)
+    for (int t = wave; t < rt1 * ct1; t += 4) {
+      const int rt = t / ct1, ct = t - rt * ct1;
+      floatx4 d = {0.f, 0.f, 0.f, 0.f};
+      const E* arow = A1 + (rt * 16 + lr) * pw;
+      const E* brow = Bx + (ct * 16 + lr) * pw;
+      if (SPLIT) {
+        for (int k0 = 0; k0 < w; k0 += 32) {
+          // k-slots: 8 consecutive k per lane group; a ragged last step (w % 32 == 16)
+          // zero-fills the upper lane groups
+          half8 ah, al, bh, bl;
+          const int kb = k0 + 8 * q;
+          if (kb < w) {   // 16-byte aligned: pitches, plane offsets and kb are multiples of 8
+            ah = *reinterpret_cast<const half8*>(arow + kb);
+            al = *reinterpret_cast<const half8*>(arow + 32 * pw + kb);
+            bh = *reinterpret_cast<const half8*>(brow + kb);
+            bl = *reinterpret_cast<const half8*>(brow + Gp * pw + kb);
+          } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) ah[j] = al[j] = bh[j] = bl[j] = (_Float16)0.f;
+          }
+          d = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, d, 0, 0, 0);
+          d = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, d, 0, 0, 0);
+          d = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, d, 0, 0, 0);
+        }
+      } else {
+        for (int k0 = 0; k0 < w; k0 += 4) {
+          d = __builtin_amdgcn_mfma_f32_16x16x4f32((float)arow[k0 + q], (float)brow[k0 + q], d, 0, 0, 0);
+        }
+      }
+      // D[r][m], r = rt*16 + 4q + i = (yy, c)  ->  A2[(m, c)][yy]
+      const int m = ct * 16 + lr;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = rt * 16 + 4 * q + i;
+        const int yy = r >> 1, c = r & 1;
+        const int row2 = 2 * m + c;
+        if (SPLIT) {
+          _Float16 hi, lo;
+          split16(d[i], hi, lo);
+          A2[row2 * pk + yy] = hi;
+          A2[2 * Gp * pk + row2 * pk + yy] = lo;
+        } else {
+          A2[row2 * pk + yy] = d[i];
+        }
+      }
+    }
+    __syncthreads();
+    // stage 2: acc[(m,c)][n] += sum_yy A2[(m,c)][yy] HGy[n][y0+yy]
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) {
+      if (i < nMine) {
+        const int t = myTile[i];
+        const int mt = t / ct2, nt = t - mt * ct2;
+        const E* arow = A2 + (mt * 16 + lr) * pk;
+        const E* brow = By + (nt * 16 + lr) * ph + y0;
+        if (SPLIT) {
+          // K = 16: lane groups 0,1 carry k 0..15, groups 2,3 are zero
+          half8 ah, al, bh, bl;
+          if (q < 2) {
+            ah = *reinterpret_cast<const half8*>(arow + 8 * q);
+            al = *reinterpret_cast<const half8*>(arow + 2 * Gp * pk + 8 * q);
+            bh = *reinterpret_cast<const half8*>(brow + 8 * q);
+            bl = *reinterpret_cast<const half8*>(brow + Gp * ph + 8 * q);
+          } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) ah[j] = al[j] = bh[j] = bl[j] = (_Float16)0.f;
+          }
+          acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc[i], 0, 0, 0);
+          acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc[i], 0, 0, 0);
+          acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc[i], 0, 0, 0);
+        } else {
+#pragma unroll
+          for (int k0 = 0; k0 < kChunk; k0 += 4)
+            acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)arow[k0 + q], (float)brow[k0 + q], acc[i], 0, 0, 0);
+        }
+      }
+    }
+  }
+
+  // --- write the triangle m + n < G ---
+  const float inv = 1.f / scale;
+  float2* out = coefs + (size_t)(b * g.P + p) * M;
+#pragma unroll
+  for (int i = 0; i < TPW; ++i) {
+    if (i < nMine) {
+      const int t = myTile[i];
+      const int mt = t / ct2, nt = t - mt * ct2;
+      const int n = nt * 16 + lr;
+#pragma unroll
+      for (int ii = 0; ii < 4; ii += 2) {
+        const int m = (mt * 16 + 4 * q + ii) >> 1;      // rows ii, ii+1 are (m, re), (m, im)
+        if (n < G && m < G && m + n < G) {
+          const int gsum = m + n;
+          out[gsum * (gsum + 1) / 2 + n] = make_float2(acc[i][ii] * inv, acc[i][ii + 1] * inv);
+        }
+      }
+    }
+  }
+}
+
+static size_t overlap_lds(const Geometry& g, bool split) {
+  const int Gp = (g.G + 15) & ~15;
+  const size_t e = split ? 2 : 4;
+  const int nL = split ? 2 : 1;
+  return e * nL * ((size_t)Gp * pitch_of(g.w) + (size_t)Gp * pitch_of(g.h) + 32 * (size_t)pitch_of(g.w) +
+                   2 * (size_t)Gp * pitch_of(kChunk));
+}
+
+static int tiles_per_wave(const Geometry& g) {
+  const int Gp = (g.G + 15) & ~15;
+  const int rt2 = (2 * Gp) / 16, ct2 = Gp / 16;
+  int cnt = 0;
+  for (int t = 0; t < rt2 * ct2; ++t) {
+    const int mt = t / ct2, nt = t - mt * ct2;
+    if (8 * mt + 16 * nt < g.G) ++cnt;
+  }
+  return (cnt + 3) / 4;
+}
+
+template <bool SPLIT, int TPW>
+static hipError_t ov_launch(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
+                            float2* coefs, hipStream_t s) {
+  const size_t lds = overlap_lds(g, SPLIT);
+  if (lds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_overlap_mfma<SPLIT, TPW>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  dim3 grid(g.P, g.B);
+  k_overlap_mfma<SPLIT, TPW><<<grid, kOvThreads, lds, s>>>(g, fields, hgx, hgy, coefs);
+  return hipGetLastError();
+}
+
+template <bool SPLIT>
+static hipError_t ov_dispatch(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
+                              float2* coefs, hipStream_t s) {
+  const int tpw = tiles_per_wave(g);
+  if (tpw <= 2) return ov_launch<SPLIT, 2>(g, fields, hgx, hgy, coefs, s);
+  if (tpw <= 4) return ov_launch<SPLIT, 4>(g, fields, hgx, hgy, coefs, s);
+  if (tpw <= 8) return ov_launch<SPLIT, 8>(g, fields, hgx, hgy, coefs, s);
+  if (tpw <= 16) return ov_launch<SPLIT, 16>(g, fields, hgx, hgy, coefs, s);
+  if (tpw <= 32) return ov_launch<SPLIT, 32>(g, fields, hgx, hgy, coefs, s);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_overlap_mfma(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
+                               float2* coefs, int split, hipStream_t s) {
+  if (g.w % 16 || g.h % kChunk || g.G <= 0) return hipErrorInvalidValue;
+  if (overlap_lds(g, split != 0) > 160 * 1024) return hipErrorInvalidValue;
+  return split ? ov_dispatch<true>(g, fields, hgx, hgy, coefs, s)
+               : ov_dispatch<false>(g, fields, hgx, hgy, coefs, s);
+}
+
+}  // namespace dh

@@ -210,9 +210,10 @@ EXT_C void* digHoloGetStream(int handleIdx);
  * the GPU: out[0..11] = W, H, w, h, cx0, cy0, cx1, cy1, SX0, SY0, SX1, SY1. */
 EXT_C int digHoloGetWindowGeometry(int handleIdx, int* out);
 
-/* Per-stage GPU time of the most recent processing call, measured with HIP
- * events on the handle's stream when enabled:  ms[0..4] = row FFT, column FFT
- * + Fourier window, IFFT + tilt, HG basis/tables, overlap. */
+/* Per-stage GPU time in ms, measured with HIP events on the handle's stream
+ * when enabled, averaged over every batch processed since the previous call:
+ * ms[0..4] = row FFT, column FFT + Fourier window, IFFT + tilt,
+ * HG basis/tables (0 when cached), overlap. */
 EXT_C int digHoloConfigSetStageTiming(int handleIdx, int enabled);
 EXT_C int digHoloGetStageTimes(int handleIdx, float* ms);
 

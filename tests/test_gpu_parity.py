@@ -180,6 +180,22 @@ def test_device_pointer_frames_and_device_outputs(lib):
     assert O.relative_l2(host.reshape(c_ref.shape), c_ref) <= TOL
 
 
+@pytest.mark.parametrize("name,B", [("small", 6), ("dual640", 3), ("radix5", 3), ("large1024", 1)])
+def test_split16_overlap_parity(lib, name, B):
+    # split-FP16 MFMA overlap (hi+lo operands, FP32 accumulate) meets the same bar
+    frames, cfg = make_case(name, B, seed=21)
+    h = lib.digHoloCreate()
+    api.configure(lib, h, **cfg)
+    assert lib.digHoloConfigSetOverlapPrecision(h, api.DIGHOLO_PRECISION_SPLIT16) == 0
+    coefs16, fields16, _, _, _ = run_gpu(lib, frames, cfg, h=h)
+    check(frames, cfg, coefs16, fields16)
+    assert lib.digHoloConfigSetOverlapPrecision(h, api.DIGHOLO_PRECISION_FP32) == 0
+    coefs32, _, _, _, _ = run_gpu(lib, frames, cfg, h=h)
+    _, c_ref, _ = O.process_batch(frames, oracle_for(cfg))
+    e16, e32 = O.relative_l2(coefs16, c_ref), O.relative_l2(coefs32, c_ref)
+    print(f"{name}: coef rel-L2 vs FP64  split16 {e16:.2e}  fp32 {e32:.2e}", flush=True)
+
+
 def test_large_window_high_order(lib):
     frames, cfg = make_case("large1024", 1, seed=12)
     coefs, fields, _, _, _ = run_gpu(lib, frames, cfg)
