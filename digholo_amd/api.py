@@ -120,6 +120,8 @@ SIGNATURES: Dict[str, Tuple[object, List[object]]] = {
     "digHoloSetStream": (c_int, [c_int, c_void_p]),
     "digHoloGetStream": (c_void_p, [c_int]),
     "digHoloGetWindowGeometry": (c_int, [c_int, P_int]),
+    "digHoloConfigSetFusedWindow": (c_int, [c_int, c_int]),
+    "digHoloGetWindowPath": (c_int, [c_int]),
     "digHoloConfigSetStageTiming": (c_int, [c_int, c_int]),
     "digHoloGetStageTimes": (c_int, [c_int, P_float]),
 }

@@ -140,6 +140,9 @@ struct Handle {
 
   // Stage timing: one set of 6 events per batch enqueued since the last
   // collection, so back-to-back batches need no host synchronisation.
+  bool allowFused = true;   // false forces the staged kernels
+  bool lastFused = false;   // which window path the last batch took
+
   bool stageTiming = false;
   std::vector<std::array<hipEvent_t, 6>> evPool;
   size_t evUsed = 0;
@@ -365,6 +368,12 @@ bool factorise(int n, dh::FFTPlan& pl) {
       m /= r;
     }
   }
+  int Ns = 1;
+  for (int s = 0; s < pl.nstages; ++s) {
+    pl.magNbf[s] = dh::magic_div((unsigned)(n / pl.radix[s]));
+    pl.magNs[s] = dh::magic_div((unsigned)Ns);
+    Ns *= pl.radix[s];
+  }
   return true;
 }
 
@@ -384,6 +393,34 @@ const dh::FFTPlan* fft_plan(Handle& h, int n) {
   const dh::FFTPlan* r = &e->plan;
   h.ffts[n] = std::move(e);
   return r;
+}
+
+// Picks the number of row residues R for the fused window kernel: the
+// smallest divisor of H whose H/R-point column transforms fit its budget.
+bool plan_fused(Handle& h, const dh::Geometry& g, dh::FusedArgs& fa) {
+  const dh::FFTPlan* row = fft_plan(h, g.W / 2);
+  const dh::FFTPlan* ix = fft_plan(h, g.w);
+  const dh::FFTPlan* iy = fft_plan(h, g.h);
+  if (!row || !ix || !iy) return false;
+  for (int R = 1; R <= g.H; ++R) {
+    if (g.H % R) continue;
+    const int Hc = g.H / R;
+    if (Hc < 8) break;
+    const dh::FFTPlan* col = fft_plan(h, Hc);
+    if (!col) return false;
+    fa = dh::FusedArgs{};
+    fa.g = g;
+    fa.row = *row;
+    fa.col = *col;
+    fa.ix = *ix;
+    fa.iy = *iy;
+    fa.R = R;
+    fa.Hc = Hc;
+    fa.RB = std::min(Hc, std::max(1, 4096 / (g.W / 2)));
+    fa.magW = dh::magic_div((unsigned)g.w);
+    if (dh::fused_supported(fa)) return true;
+  }
+  return false;
 }
 
 bool is_device_pointer(const void* ptr) {
@@ -469,8 +506,6 @@ int enqueue_batch(Handle& h) {
            "frame upload");
     dframes = h.staging.p;
   }
-  DH_TRY(h.rowspec.ensure(B * P * g.H * g.w), "alloc");
-  DH_TRY(h.win.ensure(B * P * g.h * g.w), "alloc");
   DH_TRY(h.fields.ensure(B * P * g.h * g.w), "alloc");
   DH_TRY(h.coefs.ensure(std::max<size_t>(B * P * g.M, 1)), "alloc");
 
@@ -518,13 +553,30 @@ int enqueue_batch(Handle& h) {
     h.tablesValid = true;
   }
   if (timing) DH_TRY(hipEventRecord(ev[1], h.stream), "event");
-  DH_TRY(dh::launch_row_fft(g, *pRow, h.twW.p, dframes, h.frameU16, h.rowspec.p, h.stream), "row FFT");
-  if (timing) DH_TRY(hipEventRecord(ev[2], h.stream), "event");
-  DH_TRY(dh::launch_col_fft_window(g, *pCol, h.rowspec.p, h.factor.p, h.win.p, h.stream), "column FFT");
-  if (timing) DH_TRY(hipEventRecord(ev[3], h.stream), "event");
-  DH_TRY(dh::launch_ifft_rows(g, *pIx, h.win.p, h.stream), "IFFT rows");
-  DH_TRY(dh::launch_ifft_cols_tilt(g, *pIy, h.win.p, h.ref.p, h.fields.p, h.stream), "IFFT columns");
-  if (timing) DH_TRY(hipEventRecord(ev[4], h.stream), "event");
+  dh::FusedArgs fa;
+  if (h.allowFused && plan_fused(h, g, fa)) {
+    // one kernel from camera window to field
+    DH_TRY(dh::launch_fused_window(fa, h.twW.p, pCol->tw, dframes, h.frameU16, h.factor.p, h.ref.p, h.fields.p,
+                                   h.stream), "fused window");
+    h.lastFused = true;
+    if (timing) {
+      DH_TRY(hipEventRecord(ev[2], h.stream), "event");
+      DH_TRY(hipEventRecord(ev[3], h.stream), "event");
+      DH_TRY(hipEventRecord(ev[4], h.stream), "event");
+    }
+  } else {
+    // staged kernels through HBM intermediates (any window size)
+    DH_TRY(h.rowspec.ensure(B * P * g.H * g.w), "alloc");
+    DH_TRY(h.win.ensure(B * P * g.h * g.w), "alloc");
+    DH_TRY(dh::launch_row_fft(g, *pRow, h.twW.p, dframes, h.frameU16, h.rowspec.p, h.stream), "row FFT");
+    if (timing) DH_TRY(hipEventRecord(ev[2], h.stream), "event");
+    DH_TRY(dh::launch_col_fft_window(g, *pCol, h.rowspec.p, h.factor.p, h.win.p, h.stream), "column FFT");
+    if (timing) DH_TRY(hipEventRecord(ev[3], h.stream), "event");
+    DH_TRY(dh::launch_ifft_rows(g, *pIx, h.win.p, h.stream), "IFFT rows");
+    DH_TRY(dh::launch_ifft_cols_tilt(g, *pIy, h.win.p, h.ref.p, h.fields.p, h.stream), "IFFT columns");
+    if (timing) DH_TRY(hipEventRecord(ev[4], h.stream), "event");
+    h.lastFused = false;
+  }
   if (g.G > 0) {
     DH_TRY(dh::launch_overlap_mfma(g, h.fields.p, h.hgx.p, h.hgy.p, h.coefs.p,
                                    c.precision == DIGHOLO_PRECISION_SPLIT16, h.stream), "overlap");
@@ -1207,6 +1259,16 @@ EXT_C int digHoloGetWindowGeometry(int handleIdx, int* out) {
   const int vals[12] = {g.W, g.H, g.w, g.h, g.cx[0], g.cy[0], g.cx[1], g.cy[1], d.SX[0], d.SY[0], d.SX[1], d.SY[1]};
   std::memcpy(out, vals, sizeof(vals));
   return DIGHOLO_ERROR_SUCCESS;
+}
+
+EXT_C int digHoloConfigSetFusedWindow(int handleIdx, int enabled) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  h->allowFused = enabled != 0;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C int digHoloGetWindowPath(int handleIdx) {
+  H_OR(-1);
+  return h->haveBatch ? (h->lastFused ? 1 : 0) : -1;
 }
 
 EXT_C int digHoloConfigSetStageTiming(int handleIdx, int enabled) {

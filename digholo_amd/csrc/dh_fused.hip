@@ -1,0 +1,275 @@
+// Fused window path: one workgroup per (frame, pol) turns the camera window
+// into the reconstructed field without any intermediate leaving LDS.
+//
+//   1. rows of the window (uint16/float) -> packed r2c FFT of W/2 points,
+//      keeping only the w columns [c0, c0+w) the Fourier window reads;
+//   2. the column transform is split by decimation in time over R row
+//      residues:  F[k][.] = sum_r W_H^(k r) G_r[k mod Hc][.],  G_r the
+//      Hc = H/R point FFT of rows y = R t + r.  Each thread accumulates its
+//      own window pixels (k = the h window rows) in registers;
+//   3. window factor (mask, fill-factor envelope, centring ramp, norm);
+//   4. 2-D inverse FFT (w along x, h along y) in place;
+//   5. V = -conj(E * ref) -> fields.
+//
+// Every transform is an in-place Stockham FFT: in each stage a thread reads
+// all of its butterflies into registers, the block synchronises, then writes.
+// Supported radices 2/3/4/5/8; shapes outside the LDS/register budget use the
+// staged kernels of dh_kernels.hip.
+#include <algorithm>
+
+#include "dh_fft.h"
+#include "dh_internal.h"
+
+namespace dh {
+
+constexpr int kFThreads = 512;
+constexpr int kMaxV = 16;                 // complex values per thread in one stage
+constexpr size_t kFusedLdsMax = 80 * 1024;  // two workgroups per CU
+
+__device__ __forceinline__ int fdiv(int q, unsigned mag) {
+  return mag ? (int)__umulhi((unsigned)q, mag) : q;
+}
+
+// In-place radix-R Stockham stage over nb transforms of length n; element
+// (t, k) lives at buf[t*ts + k*es].  tmajor: consecutive threads take
+// consecutive transforms (use when ts == 1), else consecutive butterflies.
+template <int R, bool INV>
+__device__ __forceinline__ void stage_ip(float2* buf, int n, int nb, int ts, int es, int Ns, unsigned magNbf,
+                                         unsigned magNs, unsigned magNb, bool tmajor, const float2* tw) {
+  constexpr int BMAX = kMaxV / R;
+  const int nbf = n / R;
+  const int total = nb * nbf;
+  const int step = n / (Ns * R);
+  float2 v[BMAX][R];
+  int wb[BMAX];
+#pragma unroll
+  for (int bb = 0; bb < BMAX; ++bb) {
+    const int q = threadIdx.x + bb * kFThreads;
+    wb[bb] = -1;
+    if (q < total) {
+      int t, j;
+      if (tmajor) {
+        j = fdiv(q, magNb);
+        t = q - j * nb;
+      } else {
+        t = fdiv(q, magNbf);
+        j = q - t * nbf;
+      }
+      const int jm = j - fdiv(j, magNs) * Ns;
+      const int tb = t * ts;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[bb][r] = buf[tb + (j + r * nbf) * es];
+      if (jm) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[bb][r] = cmul(v[bb][r], twiddle<INV>(tw, jm * r * step));
+      }
+      dftR<R, INV>(v[bb]);
+      wb[bb] = tb + ((j - jm) * R + jm) * es;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int bb = 0; bb < BMAX; ++bb) {
+    if (wb[bb] >= 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[wb[bb] + r * Ns * es] = v[bb][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <bool INV>
+__device__ void lds_fft_ip(float2* buf, int nb, int ts, int es, unsigned magNb, bool tmajor, const FFTPlan& pl,
+                           const float2* tw) {
+  int Ns = 1;
+  for (int s = 0; s < pl.nstages; ++s) {
+    const int R = pl.radix[s];
+    switch (R) {
+      case 8: stage_ip<8, INV>(buf, pl.n, nb, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
+      case 4: stage_ip<4, INV>(buf, pl.n, nb, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
+      case 2: stage_ip<2, INV>(buf, pl.n, nb, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
+      case 3: stage_ip<3, INV>(buf, pl.n, nb, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
+      default: stage_ip<5, INV>(buf, pl.n, nb, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
+    }
+    Ns *= R;
+  }
+}
+
+struct FusedLayout {
+  int regionX;   // float2 elements shared by (row buffer + band) and the field
+  int tables;    // float2 elements of twiddle tables
+  size_t bytes;
+};
+
+__host__ __device__ inline FusedLayout fused_layout(const FusedArgs& a) {
+  FusedLayout L;
+  const int fwd = a.RB * a.row.n + a.Hc * a.g.w;
+  const int inv = a.g.h * a.g.w;
+  L.regionX = fwd > inv ? fwd : inv;
+  L.tables = a.row.n + a.Hc + a.g.w + a.g.H + a.g.w + a.g.h;
+  L.bytes = sizeof(float2) * (size_t)(L.regionX + L.tables) + sizeof(int) * (size_t)a.g.h;
+  return L;
+}
+
+struct FusedMagic {
+  unsigned n, w, Hc, H;
+};
+
+template <int FPT>
+__global__ __launch_bounds__(kFThreads) void k_fused_window(FusedArgs a, FusedMagic mg,
+                                                            const float2* __restrict__ twW,
+                                                            const float2* __restrict__ twH,
+                                                            const void* __restrict__ frames, int u16,
+                                                            const float2* __restrict__ factor,
+                                                            const float2* __restrict__ ref,
+                                                            float2* __restrict__ fields) {
+  extern __shared__ float2 fsm[];
+  const Geometry& g = a.g;
+  const int p = blockIdx.x, b = blockIdx.y;
+  const int n = a.row.n, Hc = a.Hc, R = a.R, RB = a.RB;
+  const int H = g.H, w = g.w, h = g.h, hw = h * w;
+  const int tid = threadIdx.x;
+  const FusedLayout L = fused_layout(a);
+  float2* rowbuf = fsm;                   // [RB][n]
+  float2* band = fsm + RB * n;            // [Hc][w]
+  float2* fbuf = fsm;                     // [h][w], after the forward pass
+  float2* twn = fsm + L.regionX;          // n
+  float2* twc = twn + n;                  // Hc
+  float2* twr = twc + Hc;                 // w: exp(-2 pi i (c0 + i) / W)
+  float2* twy = twr + w;                  // H: exp(-2 pi i k / H)
+  float2* twix = twy + H;                 // w
+  float2* twiy = twix + w;                // h
+  int* kyTab = reinterpret_cast<int*>(twiy + h);   // h: r2c row of window row j
+  const int c0 = g.c0[p];
+  for (int e = tid; e < n; e += kFThreads) twn[e] = a.row.tw[e];
+  for (int e = tid; e < Hc; e += kFThreads) twc[e] = a.col.tw[e];
+  for (int e = tid; e < w; e += kFThreads) {
+    twr[e] = twW[c0 + e];
+    twix[e] = a.ix.tw[e];
+  }
+  for (int e = tid; e < H; e += kFThreads) twy[e] = twH[e];
+  for (int e = tid; e < h; e += kFThreads) {
+    twiy[e] = a.iy.tw[e];
+    int row = (e + g.SY[p] + H / 2) % H;
+    kyTab[e] = row < 0 ? row + H : row;
+  }
+  __syncthreads();
+
+  float2 F[FPT];
+#pragma unroll
+  for (int s = 0; s < FPT; ++s) F[s] = make_float2(0.f, 0.f);
+
+  const size_t fbase = (size_t)b * g.frameH * g.frameW + (size_t)g.cy[p] * g.frameW + g.cx[p];
+  for (int r = 0; r < R; ++r) {
+    for (int t0 = 0; t0 < Hc; t0 += RB) {
+      const int rows = min(RB, Hc - t0);
+      for (int e = tid; e < rows * n; e += kFThreads) {
+        const int tt = fdiv(e, mg.n), k = e - tt * n;
+        const size_t off = fbase + (size_t)(R * (t0 + tt) + r) * g.frameW + 2 * k;
+        float2 z;
+        if (u16) {
+          const ushort2 v = *reinterpret_cast<const ushort2*>(static_cast<const unsigned short*>(frames) + off);
+          z = make_float2((float)v.x, (float)v.y);
+        } else {
+          z = *reinterpret_cast<const float2*>(static_cast<const float*>(frames) + off);
+        }
+        rowbuf[e] = z;
+      }
+      __syncthreads();
+      lds_fft_ip<false>(rowbuf, rows, n, 1, 0u, false, a.row, twn);
+      // unpack the packed real transform at the window columns
+      for (int e = tid; e < rows * w; e += kFThreads) {
+        const int tt = fdiv(e, mg.w), i = e - tt * w;
+        const int k = c0 + i;
+        const float2 zk = rowbuf[tt * n + k];
+        float2 zc = rowbuf[tt * n + (k == 0 ? 0 : n - k)];
+        zc.y = -zc.y;
+        const float2 ev = cscale(cadd(zk, zc), 0.5f);
+        const float2 od = mul_mi<false>(cscale(csub(zk, zc), 0.5f));
+        band[(t0 + tt) * w + i] = cadd(ev, cmul(twr[i], od));
+      }
+      __syncthreads();
+    }
+    // Hc-point FFT down each of the w band columns
+    lds_fft_ip<false>(band, w, 1, w, mg.w, true, a.col, twc);
+    // accumulate this residue into the window pixels each thread owns
+#pragma unroll
+    for (int s = 0; s < FPT; ++s) {
+      const int e = tid + s * kFThreads;
+      if (e < hw) {
+        const int j = fdiv(e, mg.w), i = e - j * w;
+        const int ky = kyTab[j];
+        const int kk = ky - fdiv(ky, mg.Hc) * Hc;
+        const int pr = ky * r;
+        const int idx = pr - fdiv(pr, mg.H) * H;
+        F[s] = cadd(F[s], cmul(twy[idx], band[kk * w + i]));
+      }
+    }
+    __syncthreads();
+  }
+  // window factor, then the field plane in LDS
+  const float2* fac = factor + (size_t)p * hw;
+#pragma unroll
+  for (int s = 0; s < FPT; ++s) {
+    const int e = tid + s * kFThreads;
+    if (e < hw) fbuf[e] = cmul(F[s], fac[e]);
+  }
+  __syncthreads();
+  lds_fft_ip<true>(fbuf, h, w, 1, 0u, false, a.ix, twix);       // along x
+  lds_fft_ip<true>(fbuf, w, 1, w, mg.w, true, a.iy, twiy);       // along y
+  const float2* rp = ref + (size_t)p * hw;
+  float2* out = fields + (size_t)(b * g.P + p) * hw;
+  for (int e = tid; e < hw; e += kFThreads) {
+    const float2 t = cmul(fbuf[e], rp[e]);
+    out[e] = make_float2(-t.x, t.y);
+  }
+}
+
+// every stage of the plan fits the per-thread register budget of stage_ip
+static bool ip_fits(const FFTPlan& pl, int nb) {
+  for (int s = 0; s < pl.nstages; ++s) {
+    const int R = pl.radix[s];
+    if (R != 2 && R != 3 && R != 4 && R != 5 && R != 8) return false;
+    const long total = (long)nb * (pl.n / R);
+    if (total > (long)(kMaxV / R) * kFThreads) return false;
+  }
+  return true;
+}
+
+static int fpt_of(const FusedArgs& a) { return (a.g.h * a.g.w + kFThreads - 1) / kFThreads; }
+
+bool fused_supported(const FusedArgs& a) {
+  if (a.R < 1 || a.Hc * a.R != a.g.H || a.RB < 1) return false;
+  if (fpt_of(a) > 16) return false;
+  if (!ip_fits(a.row, a.RB) || !ip_fits(a.col, a.g.w) || !ip_fits(a.ix, a.g.h) || !ip_fits(a.iy, a.g.w))
+    return false;
+  return fused_layout(a).bytes <= kFusedLdsMax;
+}
+
+template <int FPT>
+static hipError_t fused_launch(const FusedArgs& a, const FusedMagic& mg, const float2* twW, const float2* twH,
+                               const void* frames, int u16, const float2* factor, const float2* ref,
+                               float2* fields, hipStream_t s) {
+  const size_t lds = fused_layout(a).bytes;
+  if (lds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_fused_window<FPT>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  dim3 grid(a.g.P, a.g.B);
+  k_fused_window<FPT><<<grid, kFThreads, lds, s>>>(a, mg, twW, twH, frames, u16, factor, ref, fields);
+  return hipGetLastError();
+}
+
+hipError_t launch_fused_window(const FusedArgs& a, const float2* twW, const float2* twH, const void* frames,
+                               int frame_is_u16, const float2* factor, const float2* ref, float2* fields,
+                               hipStream_t s) {
+  if (!fused_supported(a)) return hipErrorNotSupported;
+  FusedMagic mg{magic_div(a.row.n), magic_div(a.g.w), magic_div(a.Hc), magic_div(a.g.H)};
+  const int fpt = fpt_of(a);
+  if (fpt <= 4) return fused_launch<4>(a, mg, twW, twH, frames, frame_is_u16, factor, ref, fields, s);
+  if (fpt <= 8) return fused_launch<8>(a, mg, twW, twH, frames, frame_is_u16, factor, ref, fields, s);
+  return fused_launch<16>(a, mg, twW, twH, frames, frame_is_u16, factor, ref, fields, s);
+}
+
+}  // namespace dh

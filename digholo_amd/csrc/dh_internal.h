@@ -14,8 +14,15 @@ struct FFTPlan {
   int n;                      // transform length (complex points)
   int nstages;
   int radix[kMaxStages];
+  // q / d == __umulhi(q, mag) for q < 2^20 (mag = ceil(2^32 / d); 0 when d == 1)
+  unsigned magNbf[kMaxStages];  // d = n / radix[s]
+  unsigned magNs[kMaxStages];   // d = radix[0] * ... * radix[s-1]
   const float2* tw;           // n twiddles exp(-2 pi i k / n), device memory
 };
+
+inline unsigned magic_div(unsigned d) {
+  return d <= 1 ? 0u : (unsigned)((0xFFFFFFFFull / d) + 1ull);
+}
 
 // Everything the kernels need about the windows of one batch.
 struct Geometry {
@@ -57,6 +64,21 @@ hipError_t launch_col_fft_window(const Geometry& g, const FFTPlan& pl, const flo
 hipError_t launch_ifft_rows(const Geometry& g, const FFTPlan& pl, float2* win, hipStream_t s);
 hipError_t launch_ifft_cols_tilt(const Geometry& g, const FFTPlan& pl, const float2* win,
                                  const float2* ref, float2* fields, hipStream_t s);
+// Fused window path: frame ingest -> r2c rows -> Fourier window (DIT over row
+// residues) -> window factor -> 2-D IFFT -> tilt -> fields.  Returns
+// hipErrorNotSupported when the shape does not fit its LDS/register budget
+// (the caller then uses the staged kernels above).
+struct FusedArgs {
+  Geometry g;
+  FFTPlan row, col, ix, iy;   // W/2, H/R, w, h
+  int R, Hc, RB;              // row residues, points per residue, rows per sub-batch
+  unsigned magW;              // magic for division by w
+};
+bool fused_supported(const FusedArgs& a);
+hipError_t launch_fused_window(const FusedArgs& a, const float2* twW, const float2* twH, const void* frames,
+                               int frame_is_u16, const float2* factor, const float2* ref, float2* fields,
+                               hipStream_t s);
+
 // Separable HG overlap on MFMA: split = 0 plain FP32, 1 split-FP16 (hi+lo).
 hipError_t launch_overlap_mfma(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
                                float2* coefs, int split, hipStream_t s);

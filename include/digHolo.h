@@ -210,9 +210,16 @@ EXT_C void* digHoloGetStream(int handleIdx);
  * the GPU: out[0..11] = W, H, w, h, cx0, cy0, cx1, cy1, SX0, SY0, SX1, SY1. */
 EXT_C int digHoloGetWindowGeometry(int handleIdx, int* out);
 
+/* Window path: enabled (default) runs the fused camera-window-to-field kernel
+ * whenever the shape fits it; 0 always runs the staged kernels.  GetWindowPath
+ * reports the path of the last batch: 1 fused, 0 staged, -1 none yet. */
+EXT_C int digHoloConfigSetFusedWindow(int handleIdx, int enabled);
+EXT_C int digHoloGetWindowPath(int handleIdx);
+
 /* Per-stage GPU time in ms, measured with HIP events on the handle's stream
  * when enabled, averaged over every batch processed since the previous call:
- * ms[0..4] = row FFT, column FFT + Fourier window, IFFT + tilt,
+ * ms[0..4] = row FFT (the whole fused window kernel on the fused path),
+ * column FFT + Fourier window, IFFT + tilt (both 0 on the fused path),
  * HG basis/tables (0 when cached), overlap. */
 EXT_C int digHoloConfigSetStageTiming(int handleIdx, int enabled);
 EXT_C int digHoloGetStageTimes(int handleIdx, float* ms);

@@ -104,11 +104,16 @@ def check(frames, cfg, coefs, fields, tol=TOL):
     return plan
 
 
+@pytest.mark.parametrize("fused", [1, 0])
 @pytest.mark.parametrize("name,B", [("small", 8), ("dual640", 3), ("dual640w192", 3), ("radix5", 4)])
-def test_parity_uint16(lib, name, B):
+def test_parity_uint16(lib, name, B, fused):
     frames, cfg = make_case(name, B, seed=11)
-    print(f"case {name} B={B} frames {frames.shape}", flush=True)
-    coefs, fields, x, y, h = run_gpu(lib, frames, cfg)
+    print(f"case {name} B={B} frames {frames.shape} fused={fused}", flush=True)
+    h = lib.digHoloCreate()
+    api.configure(lib, h, **cfg)
+    assert lib.digHoloConfigSetFusedWindow(h, fused) == 0
+    coefs, fields, x, y, h = run_gpu(lib, frames, cfg, h=h)
+    assert lib.digHoloGetWindowPath(h) == fused      # these shapes fit the fused kernel
     plan = check(frames, cfg, coefs, fields)
     # reconstructed axes: (i - w/2) * px W / w (digHolo.cpp:8184-8189)
     px = np.float32(20e-6)
@@ -198,5 +203,6 @@ def test_split16_overlap_parity(lib, name, B):
 
 def test_large_window_high_order(lib):
     frames, cfg = make_case("large1024", 1, seed=12)
-    coefs, fields, _, _, _ = run_gpu(lib, frames, cfg)
+    coefs, fields, _, _, h = run_gpu(lib, frames, cfg)
+    assert lib.digHoloGetWindowPath(h) == 0          # 160x160 field: staged kernels
     check(frames, cfg, coefs, fields)
