@@ -416,9 +416,7 @@ bool plan_fused(Handle& h, const dh::Geometry& g, dh::FusedArgs& fa) {
     fa.iy = *iy;
     fa.R = R;
     fa.Hc = Hc;
-    fa.RB = std::min(Hc, std::max(1, 4096 / (g.W / 2)));
-    fa.magW = dh::magic_div((unsigned)g.w);
-    if (dh::fused_supported(fa)) return true;
+    if (dh::fused_prepare(fa)) return true;
   }
   return false;
 }

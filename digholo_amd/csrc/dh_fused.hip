@@ -12,9 +12,11 @@
 //   5. V = -conj(E * ref) -> fields.
 //
 // Every transform is an in-place Stockham FFT: in each stage a thread reads
-// all of its butterflies into registers, the block synchronises, then writes.
-// Supported radices 2/3/4/5/8; shapes outside the LDS/register budget use the
-// staged kernels of dh_kernels.hip.
+// all of its butterflies into registers, the block synchronises, then writes;
+// batches of transforms larger than the register budget run in chunks.  The
+// next row batch is fetched from HBM into registers while the current one is
+// transformed.  Supported radices 2/3/4/5/8; other shapes use the staged
+// kernels of dh_kernels.hip.
 #include <algorithm>
 
 #include "dh_fft.h"
@@ -23,8 +25,10 @@
 namespace dh {
 
 constexpr int kFThreads = 512;
-constexpr int kMaxV = 16;                 // complex values per thread in one stage
+constexpr int kMaxV = 8;                    // complex values per thread in one stage
+constexpr int kPre = 8;                     // prefetched row elements per thread
 constexpr size_t kFusedLdsMax = 80 * 1024;  // two workgroups per CU
+constexpr int kMaxFPT = 16;                 // window pixels accumulated per thread
 
 __device__ __forceinline__ int fdiv(int q, unsigned mag) {
   return mag ? (int)__umulhi((unsigned)q, mag) : q;
@@ -36,7 +40,7 @@ __device__ __forceinline__ int fdiv(int q, unsigned mag) {
 template <int R, bool INV>
 __device__ __forceinline__ void stage_ip(float2* buf, int n, int nb, int ts, int es, int Ns, unsigned magNbf,
                                          unsigned magNs, unsigned magNb, bool tmajor, const float2* tw) {
-  constexpr int BMAX = kMaxV / R;
+  constexpr int BMAX = kMaxV / R > 0 ? kMaxV / R : 1;
   const int nbf = n / R;
   const int total = nb * nbf;
   const int step = n / (Ns * R);
@@ -78,20 +82,26 @@ __device__ __forceinline__ void stage_ip(float2* buf, int n, int nb, int ts, int
   __syncthreads();
 }
 
+// nb transforms, at most `chunk` per in-place pass.
 template <bool INV>
-__device__ void lds_fft_ip(float2* buf, int nb, int ts, int es, unsigned magNb, bool tmajor, const FFTPlan& pl,
-                           const float2* tw) {
-  int Ns = 1;
-  for (int s = 0; s < pl.nstages; ++s) {
-    const int R = pl.radix[s];
-    switch (R) {
-      case 8: stage_ip<8, INV>(buf, pl.n, nb, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
-      case 4: stage_ip<4, INV>(buf, pl.n, nb, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
-      case 2: stage_ip<2, INV>(buf, pl.n, nb, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
-      case 3: stage_ip<3, INV>(buf, pl.n, nb, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
-      default: stage_ip<5, INV>(buf, pl.n, nb, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
+__device__ void lds_fft_ip(float2* buf, int nb, int ts, int es, bool tmajor, const FFTPlan& pl, const float2* tw,
+                           int chunk) {
+  for (int t0 = 0; t0 < nb; t0 += chunk) {
+    const int cnt = min(chunk, nb - t0);
+    const unsigned magNb = tmajor ? magic_div((unsigned)cnt) : 0u;
+    float2* b = buf + t0 * ts;
+    int Ns = 1;
+    for (int s = 0; s < pl.nstages; ++s) {
+      const int R = pl.radix[s];
+      switch (R) {
+        case 8: stage_ip<8, INV>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
+        case 4: stage_ip<4, INV>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
+        case 2: stage_ip<2, INV>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
+        case 3: stage_ip<3, INV>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
+        default: stage_ip<5, INV>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
+      }
+      Ns *= R;
     }
-    Ns *= R;
   }
 }
 
@@ -115,14 +125,34 @@ struct FusedMagic {
   unsigned n, w, Hc, H;
 };
 
+// fetch row batch (r, t0) of the window into registers: element e = tid + s*NT
+// of the [rows][n] packed-complex batch
+__device__ __forceinline__ void fetch_rows(float2 (&pre)[kPre], const void* frames, int u16, size_t fbase,
+                                           int frameW, int R, int r, int t0, int rows, int n, unsigned magN) {
+#pragma unroll
+  for (int s = 0; s < kPre; ++s) {
+    const int e = threadIdx.x + s * kFThreads;
+    if (e < rows * n) {
+      const int tt = fdiv(e, magN), k = e - tt * n;
+      const size_t off = fbase + (size_t)(R * (t0 + tt) + r) * frameW + 2 * k;
+      if (u16) {
+        const ushort2 v = *reinterpret_cast<const ushort2*>(static_cast<const unsigned short*>(frames) + off);
+        pre[s] = make_float2((float)v.x, (float)v.y);
+      } else {
+        pre[s] = *reinterpret_cast<const float2*>(static_cast<const float*>(frames) + off);
+      }
+    }
+  }
+}
+
 template <int FPT>
-__global__ __launch_bounds__(kFThreads) void k_fused_window(FusedArgs a, FusedMagic mg,
-                                                            const float2* __restrict__ twW,
-                                                            const float2* __restrict__ twH,
-                                                            const void* __restrict__ frames, int u16,
-                                                            const float2* __restrict__ factor,
-                                                            const float2* __restrict__ ref,
-                                                            float2* __restrict__ fields) {
+__global__ __launch_bounds__(kFThreads, 4) void k_fused_window(FusedArgs a, FusedMagic mg,
+                                                               const float2* __restrict__ twW,
+                                                               const float2* __restrict__ twH,
+                                                               const void* __restrict__ frames, int u16,
+                                                               const float2* __restrict__ factor,
+                                                               const float2* __restrict__ ref,
+                                                               float2* __restrict__ fields) {
   extern __shared__ float2 fsm[];
   const Geometry& g = a.g;
   const int p = blockIdx.x, b = blockIdx.y;
@@ -141,6 +171,13 @@ __global__ __launch_bounds__(kFThreads) void k_fused_window(FusedArgs a, FusedMa
   float2* twiy = twix + w;                // h
   int* kyTab = reinterpret_cast<int*>(twiy + h);   // h: r2c row of window row j
   const int c0 = g.c0[p];
+  const size_t fbase = (size_t)b * g.frameH * g.frameW + (size_t)g.cy[p] * g.frameW + g.cx[p];
+
+  // first row batch in flight while the tables are staged
+  float2 pre[kPre];
+  const int nbat = (Hc + RB - 1) / RB;     // row batches per residue
+  fetch_rows(pre, frames, u16, fbase, g.frameW, R, 0, 0, min(RB, Hc), n, mg.n);
+
   for (int e = tid; e < n; e += kFThreads) twn[e] = a.row.tw[e];
   for (int e = tid; e < Hc; e += kFThreads) twc[e] = a.col.tw[e];
   for (int e = tid; e < w; e += kFThreads) {
@@ -153,30 +190,31 @@ __global__ __launch_bounds__(kFThreads) void k_fused_window(FusedArgs a, FusedMa
     int row = (e + g.SY[p] + H / 2) % H;
     kyTab[e] = row < 0 ? row + H : row;
   }
-  __syncthreads();
 
   float2 F[FPT];
 #pragma unroll
   for (int s = 0; s < FPT; ++s) F[s] = make_float2(0.f, 0.f);
 
-  const size_t fbase = (size_t)b * g.frameH * g.frameW + (size_t)g.cy[p] * g.frameW + g.cx[p];
   for (int r = 0; r < R; ++r) {
-    for (int t0 = 0; t0 < Hc; t0 += RB) {
+    for (int bi = 0; bi < nbat; ++bi) {
+      const int t0 = bi * RB;
       const int rows = min(RB, Hc - t0);
-      for (int e = tid; e < rows * n; e += kFThreads) {
-        const int tt = fdiv(e, mg.n), k = e - tt * n;
-        const size_t off = fbase + (size_t)(R * (t0 + tt) + r) * g.frameW + 2 * k;
-        float2 z;
-        if (u16) {
-          const ushort2 v = *reinterpret_cast<const ushort2*>(static_cast<const unsigned short*>(frames) + off);
-          z = make_float2((float)v.x, (float)v.y);
-        } else {
-          z = *reinterpret_cast<const float2*>(static_cast<const float*>(frames) + off);
+      // the batch fetched earlier goes to LDS; start fetching the next one
+#pragma unroll
+      for (int s = 0; s < kPre; ++s) {
+        const int e = tid + s * kFThreads;
+        if (e < rows * n) rowbuf[e] = pre[s];
+      }
+      {
+        int nr = r, nb = bi + 1;
+        if (nb == nbat) {
+          nb = 0;
+          ++nr;
         }
-        rowbuf[e] = z;
+        if (nr < R) fetch_rows(pre, frames, u16, fbase, g.frameW, R, nr, nb * RB, min(RB, Hc - nb * RB), n, mg.n);
       }
       __syncthreads();
-      lds_fft_ip<false>(rowbuf, rows, n, 1, 0u, false, a.row, twn);
+      lds_fft_ip<false>(rowbuf, rows, n, 1, false, a.row, twn, RB);
       // unpack the packed real transform at the window columns
       for (int e = tid; e < rows * w; e += kFThreads) {
         const int tt = fdiv(e, mg.w), i = e - tt * w;
@@ -191,7 +229,7 @@ __global__ __launch_bounds__(kFThreads) void k_fused_window(FusedArgs a, FusedMa
       __syncthreads();
     }
     // Hc-point FFT down each of the w band columns
-    lds_fft_ip<false>(band, w, 1, w, mg.w, true, a.col, twc);
+    lds_fft_ip<false>(band, w, 1, w, true, a.col, twc, a.chunkCol);
     // accumulate this residue into the window pixels each thread owns
 #pragma unroll
     for (int s = 0; s < FPT; ++s) {
@@ -215,8 +253,8 @@ __global__ __launch_bounds__(kFThreads) void k_fused_window(FusedArgs a, FusedMa
     if (e < hw) fbuf[e] = cmul(F[s], fac[e]);
   }
   __syncthreads();
-  lds_fft_ip<true>(fbuf, h, w, 1, 0u, false, a.ix, twix);       // along x
-  lds_fft_ip<true>(fbuf, w, 1, w, mg.w, true, a.iy, twiy);       // along y
+  lds_fft_ip<true>(fbuf, h, w, 1, false, a.ix, twix, a.chunkIx);   // along x
+  lds_fft_ip<true>(fbuf, w, 1, w, true, a.iy, twiy, a.chunkIy);    // along y
   const float2* rp = ref + (size_t)p * hw;
   float2* out = fields + (size_t)(b * g.P + p) * hw;
   for (int e = tid; e < hw; e += kFThreads) {
@@ -225,24 +263,30 @@ __global__ __launch_bounds__(kFThreads) void k_fused_window(FusedArgs a, FusedMa
   }
 }
 
-// every stage of the plan fits the per-thread register budget of stage_ip
-static bool ip_fits(const FFTPlan& pl, int nb) {
+// Most transforms of plan pl one in-place pass can hold (0: unsupported radix).
+static int ip_chunk(const FFTPlan& pl) {
+  int best = 1 << 30;
   for (int s = 0; s < pl.nstages; ++s) {
     const int R = pl.radix[s];
-    if (R != 2 && R != 3 && R != 4 && R != 5 && R != 8) return false;
-    const long total = (long)nb * (pl.n / R);
-    if (total > (long)(kMaxV / R) * kFThreads) return false;
+    if (R != 2 && R != 3 && R != 4 && R != 5 && R != 8) return 0;
+    const int bmax = kMaxV / R > 0 ? kMaxV / R : 1;
+    best = std::min(best, bmax * kFThreads * R / pl.n);
   }
-  return true;
+  return best;
 }
 
 static int fpt_of(const FusedArgs& a) { return (a.g.h * a.g.w + kFThreads - 1) / kFThreads; }
 
-bool fused_supported(const FusedArgs& a) {
-  if (a.R < 1 || a.Hc * a.R != a.g.H || a.RB < 1) return false;
-  if (fpt_of(a) > 16) return false;
-  if (!ip_fits(a.row, a.RB) || !ip_fits(a.col, a.g.w) || !ip_fits(a.ix, a.g.h) || !ip_fits(a.iy, a.g.w))
-    return false;
+bool fused_prepare(FusedArgs& a) {
+  if (a.R < 1 || a.Hc * a.R != a.g.H) return false;
+  if (fpt_of(a) > kMaxFPT) return false;
+  const int cr = ip_chunk(a.row), cc = ip_chunk(a.col), cx = ip_chunk(a.ix), cy = ip_chunk(a.iy);
+  if (cr < 1 || cc < 1 || cx < 1 || cy < 1) return false;
+  a.RB = std::min(std::min(a.Hc, cr), kPre * kFThreads / a.row.n);
+  if (a.RB < 1) return false;
+  a.chunkCol = cc;
+  a.chunkIx = cx;
+  a.chunkIy = cy;
   return fused_layout(a).bytes <= kFusedLdsMax;
 }
 
@@ -261,10 +305,11 @@ static hipError_t fused_launch(const FusedArgs& a, const FusedMagic& mg, const f
   return hipGetLastError();
 }
 
-hipError_t launch_fused_window(const FusedArgs& a, const float2* twW, const float2* twH, const void* frames,
+hipError_t launch_fused_window(const FusedArgs& a0, const float2* twW, const float2* twH, const void* frames,
                                int frame_is_u16, const float2* factor, const float2* ref, float2* fields,
                                hipStream_t s) {
-  if (!fused_supported(a)) return hipErrorNotSupported;
+  FusedArgs a = a0;
+  if (!fused_prepare(a)) return hipErrorNotSupported;
   FusedMagic mg{magic_div(a.row.n), magic_div(a.g.w), magic_div(a.Hc), magic_div(a.g.H)};
   const int fpt = fpt_of(a);
   if (fpt <= 4) return fused_launch<4>(a, mg, twW, twH, frames, frame_is_u16, factor, ref, fields, s);

@@ -20,7 +20,7 @@ struct FFTPlan {
   const float2* tw;           // n twiddles exp(-2 pi i k / n), device memory
 };
 
-inline unsigned magic_div(unsigned d) {
+__host__ __device__ inline unsigned magic_div(unsigned d) {
   return d <= 1 ? 0u : (unsigned)((0xFFFFFFFFull / d) + 1ull);
 }
 
@@ -71,10 +71,13 @@ hipError_t launch_ifft_cols_tilt(const Geometry& g, const FFTPlan& pl, const flo
 struct FusedArgs {
   Geometry g;
   FFTPlan row, col, ix, iy;   // W/2, H/R, w, h
-  int R, Hc, RB;              // row residues, points per residue, rows per sub-batch
-  unsigned magW;              // magic for division by w
+  int R, Hc;                  // row residues, points per residue (H = R * Hc)
+  // set by fused_prepare: rows per row batch, and the most transforms of each
+  // plan one in-place pass may hold
+  int RB, chunkCol, chunkIx, chunkIy;
 };
-bool fused_supported(const FusedArgs& a);
+// Fills the batch/chunk sizes; false if the shape does not fit the kernel.
+bool fused_prepare(FusedArgs& a);
 hipError_t launch_fused_window(const FusedArgs& a, const float2* twW, const float2* twH, const void* frames,
                                int frame_is_u16, const float2* factor, const float2* ref, float2* fields,
                                hipStream_t s);
