@@ -84,6 +84,9 @@ SIGNATURES: Dict[str, Tuple[object, List[object]]] = {
     "digHoloConfigGetAutoAlignMode": (c_int, [c_int]),
     "digHoloConfigSetAutoAlignGoalIdx": (c_int, [c_int, c_int]),
     "digHoloConfigGetAutoAlignGoalIdx": (c_int, [c_int]),
+    "digHoloAutoAlignGetMetric": (c_float, [c_int, c_int]),
+    "digHoloAutoAlignGetMetrics": (P_float, [c_int, c_int]),
+    "digHoloAutoAlignCalcMetrics": (c_int, [c_int]),
     "digHoloConfigSetBeamCentre": (c_int, [c_int, c_int, c_int, c_float]),
     "digHoloConfigGetBeamCentre": (c_float, [c_int, c_int, c_int]),
     "digHoloConfigSetTilt": (c_int, [c_int, c_int, c_int, c_float]),

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -20,6 +21,7 @@
 #include <vector>
 
 #include "../../include/digHolo.h"
+#include "dh_eigen.h"
 #include "dh_internal.h"
 
 namespace {
@@ -135,6 +137,12 @@ struct Handle {
   bool haveBatch = false;
 
   PinnedBuf<float2> hCoefs, hFields;
+
+  // power meter: device scratch and the last metrics [metric][lambda + 1] in dB
+  DevBuf<double> metricRows;
+  DevBuf<double2> gram;
+  std::vector<float> aaMetrics;   // DIGHOLO_METRIC_COUNT x (lambdaCount + 1)
+  int aaLambdaCount = 0;
   bool coefsHostValid = false, fieldsHostValid = false;
   std::vector<float> xAxis, yAxis;
 
@@ -634,6 +642,165 @@ int download_fields(Handle& h) {
   DH_TRY(hipStreamSynchronize(h.stream), "sync");
   h.fieldsHostValid = true;
   return DIGHOLO_ERROR_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// power meter (digHolo.cpp:10714-11129): linear metrics of the coefficient
+// matrix of the last batch.  pols: -1 both polarisations (the only form the
+// C API's AutoAlignCalcMetrics uses).
+// ---------------------------------------------------------------------------
+struct PowerMeterOut {
+  double v[DIGHOLO_METRIC_COUNT];
+};
+
+int power_meter(Handle& h, bool independent, bool doIL, bool doMDL, bool doXtalk, bool mulConj, PowerMeterOut& o) {
+  const dh::Geometry& g = h.cur.g;
+  const int B = g.B, P = g.P, Mpol = g.M;
+  const float FMAX = std::numeric_limits<float>::max();
+  if (Mpol == 0) return DIGHOLO_ERROR_INVALIDARGUMENT;      // needs the Fourier-plane power (not built)
+  if (B > 1) {
+    const bool indep = independent || P == 1;
+    const int S = (P == 2 && indep) ? 2 : 1;
+    const int M = (P == 2 && !indep) ? 2 * Mpol : Mpol;    // svdM
+    const int N = B;                                       // svdN
+    const int rowStride = P * Mpol;
+    DH_TRY(h.metricRows.ensure((size_t)S * N * 5), "alloc");
+    DH_TRY(dh::launch_metric_rows(h.coefs.p, N, M, Mpol, rowStride, S, h.metricRows.p, h.stream), "metric rows");
+    std::vector<double> rows((size_t)S * N * 5);
+    DH_TRY(hipMemcpyAsync(rows.data(), h.metricRows.p, rows.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          h.stream), "D2H");
+    DH_TRY(hipStreamSynchronize(h.stream), "sync");
+
+    // Gram matrices (FP64) when the spectrum or A^H A is needed
+    const bool rowsVec = N <= M;
+    const int dim = rowsVec ? N : M;
+    std::vector<std::vector<std::complex<double>>> grams;
+    if (doMDL || (doXtalk && mulConj)) {
+      DH_TRY(h.gram.ensure((size_t)dim * dim), "alloc");
+      for (int s = 0; s < S; ++s) {
+        DH_TRY(dh::launch_gram_f64(h.coefs.p, N, M, rowStride, s, rowsVec ? 1 : 0, h.gram.p, h.stream), "gram");
+        std::vector<std::complex<double>> G((size_t)dim * dim);
+        DH_TRY(hipMemcpyAsync(G.data(), h.gram.p, G.size() * sizeof(double2), hipMemcpyDeviceToHost, h.stream),
+               "D2H");
+        DH_TRY(hipStreamSynchronize(h.stream), "sync");
+        grams.push_back(std::move(G));
+      }
+    }
+
+    double totalSignal = 0, totalNoise = 0, totalSNR = 0, sigMG = 0, noiMG = 0;
+    double minSignal = FMAX, maxSignal = -FMAX, minSNR = FMAX, maxSNR = -FMAX;
+    double signalCount = 1;
+    if (doXtalk) {
+      for (int s = 0; s < S; ++s) {
+        if (mulConj && rowsVec) {
+          // crosstalk of A^H A (N x N), digHolo.cpp:10952-10967
+          const std::vector<std::complex<double>>& G = grams[s];
+          signalCount = (double)N * N * S;
+          for (int i = 0; i < N && i < Mpol; ++i) {
+            double sig = 0, noi = 0;
+            int gi = 0;          // HG group of mode i
+            while ((gi + 1) * (gi + 2) / 2 <= i) ++gi;
+            for (int j = 0; j < N; ++j) {
+              const double pw = std::norm(G[(size_t)i * N + j]);
+              if (i == j || i == j - Mpol) sig += pw;
+              else noi += pw;
+              int gj = 0;
+              const int jm = j % Mpol;
+              while ((gj + 1) * (gj + 2) / 2 <= jm) ++gj;
+              if (gi == gj) sigMG += pw / (gi + 1);
+              else noiMG += pw / (gi + 1);
+            }
+            const double snr = sig / noi;
+            totalSignal += sig;
+            totalNoise += noi;
+            maxSignal = std::max(maxSignal, sig);
+            minSignal = std::min(minSignal, sig);
+            maxSNR = std::max(maxSNR, snr);
+            minSNR = std::min(minSNR, snr);
+          }
+        } else {
+          signalCount = (double)M * N * S;
+          for (int i = 0; i < N && i < Mpol; ++i) {
+            const double* r = &rows[((size_t)s * N + i) * 5];
+            const double sig = r[0], noi = r[1];
+            sigMG += r[2];
+            noiMG += r[3];
+            const double snr = sig / noi;
+            totalSignal += sig;
+            totalNoise += noi;
+            maxSignal = std::max(maxSignal, sig);
+            minSignal = std::min(minSignal, sig);
+            maxSNR = std::max(maxSNR, snr);
+            minSNR = std::min(minSNR, snr);
+          }
+        }
+      }
+      totalSNR = totalSignal / totalNoise;
+    }
+    double il = 0, mdl = 0;
+    if (doMDL) {
+      double smin = FMAX, smax = -FMAX, sum = 0;
+      for (int s = 0; s < S; ++s) {
+        const dh::HermitianSpectrum sp = dh::hermitian_extremes(dim, grams[s]);
+        smin = std::min(smin, std::max(sp.min, 0.0));
+        smax = std::max(smax, sp.max);
+        sum += sp.trace;
+      }
+      il = sum / ((double)dim * S);                 // mean of s^2 over min(M,N) * S values
+      mdl = smin / smax;                            // (s_min / s_max)^2
+    } else if (doIL) {
+      for (int s = 0; s < S; ++s) {
+        double tot = 0;
+        for (int i = 0; i < N; ++i) tot += rows[((size_t)s * N + i) * 5 + 4];
+        il += tot / N;                              // insertionLoss256, digHolo.cpp:3105-3143
+      }
+    }
+    o.v[DIGHOLO_METRIC_IL] = il;
+    o.v[DIGHOLO_METRIC_MDL] = mdl;
+    o.v[DIGHOLO_METRIC_DIAG] = totalSignal / signalCount;
+    o.v[DIGHOLO_METRIC_SNRAVG] = totalSNR;
+    o.v[DIGHOLO_METRIC_DIAGBEST] = maxSignal;
+    o.v[DIGHOLO_METRIC_DIAGWORST] = minSignal;
+    o.v[DIGHOLO_METRIC_SNRBEST] = maxSNR;
+    o.v[DIGHOLO_METRIC_SNRWORST] = minSNR;
+    o.v[DIGHOLO_METRIC_SNRMG] = sigMG / noiMG;
+  } else {
+    // one frame: total power and the power of mode 0 (digHolo.cpp:11093-11124)
+    std::vector<float2> c((size_t)P * Mpol);
+    DH_TRY(hipMemcpyAsync(c.data(), h.coefs.p, c.size() * sizeof(float2), hipMemcpyDeviceToHost, h.stream), "D2H");
+    DH_TRY(hipStreamSynchronize(h.stream), "sync");
+    double tot = 0, diag = 0;
+    for (int p = 0; p < P; ++p)
+      for (int m = 0; m < Mpol; ++m) {
+        const float2 v = c[(size_t)p * Mpol + m];
+        const double pw = (double)v.x * v.x + (double)v.y * v.y;
+        tot += pw;
+        if (m == 0) diag += pw;
+      }
+    o.v[DIGHOLO_METRIC_IL] = tot;
+    o.v[DIGHOLO_METRIC_MDL] = 0;
+    o.v[DIGHOLO_METRIC_DIAG] = diag;
+    o.v[DIGHOLO_METRIC_SNRAVG] = FMAX;
+    o.v[DIGHOLO_METRIC_DIAGBEST] = diag;
+    o.v[DIGHOLO_METRIC_DIAGWORST] = diag;
+    o.v[DIGHOLO_METRIC_SNRBEST] = FMAX;
+    o.v[DIGHOLO_METRIC_SNRWORST] = FMAX;
+    o.v[DIGHOLO_METRIC_SNRMG] = FMAX;
+  }
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+// metrics in dB, one wavelength: out[m][0] and the wavelength average out[m][1]
+// (digHolo.cpp:12183-12191)
+void store_metrics_db(Handle& h, const PowerMeterOut& o) {
+  const int L = 1;
+  h.aaLambdaCount = L;
+  h.aaMetrics.assign((size_t)DIGHOLO_METRIC_COUNT * (L + 1), 0.f);
+  for (int m = 0; m < DIGHOLO_METRIC_COUNT; ++m) {
+    const float lin = (float)o.v[m];
+    h.aaMetrics[(size_t)m * (L + 1) + 0] = 10.f * std::log10(lin);
+    h.aaMetrics[(size_t)m * (L + 1) + L] = 10.f * std::log10(lin / L);
+  }
 }
 
 void destroy_handle(Handle* h) {
@@ -1257,6 +1424,27 @@ EXT_C int digHoloGetWindowGeometry(int handleIdx, int* out) {
   const int vals[12] = {g.W, g.H, g.w, g.h, g.cx[0], g.cy[0], g.cx[1], g.cy[1], d.SX[0], d.SY[0], d.SX[1], d.SY[1]};
   std::memcpy(out, vals, sizeof(vals));
   return DIGHOLO_ERROR_SUCCESS;
+}
+
+EXT_C int digHoloAutoAlignCalcMetrics(int handleIdx) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (!h->haveBatch || !h->devInit) return DIGHOLO_ERROR_NULLPOINTER;
+  PowerMeterOut o;
+  const int e = power_meter(*h, CFG.aaPolIndependence != 0, true, true, true, CFG.aaMulConjTrans != 0, o);
+  if (e) return e;
+  store_metrics_db(*h, o);
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C float digHoloAutoAlignGetMetric(int handleIdx, int metricIdx) {
+  H_OR(0.f);
+  if (metricIdx < 0 || metricIdx >= DIGHOLO_METRIC_COUNT) return 0.f;
+  if (h->aaMetrics.empty()) return -std::numeric_limits<float>::max();
+  return h->aaMetrics[(size_t)metricIdx * (h->aaLambdaCount + 1) + h->aaLambdaCount];
+}
+EXT_C float* digHoloAutoAlignGetMetrics(int handleIdx, int metricIdx) {
+  H_OR(nullptr);
+  if (metricIdx < 0 || metricIdx >= DIGHOLO_METRIC_COUNT || h->aaMetrics.empty()) return nullptr;
+  return &h->aaMetrics[(size_t)metricIdx * (h->aaLambdaCount + 1)];
 }
 
 EXT_C int digHoloConfigSetFusedWindow(int handleIdx, int enabled) {

@@ -82,6 +82,14 @@ hipError_t launch_fused_window(const FusedArgs& a, const float2* twW, const floa
                                int frame_is_u16, const float2* factor, const float2* ref, float2* fields,
                                hipStream_t s);
 
+// Power-meter reductions (dh_metrics.hip).  out: [S][N][5] doubles per row
+// {diagonal, off-diagonal, in-group/g, out-of-group/g, total power}.
+hipError_t launch_metric_rows(const float2* coefs, int N, int M, int Mpol, int rowStride, int S, double* out,
+                              hipStream_t st);
+// FP64 Gram of matrix s: rows as vectors (N x N) or columns as vectors (M x M).
+hipError_t launch_gram_f64(const float2* coefs, int N, int M, int rowStride, int s, int rowsAreVectors,
+                           double2* G, hipStream_t st);
+
 // Separable HG overlap on MFMA: split = 0 plain FP32, 1 split-FP16 (hi+lo).
 hipError_t launch_overlap_mfma(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
                                float2* coefs, int split, hipStream_t s);

@@ -146,6 +146,11 @@ EXT_C int digHoloConfigGetAutoAlignMode(int handleIdx);                         
 EXT_C int digHoloConfigSetAutoAlignGoalIdx(int handleIdx, int goalIdx);           /* digHolo.h:1128 */
 EXT_C int digHoloConfigGetAutoAlignGoalIdx(int handleIdx);                        /* digHolo.h:1136 */
 
+/* ---- AutoAlign metrics ---- */
+EXT_C float digHoloAutoAlignGetMetric(int handleIdx, int metricIdx);              /* digHolo.h:1163 */
+EXT_C float* digHoloAutoAlignGetMetrics(int handleIdx, int metricIdx);            /* digHolo.h:1174 */
+EXT_C int digHoloAutoAlignCalcMetrics(int handleIdx);                             /* digHolo.h:1185 */
+
 /* ---- alignment parameters ---- */
 EXT_C int digHoloConfigSetBeamCentre(int handleIdx, int axisIdx, int polIdx, float centre); /* digHolo.h:1205 */
 EXT_C float digHoloConfigGetBeamCentre(int handleIdx, int axisIdx, int polIdx);   /* digHolo.h:1215 */

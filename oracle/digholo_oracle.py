@@ -449,6 +449,84 @@ METRIC_IL, METRIC_MDL, METRIC_DIAG, METRIC_SNRAVG, METRIC_DIAGBEST, METRIC_DIAGW
     METRIC_SNRBEST, METRIC_SNRWORST, METRIC_SNRMG = range(9)
 
 
+def _group_of(j: np.ndarray) -> np.ndarray:
+    g = np.floor((np.sqrt(8.0 * j + 1.0) - 1.0) / 2.0).astype(np.int64)
+    return g
+
+
+def power_meter(coefs: np.ndarray, independent: bool = False, mul_conj: bool = False,
+                do_il: bool = True, do_mdl: bool = True, do_xtalk: bool = True) -> np.ndarray:
+    """Linear metrics of a [B, pol, M] coefficient batch (digHolo.cpp:10714-11129, polIDX = -1).
+
+    A_s is the (modes x batch) matrix the reference hands to cgesvd: column b is
+    frame b's coefficient vector (both pols stacked unless the pols are
+    independent, in which case there is one matrix per pol).
+    """
+    c = np.asarray(coefs, dtype=np.complex128)
+    B, P, Mp = c.shape
+    out = np.zeros(9)
+    fmax = float(np.finfo(np.float32).max)
+    if B == 1:
+        pw = np.abs(c[0]) ** 2
+        out[METRIC_IL] = pw.sum()
+        out[METRIC_DIAG] = out[METRIC_DIAGBEST] = out[METRIC_DIAGWORST] = pw[:, 0].sum()
+        out[METRIC_SNRAVG] = out[METRIC_SNRBEST] = out[METRIC_SNRWORST] = out[METRIC_SNRMG] = fmax
+        return out
+    indep = independent or P == 1
+    if P == 2 and indep:
+        mats = [c[:, 0, :], c[:, 1, :]]                      # rows = frames
+    else:
+        mats = [c.reshape(B, P * Mp)]
+    S = len(mats)
+    tot_sig = tot_noi = sig_mg = noi_mg = 0.0
+    min_sig, max_sig, min_snr, max_snr = fmax, -fmax, fmax, -fmax
+    count = 1.0
+    if do_xtalk:
+        for A in mats:
+            X = A
+            if mul_conj:
+                X = A.conj() @ A.T                          # (A^H A) in the modes x batch sense, N x N
+            N, M = X.shape
+            count = float(M * N * S)
+            for i in range(min(N, Mp)):
+                pw = np.abs(X[i]) ** 2
+                j = np.arange(M)
+                diag = (j == i) | (j - Mp == i)
+                sig, noi = pw[diag].sum(), pw[~diag].sum()
+                gin = _group_of(np.array([i % Mp]))[0] + 1
+                gout = _group_of(j % Mp) + 1
+                sig_mg += (pw[gout == gin] / gin).sum()
+                noi_mg += (pw[gout != gin] / gin).sum()
+                snr = sig / noi
+                tot_sig += sig
+                tot_noi += noi
+                max_sig, min_sig = max(max_sig, sig), min(min_sig, sig)
+                max_snr, min_snr = max(max_snr, snr), min(min_snr, snr)
+    il = mdl = 0.0
+    if do_mdl:
+        svals = np.concatenate([np.linalg.svd(A.T, compute_uv=False) for A in mats])
+        il = float(np.mean(svals ** 2))
+        mdl = float(svals.min() ** 2 / svals.max() ** 2)
+    elif do_il:
+        il = sum(float((np.abs(A) ** 2).sum()) / B for A in mats)
+    out[METRIC_IL] = il
+    out[METRIC_MDL] = mdl
+    out[METRIC_DIAG] = tot_sig / count
+    out[METRIC_SNRAVG] = tot_sig / tot_noi if do_xtalk else 0.0
+    out[METRIC_DIAGBEST] = max_sig
+    out[METRIC_DIAGWORST] = min_sig
+    out[METRIC_SNRBEST] = max_snr
+    out[METRIC_SNRWORST] = min_snr
+    out[METRIC_SNRMG] = sig_mg / noi_mg if do_xtalk else float("nan")
+    return out
+
+
+def metrics_db(lin: np.ndarray) -> np.ndarray:
+    """10 log10 of the linear metrics (digHolo.cpp:12183-12191), one wavelength."""
+    with np.errstate(divide="ignore"):
+        return 10.0 * np.log10(np.asarray(lin, dtype=np.float64))
+
+
 def relative_l2(a, b) -> float:
     a = np.asarray(a)
     b = np.asarray(b)

@@ -201,3 +201,64 @@ def test_linearity():
     _, c1, _ = O.process_batch(f1, cfg)
     _, c2, _ = O.process_batch(f2, cfg)
     assert O.relative_l2(cA, a * c1 + b * c2) < 1e-12
+
+
+def test_power_meter_known_answers():
+    # identity transfer matrix: every frame is exactly its own mode
+    M = 10
+    G = 4
+    assert G * (G + 1) // 2 == M
+    c = np.zeros((M, 1, M), dtype=complex)
+    c[np.arange(M), 0, np.arange(M)] = 1.0
+    lin = O.power_meter(c)
+    assert np.isclose(lin[O.METRIC_IL], 1.0) and np.isclose(lin[O.METRIC_MDL], 1.0)
+    assert np.isclose(lin[O.METRIC_DIAG], M / (M * M))        # total diagonal / (M N)
+    assert np.isinf(lin[O.METRIC_SNRAVG])
+    # diagonal gains: singular values are the gains
+    gains = np.linspace(0.5, 2.0, M)
+    c2 = c * gains[:, None, None]
+    lin2 = O.power_meter(c2)
+    assert np.isclose(lin2[O.METRIC_MDL], (0.5 / 2.0) ** 2)
+    assert np.isclose(lin2[O.METRIC_IL], np.mean(gains ** 2))
+    assert np.isclose(lin2[O.METRIC_DIAGBEST], 4.0) and np.isclose(lin2[O.METRIC_DIAGWORST], 0.25)
+    # hand-computed crosstalk: 2 frames x 3 modes (G = 2)
+    c3 = np.array([[[1.0, 0.5, 0.0]], [[0.25, 2.0, 1.0]]], dtype=complex)
+    lin3 = O.power_meter(c3, do_mdl=False)
+    sig = [1.0, 4.0]
+    noi = [0.25, 0.0625 + 1.0]
+    assert np.isclose(lin3[O.METRIC_SNRAVG], sum(sig) / sum(noi))
+    assert np.isclose(lin3[O.METRIC_SNRBEST], max(s / n for s, n in zip(sig, noi)))
+    assert np.isclose(lin3[O.METRIC_IL], (1 + 0.25 + 0.0625 + 4 + 1) / 2)
+    # mode-group crosstalk: frame 0 is mode (0,0), group 1; frame 1 is mode (1,0), group 2
+    mg_sig = 1.0 / 1 + (4.0 + 1.0) / 2
+    mg_noi = 0.25 / 1 + 0.0625 / 2
+    assert np.isclose(lin3[O.METRIC_SNRMG], mg_sig / mg_noi)
+
+
+def test_power_meter_gram_eigenvalues_equal_svd():
+    # the GPU path takes the singular values from the Gram matrix; both orientations agree
+    rng = np.random.default_rng(9)
+    for B, M in ((7, 15), (15, 7)):
+        A = rng.standard_normal((B, M)) + 1j * rng.standard_normal((B, M))
+        s2 = np.linalg.svd(A.T, compute_uv=False) ** 2
+        if B <= M:
+            G = A @ A.conj().T
+        else:
+            G = A.T.conj() @ A
+        ev = np.linalg.eigvalsh(G)
+        assert np.allclose(np.sort(ev)[::-1][:len(s2)], s2)
+
+
+def test_power_meter_independent_pols_and_single_frame():
+    rng = np.random.default_rng(1)
+    c = rng.standard_normal((6, 2, 10)) + 1j * rng.standard_normal((6, 2, 10))
+    joint = O.power_meter(c)
+    indep = O.power_meter(c, independent=True)
+    s0 = np.linalg.svd(c[:, 0, :].T, compute_uv=False)
+    s1 = np.linalg.svd(c[:, 1, :].T, compute_uv=False)
+    s = np.concatenate([s0, s1])
+    assert np.isclose(indep[O.METRIC_MDL], s.min() ** 2 / s.max() ** 2)
+    assert not np.isclose(joint[O.METRIC_MDL], indep[O.METRIC_MDL])
+    one = O.power_meter(c[:1])
+    assert np.isclose(one[O.METRIC_IL], (np.abs(c[0]) ** 2).sum())
+    assert np.isclose(one[O.METRIC_DIAG], (np.abs(c[0, :, 0]) ** 2).sum())
