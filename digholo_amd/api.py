@@ -84,6 +84,7 @@ SIGNATURES: Dict[str, Tuple[object, List[object]]] = {
     "digHoloConfigGetAutoAlignMode": (c_int, [c_int]),
     "digHoloConfigSetAutoAlignGoalIdx": (c_int, [c_int, c_int]),
     "digHoloConfigGetAutoAlignGoalIdx": (c_int, [c_int]),
+    "digHoloAutoAlign": (c_float, [c_int]),
     "digHoloAutoAlignGetMetric": (c_float, [c_int, c_int]),
     "digHoloAutoAlignGetMetrics": (P_float, [c_int, c_int]),
     "digHoloAutoAlignCalcMetrics": (c_int, [c_int]),
@@ -138,6 +139,10 @@ DIGHOLO_ERROR_INVALIDDIMENSION = 5
 DIGHOLO_ERROR_INVALIDPOLARISATION = 6
 DIGHOLO_ERROR_INVALIDAXIS = 7
 DIGHOLO_ERROR_INVALIDARGUMENT = 8
+DIGHOLO_AUTOALIGNMODE_FULL = 0
+DIGHOLO_AUTOALIGNMODE_TWEAK = 1
+DIGHOLO_AUTOALIGNMODE_ESTIMATE = 2
+DIGHOLO_METRIC_COUNT = 9
 DIGHOLO_PRECISION_FP32 = 0
 DIGHOLO_PRECISION_SPLIT16 = 1
 

@@ -125,6 +125,11 @@ struct Handle {
   hipStream_t stream = nullptr;
 
   DevBuf<char> staging;
+  const void* stagedFrames = nullptr;   // host buffer the staging copy was made from
+  size_t stagedBytes = 0;
+  bool holdStaging = false;             // reuse the staging copy (set only inside AutoAlign)
+  bool holdCrop = false, holdWin = false;   // AutoAlign: keep FFT crops / Fourier window
+  Derived heldCrop, heldWin;
   DevBuf<float2> rowspec, win, fields, coefs, factor, ref, twW;
   DevBuf<float> hgx, hgy;
   DevBuf<uint8_t> mask;
@@ -141,6 +146,7 @@ struct Handle {
   // power meter: device scratch and the last metrics [metric][lambda + 1] in dB
   DevBuf<double> metricRows;
   DevBuf<double2> gram;
+  DevBuf<float2> coefsBackup, coefsMix;   // AutoAlign: best-so-far coefficients per pol
   std::vector<float> aaMetrics;   // DIGHOLO_METRIC_COUNT x (lambdaCount + 1)
   int aaLambdaCount = 0;
   bool coefsHostValid = false, fieldsHostValid = false;
@@ -188,7 +194,11 @@ int imod(int a, int n) {
 }
 
 // Returns false (and an error code) if the configuration cannot be processed.
-bool derive(const Config& c, int B, Derived& d, int& err) {
+// crop: keep the camera-plane FFT windows of an earlier derivation; win: keep
+// its Fourier window (position, size and mask).  AutoAlign holds them the way
+// the reference skips its FFT and IFFT while tweaking (digHolo.cpp:12620-12640).
+bool derive(const Config& c, int B, Derived& d, int& err, const Derived* crop = nullptr,
+            const Derived* win = nullptr) {
   err = DIGHOLO_ERROR_SUCCESS;
   const int P = c.polCount;
   // --- FFT window, digHolo.cpp:7283-7377 ---
@@ -217,6 +227,10 @@ bool derive(const Config& c, int B, Derived& d, int& err) {
     if (y + H > c.frameHeight) y = c.frameHeight - H;
     cx[p] = x;
     cy[p] = y;
+    if (crop) {
+      cx[p] = crop->g.cx[p];
+      cy[p] = crop->g.cy[p];
+    }
   }
   // --- IFFT window, digHolo.cpp:7868-8048 ---
   const float lam = c.wavelengthCentre;
@@ -227,7 +241,7 @@ bool derive(const Config& c, int B, Derived& d, int& err) {
   const float dky = k_axis(1, H, kfy) - k_axis(0, H, kfy);
   const float r0 = c.apertureSize * UNIT_ANGLE_F;
   int w = 0, h = 0, wc[2] = {0, 0}, hc[2] = {0, 0};
-  for (int p = 0; p < P; ++p) {
+  for (int p = 0; p < P && !win; ++p) {
     const int pp = c.polLockTilt ? 0 : p;
     const float tx = c.tiltX[pp] * UNIT_ANGLE_F;
     const float ty = c.tiltY[pp] * UNIT_ANGLE_F;
@@ -246,11 +260,20 @@ bool derive(const Config& c, int B, Derived& d, int& err) {
   }
   w = std::max(w, 16);
   h = std::max(h, 16);
+  if (win) {
+    w = win->g.w;
+    h = win->g.h;
+  }
   if (w > W / 2 || h > H) {
     err = DIGHOLO_ERROR_INVALIDDIMENSION;   // Fourier window larger than the r2c half plane
     return false;
   }
   for (int p = 0; p < P; ++p) {
+    if (win) {
+      d.SX[p] = win->SX[p];
+      d.SY[p] = win->SY[p];
+      continue;
+    }
     int sx = (int)(wc[p] - w / 2.0);
     if (sx < W / 2) sx = W / 2;
     if (sx < 0) sx = 0;
@@ -279,8 +302,9 @@ bool derive(const Config& c, int B, Derived& d, int& err) {
 
   // --- Fourier-window mask in float32 (digHolo.cpp:1950-2003, :8647-8656) ---
   d.mask.assign((size_t)P * h * w, 0);
+  if (win) d.mask = win->mask;
   const float fullShift = std::fabs((k_axis(1, H, kfy) - k_axis(0, H, kfy)) * (float)H);
-  for (int p = 0; p < P; ++p) {
+  for (int p = 0; p < P && !win; ++p) {
     const float k0 = 2.0f * PI_F / lam;
     const float tx = c.tiltX[p] * UNIT_ANGLE_F;
     const float ty = c.tiltY[p] * UNIT_ANGLE_F;
@@ -494,7 +518,7 @@ int enqueue_batch(Handle& h) {
 
   Derived d;
   int err;
-  if (!derive(c, c.batchCount, d, err)) {
+  if (!derive(c, c.batchCount, d, err, h.holdCrop ? &h.heldCrop : nullptr, h.holdWin ? &h.heldWin : nullptr)) {
     fprintf(stderr, "digHolo: window configuration is invalid for this frame (error %d)\n", err);
     return err;
   }
@@ -507,9 +531,15 @@ int enqueue_batch(Handle& h) {
   const size_t elemBytes = h.frameU16 ? 2 : 4;
   const void* dframes = h.frames;
   if (!is_device_pointer(h.frames) || (reinterpret_cast<uintptr_t>(h.frames) % 8) != 0) {
-    DH_TRY(h.staging.ensure(frameElems * elemBytes), "frame staging alloc");
-    DH_TRY(hipMemcpyAsync(h.staging.p, h.frames, frameElems * elemBytes, hipMemcpyDefault, h.stream),
-           "frame upload");
+    // AutoAlign reprocesses one frame buffer many times: upload it once
+    const bool staged = h.holdStaging && h.stagedFrames == h.frames && h.stagedBytes == frameElems * elemBytes;
+    if (!staged) {
+      DH_TRY(h.staging.ensure(frameElems * elemBytes), "frame staging alloc");
+      DH_TRY(hipMemcpyAsync(h.staging.p, h.frames, frameElems * elemBytes, hipMemcpyDefault, h.stream),
+             "frame upload");
+      h.stagedFrames = h.frames;
+      h.stagedBytes = frameElems * elemBytes;
+    }
     dframes = h.staging.p;
   }
   DH_TRY(h.fields.ensure(B * P * g.h * g.w), "alloc");
@@ -653,19 +683,41 @@ struct PowerMeterOut {
   double v[DIGHOLO_METRIC_COUNT];
 };
 
-int power_meter(Handle& h, bool independent, bool doIL, bool doMDL, bool doXtalk, bool mulConj, PowerMeterOut& o) {
+// polIdx < 0: both polarisations of the last batch.  polIdx = p: the metric
+// seen while optimising pol p alone (digHolo.cpp:10770-10860): its own
+// coefficients only when the pols are independent, else its coefficients
+// beside the other pol's best-so-far ones from h.coefsBackup.
+int power_meter(Handle& h, int polIdx, bool independent, bool doIL, bool doMDL, bool doXtalk, bool mulConj,
+                PowerMeterOut& o) {
   const dh::Geometry& g = h.cur.g;
   const int B = g.B, P = g.P, Mpol = g.M;
   const float FMAX = std::numeric_limits<float>::max();
   if (Mpol == 0) return DIGHOLO_ERROR_INVALIDARGUMENT;      // needs the Fourier-plane power (not built)
   if (B > 1) {
     const bool indep = independent || P == 1;
-    const int S = (P == 2 && indep) ? 2 : 1;
-    const int M = (P == 2 && !indep) ? 2 * Mpol : Mpol;    // svdM
+    int S = (P == 2 && indep) ? 2 : 1;
+    int M = (P == 2 && !indep) ? 2 * Mpol : Mpol;          // svdM
     const int N = B;                                       // svdN
-    const int rowStride = P * Mpol;
+    int rowStride = P * Mpol;
+    const float2* A = h.coefs.p;
+    if (polIdx >= 0 && P == 2) {
+      S = 1;
+      if (indep) {
+        A = h.coefs.p + (size_t)polIdx * Mpol;
+      } else {
+        DH_TRY(h.coefsMix.ensure((size_t)B * 2 * Mpol), "alloc");
+        DH_TRY(hipMemcpy2DAsync(h.coefsMix.p, 2 * Mpol * sizeof(float2), h.coefs.p + (size_t)polIdx * Mpol,
+                                rowStride * sizeof(float2), Mpol * sizeof(float2), B, hipMemcpyDeviceToDevice,
+                                h.stream), "mix");
+        DH_TRY(hipMemcpy2DAsync(h.coefsMix.p + Mpol, 2 * Mpol * sizeof(float2),
+                                h.coefsBackup.p + (size_t)(1 - polIdx) * Mpol, rowStride * sizeof(float2),
+                                Mpol * sizeof(float2), B, hipMemcpyDeviceToDevice, h.stream), "mix");
+        A = h.coefsMix.p;
+        rowStride = 2 * Mpol;
+      }
+    }
     DH_TRY(h.metricRows.ensure((size_t)S * N * 5), "alloc");
-    DH_TRY(dh::launch_metric_rows(h.coefs.p, N, M, Mpol, rowStride, S, h.metricRows.p, h.stream), "metric rows");
+    DH_TRY(dh::launch_metric_rows(A, N, M, Mpol, rowStride, S, h.metricRows.p, h.stream), "metric rows");
     std::vector<double> rows((size_t)S * N * 5);
     DH_TRY(hipMemcpyAsync(rows.data(), h.metricRows.p, rows.size() * sizeof(double), hipMemcpyDeviceToHost,
                           h.stream), "D2H");
@@ -678,7 +730,7 @@ int power_meter(Handle& h, bool independent, bool doIL, bool doMDL, bool doXtalk
     if (doMDL || (doXtalk && mulConj)) {
       DH_TRY(h.gram.ensure((size_t)dim * dim), "alloc");
       for (int s = 0; s < S; ++s) {
-        DH_TRY(dh::launch_gram_f64(h.coefs.p, N, M, rowStride, s, rowsVec ? 1 : 0, h.gram.p, h.stream), "gram");
+        DH_TRY(dh::launch_gram_f64(A, N, M, rowStride, s, rowsVec ? 1 : 0, h.gram.p, h.stream), "gram");
         std::vector<std::complex<double>> G((size_t)dim * dim);
         DH_TRY(hipMemcpyAsync(G.data(), h.gram.p, G.size() * sizeof(double2), hipMemcpyDeviceToHost, h.stream),
                "D2H");
@@ -803,6 +855,307 @@ void store_metrics_db(Handle& h, const PowerMeterOut& o) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// AutoAlign, tweak stage (digHolo.cpp:12315-12885).  One parameter at a time,
+// in the order waist, defocus, tilt x, centre x, tilt y, centre y: three
+// samples at -step/2, 0, +step/2 about the current value, then three samples
+// at the vertex of the least-squares quadratic through every sample so far
+// (held within 10 steps of the best value).  Every sample reprocesses the
+// whole batch on the GPU and evaluates the goal metric there; unlocked
+// parameters of the two polarisations are searched side by side, each judged
+// with the other pol's best-so-far coefficients.  Attempts repeat until the
+// best goal value moves by at most the tolerance (or 20 attempts).
+// ---------------------------------------------------------------------------
+enum { AA_WAIST = 0, AA_DEFOCUS = 1, AA_TILTX = 2, AA_CX = 3, AA_TILTY = 4, AA_CY = 5, AA_PARS = 6 };
+
+float& aa_param(Config& c, int p, int par) {               // digHolo.cpp:13021-13110
+  switch (par) {
+    case AA_WAIST: return c.waist[p];
+    case AA_DEFOCUS: return c.defocus[p];
+    case AA_TILTX: return c.tiltX[p];
+    case AA_CX: return c.beamCX[p];
+    case AA_TILTY: return c.tiltY[p];
+    default: return c.beamCY[p];
+  }
+}
+
+// Largest window radius any tilt can use without aliasing (digHolo.cpp:11964-11971)
+float window_radius_max(float px, float lambda) {
+  const float wc = (float)((-2.0 + std::sqrt(68.0)) / 16.0);
+  const float wmax = (float)(lambda / (2.0 * px));
+  return (wc * wmax) / UNIT_ANGLE_F;
+}
+// A third of the smallest tilt magnitude over the pols (digHolo.cpp:11978-11993)
+float window_radius_rec(const Config& c) {
+  float m = std::numeric_limits<float>::max();
+  for (int p = 0; p < c.polCount; ++p) m = std::min(m, std::sqrt(c.tiltX[p] * c.tiltX[p] + c.tiltY[p] * c.tiltY[p]));
+  return m / 3.0f;
+}
+
+// Vertex -b/(2a) of the least-squares quadratic a x^2 + b x + c through the
+// samples, or NaN when the fit is degenerate.  The reference iterates
+// Gauss-Newton steps on this model (digHolo.cpp:3146-3257); the model is
+// linear in a, b, c, so its fixed point is this closed-form solution, solved
+// here in double on centred, scaled abscissae.
+double quadratic_vertex(const double* x, const double* y, int n) {
+  double xm = 0, s = 0;
+  for (int i = 0; i < n; ++i) {
+    if (!std::isfinite(x[i]) || !std::isfinite(y[i])) return NAN;
+    xm += x[i];
+  }
+  xm /= n;
+  for (int i = 0; i < n; ++i) s = std::max(s, std::fabs(x[i] - xm));
+  if (!(s > 0)) return NAN;
+  double S0 = 0, S1 = 0, S2 = 0, S3 = 0, S4 = 0, T0 = 0, T1 = 0, T2 = 0;
+  for (int i = 0; i < n; ++i) {
+    const double u = (x[i] - xm) / s, u2 = u * u;
+    S0 += 1; S1 += u; S2 += u2; S3 += u2 * u; S4 += u2 * u2;
+    T0 += y[i]; T1 += y[i] * u; T2 += y[i] * u2;
+  }
+  // normal equations [S4 S3 S2; S3 S2 S1; S2 S1 S0] [a b c]' = [T2 T1 T0]', Cramer's rule
+  auto det3 = [](double a, double b, double c, double d, double e, double f, double g, double hh, double i) {
+    return a * (e * i - f * hh) - b * (d * i - f * g) + c * (d * hh - e * g);
+  };
+  const double D = det3(S4, S3, S2, S3, S2, S1, S2, S1, S0);
+  if (D == 0 || !std::isfinite(D)) return NAN;
+  const double a = det3(T2, S3, S2, T1, S2, S1, T0, S1, S0) / D;
+  const double b = det3(S4, T2, S2, S3, T1, S1, S2, T0, S0) / D;
+  if (a == 0) return NAN;
+  return xm + s * (-b / (2 * a));
+}
+
+void to_db(const PowerMeterOut& o, float* db) {
+  for (int m = 0; m < DIGHOLO_METRIC_COUNT; ++m) db[m] = 10.f * std::log10((float)o.v[m]);
+}
+
+struct HoldGuard {     // frames upload once for the whole search; holds end with it
+  Handle& h;
+  explicit HoldGuard(Handle& hh) : h(hh) { h.holdStaging = true; }
+  ~HoldGuard() { h.holdStaging = h.holdCrop = h.holdWin = false; }
+};
+
+int autoalign_tweak(Handle& h, float& goalDb) {
+  Config& c = h.cfg;
+  const int P = c.polCount;
+  const int goal0 = c.aaGoalIdx;
+  const bool indep = c.aaPolIndependence != 0, mulConj = c.aaMulConjTrans != 0;
+  const float tol = c.aaTol;
+  const float FMAX = std::numeric_limits<float>::max();
+  HoldGuard hold(h);
+  bool processed = false;   // the first evaluation fixes the FFT crops (digHolo.cpp:12620-12625)
+
+  // window radius and step sizes (digHolo.cpp:12277-12327)
+  if (c.aaFourierRadius)
+    c.apertureSize = c.aaTilt ? window_radius_max(c.pixelSize, c.wavelengthCentre) : window_radius_rec(c);
+  float baseStep[AA_PARS];
+  baseStep[AA_TILTX] = baseStep[AA_TILTY] = c.apertureSize * 0.1f;
+  baseStep[AA_CX] = baseStep[AA_CY] = c.waist[0] * 0.1f;
+  baseStep[AA_WAIST] = c.waist[0] * 0.1f;
+  baseStep[AA_DEFOCUS] = 0.4f;
+  for (int p = 0; p < P; ++p) baseStep[AA_DEFOCUS] = std::max(baseStep[AA_DEFOCUS], std::fabs(c.defocus[p]) * 0.1f);
+
+  // bounds on what may be tried (digHolo.cpp:12336-12359)
+  float lim[AA_PARS][2];
+  const float tiltMaxX = (c.wavelengthCentre / (2 * c.pixelSize)) / UNIT_ANGLE_F;
+  lim[AA_TILTX][0] = 0;
+  lim[AA_TILTX][1] = tiltMaxX;
+  lim[AA_TILTY][0] = -(c.apertureSize + tiltMaxX);
+  lim[AA_TILTY][1] = c.apertureSize + tiltMaxX;
+  lim[AA_DEFOCUS][0] = -100;
+  lim[AA_DEFOCUS][1] = 100;
+  lim[AA_WAIST][0] = c.pixelSize / 3.0f;
+  lim[AA_WAIST][1] = c.pixelSize * (float)std::max(c.frameHeight, c.frameWidth);
+  lim[AA_CX][0] = -(c.frameWidth / 2.0f) * c.pixelSize;
+  lim[AA_CX][1] = (c.frameWidth / 2.0f) * c.pixelSize;
+  lim[AA_CY][0] = -(c.frameHeight / 2.0f) * c.pixelSize;
+  lim[AA_CY][1] = (c.frameHeight / 2.0f) * c.pixelSize;
+
+  float stepBound[AA_PARS];
+  for (int i = 0; i < AA_PARS; ++i) stepBound[i] = baseStep[i] * 10;
+
+  float best[2][AA_PARS];
+  auto save = [&](int p) { for (int i = 0; i < AA_PARS; ++i) best[p][i] = aa_param(c, p, i); };
+  auto load = [&](int p) { for (int i = 0; i < AA_PARS; ++i) aa_param(c, p, i) = best[p][i]; };
+  save(0);
+  save(1);
+
+  constexpr int sampleCount = 3, stepCount = 5;
+  float outBest[2][DIGHOLO_METRIC_COUNT];
+  for (auto& r : outBest) std::fill(r, r + DIGHOLO_METRIC_COUNT, -FMAX);
+  PowerMeterOut outf{};
+  float stepSize[AA_PARS];
+  std::copy(baseStep, baseStep + AA_PARS, stepSize);
+  const int goalCount = goal0 ? 2 : 1;
+  float lastErrTol = goal0 ? 0.1f + tol * 10.0f : tol;
+
+  // device copies of coefficients (digHolo.cpp:12632-12637, :12692-12699)
+  auto backup_all = [&]() -> int {
+    const dh::Geometry& g = h.cur.g;
+    const size_t n = (size_t)g.B * g.P * g.M;
+    DH_TRY(h.coefsBackup.ensure(std::max<size_t>(n, 1)), "alloc");
+    DH_TRY(hipMemcpyAsync(h.coefsBackup.p, h.coefs.p, n * sizeof(float2), hipMemcpyDeviceToDevice, h.stream), "copy");
+    return 0;
+  };
+  auto process = [&]() -> int {
+    const int e = enqueue_batch(h);
+    if (e) return e;
+    if (!processed) {
+      h.heldCrop = h.cur;
+      h.holdCrop = true;
+      processed = true;
+    }
+    return 0;
+  };
+  auto backup_pol = [&](int p) -> int {
+    const dh::Geometry& g = h.cur.g;
+    const size_t pitch = (size_t)g.P * g.M * sizeof(float2);
+    DH_TRY(hipMemcpy2DAsync(h.coefsBackup.p + (size_t)p * g.M, pitch, h.coefs.p + (size_t)p * g.M, pitch,
+                            (size_t)g.M * sizeof(float2), g.B, hipMemcpyDeviceToDevice, h.stream), "copy");
+    return 0;
+  };
+
+  for (int gp = 0; gp < goalCount; ++gp) {
+    const int goal = gp ? goal0 : 0;
+    const bool doIL = goal == 0, doMDL = goal == 1, doXtalk = goal >= 2;
+    bool converged = false, firstRun = true;
+    float bestPwr[2] = {-FMAX, -FMAX};
+    double lastErr[2] = {FMAX, FMAX};
+    outBest[0][goal] = outBest[1][goal] = -FMAX;
+    int attempts = 0;
+    if (gp) {    // a second pass for a goal other than IL starts near the answer: finer steps
+      for (int i = 0; i < AA_PARS; ++i) stepSize[i] = baseStep[i] / 10.0f;
+      lastErrTol = tol;
+    }
+    while (!converged) {
+      bool enabled[AA_PARS];
+      enabled[AA_TILTX] = enabled[AA_TILTY] = c.aaTilt != 0;
+      enabled[AA_CX] = enabled[AA_CY] = c.aaCentre != 0;
+      enabled[AA_WAIST] = c.aaWaist != 0;
+      enabled[AA_DEFOCUS] = c.aaDefocus != 0;
+      for (int par = 0; par < AA_PARS; ++par) {
+        if (!enabled[par]) continue;
+        const bool lock = ((par == AA_TILTX || par == AA_TILTY) && c.polLockTilt) ||
+                          (par == AA_DEFOCUS && c.polLockDefocus) || (par == AA_WAIST && c.polLockWaist);
+        float var0[2];
+        var0[0] = aa_param(c, 0, par);
+        var0[1] = lock ? var0[0] : aa_param(c, 1, par);
+        float maxPwr[2] = {-FMAX, -FMAX};
+        // a tilt sweep only re-applies the reference wave: the Fourier window of
+        // the last full evaluation stays (digHolo.cpp:12626-12645)
+        const bool tiltPar = par == AA_TILTX || par == AA_TILTY;
+        if (tiltPar && processed) {
+          h.heldWin = h.cur;
+          h.holdWin = true;
+        }
+        float dTxs[stepCount + 2][2] = {};
+        double xs[2][stepCount + 1], ys[2][stepCount + 1];
+        for (int step = 0; step <= stepCount; ++step) {
+          float* dTx = dTxs[step];
+          bool valid[2] = {true, true};
+          for (int p = 0; p < P; ++p) {
+            if (step < sampleCount)
+              dTx[p] = (float)(var0[p] + ((step - (sampleCount - 1) / 2.0) * stepSize[par] / (sampleCount - 1)));
+            if (lock) dTx[1] = dTx[0];
+            dTx[p] = std::min(std::max(dTx[p], lim[par][0]), lim[par][1]);
+            valid[p] = std::isfinite(dTx[p]);
+            if (valid[p]) aa_param(c, p, par) = dTx[p];
+          }
+          if (c.aaFourierRadius) c.apertureSize = std::min(c.apertureSize, window_radius_rec(c));
+          int e = process();
+          if (e) return e;
+          if (tiltPar && !h.holdWin) {
+            h.heldWin = h.cur;
+            h.holdWin = true;
+          }
+          if (firstRun) {
+            if ((e = backup_all())) return e;
+            firstRun = false;
+          }
+          float out[2][DIGHOLO_METRIC_COUNT];
+          PowerMeterOut o;
+          if ((!lock && P > 1) || indep) {
+            for (int p = 0; p < P; ++p) {
+              if ((e = power_meter(h, p, indep, doIL, doMDL, doXtalk, mulConj, o))) return e;
+              to_db(o, out[p]);
+            }
+          } else {
+            if ((e = power_meter(h, -1, indep, doIL, doMDL, doXtalk, mulConj, o))) return e;
+            to_db(o, out[0]);
+            std::copy(out[0], out[0] + DIGHOLO_METRIC_COUNT, out[1]);
+          }
+          for (int p = 0; p < P; ++p) {
+            const float pwr = out[p][goal];
+            xs[p][step] = dTx[p];
+            ys[p][step] = pwr;
+            if (pwr > maxPwr[p] && valid[p]) {
+              maxPwr[p] = pwr;
+              if (pwr > bestPwr[p]) {
+                save(p);
+                std::copy(out[p], out[p] + DIGHOLO_METRIC_COUNT, outBest[p]);
+                if ((e = backup_pol(p))) return e;
+                bestPwr[p] = pwr;
+              }
+            }
+            if (step >= sampleCount - 1) {
+              const double vx = quadratic_vertex(xs[p], ys[p], step + 1);
+              double next;
+              if (std::isfinite(vx)) {
+                const double d = vx - best[p][par];
+                next = std::fabs(d) > stepBound[par] ? best[p][par] + (d > 0 ? stepBound[par] : -stepBound[par]) : vx;
+              } else {   // no usable fit: a point near the best so far
+                next = best[p][par] + ((step - (stepCount - 1) / 2.0) * stepSize[par] / (stepCount - 1));
+              }
+              dTxs[step + 1][p] = (float)next;
+            }
+          }
+        }
+        h.holdWin = false;
+        for (int p = 0; p < P; ++p) load(p);
+        int e = process();
+        if (e) return e;
+      }
+      if (firstRun) {    // every parameter switched off
+        int e = process();
+        if (e || (e = backup_all())) return e;
+        firstRun = false;
+      }
+      int e = power_meter(h, -1, indep, true, true, true, mulConj, outf);
+      if (e || (e = backup_all())) return e;
+      double dErr = 0;
+      for (int p = 0; p < P; ++p) {
+        dErr += std::fabs((double)outBest[p][goal] - lastErr[p]);
+        lastErr[p] = outBest[p][goal];
+      }
+      dErr /= P;
+      ++attempts;
+      if (c.verbosity >= DIGHOLO_VERBOSITY_DEBUG) {
+        float db[DIGHOLO_METRIC_COUNT];
+        to_db(outf, db);
+        fprintf(stdout, "AutoAlign attempt %d  goal %f  IL %f  MDL %f  dErr %f\n", attempts, db[goal0],
+                db[DIGHOLO_METRIC_IL], db[DIGHOLO_METRIC_MDL], dErr);
+        for (int p = 0; p < P; ++p)
+          fprintf(stdout, "  pol %d  tilt %.4f %.4f  defocus %.4f  waist %.3f um  centre %.3f %.3f um\n", p,
+                  best[p][AA_TILTX], best[p][AA_TILTY], best[p][AA_DEFOCUS], best[p][AA_WAIST] * 1e6,
+                  best[p][AA_CX] * 1e6, best[p][AA_CY] * 1e6);
+        fflush(stdout);
+      }
+      if (dErr <= lastErrTol || attempts == 20) converged = true;
+    }
+  }
+  // leave the handle as a plain ProcessBatch at the final parameters would:
+  // FFT crops follow the final beam centres, and the metrics are theirs
+  h.holdCrop = h.holdWin = false;
+  int e = enqueue_batch(h);
+  if (e || (e = power_meter(h, -1, indep, true, true, true, mulConj, outf))) return e;
+  store_metrics_db(h, outf);
+  goalDb = h.aaMetrics[(size_t)goal0 * (h.aaLambdaCount + 1) + h.aaLambdaCount];
+  if (c.verbosity >= DIGHOLO_VERBOSITY_BASIC) {
+    fprintf(stdout, "AutoAlign goal %f  IL %f\n", goalDb, h.aaMetrics[h.aaLambdaCount]);
+    fflush(stdout);
+  }
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
 void destroy_handle(Handle* h) {
   if (!h) return;
   if (h->devInit) {
@@ -821,6 +1174,10 @@ void destroy_handle(Handle* h) {
     for (auto& kv : h->ffts) kv.second->tw.release();
     h->hCoefs.release();
     h->hFields.release();
+    h->metricRows.release();
+    h->gram.release();
+    h->coefsBackup.release();
+    h->coefsMix.release();
     for (auto& set : h->evPool)
       for (auto& e : set) (void)hipEventDestroy(e);
     if (h->ownStream) (void)hipStreamDestroy(h->ownStream);
@@ -1430,10 +1787,29 @@ EXT_C int digHoloAutoAlignCalcMetrics(int handleIdx) {
   H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
   if (!h->haveBatch || !h->devInit) return DIGHOLO_ERROR_NULLPOINTER;
   PowerMeterOut o;
-  const int e = power_meter(*h, CFG.aaPolIndependence != 0, true, true, true, CFG.aaMulConjTrans != 0, o);
+  const int e = power_meter(*h, -1, CFG.aaPolIndependence != 0, true, true, true, CFG.aaMulConjTrans != 0, o);
   if (e) return e;
   store_metrics_db(*h, o);
   return DIGHOLO_ERROR_SUCCESS;
+}
+// The tweak stage only: the parameter-estimate ("snap") stage of the FULL and
+// ESTIMATE modes is not built, so those modes report an error.
+EXT_C float digHoloAutoAlign(int handleIdx) {
+  H_OR(0.f);
+  const float FMAX = std::numeric_limits<float>::max();
+  if (CFG.aaMode != DIGHOLO_AUTOALIGNMODE_TWEAK || CFG.maxMG <= 0) {
+    fprintf(stderr, "digHolo: AutoAlign needs DIGHOLO_AUTOALIGNMODE_TWEAK and a basis group count > 0 in this "
+                    "build (the parameter-estimate stage is not implemented)\n");
+    return -FMAX;
+  }
+  if (!h->frames) return -FMAX;
+  float goal = -FMAX;
+  const int e = autoalign_tweak(*h, goal);
+  if (e) {
+    fprintf(stderr, "digHolo: AutoAlign failed (error %d)\n", e);
+    return -FMAX;
+  }
+  return goal;
 }
 EXT_C float digHoloAutoAlignGetMetric(int handleIdx, int metricIdx) {
   H_OR(0.f);

@@ -146,7 +146,10 @@ EXT_C int digHoloConfigGetAutoAlignMode(int handleIdx);                         
 EXT_C int digHoloConfigSetAutoAlignGoalIdx(int handleIdx, int goalIdx);           /* digHolo.h:1128 */
 EXT_C int digHoloConfigGetAutoAlignGoalIdx(int handleIdx);                        /* digHolo.h:1136 */
 
-/* ---- AutoAlign metrics ---- */
+/* ---- AutoAlign ---- */
+/* Tweak mode (DIGHOLO_AUTOALIGNMODE_TWEAK) only; returns the goal metric in dB,
+   -FLT_MAX for the modes that need the parameter-estimate stage. */
+EXT_C float digHoloAutoAlign(int handleIdx);                                      /* digHolo.h:1152 */
 EXT_C float digHoloAutoAlignGetMetric(int handleIdx, int metricIdx);              /* digHolo.h:1163 */
 EXT_C float* digHoloAutoAlignGetMetrics(int handleIdx, int metricIdx);            /* digHolo.h:1174 */
 EXT_C int digHoloAutoAlignCalcMetrics(int handleIdx);                             /* digHolo.h:1185 */

@@ -155,3 +155,22 @@ def test_process_without_gpu_fails_loudly(lib, gpu_available):
     frames = np.zeros((2, 256, 320), dtype=np.uint16)
     with pytest.raises(RuntimeError):
         api.process_batch(lib, h, frames)
+
+
+def test_autoalign_modes_without_estimate_stage(lib):
+    fmax = np.finfo(np.float32).max
+    assert lib.digHoloAutoAlign(12345) == 0.0                       # invalid handle (digHolo.cpp:14989-14999)
+    h = lib.digHoloCreate()
+    api.configure(lib, h, frame_width=320, frame_height=256, fft_window=(128, 128), group_count=4)
+    frames = np.zeros((2, 256, 320), dtype=np.uint16)
+    assert lib.digHoloSetBatchUint16(h, 2, frames.ctypes.data_as(C.POINTER(C.c_ushort)), 0) == 0
+    # FULL (the default) and ESTIMATE need the parameter-estimate stage: refused before any GPU work
+    assert lib.digHoloConfigGetAutoAlignMode(h) == api.DIGHOLO_AUTOALIGNMODE_FULL
+    assert lib.digHoloAutoAlign(h) == -fmax
+    assert lib.digHoloConfigSetAutoAlignMode(h, api.DIGHOLO_AUTOALIGNMODE_ESTIMATE) == 0
+    assert lib.digHoloAutoAlign(h) == -fmax
+    # TWEAK with no modes (group count 0) needs it too
+    assert lib.digHoloConfigSetAutoAlignMode(h, api.DIGHOLO_AUTOALIGNMODE_TWEAK) == 0
+    assert lib.digHoloConfigSetBasisGroupCount(h, 0) == 0
+    assert lib.digHoloAutoAlign(h) == -fmax
+    lib.digHoloDestroy(h)
