@@ -85,6 +85,11 @@ SIGNATURES: Dict[str, Tuple[object, List[object]]] = {
     "digHoloConfigSetAutoAlignGoalIdx": (c_int, [c_int, c_int]),
     "digHoloConfigGetAutoAlignGoalIdx": (c_int, [c_int]),
     "digHoloAutoAlign": (c_float, [c_int]),
+    "digHoloGetFields16": (c_int, [c_int, P_int, P_int, C.POINTER(C.POINTER(C.c_short)),
+                                   C.POINTER(C.POINTER(C.c_short)), C.POINTER(P_float), C.POINTER(P_float),
+                                   C.POINTER(P_float), P_int, P_int]),
+    "digHoloBatchGetSummary": (c_int, [c_int, c_int, P_int, P_int, P_int, C.POINTER(P_float), P_int, P_int,
+                                       C.POINTER(P_float), C.POINTER(P_float), C.POINTER(P_float)]),
     "digHoloAutoAlignGetMetric": (c_float, [c_int, c_int]),
     "digHoloAutoAlignGetMetrics": (P_float, [c_int, c_int]),
     "digHoloAutoAlignCalcMetrics": (c_int, [c_int]),
@@ -143,6 +148,9 @@ DIGHOLO_AUTOALIGNMODE_FULL = 0
 DIGHOLO_AUTOALIGNMODE_TWEAK = 1
 DIGHOLO_AUTOALIGNMODE_ESTIMATE = 2
 DIGHOLO_METRIC_COUNT = 9
+DIGHOLO_ANALYSIS_COUNT = 7
+DIGHOLO_ANALYSIS_MAXABS = 3
+DIGHOLO_ANALYSIS_MAXABSIDX = 4
 DIGHOLO_PRECISION_FP32 = 0
 DIGHOLO_PRECISION_SPLIT16 = 1
 
@@ -246,3 +254,47 @@ def process_batch(lib, h, frames: np.ndarray):
     x = np.ctypeslib.as_array(xp, shape=(w.value,)).copy()
     y = np.ctypeslib.as_array(yp, shape=(hh.value,)).copy()
     return coefs, fields, x, y
+
+
+def batch_summary(lib, h, plane: int):
+    """digHoloBatchGetSummary as numpy arrays.
+
+    Returns (params [7, pol, B + 1] float32, peak_index [pol, B + 1] int32,
+    summed [pol, Ny, Nx] float32, x [Nx], y [Ny]).
+    """
+    pc, tc, pol, nx, ny = c_int(), c_int(), c_int(), c_int(), c_int()
+    par, xp, yp, tot = P_float(), P_float(), P_float(), P_float()
+    rc = lib.digHoloBatchGetSummary(h, plane, C.byref(pc), C.byref(tc), C.byref(pol), C.byref(par), C.byref(nx),
+                                    C.byref(ny), C.byref(xp), C.byref(yp), C.byref(tot))
+    if rc != DIGHOLO_ERROR_SUCCESS:
+        raise RuntimeError(f"digHoloBatchGetSummary returned {rc}")
+    P, T, Nx, Ny = pol.value, tc.value, nx.value, ny.value
+    flat = np.ctypeslib.as_array(par, shape=(pc.value * P * T,)).copy()
+    params = flat.reshape(pc.value, P, T)
+    peak_index = flat.view(np.int32).reshape(pc.value, P, T)[DIGHOLO_ANALYSIS_MAXABSIDX].copy()
+    # the summed maps are [pol][W * H] in the reference's layout; Ny * Nx used per pol
+    geo = window_geometry(lib, h)
+    stride = geo["W"] * geo["H"]
+    maps = np.ctypeslib.as_array(tot, shape=(P * stride,)).copy().reshape(P, stride)[:, :Ny * Nx]
+    x = np.ctypeslib.as_array(xp, shape=(Nx,)).copy()
+    y = np.ctypeslib.as_array(yp, shape=(Ny,)).copy()
+    return params, peak_index, maps.reshape(P, Ny, Nx), x, y
+
+
+def get_fields16(lib, h):
+    """digHoloGetFields16: (re [B, pol, h, w] int16, im, scale [pol, B] float32, x, y)."""
+    b, p, w, hh = c_int(), c_int(), c_int(), c_int()
+    re, im = C.POINTER(C.c_short)(), C.POINTER(C.c_short)()
+    sc, xp, yp = P_float(), P_float(), P_float()
+    rc = lib.digHoloGetFields16(h, C.byref(b), C.byref(p), C.byref(re), C.byref(im), C.byref(sc), C.byref(xp),
+                                C.byref(yp), C.byref(w), C.byref(hh))
+    if rc != DIGHOLO_ERROR_SUCCESS:
+        raise RuntimeError(f"digHoloGetFields16 returned {rc}")
+    B, P, W, H = b.value, p.value, w.value, hh.value
+    n = B * P * H * W
+    R = np.ctypeslib.as_array(re, shape=(n,)).copy().reshape(B, P, H, W)
+    I = np.ctypeslib.as_array(im, shape=(n,)).copy().reshape(B, P, H, W)
+    scale = np.ctypeslib.as_array(sc, shape=(P * (B + 1),)).copy().reshape(P, B + 1)[:, :B]
+    x = np.ctypeslib.as_array(xp, shape=(W,)).copy()
+    y = np.ctypeslib.as_array(yp, shape=(H,)).copy()
+    return R, I, scale, x, y

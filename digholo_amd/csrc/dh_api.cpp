@@ -152,6 +152,22 @@ struct Handle {
   bool coefsHostValid = false, fieldsHostValid = false;
   std::vector<float> xAxis, yAxis;
 
+  // Plane analysis, computed on request from the last batch's device state:
+  // plane 0 the R2C Fourier plane of the frames, plane 1 the fields.
+  const void* lastFrames = nullptr;   // device frames the last batch read
+  int lastFrameU16 = 0;
+  bool analysisValid[2] = {false, false};
+  std::vector<float> analysis[2];     // [7][P][B + 1]
+  std::vector<float> anaSum[2];       // [P][W * H], Ny * Nx used per pol
+  std::vector<float> anaX[2], anaY[2];
+  DevBuf<float2> spec0;
+  DevBuf<float> anaPw, anaSumDev, anaAxes, anaScale;
+  DevBuf<uint8_t> anaInc;
+  DevBuf<double> anaStats;
+  DevBuf<short> f16;                  // int16 fields [2][B][P][h][w]
+  PinnedBuf<short> hF16;
+  bool f16Valid = false;
+
   // Stage timing: one set of 6 events per batch enqueued since the last
   // collection, so back-to-back batches need no host synchronisation.
   bool allowFused = true;   // false forces the staged kernels
@@ -623,6 +639,10 @@ int enqueue_batch(Handle& h) {
   h.haveBatch = true;
   h.coefsHostValid = false;
   h.fieldsHostValid = false;
+  h.lastFrames = dframes;
+  h.lastFrameU16 = h.frameU16;
+  h.analysisValid[0] = h.analysisValid[1] = false;
+  h.f16Valid = false;
   // reconstructed-plane axes, float32 like digHolo.cpp:8184-8189 / :2875-2880
   h.xAxis.resize(g.w);
   h.yAxis.resize(g.h);
@@ -671,6 +691,148 @@ int download_fields(Handle& h) {
   DH_TRY(hipMemcpyAsync(h.hFields.p, h.fields.p, n * sizeof(float2), hipMemcpyDeviceToHost, h.stream), "D2H");
   DH_TRY(hipStreamSynchronize(h.stream), "sync");
   h.fieldsHostValid = true;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// Plane analysis (digHolo.cpp:6931-6984 Fourier plane, :8840-8870 field plane,
+// statistics :2354-2654 finished as :9112-9140).  The reference runs both inside
+// every batch; here they run when BatchGetSummary/GetFields16 ask, from the
+// device frames and fields the last batch left (the staged copy of host frames,
+// or the caller's device frames, which must then be unchanged).
+// ---------------------------------------------------------------------------
+int compute_analysis(Handle& h, int plane) {
+  if (!h.haveBatch || !h.devInit) return DIGHOLO_ERROR_NULLPOINTER;
+  if (h.analysisValid[plane]) return DIGHOLO_ERROR_SUCCESS;
+  const Derived& d = h.cur;
+  const dh::Geometry& g = d.g;
+  const int B = g.B, P = g.P;
+  const Config& c = h.cfg;
+  dh::PlaneArgs a{};
+  std::vector<float> xs, ys;
+  std::vector<uint8_t> inc;
+  const float2* src = nullptr;
+  size_t planeStride = 0;
+  if (plane == 0) {
+    const int Nx = g.W / 2 + 1, Ny = g.H;
+    dh::Geometry g2 = g;          // the whole R2C plane: every column from kx = 0
+    g2.w = Nx;
+    for (int p = 0; p < P; ++p) g2.c0[p] = 0;
+    const dh::FFTPlan* pRow = fft_plan(h, g.W / 2);
+    const dh::FFTPlan* pCol = fft_plan(h, g.H);
+    if (!pRow || !pCol || !h.lastFrames) return DIGHOLO_ERROR_NULLPOINTER;
+    DH_TRY(h.spec0.ensure((size_t)B * P * Ny * Nx), "alloc");
+    DH_TRY(dh::launch_row_fft(g2, *pRow, h.twW.p, h.lastFrames, h.lastFrameU16, h.spec0.p, h.stream), "row FFT");
+    DH_TRY(dh::launch_col_fft_full(g2, *pCol, h.spec0.p, h.stream), "column FFT");
+    const float kfx = k_factor(g.W, c.pixelSize), kfy = k_factor(g.H, c.pixelSize);
+    xs.resize(Nx);
+    ys.resize(Ny);
+    for (int i = 0; i < Nx; ++i) xs[i] = k_axis(i, g.W, kfx);
+    for (int j = 0; j < Ny; ++j) ys[j] = k_axis(j, g.H, kfy);
+    // zero order left out: k^2 >= (2 k0 sin r)^2 (digHolo.cpp:6936-6941, :2385-2397)
+    const float k0 = 2 * PI_F / c.wavelengthCentre;
+    const float R = 2 * k0 * std::sin(c.apertureSize * UNIT_ANGLE_F);
+    const float R2 = R * R;
+    inc.resize((size_t)Nx * Ny);
+    for (int j = 0; j < Ny; ++j)
+      for (int i = 0; i < Nx; ++i) {
+        const float xx = xs[i] * xs[i], yy = ys[j] * ys[j];
+        inc[(size_t)j * Nx + i] = (xx + yy) >= R2 ? 1 : 0;
+      }
+    a.Nx = Nx;
+    a.Ny = Ny;
+    a.x0 = 1;                      // kx = 0 column out; columns past 8 * floor(Nx / 8) not visited
+    a.x1 = 8 * (Nx / 8);
+    a.skip0 = 0;                   // ky = 0 and Nyquist rows out
+    a.skip1 = Ny / 2;
+    src = h.spec0.p;
+    planeStride = (size_t)Ny * Nx;
+  } else {
+    a.Nx = g.w;
+    a.Ny = g.h;
+    a.x0 = 0;
+    a.x1 = 8 * (g.w / 8);
+    a.skip0 = a.skip1 = -1;
+    xs = h.xAxis;
+    ys = h.yAxis;
+    src = h.fields.p;
+    planeStride = (size_t)g.h * g.w;
+  }
+  const int Nx = a.Nx, Ny = a.Ny, N = Nx * Ny;
+  a.inStride = Nx;
+  a.norm = (float)(1.0 / ((size_t)Nx * Ny));
+  const float yShift = std::fabs((ys[1] - ys[0]) * Ny);
+  std::vector<float> axes(Nx + 2 * Ny);
+  std::copy(xs.begin(), xs.end(), axes.begin());
+  std::copy(ys.begin(), ys.end(), axes.begin() + Nx);
+  for (int j = 0; j < Ny; ++j) axes[Nx + Ny + j] = ys[j] + (ys[j] < 0 ? yShift : 0.f);
+  DH_TRY(h.anaAxes.ensure(axes.size()), "alloc");
+  DH_TRY(hipMemcpyAsync(h.anaAxes.p, axes.data(), axes.size() * sizeof(float), hipMemcpyHostToDevice, h.stream),
+         "axes upload");
+  a.x = h.anaAxes.p;
+  a.y = h.anaAxes.p + Nx;
+  a.yw = h.anaAxes.p + Nx + Ny;
+  if (!inc.empty()) {
+    DH_TRY(h.anaInc.ensure(inc.size()), "alloc");
+    DH_TRY(hipMemcpyAsync(h.anaInc.p, inc.data(), inc.size(), hipMemcpyHostToDevice, h.stream), "mask upload");
+    a.include = h.anaInc.p;
+  }
+  const size_t polStride = (size_t)g.W * g.H;     // pol stride of the reference's summed maps
+  DH_TRY(h.anaPw.ensure((size_t)B * P * N), "alloc");
+  DH_TRY(h.anaSumDev.ensure((size_t)P * polStride), "alloc");
+  DH_TRY(h.anaStats.ensure((size_t)(B + 1) * P * 8), "alloc");
+  DH_TRY(hipMemsetAsync(h.anaSumDev.p, 0, (size_t)P * polStride * sizeof(float), h.stream), "memset");
+  DH_TRY(dh::launch_plane_stats(a, src, nullptr, B * P, planeStride, h.anaPw.p, h.anaStats.p, h.stream), "stats");
+  DH_TRY(dh::launch_batch_sum(h.anaPw.p, B, P, N, h.anaSumDev.p, polStride, h.stream), "batch sum");
+  DH_TRY(dh::launch_plane_stats(a, nullptr, h.anaSumDev.p, P, polStride, nullptr, h.anaStats.p + (size_t)B * P * 8,
+                                h.stream), "stats");
+  std::vector<double> st((size_t)(B + 1) * P * 8);
+  h.anaSum[plane].assign((size_t)P * polStride, 0.f);
+  DH_TRY(hipMemcpyAsync(st.data(), h.anaStats.p, st.size() * sizeof(double), hipMemcpyDeviceToHost, h.stream), "D2H");
+  DH_TRY(hipMemcpyAsync(h.anaSum[plane].data(), h.anaSumDev.p, h.anaSum[plane].size() * sizeof(float),
+                        hipMemcpyDeviceToHost, h.stream), "D2H");
+  DH_TRY(hipStreamSynchronize(h.stream), "sync");
+
+  const float dA = std::fabs(xs[2] - xs[1]) * std::fabs(ys[2] - ys[1]);
+  std::vector<float>& out = h.analysis[plane];
+  out.assign((size_t)7 * P * (B + 1), 0.f);
+  auto at = [&](int par, int p, int b) -> float& { return out[((size_t)par * P + p) * (B + 1) + b]; };
+  for (int p = 0; p < P; ++p)
+    for (int b = 0; b <= B; ++b) {
+      const double* r = &st[((b < B ? (size_t)b * P : (size_t)B * P) + p) * 8];
+      const float tot = (float)r[0];
+      at(0, p, b) = tot;                                        // TOTALPOWER
+      at(1, p, b) = (float)r[1] / tot;                          // COMX
+      at(2, p, b) = (float)r[2] / tot;                          // COMY
+      at(3, p, b) = (float)(32767.0 / std::sqrt((float)r[5]));  // MAXABS (scale to int16 full range)
+      const int idx = (int)r[6];
+      std::memcpy(&at(4, p, b), &idx, sizeof(int));             // MAXABSIDX: int32 bits
+      at(5, p, b) = dA * (tot * tot / (float)r[4]);             // AEFF (Petermann II)
+      at(6, p, b) = (float)r[3] / tot;                          // COMYWRAP
+    }
+  h.anaX[plane] = xs;
+  h.anaY[plane] = ys;
+  h.analysisValid[plane] = true;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+int compute_fields16(Handle& h) {
+  if (h.f16Valid) return DIGHOLO_ERROR_SUCCESS;
+  const int e = compute_analysis(h, 1);
+  if (e) return e;
+  const dh::Geometry& g = h.cur.g;
+  const int B = g.B, P = g.P, hw = g.h * g.w;
+  const size_t n = (size_t)B * P * hw;
+  const float* scale = &h.analysis[1][(size_t)3 * P * (B + 1)];
+  DH_TRY(h.anaScale.ensure((size_t)P * (B + 1)), "alloc");
+  DH_TRY(hipMemcpyAsync(h.anaScale.p, scale, (size_t)P * (B + 1) * sizeof(float), hipMemcpyHostToDevice, h.stream),
+         "scale upload");
+  DH_TRY(h.f16.ensure(2 * n), "alloc");
+  DH_TRY(dh::launch_fields16(h.fields.p, B, P, hw, h.anaScale.p, h.f16.p, h.f16.p + n, h.stream), "fields16");
+  DH_TRY(h.hF16.ensure(2 * n), "pinned alloc");
+  DH_TRY(hipMemcpyAsync(h.hF16.p, h.f16.p, 2 * n * sizeof(short), hipMemcpyDeviceToHost, h.stream), "D2H");
+  DH_TRY(hipStreamSynchronize(h.stream), "sync");
+  h.f16Valid = true;
   return DIGHOLO_ERROR_SUCCESS;
 }
 
@@ -1178,6 +1340,15 @@ void destroy_handle(Handle* h) {
     h->gram.release();
     h->coefsBackup.release();
     h->coefsMix.release();
+    h->spec0.release();
+    h->anaPw.release();
+    h->anaSumDev.release();
+    h->anaAxes.release();
+    h->anaScale.release();
+    h->anaInc.release();
+    h->anaStats.release();
+    h->f16.release();
+    h->hF16.release();
     for (auto& set : h->evPool)
       for (auto& e : set) (void)hipEventDestroy(e);
     if (h->ownStream) (void)hipStreamDestroy(h->ownStream);
@@ -1821,6 +1992,60 @@ EXT_C float* digHoloAutoAlignGetMetrics(int handleIdx, int metricIdx) {
   H_OR(nullptr);
   if (metricIdx < 0 || metricIdx >= DIGHOLO_METRIC_COUNT || h->aaMetrics.empty()) return nullptr;
   return &h->aaMetrics[(size_t)metricIdx * (h->aaLambdaCount + 1)];
+}
+
+EXT_C int digHoloBatchGetSummary(int handleIdx, int planeIdx, int* parameterCount, int* totalCount, int* polCount,
+                                 float** parameters, int* pixelCountX, int* pixelCountY, float** xAxis, float** yAxis,
+                                 float** totalIntensity) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (!parameterCount || !totalCount || !polCount || !parameters || !pixelCountX || !pixelCountY || !xAxis ||
+      !yAxis || !totalIntensity)
+    return DIGHOLO_ERROR_NULLPOINTER;
+  if (!h->haveBatch) return DIGHOLO_ERROR_NULLPOINTER;
+  const dh::Geometry& g = h->cur.g;
+  *totalCount = g.B + 1;
+  *polCount = g.P;
+  *parameterCount = DIGHOLO_ANALYSIS_COUNT;
+  if (planeIdx != 0 && planeIdx != 1) return DIGHOLO_ERROR_INVALIDDIMENSION;
+  const int e = compute_analysis(*h, planeIdx);
+  if (e) return e;
+  *pixelCountX = planeIdx == 0 ? g.W / 2 + 1 : g.w;
+  *pixelCountY = planeIdx == 0 ? g.H : g.h;
+  *xAxis = h->anaX[planeIdx].data();
+  *yAxis = h->anaY[planeIdx].data();
+  *parameters = h->analysis[planeIdx].data();
+  *totalIntensity = h->anaSum[planeIdx].data();
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+EXT_C int digHoloGetFields16(int handleIdx, int* batchCount, int* polCount, short** fieldR, short** fieldI,
+                             float** fieldScale, float** x, float** y, int* width, int* height) {
+  auto zero = [&]() {
+    *batchCount = *polCount = *width = *height = 0;
+    *fieldR = *fieldI = nullptr;
+    *fieldScale = *x = *y = nullptr;
+  };
+  Handle* h = get(handleIdx);
+  if (!h) {
+    zero();
+    return DIGHOLO_ERROR_INVALIDHANDLE;
+  }
+  if (!h->haveBatch || compute_fields16(*h) != DIGHOLO_ERROR_SUCCESS) {
+    zero();
+    return DIGHOLO_ERROR_NULLPOINTER;
+  }
+  const dh::Geometry& g = h->cur.g;
+  const size_t n = (size_t)g.B * g.P * g.h * g.w;
+  *batchCount = g.B;
+  *polCount = g.P;
+  *fieldR = h->hF16.p;
+  *fieldI = h->hF16.p + n;
+  *fieldScale = &h->analysis[1][(size_t)DIGHOLO_ANALYSIS_MAXABS * g.P * (g.B + 1)];
+  *x = h->xAxis.data();
+  *y = h->yAxis.data();
+  *width = g.w;
+  *height = g.h;
+  return DIGHOLO_ERROR_SUCCESS;
 }
 
 EXT_C int digHoloConfigSetFusedWindow(int handleIdx, int enabled) {

@@ -90,6 +90,31 @@ hipError_t launch_metric_rows(const float2* coefs, int N, int M, int Mpol, int r
 hipError_t launch_gram_f64(const float2* coefs, int N, int M, int rowStride, int s, int rowsAreVectors,
                            double2* G, hipStream_t st);
 
+// Plane analysis (dh_analysis.hip).  Statistics of Nx x Ny planes: columns
+// [x0, x1) of rows other than skip0/skip1 are visited; include (Ny x Nx bytes,
+// or null for all) masks the complex input; I = norm |V|^2.
+struct PlaneArgs {
+  int Nx, Ny, inStride;        // plane size, input row stride (elements)
+  int x0, x1, skip0, skip1;    // visited columns, rows left out (-1: none)
+  float norm;
+  const float* x;              // [Nx] axis
+  const float* y;              // [Ny] axis
+  const float* yw;             // [Ny] axis with negative values wrapped
+  const uint8_t* include;
+};
+// out [planes][8] = {sum I, sum x I, sum y I, sum yw I, sum I^2, peak, peak index, 0};
+// pw (optional) [planes][Ny][Nx] receives I (0 where not visited or masked).
+// Exactly one of inC (complex) / inR (intensity) is non-null.
+hipError_t launch_plane_stats(const PlaneArgs& a, const float2* inC, const float* inR, int planes, size_t planeStride,
+                              float* pw, double* out, hipStream_t s);
+// sum[p * polStride + e] = sum_b pw[b][p][e]
+hipError_t launch_batch_sum(const float* pw, int B, int P, int N, float* sum, size_t polStride, hipStream_t s);
+// int16 export: re/im [B][P][hw] = round(scale[p][b] * V), scale [P][B + 1]
+hipError_t launch_fields16(const float2* fields, int B, int P, int hw, const float* scale, short* re, short* im,
+                           hipStream_t s);
+// In-place FFT along y of all g.w columns of [B][P][H][g.w] R2C rows
+hipError_t launch_col_fft_full(const Geometry& g, const FFTPlan& pl, float2* spec, hipStream_t s);
+
 // Separable HG overlap on MFMA: split = 0 plain FP32, 1 split-FP16 (hi+lo).
 hipError_t launch_overlap_mfma(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
                                float2* coefs, int split, hipStream_t s);

@@ -131,9 +131,9 @@ __global__ __launch_bounds__(kThreads) void k_row_fft(Geometry g, FFTPlan pl, co
   float2* out = rowspec + ((size_t)(b * g.P + p) * g.H + y0) * g.w;
   for (int e = threadIdx.x; e < rows * g.w; e += blockDim.x) {
     const int r = e / g.w, kk = e - r * g.w;
-    const int k = c0 + kk;
-    const float2 zk = Z[r * n + k];
-    float2 zc = Z[r * n + (k == 0 ? 0 : n - k)];
+    const int k = c0 + kk;                   // k <= n (= W/2, the Nyquist column)
+    const float2 zk = Z[r * n + (k == n ? 0 : k)];
+    float2 zc = Z[r * n + ((k == 0 || k == n) ? 0 : n - k)];
     zc.y = -zc.y;
     const float2 ev = cscale(cadd(zk, zc), 0.5f);
     const float2 od = mul_mi<false>(cscale(csub(zk, zc), 0.5f));   // (zk - zc)/(2i)
@@ -171,6 +171,29 @@ __global__ __launch_bounds__(kThreads) void k_col_fft_window(Geometry g, FFTPlan
     int row = (j + sy) % n;
     if (row < 0) row += n;
     out[(size_t)j * g.w + c] = cmul(F[c * n + row], fac[(size_t)j * g.w + c]);
+  }
+}
+
+// FFT along y of every column of the R2C rows, in place (the whole Fourier
+// plane, for its analysis).  grid (ceil(w/CB), P, B)
+__global__ __launch_bounds__(kThreads) void k_col_fft_full(Geometry g, FFTPlan pl, float2* __restrict__ spec,
+                                                           int CB) {
+  extern __shared__ float2 smem[];
+  const int n = pl.n;
+  float2* A = smem;
+  float2* Bf = smem + CB * n;
+  const int b = blockIdx.z, p = blockIdx.y, i0 = blockIdx.x * CB;
+  const int cols = min(CB, g.w - i0);
+  float2* base = spec + (size_t)(b * g.P + p) * g.H * g.w + i0;
+  for (int e = threadIdx.x; e < n * cols; e += blockDim.x) {
+    const int y = e / cols, c = e - y * cols;
+    A[c * n + y] = base[(size_t)y * g.w + c];
+  }
+  __syncthreads();
+  const float2* F = lds_fft<false>(A, Bf, cols, pl);
+  for (int e = threadIdx.x; e < n * cols; e += blockDim.x) {
+    const int y = e / cols, c = e - y * cols;
+    base[(size_t)y * g.w + c] = F[c * n + y];
   }
 }
 
@@ -349,6 +372,16 @@ hipError_t launch_col_fft_window(const Geometry& g, const FFTPlan& pl, const flo
   if (e != hipSuccess) return e;
   dim3 grid((g.w + CB - 1) / CB, g.P, g.B);
   k_col_fft_window<<<grid, kThreads, lds, s>>>(g, pl, rowspec, factor, win, CB);
+  return hipGetLastError();
+}
+
+hipError_t launch_col_fft_full(const Geometry& g, const FFTPlan& pl, float2* spec, hipStream_t s) {
+  const int CB = std::min(16, std::min(g.w, batch_per_block(pl.n, 4096)));
+  const size_t lds = 2 * (size_t)CB * pl.n * sizeof(float2);
+  hipError_t e = set_lds(k_col_fft_full, lds);
+  if (e != hipSuccess) return e;
+  dim3 grid((g.w + CB - 1) / CB, g.P, g.B);
+  k_col_fft_full<<<grid, kThreads, lds, s>>>(g, pl, spec, CB);
   return hipGetLastError();
 }
 

@@ -64,6 +64,16 @@ typedef float complex64[2];                       /* digHolo.h:163 */
 #define DIGHOLO_METRIC_SNRMG 8
 #define DIGHOLO_METRIC_COUNT 9
 
+/* Analysis parameters: digHolo.h:1624-1631 (MAXABSIDX holds int32 bits) */
+#define DIGHOLO_ANALYSIS_TOTALPOWER 0
+#define DIGHOLO_ANALYSIS_COMX 1
+#define DIGHOLO_ANALYSIS_COMY 2
+#define DIGHOLO_ANALYSIS_MAXABS 3
+#define DIGHOLO_ANALYSIS_MAXABSIDX 4
+#define DIGHOLO_ANALYSIS_AEFF 5
+#define DIGHOLO_ANALYSIS_COMYWRAP 6
+#define DIGHOLO_ANALYSIS_COUNT 7
+
 /* Benchmark info indices: digHolo.h:1783-1788 */
 #define DIGHOLO_BENCHMARK_FFT 0
 #define DIGHOLO_BENCHMARK_IFFT 1
@@ -181,6 +191,16 @@ EXT_C int digHoloConfigGetBasisType(int handleIdx);                             
 EXT_C complex64* digHoloProcessBatch(int handleIdx, int* batchCount, int* modeCount, int* polCount); /* digHolo.h:1494 */
 EXT_C complex64* digHoloGetFields(int handleIdx, int* batchCount, int* polCount, float** x, float** y, int* width, int* height); /* digHolo.h:1549 */
 EXT_C complex64* digHoloBasisGetCoefs(int handleIdx, int* batchCount, int* modeCount, int* polCount); /* digHolo.h:1603 */
+/* int16 fields R, I [batch][pol][h][w] with per (pol, batch) scale 32767 / max|V| */
+EXT_C int digHoloGetFields16(int handleIdx, int* batchCount, int* polCount, short** fieldR, short** fieldI,
+                             float** fieldScale, float** x, float** y, int* width, int* height); /* digHolo.h:1568 */
+/* planeIdx 0: R2C Fourier plane of the frames, 1: reconstructed fields.  parameters
+   [DIGHOLO_ANALYSIS_COUNT][pol][batch + 1] (last: the batch-summed intensity),
+   totalIntensity [pol][W * H] (pixelCountY x pixelCountX used per pol).  Computed on
+   request from the last batch's device frames and fields. */
+EXT_C int digHoloBatchGetSummary(int handleIdx, int planeIdx, int* parameterCount, int* totalCount, int* polCount,
+                                 float** parameters, int* pixelCountX, int* pixelCountY, float** xAxis,
+                                 float** yAxis, float** totalIntensity);           /* digHolo.h:1659 */
 
 /* ---- threads / console ---- */
 EXT_C int digHoloConfigSetThreadCount(int handleIdx, int threadCount);            /* digHolo.h:1749 */

@@ -17,6 +17,7 @@ What it restates (all citations are digHolo.cpp unless noted):
 * HG indices and FP64 Hermite basis ............... :1245-1484, :10044-10645
 * separable overlap c = sum_y HGy sum_x HGx V ...... :2149-2341, :9348-9467
 * power-meter metrics (IL, MDL, diag, SNR) ........ :10714-11129
+* plane analysis (power, COM, peak, Aeff) ......... :2354-2654, :8990-9188
 
 Numerical contract
 ------------------
@@ -439,6 +440,108 @@ def quantise_int16(V: np.ndarray) -> np.ndarray:
         inv = (1.0 / scale) if scale != 0 else 0.0
         out[b] = (qr + 1j * qi) * inv
     return out
+
+
+# ---------------------------------------------------------------------------
+# Plane analysis (digHolo.cpp:2354-2654, :8990-9188, :6931-6984, :8840-8870):
+# power, centre of mass, peak and effective area of every (frame, pol) plane and
+# of the batch-summed intensity
+# ---------------------------------------------------------------------------
+
+ANALYSIS_TOTALPOWER, ANALYSIS_COMX, ANALYSIS_COMY, ANALYSIS_MAXABS, ANALYSIS_MAXABSIDX, ANALYSIS_AEFF, \
+    ANALYSIS_COMYWRAP = range(7)
+
+
+def plane_analysis(planes: np.ndarray, x: np.ndarray, y: np.ndarray, include=None, skip_rows=(),
+                   col_range=None):
+    """Statistics of ``digHoloFieldAnalysis`` (digHolo.cpp:2354-2654) finished as in
+    ``digHoloFieldAnalysisRoutine`` (:9112-9140).
+
+    planes [B, P, Ny, Nx] complex; x [Nx], y [Ny] float32 axes; include [Ny, Nx]
+    bool window test (None: every pixel); rows in ``skip_rows`` and columns
+    outside ``col_range`` are not visited at all.  Per visited pixel the power is
+    I = |V|^2 / (Nx Ny) (zero outside ``include``).  Returns (params, summed):
+    params [7, P, B+1] float64 (index B is the batch-summed intensity map;
+    MAXABSIDX holds the integer index 2*j*Nx + i of the peak) and summed
+    [P, Ny, Nx], the per-pixel sum of I over the batch.
+    """
+    planes = np.asarray(planes)
+    B, P, Ny, Nx = planes.shape
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    x0, x1 = col_range if col_range is not None else (0, Nx)
+    visited = np.zeros((Ny, Nx), dtype=bool)
+    visited[:, x0:x1] = True
+    for j in skip_rows:
+        visited[j, :] = False
+    inc = visited if include is None else (visited & np.asarray(include, dtype=bool))
+    v2 = np.where(inc, np.abs(planes) ** 2, 0.0)                  # |V|^2, masked
+    pw = v2 / (Nx * Ny)
+    y_shift = abs(float((F32(y[1]) - F32(y[0])) * F32(Ny)))
+    ywrap = y + y_shift * (y < 0)
+    dA = float(F32(abs(F32(x[2]) - F32(x[1]))) * F32(abs(F32(y[2]) - F32(y[1]))))
+    flat_idx = (2 * np.arange(Ny)[:, None] * Nx + np.arange(Nx)[None, :])
+    out = np.zeros((7, P, B + 1))
+    summed = pw.sum(axis=0)
+
+    def stats(I, peak_vals):
+        tot = I.sum()
+        e4 = (I * I).sum()
+        pv = np.where(visited, peak_vals, -1.0)
+        k = np.flatnonzero(pv.ravel() == pv.max())[-1]        # ties: the later pixel
+        return (tot, (I * x[None, :]).sum() / tot, (I * y[:, None]).sum() / tot,
+                32767.0 / math.sqrt(pv.ravel()[k]), flat_idx.ravel()[k], dA * tot * tot / e4,
+                (I * ywrap[:, None]).sum() / tot)
+
+    for p in range(P):
+        for b in range(B):
+            out[:, p, b] = stats(pw[b, p], v2[b, p])
+        out[:, p, B] = stats(summed[p], summed[p])
+    return out, summed
+
+
+def fourier_plane_analysis(frames: np.ndarray, cfg: Config):
+    """Analysis of the R2C Fourier plane of every window (digHolo.cpp:6931-6984):
+    the zero-order region k < 2 k0 sin(r) is excluded, as are the kx = 0 column,
+    the ky = 0 and Nyquist rows, and the columns past the last multiple of 8."""
+    cfg = apply_pol_locks(cfg)
+    plan = make_plan(cfg)
+    W, H = plan.W, plan.H
+    frames = np.asarray(frames)
+    planes = np.stack([np.fft.rfft2(frames[:, plan.cy[p]:plan.cy[p] + H, plan.cx[p]:plan.cx[p] + W]
+                                    .astype(np.float64), axes=(-2, -1)) for p in range(cfg.pol_count)], axis=1)
+    Nx = W // 2 + 1
+    kx = k_axis_f32(W, cfg.pixel_size)[:Nx]
+    ky = k_axis_f32(H, cfg.pixel_size)
+    k0 = F32(F32(2.0) * PI_F32 / F32(cfg.wavelength))
+    r0 = F32(F32(cfg.aperture_deg) * UNIT_ANGLE_F32)
+    R = F32(F32(F32(2.0) * k0) * F32(math.sin(float(r0))))
+    inc = (kx[None, :] * kx[None, :] + ky[:, None] * ky[:, None]).astype(np.float32) >= F32(R * R)
+    return plane_analysis(planes, kx, ky, include=inc, skip_rows=(0, H // 2), col_range=(1, 8 * (Nx // 8)))
+
+
+def field_axes(cfg: Config, plan: Plan):
+    """Reconstructed-plane axes (float32): x_i = (i - w/2) * (px W / w), digHolo.cpp:8184-8189."""
+    px = F32(cfg.pixel_size)
+    pxx = F32(F32(px * F32(plan.W)) / F32(plan.w))
+    pxy = F32(F32(px * F32(plan.H)) / F32(plan.h))
+    x = (np.arange(plan.w) - plan.w // 2).astype(np.float32) * pxx
+    y = (np.arange(plan.h) - plan.h // 2).astype(np.float32) * pxy
+    return x.astype(np.float32), y.astype(np.float32)
+
+
+def field_plane_analysis(fields: np.ndarray, cfg: Config, plan: Plan):
+    """Analysis of the reconstructed fields (digHolo.cpp:8840-8870): every pixel."""
+    x, y = field_axes(cfg, plan)
+    return plane_analysis(fields, x, y)
+
+
+def fields_int16(fields: np.ndarray, maxabs: np.ndarray):
+    """int16 export of ``digHoloGetFields16`` (digHolo.cpp:4880-4906, :2820-2832):
+    round(V * MAXABS) per (frame, pol), MAXABS = 32767 / max|V|."""
+    s = np.asarray(maxabs, dtype=np.float32)[..., None, None]
+    q = lambda v: np.clip(np.rint((s * v.astype(np.float32)).astype(np.float32)), -32768, 32767).astype(np.int16)
+    return q(fields.real), q(fields.imag)
 
 
 # ---------------------------------------------------------------------------

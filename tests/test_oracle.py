@@ -262,3 +262,92 @@ def test_power_meter_independent_pols_and_single_frame():
     one = O.power_meter(c[:1])
     assert np.isclose(one[O.METRIC_IL], (np.abs(c[0]) ** 2).sum())
     assert np.isclose(one[O.METRIC_DIAG], (np.abs(c[0, :, 0]) ** 2).sum())
+
+
+# ---------------------------------------------------------------------------
+# plane analysis (power, centre of mass, peak, effective area)
+# ---------------------------------------------------------------------------
+def _axes(Nx, Ny, d=1.5):
+    return ((np.arange(Nx) - Nx // 2) * d).astype(np.float32), ((np.arange(Ny) - Ny // 2) * d).astype(np.float32)
+
+
+def test_analysis_single_pixel_known_answers():
+    Nx, Ny = 16, 12
+    x, y = _axes(Nx, Ny)
+    planes = np.zeros((1, 1, Ny, Nx), complex)
+    a = 3.0 - 4.0j                                        # |a| = 5
+    planes[0, 0, 2, 11] = a
+    out, summed = O.plane_analysis(planes, x, y)
+    for b in (0, 1):                                      # the frame and the batch sum agree
+        assert out[O.ANALYSIS_TOTALPOWER, 0, b] == pytest.approx(25.0 / (Nx * Ny))
+        assert out[O.ANALYSIS_COMX, 0, b] == pytest.approx(x[11])
+        assert out[O.ANALYSIS_COMY, 0, b] == pytest.approx(y[2])
+        assert out[O.ANALYSIS_COMYWRAP, 0, b] == pytest.approx(y[2] + 1.5 * Ny)   # y < 0 wraps by Ny*dy
+        assert out[O.ANALYSIS_AEFF, 0, b] == pytest.approx(1.5 * 1.5)            # one pixel of area dA
+        assert out[O.ANALYSIS_MAXABSIDX, 0, b] == 2 * 2 * Nx + 11
+    assert out[O.ANALYSIS_MAXABS, 0, 0] == pytest.approx(32767.0 / 5.0)          # peak of |V|
+    assert out[O.ANALYSIS_MAXABS, 0, 1] == pytest.approx(32767.0 / math.sqrt(25.0 / (Nx * Ny)))
+    assert summed[0, 2, 11] == pytest.approx(25.0 / (Nx * Ny))
+
+
+def test_analysis_gaussian_effective_area_and_centre():
+    # |E|^2 = exp(-2 r^2 / w^2): Petermann II area (int I)^2 / int I^2 = pi w^2
+    Nx = Ny = 128
+    d = 0.25
+    x, y = _axes(Nx, Ny, d)
+    w, x0, y0 = 4.0, 1.75, -2.5
+    xd, yd = x.astype(np.float64), y.astype(np.float64)
+    E = np.exp(-((xd[None, :] - x0) ** 2 + (yd[:, None] - y0) ** 2) / w ** 2)
+    out, _ = O.plane_analysis(E[None, None], x, y)
+    assert out[O.ANALYSIS_AEFF, 0, 0] == pytest.approx(math.pi * w * w, rel=1e-6)
+    assert out[O.ANALYSIS_COMX, 0, 0] == pytest.approx(x0, abs=1e-9)
+    assert out[O.ANALYSIS_COMY, 0, 0] == pytest.approx(y0, abs=1e-9)
+    assert out[O.ANALYSIS_TOTALPOWER, 0, 0] == pytest.approx(math.pi * w * w / 2 / d / d / (Nx * Ny), rel=1e-6)
+
+
+def test_analysis_batch_sum_is_power_weighted():
+    Nx, Ny = 8, 8
+    x, y = _axes(Nx, Ny)
+    planes = np.zeros((2, 1, Ny, Nx), complex)
+    planes[0, 0, 1, 1] = 1.0
+    planes[1, 0, 6, 5] = 2.0                              # 4x the power
+    out, summed = O.plane_analysis(planes, x, y)
+    assert out[O.ANALYSIS_COMX, 0, 2] == pytest.approx((x[1] + 4 * x[5]) / 5)
+    assert out[O.ANALYSIS_COMY, 0, 2] == pytest.approx((y[1] + 4 * y[6]) / 5)
+    assert out[O.ANALYSIS_MAXABSIDX, 0, 2] == 2 * 6 * Nx + 5
+    assert out[O.ANALYSIS_TOTALPOWER, 0, 2] == pytest.approx(out[O.ANALYSIS_TOTALPOWER, 0, :2].sum())
+    assert out[O.ANALYSIS_AEFF, 0, 2] == pytest.approx(1.5 * 1.5 * 25 / 17)      # (1+4)^2 / (1+16)
+    assert np.allclose(summed[0], (np.abs(planes[:, 0]) ** 2).sum(0) / (Nx * Ny))
+
+
+def test_fourier_plane_analysis_exclusions():
+    # a DC offset, a pure-x and pure-y fringe and a Nyquist-row fringe all land on
+    # excluded lines; an off-axis fringe outside 2 k0 sin(r) is the only power seen
+    cfg = O.Config(frame_width=64, frame_height=64, pol_count=1, aperture_deg=0.3)
+    W = H = 64
+    yy, xx = np.mgrid[0:H, 0:W]
+    base = 100.0 + 7 * np.cos(2 * np.pi * 3 * xx / W) + 5 * np.cos(2 * np.pi * 4 * yy / H) + 2 * (-1.0) ** yy
+    out, summed = O.fourier_plane_analysis(base[None], cfg)
+    assert out[O.ANALYSIS_TOTALPOWER, 0, 0] == pytest.approx(0.0, abs=1e-9)
+    fr = base + 3 * np.cos(2 * np.pi * (12 * xx / W + 10 * yy / H))
+    out, summed = O.fourier_plane_analysis(fr[None], cfg)
+    Nx = W // 2 + 1
+    # the off-axis term lands at (kx, ky) = (12, 10): power (3/2 * W * H)^2 / (Nx * H)
+    assert out[O.ANALYSIS_TOTALPOWER, 0, 0] == pytest.approx((1.5 * W * H) ** 2 / (Nx * H), rel=1e-9)
+    assert out[O.ANALYSIS_MAXABSIDX, 0, 0] == 2 * 10 * Nx + 12
+    kx = O.k_axis_f32(W, cfg.pixel_size)
+    assert out[O.ANALYSIS_COMX, 0, 0] == pytest.approx(kx[12], rel=1e-6)
+    assert out[O.ANALYSIS_COMY, 0, 0] == pytest.approx(kx[10], rel=1e-6)
+
+
+def test_fields_int16_scale_uses_full_range():
+    rng = np.random.default_rng(5)
+    V = rng.standard_normal((3, 2, 8, 8)) + 1j * rng.standard_normal((3, 2, 8, 8))
+    cfg = O.Config()
+    plan = O.Plan(W=64, H=64, cx=[0, 0], cy=[0, 0], w=8, h=8, SX=[0, 0], SY=[0, 0])
+    out, _ = O.field_plane_analysis(V, cfg, plan)
+    s = out[O.ANALYSIS_MAXABS, :, :3].T                   # [B, P]
+    R, I = O.fields_int16(V, s)
+    assert np.max(np.maximum(np.abs(R), np.abs(I))) <= 32767
+    back = (R + 1j * I) / s[..., None, None]
+    assert np.max(np.abs(back - V)) <= 0.51 / s.min() * math.sqrt(2)
