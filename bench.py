@@ -153,7 +153,7 @@ def main():
 
     Wd, Hd, w, hh, M = geo["W"], geo["H"], geo["w"], geo["h"], m.value
     frames_s = B / (ms_per_step * 1e-3)
-    fused = lib.digHoloGetWindowPath(h) == 1
+    path = lib.digHoloGetWindowPath(h)
     fft_ms = st[0] + st[1] + st[2]          # camera window -> fields (one kernel when fused)
     fft_bytes = fft_stage_bytes_per_frame(POLS, Wd, Hd, w, hh) * B
     achieved = fft_bytes / (fft_ms * 1e-3) / 1e9
@@ -176,7 +176,7 @@ def main():
                    "global_batch": B, "overlap_precision": "split16" if headline else "fp32",
                    "fft_window": Wd, "ifft_window": w, "modes_per_pol": M, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm",
-                     "kernel": "k_fused_window" if fused else
+                     "kernel": {2: "k_window_mfma", 1: "k_fused_window"}[path] if path in (1, 2) else
                      "k_row_fft + k_col_fft_window + k_ifft_rows + k_ifft_cols_tilt",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

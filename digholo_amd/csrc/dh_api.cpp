@@ -170,8 +170,11 @@ struct Handle {
 
   // Stage timing: one set of 6 events per batch enqueued since the last
   // collection, so back-to-back batches need no host synchronisation.
-  bool allowFused = true;   // false forces the staged kernels
-  bool lastFused = false;   // which window path the last batch took
+  // window path: 0 staged kernels, 1 fused (FFT columns), 2 fused with the
+  // column transform on MFMA (uint16 frames); each falls back to the next lower
+  int windowMode = 2;
+  int lastPath = -1;        // path the last batch took
+  DevBuf<uint16_t> colTab;  // fp16 hi/lo column-DFT operands of path 2
 
   bool stageTiming = false;
   std::vector<std::array<hipEvent_t, 6>> evPool;
@@ -469,6 +472,20 @@ bool plan_fused(Handle& h, const dh::Geometry& g, dh::FusedArgs& fa) {
   return false;
 }
 
+bool plan_window_mfma(Handle& h, const dh::Geometry& g, dh::FusedArgs& fa) {
+  const dh::FFTPlan* row = fft_plan(h, g.W / 2);
+  const dh::FFTPlan* ix = fft_plan(h, g.w);
+  const dh::FFTPlan* iy = fft_plan(h, g.h);
+  if (!row || !ix || !iy) return false;
+  fa = dh::FusedArgs{};
+  fa.g = g;
+  fa.row = *row;
+  fa.col = *row;
+  fa.ix = *ix;
+  fa.iy = *iy;
+  return dh::window_mfma_prepare(fa);
+}
+
 bool is_device_pointer(const void* ptr) {
   hipPointerAttribute_t a;
   if (hipPointerGetAttributes(&a, ptr) != hipSuccess) {
@@ -600,17 +617,26 @@ int enqueue_batch(Handle& h) {
     DH_TRY(h.hgy.ensure(std::max<size_t>(P * g.G * g.h, 1)), "alloc");
     DH_TRY(hipMemcpyAsync(h.mask.p, d.mask.data(), T, hipMemcpyHostToDevice, h.stream), "mask upload");
     DH_TRY(dh::launch_tables(d.tab, h.mask.p, h.factor.p, h.ref.p, h.hgx.p, h.hgy.p, h.stream), "tables");
+    DH_TRY(h.colTab.ensure(dh::coldft_table_elems(g)), "alloc");
+    DH_TRY(dh::launch_coldft_table(g, h.colTab.p, h.stream), "column table");
     DH_TRY(hipStreamSynchronize(h.stream), "tables sync");   // the host mask vector goes out of scope
     h.tabKey = d;
     h.tablesValid = true;
   }
   if (timing) DH_TRY(hipEventRecord(ev[1], h.stream), "event");
   dh::FusedArgs fa;
-  if (h.allowFused && plan_fused(h, g, fa)) {
+  int path = 0;
+  if (h.windowMode >= 2 && h.frameU16 && plan_window_mfma(h, g, fa)) path = 2;
+  else if (h.windowMode >= 1 && plan_fused(h, g, fa)) path = 1;
+  if (path) {
     // one kernel from camera window to field
-    DH_TRY(dh::launch_fused_window(fa, h.twW.p, pCol->tw, dframes, h.frameU16, h.factor.p, h.ref.p, h.fields.p,
-                                   h.stream), "fused window");
-    h.lastFused = true;
+    if (path == 2)
+      DH_TRY(dh::launch_window_mfma(fa, h.twW.p, dframes, h.colTab.p, h.factor.p, h.ref.p, h.fields.p, h.stream),
+             "fused window (MFMA columns)");
+    else
+      DH_TRY(dh::launch_fused_window(fa, h.twW.p, pCol->tw, dframes, h.frameU16, h.factor.p, h.ref.p, h.fields.p,
+                                     h.stream), "fused window");
+    h.lastPath = path;
     if (timing) {
       DH_TRY(hipEventRecord(ev[2], h.stream), "event");
       DH_TRY(hipEventRecord(ev[3], h.stream), "event");
@@ -627,7 +653,7 @@ int enqueue_batch(Handle& h) {
     DH_TRY(dh::launch_ifft_rows(g, *pIx, h.win.p, h.stream), "IFFT rows");
     DH_TRY(dh::launch_ifft_cols_tilt(g, *pIy, h.win.p, h.ref.p, h.fields.p, h.stream), "IFFT columns");
     if (timing) DH_TRY(hipEventRecord(ev[4], h.stream), "event");
-    h.lastFused = false;
+    h.lastPath = 0;
   }
   if (g.G > 0) {
     DH_TRY(dh::launch_overlap_mfma(g, h.fields.p, h.hgx.p, h.hgy.p, h.coefs.p,
@@ -1333,6 +1359,7 @@ void destroy_handle(Handle* h) {
     h->hgx.release();
     h->hgy.release();
     h->mask.release();
+    h->colTab.release();
     for (auto& kv : h->ffts) kv.second->tw.release();
     h->hCoefs.release();
     h->hFields.release();
@@ -2050,12 +2077,13 @@ EXT_C int digHoloGetFields16(int handleIdx, int* batchCount, int* polCount, shor
 
 EXT_C int digHoloConfigSetFusedWindow(int handleIdx, int enabled) {
   H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
-  h->allowFused = enabled != 0;
+  if (enabled < 0 || enabled > 2) return DIGHOLO_ERROR_INVALIDARGUMENT;
+  h->windowMode = enabled;
   return DIGHOLO_ERROR_SUCCESS;
 }
 EXT_C int digHoloGetWindowPath(int handleIdx) {
   H_OR(-1);
-  return h->haveBatch ? (h->lastFused ? 1 : 0) : -1;
+  return h->haveBatch ? h->lastPath : -1;
 }
 
 EXT_C int digHoloConfigSetStageTiming(int handleIdx, int enabled) {

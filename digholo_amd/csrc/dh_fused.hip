@@ -305,6 +305,271 @@ static hipError_t fused_launch(const FusedArgs& a, const FusedMagic& mg, const f
   return hipGetLastError();
 }
 
+// ===========================================================================
+// MFMA column variant (uint16 frames).  Rows are transformed as above, RB rows
+// at a time in natural order, and the column transform restricted to the h
+// window rows is a complex GEMM on the matrix cores:
+//     F[j][c] = sum_y D[j][y] X[y][c],   D[j][y] = exp(-2 pi i ky(j) y / H),
+// X the band of row spectra, ky(j) the spectrum row of window row j.  Real form
+// over k = 2y + {re, im} (K = 2 RB per row batch):
+//     Fr = Ar . B,  Fi = Ai . B,  B[2y][c] = Xr, B[2y+1][c] = Xi,
+//     Ar[j][2y] = Dr, Ar[j][2y+1] = -Di,  Ai[j][2y] = Di, Ai[j][2y+1] = Dr.
+// Every operand is split x = hi + lo (fp16) and hi*hi + hi*lo + lo*hi run on
+// v_mfma_f32_16x16x32_f16 with FP32 accumulation.  X is scaled by a fixed power
+// of two chosen from the bound |X| <= W * 65535 of uint16 frames, which keeps
+// every value inside the fp16 range; D needs no scale.  A (= D, per pol, built
+// once with the tables) is read from global memory straight into registers;
+// B goes through LDS.  Combos (j tile, re/im part) are dealt to the waves; a
+// wave keeps its combos' accumulators for every column tile.
+// ===========================================================================
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__host__ __device__ inline int mfma_bpitch(int RB) { return 2 * RB + 8; }   // halfs per B row (one column c)
+
+struct MfmaLayout {
+  int region;    // bytes of (row buffer + B planes) / field plane
+  size_t bytes;
+};
+
+__host__ __device__ inline MfmaLayout mfma_layout(const FusedArgs& a) {
+  MfmaLayout L;
+  const int fwd = a.RB * a.row.n * 8 + 2 * a.g.w * mfma_bpitch(a.RB) * 2;
+  const int inv = a.g.h * a.g.w * 8;
+  L.region = ((fwd > inv ? fwd : inv) + 15) & ~15;
+  L.bytes = (size_t)L.region + sizeof(float2) * (size_t)(a.row.n + a.g.w + a.g.w + a.g.h);
+  return L;
+}
+
+template <int CPW, int NU>
+__global__ __launch_bounds__(kFThreads, 4) void k_window_mfma(FusedArgs a, FusedMagic mg, float xscale,
+                                                              const float2* __restrict__ twW,
+                                                              const unsigned short* __restrict__ frames,
+                                                              const _Float16* __restrict__ dtab,
+                                                              const float2* __restrict__ factor,
+                                                              const float2* __restrict__ ref,
+                                                              float2* __restrict__ fields) {
+  extern __shared__ __align__(16) unsigned char wm_smem[];
+  const Geometry& g = a.g;
+  const int p = blockIdx.x, b = blockIdx.y;
+  const int n = a.row.n, RB = a.RB, H = g.H, w = g.w, h = g.h, hw = h * w;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, q = lane >> 4;
+  const MfmaLayout L = mfma_layout(a);
+  float2* rowbuf = reinterpret_cast<float2*>(wm_smem);                                 // [RB][n]
+  const int pk = mfma_bpitch(RB);
+  _Float16* Bl = reinterpret_cast<_Float16*>(wm_smem + (size_t)RB * n * sizeof(float2));   // [2][w][pk]
+  float2* fbuf = reinterpret_cast<float2*>(wm_smem);                                   // [h][w] at the end
+  float2* twn = reinterpret_cast<float2*>(wm_smem + L.region);                         // n
+  float2* twr = twn + n;                                                               // w
+  float2* twix = twr + w;                                                              // w
+  float2* twiy = twix + w;                                                             // h
+  const int c0 = g.c0[p];
+  const size_t fbase = (size_t)b * g.frameH * g.frameW + (size_t)g.cy[p] * g.frameW + g.cx[p];
+  const int K = 2 * H;
+  const int nCombo = 2 * (h / 16);
+  const _Float16* tabp = dtab + (size_t)p * 4 * h * K;
+
+  float2 pre[kPre];
+  fetch_rows(pre, frames, 1, fbase, g.frameW, 1, 0, 0, RB, n, mg.n);
+  for (int e = tid; e < n; e += kFThreads) twn[e] = a.row.tw[e];
+  for (int e = tid; e < w; e += kFThreads) {
+    twr[e] = twW[c0 + e];
+    twix[e] = a.ix.tw[e];
+  }
+  for (int e = tid; e < h; e += kFThreads) twiy[e] = a.iy.tw[e];
+
+  floatx4 acc[CPW][NU];
+#pragma unroll
+  for (int ci = 0; ci < CPW; ++ci)
+#pragma unroll
+    for (int u = 0; u < NU; ++u) acc[ci][u] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  const int nch = H / RB;
+  for (int ch = 0; ch < nch; ++ch) {
+#pragma unroll
+    for (int s = 0; s < kPre; ++s) {
+      const int e = tid + s * kFThreads;
+      if (e < RB * n) rowbuf[e] = pre[s];
+    }
+    if (ch + 1 < nch) fetch_rows(pre, frames, 1, fbase, g.frameW, 1, 0, (ch + 1) * RB, RB, n, mg.n);
+    __syncthreads();
+    lds_fft_ip<false>(rowbuf, RB, n, 1, false, a.row, twn, RB);
+    // unpack the window columns of the packed real transform, scale, split into B
+    for (int e = tid; e < RB * w; e += kFThreads) {
+      const int tt = fdiv(e, mg.w), i = e - tt * w;
+      const int k = c0 + i;
+      const float2 zk = rowbuf[tt * n + k];
+      float2 zc = rowbuf[tt * n + (k == 0 ? 0 : n - k)];
+      zc.y = -zc.y;
+      const float2 ev = cscale(cadd(zk, zc), 0.5f);
+      const float2 od = mul_mi<false>(cscale(csub(zk, zc), 0.5f));
+      const float2 X = cscale(cadd(ev, cmul(twr[i], od)), xscale);
+      const _Float16 rh = (_Float16)X.x, ih = (_Float16)X.y;
+      const _Float16 rl = (_Float16)(X.x - (float)rh), il = (_Float16)(X.y - (float)ih);
+      _Float16* bp = Bl + i * pk + 2 * tt;
+      bp[0] = rh;
+      bp[1] = ih;
+      bp[w * pk] = rl;
+      bp[w * pk + 1] = il;
+    }
+    __syncthreads();
+    // F += A[:, chunk] . B_chunk for this wave's combos
+    const int kb = ch * 2 * RB;
+    for (int s = 0; s < 2 * RB; s += 32) {
+      half8 ah[CPW], al[CPW];
+#pragma unroll
+      for (int ci = 0; ci < CPW; ++ci) {
+        const int cb = wave + 8 * ci;
+        if (cb < nCombo) {
+          const int t = cb >> 1, part = cb & 1;
+          const _Float16* ap = tabp + ((size_t)(part * 2) * h + 16 * t + lr) * K + kb + s + 8 * q;
+          ah[ci] = *reinterpret_cast<const half8*>(ap);
+          al[ci] = *reinterpret_cast<const half8*>(ap + (size_t)h * K);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        const _Float16* bq = Bl + (16 * u + lr) * pk + s + 8 * q;
+        const half8 bh = *reinterpret_cast<const half8*>(bq);
+        const half8 bl = *reinterpret_cast<const half8*>(bq + w * pk);
+#pragma unroll
+        for (int ci = 0; ci < CPW; ++ci) {
+          if (wave + 8 * ci < nCombo) {
+            acc[ci][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[ci], bh, acc[ci][u], 0, 0, 0);
+            acc[ci][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[ci], bl, acc[ci][u], 0, 0, 0);
+            acc[ci][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[ci], bh, acc[ci][u], 0, 0, 0);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();   // B and the row buffer are free: the field plane takes their place
+  const float inv = 1.f / xscale;
+  float* fl = reinterpret_cast<float*>(fbuf);
+#pragma unroll
+  for (int ci = 0; ci < CPW; ++ci) {
+    const int cb = wave + 8 * ci;
+    if (cb < nCombo) {
+      const int t = cb >> 1, part = cb & 1;
+#pragma unroll
+      for (int u = 0; u < NU; ++u)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) fl[2 * ((16 * t + 4 * q + i) * w + 16 * u + lr) + part] = acc[ci][u][i] * inv;
+    }
+  }
+  __syncthreads();
+  const float2* fac = factor + (size_t)p * hw;
+  for (int e = tid; e < hw; e += kFThreads) fbuf[e] = cmul(fbuf[e], fac[e]);
+  __syncthreads();
+  lds_fft_ip<true>(fbuf, h, w, 1, false, a.ix, twix, a.chunkIx);   // along x
+  lds_fft_ip<true>(fbuf, w, 1, w, true, a.iy, twiy, a.chunkIy);    // along y
+  const float2* rp = ref + (size_t)p * hw;
+  float2* out = fields + (size_t)(b * g.P + p) * hw;
+  for (int e = tid; e < hw; e += kFThreads) {
+    const float2 t = cmul(fbuf[e], rp[e]);
+    out[e] = make_float2(-t.x, t.y);
+  }
+}
+
+// D as the A operands: tab[p][part][hi/lo][j][k], k = 2y + {0, 1}, FP64 angles
+__global__ void k_coldft_table(Geometry g, _Float16* __restrict__ tab) {
+  const size_t per = (size_t)g.h * g.H;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)g.P * per) return;
+  const int p = (int)(e / per);
+  const int rem = (int)(e - (size_t)p * per);
+  const int j = rem / g.H, y = rem - j * g.H;
+  int ky = (j + g.SY[p] + g.H / 2) % g.H;
+  if (ky < 0) ky += g.H;
+  const long long m = ((long long)ky * y) % g.H;
+  const double ang = -2.0 * 3.14159265358979323846 * (double)m / g.H;
+  double sn, cs;
+  sincos(ang, &sn, &cs);
+  const int K = 2 * g.H;
+  auto put = [&](int part, int k, double v) {
+    const _Float16 hi = (_Float16)v;
+    const _Float16 lo = (_Float16)(v - (double)hi);
+    _Float16* base = tab + ((size_t)(p * 2 + part) * 2 * g.h) * K;
+    base[(size_t)j * K + k] = hi;
+    base[(size_t)(g.h + j) * K + k] = lo;
+  };
+  put(0, 2 * y, cs);
+  put(0, 2 * y + 1, -sn);
+  put(1, 2 * y, sn);
+  put(1, 2 * y + 1, cs);
+}
+
+size_t coldft_table_elems(const Geometry& g) { return (size_t)g.P * 4 * g.h * 2 * g.H; }
+
+hipError_t launch_coldft_table(const Geometry& g, void* tab, hipStream_t s) {
+  const size_t total = (size_t)g.P * g.h * g.H;
+  if (total == 0) return hipErrorInvalidValue;
+  k_coldft_table<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(g, static_cast<_Float16*>(tab));
+  return hipGetLastError();
+}
+
+static int mfma_cpw(const Geometry& g) { return (2 * (g.h / 16) + 7) / 8; }
+
+bool window_mfma_prepare(FusedArgs& a) {
+  const Geometry& g = a.g;
+  if (g.w % 16 || g.h % 16 || g.w < 16 || g.h < 16) return false;
+  const int nu = g.w / 16, cpw = mfma_cpw(g);
+  const bool inst = (cpw == 1 && (nu == 3 || nu == 4 || nu == 5 || nu == 6 || nu == 8)) ||
+                    (cpw == 2 && (nu == 5 || nu == 6 || nu == 8));
+  if (!inst) return false;
+  const int cr = ip_chunk(a.row), cx = ip_chunk(a.ix), cy = ip_chunk(a.iy);
+  if (cr < 1 || cx < 1 || cy < 1) return false;
+  int RB = std::min(32, std::min(cr, kPre * kFThreads / a.row.n));
+  RB -= RB % 16;
+  if (RB < 16) return false;
+  if (g.H % RB) RB = 16;
+  if (g.H % RB) return false;
+  a.R = 1;
+  a.Hc = g.H;
+  a.RB = RB;
+  a.chunkCol = 0;
+  a.chunkIx = cx;
+  a.chunkIy = cy;
+  return mfma_layout(a).bytes <= kFusedLdsMax;
+}
+
+template <int CPW, int NU>
+static hipError_t wm_launch(const FusedArgs& a, const FusedMagic& mg, float xscale, const float2* twW,
+                            const void* frames, const void* dtab, const float2* factor, const float2* ref,
+                            float2* fields, hipStream_t s) {
+  const size_t lds = mfma_layout(a).bytes;
+  if (lds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_window_mfma<CPW, NU>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  dim3 grid(a.g.P, a.g.B);
+  k_window_mfma<CPW, NU><<<grid, kFThreads, lds, s>>>(a, mg, xscale, twW,
+                                                       static_cast<const unsigned short*>(frames),
+                                                       static_cast<const _Float16*>(dtab), factor, ref, fields);
+  return hipGetLastError();
+}
+
+hipError_t launch_window_mfma(const FusedArgs& a0, const float2* twW, const void* frames_u16, const void* dtab,
+                              const float2* factor, const float2* ref, float2* fields, hipStream_t s) {
+  FusedArgs a = a0;
+  if (!window_mfma_prepare(a)) return hipErrorNotSupported;
+  FusedMagic mg{magic_div(a.row.n), magic_div(a.g.w), 0u, 0u};
+  // |X| <= W * 65535 < 2^(16 + log2 W): scale into [2^14, 2^15) at most
+  int lw = 0;
+  while ((1 << lw) < a.g.W) ++lw;
+  const float xscale = ldexpf(1.f, 15 - (16 + lw));
+  const int nu = a.g.w / 16, cpw = mfma_cpw(a.g);
+#define WM_CASE(C, N) \
+  if (cpw == C && nu == N) return wm_launch<C, N>(a, mg, xscale, twW, frames_u16, dtab, factor, ref, fields, s);
+  WM_CASE(1, 3) WM_CASE(1, 4) WM_CASE(1, 5) WM_CASE(1, 6) WM_CASE(1, 8)
+  WM_CASE(2, 5) WM_CASE(2, 6) WM_CASE(2, 8)
+#undef WM_CASE
+  return hipErrorNotSupported;
+}
+
 hipError_t launch_fused_window(const FusedArgs& a0, const float2* twW, const float2* twH, const void* frames,
                                int frame_is_u16, const float2* factor, const float2* ref, float2* fields,
                                hipStream_t s) {

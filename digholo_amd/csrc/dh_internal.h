@@ -82,6 +82,15 @@ hipError_t launch_fused_window(const FusedArgs& a, const float2* twW, const floa
                                int frame_is_u16, const float2* factor, const float2* ref, float2* fields,
                                hipStream_t s);
 
+// Fused window path with the column transform as a split-fp16 GEMM on MFMA
+// (uint16 frames).  dtab: the column-DFT operands built by launch_coldft_table
+// (coldft_table_elems(g) halfs) for the same geometry.
+bool window_mfma_prepare(FusedArgs& a);
+size_t coldft_table_elems(const Geometry& g);
+hipError_t launch_coldft_table(const Geometry& g, void* tab, hipStream_t s);
+hipError_t launch_window_mfma(const FusedArgs& a, const float2* twW, const void* frames_u16, const void* dtab,
+                              const float2* factor, const float2* ref, float2* fields, hipStream_t s);
+
 // Power-meter reductions (dh_metrics.hip).  out: [S][N][5] doubles per row
 // {diagonal, off-diagonal, in-group/g, out-of-group/g, total power}.
 hipError_t launch_metric_rows(const float2* coefs, int N, int M, int Mpol, int rowStride, int S, double* out,
