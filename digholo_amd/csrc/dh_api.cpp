@@ -171,10 +171,12 @@ struct Handle {
   // Stage timing: one set of 6 events per batch enqueued since the last
   // collection, so back-to-back batches need no host synchronisation.
   // window path: 0 staged kernels, 1 fused (FFT columns), 2 fused with the
-  // column transform on MFMA (uint16 frames); each falls back to the next lower
-  int windowMode = 2;
+  // column transform on MFMA (uint16 frames), 3 size-specialised fused kernel
+  // (register row transforms, uint16 frames); each falls back to the next lower
+  int windowMode = 3;
   int lastPath = -1;        // path the last batch took
   DevBuf<uint16_t> colTab;  // fp16 hi/lo column-DFT operands of path 2
+  DevBuf<float2> w3Tab;     // twiddle tables of path 3
 
   bool stageTiming = false;
   std::vector<std::array<hipEvent_t, 6>> evPool;
@@ -486,6 +488,17 @@ bool plan_window_mfma(Handle& h, const dh::Geometry& g, dh::FusedArgs& fa) {
   return dh::window_mfma_prepare(fa);
 }
 
+bool plan_window_v3(Handle& h, const dh::Geometry& g, dh::FusedArgs& fa) {
+  const dh::FFTPlan* ix = fft_plan(h, g.w);
+  const dh::FFTPlan* iy = fft_plan(h, g.h);
+  if (!ix || !iy) return false;
+  fa = dh::FusedArgs{};
+  fa.g = g;
+  fa.ix = *ix;
+  fa.iy = *iy;
+  return dh::window_v3_supported(fa);
+}
+
 bool is_device_pointer(const void* ptr) {
   hipPointerAttribute_t a;
   if (hipPointerGetAttributes(&a, ptr) != hipSuccess) {
@@ -619,6 +632,10 @@ int enqueue_batch(Handle& h) {
     DH_TRY(dh::launch_tables(d.tab, h.mask.p, h.factor.p, h.ref.p, h.hgx.p, h.hgy.p, h.stream), "tables");
     DH_TRY(h.colTab.ensure(dh::coldft_table_elems(g)), "alloc");
     DH_TRY(dh::launch_coldft_table(g, h.colTab.p, h.stream), "column table");
+    if (const size_t n3 = dh::window_v3_table_elems(g)) {
+      DH_TRY(h.w3Tab.ensure(n3), "alloc");
+      DH_TRY(dh::launch_window_v3_tables(g, h.w3Tab.p, h.stream), "v3 tables");
+    }
     DH_TRY(hipStreamSynchronize(h.stream), "tables sync");   // the host mask vector goes out of scope
     h.tabKey = d;
     h.tablesValid = true;
@@ -626,11 +643,15 @@ int enqueue_batch(Handle& h) {
   if (timing) DH_TRY(hipEventRecord(ev[1], h.stream), "event");
   dh::FusedArgs fa;
   int path = 0;
-  if (h.windowMode >= 2 && h.frameU16 && plan_window_mfma(h, g, fa)) path = 2;
+  if (h.windowMode >= 3 && h.frameU16 && dh::window_v3_table_elems(g) && plan_window_v3(h, g, fa)) path = 3;
+  else if (h.windowMode >= 2 && h.frameU16 && plan_window_mfma(h, g, fa)) path = 2;
   else if (h.windowMode >= 1 && plan_fused(h, g, fa)) path = 1;
   if (path) {
     // one kernel from camera window to field
-    if (path == 2)
+    if (path == 3)
+      DH_TRY(dh::launch_window_v3(fa, dframes, h.w3Tab.p, h.factor.p, h.ref.p, h.fields.p, h.stream),
+             "fused window (register rows)");
+    else if (path == 2)
       DH_TRY(dh::launch_window_mfma(fa, h.twW.p, dframes, h.colTab.p, h.factor.p, h.ref.p, h.fields.p, h.stream),
              "fused window (MFMA columns)");
     else
@@ -1360,6 +1381,7 @@ void destroy_handle(Handle* h) {
     h->hgy.release();
     h->mask.release();
     h->colTab.release();
+    h->w3Tab.release();
     for (auto& kv : h->ffts) kv.second->tw.release();
     h->hCoefs.release();
     h->hFields.release();
@@ -2077,7 +2099,7 @@ EXT_C int digHoloGetFields16(int handleIdx, int* batchCount, int* polCount, shor
 
 EXT_C int digHoloConfigSetFusedWindow(int handleIdx, int enabled) {
   H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
-  if (enabled < 0 || enabled > 2) return DIGHOLO_ERROR_INVALIDARGUMENT;
+  if (enabled < 0 || enabled > 3) return DIGHOLO_ERROR_INVALIDARGUMENT;
   h->windowMode = enabled;
   return DIGHOLO_ERROR_SUCCESS;
 }

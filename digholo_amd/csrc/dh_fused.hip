@@ -18,6 +18,7 @@
 // transformed.  Supported radices 2/3/4/5/8; other shapes use the staged
 // kernels of dh_kernels.hip.
 #include <algorithm>
+#include <type_traits>
 
 #include "dh_fft.h"
 #include "dh_internal.h"
@@ -37,7 +38,7 @@ __device__ __forceinline__ int fdiv(int q, unsigned mag) {
 // In-place radix-R Stockham stage over nb transforms of length n; element
 // (t, k) lives at buf[t*ts + k*es].  tmajor: consecutive threads take
 // consecutive transforms (use when ts == 1), else consecutive butterflies.
-template <int R, bool INV>
+template <int R, bool INV, int NT = kFThreads>
 __device__ __forceinline__ void stage_ip(float2* buf, int n, int nb, int ts, int es, int Ns, unsigned magNbf,
                                          unsigned magNs, unsigned magNb, bool tmajor, const float2* tw) {
   constexpr int BMAX = kMaxV / R > 0 ? kMaxV / R : 1;
@@ -48,7 +49,7 @@ __device__ __forceinline__ void stage_ip(float2* buf, int n, int nb, int ts, int
   int wb[BMAX];
 #pragma unroll
   for (int bb = 0; bb < BMAX; ++bb) {
-    const int q = threadIdx.x + bb * kFThreads;
+    const int q = threadIdx.x + bb * NT;
     wb[bb] = -1;
     if (q < total) {
       int t, j;
@@ -83,7 +84,7 @@ __device__ __forceinline__ void stage_ip(float2* buf, int n, int nb, int ts, int
 }
 
 // nb transforms, at most `chunk` per in-place pass.
-template <bool INV>
+template <bool INV, int NT = kFThreads>
 __device__ void lds_fft_ip(float2* buf, int nb, int ts, int es, bool tmajor, const FFTPlan& pl, const float2* tw,
                            int chunk) {
   for (int t0 = 0; t0 < nb; t0 += chunk) {
@@ -94,11 +95,11 @@ __device__ void lds_fft_ip(float2* buf, int nb, int ts, int es, bool tmajor, con
     for (int s = 0; s < pl.nstages; ++s) {
       const int R = pl.radix[s];
       switch (R) {
-        case 8: stage_ip<8, INV>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
-        case 4: stage_ip<4, INV>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
-        case 2: stage_ip<2, INV>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
-        case 3: stage_ip<3, INV>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
-        default: stage_ip<5, INV>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
+        case 8: stage_ip<8, INV, NT>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
+        case 4: stage_ip<4, INV, NT>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
+        case 2: stage_ip<2, INV, NT>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
+        case 3: stage_ip<3, INV, NT>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
+        default: stage_ip<5, INV, NT>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
       }
       Ns *= R;
     }
@@ -263,14 +264,15 @@ __global__ __launch_bounds__(kFThreads, 4) void k_fused_window(FusedArgs a, Fuse
   }
 }
 
-// Most transforms of plan pl one in-place pass can hold (0: unsupported radix).
-static int ip_chunk(const FFTPlan& pl) {
+// Most transforms of plan pl one in-place pass of nt threads can hold (0:
+// unsupported radix).
+static int ip_chunk(const FFTPlan& pl, int nt = kFThreads) {
   int best = 1 << 30;
   for (int s = 0; s < pl.nstages; ++s) {
     const int R = pl.radix[s];
     if (R != 2 && R != 3 && R != 4 && R != 5 && R != 8) return 0;
     const int bmax = kMaxV / R > 0 ? kMaxV / R : 1;
-    best = std::min(best, bmax * kFThreads * R / pl.n);
+    best = std::min(best, bmax * nt * R / pl.n);
   }
   return best;
 }
@@ -567,6 +569,498 @@ hipError_t launch_window_mfma(const FusedArgs& a0, const float2* twW, const void
   WM_CASE(1, 3) WM_CASE(1, 4) WM_CASE(1, 5) WM_CASE(1, 6) WM_CASE(1, 8)
   WM_CASE(2, 5) WM_CASE(2, 6) WM_CASE(2, 8)
 #undef WM_CASE
+  return hipErrorNotSupported;
+}
+
+// ===========================================================================
+// Size-specialised window kernel (uint16 frames), 256 threads per (frame, pol).
+//
+// Row transform, in registers.  A row of W = 2 NR reals is packed into NR
+// complex points z[n] = I[2n] + i I[2n+1] and transformed by L lanes holding M
+// points each (NR = L M, n = l + L m):
+//     pass A (per lane):   v[a] = w_NR^(l a) sum_m z[l + L m] w_M^(m a)
+//     pass B (over lanes): Z[a + M b] = sum_l w_L^(l b) v_l[a]
+// The transpose between the passes goes through a wave-private LDS region in
+// two halves, so no block barrier is involved.  Pass B deals the a values to
+// lanes in pairs (a, M - a) (and (0, M/2)), so that Z[k] and its r2c partner
+// Z[NR - k] sit in the same lane and the real-to-complex unpack
+//     X[k] = (Z[k] + Z*[NR-k])/2 - i w_W^k (Z[k] - Z*[NR-k])/2
+// needs no further exchange.  Only columns c0 <= k < c0 + w are kept.
+//
+// Column transform restricted to the h window rows, a complex GEMM on MFMA:
+//     F[j][c] = sum_y D[j][y] X[y][c],  D[j][y] = exp(-2 pi i ky(j) y / H),
+// split into fp16 hi + lo with hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_f16
+// and FP32 accumulation (the operand layout of k_window_mfma).  The D operands
+// are built per k-step from an LDS table of exp(-2 pi i m / H), so nothing but
+// the frame streams from memory.  A round is 4 RPW window rows (each wave
+// transforms RPW of them); units (j-tile, re/im part) are dealt to the waves.
+//
+// Epilogue as k_window_mfma: window factor, 2-D inverse FFT in LDS, tilt.
+// ===========================================================================
+
+template <int B0, int E, class F>
+__device__ __forceinline__ void sfor(F&& f) {
+  if constexpr (B0 < E) {
+    f(std::integral_constant<int, B0>{});
+    sfor<B0 + 1, E>(f);
+  }
+}
+
+constexpr double kPiD = 3.14159265358979323846;
+
+// sin(x) by its Taylor series, |x| <= 3 pi / 2 (compile time only)
+constexpr double ct_sin(double x) {
+  double t = x, s = x;
+  for (int n = 1; n < 26; ++n) {
+    t *= -x * x / ((2.0 * n) * (2.0 * n + 1.0));
+    s += t;
+  }
+  return s;
+}
+
+// v * exp(-/+ 2 pi i E / N) for compile-time E, N
+template <int E, int N, bool INV>
+__device__ __forceinline__ float2 tw_mul(float2 v) {
+  constexpr int e = ((E % N) + N) % N;
+  if constexpr (e == 0) {
+    return v;
+  } else if constexpr (4 * e == N) {
+    return mul_mi<INV>(v);
+  } else if constexpr (2 * e == N) {
+    return make_float2(-v.x, -v.y);
+  } else if constexpr (4 * e == 3 * N) {
+    return mul_mi<!INV>(v);
+  } else {
+    constexpr double th = 2.0 * kPiD * (double)(2 * e <= N ? e : e - N) / (double)N;   // (-pi, pi]
+    constexpr float c = (float)ct_sin(kPiD / 2 - th);
+    constexpr float sn = (float)ct_sin(th);
+    return cmul(v, make_float2(c, INV ? sn : -sn));
+  }
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft_any(float2* v);
+
+// In-register DFT of P Q points, natural order in and out:
+//   V[a1 + Q a2] = sum_p w_P^(p a2) w_PQ^(p a1) sum_q v[p + P q] w_Q^(q a1)
+template <int P, int Q, bool INV>
+__device__ __forceinline__ void fft_pq(float2* v) {
+  constexpr int M = P * Q;
+  float2 y[M];
+  sfor<0, P>([&](auto pc) {
+    constexpr int p = decltype(pc)::value;
+    float2 t[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) t[q] = v[p + P * q];
+    dft_any<Q, INV>(t);
+    sfor<0, Q>([&](auto ac) {
+      constexpr int a1 = decltype(ac)::value;
+      y[a1 * P + p] = tw_mul<p * a1, M, INV>(t[a1]);
+    });
+  });
+#pragma unroll
+  for (int a1 = 0; a1 < Q; ++a1) {
+    float2 t[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) t[p] = y[a1 * P + p];
+    dft_any<P, INV>(t);
+#pragma unroll
+    for (int a2 = 0; a2 < P; ++a2) v[a1 + Q * a2] = t[a2];
+  }
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft_any(float2* v) {
+  if constexpr (R == 16) fft_pq<4, 4, INV>(v);
+  else if constexpr (R == 24) fft_pq<3, 8, INV>(v);
+  else if constexpr (R == 1) (void)v;
+  else dftR<R, INV>(v);
+}
+
+// LDS order point within a wave: every lane's earlier LDS accesses have
+// completed before any later one is issued.
+__device__ __forceinline__ void wave_lds_sync() {
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  __builtin_amdgcn_wave_barrier();
+}
+
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+
+constexpr int kW3Threads = 256;
+
+template <int L, int M, int U, int J>
+struct W3Shape {
+  static constexpr int NR = L * M;            // complex points per row transform
+  static constexpr int RPW = 64 / L;          // rows per wave per round
+  static constexpr int RR = 4 * RPW;          // rows per round
+  static constexpr int KS = RR / 16;          // MFMA k-steps per round
+  static constexpr int S = M / L / 2;         // (a, M - a) pairs per lane
+  static constexpr int w = 16 * U, h = 16 * J;
+  static constexpr int NU = (2 * J + 3) / 4;  // (j-tile, part) units per wave
+  static constexpr int LP = M / 2 + 2;        // exchange: complex per lane (half)
+  static constexpr int RPc = L * LP + (8 - (L * LP) % 16 + 16) % 16;   // per row, = 8 mod 16
+  static constexpr int PK = 2 * RR + 8;       // halfs per B column
+  static constexpr int EBYTES = 4 * RPW * RPc * 8;
+  static constexpr int BBYTES = 2 * w * PK * 2;
+};
+
+// LDS bytes of the row phase, the field plane, and the whole kernel
+__host__ __device__ inline int w3_region(int ebytes, int bbytes, int H, int L, int M, int w, int h) {
+  const int row = ebytes + bbytes + 8 * H + 8 * L * (M + 1);
+  const int pl = 8 * h * w;
+  return ((row > pl ? row : pl) + 15) & ~15;
+}
+
+struct W3Args {
+  Geometry g;
+  FFTPlan ix, iy;
+  int chunkIx, chunkIy;
+  int region;         // bytes of the aliased row-phase / field-plane region
+  unsigned magH;      // q / H == __umulhi(q, magH) for q < 2^20
+  float xscale;       // power of two keeping the row spectra inside fp16
+};
+
+template <int L, int M, int U, int J>
+__global__ __launch_bounds__(kW3Threads) void k_window_v3(W3Args a, const unsigned short* __restrict__ frames,
+                                                          const float2* __restrict__ tabs,
+                                                          const float2* __restrict__ factor,
+                                                          const float2* __restrict__ ref,
+                                                          float2* __restrict__ fields) {
+  using Sh = W3Shape<L, M, U, J>;
+  constexpr int NT = kW3Threads, RPW = Sh::RPW, RR = Sh::RR, KS = Sh::KS, S = Sh::S, w = Sh::w, h = Sh::h;
+  constexpr int NU = Sh::NU, LP = Sh::LP, RPc = Sh::RPc, PK = Sh::PK, NR = Sh::NR;
+  extern __shared__ __align__(16) unsigned char w3_smem[];
+  const Geometry& g = a.g;
+  const int H = g.H;
+  const int p = blockIdx.x, b = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wv = (wave + b) & 3;              // units rotate with the frame so every SIMD gets its share
+  const int r = lane / L, l = lane % L;       // row of the wave's pass, lane within the row
+  const int lr = lane & 15, q = lane >> 4;    // MFMA operand coordinates
+  const int c0 = g.c0[p];
+
+  float2* E = reinterpret_cast<float2*>(w3_smem) + (wave * RPW + r) * RPc;      // this lane's exchange row
+  _Float16* Bs = reinterpret_cast<_Float16*>(w3_smem + Sh::EBYTES);             // [2][w][PK]
+  float2* twH = reinterpret_cast<float2*>(w3_smem + Sh::EBYTES + Sh::BBYTES);   // [H]
+  float2* twA = twH + H;                                                        // [L][M + 1]
+  float2* fbuf = reinterpret_cast<float2*>(w3_smem);                            // [h][w] after the rows
+  float2* twix = reinterpret_cast<float2*>(w3_smem + a.region);                 // [w]
+  float2* twiy = twix + w;                                                      // [h]
+
+  // the first round of this lane's row in flight while the tables are staged
+  const size_t fw2 = (size_t)(g.frameW >> 1);
+  const unsigned* src = reinterpret_cast<const unsigned*>(frames + ((size_t)b * g.frameH + g.cy[p]) * g.frameW +
+                                                          g.cx[p]) + l + (size_t)(wave * RPW + r) * fw2;
+  unsigned pre[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) pre[m] = __builtin_nontemporal_load(src + L * m);
+
+  for (int e = tid; e < H; e += NT) twH[e] = tabs[e];
+  for (int e = tid; e < L * (M + 1); e += NT) twA[e] = tabs[H + e];
+  for (int e = tid; e < w; e += NT) twix[e] = a.ix.tw[e];
+  for (int e = tid; e < h; e += NT) twiy[e] = a.iy.tw[e];
+
+  // this lane's a values in pass B and exp(-2 pi i a / W) for them
+  int alo[S], ahi[S];
+  float2 wlo[S], whi[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int pi = l * S + s;
+    alo[s] = pi;
+    ahi[s] = pi == 0 ? M / 2 : M - pi;
+    wlo[s] = tabs[H + L * (M + 1) + alo[s]];
+    whi[s] = tabs[H + L * (M + 1) + ahi[s]];
+  }
+  // window rows of this lane's MFMA units
+  int ky[NU];
+#pragma unroll
+  for (int i = 0; i < NU; ++i) {
+    const int jt = (wv + 4 * i) >> 1;
+    int t = (16 * jt + lr + g.SY[p] + H / 2) % H;
+    ky[i] = t < 0 ? t + H : t;
+  }
+
+  floatx4 acc[NU][U];
+#pragma unroll
+  for (int i = 0; i < NU; ++i)
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc[i][u] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  const float sc = 0.5f * a.xscale;           // ev + w od = 2 X
+  const int rho = wave * RPW + r;             // this lane's row within a round
+  auto put = [&](int k, float2 X) {
+    const int c = k - c0;
+    if ((unsigned)c < (unsigned)w) {
+      const float xr = X.x * sc, xi = X.y * sc;
+      const _Float16 rh = (_Float16)xr, ih = (_Float16)xi;
+      const half2v hi = {rh, ih};
+      const half2v lo = {(_Float16)(xr - (float)rh), (_Float16)(xi - (float)ih)};
+      *reinterpret_cast<half2v*>(Bs + c * PK + 2 * rho) = hi;
+      *reinterpret_cast<half2v*>(Bs + (w + c) * PK + 2 * rho) = lo;
+    }
+  };
+  auto unpack = [](float2 zk, float2 zp, float2 wk) {
+    const float2 ev = make_float2(zk.x + zp.x, zk.y - zp.y);
+    const float2 od = make_float2(zk.y + zp.y, zp.x - zk.x);     // -i (zk - conj zp)
+    return cadd(ev, cmul(wk, od));
+  };
+
+  __syncthreads();   // tables staged
+  const int nrd = H / RR;
+  for (int rd = 0; rd < nrd; ++rd) {
+    float2 v[M];
+#pragma unroll
+    for (int m = 0; m < M; ++m) v[m] = make_float2((float)(pre[m] & 0xffffu), (float)(pre[m] >> 16));
+    if (rd + 1 < nrd) {
+      const unsigned* s2 = src + (size_t)(rd + 1) * RR * fw2;
+#pragma unroll
+      for (int m = 0; m < M; ++m) pre[m] = __builtin_nontemporal_load(s2 + L * m);
+    }
+    // pass A and the inter-pass twiddle w_NR^(l a)
+    dft_any<M, false>(v);
+#pragma unroll
+    for (int aa = 1; aa < M; ++aa) v[aa] = cmul(v[aa], twA[l * (M + 1) + aa]);
+    // transpose through the wave-private rows, low then high half of a
+    float2 zl[S][L], zh[S][L];
+#pragma unroll
+    for (int aa = 0; aa < M / 2; aa += 2)
+      *reinterpret_cast<float4*>(E + l * LP + aa) = make_float4(v[aa].x, v[aa].y, v[aa + 1].x, v[aa + 1].y);
+    wave_lds_sync();
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+      for (int ls = 0; ls < L; ++ls) zl[s][ls] = E[ls * LP + alo[s]];
+    wave_lds_sync();
+#pragma unroll
+    for (int aa = 0; aa < M / 2; aa += 2)
+      *reinterpret_cast<float4*>(E + l * LP + aa) =
+          make_float4(v[M / 2 + aa].x, v[M / 2 + aa].y, v[M / 2 + aa + 1].x, v[M / 2 + aa + 1].y);
+    wave_lds_sync();
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+      for (int ls = 0; ls < L; ++ls) zh[s][ls] = E[ls * LP + ahi[s] - M / 2];
+    wave_lds_sync();
+    // pass B: zl[s][b] = Z[alo + M b], zh[s][b] = Z[ahi + M b]
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      dft_any<L, false>(zl[s]);
+      dft_any<L, false>(zh[s]);
+    }
+    // unpack at the window columns into the B operand (fp16 hi / lo)
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const bool selfpair = (l * S + s) == 0;   // a = 0 and a = M/2 pair within their own group
+      sfor<0, L>([&](auto bc) {
+        constexpr int bb = decltype(bc)::value;
+        // partners as values first: a select between two array elements would
+        // become a dynamically indexed (scratch) load
+        const float2 zlr = zl[s][(L - bb) % L], zhr = zh[s][L - 1 - bb], zlm = zl[s][L - 1 - bb];
+        const float2 pl = make_float2(selfpair ? zlr.x : zhr.x, selfpair ? zlr.y : zhr.y);
+        const float2 ph = make_float2(selfpair ? zhr.x : zlm.x, selfpair ? zhr.y : zlm.y);
+        put(alo[s] + M * bb, unpack(zl[s][bb], pl, tw_mul<bb, 2 * L, false>(wlo[s])));
+        put(ahi[s] + M * bb, unpack(zh[s][bb], ph, tw_mul<bb, 2 * L, false>(whi[s])));
+      });
+      if (selfpair) {   // Nyquist column k = NR from Z[0]
+        const float2 z0 = zl[s][0];
+        put(NR, make_float2(2.f * (z0.x - z0.y), 0.f));
+      }
+    }
+    __syncthreads();   // B complete
+    // F += D . X over this round's rows
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      half8 ah[NU], al[NU];
+      const int y0 = rd * RR + 16 * ks + 4 * q;
+#pragma unroll
+      for (int i = 0; i < NU; ++i) {
+        const int unit = wv + 4 * i;
+        if (unit < 2 * J) {
+          const unsigned qq = (unsigned)(ky[i] * y0);
+          int mm = (int)(qq - __umulhi(qq, a.magH) * (unsigned)H);
+#pragma unroll
+          for (int e2 = 0; e2 < 4; ++e2) {
+            const float2 D = twH[mm];
+            mm += ky[i];
+            if (mm >= H) mm -= H;
+            const float s0 = (unit & 1) ? D.y : D.x;
+            const float s1 = (unit & 1) ? D.x : -D.y;
+            const _Float16 h0 = (_Float16)s0, h1 = (_Float16)s1;
+            ah[i][2 * e2] = h0;
+            ah[i][2 * e2 + 1] = h1;
+            al[i][2 * e2] = (_Float16)(s0 - (float)h0);
+            al[i][2 * e2 + 1] = (_Float16)(s1 - (float)h1);
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const _Float16* bq = Bs + (16 * u + lr) * PK + 32 * ks + 8 * q;
+        const half8 bh = *reinterpret_cast<const half8*>(bq);
+        const half8 bl = *reinterpret_cast<const half8*>(bq + w * PK);
+#pragma unroll
+        for (int i = 0; i < NU; ++i) {
+          if (wv + 4 * i < 2 * J) {
+            acc[i][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh, acc[i][u], 0, 0, 0);
+            acc[i][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl, acc[i][u], 0, 0, 0);
+            acc[i][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh, acc[i][u], 0, 0, 0);
+          }
+        }
+      }
+    }
+    __syncthreads();   // B and the D table reads done before the next round (or the plane) overwrites
+  }
+
+  // window plane: F / xscale, then the window factor
+  const float inv = 1.f / a.xscale;
+  float* fl = reinterpret_cast<float*>(fbuf);
+#pragma unroll
+  for (int i = 0; i < NU; ++i) {
+    const int unit = wv + 4 * i;
+    if (unit < 2 * J) {
+      const int jt = unit >> 1, part = unit & 1;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) fl[2 * ((16 * jt + 4 * q + e) * w + 16 * u + lr) + part] = acc[i][u][e] * inv;
+    }
+  }
+  __syncthreads();
+  const float2* fac = factor + (size_t)p * (h * w);
+  for (int e = tid; e < h * w; e += NT) fbuf[e] = cmul(fbuf[e], fac[e]);
+  __syncthreads();
+  lds_fft_ip<true, NT>(fbuf, h, w, 1, false, a.ix, twix, a.chunkIx);   // along x
+  lds_fft_ip<true, NT>(fbuf, w, 1, w, true, a.iy, twiy, a.chunkIy);    // along y
+  const float2* rp = ref + (size_t)p * (h * w);
+  float2* out = fields + (size_t)(b * g.P + p) * (h * w);
+  for (int e = tid; e < h * w; e += NT) {
+    const float2 t = cmul(fbuf[e], rp[e]);
+    out[e] = make_float2(-t.x, t.y);
+  }
+}
+
+// Tables of the v3 kernel: [H] exp(-2 pi i m / H), [L][M + 1] exp(-2 pi i l a / NR),
+// [M] exp(-2 pi i a / W), from FP64.
+__global__ void k_w3_tables(int H, int L, int M, float2* __restrict__ tab) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int NR = L * M;
+  double ang;   // in units of pi
+  if (e < H) {
+    ang = -2.0 * e / H;
+  } else if ((e -= H) < L * (M + 1)) {
+    const int ll = e / (M + 1), aa = e - ll * (M + 1);
+    ang = -2.0 * (double)((ll * aa) % NR) / NR;
+  } else if ((e -= L * (M + 1)) < M) {
+    ang = -1.0 * e / NR;
+  } else {
+    return;
+  }
+  double sn, cs;
+  sincospi(ang, &sn, &cs);
+  tab[blockIdx.x * blockDim.x + threadIdx.x] = make_float2((float)cs, (float)sn);
+}
+
+// (L, M) of a row transform of nr complex points; false if not specialised
+static bool w3_lm(int nr, int& L, int& M) {
+  if (nr == 128) { L = 8; M = 16; return true; }
+  if (nr == 64) { L = 4; M = 16; return true; }
+  if (nr == 96) { L = 4; M = 24; return true; }
+  return false;
+}
+
+size_t window_v3_table_elems(const Geometry& g) {
+  int L, M;
+  if (!w3_lm(g.W / 2, L, M)) return 0;
+  return (size_t)g.H + (size_t)L * (M + 1) + M;
+}
+
+hipError_t launch_window_v3_tables(const Geometry& g, float2* tab, hipStream_t s) {
+  int L, M;
+  if (!w3_lm(g.W / 2, L, M)) return hipErrorNotSupported;
+  const int n = (int)window_v3_table_elems(g);
+  k_w3_tables<<<(n + 255) / 256, 256, 0, s>>>(g.H, L, M, tab);
+  return hipGetLastError();
+}
+
+template <int L, int M, int U, int J>
+static size_t w3_lds(const W3Args& a) {
+  return (size_t)a.region + 8 * (size_t)(16 * U + 16 * J);
+}
+
+template <int L, int M, int U, int J>
+static hipError_t w3_launch(W3Args a, const void* frames, const float2* tab, const float2* factor,
+                            const float2* ref, float2* fields, hipStream_t s) {
+  using Sh = W3Shape<L, M, U, J>;
+  const Geometry& g = a.g;
+  if (g.H % Sh::RR) return hipErrorNotSupported;
+  a.region = w3_region(Sh::EBYTES, Sh::BBYTES, g.H, L, M, Sh::w, Sh::h);
+  const size_t lds = w3_lds<L, M, U, J>(a);
+  if (lds > 160 * 1024) return hipErrorNotSupported;
+  if (lds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_window_v3<L, M, U, J>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  dim3 grid(g.P, g.B);
+  k_window_v3<L, M, U, J><<<grid, kW3Threads, lds, s>>>(a, static_cast<const unsigned short*>(frames), tab, factor,
+                                                        ref, fields);
+  return hipGetLastError();
+}
+
+// Calls f<L, M, U, J>() for the specialised shapes; returns false otherwise.
+#define W3_SHAPES(X)                                                            \
+  X(8, 16, 5, 5) X(8, 16, 4, 4) X(8, 16, 3, 3) X(8, 16, 6, 6) X(4, 16, 3, 3) \
+  X(4, 16, 4, 4) X(4, 24, 4, 4) X(4, 24, 3, 3) X(4, 24, 5, 5)
+
+static bool w3_common(const Geometry& g, int& L, int& M) {
+  if (!w3_lm(g.W / 2, L, M)) return false;
+  if (g.w % 16 || g.h % 16 || g.H > 1024) return false;
+  if ((g.frameW & 1) || (g.cx[0] & 1) || (g.P > 1 && (g.cx[1] & 1))) return false;   // dword row loads
+  for (int p = 0; p < g.P; ++p)
+    if (g.c0[p] < 0 || g.c0[p] + g.w > g.W / 2 + 1) return false;
+  return true;
+}
+
+bool window_v3_supported(const FusedArgs& a) {
+  const Geometry& g = a.g;
+  int L, M;
+  if (!w3_common(g, L, M)) return false;
+  if (ip_chunk(a.ix, kW3Threads) < 1 || ip_chunk(a.iy, kW3Threads) < 1) return false;
+  const int U = g.w / 16, J = g.h / 16;
+#define W3_MATCH(LL, MM, UU, JJ)                                                             \
+  if (L == LL && M == MM && U == UU && J == JJ)                                              \
+    return g.H % W3Shape<LL, MM, UU, JJ>::RR == 0 &&                                         \
+           w3_region(W3Shape<LL, MM, UU, JJ>::EBYTES, W3Shape<LL, MM, UU, JJ>::BBYTES, g.H, LL, MM, \
+                     16 * UU, 16 * JJ) + 8 * (16 * UU + 16 * JJ) <= 160 * 1024;
+  W3_SHAPES(W3_MATCH)
+#undef W3_MATCH
+  return false;
+}
+
+hipError_t launch_window_v3(const FusedArgs& fa, const void* frames_u16, const float2* tab, const float2* factor,
+                            const float2* ref, float2* fields, hipStream_t s) {
+  if (!window_v3_supported(fa)) return hipErrorNotSupported;
+  const Geometry& g = fa.g;
+  int L, M;
+  w3_lm(g.W / 2, L, M);
+  W3Args a{};
+  a.g = g;
+  a.ix = fa.ix;
+  a.iy = fa.iy;
+  a.chunkIx = ip_chunk(fa.ix, kW3Threads);
+  a.chunkIy = ip_chunk(fa.iy, kW3Threads);
+  a.magH = magic_div((unsigned)g.H);
+  if (a.magH == 0) return hipErrorNotSupported;
+  // |X| <= W * 65535 < 2^(16 + log2 W): scale into [2^14, 2^15) at most
+  int lw = 0;
+  while ((1 << lw) < g.W) ++lw;
+  a.xscale = ldexpf(1.f, 15 - (16 + lw));
+  const int U = g.w / 16, J = g.h / 16;
+#define W3_CALL(LL, MM, UU, JJ) \
+  if (L == LL && M == MM && U == UU && J == JJ) return w3_launch<LL, MM, UU, JJ>(a, frames_u16, tab, factor, ref, fields, s);
+  W3_SHAPES(W3_CALL)
+#undef W3_CALL
   return hipErrorNotSupported;
 }
 

@@ -91,6 +91,16 @@ hipError_t launch_coldft_table(const Geometry& g, void* tab, hipStream_t s);
 hipError_t launch_window_mfma(const FusedArgs& a, const float2* twW, const void* frames_u16, const void* dtab,
                               const float2* factor, const float2* ref, float2* fields, hipStream_t s);
 
+// Size-specialised fused window path (uint16 frames, W/2 in {64, 96, 128},
+// w and h multiples of 16 from a fixed list): register row transforms and the
+// column transform on MFMA with D built from an LDS twiddle table.  tab holds
+// window_v3_table_elems(g) complex values built by launch_window_v3_tables.
+bool window_v3_supported(const FusedArgs& a);
+size_t window_v3_table_elems(const Geometry& g);
+hipError_t launch_window_v3_tables(const Geometry& g, float2* tab, hipStream_t s);
+hipError_t launch_window_v3(const FusedArgs& a, const void* frames_u16, const float2* tab, const float2* factor,
+                            const float2* ref, float2* fields, hipStream_t s);
+
 // Power-meter reductions (dh_metrics.hip).  out: [S][N][5] doubles per row
 // {diagonal, off-diagonal, in-group/g, out-of-group/g, total power}.
 hipError_t launch_metric_rows(const float2* coefs, int N, int M, int Mpol, int rowStride, int S, double* out,

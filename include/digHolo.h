@@ -238,11 +238,13 @@ EXT_C void* digHoloGetStream(int handleIdx);
  * the GPU: out[0..11] = W, H, w, h, cx0, cy0, cx1, cy1, SX0, SY0, SX1, SY1. */
 EXT_C int digHoloGetWindowGeometry(int handleIdx, int* out);
 
-/* Window path.  mode 2 (default): the fused camera-window-to-field kernel with
- * the column transform as a split-fp16 GEMM on MFMA (uint16 frames); 1: the
- * fused kernel with FFT column transforms; 0: the staged kernels.  A mode the
- * shape or input type does not fit falls back to the next lower one.
- * GetWindowPath reports the path of the last batch (2, 1, 0; -1 none yet). */
+/* Window path.  mode 3 (default): the size-specialised camera-window-to-field
+ * kernel (row transforms in registers, column transform as a split-fp16 GEMM on
+ * MFMA; uint16 frames, W/2 in {64, 96, 128}); 2: the general fused kernel with
+ * the MFMA column transform (uint16 frames); 1: the fused kernel with FFT column
+ * transforms; 0: the staged kernels.  A mode the shape or input type does not
+ * fit falls back to the next lower one.  GetWindowPath reports the path of the
+ * last batch (3, 2, 1, 0; -1 none yet). */
 EXT_C int digHoloConfigSetFusedWindow(int handleIdx, int enabled);
 EXT_C int digHoloGetWindowPath(int handleIdx);
 

@@ -104,7 +104,7 @@ def check(frames, cfg, coefs, fields, tol=TOL):
     return plan
 
 
-@pytest.mark.parametrize("fused", [2, 1, 0])
+@pytest.mark.parametrize("fused", [3, 2, 1, 0])
 @pytest.mark.parametrize("name,B", [("small", 8), ("dual640", 3), ("dual640w192", 3), ("radix5", 4)])
 def test_parity_uint16(lib, name, B, fused):
     frames, cfg = make_case(name, B, seed=11)
@@ -113,7 +113,9 @@ def test_parity_uint16(lib, name, B, fused):
     api.configure(lib, h, **cfg)
     assert lib.digHoloConfigSetFusedWindow(h, fused) == 0
     coefs, fields, x, y, h = run_gpu(lib, frames, cfg, h=h)
-    assert lib.digHoloGetWindowPath(h) == fused      # these shapes fit the fused kernels
+    # these shapes fit the fused kernels; the specialised one has no 80-point row transform
+    expect = 2 if (fused == 3 and name == "radix5") else fused
+    assert lib.digHoloGetWindowPath(h) == expect
     plan = check(frames, cfg, coefs, fields)
     # reconstructed axes: (i - w/2) * px W / w (digHolo.cpp:8184-8189)
     px = np.float32(20e-6)
