@@ -591,11 +591,17 @@ hipError_t launch_window_mfma(const FusedArgs& a0, const float2* twW, const void
 //     F[j][c] = sum_y D[j][y] X[y][c],  D[j][y] = exp(-2 pi i ky(j) y / H),
 // split into fp16 hi + lo with hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_f16
 // and FP32 accumulation (the operand layout of k_window_mfma).  The D operands
-// are built per k-step from an LDS table of exp(-2 pi i m / H), so nothing but
-// the frame streams from memory.  A round is 4 RPW window rows (each wave
+// are read per k-step from an LDS table holding, for each m, cos and sin of
+// 2 pi m / H already split into fp16 hi and lo, so nothing but the frame
+// streams from memory.  Row-spectrum values outside the window are written to
+// a spare B column instead of being branched around.  A round is 4 RPW window rows (each wave
 // transforms RPW of them); units (j-tile, re/im part) are dealt to the waves.
 //
-// Epilogue as k_window_mfma: window factor, 2-D inverse FFT in LDS, tilt.
+// Epilogue: the window plane goes to LDS (row pitch w + 4), and each inverse
+// transform of N = 16 U points is done by 4 lanes holding N/4 points (the
+// same two-pass split, in place in the plane), 16 transforms per wave pass:
+// rows with the window factor applied on load, then columns with the
+// reference-wave tilt applied on the way to HBM.
 // ===========================================================================
 
 template <int B0, int E, class F>
@@ -673,6 +679,8 @@ template <int R, bool INV>
 __device__ __forceinline__ void dft_any(float2* v) {
   if constexpr (R == 16) fft_pq<4, 4, INV>(v);
   else if constexpr (R == 24) fft_pq<3, 8, INV>(v);
+  else if constexpr (R == 12) fft_pq<3, 4, INV>(v);
+  else if constexpr (R == 20) fft_pq<4, 5, INV>(v);
   else if constexpr (R == 1) (void)v;
   else dftR<R, INV>(v);
 }
@@ -701,34 +709,34 @@ struct W3Shape {
   static constexpr int RPc = L * LP + (8 - (L * LP) % 16 + 16) % 16;   // per row, = 8 mod 16
   static constexpr int PK = 2 * RR + 8;       // halfs per B column
   static constexpr int EBYTES = 4 * RPW * RPc * 8;
-  static constexpr int BBYTES = 2 * w * PK * 2;
+  static constexpr int BBYTES = 2 * (w + 1) * PK * 2;   // column w takes discarded values
+  static constexpr int WP = w + 4;            // field-plane row pitch (complex)
 };
 
-// LDS bytes of the row phase, the field plane, and the whole kernel
+// LDS bytes of the aliased region: the row phase or the field plane
 __host__ __device__ inline int w3_region(int ebytes, int bbytes, int H, int L, int M, int w, int h) {
   const int row = ebytes + bbytes + 8 * H + 8 * L * (M + 1);
-  const int pl = 8 * h * w;
+  const int pl = 8 * h * (w + 4);
   return ((row > pl ? row : pl) + 15) & ~15;
 }
 
 struct W3Args {
   Geometry g;
-  FFTPlan ix, iy;
-  int chunkIx, chunkIy;
+  FFTPlan ix, iy;                     // twiddle tables of the w- and h-point transforms
   int region;         // bytes of the aliased row-phase / field-plane region
   unsigned magH;      // q / H == __umulhi(q, magH) for q < 2^20
   float xscale;       // power of two keeping the row spectra inside fp16
 };
 
 template <int L, int M, int U, int J>
-__global__ __launch_bounds__(kW3Threads) void k_window_v3(W3Args a, const unsigned short* __restrict__ frames,
+__global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const unsigned short* __restrict__ frames,
                                                           const float2* __restrict__ tabs,
                                                           const float2* __restrict__ factor,
                                                           const float2* __restrict__ ref,
                                                           float2* __restrict__ fields) {
   using Sh = W3Shape<L, M, U, J>;
   constexpr int NT = kW3Threads, RPW = Sh::RPW, RR = Sh::RR, KS = Sh::KS, S = Sh::S, w = Sh::w, h = Sh::h;
-  constexpr int NU = Sh::NU, LP = Sh::LP, RPc = Sh::RPc, PK = Sh::PK, NR = Sh::NR;
+  constexpr int NU = Sh::NU, LP = Sh::LP, RPc = Sh::RPc, PK = Sh::PK, NR = Sh::NR, WP = Sh::WP;
   extern __shared__ __align__(16) unsigned char w3_smem[];
   const Geometry& g = a.g;
   const int H = g.H;
@@ -741,10 +749,10 @@ __global__ __launch_bounds__(kW3Threads) void k_window_v3(W3Args a, const unsign
   const int c0 = g.c0[p];
 
   float2* E = reinterpret_cast<float2*>(w3_smem) + (wave * RPW + r) * RPc;      // this lane's exchange row
-  _Float16* Bs = reinterpret_cast<_Float16*>(w3_smem + Sh::EBYTES);             // [2][w][PK]
-  float2* twH = reinterpret_cast<float2*>(w3_smem + Sh::EBYTES + Sh::BBYTES);   // [H]
-  float2* twA = twH + H;                                                        // [L][M + 1]
-  float2* fbuf = reinterpret_cast<float2*>(w3_smem);                            // [h][w] after the rows
+  _Float16* Bs = reinterpret_cast<_Float16*>(w3_smem + Sh::EBYTES);             // [2][w + 1][PK]
+  uint2* twH = reinterpret_cast<uint2*>(w3_smem + Sh::EBYTES + Sh::BBYTES);     // [H] {cos, sin} hi, lo
+  float2* twA = reinterpret_cast<float2*>(twH + H);                            // [L][M + 1]
+  float2* fbuf = reinterpret_cast<float2*>(w3_smem);                            // [h][WP] after the rows
   float2* twix = reinterpret_cast<float2*>(w3_smem + a.region);                 // [w]
   float2* twiy = twix + w;                                                      // [h]
 
@@ -756,7 +764,7 @@ __global__ __launch_bounds__(kW3Threads) void k_window_v3(W3Args a, const unsign
 #pragma unroll
   for (int m = 0; m < M; ++m) pre[m] = __builtin_nontemporal_load(src + L * m);
 
-  for (int e = tid; e < H; e += NT) twH[e] = tabs[e];
+  for (int e = tid; e < H; e += NT) twH[e] = reinterpret_cast<const uint2*>(tabs)[e];
   for (int e = tid; e < L * (M + 1); e += NT) twA[e] = tabs[H + e];
   for (int e = tid; e < w; e += NT) twix[e] = a.ix.tw[e];
   for (int e = tid; e < h; e += NT) twiy[e] = a.iy.tw[e];
@@ -790,15 +798,14 @@ __global__ __launch_bounds__(kW3Threads) void k_window_v3(W3Args a, const unsign
   const float sc = 0.5f * a.xscale;           // ev + w od = 2 X
   const int rho = wave * RPW + r;             // this lane's row within a round
   auto put = [&](int k, float2 X) {
-    const int c = k - c0;
-    if ((unsigned)c < (unsigned)w) {
-      const float xr = X.x * sc, xi = X.y * sc;
-      const _Float16 rh = (_Float16)xr, ih = (_Float16)xi;
-      const half2v hi = {rh, ih};
-      const half2v lo = {(_Float16)(xr - (float)rh), (_Float16)(xi - (float)ih)};
-      *reinterpret_cast<half2v*>(Bs + c * PK + 2 * rho) = hi;
-      *reinterpret_cast<half2v*>(Bs + (w + c) * PK + 2 * rho) = lo;
-    }
+    const unsigned c0k = (unsigned)(k - c0);
+    const int c = c0k < (unsigned)w ? (int)c0k : w;
+    const float xr = X.x * sc, xi = X.y * sc;
+    const _Float16 rh = (_Float16)xr, ih = (_Float16)xi;
+    const half2v hi = {rh, ih};
+    const half2v lo = {(_Float16)(xr - (float)rh), (_Float16)(xi - (float)ih)};
+    *reinterpret_cast<half2v*>(Bs + c * PK + 2 * rho) = hi;
+    *reinterpret_cast<half2v*>(Bs + (w + 1 + c) * PK + 2 * rho) = lo;
   };
   auto unpack = [](float2 zk, float2 zp, float2 wk) {
     const float2 ev = make_float2(zk.x + zp.x, zk.y - zp.y);
@@ -862,10 +869,9 @@ __global__ __launch_bounds__(kW3Threads) void k_window_v3(W3Args a, const unsign
         put(alo[s] + M * bb, unpack(zl[s][bb], pl, tw_mul<bb, 2 * L, false>(wlo[s])));
         put(ahi[s] + M * bb, unpack(zh[s][bb], ph, tw_mul<bb, 2 * L, false>(whi[s])));
       });
-      if (selfpair) {   // Nyquist column k = NR from Z[0]
-        const float2 z0 = zl[s][0];
-        put(NR, make_float2(2.f * (z0.x - z0.y), 0.f));
-      }
+      // Nyquist column k = NR from Z[0] (other lanes discard)
+      const float2 z0 = zl[s][0];
+      put(selfpair ? NR : c0 - 1, make_float2(2.f * (z0.x - z0.y), 0.f));
     }
     __syncthreads();   // B complete
     // F += D . X over this round's rows
@@ -879,26 +885,33 @@ __global__ __launch_bounds__(kW3Threads) void k_window_v3(W3Args a, const unsign
         if (unit < 2 * J) {
           const unsigned qq = (unsigned)(ky[i] * y0);
           int mm = (int)(qq - __umulhi(qq, a.magH) * (unsigned)H);
+          uint4 dh, dl;   // k-slot pairs (Re, Im) of A: part 0 (cos, sin), part 1 (-sin, cos)
+          unsigned* ph = reinterpret_cast<unsigned*>(&dh);
+          unsigned* pl = reinterpret_cast<unsigned*>(&dl);
 #pragma unroll
           for (int e2 = 0; e2 < 4; ++e2) {
-            const float2 D = twH[mm];
+            const uint2 t = twH[mm];
             mm += ky[i];
             if (mm >= H) mm -= H;
-            const float s0 = (unit & 1) ? D.y : D.x;
-            const float s1 = (unit & 1) ? D.x : -D.y;
-            const _Float16 h0 = (_Float16)s0, h1 = (_Float16)s1;
-            ah[i][2 * e2] = h0;
-            ah[i][2 * e2 + 1] = h1;
-            al[i][2 * e2] = (_Float16)(s0 - (float)h0);
-            al[i][2 * e2 + 1] = (_Float16)(s1 - (float)h1);
+            ph[e2] = t.x;
+            pl[e2] = t.y;
           }
+          if (unit & 1) {
+#pragma unroll
+            for (int e2 = 0; e2 < 4; ++e2) {
+              ph[e2] = __builtin_amdgcn_alignbit(ph[e2], ph[e2], 16) ^ 0x8000u;
+              pl[e2] = __builtin_amdgcn_alignbit(pl[e2], pl[e2], 16) ^ 0x8000u;
+            }
+          }
+          ah[i] = __builtin_bit_cast(half8, dh);
+          al[i] = __builtin_bit_cast(half8, dl);
         }
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const _Float16* bq = Bs + (16 * u + lr) * PK + 32 * ks + 8 * q;
         const half8 bh = *reinterpret_cast<const half8*>(bq);
-        const half8 bl = *reinterpret_cast<const half8*>(bq + w * PK);
+        const half8 bl = *reinterpret_cast<const half8*>(bq + (w + 1) * PK);
 #pragma unroll
         for (int i = 0; i < NU; ++i) {
           if (wv + 4 * i < 2 * J) {
@@ -912,7 +925,7 @@ __global__ __launch_bounds__(kW3Threads) void k_window_v3(W3Args a, const unsign
     __syncthreads();   // B and the D table reads done before the next round (or the plane) overwrites
   }
 
-  // window plane: F / xscale, then the window factor
+  // window plane: F / xscale
   const float inv = 1.f / a.xscale;
   float* fl = reinterpret_cast<float*>(fbuf);
 #pragma unroll
@@ -923,31 +936,102 @@ __global__ __launch_bounds__(kW3Threads) void k_window_v3(W3Args a, const unsign
 #pragma unroll
       for (int u = 0; u < U; ++u)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) fl[2 * ((16 * jt + 4 * q + e) * w + 16 * u + lr) + part] = acc[i][u][e] * inv;
+        for (int e = 0; e < 4; ++e) fl[2 * ((16 * jt + 4 * q + e) * WP + 16 * u + lr) + part] = acc[i][u][e] * inv;
     }
   }
   __syncthreads();
-  const float2* fac = factor + (size_t)p * (h * w);
-  for (int e = tid; e < h * w; e += NT) fbuf[e] = cmul(fbuf[e], fac[e]);
+  const int t16 = lane >> 2, l4 = lane & 3;   // transform within a wave pass, lane within the transform
+  // inverse transforms along x (rows), window factor on load
+  {
+    constexpr int MM = w / 4, US = MM / 4;
+    const float2* fac = factor + (size_t)p * (h * w);
+    for (int ps = wave; ps < J; ps += 4) {
+      const int j = 16 * ps + t16;
+      float2* row = fbuf + j * WP;
+      float2 v[MM];
+#pragma unroll
+      for (int m = 0; m < MM; ++m) v[m] = cmul(row[l4 + 4 * m], fac[j * w + l4 + 4 * m]);
+      dft_any<MM, true>(v);
+#pragma unroll
+      for (int aa = 1; aa < MM; ++aa) {
+        const float2 tw = twix[l4 * aa];
+        v[aa] = cmul(v[aa], make_float2(tw.x, -tw.y));
+      }
+      wave_lds_sync();
+#pragma unroll
+      for (int aa = 0; aa < MM; aa += 2)
+        *reinterpret_cast<float4*>(row + l4 * MM + aa) = make_float4(v[aa].x, v[aa].y, v[aa + 1].x, v[aa + 1].y);
+      wave_lds_sync();
+      float2 z[US][4];
+#pragma unroll
+      for (int s2 = 0; s2 < US; ++s2)
+#pragma unroll
+        for (int ll = 0; ll < 4; ++ll) z[s2][ll] = row[ll * MM + l4 * US + s2];
+      wave_lds_sync();
+#pragma unroll
+      for (int s2 = 0; s2 < US; ++s2) {
+        dftR<4, true>(z[s2]);
+#pragma unroll
+        for (int bb = 0; bb < 4; ++bb) row[l4 * US + s2 + MM * bb] = z[s2][bb];
+      }
+    }
+  }
   __syncthreads();
-  lds_fft_ip<true, NT>(fbuf, h, w, 1, false, a.ix, twix, a.chunkIx);   // along x
-  lds_fft_ip<true, NT>(fbuf, w, 1, w, true, a.iy, twiy, a.chunkIy);    // along y
-  const float2* rp = ref + (size_t)p * (h * w);
-  float2* out = fields + (size_t)(b * g.P + p) * (h * w);
-  for (int e = tid; e < h * w; e += NT) {
-    const float2 t = cmul(fbuf[e], rp[e]);
-    out[e] = make_float2(-t.x, t.y);
+  // inverse transforms along y (columns), tilt on the way out
+  {
+    constexpr int MM = h / 4, US = MM / 4;
+    const float2* rp = ref + (size_t)p * (h * w);
+    float2* out = fields + (size_t)(b * g.P + p) * (h * w);
+    for (int ps = wave; ps < U; ps += 4) {
+      const int c = 16 * ps + t16;
+      float2* col = fbuf + c;
+      float2 v[MM];
+#pragma unroll
+      for (int m = 0; m < MM; ++m) v[m] = col[(l4 + 4 * m) * WP];
+      dft_any<MM, true>(v);
+#pragma unroll
+      for (int aa = 1; aa < MM; ++aa) {
+        const float2 tw = twiy[l4 * aa];
+        v[aa] = cmul(v[aa], make_float2(tw.x, -tw.y));
+      }
+      wave_lds_sync();
+#pragma unroll
+      for (int aa = 0; aa < MM; ++aa) col[(l4 * MM + aa) * WP] = v[aa];
+      wave_lds_sync();
+      float2 z[US][4];
+#pragma unroll
+      for (int s2 = 0; s2 < US; ++s2)
+#pragma unroll
+        for (int ll = 0; ll < 4; ++ll) z[s2][ll] = col[(ll * MM + l4 * US + s2) * WP];
+#pragma unroll
+      for (int s2 = 0; s2 < US; ++s2) {
+        dftR<4, true>(z[s2]);
+#pragma unroll
+        for (int bb = 0; bb < 4; ++bb) {
+          const int k = l4 * US + s2 + MM * bb;
+          const float2 tt = cmul(z[s2][bb], rp[k * w + c]);
+          out[k * w + c] = make_float2(-tt.x, tt.y);
+        }
+      }
+    }
   }
 }
 
-// Tables of the v3 kernel: [H] exp(-2 pi i m / H), [L][M + 1] exp(-2 pi i l a / NR),
-// [M] exp(-2 pi i a / W), from FP64.
+// Tables of the v3 kernel, from FP64: [H] cos and sin of 2 pi m / H as fp16
+// hi/lo pairs {(cos_hi, sin_hi), (cos_lo, sin_lo)}; [L][M + 1]
+// exp(-2 pi i l a / NR); [M] exp(-2 pi i a / W).
 __global__ void k_w3_tables(int H, int L, int M, float2* __restrict__ tab) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   const int NR = L * M;
   double ang;   // in units of pi
   if (e < H) {
-    ang = -2.0 * e / H;
+    double sn, cs;
+    sincospi(2.0 * e / H, &sn, &cs);
+    const _Float16 ch = (_Float16)cs, sh = (_Float16)sn;
+    const _Float16 cl = (_Float16)(cs - (double)ch), sl = (_Float16)(sn - (double)sh);
+    const half2v hi = {ch, sh}, lo = {cl, sl};
+    reinterpret_cast<uint2*>(tab)[e] = make_uint2(__builtin_bit_cast(unsigned, hi), __builtin_bit_cast(unsigned, lo));
+    return;
   } else if ((e -= H) < L * (M + 1)) {
     const int ll = e / (M + 1), aa = e - ll * (M + 1);
     ang = -2.0 * (double)((ll * aa) % NR) / NR;
@@ -1011,7 +1095,7 @@ static hipError_t w3_launch(W3Args a, const void* frames, const float2* tab, con
 // Calls f<L, M, U, J>() for the specialised shapes; returns false otherwise.
 #define W3_SHAPES(X)                                                            \
   X(8, 16, 5, 5) X(8, 16, 4, 4) X(8, 16, 3, 3) X(8, 16, 6, 6) X(4, 16, 3, 3) \
-  X(4, 16, 4, 4) X(4, 24, 4, 4) X(4, 24, 3, 3) X(4, 24, 5, 5)
+  X(4, 16, 4, 4) X(4, 24, 4, 4)
 
 static bool w3_common(const Geometry& g, int& L, int& M) {
   if (!w3_lm(g.W / 2, L, M)) return false;
@@ -1026,7 +1110,6 @@ bool window_v3_supported(const FusedArgs& a) {
   const Geometry& g = a.g;
   int L, M;
   if (!w3_common(g, L, M)) return false;
-  if (ip_chunk(a.ix, kW3Threads) < 1 || ip_chunk(a.iy, kW3Threads) < 1) return false;
   const int U = g.w / 16, J = g.h / 16;
 #define W3_MATCH(LL, MM, UU, JJ)                                                             \
   if (L == LL && M == MM && U == UU && J == JJ)                                              \
@@ -1048,8 +1131,6 @@ hipError_t launch_window_v3(const FusedArgs& fa, const void* frames_u16, const f
   a.g = g;
   a.ix = fa.ix;
   a.iy = fa.iy;
-  a.chunkIx = ip_chunk(fa.ix, kW3Threads);
-  a.chunkIy = ip_chunk(fa.iy, kW3Threads);
   a.magH = magic_div((unsigned)g.H);
   if (a.magH == 0) return hipErrorNotSupported;
   // |X| <= W * 65535 < 2^(16 + log2 W): scale into [2^14, 2^15) at most
