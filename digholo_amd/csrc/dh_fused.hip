@@ -709,7 +709,7 @@ struct W3Shape {
   static constexpr int RPc = L * LP + (8 - (L * LP) % 16 + 16) % 16;   // per row, = 8 mod 16
   static constexpr int PK = 2 * RR + 8;       // halfs per B column
   static constexpr int EBYTES = 4 * RPW * RPc * 8;
-  static constexpr int BBYTES = 2 * (w + 1) * PK * 2;   // column w takes discarded values
+  static constexpr int BBYTES = 2 * 2 * (w + 1) * PK * 2;   // two buffers; column w takes discarded values
   static constexpr int WP = w + 4;            // field-plane row pitch (complex)
 };
 
@@ -749,7 +749,8 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
   const int c0 = g.c0[p];
 
   float2* E = reinterpret_cast<float2*>(w3_smem) + (wave * RPW + r) * RPc;      // this lane's exchange row
-  _Float16* Bs = reinterpret_cast<_Float16*>(w3_smem + Sh::EBYTES);             // [2][w + 1][PK]
+  _Float16* Bs0 = reinterpret_cast<_Float16*>(w3_smem + Sh::EBYTES);            // [2 buffers][2][w + 1][PK]
+  constexpr int BSTRIDE = 2 * (w + 1) * PK;
   uint2* twH = reinterpret_cast<uint2*>(w3_smem + Sh::EBYTES + Sh::BBYTES);     // [H] {cos, sin} hi, lo
   float2* twA = reinterpret_cast<float2*>(twH + H);                            // [L][M + 1]
   float2* fbuf = reinterpret_cast<float2*>(w3_smem);                            // [h][WP] after the rows
@@ -797,25 +798,26 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
 
   const float sc = 0.5f * a.xscale;           // ev + w od = 2 X
   const int rho = wave * RPW + r;             // this lane's row within a round
-  auto put = [&](int k, float2 X) {
-    const unsigned c0k = (unsigned)(k - c0);
-    const int c = c0k < (unsigned)w ? (int)c0k : w;
-    const float xr = X.x * sc, xi = X.y * sc;
-    const _Float16 rh = (_Float16)xr, ih = (_Float16)xi;
-    const half2v hi = {rh, ih};
-    const half2v lo = {(_Float16)(xr - (float)rh), (_Float16)(xi - (float)ih)};
-    *reinterpret_cast<half2v*>(Bs + c * PK + 2 * rho) = hi;
-    *reinterpret_cast<half2v*>(Bs + (w + 1 + c) * PK + 2 * rho) = lo;
-  };
   auto unpack = [](float2 zk, float2 zp, float2 wk) {
     const float2 ev = make_float2(zk.x + zp.x, zk.y - zp.y);
     const float2 od = make_float2(zk.y + zp.y, zp.x - zk.x);     // -i (zk - conj zp)
     return cadd(ev, cmul(wk, od));
   };
-
-  __syncthreads();   // tables staged
   const int nrd = H / RR;
-  for (int rd = 0; rd < nrd; ++rd) {
+
+  // Row transforms of round rd (data in pre[]) into B buffer Bs; fetches the
+  // next round's data into pre[].
+  auto rows = [&](int rd, _Float16* Bs) {
+    auto put = [&](int k, float2 X) {
+      const unsigned c0k = (unsigned)(k - c0);
+      const int c = c0k < (unsigned)w ? (int)c0k : w;
+      const float xr = X.x * sc, xi = X.y * sc;
+      const _Float16 rh = (_Float16)xr, ih = (_Float16)xi;
+      const half2v hi = {rh, ih};
+      const half2v lo = {(_Float16)(xr - (float)rh), (_Float16)(xi - (float)ih)};
+      *reinterpret_cast<half2v*>(Bs + c * PK + 2 * rho) = hi;
+      *reinterpret_cast<half2v*>(Bs + (w + 1 + c) * PK + 2 * rho) = lo;
+    };
     float2 v[M];
 #pragma unroll
     for (int m = 0; m < M; ++m) v[m] = make_float2((float)(pre[m] & 0xffffu), (float)(pre[m] >> 16));
@@ -873,8 +875,10 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
       const float2 z0 = zl[s][0];
       put(selfpair ? NR : c0 - 1, make_float2(2.f * (z0.x - z0.y), 0.f));
     }
-    __syncthreads();   // B complete
-    // F += D . X over this round's rows
+  };
+
+  // F += D . X over the rows of round rd (B buffer Bs)
+  auto gemm = [&](int rd, const _Float16* Bs) {
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
       half8 ah[NU], al[NU];
@@ -922,7 +926,19 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
         }
       }
     }
-    __syncthreads();   // B and the D table reads done before the next round (or the plane) overwrites
+  };
+
+  // Two B buffers: round rd's rows go to buffer rd & 1, so one barrier per
+  // round separates the writers of a buffer from its readers (a buffer is
+  // rewritten two rounds later, after every wave has passed the barrier that
+  // follows its last reads).
+  __syncthreads();   // tables staged
+  rows(0, Bs0);
+  __syncthreads();
+  for (int rd = 0; rd < nrd; ++rd) {
+    if (rd + 1 < nrd) rows(rd + 1, Bs0 + ((rd + 1) & 1) * BSTRIDE);
+    gemm(rd, Bs0 + (rd & 1) * BSTRIDE);
+    __syncthreads();
   }
 
   // window plane: F / xscale
