@@ -177,6 +177,9 @@ struct Handle {
   int lastPath = -1;        // path the last batch took
   DevBuf<uint16_t> colTab;  // fp16 hi/lo column-DFT operands of path 2
   DevBuf<float2> w3Tab;     // twiddle tables of path 3
+  DevBuf<uint16_t> hgx16, hgy16;   // split-fp16 basis of the fused overlap
+  bool fusedOverlap = true; // path 3 computes split-fp16 coefficients in the window kernel
+  bool lastFusedOverlap = false;
 
   bool stageTiming = false;
   std::vector<std::array<hipEvent_t, 6>> evPool;
@@ -636,6 +639,11 @@ int enqueue_batch(Handle& h) {
       DH_TRY(h.w3Tab.ensure(n3), "alloc");
       DH_TRY(dh::launch_window_v3_tables(g, h.w3Tab.p, h.stream), "v3 tables");
     }
+    if (g.G > 0) {
+      DH_TRY(h.hgx16.ensure(dh::hg_split_elems(g, g.w)), "alloc");
+      DH_TRY(h.hgy16.ensure(dh::hg_split_elems(g, g.h)), "alloc");
+      DH_TRY(dh::launch_hg_split(g, h.hgx.p, h.hgy.p, h.hgx16.p, h.hgy16.p, h.stream), "split basis");
+    }
     DH_TRY(hipStreamSynchronize(h.stream), "tables sync");   // the host mask vector goes out of scope
     h.tabKey = d;
     h.tablesValid = true;
@@ -648,8 +656,13 @@ int enqueue_batch(Handle& h) {
   else if (h.windowMode >= 1 && plan_fused(h, g, fa)) path = 1;
   if (path) {
     // one kernel from camera window to field
+    const bool fuseOv = path == 3 && h.fusedOverlap && c.precision == DIGHOLO_PRECISION_SPLIT16 &&
+                        dh::window_v3_overlap_fits(g);
+    h.lastFusedOverlap = fuseOv;
     if (path == 3)
-      DH_TRY(dh::launch_window_v3(fa, dframes, h.w3Tab.p, h.factor.p, h.ref.p, h.fields.p, h.stream),
+      DH_TRY(dh::launch_window_v3(fa, dframes, h.w3Tab.p, h.factor.p, h.ref.p, h.fields.p,
+                                  fuseOv ? h.hgx16.p : nullptr, fuseOv ? h.hgy16.p : nullptr,
+                                  fuseOv ? h.coefs.p : nullptr, h.stream),
              "fused window (register rows)");
     else if (path == 2)
       DH_TRY(dh::launch_window_mfma(fa, h.twW.p, dframes, h.colTab.p, h.factor.p, h.ref.p, h.fields.p, h.stream),
@@ -675,8 +688,9 @@ int enqueue_batch(Handle& h) {
     DH_TRY(dh::launch_ifft_cols_tilt(g, *pIy, h.win.p, h.ref.p, h.fields.p, h.stream), "IFFT columns");
     if (timing) DH_TRY(hipEventRecord(ev[4], h.stream), "event");
     h.lastPath = 0;
+    h.lastFusedOverlap = false;
   }
-  if (g.G > 0) {
+  if (g.G > 0 && !h.lastFusedOverlap) {
     DH_TRY(dh::launch_overlap_mfma(g, h.fields.p, h.hgx.p, h.hgy.p, h.coefs.p,
                                    c.precision == DIGHOLO_PRECISION_SPLIT16, h.stream), "overlap");
   }
@@ -1382,6 +1396,8 @@ void destroy_handle(Handle* h) {
     h->mask.release();
     h->colTab.release();
     h->w3Tab.release();
+    h->hgx16.release();
+    h->hgy16.release();
     for (auto& kv : h->ffts) kv.second->tw.release();
     h->hCoefs.release();
     h->hFields.release();

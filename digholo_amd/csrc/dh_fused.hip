@@ -597,6 +597,13 @@ hipError_t launch_window_mfma(const FusedArgs& a0, const float2* twW, const void
 // a spare B column instead of being branched around.  A round is 4 RPW window rows (each wave
 // transforms RPW of them); units (j-tile, re/im part) are dealt to the waves.
 //
+// With hgx16 set, the epilogue also computes the HG coefficients from the
+// field plane left in LDS: the split-fp16 separable overlap of k_overlap_mfma
+// (field scaled by a power of two to a peak in [128, 256), stage 1 per 16 field
+// rows T = V HGx^T for the real and imaginary parts, stage 2 C += T^T HGy^T),
+// with the basis operands read from L2 and the stage-1 output double-buffered
+// in LDS, so coefficients leave the kernel with the fields.
+//
 // Epilogue: the window plane goes to LDS (row pitch w + 4), and each inverse
 // transform of N = 16 U points is done by 4 lanes holding N/4 points (the
 // same two-pass split, in place in the plane), 16 transforms per wave pass:
@@ -723,10 +730,178 @@ __host__ __device__ inline int w3_region(int ebytes, int bbytes, int H, int L, i
 struct W3Args {
   Geometry g;
   FFTPlan ix, iy;                     // twiddle tables of the w- and h-point transforms
+  // fused split-fp16 HG overlap (null hgx16: coefficients computed elsewhere)
+  const _Float16* hgx16;              // [P][hi/lo][Gp][w], rows m >= G zero
+  const _Float16* hgy16;              // [P][hi/lo][Gp][h]
+  float2* coefs;                      // [B][P][M]
   int region;         // bytes of the aliased row-phase / field-plane region
+  int ovlOff;         // byte offset of the overlap's stage-1 buffers (after the plane)
   unsigned magH;      // q / H == __umulhi(q, magH) for q < 2^20
   float xscale;       // power of two keeping the row spectra inside fp16
 };
+
+constexpr int kOvPK = 24;   // halfs per stage-1 output row (16 field rows + pad)
+constexpr int kOvMaxTiles = 8;
+
+// Split-fp16 separable overlap of the h x w field in the LDS plane (pitch
+// w + 4), 256 threads; stage-1 buffers at st1 (2 x [hi/lo][2 Gp][kOvPK] halfs).
+// Writes coefs[b][p][mode], modes ordered by group g, k = 0..g (m = g - k, n = k).
+template <int U, int J>
+__device__ __forceinline__ void w3_overlap(const W3Args& a, const float2* plane, float vmax, float* red,
+                                           unsigned char* st1, int p, int b, int wave, int lane) {
+  constexpr int w = 16 * U, h = 16 * J, WP = w + 4;
+  const Geometry& g = a.g;
+  const int G = g.G, Gp = (G + 15) & ~15, ct1 = Gp / 16;
+  const int lr = lane & 15, q = lane >> 4;
+  // per-field power-of-two scale: peak magnitude into [128, 256)
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o));
+  if (lane == 0) red[wave] = vmax;
+  __syncthreads();
+  vmax = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+  float scale = 1.f;
+  if (vmax > 0.f) {
+    int ex;
+    frexpf(vmax, &ex);
+    scale = ldexpf(1.f, 8 - ex);
+  }
+  const _Float16* bx = a.hgx16 + (size_t)p * 2 * Gp * w;
+  const _Float16* by = a.hgy16 + (size_t)p * 2 * Gp * h;
+  _Float16* A2b = reinterpret_cast<_Float16*>(st1);
+  const int bufH = 2 * 2 * Gp * kOvPK;   // halfs per buffer
+  // stage-2 tiles of this wave: rows (m, part) in 16s, columns n in 16s
+  const int rt2 = (2 * Gp) / 16, ct2 = Gp / 16;
+  int myTile[kOvMaxTiles];
+  int nMine = 0;
+  {
+    int cnt = 0;
+    for (int t = 0; t < rt2 * ct2; ++t) {
+      const int mt = t / ct2, nt = t - mt * ct2;
+      if (8 * mt + 16 * nt >= G) continue;   // no (m, n) of the tile has m + n < G
+      if (cnt % 4 == wave && nMine < kOvMaxTiles) myTile[nMine++] = t;
+      ++cnt;
+    }
+  }
+  floatx4 cacc[kOvMaxTiles];
+#pragma unroll
+  for (int i = 0; i < kOvMaxTiles; ++i) cacc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
+  auto split8 = [&](const float* v, half8& hi, half8& lo) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const float x = v[e] * scale;
+      const _Float16 hh = (_Float16)x;
+      hi[e] = hh;
+      lo[e] = (_Float16)(x - (float)hh);
+    }
+  };
+  for (int ch = 0; ch < J; ++ch) {
+    _Float16* A2 = A2b + (ch & 1) * bufH;
+    // stage 1: T_part[yy][m] = sum_x V_part[16 ch + yy][x] HGx[m][x]
+    for (int mt = wave; mt < ct1; mt += 4) {
+      floatx4 dre = {0.f, 0.f, 0.f, 0.f}, dim = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < (w + 31) / 32; ++ks) {
+        const int kb = 32 * ks + 8 * q;
+        half8 arh, arl, aih, ail, bh, bl;
+        if (kb < w) {   // a ragged last step (w % 32 == 16) zero-fills the upper lane groups
+          const float4* vp = reinterpret_cast<const float4*>(plane + (16 * ch + lr) * WP + kb);
+          float re[8], im[8];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const float4 t = vp[e];
+            re[2 * e] = t.x;
+            im[2 * e] = t.y;
+            re[2 * e + 1] = t.z;
+            im[2 * e + 1] = t.w;
+          }
+          split8(re, arh, arl);
+          split8(im, aih, ail);
+          const _Float16* bp = bx + (16 * mt + lr) * w + kb;
+          bh = *reinterpret_cast<const half8*>(bp);
+          bl = *reinterpret_cast<const half8*>(bp + Gp * w);
+        } else {
+#pragma unroll
+          for (int e = 0; e < 8; ++e) arh[e] = arl[e] = aih[e] = ail[e] = bh[e] = bl[e] = (_Float16)0.f;
+        }
+        dre = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bh, dre, 0, 0, 0);
+        dre = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bl, dre, 0, 0, 0);
+        dre = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, bh, dre, 0, 0, 0);
+        dim = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bh, dim, 0, 0, 0);
+        dim = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bl, dim, 0, 0, 0);
+        dim = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, bh, dim, 0, 0, 0);
+      }
+      // D[yy = 4q + i][m = 16 mt + lr] -> A2[hi/lo][(m, part)][yy]
+      const int m = 16 * mt + lr;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int yy = 4 * q + i;
+        const _Float16 rh = (_Float16)dre[i], ih = (_Float16)dim[i];
+        A2[(2 * m) * kOvPK + yy] = rh;
+        A2[(2 * m + 1) * kOvPK + yy] = ih;
+        A2[2 * Gp * kOvPK + (2 * m) * kOvPK + yy] = (_Float16)(dre[i] - (float)rh);
+        A2[2 * Gp * kOvPK + (2 * m + 1) * kOvPK + yy] = (_Float16)(dim[i] - (float)ih);
+      }
+    }
+    __syncthreads();   // this chunk's A2 is complete (the other buffer is free: see below)
+    // stage 2: C[(m, part)][n] += sum_yy A2[(m, part)][yy] HGy[n][16 ch + yy]
+#pragma unroll
+    for (int i = 0; i < kOvMaxTiles; ++i) {
+      if (i < nMine) {
+        const int t = myTile[i];
+        const int mt = t / ct2, nt = t - mt * ct2;
+        half8 ah, al, bh, bl;
+        if (q < 2) {   // K = 16: lane groups 0, 1 carry yy 0..15, groups 2, 3 are zero
+          const _Float16* arow = A2 + (mt * 16 + lr) * kOvPK + 8 * q;
+          const _Float16* brow = by + (nt * 16 + lr) * h + 16 * ch + 8 * q;
+          ah = *reinterpret_cast<const half8*>(arow);
+          al = *reinterpret_cast<const half8*>(arow + 2 * Gp * kOvPK);
+          bh = *reinterpret_cast<const half8*>(brow);
+          bl = *reinterpret_cast<const half8*>(brow + Gp * h);
+        } else {
+#pragma unroll
+          for (int e = 0; e < 8; ++e) ah[e] = al[e] = bh[e] = bl[e] = (_Float16)0.f;
+        }
+        cacc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, cacc[i], 0, 0, 0);
+        cacc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, cacc[i], 0, 0, 0);
+        cacc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, cacc[i], 0, 0, 0);
+      }
+    }
+    // No barrier here: the next chunk writes the other buffer, and this one is
+    // rewritten only after the next chunk's barrier, which every wave reaches
+    // after finishing these reads.
+  }
+  // the triangle m + n < G
+  const float inv = 1.f / scale;
+  float2* out = a.coefs + (size_t)(b * g.P + p) * g.M;
+#pragma unroll
+  for (int i = 0; i < kOvMaxTiles; ++i) {
+    if (i < nMine) {
+      const int t = myTile[i];
+      const int mt = t / ct2, nt = t - mt * ct2;
+      const int n = nt * 16 + lr;
+#pragma unroll
+      for (int ii = 0; ii < 4; ii += 2) {
+        const int m = (mt * 16 + 4 * q + ii) >> 1;   // rows ii, ii + 1 are (m, re), (m, im)
+        if (n < G && m < G && m + n < G) {
+          const int gs = m + n;
+          out[gs * (gs + 1) / 2 + n] = make_float2(cacc[i][ii] * inv, cacc[i][ii + 1] * inv);
+        }
+      }
+    }
+  }
+}
+
+// Split basis tables for the fused overlap: dst[p][hl][m][x] for m < Gp (0 for m >= G)
+__global__ void k_hg_split(const float* __restrict__ hg, int P, int G, int n, _Float16* __restrict__ dst) {
+  const int Gp = (G + 15) & ~15;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= P * Gp * n) return;
+  const int pp = e / (Gp * n), rem = e - pp * Gp * n, m = rem / n, x = rem - m * n;
+  const float v = m < G ? hg[((size_t)pp * G + m) * n + x] : 0.f;
+  const _Float16 hi = (_Float16)v;
+  dst[((size_t)pp * 2 + 0) * Gp * n + rem] = hi;
+  dst[((size_t)pp * 2 + 1) * Gp * n + rem] = (_Float16)(v - (float)hi);
+}
 
 template <int L, int M, int U, int J>
 __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const unsigned short* __restrict__ frames,
@@ -994,6 +1169,9 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
   }
   __syncthreads();
   // inverse transforms along y (columns), tilt on the way out
+  const bool ovl = a.hgx16 != nullptr;
+  float vmax = 0.f;
+  float* red = reinterpret_cast<float*>(twiy + h);   // [4] per-wave peaks
   {
     constexpr int MM = h / 4, US = MM / 4;
     const float2* rp = ref + (size_t)p * (h * w);
@@ -1019,6 +1197,7 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
       for (int s2 = 0; s2 < US; ++s2)
 #pragma unroll
         for (int ll = 0; ll < 4; ++ll) z[s2][ll] = col[(ll * MM + l4 * US + s2) * WP];
+      wave_lds_sync();
 #pragma unroll
       for (int s2 = 0; s2 < US; ++s2) {
         dftR<4, true>(z[s2]);
@@ -1026,11 +1205,17 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
         for (int bb = 0; bb < 4; ++bb) {
           const int k = l4 * US + s2 + MM * bb;
           const float2 tt = cmul(z[s2][bb], rp[k * w + c]);
-          out[k * w + c] = make_float2(-tt.x, tt.y);
+          const float2 V = make_float2(-tt.x, tt.y);
+          out[k * w + c] = V;
+          if (ovl) {   // the field stays in the plane for the overlap
+            col[k * WP] = V;
+            vmax = fmaxf(vmax, fmaxf(fabsf(V.x), fabsf(V.y)));
+          }
         }
       }
     }
   }
+  if (ovl) w3_overlap<U, J>(a, fbuf, vmax, red, w3_smem + a.ovlOff, p, b, wave, lane);
 }
 
 // Tables of the v3 kernel, from FP64: [H] cos and sin of 2 pi m / H as fp16
@@ -1085,8 +1270,21 @@ hipError_t launch_window_v3_tables(const Geometry& g, float2* tab, hipStream_t s
 
 template <int L, int M, int U, int J>
 static size_t w3_lds(const W3Args& a) {
-  return (size_t)a.region + 8 * (size_t)(16 * U + 16 * J);
+  return (size_t)a.region + 8 * (size_t)(16 * U + 16 * J) + 16;   // + twiddles, per-wave peaks
 }
+
+// stage-2 tiles of the overlap (rows (m, part) x columns n with some m + n < G)
+static int ov_tiles(int G) {
+  const int Gp = (G + 15) & ~15, rt2 = (2 * Gp) / 16, ct2 = Gp / 16;
+  int cnt = 0;
+  for (int t = 0; t < rt2 * ct2; ++t)
+    if (8 * (t / ct2) + 16 * (t % ct2) < G) ++cnt;
+  return cnt;
+}
+
+bool window_v3_overlap_fits(const Geometry& g) { return g.G > 0 && ov_tiles(g.G) <= 4 * kOvMaxTiles; }
+
+static int w3_ovl_bytes(const Geometry& g) { return 2 * (2 * 2 * ((g.G + 15) & ~15) * kOvPK * 2); }
 
 template <int L, int M, int U, int J>
 static hipError_t w3_launch(W3Args a, const void* frames, const float2* tab, const float2* factor,
@@ -1095,6 +1293,8 @@ static hipError_t w3_launch(W3Args a, const void* frames, const float2* tab, con
   const Geometry& g = a.g;
   if (g.H % Sh::RR) return hipErrorNotSupported;
   a.region = w3_region(Sh::EBYTES, Sh::BBYTES, g.H, L, M, Sh::w, Sh::h);
+  a.ovlOff = 8 * Sh::h * Sh::WP;
+  if (a.hgx16) a.region = std::max(a.region, a.ovlOff + w3_ovl_bytes(g));
   const size_t lds = w3_lds<L, M, U, J>(a);
   if (lds > 160 * 1024) return hipErrorNotSupported;
   if (lds > 64 * 1024) {
@@ -1105,6 +1305,18 @@ static hipError_t w3_launch(W3Args a, const void* frames, const float2* tab, con
   dim3 grid(g.P, g.B);
   k_window_v3<L, M, U, J><<<grid, kW3Threads, lds, s>>>(a, static_cast<const unsigned short*>(frames), tab, factor,
                                                         ref, fields);
+  return hipGetLastError();
+}
+
+size_t hg_split_elems(const Geometry& g, int n) { return (size_t)g.P * 2 * ((g.G + 15) & ~15) * n; }
+
+hipError_t launch_hg_split(const Geometry& g, const float* hgx, const float* hgy, void* hgx16, void* hgy16,
+                           hipStream_t s) {
+  if (g.G <= 0) return hipSuccess;
+  const int Gp = (g.G + 15) & ~15;
+  const int nx = g.P * Gp * g.w, ny = g.P * Gp * g.h;
+  k_hg_split<<<(nx + 255) / 256, 256, 0, s>>>(hgx, g.P, g.G, g.w, static_cast<_Float16*>(hgx16));
+  k_hg_split<<<(ny + 255) / 256, 256, 0, s>>>(hgy, g.P, g.G, g.h, static_cast<_Float16*>(hgy16));
   return hipGetLastError();
 }
 
@@ -1138,8 +1350,10 @@ bool window_v3_supported(const FusedArgs& a) {
 }
 
 hipError_t launch_window_v3(const FusedArgs& fa, const void* frames_u16, const float2* tab, const float2* factor,
-                            const float2* ref, float2* fields, hipStream_t s) {
+                            const float2* ref, float2* fields, const void* hgx16, const void* hgy16, float2* coefs,
+                            hipStream_t s) {
   if (!window_v3_supported(fa)) return hipErrorNotSupported;
+  if (hgx16 && !window_v3_overlap_fits(fa.g)) return hipErrorNotSupported;
   const Geometry& g = fa.g;
   int L, M;
   w3_lm(g.W / 2, L, M);
@@ -1147,6 +1361,9 @@ hipError_t launch_window_v3(const FusedArgs& fa, const void* frames_u16, const f
   a.g = g;
   a.ix = fa.ix;
   a.iy = fa.iy;
+  a.hgx16 = static_cast<const _Float16*>(hgx16);
+  a.hgy16 = static_cast<const _Float16*>(hgy16);
+  a.coefs = coefs;
   a.magH = magic_div((unsigned)g.H);
   if (a.magH == 0) return hipErrorNotSupported;
   // |X| <= W * 65535 < 2^(16 + log2 W): scale into [2^14, 2^15) at most

@@ -98,8 +98,16 @@ hipError_t launch_window_mfma(const FusedArgs& a, const float2* twW, const void*
 bool window_v3_supported(const FusedArgs& a);
 size_t window_v3_table_elems(const Geometry& g);
 hipError_t launch_window_v3_tables(const Geometry& g, float2* tab, hipStream_t s);
+// With hgx16/hgy16 (split basis from launch_hg_split) and coefs set, the same
+// launch also computes the split-fp16 HG overlap into coefs.
 hipError_t launch_window_v3(const FusedArgs& a, const void* frames_u16, const float2* tab, const float2* factor,
-                            const float2* ref, float2* fields, hipStream_t s);
+                            const float2* ref, float2* fields, const void* hgx16, const void* hgy16, float2* coefs,
+                            hipStream_t s);
+bool window_v3_overlap_fits(const Geometry& g);
+// Split-fp16 basis [P][hi/lo][Gp][n] (Gp = G rounded up to 16) from hgx/hgy.
+size_t hg_split_elems(const Geometry& g, int n);
+hipError_t launch_hg_split(const Geometry& g, const float* hgx, const float* hgy, void* hgx16, void* hgy16,
+                           hipStream_t s);
 
 // Power-meter reductions (dh_metrics.hip).  out: [S][N][5] doubles per row
 // {diagonal, off-diagonal, in-group/g, out-of-group/g, total power}.
