@@ -46,6 +46,21 @@ def overlap_flops_per_frame(P, G, w, h):
     return P * 4 * (G * w * h + M * h)
 
 
+def measured_traffic(kernel, fused_overlap):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (scaled by a copy kernel of known size), or
+    None when no summary for this kernel and mode is committed."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    key = kernel + (":fused_overlap" if fused_overlap else "")
+    ent = rec.get(key)
+    return ent.get("bytes_per_launch") if isinstance(ent, dict) else None
+
+
 def make_frames(n_unique=100, seed=1):
     """n_unique rendered holograms; every one of the B frames gets its own read noise."""
     import numpy as np
@@ -96,6 +111,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", choices=["fp32", "split16"], default="split16")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="for rocprofv3 --pmc: two batches with the fused split overlap, two with the FP32 "
+                         "overlap, then a 512 MiB device copy that scales the counters; prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -123,6 +141,19 @@ def main():
 
     step()
     assert lib.digHoloSynchronize(h) == 0
+    if args.traffic_probe:
+        for prec in (api.DIGHOLO_PRECISION_SPLIT16, api.DIGHOLO_PRECISION_SPLIT16,
+                     api.DIGHOLO_PRECISION_FP32, api.DIGHOLO_PRECISION_FP32):
+            assert lib.digHoloConfigSetOverlapPrecision(h, prec) == 0
+            step()
+            assert lib.digHoloSynchronize(h) == 0
+        x = torch.empty(128 * 1024 * 1024, dtype=torch.int32, device="cuda").fill_(1)
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(f"traffic probe done, window path {lib.digHoloGetWindowPath(h)}", flush=True)
+        return
     # time on the stream the kernels run on (the handle's own HIP stream)
     stream = torch.cuda.ExternalStream(lib.digHoloGetStream(h))
 
@@ -154,10 +185,19 @@ def main():
     Wd, Hd, w, hh, M = geo["W"], geo["H"], geo["w"], geo["h"], m.value
     frames_s = B / (ms_per_step * 1e-3)
     path = lib.digHoloGetWindowPath(h)
+    fused_overlap = headline == api.DIGHOLO_PRECISION_SPLIT16 and path == 3 and st[4] < 0.05 * st[0]
     fft_ms = st[0] + st[1] + st[2]          # camera window -> fields (one kernel when fused)
-    fft_bytes = fft_stage_bytes_per_frame(POLS, Wd, Hd, w, hh) * B
+    if fused_overlap:   # the window kernel also writes the coefficients
+        fft_bytes = algorithmic_bytes_per_frame(POLS, Wd, Hd, w, hh, M) * B
+    else:
+        fft_bytes = fft_stage_bytes_per_frame(POLS, Wd, Hd, w, hh) * B
     achieved = fft_bytes / (fft_ms * 1e-3) / 1e9
     ov_flops = overlap_flops_per_frame(POLS, GROUPS, w, hh) * B
+    kernel = {3: "k_window_v3", 2: "k_window_mfma", 1: "k_fused_window"}.get(
+        path, "k_row_fft + k_col_fft_window + k_ifft_rows + k_ifft_cols_tilt")
+    traffic = measured_traffic(kernel, fused_overlap)
+    # split overlap cost: its own kernel, or what fusing it adds to the window kernel
+    ov16 = st16[4] if st16[4] >= 0.05 * st16[0] else max(st16[0] - st32[0], 0.0)
     out = {
         "metric": METRIC,
         "value": round(frames_s, 1),
@@ -176,20 +216,19 @@ def main():
                    "global_batch": B, "overlap_precision": "split16" if headline else "fp32",
                    "fft_window": Wd, "ifft_window": w, "modes_per_pol": M, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm",
-                     "kernel": {2: "k_window_mfma", 1: "k_fused_window"}[path] if path in (1, 2) else
-                     "k_row_fft + k_col_fft_window + k_ifft_rows + k_ifft_cols_tilt",
+                     "kernel": kernel + (" (window + split-fp16 overlap)" if fused_overlap else ""),
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": fft_bytes, "avg_launch_ms": round(fft_ms, 4)},
         "fp32_overlap": {"frames_s": round(B / (ms32 * 1e-3), 1), "ms_per_step": round(ms32, 4),
                          "overlap_ms": round(st32[4], 4)},
         "split16_overlap": {"frames_s": round(B / (ms16 * 1e-3), 1), "ms_per_step": round(ms16, 4),
-                            "overlap_ms": round(st16[4], 4)},
+                            "overlap_ms": round(ov16, 4), "fused_into_window_kernel": fused_overlap},
         "split_vs_fp32_end_to_end": round(ms32 / ms16, 3),
-        "split_vs_fp32_overlap_kernel": round(st32[4] / st16[4], 3) if st16[4] > 0 else None,
-        "stage_ms": {"row_fft": st[0], "col_fft_window": st[1], "ifft_tilt": st[2], "tables": st[3],
+        "split_vs_fp32_overlap_stage": round(st32[4] / ov16, 3) if ov16 > 0 else None,
+        "stage_ms": {"window": st[0], "col_fft_window": st[1], "ifft_tilt": st[2], "tables": st[3],
                      "overlap": st[4]},
-        "overlap_tflops": round(ov_flops / (st[4] * 1e-3) / 1e12, 3) if st[4] > 0 else None,
+        "overlap_tflops": round(ov_flops / (ov16 * 1e-3) / 1e12, 3) if ov16 > 0 else None,
         "end_to_end_hbm_frac": round(algorithmic_bytes_per_frame(POLS, Wd, Hd, w, hh, M) * frames_s / 1e9
                                      / HBM_PEAK_GBS, 4),
         "host_wall_s": round(wall, 3),

@@ -741,7 +741,7 @@ struct W3Args {
 };
 
 constexpr int kOvPK = 24;   // halfs per stage-1 output row (16 field rows + pad)
-constexpr int kOvMaxTiles = 8;
+constexpr int kOvMaxTiles = 4;   // stage-2 tiles per wave (G up to ~50)
 
 // Split-fp16 separable overlap of the h x w field in the LDS plane (pitch
 // w + 4), 256 threads; stage-1 buffers at st1 (2 x [hi/lo][2 Gp][kOvPK] halfs).
@@ -794,52 +794,89 @@ __device__ __forceinline__ void w3_overlap(const W3Args& a, const float2* plane,
       lo[e] = (_Float16)(x - (float)hh);
     }
   };
-  for (int ch = 0; ch < J; ++ch) {
-    _Float16* A2 = A2b + (ch & 1) * bufH;
-    // stage 1: T_part[yy][m] = sum_x V_part[16 ch + yy][x] HGx[m][x]
-    for (int mt = wave; mt < ct1; mt += 4) {
-      floatx4 dre = {0.f, 0.f, 0.f, 0.f}, dim = {0.f, 0.f, 0.f, 0.f};
+  // stage-1 units (m-tile, part) of this wave and their basis operands, which
+  // are the same for every chunk
+  constexpr int KS1 = (w + 31) / 32;
+  constexpr int NU1 = 2;                       // units per wave: 2 ct1 <= 8 (Gp <= 64)
+  half8 b1h[NU1][KS1], b1l[NU1][KS1];
 #pragma unroll
-      for (int ks = 0; ks < (w + 31) / 32; ++ks) {
+  for (int i = 0; i < NU1; ++i) {
+    const int u1 = wave + 4 * i;
+    if (u1 < 2 * ct1) {
+      const int mt = u1 >> 1;
+#pragma unroll
+      for (int ks = 0; ks < KS1; ++ks) {
         const int kb = 32 * ks + 8 * q;
-        half8 arh, arl, aih, ail, bh, bl;
-        if (kb < w) {   // a ragged last step (w % 32 == 16) zero-fills the upper lane groups
-          const float4* vp = reinterpret_cast<const float4*>(plane + (16 * ch + lr) * WP + kb);
-          float re[8], im[8];
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const float4 t = vp[e];
-            re[2 * e] = t.x;
-            im[2 * e] = t.y;
-            re[2 * e + 1] = t.z;
-            im[2 * e + 1] = t.w;
-          }
-          split8(re, arh, arl);
-          split8(im, aih, ail);
+        if (kb < w) {
           const _Float16* bp = bx + (16 * mt + lr) * w + kb;
-          bh = *reinterpret_cast<const half8*>(bp);
-          bl = *reinterpret_cast<const half8*>(bp + Gp * w);
+          b1h[i][ks] = *reinterpret_cast<const half8*>(bp);
+          b1l[i][ks] = *reinterpret_cast<const half8*>(bp + Gp * w);
         } else {
 #pragma unroll
-          for (int e = 0; e < 8; ++e) arh[e] = arl[e] = aih[e] = ail[e] = bh[e] = bl[e] = (_Float16)0.f;
+          for (int e = 0; e < 8; ++e) b1h[i][ks][e] = b1l[i][ks][e] = (_Float16)0.f;
         }
-        dre = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bh, dre, 0, 0, 0);
-        dre = __builtin_amdgcn_mfma_f32_16x16x32_f16(arh, bl, dre, 0, 0, 0);
-        dre = __builtin_amdgcn_mfma_f32_16x16x32_f16(arl, bh, dre, 0, 0, 0);
-        dim = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bh, dim, 0, 0, 0);
-        dim = __builtin_amdgcn_mfma_f32_16x16x32_f16(aih, bl, dim, 0, 0, 0);
-        dim = __builtin_amdgcn_mfma_f32_16x16x32_f16(ail, bh, dim, 0, 0, 0);
       }
-      // D[yy = 4q + i][m = 16 mt + lr] -> A2[hi/lo][(m, part)][yy]
-      const int m = 16 * mt + lr;
+    }
+  }
+  // stage-2 basis operands of chunk ch (loaded before the chunk's stage 1)
+  half8 b2h[kOvMaxTiles], b2l[kOvMaxTiles];
+  auto load_b2 = [&](int ch) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int yy = 4 * q + i;
-        const _Float16 rh = (_Float16)dre[i], ih = (_Float16)dim[i];
-        A2[(2 * m) * kOvPK + yy] = rh;
-        A2[(2 * m + 1) * kOvPK + yy] = ih;
-        A2[2 * Gp * kOvPK + (2 * m) * kOvPK + yy] = (_Float16)(dre[i] - (float)rh);
-        A2[2 * Gp * kOvPK + (2 * m + 1) * kOvPK + yy] = (_Float16)(dim[i] - (float)ih);
+    for (int i = 0; i < kOvMaxTiles; ++i) {
+      if (i < nMine) {
+        const int nt = myTile[i] % ct2;
+        if (q < 2) {
+          const _Float16* brow = by + (nt * 16 + lr) * h + 16 * ch + 8 * q;
+          b2h[i] = *reinterpret_cast<const half8*>(brow);
+          b2l[i] = *reinterpret_cast<const half8*>(brow + Gp * h);
+        } else {
+#pragma unroll
+          for (int e = 0; e < 8; ++e) b2h[i][e] = b2l[i][e] = (_Float16)0.f;
+        }
+      }
+    }
+  };
+  for (int ch = 0; ch < J; ++ch) {
+    _Float16* A2 = A2b + (ch & 1) * bufH;
+    load_b2(ch);
+    // stage 1: T_part[yy][m] = sum_x V_part[16 ch + yy][x] HGx[m][x]
+#pragma unroll
+    for (int i = 0; i < NU1; ++i) {
+      const int u1 = wave + 4 * i;
+      if (u1 < 2 * ct1) {
+        const int mt = u1 >> 1, part = u1 & 1;
+        floatx4 d = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < KS1; ++ks) {
+          const int kb = 32 * ks + 8 * q;
+          half8 ah, al;
+          if (kb < w) {   // a ragged last step (w % 32 == 16) zero-fills the upper lane groups
+            const float4* vp = reinterpret_cast<const float4*>(plane + (16 * ch + lr) * WP + kb);
+            float vv[8];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const float4 t = vp[e];
+              vv[2 * e] = part ? t.y : t.x;
+              vv[2 * e + 1] = part ? t.w : t.z;
+            }
+            split8(vv, ah, al);
+          } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) ah[e] = al[e] = (_Float16)0.f;
+          }
+          d = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, b1h[i][ks], d, 0, 0, 0);
+          d = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, b1l[i][ks], d, 0, 0, 0);
+          d = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, b1h[i][ks], d, 0, 0, 0);
+        }
+        // D[yy = 4q + e][m = 16 mt + lr] -> A2[hi/lo][(m, part)][yy]
+        const int row = 2 * (16 * mt + lr) + part;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int yy = 4 * q + e;
+          const _Float16 dh = (_Float16)d[e];
+          A2[row * kOvPK + yy] = dh;
+          A2[2 * Gp * kOvPK + row * kOvPK + yy] = (_Float16)(d[e] - (float)dh);
+        }
       }
     }
     __syncthreads();   // this chunk's A2 is complete (the other buffer is free: see below)
@@ -847,23 +884,19 @@ __device__ __forceinline__ void w3_overlap(const W3Args& a, const float2* plane,
 #pragma unroll
     for (int i = 0; i < kOvMaxTiles; ++i) {
       if (i < nMine) {
-        const int t = myTile[i];
-        const int mt = t / ct2, nt = t - mt * ct2;
-        half8 ah, al, bh, bl;
+        const int mt = myTile[i] / ct2;
+        half8 ah, al;
         if (q < 2) {   // K = 16: lane groups 0, 1 carry yy 0..15, groups 2, 3 are zero
           const _Float16* arow = A2 + (mt * 16 + lr) * kOvPK + 8 * q;
-          const _Float16* brow = by + (nt * 16 + lr) * h + 16 * ch + 8 * q;
           ah = *reinterpret_cast<const half8*>(arow);
           al = *reinterpret_cast<const half8*>(arow + 2 * Gp * kOvPK);
-          bh = *reinterpret_cast<const half8*>(brow);
-          bl = *reinterpret_cast<const half8*>(brow + Gp * h);
         } else {
 #pragma unroll
-          for (int e = 0; e < 8; ++e) ah[e] = al[e] = bh[e] = bl[e] = (_Float16)0.f;
+          for (int e = 0; e < 8; ++e) ah[e] = al[e] = (_Float16)0.f;
         }
-        cacc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, cacc[i], 0, 0, 0);
-        cacc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, cacc[i], 0, 0, 0);
-        cacc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, cacc[i], 0, 0, 0);
+        cacc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, b2h[i], cacc[i], 0, 0, 0);
+        cacc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, b2l[i], cacc[i], 0, 0, 0);
+        cacc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, b2h[i], cacc[i], 0, 0, 0);
       }
     }
     // No barrier here: the next chunk writes the other buffer, and this one is
@@ -1282,7 +1315,9 @@ static int ov_tiles(int G) {
   return cnt;
 }
 
-bool window_v3_overlap_fits(const Geometry& g) { return g.G > 0 && ov_tiles(g.G) <= 4 * kOvMaxTiles; }
+bool window_v3_overlap_fits(const Geometry& g) {
+  return g.G > 0 && ov_tiles(g.G) <= 4 * kOvMaxTiles && ((g.G + 15) & ~15) <= 64;
+}
 
 static int w3_ovl_bytes(const Geometry& g) { return 2 * (2 * 2 * ((g.G + 15) & ~15) * kOvPK * 2); }
 
