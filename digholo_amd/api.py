@@ -47,6 +47,11 @@ SIGNATURES: Dict[str, Tuple[object, List[object]]] = {
     "digHoloSetBatchUint16": (c_int, [c_int, c_int, c_void_p, c_int]),
     "digHoloSetBatchAvgUint16": (c_int, [c_int, c_int, c_void_p, c_int, c_int, c_int]),
     "digHoloConfigSetBatchCount": (c_int, [c_int, c_int]),
+    "digHoloConfigSetBatchCalibration": (c_int, [c_int, c_void_p, c_int, c_int]),
+    "digHoloConfigGetBatchCalibration": (c_void_p, [c_int, P_int, P_int]),
+    "digHoloConfigSetBatchCalibrationFromFile": (c_int, [c_int, C.c_char_p, c_int, c_int]),
+    "digHoloConfigSetBatchCalibrationEnabled": (c_int, [c_int, c_int]),
+    "digHoloConfigGetBatchCalibrationEnabled": (c_int, [c_int]),
     "digHoloConfigGetBatchCount": (c_int, [c_int]),
     "digHoloConfigSetBatchAvgCount": (c_int, [c_int, c_int]),
     "digHoloConfigGetBatchAvgCount": (c_int, [c_int]),
@@ -115,6 +120,10 @@ SIGNATURES: Dict[str, Tuple[object, List[object]]] = {
     "digHoloProcessBatch": (c_void_p, [c_int, P_int, P_int, P_int]),
     "digHoloGetFields": (c_void_p, [c_int, P_int, P_int, PP_float, PP_float, P_int, P_int]),
     "digHoloBasisGetCoefs": (c_void_p, [c_int, P_int, P_int, P_int]),
+    "digHoloProcessFFT": (c_int, [c_int]),
+    "digHoloProcessIFFT": (c_int, [c_int]),
+    "digHoloProcessRemoveTilt": (c_int, [c_int]),
+    "digHoloProcessBasisExtractCoefs": (c_int, [c_int]),
     "digHoloConfigSetThreadCount": (c_int, [c_int, c_int]),
     "digHoloConfigGetThreadCount": (c_int, [c_int]),
     "digHoloBenchmark": (c_float, [c_int, c_float, P_float]),
@@ -219,6 +228,31 @@ def window_geometry(lib, h) -> Dict[str, object]:
     v = list(out)
     return dict(W=v[0], H=v[1], w=v[2], h=v[3], cx=[v[4], v[6]], cy=[v[5], v[7]], SX=[v[8], v[10]],
                 SY=[v[9], v[11]])
+
+
+def get_coefs(lib, h):
+    """digHoloBasisGetCoefs as a [B, pol, M] complex64 array (None when there is no basis)."""
+    b, m, p = c_int(), c_int(), c_int()
+    ptr = lib.digHoloBasisGetCoefs(h, C.byref(b), C.byref(m), C.byref(p))
+    if not ptr:
+        return None
+    out = np.ctypeslib.as_array(C.cast(ptr, P_float), shape=(b.value * p.value * m.value * 2,))
+    return out.view(np.complex64).reshape(b.value, p.value, m.value).copy()
+
+
+def get_fields(lib, h):
+    """digHoloGetFields: (fields [B, pol, h, w] complex64, x, y)."""
+    fb, fp, w, hh = c_int(), c_int(), c_int(), c_int()
+    xp, yp = P_float(), P_float()
+    fptr = lib.digHoloGetFields(h, C.byref(fb), C.byref(fp), C.byref(xp), C.byref(yp), C.byref(w), C.byref(hh))
+    if not fptr:
+        raise RuntimeError("digHoloGetFields failed")
+    n = fb.value * fp.value * hh.value * w.value
+    fields = np.ctypeslib.as_array(C.cast(fptr, P_float), shape=(2 * n,)).view(np.complex64)
+    fields = fields.reshape(fb.value, fp.value, hh.value, w.value).copy()
+    x = np.ctypeslib.as_array(xp, shape=(w.value,)).copy()
+    y = np.ctypeslib.as_array(yp, shape=(hh.value,)).copy()
+    return fields, x, y
 
 
 def process_batch(lib, h, frames: np.ndarray):

@@ -58,6 +58,10 @@ struct Config {
   int verbosity = 0;
   int threadCount = (int)std::thread::hardware_concurrency();
   int precision = DIGHOLO_PRECISION_FP32;
+  // batch calibration (digHolo.cpp:4469-4530): [calP][calB] complex, enabled flag
+  std::vector<float2> cal;
+  int calP = 0, calB = 0, calEnabled = 0;
+  unsigned calVersion = 0;
 };
 
 template <typename T>
@@ -150,6 +154,16 @@ struct Handle {
   std::vector<float> aaMetrics;   // DIGHOLO_METRIC_COUNT x (lambdaCount + 1)
   int aaLambdaCount = 0;
   bool coefsHostValid = false, fieldsHostValid = false;
+  bool coefsDevValid = false;   // coefs belong to the published fields
+  // pipeline run step by step (ProcessFFT -> ProcessIFFT -> ProcessRemoveTilt ->
+  // ProcessBasisExtractCoefs): 0 none, 1 Fourier band in rowspec, 2 untilted
+  // field in fields, 3 published
+  int stepState = 0;
+  Derived stepD;
+  int stepA = 1, stepMode = 0;
+  dh::BatchCal stepCal;
+  const void* stepFrames = nullptr;
+  bool stepU16 = false;
   std::vector<float> xAxis, yAxis;
 
   // Plane analysis, computed on request from the last batch's device state:
@@ -177,6 +191,10 @@ struct Handle {
   int lastPath = -1;        // path the last batch took
   DevBuf<uint16_t> colTab;  // fp16 hi/lo column-DFT operands of path 2
   DevBuf<float2> w3Tab;     // twiddle tables of path 3
+  DevBuf<uint16_t> tstage;  // uint16 frames after the transpose pre-pass
+  DevBuf<float2> winAll;    // every frame's Fourier window when averaging
+  DevBuf<float2> calDev;    // batch calibration on the device
+  unsigned calDevVersion = ~0u;
   DevBuf<uint16_t> hgx16, hgy16;   // split-fp16 basis of the fused overlap
   bool fusedOverlap = true; // path 3 computes split-fp16 coefficients in the window kernel
   bool lastFusedOverlap = false;
@@ -536,21 +554,17 @@ int ensure_device(Handle& h) {
 
 // Queues one full batch on the handle's stream.  On return the coefficients and
 // fields are (or will be, once the stream drains) in device memory.
-int enqueue_batch(Handle& h) {
+void publish_batch(Handle& h, const Derived& d, const void* dframes, bool u16);
+
+// step: 0 the whole batch; 1 the FFT step only (frames -> column-transformed
+// band of the window columns, [B][P][H][w] in rowspec).
+int enqueue_batch(Handle& h, int step = 0) {
   Config& c = h.cfg;
   if (!h.frames) return DIGHOLO_ERROR_NULLPOINTER;
   if (c.batchCount < 1) c.batchCount = 1;
-  if (c.avgCount > 1) {
-    fprintf(stderr, "digHolo: frame averaging (avgCount > 1) is not implemented in this build\n");
-    return DIGHOLO_ERROR_INVALIDARGUMENT;
-  }
   if (c.resolutionIdx != 1) {
     fprintf(stderr, "digHolo: IFFTResolutionMode 0 (full resolution) is not implemented in this build; "
                     "set digHoloConfigSetIFFTResolutionMode(handle, 1)\n");
-    return DIGHOLO_ERROR_INVALIDARGUMENT;
-  }
-  if (h.transpose) {
-    fprintf(stderr, "digHolo: transposed uint16 frames are not implemented in this build\n");
     return DIGHOLO_ERROR_INVALIDARGUMENT;
   }
   if (c.basisType != DIGHOLO_BASISTYPE_HG && c.maxMG > 0) {
@@ -576,7 +590,9 @@ int enqueue_batch(Handle& h) {
   if (e0) return e0;
 
   const size_t B = g.B, P = g.P;
-  const size_t frameElems = B * (size_t)g.frameW * g.frameH;
+  const int A = std::max(1, c.avgCount);      // frames averaged into each batch element
+  const size_t nFrames = B * (size_t)A;
+  const size_t frameElems = nFrames * (size_t)g.frameW * g.frameH;
   const size_t elemBytes = h.frameU16 ? 2 : 4;
   const void* dframes = h.frames;
   if (!is_device_pointer(h.frames) || (reinterpret_cast<uintptr_t>(h.frames) % 8) != 0) {
@@ -591,6 +607,26 @@ int enqueue_batch(Handle& h) {
     }
     dframes = h.staging.p;
   }
+  if (h.transpose && h.frameU16) {   // digHolo.cpp:1206-1229, once per call like the reference's conversion
+    DH_TRY(h.tstage.ensure(frameElems), "alloc");
+    DH_TRY(dh::launch_u16_transpose(dframes, h.tstage.p, g.frameW, g.frameH, nFrames, h.stream), "transpose");
+    dframes = h.tstage.p;
+  }
+  dh::BatchCal cal;
+  if (c.calEnabled && c.calP > 0 && c.calB > 0) {
+    if (h.calDevVersion != c.calVersion) {
+      DH_TRY(h.calDev.ensure(c.cal.size()), "alloc");
+      DH_TRY(hipMemcpyAsync(h.calDev.p, c.cal.data(), c.cal.size() * sizeof(float2), hipMemcpyHostToDevice,
+                            h.stream), "calibration upload");
+      DH_TRY(hipStreamSynchronize(h.stream), "calibration upload");
+      h.calDevVersion = c.calVersion;
+    }
+    cal.p = h.calDev.p;
+    cal.P = c.calP;
+    cal.B = c.calB;
+  }
+  dh::Geometry gA = g;   // every frame of the buffer (averaging reads A per batch element)
+  gA.B = (int)nFrames;
   DH_TRY(h.fields.ensure(B * P * g.h * g.w), "alloc");
   DH_TRY(h.coefs.ensure(std::max<size_t>(B * P * g.M, 1)), "alloc");
 
@@ -649,9 +685,27 @@ int enqueue_batch(Handle& h) {
     h.tablesValid = true;
   }
   if (timing) DH_TRY(hipEventRecord(ev[1], h.stream), "event");
+  if (step == 1) {
+    DH_TRY(h.rowspec.ensure(nFrames * P * g.H * g.w), "alloc");
+    DH_TRY(dh::launch_row_fft(gA, *pRow, h.twW.p, dframes, h.frameU16, h.rowspec.p, h.stream), "row FFT");
+    DH_TRY(dh::launch_col_fft_full(gA, *pCol, h.rowspec.p, h.stream), "column FFT");
+    h.stepA = A;
+    h.stepMode = c.avgMode;
+    h.stepCal = cal;
+    if (timing)
+      for (int k = 2; k < 6; ++k) DH_TRY(hipEventRecord(ev[k], h.stream), "event");
+    h.stepState = 1;
+    h.stepD = d;
+    h.stepFrames = dframes;
+    h.stepU16 = h.frameU16;
+    return DIGHOLO_ERROR_SUCCESS;
+  }
+  h.stepState = 0;
   dh::FusedArgs fa;
   int path = 0;
-  if (h.windowMode >= 3 && h.frameU16 && dh::window_v3_table_elems(g) && plan_window_v3(h, g, fa)) path = 3;
+  if (A > 1) path = 0;   // averaging runs on the staged kernels
+  else if (h.windowMode >= 3 && h.frameU16 && dh::window_v3_table_elems(g) && plan_window_v3(h, g, fa)) path = 3;
+  else if (cal.p) path = 0;   // only the specialised and the staged kernels apply a calibration
   else if (h.windowMode >= 2 && h.frameU16 && plan_window_mfma(h, g, fa)) path = 2;
   else if (h.windowMode >= 1 && plan_fused(h, g, fa)) path = 1;
   if (path) {
@@ -662,7 +716,7 @@ int enqueue_batch(Handle& h) {
     if (path == 3)
       DH_TRY(dh::launch_window_v3(fa, dframes, h.w3Tab.p, h.factor.p, h.ref.p, h.fields.p,
                                   fuseOv ? h.hgx16.p : nullptr, fuseOv ? h.hgy16.p : nullptr,
-                                  fuseOv ? h.coefs.p : nullptr, h.stream),
+                                  fuseOv ? h.coefs.p : nullptr, cal, h.stream),
              "fused window (register rows)");
     else if (path == 2)
       DH_TRY(dh::launch_window_mfma(fa, h.twW.p, dframes, h.colTab.p, h.factor.p, h.ref.p, h.fields.p, h.stream),
@@ -677,12 +731,20 @@ int enqueue_batch(Handle& h) {
       DH_TRY(hipEventRecord(ev[4], h.stream), "event");
     }
   } else {
-    // staged kernels through HBM intermediates (any window size)
-    DH_TRY(h.rowspec.ensure(B * P * g.H * g.w), "alloc");
+    // staged kernels through HBM intermediates (any window size; averaging)
+    DH_TRY(h.rowspec.ensure(nFrames * P * g.H * g.w), "alloc");
     DH_TRY(h.win.ensure(B * P * g.h * g.w), "alloc");
-    DH_TRY(dh::launch_row_fft(g, *pRow, h.twW.p, dframes, h.frameU16, h.rowspec.p, h.stream), "row FFT");
+    DH_TRY(dh::launch_row_fft(gA, *pRow, h.twW.p, dframes, h.frameU16, h.rowspec.p, h.stream), "row FFT");
     if (timing) DH_TRY(hipEventRecord(ev[2], h.stream), "event");
-    DH_TRY(dh::launch_col_fft_window(g, *pCol, h.rowspec.p, h.factor.p, h.win.p, h.stream), "column FFT");
+    if (A > 1) {
+      DH_TRY(h.winAll.ensure(nFrames * P * g.h * g.w), "alloc");
+      DH_TRY(dh::launch_col_fft_window(gA, *pCol, h.rowspec.p, h.factor.p, h.winAll.p, dh::BatchCal{}, h.stream),
+             "column FFT");
+      DH_TRY(dh::launch_average_windows(g, A, c.avgMode, (int)nFrames, h.winAll.p, h.win.p, cal, h.stream),
+             "averaging");
+    } else {
+      DH_TRY(dh::launch_col_fft_window(g, *pCol, h.rowspec.p, h.factor.p, h.win.p, cal, h.stream), "column FFT");
+    }
     if (timing) DH_TRY(hipEventRecord(ev[3], h.stream), "event");
     DH_TRY(dh::launch_ifft_rows(g, *pIx, h.win.p, h.stream), "IFFT rows");
     DH_TRY(dh::launch_ifft_cols_tilt(g, *pIy, h.win.p, h.ref.p, h.fields.p, h.stream), "IFFT columns");
@@ -696,12 +758,20 @@ int enqueue_batch(Handle& h) {
   }
   if (timing) DH_TRY(hipEventRecord(ev[5], h.stream), "event");
 
+  publish_batch(h, d, dframes, h.frameU16);
+  h.coefsDevValid = true;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+// The fields in h.fields now belong to configuration d and frames dframes.
+void publish_batch(Handle& h, const Derived& d, const void* dframes, bool u16) {
+  const dh::Geometry& g = d.g;
   h.cur = d;
   h.haveBatch = true;
   h.coefsHostValid = false;
   h.fieldsHostValid = false;
   h.lastFrames = dframes;
-  h.lastFrameU16 = h.frameU16;
+  h.lastFrameU16 = u16;
   h.analysisValid[0] = h.analysisValid[1] = false;
   h.f16Valid = false;
   // reconstructed-plane axes, float32 like digHolo.cpp:8184-8189 / :2875-2880
@@ -709,6 +779,61 @@ int enqueue_batch(Handle& h) {
   h.yAxis.resize(g.h);
   for (int i = 0; i < g.w; ++i) h.xAxis[i] = (float)(i - g.w / 2) * d.pxx;
   for (int j = 0; j < g.h; ++j) h.yAxis[j] = (float)(j - g.h / 2) * d.pxy;
+}
+
+// IFFT step: window rows of the band times the window factor, 2-D inverse FFT,
+// field before tilt removal into h.fields.
+int step_ifft(Handle& h) {
+  if (h.stepState < 1) return DIGHOLO_ERROR_NULLPOINTER;
+  const dh::Geometry& g = h.stepD.g;
+  const dh::FFTPlan* pIx = fft_plan(h, g.w);
+  const dh::FFTPlan* pIy = fft_plan(h, g.h);
+  if (!pIx || !pIy) return DIGHOLO_ERROR_MEMORYALLOCATION;
+  const size_t n = (size_t)g.B * g.P * g.h * g.w;
+  DH_TRY(h.win.ensure(n), "alloc");
+  DH_TRY(h.fields.ensure(n), "alloc");
+  if (h.stepA > 1) {
+    dh::Geometry gA = g;
+    gA.B = g.B * h.stepA;
+    DH_TRY(h.winAll.ensure(n * h.stepA), "alloc");
+    DH_TRY(dh::launch_select_window(gA, h.rowspec.p, h.factor.p, h.winAll.p, dh::BatchCal{}, h.stream),
+           "window select");
+    DH_TRY(dh::launch_average_windows(g, h.stepA, h.stepMode, gA.B, h.winAll.p, h.win.p, h.stepCal, h.stream),
+           "averaging");
+  } else {
+    DH_TRY(dh::launch_select_window(g, h.rowspec.p, h.factor.p, h.win.p, h.stepCal, h.stream), "window select");
+  }
+  DH_TRY(dh::launch_ifft_rows(g, *pIx, h.win.p, h.stream), "IFFT rows");
+  DH_TRY(dh::launch_ifft_cols_tilt(g, *pIy, h.win.p, nullptr, h.fields.p, h.stream), "IFFT columns");
+  h.stepState = 2;
+  h.haveBatch = false;       // h.fields holds a field that still carries the reference tilt
+  h.fieldsHostValid = false;
+  h.coefsDevValid = false;
+  h.lastPath = 0;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+// RemoveTilt step: fields = -conj(fields * ref); publishes the batch.
+int step_remove_tilt(Handle& h) {
+  if (h.stepState < 2) return DIGHOLO_ERROR_NULLPOINTER;
+  const dh::Geometry& g = h.stepD.g;
+  DH_TRY(dh::launch_apply_tilt(g, h.ref.p, h.fields.p, h.stream), "tilt");
+  h.stepState = 3;
+  publish_batch(h, h.stepD, h.stepFrames, h.stepU16);
+  h.coefsDevValid = false;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+// BasisExtractCoefs step: the overlap of the published fields.
+int step_extract_coefs(Handle& h) {
+  if (!h.haveBatch) return DIGHOLO_ERROR_NULLPOINTER;
+  const dh::Geometry& g = h.cur.g;
+  if (g.G <= 0) return DIGHOLO_ERROR_SUCCESS;
+  DH_TRY(h.coefs.ensure((size_t)g.B * g.P * g.M), "alloc");
+  DH_TRY(dh::launch_overlap_mfma(g, h.fields.p, h.hgx.p, h.hgy.p, h.coefs.p,
+                                 h.cfg.precision == DIGHOLO_PRECISION_SPLIT16, h.stream), "overlap");
+  h.coefsDevValid = true;
+  h.coefsHostValid = false;
   return DIGHOLO_ERROR_SUCCESS;
 }
 
@@ -734,6 +859,7 @@ void collect_stage_times(Handle& h) {
 }
 
 int download_coefs(Handle& h) {
+  if (!h.coefsDevValid) return DIGHOLO_ERROR_NULLPOINTER;
   if (h.coefsHostValid) return DIGHOLO_ERROR_SUCCESS;
   const dh::Geometry& g = h.cur.g;
   const size_t n = (size_t)g.B * g.P * g.M;
@@ -1585,6 +1711,66 @@ EXT_C int digHoloConfigGetBatchAvgMode(int handleIdx) {
   return CFG.avgMode;
 }
 
+// ---- batch calibration (digHolo.cpp:4469-4530, :14917-14985) ----
+EXT_C int digHoloConfigSetBatchCalibration(int handleIdx, complex64* cal, int polCount, int batchCount) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (polCount > 0 && batchCount > 0 && cal) {
+    CFG.cal.resize((size_t)polCount * batchCount);
+    std::memcpy(CFG.cal.data(), cal, CFG.cal.size() * sizeof(float2));
+    CFG.calP = polCount;
+    CFG.calB = batchCount;
+    CFG.calEnabled = 1;
+    ++CFG.calVersion;
+    return DIGHOLO_ERROR_SUCCESS;
+  }
+  CFG.calEnabled = 0;
+  return cal ? DIGHOLO_ERROR_INVALIDARGUMENT : DIGHOLO_ERROR_NULLPOINTER;
+}
+EXT_C complex64* digHoloConfigGetBatchCalibration(int handleIdx, int* polCount, int* batchCount) {
+  Handle* h = get(handleIdx);
+  if (!h) return nullptr;
+  if (CFG.cal.empty()) {
+    *polCount = 0;
+    *batchCount = 0;
+    return nullptr;
+  }
+  *polCount = CFG.calP;
+  *batchCount = CFG.calB;
+  return reinterpret_cast<complex64*>(CFG.cal.data());
+}
+EXT_C int digHoloConfigSetBatchCalibrationFromFile(int handleIdx, const char* fname, int polCount, int batchCount) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  FILE* f = fname ? std::fopen(fname, "rb") : nullptr;
+  if (!f) return DIGHOLO_ERROR_FILENOTFOUND;
+  std::fseek(f, 0, SEEK_END);
+  const long bytes = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  // the file must hold exactly polCount x batchCount complex64 (digHolo.cpp:4543-4580)
+  const long long expect = (long long)polCount * batchCount * (long long)sizeof(float2);
+  if (polCount < 1 || batchCount < 1 || bytes != expect) {
+    std::fclose(f);
+    CFG.calEnabled = 0;
+    return DIGHOLO_ERROR_INVALIDDIMENSION;
+  }
+  std::vector<float2> buf((size_t)polCount * batchCount);
+  const size_t got = std::fread(buf.data(), sizeof(float2), buf.size(), f);
+  std::fclose(f);
+  if (got != buf.size()) {
+    CFG.calEnabled = 0;
+    return DIGHOLO_ERROR_INVALIDDIMENSION;
+  }
+  return digHoloConfigSetBatchCalibration(handleIdx, reinterpret_cast<complex64*>(buf.data()), polCount, batchCount);
+}
+EXT_C int digHoloConfigSetBatchCalibrationEnabled(int handleIdx, int enabled) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  CFG.calEnabled = enabled ? 1 : 0;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C int digHoloConfigGetBatchCalibrationEnabled(int handleIdx) {
+  H_OR(0);
+  return CFG.calEnabled;
+}
+
 // ---- windows (digHolo.cpp:14234-14330, :13518-13540, :14414-14436, :14589-14611) ----
 EXT_C int digHoloConfigSetfftWindowSize(int handleIdx, int width, int height) {
   H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
@@ -2057,6 +2243,34 @@ EXT_C float* digHoloAutoAlignGetMetrics(int handleIdx, int metricIdx) {
   H_OR(nullptr);
   if (metricIdx < 0 || metricIdx >= DIGHOLO_METRIC_COUNT || h->aaMetrics.empty()) return nullptr;
   return &h->aaMetrics[(size_t)metricIdx * (h->aaLambdaCount + 1)];
+}
+
+// Pipeline steps (digHolo.cpp:15146-15192 wrap the same four stages).  Each is
+// synchronous like the reference call.
+EXT_C int digHoloProcessFFT(int handleIdx) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  int e = enqueue_batch(*h, 1);
+  if (!e && hipStreamSynchronize(h->stream) != hipSuccess) e = DIGHOLO_ERROR_ERROR;
+  collect_stage_times(*h);
+  return e;
+}
+EXT_C int digHoloProcessIFFT(int handleIdx) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  int e = step_ifft(*h);
+  if (!e && hipStreamSynchronize(h->stream) != hipSuccess) e = DIGHOLO_ERROR_ERROR;
+  return e;
+}
+EXT_C int digHoloProcessRemoveTilt(int handleIdx) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  int e = step_remove_tilt(*h);
+  if (!e && hipStreamSynchronize(h->stream) != hipSuccess) e = DIGHOLO_ERROR_ERROR;
+  return e;
+}
+EXT_C int digHoloProcessBasisExtractCoefs(int handleIdx) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  int e = step_extract_coefs(*h);
+  if (!e && hipStreamSynchronize(h->stream) != hipSuccess) e = DIGHOLO_ERROR_ERROR;
+  return e;
 }
 
 EXT_C int digHoloBatchGetSummary(int handleIdx, int planeIdx, int* parameterCount, int* totalCount, int* polCount,

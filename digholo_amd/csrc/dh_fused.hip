@@ -734,6 +734,7 @@ struct W3Args {
   const _Float16* hgx16;              // [P][hi/lo][Gp][w], rows m >= G zero
   const _Float16* hgy16;              // [P][hi/lo][Gp][h]
   float2* coefs;                      // [B][P][M]
+  BatchCal cal;                       // per-frame window calibration
   int region;         // bytes of the aliased row-phase / field-plane region
   int ovlOff;         // byte offset of the overlap's stage-1 buffers (after the plane)
   unsigned magH;      // q / H == __umulhi(q, magH) for q < 2^20
@@ -1169,12 +1170,13 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
   {
     constexpr int MM = w / 4, US = MM / 4;
     const float2* fac = factor + (size_t)p * (h * w);
+    const float2 cb = a.cal.at(p, b);
     for (int ps = wave; ps < J; ps += 4) {
       const int j = 16 * ps + t16;
       float2* row = fbuf + j * WP;
       float2 v[MM];
 #pragma unroll
-      for (int m = 0; m < MM; ++m) v[m] = cmul(row[l4 + 4 * m], fac[j * w + l4 + 4 * m]);
+      for (int m = 0; m < MM; ++m) v[m] = cmul(cmul(row[l4 + 4 * m], fac[j * w + l4 + 4 * m]), cb);
       dft_any<MM, true>(v);
 #pragma unroll
       for (int aa = 1; aa < MM; ++aa) {
@@ -1386,7 +1388,7 @@ bool window_v3_supported(const FusedArgs& a) {
 
 hipError_t launch_window_v3(const FusedArgs& fa, const void* frames_u16, const float2* tab, const float2* factor,
                             const float2* ref, float2* fields, const void* hgx16, const void* hgy16, float2* coefs,
-                            hipStream_t s) {
+                            const BatchCal& cal, hipStream_t s) {
   if (!window_v3_supported(fa)) return hipErrorNotSupported;
   if (hgx16 && !window_v3_overlap_fits(fa.g)) return hipErrorNotSupported;
   const Geometry& g = fa.g;
@@ -1399,6 +1401,7 @@ hipError_t launch_window_v3(const FusedArgs& fa, const void* frames_u16, const f
   a.hgx16 = static_cast<const _Float16*>(hgx16);
   a.hgy16 = static_cast<const _Float16*>(hgy16);
   a.coefs = coefs;
+  a.cal = cal;
   a.magH = magic_div((unsigned)g.H);
   if (a.magH == 0) return hipErrorNotSupported;
   // |X| <= W * 65535 < 2^(16 + log2 W): scale into [2^14, 2^15) at most

@@ -36,6 +36,16 @@ struct Geometry {
   int G, M;                   // HG groups and modes per pol
 };
 
+// Per-frame complex calibration of the Fourier window (digHolo.cpp:4469-4500,
+// :1831-1840): cal[p % P][b % B]; p == nullptr means none.
+struct BatchCal {
+  const float2* p = nullptr;
+  int P = 1, B = 1;
+  __host__ __device__ float2 at(int pol, int b) const {
+    return p ? p[(pol % P) * B + (b % B)] : make_float2(1.f, 0.f);
+  }
+};
+
 // FP64 per-pol parameters of the Fourier-window factor and the reference wave.
 struct PolParams {
   double dkx, dky;            // 2 pi / (px W), 2 pi / (px H)
@@ -60,10 +70,22 @@ hipError_t launch_tables(const TableArgs& a, const uint8_t* d_mask, float2* d_fa
 hipError_t launch_row_fft(const Geometry& g, const FFTPlan& pl, const float2* twW, const void* frames,
                           int frame_is_u16, float2* rowspec, hipStream_t s);
 hipError_t launch_col_fft_window(const Geometry& g, const FFTPlan& pl, const float2* rowspec,
-                                 const float2* factor, float2* win, hipStream_t s);
+                                 const float2* factor, float2* win, const BatchCal& cal, hipStream_t s);
 hipError_t launch_ifft_rows(const Geometry& g, const FFTPlan& pl, float2* win, hipStream_t s);
+// ref == nullptr: the inverse transform only (field before tilt removal)
 hipError_t launch_ifft_cols_tilt(const Geometry& g, const FFTPlan& pl, const float2* win,
                                  const float2* ref, float2* fields, hipStream_t s);
+// Step kernels: window rows of a column-transformed band times the window
+// factor; tilt removal fields = -conj(fields * ref) in place.
+hipError_t launch_select_window(const Geometry& g, const float2* spec, const float2* factor, float2* win,
+                                const BatchCal& cal, hipStream_t s);
+// uint16 frames given transposed: out[f][i H + j] = in[f][j W + i] (W, H the frame dims)
+hipError_t launch_u16_transpose(const void* in, void* out, int W, int H, size_t frames, hipStream_t s);
+// In-phase averaging of A windows per output (avgMode 0 sequential, 1 interlaced,
+// 2 sequential sweep with one wavelength): winAll [frames][P][h][w] -> win [g.B][P][h][w]
+hipError_t launch_average_windows(const Geometry& g, int A, int mode, int frames, const float2* winAll, float2* win,
+                                  const BatchCal& cal, hipStream_t s);
+hipError_t launch_apply_tilt(const Geometry& g, const float2* ref, float2* fields, hipStream_t s);
 // Fused window path: frame ingest -> r2c rows -> Fourier window (DIT over row
 // residues) -> window factor -> 2-D IFFT -> tilt -> fields.  Returns
 // hipErrorNotSupported when the shape does not fit its LDS/register budget
@@ -102,7 +124,7 @@ hipError_t launch_window_v3_tables(const Geometry& g, float2* tab, hipStream_t s
 // launch also computes the split-fp16 HG overlap into coefs.
 hipError_t launch_window_v3(const FusedArgs& a, const void* frames_u16, const float2* tab, const float2* factor,
                             const float2* ref, float2* fields, const void* hgx16, const void* hgy16, float2* coefs,
-                            hipStream_t s);
+                            const BatchCal& cal, hipStream_t s);
 bool window_v3_overlap_fits(const Geometry& g);
 // Split-fp16 basis [P][hi/lo][Gp][n] (Gp = G rounded up to 16) from hgx/hgy.
 size_t hg_split_elems(const Geometry& g, int n);

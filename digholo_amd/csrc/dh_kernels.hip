@@ -149,7 +149,7 @@ __global__ __launch_bounds__(kThreads) void k_row_fft(Geometry g, FFTPlan pl, co
 __global__ __launch_bounds__(kThreads) void k_col_fft_window(Geometry g, FFTPlan pl,
                                                              const float2* __restrict__ rowspec,
                                                              const float2* __restrict__ factor,
-                                                             float2* __restrict__ win, int CB) {
+                                                             float2* __restrict__ win, int CB, BatchCal cal) {
   extern __shared__ float2 smem[];
   const int n = pl.n;
   float2* A = smem;
@@ -166,11 +166,12 @@ __global__ __launch_bounds__(kThreads) void k_col_fft_window(Geometry g, FFTPlan
   const float2* fac = factor + (size_t)p * g.h * g.w + i0;
   float2* out = win + (size_t)(b * g.P + p) * g.h * g.w + i0;
   const int sy = g.SY[p] + n / 2;
+  const float2 cb = cal.at(p, b);
   for (int e = threadIdx.x; e < g.h * cols; e += blockDim.x) {
     const int j = e / cols, c = e - j * cols;
     int row = (j + sy) % n;
     if (row < 0) row += n;
-    out[(size_t)j * g.w + c] = cmul(F[c * n + row], fac[(size_t)j * g.w + c]);
+    out[(size_t)j * g.w + c] = cmul(cmul(F[c * n + row], fac[(size_t)j * g.w + c]), cb);
   }
 }
 
@@ -236,12 +237,146 @@ __global__ __launch_bounds__(kThreads) void k_ifft_cols_tilt(Geometry g, FFTPlan
   }
   __syncthreads();
   const float2* E = lds_fft<true>(A, Bf, cols, pl);
+  if (!ref) {   // the field before tilt removal (the IFFT step on its own)
+    for (int e = threadIdx.x; e < n * cols; e += blockDim.x) {
+      const int j = e / cols, c = e - j * cols;
+      fields[base + (size_t)j * g.w + c] = E[c * n + j];
+    }
+    return;
+  }
   const float2* rp = ref + (size_t)p * g.h * g.w + i0;
   for (int e = threadIdx.x; e < n * cols; e += blockDim.x) {
     const int j = e / cols, c = e - j * cols;
     const float2 t = cmul(E[c * n + j], rp[(size_t)j * g.w + c]);
     fields[base + (size_t)j * g.w + c] = make_float2(-t.x, t.y);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Step kernels (the pipeline run one stage at a time)
+// ---------------------------------------------------------------------------
+// win[b][p][j][c] = spec[b][p][row(j)][c] * factor[p][j][c], spec the column-
+// transformed band of the w window columns ([B][P][H][w]).
+__global__ void k_select_window(Geometry g, const float2* __restrict__ spec, const float2* __restrict__ factor,
+                                float2* __restrict__ win, BatchCal cal) {
+  const size_t hw = (size_t)g.h * g.w;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)g.B * g.P * hw) return;
+  const size_t bp = e / hw;
+  const int p = (int)(bp % g.P);
+  const int rem = (int)(e - bp * hw);
+  const int j = rem / g.w, c = rem - j * g.w;
+  int row = (j + g.SY[p] + g.H / 2) % g.H;
+  if (row < 0) row += g.H;
+  win[e] = cmul(cmul(spec[(bp * g.H + row) * g.w + c], factor[(size_t)p * hw + rem]), cal.at(p, (int)(bp / g.P)));
+}
+
+// uint16 frames given transposed (digHolo.cpp:1206-1229): out[f][i H + j] = in[f][j W + i]
+__global__ void k_u16_transpose(const unsigned short* __restrict__ in, unsigned short* __restrict__ out, int W,
+                                int H, size_t frames) {
+  const size_t n = (size_t)W * H;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= frames * n) return;
+  const size_t f = e / n;
+  const int o = (int)(e - f * n);
+  const int i = o / H, j = o - i * H;
+  out[e] = in[f * n + (size_t)j * W + i];
+}
+
+// In-phase averaging of A Fourier windows per (batch element, pol)
+// (digHolo.cpp:1885-1928, :2066-2114): W = o_0 / A; for a >= 1,
+// u = phi / |phi| with phi = sum_pixels W conj(o_a), W += o_a u / A.  o_a is the
+// calibrated window of frame f(b, a) (layout by avgMode, digHolo.cpp:8661-8672).
+// One 256-thread block per (b, p); W is held in registers (up to kAvgPer per thread).
+constexpr int kAvgPer = 32;
+__global__ __launch_bounds__(256) void k_average_windows(Geometry g, int A, int mode, int frames,
+                                                         const float2* __restrict__ winAll,
+                                                         float2* __restrict__ win, BatchCal cal) {
+  __shared__ float2 red[256 / 64];
+  const int p = blockIdx.x, b = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int hw = g.h * g.w;
+  const float inv = 1.f / (float)A;
+  const float2 cb = cal.at(p, b);
+  float2 Wv[kAvgPer];
+#pragma unroll
+  for (int s = 0; s < kAvgPer; ++s) Wv[s] = make_float2(0.f, 0.f);
+  for (int a = 0; a < A; ++a) {
+    int f = mode == 0 ? b * A + a : (mode == 1 ? b + a * g.B : b + a);
+    if (f >= frames) f = frames - 1;   // guarded on the host; never past the buffer
+    const float2* o = winAll + (size_t)(f * g.P + p) * hw;
+    float2 u = make_float2(1.f, 0.f);
+    if (a > 0) {
+      float2 phi = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int s = 0; s < kAvgPer; ++s) {
+        const int e = tid + 256 * s;
+        if (e < hw) {
+          const float2 oa = cmul(o[e], cb);
+          phi.x += Wv[s].x * oa.x + Wv[s].y * oa.y;   // W conj(o)
+          phi.y += Wv[s].y * oa.x - Wv[s].x * oa.y;
+        }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        phi.x += __shfl_xor(phi.x, off);
+        phi.y += __shfl_xor(phi.y, off);
+      }
+      __syncthreads();   // red[] of the previous frame has been read by everyone
+      if (lane == 0) red[wave] = phi;
+      __syncthreads();
+      phi = make_float2(red[0].x + red[1].x + red[2].x + red[3].x, red[0].y + red[1].y + red[2].y + red[3].y);
+      const float mag = sqrtf(phi.x * phi.x + phi.y * phi.y);
+      u = mag > 0.f ? make_float2(phi.x / mag, phi.y / mag) : make_float2(1.f, 0.f);
+    }
+#pragma unroll
+    for (int s = 0; s < kAvgPer; ++s) {
+      const int e = tid + 256 * s;
+      if (e < hw) Wv[s] = cadd(Wv[s], cscale(cmul(cmul(o[e], cb), u), inv));
+    }
+  }
+  float2* out = win + (size_t)(b * g.P + p) * hw;
+#pragma unroll
+  for (int s = 0; s < kAvgPer; ++s) {
+    const int e = tid + 256 * s;
+    if (e < hw) out[e] = Wv[s];
+  }
+}
+
+hipError_t launch_u16_transpose(const void* in, void* out, int W, int H, size_t frames, hipStream_t s) {
+  const size_t n = frames * W * H;
+  k_u16_transpose<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(static_cast<const unsigned short*>(in),
+                                                               static_cast<unsigned short*>(out), W, H, frames);
+  return hipGetLastError();
+}
+
+hipError_t launch_average_windows(const Geometry& g, int A, int mode, int frames, const float2* winAll, float2* win,
+                                  const BatchCal& cal, hipStream_t s) {
+  if (g.h * g.w > 256 * kAvgPer) return hipErrorNotSupported;
+  dim3 grid(g.P, g.B);
+  k_average_windows<<<grid, 256, 0, s>>>(g, A, mode, frames, winAll, win, cal);
+  return hipGetLastError();
+}
+
+// fields = -conj(fields * ref), in place
+__global__ void k_apply_tilt(Geometry g, const float2* __restrict__ ref, float2* __restrict__ fields) {
+  const size_t hw = (size_t)g.h * g.w;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)g.B * g.P * hw) return;
+  const int p = (int)((e / hw) % g.P);
+  const float2 t = cmul(fields[e], ref[(size_t)p * hw + e % hw]);
+  fields[e] = make_float2(-t.x, t.y);
+}
+
+hipError_t launch_select_window(const Geometry& g, const float2* spec, const float2* factor, float2* win,
+                                const BatchCal& cal, hipStream_t s) {
+  const size_t n = (size_t)g.B * g.P * g.h * g.w;
+  k_select_window<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, spec, factor, win, cal);
+  return hipGetLastError();
+}
+
+hipError_t launch_apply_tilt(const Geometry& g, const float2* ref, float2* fields, hipStream_t s) {
+  const size_t n = (size_t)g.B * g.P * g.h * g.w;
+  k_apply_tilt<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, ref, fields);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -365,13 +500,13 @@ hipError_t launch_row_fft(const Geometry& g, const FFTPlan& pl, const float2* tw
 }
 
 hipError_t launch_col_fft_window(const Geometry& g, const FFTPlan& pl, const float2* rowspec,
-                                 const float2* factor, float2* win, hipStream_t s) {
+                                 const float2* factor, float2* win, const BatchCal& cal, hipStream_t s) {
   const int CB = std::min(16, std::min(g.w, batch_per_block(pl.n, 4096)));
   const size_t lds = 2 * (size_t)CB * pl.n * sizeof(float2);
   hipError_t e = set_lds(k_col_fft_window, lds);
   if (e != hipSuccess) return e;
   dim3 grid((g.w + CB - 1) / CB, g.P, g.B);
-  k_col_fft_window<<<grid, kThreads, lds, s>>>(g, pl, rowspec, factor, win, CB);
+  k_col_fft_window<<<grid, kThreads, lds, s>>>(g, pl, rowspec, factor, win, CB, cal);
   return hipGetLastError();
 }
 

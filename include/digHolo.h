@@ -114,6 +114,11 @@ EXT_C int digHoloSetBatchAvg(int handleIdx, int batchCount, float* frameBuffer, 
 EXT_C int digHoloSetBatchUint16(int handleIdx, int batchCount, unsigned short* frameBuffer, int transpose); /* digHolo.h:519 */
 EXT_C int digHoloSetBatchAvgUint16(int handleIdx, int batchCount, unsigned short* frameBuffer, int avgCount, int avgMode, int transpose); /* digHolo.h:540 */
 EXT_C int digHoloConfigSetBatchCount(int handleIdx, int batchCount);              /* digHolo.h:551 */
+EXT_C int digHoloConfigSetBatchCalibration(int handleIdx, complex64* cal, int polCount, int batchCount); /* digHolo.h:599 */
+EXT_C complex64* digHoloConfigGetBatchCalibration(int handleIdx, int* polCount, int* batchCount);       /* digHolo.h:611 */
+EXT_C int digHoloConfigSetBatchCalibrationFromFile(int handleIdx, const char* fname, int polCount, int batchCount); /* digHolo.h:624 */
+EXT_C int digHoloConfigSetBatchCalibrationEnabled(int handleIdx, int enabled);    /* digHolo.h:635 */
+EXT_C int digHoloConfigGetBatchCalibrationEnabled(int handleIdx);                 /* digHolo.h:645 */
 EXT_C int digHoloConfigGetBatchCount(int handleIdx);                              /* digHolo.h:559 */
 EXT_C int digHoloConfigSetBatchAvgCount(int handleIdx, int avgCount);             /* digHolo.h:570 */
 EXT_C int digHoloConfigGetBatchAvgCount(int handleIdx);                           /* digHolo.h:580 */
@@ -203,6 +208,17 @@ EXT_C int digHoloBatchGetSummary(int handleIdx, int planeIdx, int* parameterCoun
                                  float** yAxis, float** totalIntensity);           /* digHolo.h:1659 */
 
 /* ---- threads / console ---- */
+/* The processing pipeline one step at a time (digHoloProcessBatch runs all
+ * four).  FFT: frames -> Fourier plane of the window columns; IFFT: Fourier
+ * window -> field still carrying the reference tilt; RemoveTilt: the final
+ * field (GetFields valid); BasisExtractCoefs: coefficients of the current
+ * fields (BasisGetCoefs valid).  A step whose input is missing returns
+ * DIGHOLO_ERROR_NULLPOINTER. */
+EXT_C int digHoloProcessFFT(int handleIdx);                                     /* digHolo.h:1670 */
+EXT_C int digHoloProcessIFFT(int handleIdx);                                    /* digHolo.h:1682 */
+EXT_C int digHoloProcessRemoveTilt(int handleIdx);                              /* digHolo.h:1694 */
+EXT_C int digHoloProcessBasisExtractCoefs(int handleIdx);                       /* digHolo.h:1706 */
+
 EXT_C int digHoloConfigSetThreadCount(int handleIdx, int threadCount);            /* digHolo.h:1749 */
 EXT_C int digHoloConfigGetThreadCount(int handleIdx);                             /* digHolo.h:1757 */
 EXT_C float digHoloBenchmark(int handleIdx, float goalDuration, float* info);     /* digHolo.h:1776 */

@@ -11,6 +11,8 @@ What it restates (all citations are digHolo.cpp unless noted):
 * Fourier/camera axes ............................. :2841-2883, :7726-7731
 * IFFT-window bounds (low-res, r2c) ............... :7868-8048
 * Fourier-window copy (mask, fill-factor, phase) .. :1808-2122, :8558-8700
+* in-phase frame averaging, batch calibration ..... :1831-1840, :1885-1928, :2066-2114, :8639-8672
+* transposed uint16 input ......................... :1206-1229
 * inverse FFT of the window ....................... :7533-7582, :8717-8735
 * reference wave (tilt, defocus, centre) .......... :10201-10504
 * tilt removal (V = -conj(E * ref)) ............... :2701-2837
@@ -396,14 +398,53 @@ def fft_plane(window: np.ndarray) -> np.ndarray:
     return np.fft.rfft2(np.asarray(window, dtype=np.float64))
 
 
-def process_batch(frames: np.ndarray, cfg: Config, emulate_int16: bool = False):
+def transpose_u16(frames: np.ndarray) -> np.ndarray:
+    """uint16 frames given transposed (convert_int16tofloat32, digHolo.cpp:1206-1229):
+    the float frame at flat index i*H + j is the input at flat index j*W + i."""
+    frames = np.asarray(frames)
+    N, H, W = frames.shape
+    return np.ascontiguousarray(frames.reshape(N, H, W).transpose(0, 2, 1)).reshape(N, H, W)
+
+
+def average_frame_index(b: int, a: int, B: int, A: int, mode: int) -> int:
+    """Frame read for batch element b, average slot a (digHolo.cpp:8661-8672, one
+    wavelength): sequential b*A + a, interlaced b + a*B, sequential sweep b + a."""
+    if mode == 0:
+        return b * A + a
+    if mode == 1:
+        return b + a * B
+    return b + a
+
+
+def average_windows(o: np.ndarray) -> np.ndarray:
+    """In-phase average of the A Fourier windows o [A, h, w] of one output
+    (digCopyWindow's two passes, digHolo.cpp:1885-1928, :2066-2114):
+    W = o_0 / A, then for a >= 1, u = phi / |phi| with phi = sum W conj(o_a),
+    W += o_a u / A."""
+    A = o.shape[0]
+    W = o[0] / A
+    for a in range(1, A):
+        phi = np.sum(W * np.conj(o[a]))
+        u = phi / abs(phi) if abs(phi) > 0 else 1.0
+        W = W + o[a] * u / A
+    return W
+
+
+def process_batch(frames: np.ndarray, cfg: Config, emulate_int16: bool = False, avg_count: int = 1,
+                  avg_mode: int = 0, batch_cal=None, transpose: bool = False):
     """Full ``digHoloProcessBatch`` path.
 
-    frames: [B, frameH, frameW] (uint16 or float).  Returns (fields, coefs, plan)
-    with fields [B, pol, h, w] complex128 and coefs [B, pol, M] complex128.
+    frames: [B * avg_count, frameH, frameW] (uint16 or float).  Returns (fields,
+    coefs, plan) with fields [B, pol, h, w] complex128 and coefs [B, pol, M]
+    complex128.  batch_cal [calP, calB] complex multiplies the Fourier window of
+    batch element b, pol p by batch_cal[p % calP, b % calB] (digHolo.cpp:1831-1840,
+    :8639-8642); transpose applies the uint16 transposed-input conversion.
     """
     frames = np.asarray(frames)
-    B = frames.shape[0]
+    if transpose:
+        frames = transpose_u16(frames)
+    A = max(1, int(avg_count))
+    B = frames.shape[0] // A
     cfg = apply_pol_locks(cfg)
     plan = make_plan(cfg)
     P = cfg.pol_count
@@ -417,6 +458,16 @@ def process_batch(frames: np.ndarray, cfg: Config, emulate_int16: bool = False):
         win = frames[:, plan.cy[p]:plan.cy[p] + plan.H, plan.cx[p]:plan.cx[p] + plan.W]
         F = np.fft.rfft2(win.astype(np.float64), axes=(-2, -1))
         Fw = F[:, t.row[:, None], t.col[None, :]] * t.factor[None]
+        if batch_cal is not None:
+            cal = np.asarray(batch_cal, dtype=np.complex128)
+            cal_b = lambda b: cal[p % cal.shape[0], b % cal.shape[1]]
+        else:
+            cal_b = lambda b: 1.0
+        if A > 1:
+            Fw = np.stack([average_windows(np.stack([Fw[average_frame_index(b, a, B, A, avg_mode)] * cal_b(b)
+                                                     for a in range(A)])) for b in range(B)])
+        elif batch_cal is not None:
+            Fw = Fw * np.array([cal_b(b) for b in range(B)])[:, None, None]
         E = np.fft.ifft2(Fw, axes=(-2, -1)) * (plan.w * plan.h)      # unnormalised e^{+i}
         V = -np.conj(E * t.ref[None])
         if emulate_int16:

@@ -208,3 +208,29 @@ def test_large_window_high_order(lib):
     coefs, fields, _, _, h = run_gpu(lib, frames, cfg)
     assert lib.digHoloGetWindowPath(h) == 0          # 160x160 field: staged kernels
     check(frames, cfg, coefs, fields)
+
+
+@pytest.mark.parametrize("name,B", [("small", 4), ("dual640", 2)])
+def test_pipeline_steps_match_process_batch(lib, name, B):
+    """ProcessFFT -> ProcessIFFT -> ProcessRemoveTilt -> ProcessBasisExtractCoefs
+    (digHolo.h:1670-1706) gives the batch result, and each step refuses to run
+    before its input exists."""
+    frames, cfg = make_case(name, B, seed=5)
+    h = lib.digHoloCreate()
+    api.configure(lib, h, **cfg)
+    assert lib.digHoloProcessIFFT(h) == api.DIGHOLO_ERROR_NULLPOINTER
+    assert lib.digHoloProcessRemoveTilt(h) == api.DIGHOLO_ERROR_NULLPOINTER
+    assert lib.digHoloProcessBasisExtractCoefs(h) == api.DIGHOLO_ERROR_NULLPOINTER
+    buf = np.ascontiguousarray(frames)
+    assert lib.digHoloSetBatchUint16(h, B, buf.ctypes.data_as(C.c_void_p), 0) == 0
+    assert lib.digHoloProcessFFT(h) == 0
+    assert lib.digHoloProcessRemoveTilt(h) == api.DIGHOLO_ERROR_NULLPOINTER
+    assert lib.digHoloProcessIFFT(h) == 0
+    assert lib.digHoloProcessRemoveTilt(h) == 0
+    assert lib.digHoloProcessBasisExtractCoefs(h) == 0
+    coefs = api.get_coefs(lib, h)
+    fields = api.get_fields(lib, h)[0]
+    check(frames, cfg, coefs, fields)
+    c2, f2, _, _ = api.process_batch(lib, h, frames)
+    assert O.relative_l2(fields, f2) < 1e-6 and O.relative_l2(coefs, c2) < 1e-6
+    lib.digHoloDestroy(h)

@@ -351,3 +351,88 @@ def test_fields_int16_scale_uses_full_range():
     assert np.max(np.maximum(np.abs(R), np.abs(I))) <= 32767
     back = (R + 1j * I) / s[..., None, None]
     assert np.max(np.abs(back - V)) <= 0.51 / s.min() * math.sqrt(2)
+
+
+# ---- input variants: transposed uint16, in-phase averaging, batch calibration ----
+
+def test_transpose_u16_matches_index_definition():
+    """convert_int16tofloat32 with transpose (digHolo.cpp:1206-1229) writes the
+    input pixel at flat j*W + i to flat i*H + j; checked by an explicit loop on a
+    non-square frame, and for square frames it is the matrix transpose."""
+    rng = np.random.default_rng(0)
+    H, W = 6, 10
+    fr = rng.integers(0, 65535, (2, H, W)).astype(np.uint16)
+    out = O.transpose_u16(fr)
+    for f in range(2):
+        raw, got = fr[f].reshape(-1), out[f].reshape(-1)
+        for i in range(W):
+            for j in range(H):
+                assert got[i * H + j] == raw[j * W + i]
+    sq = rng.integers(0, 65535, (1, 8, 8)).astype(np.uint16)
+    assert np.array_equal(O.transpose_u16(sq)[0], sq[0].T)
+
+
+def test_transposed_square_frames_give_the_untransposed_result():
+    spec = synth.HologramSpec(frame_width=256, frame_height=256, pol_count=1, waist=200e-6, group_count=8)
+    frames, _, _, _ = synth.make_batch(spec, 2, seed=3)
+    cfg = small_cfg(frame_width=256, frame_height=256, group_count=8)
+    f0, c0, _ = O.process_batch(frames, cfg)
+    f1, c1, _ = O.process_batch(np.ascontiguousarray(frames.transpose(0, 2, 1)), cfg, transpose=True)
+    assert O.relative_l2(f1, f0) < 1e-12 and O.relative_l2(c1, c0) < 1e-12
+
+
+def test_average_windows_aligns_phases():
+    """Copies of one window under arbitrary global phases average back to the first
+    copy (the phase of each is aligned to the running sum); with the conjugate on the
+    wrong operand the phases would add instead and this fails."""
+    rng = np.random.default_rng(1)
+    o = rng.normal(size=(8, 8)) + 1j * rng.normal(size=(8, 8))
+    th = rng.uniform(0, 2 * np.pi, 5)
+    stack = np.stack([o * np.exp(1j * t) for t in th])
+    assert np.allclose(O.average_windows(stack), stack[0], rtol=1e-12, atol=1e-12)
+    # a sign flip is a phase too: o and -o average to o, not to zero
+    assert np.allclose(O.average_windows(np.stack([o, -o])), o)
+    # independent noise averages down: the mean of A aligned noisy copies
+    noisy = np.stack([o + 0.3 * (rng.normal(size=o.shape) + 1j * rng.normal(size=o.shape)) for _ in range(16)])
+    assert O.relative_l2(O.average_windows(noisy), o) < 0.5 * O.relative_l2(noisy[0], o)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_average_frame_layouts(mode):
+    """AVGMODE_SEQUENTIAL reads [A A A B B B], INTERLACED [A B A B] (digHolo.h:655-657):
+    phase-rotated copies of two holograms laid out per mode average back to each
+    hologram's own single-frame result."""
+    spec = synth.spec_small()
+    frames, _, _, _ = synth.make_batch(spec, 2, seed=4, quantise=False)
+    cfg = small_cfg()
+    f_ref, c_ref, _ = O.process_batch(frames, cfg)
+    A = 3
+    # frames are real intensities: repeat them (the Fourier windows then differ by nothing)
+    order = [b * A + a for b in range(2) for a in range(A)] if mode == 0 else [a * 2 + b for a in range(A) for b in range(2)]
+    stacked = np.empty((2 * A,) + frames.shape[1:], frames.dtype)
+    for b in range(2):
+        for a in range(A):
+            stacked[O.average_frame_index(b, a, 2, A, mode)] = frames[b]
+    assert sorted(O.average_frame_index(b, a, 2, A, mode) for b in range(2) for a in range(A)) == sorted(order)
+    f_avg, c_avg, _ = O.process_batch(stacked, cfg, avg_count=A, avg_mode=mode)
+    assert O.relative_l2(f_avg, f_ref) < 1e-12 and O.relative_l2(c_avg, c_ref) < 1e-12
+
+
+def test_batch_calibration_multiplies_fields_by_its_conjugate():
+    """The calibration multiplies the Fourier window; the tilt step V = -conj(E ref)
+    turns that into conj(cal) on the field and coefficients, cal[p % P][b % B]."""
+    spec = synth.spec_dual_640()
+    frames, _, _, _ = synth.make_batch(spec, 3, seed=6, quantise=False)
+    cfg = O.Config(frame_width=640, frame_height=512, pol_count=2, fft_window_x=256, fft_window_y=256,
+                   aperture_deg=0.7, tilt_x=[1.5, 1.5], tilt_y=[1.5, 1.5], group_count=10,
+                   waist=[300e-6, 300e-6], beam_centre_x=[-1.6e-3, 1.6e-3])
+    f0, c0, _ = O.process_batch(frames, cfg)
+    cal = np.array([[0.5 + 0.2j, -1.1j], [2.0, 0.3 - 0.4j]])      # [pol][batch], batch wraps at 2
+    f1, c1, _ = O.process_batch(frames, cfg, batch_cal=cal)
+    for b in range(3):
+        for p in range(2):
+            k = np.conj(cal[p, b % 2])
+            assert np.allclose(f1[b, p], k * f0[b, p]) and np.allclose(c1[b, p], k * c0[b, p])
+    # a pol-independent calibration (one row) applies to both pols
+    f2, _, _ = O.process_batch(frames, cfg, batch_cal=cal[:1])
+    assert np.allclose(f2[1, 1], np.conj(cal[0, 1]) * f0[1, 1])
