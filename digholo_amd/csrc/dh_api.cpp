@@ -193,6 +193,7 @@ struct Handle {
   DevBuf<float2> w3Tab;     // twiddle tables of path 3
   DevBuf<uint16_t> tstage;  // uint16 frames after the transpose pre-pass
   DevBuf<float2> winAll;    // every frame's Fourier window when averaging
+  DevBuf<float2> plane;     // full-resolution IFFT input [B][P][H][W]
   DevBuf<float2> calDev;    // batch calibration on the device
   unsigned calDevVersion = ~0u;
   DevBuf<uint16_t> hgx16, hgy16;   // split-fp16 basis of the fused overlap
@@ -333,6 +334,11 @@ bool derive(const Config& c, int B, Derived& d, int& err, const Derived* crop = 
   g.H = H;
   g.w = w;
   g.h = h;
+  // field plane: the window itself (low-res IFFT) or the whole FFT window
+  // (full resolution, digHolo.cpp:8081-8082, :8175-8189)
+  const bool lowRes = c.resolutionIdx == 1;
+  g.fw = lowRes ? w : W;
+  g.fh = lowRes ? h : H;
   for (int p = 0; p < P; ++p) {
     g.cx[p] = cx[p];
     g.cy[p] = cy[p];
@@ -386,34 +392,37 @@ bool derive(const Config& c, int B, Derived& d, int& err, const Derived* crop = 
     q.bcx = (double)c.beamCX[p] - cx0 * pxd;
     q.bcy = (double)c.beamCY[p] - cy0 * pxd;
     q.px = pxd;
-    q.norm = 1.0 / std::sqrt((double)W * H * w * h);
+    q.norm = 1.0 / std::sqrt((double)W * H * g.fw * g.fh);   // digHolo.cpp:1813-1822
     q.fill = c.fillFactor ? 1 : 0;
     // reference wave: pol-lock index rules of digHolo.cpp:10289-10325
     const int tp = (c.polLockTilt || c.polLockDefocus) ? 0 : p;
     const int ip = c.polLockDefocus ? 0 : p;
     const double thx = (double)c.tiltX[tp] * PI_D / 180.0;
     const double thy = (double)c.tiltY[tp] * PI_D / 180.0;
-    const double thoffx = (d.SX[ip] - W / 2.0) * q.dkx;
-    const double thoffy = (d.SY[ip] - H / 2.0) * q.dky;
+    // window-offset tilt only in low-res mode (digHolo.cpp:10326-10334)
+    const double thoffx = lowRes ? (d.SX[ip] - W / 2.0) * q.dkx : 0.0;
+    const double thoffy = lowRes ? (d.SY[ip] - H / 2.0) * q.dky : 0.0;
     q.kxr = -k0d * thx + thoffx;
     q.kyr = -k0d * thy + thoffy;
     q.pistx = (double)c.beamCX[p] * (-k0d * thx);
     q.pisty = (double)c.beamCY[p] * (-k0d * thy);
     q.focal = -PI_D * (double)c.defocus[ip] / lamd;
-    q.pxx = pxd * W / w;
-    q.pxy = pxd * H / h;
+    q.pxx = pxd * W / g.fw;
+    q.pxy = pxd * H / g.fh;
     float wst = c.waist[c.polLockWaist ? 0 : p];
     if (wst == 0.f) wst = 400e-6f;
     q.waist = (double)wst;
   }
-  d.pxx = (px * W) / w;                 // digHolo.cpp:8184-8185 (float)
-  d.pxy = (px * H) / h;
+  d.pxx = (px * W) / g.fw;              // digHolo.cpp:8184-8185 (float)
+  d.pxy = (px * H) / g.fh;
   return true;
 }
 
 bool same_tables(const Derived& a, const Derived& b) {
   const dh::Geometry &x = a.g, &y = b.g;
-  if (x.P != y.P || x.W != y.W || x.H != y.H || x.w != y.w || x.h != y.h || x.G != y.G) return false;
+  if (x.P != y.P || x.W != y.W || x.H != y.H || x.w != y.w || x.h != y.h || x.G != y.G || x.fw != y.fw ||
+      x.fh != y.fh)
+    return false;
   for (int p = 0; p < x.P; ++p)
     if (x.c0[p] != y.c0[p] || x.SY[p] != y.SY[p]) return false;
   if (std::memcmp(a.tab.pp, b.tab.pp, sizeof(a.tab.pp)) != 0) return false;
@@ -562,11 +571,6 @@ int enqueue_batch(Handle& h, int step = 0) {
   Config& c = h.cfg;
   if (!h.frames) return DIGHOLO_ERROR_NULLPOINTER;
   if (c.batchCount < 1) c.batchCount = 1;
-  if (c.resolutionIdx != 1) {
-    fprintf(stderr, "digHolo: IFFTResolutionMode 0 (full resolution) is not implemented in this build; "
-                    "set digHoloConfigSetIFFTResolutionMode(handle, 1)\n");
-    return DIGHOLO_ERROR_INVALIDARGUMENT;
-  }
   if (c.basisType != DIGHOLO_BASISTYPE_HG && c.maxMG > 0) {
     fprintf(stderr, "digHolo: only the HG basis is implemented in this build\n");
     return DIGHOLO_ERROR_INVALIDARGUMENT;
@@ -627,13 +631,17 @@ int enqueue_batch(Handle& h, int step = 0) {
   }
   dh::Geometry gA = g;   // every frame of the buffer (averaging reads A per batch element)
   gA.B = (int)nFrames;
-  DH_TRY(h.fields.ensure(B * P * g.h * g.w), "alloc");
+  DH_TRY(h.fields.ensure(B * P * g.fh * g.fw), "alloc");
   DH_TRY(h.coefs.ensure(std::max<size_t>(B * P * g.M, 1)), "alloc");
+  dh::Geometry gF = g;   // the field plane, for the inverse transforms, tilt and overlap
+  gF.w = g.fw;
+  gF.h = g.fh;
+  const bool fullRes = g.fw != g.w || g.fh != g.h;
 
   const dh::FFTPlan* pRow = fft_plan(h, g.W / 2);
   const dh::FFTPlan* pCol = fft_plan(h, g.H);
-  const dh::FFTPlan* pIx = fft_plan(h, g.w);
-  const dh::FFTPlan* pIy = fft_plan(h, g.h);
+  const dh::FFTPlan* pIx = fft_plan(h, g.fw);
+  const dh::FFTPlan* pIy = fft_plan(h, g.fh);
   if (!pRow || !pCol || !pIx || !pIy) {
     fprintf(stderr, "digHolo: FFT plan setup failed\n");
     return DIGHOLO_ERROR_MEMORYALLOCATION;
@@ -664,9 +672,9 @@ int enqueue_batch(Handle& h, int step = 0) {
     const size_t T = P * g.h * g.w;
     DH_TRY(h.mask.ensure(T), "alloc");
     DH_TRY(h.factor.ensure(T), "alloc");
-    DH_TRY(h.ref.ensure(T), "alloc");
-    DH_TRY(h.hgx.ensure(std::max<size_t>(P * g.G * g.w, 1)), "alloc");
-    DH_TRY(h.hgy.ensure(std::max<size_t>(P * g.G * g.h, 1)), "alloc");
+    DH_TRY(h.ref.ensure(P * g.fh * g.fw), "alloc");
+    DH_TRY(h.hgx.ensure(std::max<size_t>(P * g.G * g.fw, 1)), "alloc");
+    DH_TRY(h.hgy.ensure(std::max<size_t>(P * g.G * g.fh, 1)), "alloc");
     DH_TRY(hipMemcpyAsync(h.mask.p, d.mask.data(), T, hipMemcpyHostToDevice, h.stream), "mask upload");
     DH_TRY(dh::launch_tables(d.tab, h.mask.p, h.factor.p, h.ref.p, h.hgx.p, h.hgy.p, h.stream), "tables");
     DH_TRY(h.colTab.ensure(dh::coldft_table_elems(g)), "alloc");
@@ -676,8 +684,8 @@ int enqueue_batch(Handle& h, int step = 0) {
       DH_TRY(dh::launch_window_v3_tables(g, h.w3Tab.p, h.stream), "v3 tables");
     }
     if (g.G > 0) {
-      DH_TRY(h.hgx16.ensure(dh::hg_split_elems(g, g.w)), "alloc");
-      DH_TRY(h.hgy16.ensure(dh::hg_split_elems(g, g.h)), "alloc");
+      DH_TRY(h.hgx16.ensure(dh::hg_split_elems(g, g.fw)), "alloc");
+      DH_TRY(h.hgy16.ensure(dh::hg_split_elems(g, g.fh)), "alloc");
       DH_TRY(dh::launch_hg_split(g, h.hgx.p, h.hgy.p, h.hgx16.p, h.hgy16.p, h.stream), "split basis");
     }
     DH_TRY(hipStreamSynchronize(h.stream), "tables sync");   // the host mask vector goes out of scope
@@ -746,14 +754,20 @@ int enqueue_batch(Handle& h, int step = 0) {
       DH_TRY(dh::launch_col_fft_window(g, *pCol, h.rowspec.p, h.factor.p, h.win.p, cal, h.stream), "column FFT");
     }
     if (timing) DH_TRY(hipEventRecord(ev[3], h.stream), "event");
-    DH_TRY(dh::launch_ifft_rows(g, *pIx, h.win.p, h.stream), "IFFT rows");
-    DH_TRY(dh::launch_ifft_cols_tilt(g, *pIy, h.win.p, h.ref.p, h.fields.p, h.stream), "IFFT columns");
+    float2* ifftIn = h.win.p;
+    if (fullRes) {   // the window at its own frequencies in a zero W x H plane
+      DH_TRY(h.plane.ensure(B * P * g.fh * g.fw), "alloc");
+      DH_TRY(dh::launch_scatter_full(g, h.win.p, h.plane.p, h.stream), "full-resolution plane");
+      ifftIn = h.plane.p;
+    }
+    DH_TRY(dh::launch_ifft_rows(gF, *pIx, ifftIn, h.stream), "IFFT rows");
+    DH_TRY(dh::launch_ifft_cols_tilt(gF, *pIy, ifftIn, h.ref.p, h.fields.p, h.stream), "IFFT columns");
     if (timing) DH_TRY(hipEventRecord(ev[4], h.stream), "event");
     h.lastPath = 0;
     h.lastFusedOverlap = false;
   }
   if (g.G > 0 && !h.lastFusedOverlap) {
-    DH_TRY(dh::launch_overlap_mfma(g, h.fields.p, h.hgx.p, h.hgy.p, h.coefs.p,
+    DH_TRY(dh::launch_overlap_mfma(gF, h.fields.p, h.hgx.p, h.hgy.p, h.coefs.p,
                                    c.precision == DIGHOLO_PRECISION_SPLIT16, h.stream), "overlap");
   }
   if (timing) DH_TRY(hipEventRecord(ev[5], h.stream), "event");
@@ -775,10 +789,10 @@ void publish_batch(Handle& h, const Derived& d, const void* dframes, bool u16) {
   h.analysisValid[0] = h.analysisValid[1] = false;
   h.f16Valid = false;
   // reconstructed-plane axes, float32 like digHolo.cpp:8184-8189 / :2875-2880
-  h.xAxis.resize(g.w);
-  h.yAxis.resize(g.h);
-  for (int i = 0; i < g.w; ++i) h.xAxis[i] = (float)(i - g.w / 2) * d.pxx;
-  for (int j = 0; j < g.h; ++j) h.yAxis[j] = (float)(j - g.h / 2) * d.pxy;
+  h.xAxis.resize(g.fw);
+  h.yAxis.resize(g.fh);
+  for (int i = 0; i < g.fw; ++i) h.xAxis[i] = (float)(i - g.fw / 2) * d.pxx;
+  for (int j = 0; j < g.fh; ++j) h.yAxis[j] = (float)(j - g.fh / 2) * d.pxy;
 }
 
 // IFFT step: window rows of the band times the window factor, 2-D inverse FFT,
@@ -786,12 +800,16 @@ void publish_batch(Handle& h, const Derived& d, const void* dframes, bool u16) {
 int step_ifft(Handle& h) {
   if (h.stepState < 1) return DIGHOLO_ERROR_NULLPOINTER;
   const dh::Geometry& g = h.stepD.g;
-  const dh::FFTPlan* pIx = fft_plan(h, g.w);
-  const dh::FFTPlan* pIy = fft_plan(h, g.h);
+  dh::Geometry gF = g;
+  gF.w = g.fw;
+  gF.h = g.fh;
+  const dh::FFTPlan* pIx = fft_plan(h, g.fw);
+  const dh::FFTPlan* pIy = fft_plan(h, g.fh);
   if (!pIx || !pIy) return DIGHOLO_ERROR_MEMORYALLOCATION;
   const size_t n = (size_t)g.B * g.P * g.h * g.w;
+  const size_t nF = (size_t)g.B * g.P * g.fh * g.fw;
   DH_TRY(h.win.ensure(n), "alloc");
-  DH_TRY(h.fields.ensure(n), "alloc");
+  DH_TRY(h.fields.ensure(nF), "alloc");
   if (h.stepA > 1) {
     dh::Geometry gA = g;
     gA.B = g.B * h.stepA;
@@ -803,8 +821,14 @@ int step_ifft(Handle& h) {
   } else {
     DH_TRY(dh::launch_select_window(g, h.rowspec.p, h.factor.p, h.win.p, h.stepCal, h.stream), "window select");
   }
-  DH_TRY(dh::launch_ifft_rows(g, *pIx, h.win.p, h.stream), "IFFT rows");
-  DH_TRY(dh::launch_ifft_cols_tilt(g, *pIy, h.win.p, nullptr, h.fields.p, h.stream), "IFFT columns");
+  float2* ifftIn = h.win.p;
+  if (nF != n || g.fw != g.w) {
+    DH_TRY(h.plane.ensure(nF), "alloc");
+    DH_TRY(dh::launch_scatter_full(g, h.win.p, h.plane.p, h.stream), "full-resolution plane");
+    ifftIn = h.plane.p;
+  }
+  DH_TRY(dh::launch_ifft_rows(gF, *pIx, ifftIn, h.stream), "IFFT rows");
+  DH_TRY(dh::launch_ifft_cols_tilt(gF, *pIy, ifftIn, nullptr, h.fields.p, h.stream), "IFFT columns");
   h.stepState = 2;
   h.haveBatch = false;       // h.fields holds a field that still carries the reference tilt
   h.fieldsHostValid = false;
@@ -816,8 +840,10 @@ int step_ifft(Handle& h) {
 // RemoveTilt step: fields = -conj(fields * ref); publishes the batch.
 int step_remove_tilt(Handle& h) {
   if (h.stepState < 2) return DIGHOLO_ERROR_NULLPOINTER;
-  const dh::Geometry& g = h.stepD.g;
-  DH_TRY(dh::launch_apply_tilt(g, h.ref.p, h.fields.p, h.stream), "tilt");
+  dh::Geometry gF = h.stepD.g;
+  gF.w = gF.fw;
+  gF.h = gF.fh;
+  DH_TRY(dh::launch_apply_tilt(gF, h.ref.p, h.fields.p, h.stream), "tilt");
   h.stepState = 3;
   publish_batch(h, h.stepD, h.stepFrames, h.stepU16);
   h.coefsDevValid = false;
@@ -827,7 +853,9 @@ int step_remove_tilt(Handle& h) {
 // BasisExtractCoefs step: the overlap of the published fields.
 int step_extract_coefs(Handle& h) {
   if (!h.haveBatch) return DIGHOLO_ERROR_NULLPOINTER;
-  const dh::Geometry& g = h.cur.g;
+  dh::Geometry g = h.cur.g;
+  g.w = g.fw;
+  g.h = g.fh;
   if (g.G <= 0) return DIGHOLO_ERROR_SUCCESS;
   DH_TRY(h.coefs.ensure((size_t)g.B * g.P * g.M), "alloc");
   DH_TRY(dh::launch_overlap_mfma(g, h.fields.p, h.hgx.p, h.hgy.p, h.coefs.p,
@@ -873,7 +901,7 @@ int download_coefs(Handle& h) {
 int download_fields(Handle& h) {
   if (h.fieldsHostValid) return DIGHOLO_ERROR_SUCCESS;
   const dh::Geometry& g = h.cur.g;
-  const size_t n = (size_t)g.B * g.P * g.h * g.w;
+  const size_t n = (size_t)g.B * g.P * g.fh * g.fw;
   DH_TRY(h.hFields.ensure(n), "pinned alloc");
   DH_TRY(hipMemcpyAsync(h.hFields.p, h.fields.p, n * sizeof(float2), hipMemcpyDeviceToHost, h.stream), "D2H");
   DH_TRY(hipStreamSynchronize(h.stream), "sync");
@@ -935,15 +963,15 @@ int compute_analysis(Handle& h, int plane) {
     src = h.spec0.p;
     planeStride = (size_t)Ny * Nx;
   } else {
-    a.Nx = g.w;
-    a.Ny = g.h;
+    a.Nx = g.fw;
+    a.Ny = g.fh;
     a.x0 = 0;
-    a.x1 = 8 * (g.w / 8);
+    a.x1 = 8 * (g.fw / 8);
     a.skip0 = a.skip1 = -1;
     xs = h.xAxis;
     ys = h.yAxis;
     src = h.fields.p;
-    planeStride = (size_t)g.h * g.w;
+    planeStride = (size_t)g.fh * g.fw;
   }
   const int Nx = a.Nx, Ny = a.Ny, N = Nx * Ny;
   a.inStride = Nx;
@@ -1008,7 +1036,7 @@ int compute_fields16(Handle& h) {
   const int e = compute_analysis(h, 1);
   if (e) return e;
   const dh::Geometry& g = h.cur.g;
-  const int B = g.B, P = g.P, hw = g.h * g.w;
+  const int B = g.B, P = g.P, hw = g.fh * g.fw;
   const size_t n = (size_t)B * P * hw;
   const float* scale = &h.analysis[1][(size_t)3 * P * (B + 1)];
   DH_TRY(h.anaScale.ensure((size_t)P * (B + 1)), "alloc");
@@ -1522,6 +1550,10 @@ void destroy_handle(Handle* h) {
     h->mask.release();
     h->colTab.release();
     h->w3Tab.release();
+    h->plane.release();
+    h->winAll.release();
+    h->tstage.release();
+    h->calDev.release();
     h->hgx16.release();
     h->hgy16.release();
     for (auto& kv : h->ffts) kv.second->tw.release();
@@ -2044,8 +2076,8 @@ EXT_C complex64* digHoloGetFields(int handleIdx, int* batchCount, int* polCount,
   const dh::Geometry& g = h->cur.g;
   *batchCount = g.B;
   *polCount = g.P;
-  *width = g.w;
-  *height = g.h;
+  *width = g.fw;
+  *height = g.fh;
   *x = h->xAxis.data();
   *y = h->yAxis.data();
   return reinterpret_cast<complex64*>(h->hFields.p);
@@ -2161,8 +2193,8 @@ EXT_C complex64* digHoloGetFieldsDevice(int handleIdx, int* batchCount, int* pol
   const dh::Geometry& g = h->cur.g;
   *batchCount = g.B;
   *polCount = g.P;
-  *width = g.w;
-  *height = g.h;
+  *width = g.fw;
+  *height = g.fh;
   return reinterpret_cast<complex64*>(h->fields.p);
 }
 
@@ -2288,8 +2320,8 @@ EXT_C int digHoloBatchGetSummary(int handleIdx, int planeIdx, int* parameterCoun
   if (planeIdx != 0 && planeIdx != 1) return DIGHOLO_ERROR_INVALIDDIMENSION;
   const int e = compute_analysis(*h, planeIdx);
   if (e) return e;
-  *pixelCountX = planeIdx == 0 ? g.W / 2 + 1 : g.w;
-  *pixelCountY = planeIdx == 0 ? g.H : g.h;
+  *pixelCountX = planeIdx == 0 ? g.W / 2 + 1 : g.fw;
+  *pixelCountY = planeIdx == 0 ? g.H : g.fh;
   *xAxis = h->anaX[planeIdx].data();
   *yAxis = h->anaY[planeIdx].data();
   *parameters = h->analysis[planeIdx].data();
@@ -2314,7 +2346,7 @@ EXT_C int digHoloGetFields16(int handleIdx, int* batchCount, int* polCount, shor
     return DIGHOLO_ERROR_NULLPOINTER;
   }
   const dh::Geometry& g = h->cur.g;
-  const size_t n = (size_t)g.B * g.P * g.h * g.w;
+  const size_t n = (size_t)g.B * g.P * g.fh * g.fw;
   *batchCount = g.B;
   *polCount = g.P;
   *fieldR = h->hF16.p;
@@ -2322,8 +2354,8 @@ EXT_C int digHoloGetFields16(int handleIdx, int* batchCount, int* polCount, shor
   *fieldScale = &h->analysis[1][(size_t)DIGHOLO_ANALYSIS_MAXABS * g.P * (g.B + 1)];
   *x = h->xAxis.data();
   *y = h->yAxis.data();
-  *width = g.w;
-  *height = g.h;
+  *width = g.fw;
+  *height = g.fh;
   return DIGHOLO_ERROR_SUCCESS;
 }
 

@@ -280,6 +280,7 @@ static int ip_chunk(const FFTPlan& pl, int nt = kFThreads) {
 static int fpt_of(const FusedArgs& a) { return (a.g.h * a.g.w + kFThreads - 1) / kFThreads; }
 
 bool fused_prepare(FusedArgs& a) {
+  if (a.g.fw != a.g.w || a.g.fh != a.g.h) return false;
   if (a.R < 1 || a.Hc * a.R != a.g.H) return false;
   if (fpt_of(a) > kMaxFPT) return false;
   const int cr = ip_chunk(a.row), cc = ip_chunk(a.col), cx = ip_chunk(a.ix), cy = ip_chunk(a.iy);
@@ -516,6 +517,7 @@ static int mfma_cpw(const Geometry& g) { return (2 * (g.h / 16) + 7) / 8; }
 
 bool window_mfma_prepare(FusedArgs& a) {
   const Geometry& g = a.g;
+  if (g.fw != g.w || g.fh != g.h) return false;
   if (g.w % 16 || g.h % 16 || g.w < 16 || g.h < 16) return false;
   const int nu = g.w / 16, cpw = mfma_cpw(g);
   const bool inst = (cpw == 1 && (nu == 3 || nu == 4 || nu == 5 || nu == 6 || nu == 8)) ||
@@ -1351,9 +1353,9 @@ hipError_t launch_hg_split(const Geometry& g, const float* hgx, const float* hgy
                            hipStream_t s) {
   if (g.G <= 0) return hipSuccess;
   const int Gp = (g.G + 15) & ~15;
-  const int nx = g.P * Gp * g.w, ny = g.P * Gp * g.h;
-  k_hg_split<<<(nx + 255) / 256, 256, 0, s>>>(hgx, g.P, g.G, g.w, static_cast<_Float16*>(hgx16));
-  k_hg_split<<<(ny + 255) / 256, 256, 0, s>>>(hgy, g.P, g.G, g.h, static_cast<_Float16*>(hgy16));
+  const int nx = g.P * Gp * g.fw, ny = g.P * Gp * g.fh;
+  k_hg_split<<<(nx + 255) / 256, 256, 0, s>>>(hgx, g.P, g.G, g.fw, static_cast<_Float16*>(hgx16));
+  k_hg_split<<<(ny + 255) / 256, 256, 0, s>>>(hgy, g.P, g.G, g.fh, static_cast<_Float16*>(hgy16));
   return hipGetLastError();
 }
 
@@ -1373,6 +1375,7 @@ static bool w3_common(const Geometry& g, int& L, int& M) {
 
 bool window_v3_supported(const FusedArgs& a) {
   const Geometry& g = a.g;
+  if (g.fw != g.w || g.fh != g.h) return false;   // full-resolution fields: staged kernels
   int L, M;
   if (!w3_common(g, L, M)) return false;
   const int U = g.w / 16, J = g.h / 16;

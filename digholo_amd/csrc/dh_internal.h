@@ -29,7 +29,8 @@ struct Geometry {
   int B, P;                   // batch, polarisations
   int frameW, frameH;
   int W, H;                   // FFT window
-  int w, h;                   // IFFT window
+  int w, h;                   // IFFT window (the Fourier-plane window that is inverse transformed)
+  int fw, fh;                 // reconstructed field plane: w x h (low-res IFFT), W x H (full resolution)
   int cx[kMaxPol], cy[kMaxPol];   // FFT window corners in the frame
   int c0[kMaxPol];            // first r2c column read by the IFFT window (SX - W/2)
   int SY[kMaxPol];            // IFFT window row start (shifted-spectrum coordinates)
@@ -79,6 +80,9 @@ hipError_t launch_ifft_cols_tilt(const Geometry& g, const FFTPlan& pl, const flo
 // factor; tilt removal fields = -conj(fields * ref) in place.
 hipError_t launch_select_window(const Geometry& g, const float2* spec, const float2* factor, float2* win,
                                 const BatchCal& cal, hipStream_t s);
+// Full-resolution IFFT input: win [B][P][h][w] placed at its own frequencies
+// (row (j + SY + H/2) mod H, column c0 + i) of a zero [B][P][H][W] plane.
+hipError_t launch_scatter_full(const Geometry& g, const float2* win, float2* plane, hipStream_t s);
 // uint16 frames given transposed: out[f][i H + j] = in[f][j W + i] (W, H the frame dims)
 hipError_t launch_u16_transpose(const void* in, void* out, int W, int H, size_t frames, hipStream_t s);
 // In-phase averaging of A windows per output (avgMode 0 sequential, 1 interlaced,

@@ -271,6 +271,30 @@ __global__ void k_select_window(Geometry g, const float2* __restrict__ spec, con
   win[e] = cmul(cmul(spec[(bp * g.H + row) * g.w + c], factor[(size_t)p * hw + rem]), cal.at(p, (int)(bp / g.P)));
 }
 
+// Full-resolution IFFT input (digHolo.cpp:1855-1872, :1938-1950, :2042-2065):
+// the window keeps its own frequency position in a zero W x H plane.
+__global__ void k_scatter_full(Geometry g, const float2* __restrict__ win, float2* __restrict__ plane) {
+  const size_t n = (size_t)g.H * g.W;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)g.B * g.P * n) return;
+  const size_t bp = e / n;
+  const int p = (int)(bp % g.P);
+  const int rem = (int)(e - bp * n);
+  const int row = rem / g.W, col = rem - row * g.W;
+  int j = (row - g.SY[p] - g.H / 2) % g.H;
+  if (j < 0) j += g.H;
+  const int i = col - g.c0[p];
+  float2 v = make_float2(0.f, 0.f);
+  if (j < g.h && i >= 0 && i < g.w) v = win[(bp * g.h + j) * g.w + i];
+  plane[e] = v;
+}
+
+hipError_t launch_scatter_full(const Geometry& g, const float2* win, float2* plane, hipStream_t s) {
+  const size_t n = (size_t)g.B * g.P * g.H * g.W;
+  k_scatter_full<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, win, plane);
+  return hipGetLastError();
+}
+
 // uint16 frames given transposed (digHolo.cpp:1206-1229): out[f][i H + j] = in[f][j W + i]
 __global__ void k_u16_transpose(const unsigned short* __restrict__ in, unsigned short* __restrict__ out, int W,
                                 int H, size_t frames) {
@@ -390,28 +414,37 @@ __device__ __forceinline__ double xcsc(double t) { return t == 0.0 ? 1.0 : t / s
 __global__ void k_tables(TableArgs a, const uint8_t* __restrict__ mask, float2* __restrict__ factor,
                          float2* __restrict__ ref) {
   const Geometry& g = a.g;
-  const int total = g.P * g.h * g.w;
-  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += gridDim.x * blockDim.x) {
-    const int p = idx / (g.h * g.w);
-    const int rem = idx - p * g.h * g.w;
-    const int j = rem / g.w, i = rem - j * g.w;
-    const PolParams& pp = a.pp[p];
-    const int col = g.c0[p] + i;
-    int row = (j + g.SY[p] + g.H / 2) % g.H;
-    if (row < 0) row += g.H;
-    const double kx = col * pp.dkx;
-    const double ky = (row <= g.H / 2 ? row : row - g.H) * pp.dky;
-    double env = pp.norm;
-    if (pp.fill) env *= xcsc(0.5 * kx * pp.px) * xcsc(0.5 * ky * pp.px);
+  const int nf = g.P * g.h * g.w;          // window factor over the Fourier window
+  const int nr = g.P * g.fh * g.fw;        // reference wave over the field plane
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < nf + nr; idx += gridDim.x * blockDim.x) {
     double sn, cs;
-    sincos(kx * pp.bcx + ky * pp.bcy, &sn, &cs);
-    const double m = mask[idx] ? env : 0.0;
-    factor[idx] = make_float2((float)(m * cs), (float)(m * sn));
-    const double x = (i - g.w / 2) * pp.pxx;
-    const double y = (j - g.h / 2) * pp.pxy;
-    const double phi = pp.kxr * x + pp.pistx + pp.focal * x * x + pp.kyr * y + pp.pisty + pp.focal * y * y;
-    sincos(phi, &sn, &cs);
-    ref[idx] = make_float2((float)cs, (float)sn);
+    if (idx < nf) {
+      const int p = idx / (g.h * g.w);
+      const int rem = idx - p * g.h * g.w;
+      const int j = rem / g.w, i = rem - j * g.w;
+      const PolParams& pp = a.pp[p];
+      const int col = g.c0[p] + i;
+      int row = (j + g.SY[p] + g.H / 2) % g.H;
+      if (row < 0) row += g.H;
+      const double kx = col * pp.dkx;
+      const double ky = (row <= g.H / 2 ? row : row - g.H) * pp.dky;
+      double env = pp.norm;
+      if (pp.fill) env *= xcsc(0.5 * kx * pp.px) * xcsc(0.5 * ky * pp.px);
+      sincos(kx * pp.bcx + ky * pp.bcy, &sn, &cs);
+      const double m = mask[idx] ? env : 0.0;
+      factor[idx] = make_float2((float)(m * cs), (float)(m * sn));
+    } else {
+      const int e = idx - nf;
+      const int p = e / (g.fh * g.fw);
+      const int rem = e - p * g.fh * g.fw;
+      const int j = rem / g.fw, i = rem - j * g.fw;
+      const PolParams& pp = a.pp[p];
+      const double x = (i - g.fw / 2) * pp.pxx;
+      const double y = (j - g.fh / 2) * pp.pxy;
+      const double phi = pp.kxr * x + pp.pistx + pp.focal * x * x + pp.kyr * y + pp.pisty + pp.focal * y * y;
+      sincos(phi, &sn, &cs);
+      ref[e] = make_float2((float)cs, (float)sn);
+    }
   }
 }
 
@@ -420,7 +453,7 @@ __global__ void k_tables(TableArgs a, const uint8_t* __restrict__ mask, float2* 
 // binary exponent carried separately (no under/overflow at high order).
 __global__ void k_hg_basis(TableArgs a, float* __restrict__ hgx, float* __restrict__ hgy) {
   const Geometry& g = a.g;
-  const int per = g.w + g.h;
+  const int per = g.fw + g.fh;              // the basis lives on the field plane
   const int total = g.P * per;
   const double rt2 = 1.41421356237309504880;
   const double pim4 = 0.75112554446494248286;   // pi^-1/4
@@ -429,15 +462,15 @@ __global__ void k_hg_basis(TableArgs a, float* __restrict__ hgx, float* __restri
     const int p = idx / per;
     const int k = idx - p * per;
     const PolParams& pp = a.pp[p];
-    const bool isx = k < g.w;
-    const int i = isx ? k : k - g.w;
-    const int N = isx ? g.w : g.h;
+    const bool isx = k < g.fw;
+    const int i = isx ? k : k - g.fw;
+    const int N = isx ? g.fw : g.fh;
     const double dt = isx ? pp.pxx : pp.pxy;
     const double t = (i - N / 2) * dt;
     const double u = rt2 * t / pp.waist;
     const double nrm = sqrt(dt * rt2 / pp.waist);
     const double gexp = -0.5 * u * u;
-    float* out = isx ? (hgx + (size_t)p * g.G * g.w + i) : (hgy + (size_t)p * g.G * g.h + i);
+    float* out = isx ? (hgx + (size_t)p * g.G * g.fw + i) : (hgy + (size_t)p * g.G * g.fh + i);
     const size_t stride = N;
     double m0 = pim4, e0 = 0.0;
     double m1 = rt2 * pim4 * u, e1 = 0.0;
@@ -479,11 +512,11 @@ static inline int batch_per_block(int n, int cap_elems) {
 
 hipError_t launch_tables(const TableArgs& a, const uint8_t* d_mask, float2* d_factor, float2* d_ref,
                          float* d_hgx, float* d_hgy, hipStream_t s) {
-  const int total = a.g.P * a.g.h * a.g.w;
+  const int total = a.g.P * (a.g.h * a.g.w + a.g.fh * a.g.fw);
   k_tables<<<(total + 255) / 256, 256, 0, s>>>(a, d_mask, d_factor, d_ref);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || a.g.G == 0) return e;
-  const int t2 = a.g.P * (a.g.w + a.g.h);
+  const int t2 = a.g.P * (a.g.fw + a.g.fh);
   k_hg_basis<<<(t2 + 63) / 64, 64, 0, s>>>(a, d_hgx, d_hgy);
   return hipGetLastError();
 }

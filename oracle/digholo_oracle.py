@@ -60,8 +60,9 @@ TILTX, TILTY, DEFOCUS = 0, 1, 2
 class Config:
     """Subset of ``digHoloConfig`` (digHolo.cpp:3323-3425) that the path reads.
 
-    Defaults follow the reference constructor, except ``resolution_mode`` which
-    callers set to 1 (low-res IFFT window) for this path.
+    Defaults follow the reference constructor, except ``resolution_mode``:
+    1 is the low-res IFFT of the window alone, 0 the reference's default
+    full-resolution IFFT (the window in a zero W x H plane).
     """
     frame_width: int = 320
     frame_height: int = 256
@@ -287,9 +288,9 @@ class PolTables:
     col: np.ndarray        # [w] column of the r2c plane each window column reads
     row: np.ndarray        # [h] row of the r2c plane each window row reads
     factor: np.ndarray     # [h, w] complex mask * sinc * phase * norm
-    ref: np.ndarray        # [h, w] complex reference wave exp(i(phix+phiy))
-    hgx: np.ndarray        # [G, w]
-    hgy: np.ndarray        # [G, h]
+    ref: np.ndarray        # [fh, fw] complex reference wave exp(i(phix+phiy)) on the field plane
+    hgx: np.ndarray        # [G, fw]
+    hgy: np.ndarray        # [G, fh]
 
 
 def _pol_index(cfg: Config, p: int) -> Tuple[int, int]:
@@ -359,7 +360,11 @@ def pol_tables(cfg: Config, plan: Plan, p: int) -> PolTables:
     else:
         sx = np.ones(w)
         sy = np.ones(h)
-    norm = 1.0 / math.sqrt(float(W) * H * w * h)
+    # field plane: the window (low-res) or the whole FFT window (full resolution),
+    # digHolo.cpp:8081-8082; the IFFT length in the norm follows it (:1813-1822)
+    low = cfg.resolution_mode == 1
+    fw, fh = (w, h) if low else (W, H)
+    norm = 1.0 / math.sqrt(float(W) * H * fw * fh)
     fx = sx * np.exp(1j * kx * bcx)
     fy = sy * np.exp(1j * ky * bcy)
     factor = inside * (fy[:, None] * fx[None, :]) * norm
@@ -369,13 +374,14 @@ def pol_tables(cfg: Config, plan: Plan, p: int) -> PolTables:
     k0 = 2.0 * math.pi / lam
     thx = float(F32(cfg.tilt_x[tp])) * math.pi / 180.0
     thy = float(F32(cfg.tilt_y[tp])) * math.pi / 180.0
-    thoff_x = (plan.SX[ip] - W / 2.0) * dk_x
-    thoff_y = (plan.SY[ip] - H / 2.0) * dk_y
+    # the window-offset tilt exists only in low-res mode (digHolo.cpp:10326-10334)
+    thoff_x = (plan.SX[ip] - W / 2.0) * dk_x if low else 0.0
+    thoff_y = (plan.SY[ip] - H / 2.0) * dk_y if low else 0.0
     D = float(F32(cfg.defocus[ip]))
-    pxx = px * W / w
-    pxy = px * H / h
-    x = (np.arange(w) - w // 2) * pxx
-    y = (np.arange(h) - h // 2) * pxy
+    pxx = px * W / fw
+    pxy = px * H / fh
+    x = (np.arange(fw) - fw // 2) * pxx
+    y = (np.arange(fh) - fh // 2) * pxy
     kxr = -k0 * thx + thoff_x
     kyr = -k0 * thy + thoff_y
     focal = -math.pi * D / lam
@@ -451,7 +457,9 @@ def process_batch(frames: np.ndarray, cfg: Config, emulate_int16: bool = False, 
     G = cfg.group_count
     Mi, Ni = hg_indices(G)
     tabs = [pol_tables(cfg, plan, p) for p in range(P)]
-    fields = np.zeros((B, P, plan.h, plan.w), dtype=np.complex128)
+    low = cfg.resolution_mode == 1
+    fh, fw = (plan.h, plan.w) if low else (plan.H, plan.W)
+    fields = np.zeros((B, P, fh, fw), dtype=np.complex128)
     coefs = np.zeros((B, P, len(Mi)), dtype=np.complex128)
     for p in range(P):
         t = tabs[p]
@@ -468,7 +476,13 @@ def process_batch(frames: np.ndarray, cfg: Config, emulate_int16: bool = False, 
                                                      for a in range(A)])) for b in range(B)])
         elif batch_cal is not None:
             Fw = Fw * np.array([cal_b(b) for b in range(B)])[:, None, None]
-        E = np.fft.ifft2(Fw, axes=(-2, -1)) * (plan.w * plan.h)      # unnormalised e^{+i}
+        if not low:
+            # full resolution: the window keeps its own frequencies in a zero
+            # W x H plane (digHolo.cpp:1855-1872, :1938-1950, :2042-2065)
+            full = np.zeros((B, plan.H, plan.W), dtype=np.complex128)
+            full[:, t.row[:, None], t.col[None, :]] = Fw
+            Fw = full
+        E = np.fft.ifft2(Fw, axes=(-2, -1)) * (fw * fh)              # unnormalised e^{+i}
         V = -np.conj(E * t.ref[None])
         if emulate_int16:
             V = quantise_int16(V)
@@ -572,12 +586,14 @@ def fourier_plane_analysis(frames: np.ndarray, cfg: Config):
 
 
 def field_axes(cfg: Config, plan: Plan):
-    """Reconstructed-plane axes (float32): x_i = (i - w/2) * (px W / w), digHolo.cpp:8184-8189."""
+    """Reconstructed-plane axes (float32): x_i = (i - fw/2) * (px W / fw), digHolo.cpp:8184-8189,
+    with fw = w (low-res IFFT) or W (full resolution)."""
+    fw, fh = (plan.w, plan.h) if cfg.resolution_mode == 1 else (plan.W, plan.H)
     px = F32(cfg.pixel_size)
-    pxx = F32(F32(px * F32(plan.W)) / F32(plan.w))
-    pxy = F32(F32(px * F32(plan.H)) / F32(plan.h))
-    x = (np.arange(plan.w) - plan.w // 2).astype(np.float32) * pxx
-    y = (np.arange(plan.h) - plan.h // 2).astype(np.float32) * pxy
+    pxx = F32(F32(px * F32(plan.W)) / F32(fw))
+    pxy = F32(F32(px * F32(plan.H)) / F32(fh))
+    x = (np.arange(fw) - fw // 2).astype(np.float32) * pxx
+    y = (np.arange(fh) - fh // 2).astype(np.float32) * pxy
     return x.astype(np.float32), y.astype(np.float32)
 
 

@@ -73,7 +73,7 @@ def oracle_for(cfg):
                     defocus=two("defocus", (0.0, 0.0)), waist=[cfg["waist"]] * 2,
                     group_count=cfg["group_count"], fill_factor=cfg.get("fill_factor", 1),
                     pol_lock_tilt=cfg.get("pol_lock_tilt", 0), pol_lock_defocus=cfg.get("pol_lock_defocus", 0),
-                    pol_lock_waist=cfg.get("pol_lock_waist", 1))
+                    pol_lock_waist=cfg.get("pol_lock_waist", 1), resolution_mode=cfg.get("resolution_mode", 1))
 
 
 def run_gpu(lib, frames, cfg, h=None):
@@ -233,4 +233,39 @@ def test_pipeline_steps_match_process_batch(lib, name, B):
     check(frames, cfg, coefs, fields)
     c2, f2, _, _ = api.process_batch(lib, h, frames)
     assert O.relative_l2(fields, f2) < 1e-6 and O.relative_l2(coefs, c2) < 1e-6
+    lib.digHoloDestroy(h)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("name,B", [("small", 3), ("dual640w192", 2), ("dual640", 2)])
+def test_parity_full_resolution(lib, name, B, precision):
+    """IFFTResolutionMode 0 (the reference default): the W x H field plane, on the
+    staged kernels, FP32 and split-fp16 overlap."""
+    frames, cfg = make_case(name, B, seed=12, resolution_mode=0)
+    h = lib.digHoloCreate()
+    api.configure(lib, h, **cfg)
+    assert lib.digHoloConfigSetOverlapPrecision(h, precision) == 0
+    coefs, fields, x, y, h = run_gpu(lib, frames, cfg, h=h)
+    geo = api.window_geometry(lib, h)
+    assert fields.shape[-2:] == (geo["H"], geo["W"])
+    assert lib.digHoloGetWindowPath(h) == 0
+    check(frames, cfg, coefs, fields)
+    px = np.float32(20e-6)
+    assert np.allclose(x, (np.arange(geo["W"]) - geo["W"] // 2) * px, rtol=1e-6)
+    lib.digHoloDestroy(h)
+
+
+def test_full_resolution_steps_and_averaging(lib):
+    B, A = 2, 2
+    frames, cfg = make_case("small", B * A, seed=13, resolution_mode=0)
+    h = lib.digHoloCreate()
+    api.configure(lib, h, **cfg)
+    buf = np.ascontiguousarray(frames)
+    assert lib.digHoloSetBatchAvgUint16(h, B, buf.ctypes.data_as(C.c_void_p), A, 0, 0) == 0
+    for step in (lib.digHoloProcessFFT, lib.digHoloProcessIFFT, lib.digHoloProcessRemoveTilt,
+                 lib.digHoloProcessBasisExtractCoefs):
+        assert step(h) == 0
+    coefs, fields = api.get_coefs(lib, h), api.get_fields(lib, h)[0]
+    f_ref, c_ref, _ = O.process_batch(buf, oracle_for(cfg), avg_count=A, avg_mode=0)
+    assert O.relative_l2(fields, f_ref) <= TOL and O.relative_l2(coefs, c_ref) <= TOL
     lib.digHoloDestroy(h)

@@ -436,3 +436,47 @@ def test_batch_calibration_multiplies_fields_by_its_conjugate():
     # a pol-independent calibration (one row) applies to both pols
     f2, _, _ = O.process_batch(frames, cfg, batch_cal=cal[:1])
     assert np.allclose(f2[1, 1], np.conj(cal[0, 1]) * f0[1, 1])
+
+
+# ---- full-resolution IFFT (IFFTResolutionMode 0, the reference default) ----
+
+def test_known_answer_full_resolution():
+    """Full resolution keeps the window at its own frequencies in a W x H plane,
+    drops the window-offset tilt from the reference wave and normalises by
+    1/(W H): the rendered coefficients come back with the same closed-form
+    scale as the low-res mode (a wrong norm or a leftover offset tilt fails)."""
+    spec = synth.spec_small()
+    spec.beam_centre_x = [37e-6, 0]
+    spec.beam_centre_y = [-55e-6, 0]
+    cfg = small_cfg(beam_centre_x=[37e-6, 0], beam_centre_y=[-55e-6, 0], resolution_mode=0)
+    err, ratio, plan = recover(spec, cfg, B=3)
+    assert err < 1e-5
+    assert np.allclose(ratio, 1.0, atol=1e-6)
+    fields, _, _ = O.process_batch(synth.make_batch(spec, 1, seed=2, quantise=False)[0], cfg)
+    assert fields.shape[-2:] == (plan.H, plan.W)
+
+
+def test_full_resolution_field_interpolates_the_low_res_field():
+    """Both modes reconstruct the same band-limited field with the same total
+    power (the norm 1/sqrt(W H fw fh) makes window size not matter): with an
+    integer ratio W/w = H/h = 3 (640x512, FFT window 192 -> IFFT window 64) the
+    full-resolution field at the low-res pixel positions is the low-res field
+    times sqrt(w h / (W H)), after each mode's own tilt removal."""
+    spec = synth.spec_dual_640()
+    spec.group_count = 20
+    frames, _, _, _ = synth.make_batch(spec, 1, seed=8, quantise=False)
+    kw = dict(frame_width=640, frame_height=512, pol_count=2, fft_window_x=192, fft_window_y=192,
+              aperture_deg=0.7, tilt_x=[1.5, 1.5], tilt_y=[1.5, 1.5], group_count=20,
+              waist=[300e-6, 300e-6], beam_centre_x=[-1.6e-3, 1.6e-3])
+    lo, c_lo, plan = O.process_batch(frames, O.Config(resolution_mode=1, **kw))
+    hi, c_hi, _ = O.process_batch(frames, O.Config(resolution_mode=0, **kw))
+    assert (plan.W, plan.w) == (192, 64)
+    assert abs(np.sum(np.abs(hi) ** 2) / np.sum(np.abs(lo) ** 2) - 1) < 1e-9
+    r = plan.W // plan.w
+    ks = (np.arange(plan.w) - plan.w // 2) * r + plan.W // 2
+    js = (np.arange(plan.h) - plan.h // 2) * r + plan.H // 2
+    for p in range(2):
+        sub = hi[0, p][np.ix_(js, ks)] / math.sqrt(plan.w * plan.h / (plan.W * plan.H))
+        assert O.relative_l2(sub, lo[0, p]) < 1e-9
+    # the HG coefficients are a property of the field, not of its sampling
+    assert O.relative_l2(c_hi, c_lo) < 1e-4
