@@ -26,7 +26,8 @@ DEVICE_FLAGS = [
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__",
               "-I" + os.path.join(ROCM, "include"), "-Wall", "-Wno-unused-result"]
 
-KERNEL_SOURCES = ["dh_kernels.hip", "dh_fused.hip", "dh_overlap.hip", "dh_metrics.hip", "dh_analysis.hip"]
+KERNEL_SOURCES = ["dh_kernels.hip", "dh_fused.hip", "dh_overlap.hip", "dh_metrics.hip", "dh_analysis.hip",
+                  "dh_align.hip"]
 HOST_SOURCES = ["dh_api.cpp", "dh_eigen.cpp"]
 
 

@@ -200,6 +200,13 @@ struct Handle {
   bool fusedOverlap = true; // path 3 computes split-fp16 coefficients in the window kernel
   bool lastFusedOverlap = false;
 
+  // AutoAlign parameter estimate: batch-summed Fourier maps [P][Ny][Nx],
+  // DCT work space, peak index, axes; far-field copies of the fields and maps
+  DevBuf<float> snapMap, snapAxes, ffMap;
+  DevBuf<double> snapWork, snapStats;
+  DevBuf<int> snapIdx;
+  DevBuf<float2> ffBuf;
+
   bool stageTiming = false;
   std::vector<std::array<hipEvent_t, 6>> evPool;
   size_t evUsed = 0;
@@ -237,14 +244,10 @@ int imod(int a, int n) {
 }
 
 // Returns false (and an error code) if the configuration cannot be processed.
-// crop: keep the camera-plane FFT windows of an earlier derivation; win: keep
-// its Fourier window (position, size and mask).  AutoAlign holds them the way
-// the reference skips its FFT and IFFT while tweaking (digHolo.cpp:12620-12640).
-bool derive(const Config& c, int B, Derived& d, int& err, const Derived* crop = nullptr,
-            const Derived* win = nullptr) {
+// FFT window size and per-pol corners (digHolo.cpp:7283-7377)
+bool fft_window(const Config& c, int& Wout, int& Hout, int cx[2], int cy[2], int& err) {
   err = DIGHOLO_ERROR_SUCCESS;
   const int P = c.polCount;
-  // --- FFT window, digHolo.cpp:7283-7377 ---
   int fwx = c.fftWindowSizeX > 0 ? c.fftWindowSizeX : c.frameWidth / P;
   int fwy = c.fftWindowSizeY > 0 ? c.fftWindowSizeY : c.frameHeight;
   int W = PIXEL_QUANTA * (fwx / PIXEL_QUANTA);
@@ -258,7 +261,6 @@ bool derive(const Config& c, int B, Derived& d, int& err, const Derived* crop = 
     return false;
   }
   const float px = c.pixelSize;
-  int cx[2] = {0, 0}, cy[2] = {0, 0};
   for (int p = 0; p < P; ++p) {
     const float CX = (c.beamCX[p] / px) + (float)(c.frameWidth / 2);
     const float CY = (c.beamCY[p] / px) + (float)(c.frameHeight / 2);
@@ -270,10 +272,25 @@ bool derive(const Config& c, int B, Derived& d, int& err, const Derived* crop = 
     if (y + H > c.frameHeight) y = c.frameHeight - H;
     cx[p] = x;
     cy[p] = y;
-    if (crop) {
-      cx[p] = crop->g.cx[p];
-      cy[p] = crop->g.cy[p];
-    }
+  }
+  Wout = W;
+  Hout = H;
+  return true;
+}
+
+// crop: keep the camera-plane FFT windows of an earlier derivation; win: keep
+// its Fourier window (position, size and mask).  AutoAlign holds them the way
+// the reference skips its FFT and IFFT while tweaking (digHolo.cpp:12620-12640).
+bool derive(const Config& c, int B, Derived& d, int& err, const Derived* crop = nullptr,
+            const Derived* win = nullptr) {
+  err = DIGHOLO_ERROR_SUCCESS;
+  const int P = c.polCount;
+  int W = 0, H = 0, cx[2] = {0, 0}, cy[2] = {0, 0};
+  if (!fft_window(c, W, H, cx, cy, err)) return false;
+  const float px = c.pixelSize;
+  for (int p = 0; p < P && crop; ++p) {
+    cx[p] = crop->g.cx[p];
+    cy[p] = crop->g.cy[p];
   }
   // --- IFFT window, digHolo.cpp:7868-8048 ---
   const float lam = c.wavelengthCentre;
@@ -561,6 +578,50 @@ int ensure_device(Handle& h) {
     }                                      \
   } while (0)
 
+// Device copy of the caller's frames (nFrames of frameWidth x frameHeight):
+// device pointers are used in place; host buffers are uploaded (once for the
+// whole search inside AutoAlign); transposed uint16 frames are transposed into
+// a staging buffer (digHolo.cpp:1206-1229, once per call like the reference's
+// conversion).
+int device_frames(Handle& h, size_t nFrames, const void*& dframes) {
+  const Config& c = h.cfg;
+  const size_t frameElems = nFrames * (size_t)c.frameWidth * c.frameHeight;
+  const size_t elemBytes = h.frameU16 ? 2 : 4;
+  dframes = h.frames;
+  if (!is_device_pointer(h.frames) || (reinterpret_cast<uintptr_t>(h.frames) % 8) != 0) {
+    const bool staged = h.holdStaging && h.stagedFrames == h.frames && h.stagedBytes == frameElems * elemBytes;
+    if (!staged) {
+      DH_TRY(h.staging.ensure(frameElems * elemBytes), "frame staging alloc");
+      DH_TRY(hipMemcpyAsync(h.staging.p, h.frames, frameElems * elemBytes, hipMemcpyDefault, h.stream),
+             "frame upload");
+      h.stagedFrames = h.frames;
+      h.stagedBytes = frameElems * elemBytes;
+    }
+    dframes = h.staging.p;
+  }
+  if (h.transpose && h.frameU16) {
+    DH_TRY(h.tstage.ensure(frameElems), "alloc");
+    DH_TRY(dh::launch_u16_transpose(dframes, h.tstage.p, c.frameWidth, c.frameHeight, nFrames, h.stream),
+           "transpose");
+    dframes = h.tstage.p;
+  }
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+// Twiddles exp(-2 pi i k / W), k <= W/2, of the r2c row unpack for window width W
+int ensure_row_twiddles(Handle& h, int W) {
+  if (h.twWn == W) return DIGHOLO_ERROR_SUCCESS;
+  std::vector<float2> tw(W / 2 + 1);
+  for (int k = 0; k <= W / 2; ++k) {
+    const double a = -2.0 * PI_D * (double)k / W;
+    tw[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  DH_TRY(h.twW.ensure(tw.size()), "alloc");
+  DH_TRY(hipMemcpy(h.twW.p, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice), "twiddles");
+  h.twWn = W;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
 // Queues one full batch on the handle's stream.  On return the coefficients and
 // fields are (or will be, once the stream drains) in device memory.
 void publish_batch(Handle& h, const Derived& d, const void* dframes, bool u16);
@@ -596,26 +657,8 @@ int enqueue_batch(Handle& h, int step = 0) {
   const size_t B = g.B, P = g.P;
   const int A = std::max(1, c.avgCount);      // frames averaged into each batch element
   const size_t nFrames = B * (size_t)A;
-  const size_t frameElems = nFrames * (size_t)g.frameW * g.frameH;
-  const size_t elemBytes = h.frameU16 ? 2 : 4;
-  const void* dframes = h.frames;
-  if (!is_device_pointer(h.frames) || (reinterpret_cast<uintptr_t>(h.frames) % 8) != 0) {
-    // AutoAlign reprocesses one frame buffer many times: upload it once
-    const bool staged = h.holdStaging && h.stagedFrames == h.frames && h.stagedBytes == frameElems * elemBytes;
-    if (!staged) {
-      DH_TRY(h.staging.ensure(frameElems * elemBytes), "frame staging alloc");
-      DH_TRY(hipMemcpyAsync(h.staging.p, h.frames, frameElems * elemBytes, hipMemcpyDefault, h.stream),
-             "frame upload");
-      h.stagedFrames = h.frames;
-      h.stagedBytes = frameElems * elemBytes;
-    }
-    dframes = h.staging.p;
-  }
-  if (h.transpose && h.frameU16) {   // digHolo.cpp:1206-1229, once per call like the reference's conversion
-    DH_TRY(h.tstage.ensure(frameElems), "alloc");
-    DH_TRY(dh::launch_u16_transpose(dframes, h.tstage.p, g.frameW, g.frameH, nFrames, h.stream), "transpose");
-    dframes = h.tstage.p;
-  }
+  const void* dframes = nullptr;
+  if (const int ef = device_frames(h, nFrames, dframes)) return ef;
   dh::BatchCal cal;
   if (c.calEnabled && c.calP > 0 && c.calB > 0) {
     if (h.calDevVersion != c.calVersion) {
@@ -646,16 +689,7 @@ int enqueue_batch(Handle& h, int step = 0) {
     fprintf(stderr, "digHolo: FFT plan setup failed\n");
     return DIGHOLO_ERROR_MEMORYALLOCATION;
   }
-  if (h.twWn != g.W) {
-    std::vector<float2> tw(g.W / 2 + 1);
-    for (int k = 0; k <= g.W / 2; ++k) {
-      const double a = -2.0 * PI_D * (double)k / g.W;
-      tw[k] = make_float2((float)std::cos(a), (float)std::sin(a));
-    }
-    DH_TRY(h.twW.ensure(tw.size()), "alloc");
-    DH_TRY(hipMemcpy(h.twW.p, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice), "twiddles");
-    h.twWn = g.W;
-  }
+  if (const int et = ensure_row_twiddles(h, g.W)) return et;
 
   const bool timing = h.stageTiming;
   hipEvent_t* ev = nullptr;
@@ -1311,7 +1345,9 @@ struct HoldGuard {     // frames upload once for the whole search; holds end wit
   ~HoldGuard() { h.holdStaging = h.holdCrop = h.holdWin = false; }
 };
 
-int autoalign_tweak(Handle& h, float& goalDb) {
+// snapSteps: the step sizes the parameter estimate chose (FULL mode); null in
+// TWEAK mode, where they follow from the current values (digHolo.cpp:12303-12320).
+int autoalign_tweak(Handle& h, float& goalDb, const float* snapSteps) {
   Config& c = h.cfg;
   const int P = c.polCount;
   const int goal0 = c.aaGoalIdx;
@@ -1321,15 +1357,16 @@ int autoalign_tweak(Handle& h, float& goalDb) {
   HoldGuard hold(h);
   bool processed = false;   // the first evaluation fixes the FFT crops (digHolo.cpp:12620-12625)
 
-  // window radius and step sizes (digHolo.cpp:12277-12327)
-  if (c.aaFourierRadius)
-    c.apertureSize = c.aaTilt ? window_radius_max(c.pixelSize, c.wavelengthCentre) : window_radius_rec(c);
   float baseStep[AA_PARS];
-  baseStep[AA_TILTX] = baseStep[AA_TILTY] = c.apertureSize * 0.1f;
-  baseStep[AA_CX] = baseStep[AA_CY] = c.waist[0] * 0.1f;
-  baseStep[AA_WAIST] = c.waist[0] * 0.1f;
-  baseStep[AA_DEFOCUS] = 0.4f;
-  for (int p = 0; p < P; ++p) baseStep[AA_DEFOCUS] = std::max(baseStep[AA_DEFOCUS], std::fabs(c.defocus[p]) * 0.1f);
+  if (snapSteps) {
+    std::copy(snapSteps, snapSteps + AA_PARS, baseStep);
+  } else {
+    baseStep[AA_TILTX] = baseStep[AA_TILTY] = c.apertureSize * 0.1f;
+    baseStep[AA_CX] = baseStep[AA_CY] = c.waist[0] * 0.1f;
+    baseStep[AA_WAIST] = c.waist[0] * 0.1f;
+    baseStep[AA_DEFOCUS] = 0.4f;
+    for (int p = 0; p < P; ++p) baseStep[AA_DEFOCUS] = std::max(baseStep[AA_DEFOCUS], std::fabs(c.defocus[p]) * 0.1f);
+  }
 
   // bounds on what may be tried (digHolo.cpp:12336-12359)
   float lim[AA_PARS][2];
@@ -1530,6 +1567,447 @@ int autoalign_tweak(Handle& h, float& goalDb) {
     fprintf(stdout, "AutoAlign goal %f  IL %f\n", goalDb, h.aaMetrics[h.aaLambdaCount]);
     fflush(stdout);
   }
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// AutoAlign parameter estimate (digHolo.cpp:11517-11930) and focus search
+// (:11181-11515), the first stage of the FULL and ESTIMATE modes.  From a clean
+// slate (centres at the middle of each pol's half of the frame, tilt and
+// defocus zero, FFT window the whole half frame):
+//   1. the tilt of each pol is the peak of its batch-summed Fourier intensity
+//      correlated with a Gaussian of the window's size (DCT-domain filter on
+//      the GPU, dh_align.hip), outside twice the window radius about the zero
+//      order; with the window radius aligned too, it shrinks to a third of the
+//      tilt and the search runs a second pass;
+//   2. the effective area of the Fourier power inside the window gives the
+//      beam's far-field size, the centre of mass and effective area of the
+//      reconstructed fields give its centre and waist (fitted relation
+//      AeffToMFD), and the ratio of the two sizes the scale of the defocus;
+//   3. the defocus minimising the effective area of the far field of the
+//      tilt-removed fields is searched with a quadratic fit (AutoFocus).
+// Every quantity is computed on the GPU; the host holds the scalars.
+// ---------------------------------------------------------------------------
+
+// Mode-field diameter from the effective area of a beam filling `groups` HG
+// groups: the reference's fitted relation (digHolo.cpp:2887-2900)
+float aeff_to_mfd(float Aeff, int groups) {
+  if (!groups) return 0.f;
+  const float A = (float)(Aeff / (15.1059697707643 * groups + 6.65360122093789));
+  return (float)((double)(2.0f * std::sqrt(A / PI_F)) / 0.223008930683135);
+}
+
+// Defocus magnitude from the beam size in the camera and Fourier planes
+// (digHolo.cpp:12040-12075); 0 when the ratio is not physical.
+float focus_estimate(float wCamera, float wFourier) {
+  if (!(wCamera > 0 && wFourier > 0)) return 0.f;
+  const float ratio = wFourier / wCamera;
+  const float a = 0.25f, b = PI_F / 2;
+  float r = 1.0f * 2 * (b * std::sqrt((1 - a) / (ratio - a) - 1));
+  if (!(r > 0) || std::isnan(r)) r = 0.f;
+  return r;
+}
+
+// Batch-summed Fourier-plane intensity of pol p into h.snapMap (the analysis of
+// digHolo.cpp:2354-2654 on the R2C plane in h.spec0, [nB][P][Ny][Nx]): pixels
+// with |k - wc|^2 >= R^2 (R > 0), <= R^2 (R < 0) or all (R == 0); the kx = 0
+// column, the ky = 0 and Nyquist rows and the columns past 8 floor(Nx / 8) are
+// not visited.  aeff = the effective area of the summed map.
+int fourier_pol_map(Handle& h, const dh::Geometry& g, int nB, int p, float wcx, float wcy, float R, float& aeff) {
+  const Config& c = h.cfg;
+  const int Nx = g.W / 2 + 1, Ny = g.H, N = Nx * Ny, P = g.P;
+  const float kfx = k_factor(g.W, c.pixelSize), kfy = k_factor(g.H, c.pixelSize);
+  std::vector<float> xs(Nx), ys(Ny);
+  for (int i = 0; i < Nx; ++i) xs[i] = k_axis(i, g.W, kfx);
+  for (int j = 0; j < Ny; ++j) ys[j] = k_axis(j, g.H, kfy);
+  const float sign = R >= 0 ? 1.f : -1.f;
+  const float w2 = R * R;
+  std::vector<uint8_t> inc((size_t)N);
+  for (int j = 0; j < Ny; ++j) {
+    const float dy = ys[j] + (-wcy);
+    const float dy2 = dy * dy;
+    for (int i = 0; i < Nx; ++i) {
+      const float dx = xs[i] + (-wcx);
+      const float r2 = sign * (dx * dx + dy2);
+      inc[(size_t)j * Nx + i] = r2 >= sign * w2 ? 1 : 0;
+    }
+  }
+  std::vector<float> axes(Nx + 2 * Ny);
+  std::copy(xs.begin(), xs.end(), axes.begin());
+  std::copy(ys.begin(), ys.end(), axes.begin() + Nx);
+  std::copy(ys.begin(), ys.end(), axes.begin() + Nx + Ny);   // COM y not used here
+  DH_TRY(h.anaAxes.ensure(axes.size()), "alloc");
+  DH_TRY(h.anaInc.ensure(inc.size()), "alloc");
+  DH_TRY(hipMemcpyAsync(h.anaAxes.p, axes.data(), axes.size() * sizeof(float), hipMemcpyHostToDevice, h.stream),
+         "axes upload");
+  DH_TRY(hipMemcpyAsync(h.anaInc.p, inc.data(), inc.size(), hipMemcpyHostToDevice, h.stream), "mask upload");
+  dh::PlaneArgs a{};
+  a.Nx = Nx;
+  a.Ny = Ny;
+  a.inStride = Nx;
+  a.x0 = 1;
+  a.x1 = 8 * (Nx / 8);
+  a.skip0 = 0;
+  a.skip1 = Ny / 2;
+  a.norm = (float)(1.0 / ((size_t)Nx * Ny));
+  a.x = h.anaAxes.p;
+  a.y = h.anaAxes.p + Nx;
+  a.yw = h.anaAxes.p + Nx + Ny;
+  a.include = h.anaInc.p;
+  DH_TRY(h.anaPw.ensure((size_t)nB * N), "alloc");
+  DH_TRY(h.snapMap.ensure((size_t)P * N), "alloc");
+  DH_TRY(h.snapStats.ensure((size_t)nB * 8 + 8), "alloc");
+  DH_TRY(dh::launch_plane_stats(a, h.spec0.p + (size_t)p * N, nullptr, nB, (size_t)P * N, h.anaPw.p, h.snapStats.p,
+                                h.stream), "stats");
+  DH_TRY(dh::launch_batch_sum(h.anaPw.p, nB, 1, N, h.snapMap.p + (size_t)p * N, (size_t)N, h.stream), "batch sum");
+  DH_TRY(dh::launch_plane_stats(a, nullptr, h.snapMap.p + (size_t)p * N, 1, (size_t)N, nullptr,
+                                h.snapStats.p + (size_t)nB * 8, h.stream), "stats");
+  double st[8];
+  DH_TRY(hipMemcpyAsync(st, h.snapStats.p + (size_t)nB * 8, sizeof(st), hipMemcpyDeviceToHost, h.stream), "D2H");
+  DH_TRY(hipStreamSynchronize(h.stream), "sync");
+  const float dA = std::fabs(xs[2] - xs[1]) * std::fabs(ys[2] - ys[1]);
+  const float tot = (float)st[0];
+  aeff = dA * (tot * tot / (float)st[4]);
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+// Tilt (degrees) of pol p from its map in h.snapMap (digHolo.cpp:11612-11708)
+int snap_tilt(Handle& h, const dh::Geometry& g, int p, float apRad, float& tx, float& ty) {
+  const Config& c = h.cfg;
+  const int Nx = g.W / 2 + 1, Ny = g.H, N = Nx * Ny;
+  const float k0 = (2 * PI_F) / c.wavelengthCentre;
+  const float aperturek = k0 * apRad;
+  const float wx = (float)((1.0 * PI_F / aperturek) * std::sqrt(2.0));
+  const float wy = (float)((2.0 * PI_F / aperturek) * std::sqrt(2.0));
+  const float wx2 = (float)(-1.0 / (wx * wx));
+  const float wy2 = (float)(-1.0 / (wy * wy));
+  float ad = aperturek * 2;
+  ad = ad * ad;
+  const float kfx = k_factor(g.W, c.pixelSize), kfy = k_factor(g.H, c.pixelSize);
+  std::vector<float> axes(Nx + Ny);
+  for (int i = 0; i < Nx; ++i) axes[i] = k_axis(i, g.W, kfx);
+  for (int j = 0; j < Ny; ++j) axes[Nx + j] = k_axis(j, g.H, kfy);
+  DH_TRY(h.snapAxes.ensure(axes.size()), "alloc");
+  DH_TRY(h.snapWork.ensure(dh::snap_work_elems(Ny, Nx)), "alloc");
+  DH_TRY(h.snapIdx.ensure(1), "alloc");
+  DH_TRY(hipMemcpyAsync(h.snapAxes.p, axes.data(), axes.size() * sizeof(float), hipMemcpyHostToDevice, h.stream),
+         "axes upload");
+  DH_TRY(dh::launch_snap_tilt_search(h.snapMap.p + (size_t)p * N, Ny, Nx, c.pixelSize, wx2, wy2, h.snapAxes.p,
+                                     h.snapAxes.p + Nx, ad, h.snapWork.p, h.snapIdx.p, h.stream), "tilt search");
+  int idx = -1;
+  DH_TRY(hipMemcpyAsync(&idx, h.snapIdx.p, sizeof(int), hipMemcpyDeviceToHost, h.stream), "D2H");
+  DH_TRY(hipStreamSynchronize(h.stream), "sync");
+  float kx = 0.f, ky = 0.f;
+  if (idx >= 0) {
+    kx = axes[idx % Nx];
+    ky = axes[Nx + idx / Nx];
+  }
+  tx = std::asin(kx / k0) / UNIT_ANGLE_F;
+  ty = std::asin(ky / k0) / UNIT_ANGLE_F;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+// Effective area of the batch-summed far-field intensity of each pol's fields
+// (the unnormalised inverse 2-D DFT of the tilt-removed field,
+// digHolo.cpp:11280-11380): dA (sum |F|^2)^2 / sum_pixels (sum_batch |F|^2)^2.
+int farfield_aeff(Handle& h, float* aeff) {
+  const Derived& d = h.cur;
+  const dh::Geometry& g = d.g;
+  const int B = g.B, P = g.P, N = g.fw * g.fh;
+  dh::Geometry gF = g;
+  gF.w = g.fw;
+  gF.h = g.fh;
+  const dh::FFTPlan* pIx = fft_plan(h, g.fw);
+  const dh::FFTPlan* pIy = fft_plan(h, g.fh);
+  if (!pIx || !pIy) return DIGHOLO_ERROR_MEMORYALLOCATION;
+  const size_t n = (size_t)B * P * N;
+  DH_TRY(h.ffBuf.ensure(2 * n), "alloc");
+  DH_TRY(h.ffMap.ensure((size_t)P * N + (size_t)std::max(g.fw, g.fh)), "alloc");
+  DH_TRY(h.anaPw.ensure(n), "alloc");
+  DH_TRY(h.snapStats.ensure((size_t)B * P * 8 + (size_t)P * 8), "alloc");
+  float* zeroAxis = h.ffMap.p + (size_t)P * N;
+  DH_TRY(hipMemsetAsync(zeroAxis, 0, (size_t)std::max(g.fw, g.fh) * sizeof(float), h.stream), "memset");
+  DH_TRY(hipMemcpyAsync(h.ffBuf.p, h.fields.p, n * sizeof(float2), hipMemcpyDeviceToDevice, h.stream), "copy");
+  DH_TRY(dh::launch_ifft_rows(gF, *pIx, h.ffBuf.p, h.stream), "far-field rows");
+  DH_TRY(dh::launch_ifft_cols_tilt(gF, *pIy, h.ffBuf.p, nullptr, h.ffBuf.p + n, h.stream), "far-field columns");
+  dh::PlaneArgs a{};
+  a.Nx = g.fw;
+  a.Ny = g.fh;
+  a.inStride = g.fw;
+  a.x0 = 0;
+  a.x1 = g.fw;
+  a.skip0 = a.skip1 = -1;
+  a.norm = 1.f;
+  a.x = a.y = a.yw = zeroAxis;
+  DH_TRY(dh::launch_plane_stats(a, h.ffBuf.p + n, nullptr, B * P, (size_t)N, h.anaPw.p, h.snapStats.p, h.stream),
+         "stats");
+  DH_TRY(dh::launch_batch_sum(h.anaPw.p, B, P, N, h.ffMap.p, (size_t)N, h.stream), "batch sum");
+  DH_TRY(dh::launch_plane_stats(a, nullptr, h.ffMap.p, P, (size_t)N, nullptr, h.snapStats.p + (size_t)B * P * 8,
+                                h.stream), "stats");
+  std::vector<double> st((size_t)P * 8);
+  DH_TRY(hipMemcpyAsync(st.data(), h.snapStats.p + (size_t)B * P * 8, st.size() * sizeof(double),
+                        hipMemcpyDeviceToHost, h.stream), "D2H");
+  DH_TRY(hipStreamSynchronize(h.stream), "sync");
+  const float kfx = k_factor(g.W, h.cfg.pixelSize), kfy = k_factor(g.H, h.cfg.pixelSize);
+  const float dA = std::fabs(k_axis(2, g.W, kfx) - k_axis(1, g.W, kfx)) *
+                   std::fabs(k_axis(2, g.H, kfy) - k_axis(1, g.H, kfy));
+  for (int p = 0; p < P; ++p) {
+    const float v2 = (float)st[(size_t)p * 8 + 0];
+    aeff[p] = dA * ((v2 * v2) / (float)st[(size_t)p * 8 + 4]);
+  }
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+// Defocus of least far-field effective area (digHoloAutoFocus,
+// digHolo.cpp:11181-11515).  Trials +s, -s, then along the better sign in
+// quarter steps of s (s = the estimate, at least 1 dioptre), and from the fifth
+// trial on the vertex of the least-squares quadratic through the four trials
+// of smallest area, until the vertex is within 0.01 of the best trial (25 at
+// most).  The fields are those of the beam centres cx/cy, which the
+// reference's last inverse transform used.
+int autofocus(Handle& h, float* scale, float* best, const float* cx, const float* cy) {
+  Config& c = h.cfg;
+  const int P = c.polCount;
+  constexpr int stepCount = 25, stepFitCount = 5;
+  const float tol = 0.01f;
+  float aeffs[2][stepCount + 3] = {}, defs[2][stepCount + 3] = {};
+  float bestAeff[2] = {std::numeric_limits<float>::max(), std::numeric_limits<float>::max()};
+  bool converged[2] = {false, false};
+  for (int p = 0; p < P; ++p) {
+    best[p] = 0.f;
+    if (scale[p] < 1.0f) scale[p] = 1.0f;
+  }
+  float s0[2] = {scale[0], P > 1 ? scale[1] : 0.f};
+  for (int st = 0; st < stepCount; st += 2)
+    for (int p = 0; p < P; ++p) {
+      defs[p][st] = s0[p];
+      defs[p][st + 1] = -s0[p];
+      s0[p] += scale[p] * 0.25f;
+    }
+  float keepCX[2], keepCY[2];
+  for (int p = 0; p < 2; ++p) {
+    keepCX[p] = c.beamCX[p];
+    keepCY[p] = c.beamCY[p];
+    c.beamCX[p] = cx[p];
+    c.beamCY[p] = cy[p];
+  }
+  // (with the defocus pol-locked, pol 1's fields take pol 0's trial value, as
+  // the reference's reference wave does)
+  bool done = false;
+  int e = DIGHOLO_ERROR_SUCCESS;
+  for (int st = 0; st < stepCount && !done; ++st) {
+    done = true;
+    for (int p = 0; p < P; ++p) c.defocus[p] = defs[p][st];
+    float aeff[2] = {0.f, 0.f};
+    if ((e = enqueue_batch(h)) || (e = farfield_aeff(h, aeff))) break;
+    for (int p = 0; p < P; ++p) {
+      aeffs[p][st] = aeff[p];
+      if (aeff[p] < bestAeff[p]) {
+        bestAeff[p] = aeff[p];
+        best[p] = defs[p][st];
+      }
+      if (st == 1) {   // continue on the side of the better of +s and -s
+        float stepScale, base;
+        if (aeffs[p][1] < aeffs[p][0]) {
+          stepScale = defs[p][1] - defs[p][0];
+          base = defs[p][1];
+        } else {
+          stepScale = defs[p][0] - defs[p][1];
+          base = defs[p][0];
+        }
+        int k = 1;
+        for (int i = st + 1; i < stepCount; i += 2) {
+          defs[p][i] = base + k * 0.25f * stepScale;
+          defs[p][i + 1] = base - k * 0.25f * stepScale;
+          ++k;
+        }
+      }
+      if (st >= stepFitCount - 1) {
+        // the stepFitCount - 1 trials of smallest area (stable order, as the
+        // reference's bubble sort)
+        std::vector<int> order(st + 1);
+        for (int i = 0; i <= st; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return aeffs[p][a] < aeffs[p][b]; });
+        double xs[stepFitCount - 1], ys[stepFitCount - 1];
+        for (int i = 0; i < stepFitCount - 1; ++i) {
+          xs[i] = defs[p][order[i]];
+          ys[i] = aeffs[p][order[i]];
+        }
+        const float vx = (float)quadratic_vertex(xs, ys, stepFitCount - 1);
+        if (st + 1 < stepCount - 1 && !std::isnan(vx)) defs[p][st + 1] = vx;
+        if (std::fabs(vx - best[p]) < tol) converged[p] = true;
+      }
+      done = done && converged[p];
+    }
+  }
+  for (int p = 0; p < 2; ++p) {
+    c.beamCX[p] = keepCX[p];
+    c.beamCY[p] = keepCY[p];
+  }
+  return e;
+}
+
+// The estimate stage.  doFinal: process once more and store the metrics
+// (ESTIMATE mode, or no basis).  steps: base step sizes for the tweak stage.
+int autoalign_snap(Handle& h, bool doFinal, float* steps) {
+  Config& c = h.cfg;
+  const int P = c.polCount;
+  const float px = c.pixelSize;
+  for (int p = 0; p < P; ++p) {
+    if (c.aaCentre) {
+      c.beamCX[p] = (float)((-(P - 1) * (c.frameWidth / 4.0) + (p * c.frameWidth / 2.0)) * px);
+      c.beamCY[p] = 0.f;
+    }
+    if (c.aaTilt) c.tiltX[p] = c.tiltY[p] = 0.f;
+    if (c.aaDefocus) c.defocus[p] = 0.f;
+  }
+  float waists[2][2] = {};
+  c.fftWindowSizeX = c.frameWidth / P;
+  c.fftWindowSizeY = c.frameHeight;
+  if (c.batchCount < 1) c.batchCount = 1;
+
+  // the R2C Fourier plane of every frame under the whole-half-frame window
+  int W, H, cxw[2] = {0, 0}, cyw[2] = {0, 0}, err;
+  if (!fft_window(c, W, H, cxw, cyw, err)) return err;
+  if (int e = ensure_device(h)) return e;
+  const int nB = c.batchCount * std::max(1, c.avgCount);
+  const void* dframes = nullptr;
+  if (int e = device_frames(h, (size_t)nB, dframes)) return e;
+  dh::Geometry g{};
+  g.B = nB;
+  g.P = P;
+  g.frameW = c.frameWidth;
+  g.frameH = c.frameHeight;
+  g.W = W;
+  g.H = H;
+  g.w = W / 2 + 1;
+  g.h = H;
+  g.fw = g.w;
+  g.fh = g.h;
+  for (int p = 0; p < P; ++p) {
+    g.cx[p] = cxw[p];
+    g.cy[p] = cyw[p];
+  }
+  const dh::FFTPlan* pRow = fft_plan(h, W / 2);
+  const dh::FFTPlan* pCol = fft_plan(h, H);
+  if (!pRow || !pCol) return DIGHOLO_ERROR_MEMORYALLOCATION;
+  if (int e = ensure_row_twiddles(h, W)) return e;
+  const int Nx = W / 2 + 1;
+  DH_TRY(h.spec0.ensure((size_t)nB * P * H * Nx), "alloc");
+  DH_TRY(dh::launch_row_fft(g, *pRow, h.twW.p, dframes, h.frameU16, h.spec0.p, h.stream), "row FFT");
+  DH_TRY(dh::launch_col_fft_full(g, *pCol, h.spec0.p, h.stream), "column FFT");
+  h.analysisValid[0] = false;   // spec0 no longer holds the last batch's plane
+
+  const float k0 = (2 * PI_F) / c.wavelengthCentre;
+  float apRad = c.apertureSize * UNIT_ANGLE_F;
+  auto zero_order_maps = [&]() -> int {   // digHoloFFTAnalysis (digHolo.cpp:6931-6984)
+    const float R = 2 * k0 * std::sin(apRad);
+    float unused;
+    for (int p = 0; p < P; ++p)
+      if (int e = fourier_pol_map(h, g, nB, p, 0.f, 0.f, R, unused)) return e;
+    return 0;
+  };
+  if (int e = zero_order_maps()) return e;
+  const int passCount = 1 + (c.aaFourierRadius ? 1 : 0);
+  for (int pass = 0; pass < passCount; ++pass) {
+    for (int p = 0; p < P; ++p) {
+      float tx, ty;
+      if (int e = snap_tilt(h, g, p, apRad, tx, ty)) return e;
+      if (c.aaFourierRadius) {
+        const float apMax = std::sqrt(tx * tx + ty * ty) / 3.0f;
+        if (apMax < c.apertureSize) {
+          c.apertureSize = apMax;
+          apRad = apMax * UNIT_ANGLE_F;
+          if (int e = zero_order_maps()) return e;
+        }
+      }
+      if (c.aaTilt) {
+        c.tiltX[p] = tx;
+        c.tiltY[p] = ty;
+      }
+      const float wcx = k0 * (c.tiltX[p] * UNIT_ANGLE_F);
+      const float wcy = k0 * (c.tiltY[p] * UNIT_ANGLE_F);
+      const float wr = -k0 * std::sin(apRad);
+      float aeff;
+      if (int e = fourier_pol_map(h, g, nB, p, wcx, wcy, wr, aeff)) return e;
+      const float wEst = aeff_to_mfd(aeff, c.maxMG);
+      waists[0][p] = (float)((1.0 / wEst) * PI_F);
+      if (c.verbosity >= DIGHOLO_VERBOSITY_DEBUG) {
+        fprintf(stdout, "Tilt\t%d\t%f,%f\n", p, tx, ty);
+        fflush(stdout);
+      }
+    }
+  }
+
+  // reconstructed plane: centre of mass and effective area (digHolo.cpp:11800-11905)
+  constexpr int maxAttempts = 10;
+  bool valid = true;
+  int attempts = 0;
+  float maxFocus = 0.25f;
+  float defEst[2] = {0.f, 0.f};
+  float usedCX[2] = {c.beamCX[0], c.beamCX[1]}, usedCY[2] = {c.beamCY[0], c.beamCY[1]};
+  while ((!valid || attempts == 0) && attempts < maxAttempts) {
+    valid = true;
+    for (int p = 0; p < 2; ++p) {
+      usedCX[p] = c.beamCX[p];
+      usedCY[p] = c.beamCY[p];
+    }
+    if (int e = enqueue_batch(h)) return e;
+    if (int e = compute_analysis(h, 1)) return e;
+    const int B = h.cur.g.B;
+    const std::vector<float>& an = h.analysis[1];
+    auto at = [&](int par, int p) { return an[((size_t)par * P + p) * (B + 1) + B]; };
+    for (int p = 0; p < P; ++p) {
+      const float polOffsetX = (float)(((2 * p - 1) * c.pixelSize * c.frameWidth / 4.0) * (P - 1));
+      const float comx = at(1, p) / 1 + polOffsetX;
+      const float comy = at(2, p) / 1;
+      const float Aeff = at(5, p);
+      const float wEst = (float)(aeff_to_mfd(Aeff, c.maxMG) * 0.5);
+      waists[1][p] = wEst;
+      defEst[p] = focus_estimate(waists[1][p], waists[0][p]);
+      if (c.verbosity >= DIGHOLO_VERBOSITY_DEBUG) {
+        fprintf(stdout, "%d\tBeamCentre\t(%e\t%e)\tMFD:%e\tDefocus:%f\tAeff:%e\n", p, comx, comy, wEst, defEst[p],
+                Aeff);
+        fflush(stdout);
+      }
+      if (std::isnan(comx) || std::isnan(comy) || std::isnan(Aeff)) {
+        valid = false;   // probably the zero order in the window: shrink it
+        c.apertureSize *= 0.9f;
+      }
+      if (valid) {
+        if (c.aaCentre) {
+          c.beamCX[p] = comx;
+          c.beamCY[p] = comy;
+        }
+        if (c.aaWaist) c.waist[p] = wEst / 1;
+        if (c.aaDefocus) c.defocus[p] = 0.f;
+        if (std::fabs(defEst[p]) > maxFocus) maxFocus = std::fabs(defEst[p]);
+      }
+      ++attempts;
+    }
+  }
+
+  if (c.aaDefocus) {
+    float best[2] = {0.f, 0.f};
+    if (int e = autofocus(h, defEst, best, usedCX, usedCY)) return e;
+    for (int p = 0; p < P; ++p) {
+      c.defocus[p] = best[p];
+      if (std::fabs(best[p]) > maxFocus) maxFocus = std::fabs(best[p]);
+    }
+  }
+
+  if (doFinal) {
+    if (int e = enqueue_batch(h)) return e;
+    if (h.cur.g.M > 0) {
+      PowerMeterOut o;
+      if (int e = power_meter(h, -1, c.aaPolIndependence != 0, true, true, true, c.aaMulConjTrans != 0, o)) return e;
+      store_metrics_db(h, o);
+    }
+  }
+  steps[AA_TILTX] = steps[AA_TILTY] = c.apertureSize * 0.1f;
+  steps[AA_CX] = steps[AA_CY] = c.waist[0] * 0.1f;
+  steps[AA_DEFOCUS] = 0.1f * maxFocus;
+  steps[AA_WAIST] = c.waist[0] * 0.1f;
   return DIGHOLO_ERROR_SUCCESS;
 }
 
@@ -2246,19 +2724,37 @@ EXT_C int digHoloAutoAlignCalcMetrics(int handleIdx) {
   store_metrics_db(*h, o);
   return DIGHOLO_ERROR_SUCCESS;
 }
-// The tweak stage only: the parameter-estimate ("snap") stage of the FULL and
-// ESTIMATE modes is not built, so those modes report an error.
+// digHolo.cpp:12237-12970: FULL runs the parameter estimate then the tweak
+// search, ESTIMATE the estimate alone (then a final pass whose goal metric is
+// returned), TWEAK the search from the current values.  The FFT window size is
+// restored afterwards.
 EXT_C float digHoloAutoAlign(int handleIdx) {
   H_OR(0.f);
   const float FMAX = std::numeric_limits<float>::max();
-  if (CFG.aaMode != DIGHOLO_AUTOALIGNMODE_TWEAK || CFG.maxMG <= 0) {
-    fprintf(stderr, "digHolo: AutoAlign needs DIGHOLO_AUTOALIGNMODE_TWEAK and a basis group count > 0 in this "
-                    "build (the parameter-estimate stage is not implemented)\n");
-    return -FMAX;
-  }
   if (!h->frames) return -FMAX;
+  Config& c = CFG;
+  const int mode = c.aaMode;
+  const int winX = c.fftWindowSizeX, winY = c.fftWindowSizeY;
+  HoldGuard hold(*h);
+  if (c.aaFourierRadius)   // digHolo.cpp:12274-12289
+    c.apertureSize = c.aaTilt ? window_radius_max(c.pixelSize, c.wavelengthCentre) : window_radius_rec(c);
+  float steps[AA_PARS];
+  bool snapped = false;
+  int e = DIGHOLO_ERROR_SUCCESS;
+  const bool estimateOnly = mode == DIGHOLO_AUTOALIGNMODE_ESTIMATE || c.maxMG == 0;
+  if (mode == DIGHOLO_AUTOALIGNMODE_FULL || estimateOnly) {
+    h->aaMetrics.clear();
+    e = autoalign_snap(*h, estimateOnly, steps);
+    snapped = true;
+  }
+  c.fftWindowSizeX = winX;
+  c.fftWindowSizeY = winY;
   float goal = -FMAX;
-  const int e = autoalign_tweak(*h, goal);
+  if (!e && (mode == DIGHOLO_AUTOALIGNMODE_FULL || mode == DIGHOLO_AUTOALIGNMODE_TWEAK) && c.maxMG > 0) {
+    e = autoalign_tweak(*h, goal, snapped ? steps : nullptr);
+  } else if (!e && !h->aaMetrics.empty()) {
+    goal = h->aaMetrics[(size_t)c.aaGoalIdx * (h->aaLambdaCount + 1) + h->aaLambdaCount];
+  }
   if (e) {
     fprintf(stderr, "digHolo: AutoAlign failed (error %d)\n", e);
     return -FMAX;

@@ -168,6 +168,15 @@ hipError_t launch_fields16(const float2* fields, int B, int P, int hw, const flo
 // In-place FFT along y of all g.w columns of [B][P][H][g.w] R2C rows
 hipError_t launch_col_fft_full(const Geometry& g, const FFTPlan& pl, float2* spec, hipStream_t s);
 
+// AutoAlign parameter estimate (dh_align.hip): tilt search of one batch-summed
+// R2C Fourier map [Ny][Nx] (float) -- DCT-II both axes, Gaussian filter with
+// exponent (c px)^2 wx2 + (j px)^2 wy2, DCT-III both axes -- then the index of
+// the largest filtered value with kx[i]^2 + ky[j]^2 > ad (-1: none).
+// work: snap_work_elems(Ny, Nx) doubles.
+size_t snap_work_elems(int Ny, int Nx);
+hipError_t launch_snap_tilt_search(const float* map, int Ny, int Nx, float px, float wx2, float wy2, const float* kx,
+                                   const float* ky, float ad, double* work, int* outIdx, hipStream_t s);
+
 // Separable HG overlap on MFMA: split = 0 plain FP32, 1 split-FP16 (hi+lo).
 hipError_t launch_overlap_mfma(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
                                float2* coefs, int split, hipStream_t s);

@@ -162,8 +162,10 @@ EXT_C int digHoloConfigSetAutoAlignGoalIdx(int handleIdx, int goalIdx);         
 EXT_C int digHoloConfigGetAutoAlignGoalIdx(int handleIdx);                        /* digHolo.h:1136 */
 
 /* ---- AutoAlign ---- */
-/* Tweak mode (DIGHOLO_AUTOALIGNMODE_TWEAK) only; returns the goal metric in dB,
-   -FLT_MAX for the modes that need the parameter-estimate stage. */
+/* FULL (estimate from the frames, then tweak), TWEAK (search from the current
+   values) and ESTIMATE (estimate only, then one evaluation at the whole-half-
+   frame FFT window it used); returns the goal metric in dB (-FLT_MAX on error,
+   or with no basis, where the power meter has no modes to measure). */
 EXT_C float digHoloAutoAlign(int handleIdx);                                      /* digHolo.h:1152 */
 EXT_C float digHoloAutoAlignGetMetric(int handleIdx, int metricIdx);              /* digHolo.h:1163 */
 EXT_C float* digHoloAutoAlignGetMetrics(int handleIdx, int metricIdx);            /* digHolo.h:1174 */

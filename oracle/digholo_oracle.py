@@ -702,3 +702,320 @@ def relative_l2(a, b) -> float:
     b = np.asarray(b)
     den = np.linalg.norm(b.ravel())
     return float(np.linalg.norm((a - b).ravel()) / den) if den > 0 else float(np.linalg.norm(a.ravel()))
+
+
+# ---------------------------------------------------------------------------
+# AutoAlign parameter estimate (digHolo.cpp:11517-11930), focus search
+# (:11181-11515) and the fitted beam-size relations (:2887-2900, :12040-12075)
+# ---------------------------------------------------------------------------
+
+def r2r_matrix(n: int, kind: int) -> np.ndarray:
+    """Matrix of FFTW's unnormalised real-to-real transform of length n
+    (out = M @ x): kind 10 = REDFT10 (DCT-II), Y_k = 2 sum_j x_j cos(pi (2j+1) k / 2n);
+    kind 1 = REDFT01 (DCT-III), Y_k = x_0 + 2 sum_{j>=1} x_j cos(pi j (2k+1) / 2n)."""
+    k = np.arange(n)[:, None].astype(np.float64)
+    j = np.arange(n)[None, :].astype(np.float64)
+    if kind == 10:
+        return 2.0 * np.cos(np.pi * (2 * j + 1) * k / (2 * n))
+    M = 2.0 * np.cos(np.pi * j * (2 * k + 1) / (2 * n))
+    M[:, 0] = 1.0
+    return M
+
+
+def r2r_2d(m: np.ndarray, kind: int) -> np.ndarray:
+    """2-D transform of an [Ny, Nx] map along both axes (fftwf_plan_r2r_2d, :7393-7420)."""
+    Ny, Nx = m.shape
+    return r2r_matrix(Ny, kind) @ m @ r2r_matrix(Nx, kind).T
+
+
+def snap_gauss(m: np.ndarray, px: float, wx2: np.float32, wy2: np.float32) -> np.ndarray:
+    """The Gaussian weighting of the DCT-domain map as the reference applies it
+    (:11636-11660): its loop counters run over an (8 floor(Nx/8)) x Ny array while
+    the map's rows are Nx long, so flat element f < Ny * NxR gets the weight of
+    counters j = f // NxR, c = f % NxR; the rest keep their value."""
+    Ny, Nx = m.shape
+    NxR = 8 * (Nx // 8)
+    out = m.ravel().copy()
+    f = np.arange(Ny * NxR)
+    j = f // NxR
+    c = f % NxR
+    pxf = F32(px)
+    y2 = (j * j).astype(np.float32) * F32(pxf * pxf)
+    x = c.astype(np.float32) * pxf
+    e = (F32(1) * (x * x) * F32(wx2)).astype(np.float32) + (y2 * F32(wy2)).astype(np.float32)
+    out[:Ny * NxR] *= np.exp(e.astype(np.float64))
+    return out.reshape(Ny, Nx)
+
+
+def snap_tilt(summed: np.ndarray, W: int, H: int, pixel_size: float, wavelength: float, ap_rad: np.float32):
+    """Tilt (degrees, float32) from one batch-summed R2C Fourier map [H, W/2+1]
+    (:11612-11708): DCT-II, Gaussian of the window size, DCT-III, then the first
+    maximum outside k^2 > (2 k0 r)^2."""
+    k0 = F32(F32(2.0) * PI_F32 / F32(wavelength))
+    aperturek = F32(k0 * F32(ap_rad))
+    wx = F32((1.0 * float(PI_F32) / float(aperturek)) * math.sqrt(2.0))
+    wy = F32((2.0 * float(PI_F32) / float(aperturek)) * math.sqrt(2.0))
+    wx2 = F32(-1.0 / float(F32(wx * wx)))
+    wy2 = F32(-1.0 / float(F32(wy * wy)))
+    m = r2r_2d(np.asarray(summed, dtype=np.float64), 10)
+    m = snap_gauss(m, pixel_size, wx2, wy2)
+    m = r2r_2d(m, 1)
+    Nx = W // 2 + 1
+    kx = k_axis_f32(W, pixel_size)[:Nx]
+    ky = k_axis_f32(H, pixel_size)
+    ad = F32(aperturek * F32(2))
+    ad = F32(ad * ad)
+    kr = (kx[None, :] * kx[None, :]).astype(np.float32) + (ky[:, None] * ky[:, None]).astype(np.float32)
+    cand = np.where(kr > ad, m, -np.inf).ravel()
+    if not np.isfinite(cand).any():
+        kxs, kys = F32(0), F32(0)
+    else:
+        f = int(np.argmax(cand))                       # first maximum
+        kxs, kys = kx[f % Nx], ky[f // Nx]
+    tx = F32(F32(np.arcsin(F32(kxs / k0))) / UNIT_ANGLE_F32)
+    ty = F32(F32(np.arcsin(F32(kys / k0))) / UNIT_ANGLE_F32)
+    return tx, ty
+
+
+def aeff_to_mfd(aeff: float, groups: int) -> np.float32:
+    """Mode-field diameter from the effective area, the reference's fitted
+    relation for a beam filling `groups` HG groups (:2887-2900)."""
+    if not groups:
+        return F32(0)
+    A = F32(float(aeff) / (15.1059697707643 * groups + 6.65360122093789))
+    return F32(float(F32(2.0) * F32(np.sqrt(F32(A / PI_F32)))) / 0.223008930683135)
+
+
+def focus_estimate(w_camera: float, w_fourier: float) -> np.float32:
+    """Defocus magnitude from the beam radii in the two planes (:12040-12075)."""
+    if not (w_camera > 0 and w_fourier > 0):
+        return F32(0)
+    ratio = F32(F32(w_fourier) / F32(w_camera))
+    a = F32(0.25)
+    b = F32(PI_F32 / F32(2))
+    with np.errstate(invalid="ignore"):
+        r = F32(F32(2) * F32(b * F32(np.sqrt(F32(F32(F32(1) - a) / F32(ratio - a)) - F32(1)))))
+    if not (r > 0) or np.isnan(r):
+        r = F32(0)
+    return r
+
+
+def quadratic_vertex(x, y) -> float:
+    """-b / 2a of the least-squares quadratic a x^2 + b x + c through the points
+    (the fixed point of fitToQuadratic's Gauss-Newton iteration, :3146-3257)."""
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    if not (np.all(np.isfinite(x)) and np.all(np.isfinite(y))):
+        return float("nan")
+    A = np.stack([x * x, x, np.ones_like(x)], axis=1)
+    (a, b, _), *_ = np.linalg.lstsq(A, y, rcond=None)
+    return float(-b / (2 * a)) if a != 0 else float("nan")
+
+
+def farfield_aeff(fields: np.ndarray, W: int, H: int, pixel_size: float) -> np.ndarray:
+    """Effective area of the batch-summed far-field intensity of each pol
+    (:11280-11380): the unnormalised inverse 2-D DFT of every field, summed |F|^2
+    over the batch, dA (sum I)^2 / sum I^2 with dA the Fourier-window k-step area."""
+    F = np.fft.ifft2(fields, axes=(-2, -1)) * (fields.shape[-1] * fields.shape[-2])
+    I = (np.abs(F) ** 2).sum(axis=0)                   # [P, fh, fw]
+    kx = k_axis_f32(W, pixel_size)
+    ky = k_axis_f32(H, pixel_size)
+    dA = F32(F32(abs(F32(kx[2] - kx[1]))) * F32(abs(F32(ky[2] - ky[1]))))
+    v2 = I.sum(axis=(1, 2))
+    v4 = (I * I).sum(axis=(1, 2))
+    return np.array([float(F32(dA * F32(F32(v2[p]) * F32(v2[p]) / F32(v4[p])))) for p in range(I.shape[0])])
+
+
+def _fourier_window_map(spec_p: np.ndarray, W: int, H: int, pixel_size: float, wcx, wcy, R):
+    """Batch-summed Fourier intensity of one pol ([B, H, W/2+1] R2C planes) and
+    its effective area: |k - wc|^2 >= R^2 (R > 0), <= R^2 (R < 0), all (R = 0);
+    kx = 0 column, ky = 0 / Nyquist rows and columns past 8 floor(Nx/8) left out."""
+    Nx = W // 2 + 1
+    kx = k_axis_f32(W, pixel_size)[:Nx]
+    ky = k_axis_f32(H, pixel_size)
+    R = F32(R)
+    sign = F32(1) if R >= 0 else F32(-1)
+    dx = (kx + F32(-F32(wcx))).astype(np.float32)
+    dy = (ky + F32(-F32(wcy))).astype(np.float32)
+    d2 = ((dx * dx)[None, :] + (dy * dy)[:, None]).astype(np.float32)
+    inc = (sign * d2).astype(np.float32) >= F32(sign * F32(R * R))
+    params, summed = plane_analysis(spec_p[:, None], kx, ky, include=inc, skip_rows=(0, H // 2),
+                                    col_range=(1, 8 * (Nx // 8)))
+    return summed[0], float(F32(params[ANALYSIS_AEFF, 0, -1]))
+
+
+def autoalign_estimate(frames: np.ndarray, cfg: Config, align_centre=True, align_tilt=True,
+                       align_defocus=True, align_waist=True, align_radius=True):
+    """The parameter estimate of AutoAlign's FULL / ESTIMATE modes
+    (digHolo.cpp:12274-12289, :11517-11930, :11181-11515) from the frames alone.
+
+    Returns the config it ends on (tilt, centres, waist, defocus, window radius;
+    the FFT window is left at the whole half frame, as during the estimate) and a
+    dict of the intermediate estimates."""
+    import copy
+    c = copy.deepcopy(cfg)
+    P = c.pol_count
+    px = F32(c.pixel_size)
+    lam = F32(c.wavelength)
+    info = {}
+    if align_radius:                                   # :12274-12289
+        if align_tilt:
+            wc = F32((-2.0 + math.sqrt(68.0)) / 16.0)
+            wmax = F32(float(lam) / (2.0 * float(px)))
+            c.aperture_deg = float(F32(F32(wc * wmax) / UNIT_ANGLE_F32))
+        else:
+            t = min(float(F32(np.sqrt(F32(F32(c.tilt_x[p]) ** 2 + F32(c.tilt_y[p]) ** 2)))) for p in range(P))
+            c.aperture_deg = float(F32(F32(t) / F32(3)))
+    for p in range(P):
+        if align_centre:
+            c.beam_centre_x[p] = float(F32((-(P - 1) * (c.frame_width / 4.0) + (p * c.frame_width / 2.0)) * float(px)))
+            c.beam_centre_y[p] = 0.0
+        if align_tilt:
+            c.tilt_x[p] = c.tilt_y[p] = 0.0
+        if align_defocus:
+            c.defocus[p] = 0.0
+    c.fft_window_x = c.frame_width // P
+    c.fft_window_y = c.frame_height
+    W, H, cx, cy = fft_window(c)
+    frames = np.asarray(frames)
+    spec = [np.fft.rfft2(frames[:, cy[p]:cy[p] + H, cx[p]:cx[p] + W].astype(np.float64), axes=(-2, -1))
+            for p in range(P)]
+    k0 = F32(F32(2.0) * PI_F32 / lam)
+    ap = F32(F32(c.aperture_deg) * UNIT_ANGLE_F32)
+    maps = [None] * P
+
+    def zero_order_maps():
+        R = F32(F32(F32(2) * k0) * F32(math.sin(float(ap))))
+        for q in range(P):
+            maps[q] = _fourier_window_map(spec[q], W, H, c.pixel_size, 0.0, 0.0, R)[0]
+
+    zero_order_maps()
+    waists = np.zeros((2, 2), dtype=np.float32)
+    tilts = []
+    for _ in range(1 + int(bool(align_radius))):
+        for p in range(P):
+            tx, ty = snap_tilt(maps[p], W, H, c.pixel_size, c.wavelength, ap)
+            tilts.append((float(tx), float(ty)))
+            if align_radius:
+                apmax = F32(F32(np.sqrt(F32(tx * tx + ty * ty))) / F32(3))
+                if apmax < F32(c.aperture_deg):
+                    c.aperture_deg = float(apmax)
+                    ap = F32(apmax * UNIT_ANGLE_F32)
+                    zero_order_maps()
+            if align_tilt:
+                c.tilt_x[p], c.tilt_y[p] = float(tx), float(ty)
+            wcx = F32(k0 * F32(F32(c.tilt_x[p]) * UNIT_ANGLE_F32))
+            wcy = F32(k0 * F32(F32(c.tilt_y[p]) * UNIT_ANGLE_F32))
+            wr = F32(-F32(k0 * F32(math.sin(float(ap)))))
+            maps[p], aeff = _fourier_window_map(spec[p], W, H, c.pixel_size, wcx, wcy, wr)
+            west = aeff_to_mfd(aeff, c.group_count)
+            waists[0, p] = F32((1.0 / float(west)) * float(PI_F32)) if west != 0 else F32(np.inf)
+    info["tilts"] = tilts
+    # reconstructed plane
+    max_focus = F32(0.25)
+    def_est = [F32(0), F32(0)]
+    used_cx, used_cy = list(c.beam_centre_x), list(c.beam_centre_y)
+    valid, attempts = True, 0
+    while (not valid or attempts == 0) and attempts < 10:
+        valid = True
+        used_cx, used_cy = list(c.beam_centre_x), list(c.beam_centre_y)
+        fields, _, plan = process_batch(frames, c)
+        params, _ = field_plane_analysis(fields, c, plan)
+        B = fields.shape[0]
+        for p in range(P):
+            pol_off = F32(((2 * p - 1) * float(F32(F32(px) * F32(c.frame_width))) / 4.0) * (P - 1))
+            comx = F32(F32(params[ANALYSIS_COMX, p, B]) + pol_off)
+            comy = F32(params[ANALYSIS_COMY, p, B])
+            aeff = F32(params[ANALYSIS_AEFF, p, B])
+            west = F32(float(aeff_to_mfd(aeff, c.group_count)) * 0.5)
+            waists[1, p] = west
+            def_est[p] = focus_estimate(waists[1, p], waists[0, p])
+            if np.isnan(comx) or np.isnan(comy) or np.isnan(aeff):
+                valid = False
+                c.aperture_deg = float(F32(F32(c.aperture_deg) * F32(0.9)))
+            if valid:
+                if align_centre:
+                    c.beam_centre_x[p], c.beam_centre_y[p] = float(comx), float(comy)
+                if align_waist:
+                    c.waist[p] = float(west)
+                if align_defocus:
+                    c.defocus[p] = 0.0
+                max_focus = max(max_focus, F32(abs(def_est[p])))
+            attempts += 1
+    info["defocus_estimate"] = [float(v) for v in def_est[:P]]
+    info["waists"] = waists.copy()
+    if align_defocus:
+        best, trials = autofocus(frames, c, [float(v) for v in def_est], used_cx, used_cy)
+        info["focus_trials"] = trials
+        for p in range(P):
+            c.defocus[p] = float(best[p])
+            max_focus = max(max_focus, F32(abs(best[p])))
+    info["max_focus"] = float(max_focus)
+    # the processing that follows writes the locked values into pol 1
+    # (reference wave :10296-10318, basis :10527-10531)
+    if P > 1:
+        if c.pol_lock_tilt:
+            c.tilt_x[1], c.tilt_y[1] = c.tilt_x[0], c.tilt_y[0]
+        if c.pol_lock_defocus:
+            c.defocus[1] = c.defocus[0]
+        if c.pol_lock_waist:
+            c.waist[1] = c.waist[0]
+    return c, info
+
+
+def autofocus(frames, cfg: Config, scale, centre_x, centre_y):
+    """Defocus of least far-field effective area (digHoloAutoFocus, :11181-11515)."""
+    import copy
+    c = copy.deepcopy(cfg)
+    P = c.pol_count
+    c.beam_centre_x, c.beam_centre_y = list(centre_x), list(centre_y)
+    W, H, _, _ = fft_window(c)
+    steps, fit = 25, 5
+    scale = [max(F32(s), F32(1.0)) for s in scale[:P]]
+    defs = np.zeros((2, steps + 3), dtype=np.float32)
+    aeffs = np.zeros((2, steps + 3), dtype=np.float32)
+    s0 = list(scale)
+    for st in range(0, steps, 2):
+        for p in range(P):
+            defs[p, st] = s0[p]
+            defs[p, st + 1] = -s0[p]
+            s0[p] = F32(s0[p] + F32(scale[p] * F32(0.25)))
+    best = [F32(0)] * P
+    best_aeff = [np.inf] * P
+    conv = [False] * P
+    trials = []
+    done = False
+    st = 0
+    while st < steps and not done:
+        done = True
+        for p in range(P):
+            c.defocus[p] = float(defs[p, st])
+        fields, _, _ = process_batch(frames, c)
+        ae = farfield_aeff(fields, W, H, c.pixel_size)
+        trials.append([float(defs[p, st]) for p in range(P)])
+        for p in range(P):
+            aeffs[p, st] = ae[p]
+            if aeffs[p, st] < best_aeff[p]:
+                best_aeff[p] = aeffs[p, st]
+                best[p] = defs[p, st]
+            if st == 1:
+                if aeffs[p, 1] < aeffs[p, 0]:
+                    step_scale, base = F32(defs[p, 1] - defs[p, 0]), defs[p, 1]
+                else:
+                    step_scale, base = F32(defs[p, 0] - defs[p, 1]), defs[p, 0]
+                k = 1
+                for i in range(st + 1, steps, 2):
+                    defs[p, i] = F32(base + F32(F32(k * F32(0.25)) * step_scale))
+                    defs[p, i + 1] = F32(base - F32(F32(k * F32(0.25)) * step_scale))
+                    k += 1
+            if st >= fit - 1:
+                order = sorted(range(st + 1), key=lambda i: aeffs[p, i])     # stable
+                sel = order[:fit - 1]
+                vx = F32(quadratic_vertex(defs[p, sel], aeffs[p, sel]))
+                if st + 1 < steps - 1 and not np.isnan(vx):
+                    defs[p, st + 1] = vx
+                if abs(F32(vx - best[p])) < F32(0.01):
+                    conv[p] = True
+            done = done and conv[p]
+        st += 1
+    return best, trials

@@ -480,3 +480,119 @@ def test_full_resolution_field_interpolates_the_low_res_field():
         assert O.relative_l2(sub, lo[0, p]) < 1e-9
     # the HG coefficients are a property of the field, not of its sampling
     assert O.relative_l2(c_hi, c_lo) < 1e-4
+
+
+# ---------------------------------------------------------------------------
+# AutoAlign parameter estimate (digHolo.cpp:11517-11930, :11181-11515)
+# ---------------------------------------------------------------------------
+
+def test_r2r_matrices_are_fftw_redft10_and_redft01():
+    """The unnormalised DCT-II / DCT-III FFTW defines equal scipy's type-2 /
+    type-3 DCT without normalisation, and REDFT01 inverts REDFT10 up to 2n."""
+    import scipy.fft
+    rng = np.random.default_rng(0)
+    for n in (5, 16, 129):
+        x = rng.standard_normal(n)
+        assert np.allclose(O.r2r_matrix(n, 10) @ x, scipy.fft.dct(x, type=2), rtol=1e-12, atol=1e-10)
+        assert np.allclose(O.r2r_matrix(n, 1) @ x, scipy.fft.dct(x, type=3), rtol=1e-12, atol=1e-10)
+        assert np.allclose(O.r2r_matrix(n, 1) @ O.r2r_matrix(n, 10), 2 * n * np.eye(n), atol=1e-9)
+    m = rng.standard_normal((12, 9))
+    assert np.allclose(O.r2r_2d(m, 10), scipy.fft.dctn(m, type=2), atol=1e-10)
+    assert np.allclose(O.r2r_2d(O.r2r_2d(m, 10), 1), 4 * 12 * 9 * m, atol=1e-9)
+
+
+def test_snap_gauss_layout():
+    """Rows of 8 floor(Nx/8) counters laid over rows of Nx = 8k + 1: the weight
+    sequence along the flat map equals the one of an 8k-wide map, the tail keeps
+    its values, and for an 8k-wide map the weight is the separable Gaussian."""
+    px = 20e-6
+    wx2, wy2 = np.float32(-1.0 / (3.1e-4) ** 2), np.float32(-1.0 / (6.3e-4) ** 2)
+    Ny = 10
+    aligned = O.snap_gauss(np.ones((Ny, 16)), px, wx2, wy2)
+    skewed = O.snap_gauss(np.ones((Ny, 17)), px, wx2, wy2)
+    assert np.array_equal(skewed.ravel()[:Ny * 16], aligned.ravel())
+    assert np.all(skewed.ravel()[Ny * 16:] == 1.0)
+    gx = np.exp(float(wx2) * (np.arange(16) * px) ** 2)
+    gy = np.exp(float(wy2) * (np.arange(Ny) * px) ** 2)
+    assert np.allclose(aligned, np.outer(gy, gx), rtol=1e-5)
+
+
+def _estimate_spec():
+    spec = synth.spec_small()
+    spec.tilt_x_deg = [1.52, 1.52]
+    spec.tilt_y_deg = [1.47, 1.47]
+    spec.beam_centre_x = [30e-6, 0.0]
+    spec.beam_centre_y = [-20e-6, 0.0]
+    spec.defocus = [0.5, 0.0]
+    return spec
+
+
+def test_snap_tilt_finds_the_rendered_carrier():
+    """The peak of the filtered Fourier power sits on the rendered carrier: the
+    tilt comes back within one Fourier pixel (k step 2 pi / (px W))."""
+    spec = _estimate_spec()
+    frames, _, _, _ = synth.make_batch(spec, 20, seed=3)
+    cfg = small_cfg(fft_window_x=320, fft_window_y=256, aperture_deg=0.8)
+    W, H, cx, cy = O.fft_window(cfg)
+    spec_p = np.fft.rfft2(frames[:, cy[0]:cy[0] + H, cx[0]:cx[0] + W].astype(np.float64), axes=(-2, -1))
+    k0 = 2 * math.pi / 1565e-9
+    R = 2 * k0 * math.sin(0.8 * math.pi / 180)
+    summed, _ = O._fourier_window_map(spec_p, W, H, 20e-6, 0.0, 0.0, R)
+    tx, ty = O.snap_tilt(summed, W, H, 20e-6, 1565e-9, O.F32(0.8 * math.pi / 180))
+    dkx, dky = 2 * math.pi / (20e-6 * W), 2 * math.pi / (20e-6 * H)
+    assert abs(k0 * math.sin(math.radians(float(tx))) - k0 * math.radians(1.52)) <= dkx
+    assert abs(k0 * math.sin(math.radians(float(ty))) - k0 * math.radians(1.47)) <= dky
+
+
+def test_beam_size_relations():
+    """AeffToMFD reads the rendered waist off the effective area of a beam that
+    fills its groups (within 5 % for several waists); FocusEstimate is zero when
+    both planes agree and grows as the Fourier-plane size shrinks."""
+    for waist in (150e-6, 200e-6, 260e-6):
+        spec = synth.spec_small()
+        spec.waist = waist
+        frames, _, _, _ = synth.make_batch(spec, 40, seed=7)
+        cfg = small_cfg(waist=[waist, waist])
+        fields, _, plan = O.process_batch(frames, cfg)
+        params, _ = O.field_plane_analysis(fields, cfg, plan)
+        w = float(O.aeff_to_mfd(params[O.ANALYSIS_AEFF, 0, -1], cfg.group_count)) * 0.5
+        assert abs(w / waist - 1) < 0.05, (waist, w)
+    assert O.focus_estimate(1e-4, 1e-4) == 0
+    est = [float(O.focus_estimate(1e-4, 1e-4 * r)) for r in (0.9, 0.7, 0.5, 0.3)]
+    assert all(b > a > 0 for a, b in zip(est, est[1:]))
+    assert O.focus_estimate(0.0, 1e-4) == 0 and O.aeff_to_mfd(1.0, 0) == 0
+
+
+def test_farfield_aeff_shift_invariant_and_least_without_curvature():
+    """A circular shift of the fields leaves the far-field area unchanged; a
+    quadratic phase of either sign enlarges it."""
+    n = 64
+    x = (np.arange(n) - n // 2) * 50e-6
+    X, Y = np.meshgrid(x, x)
+    E = np.exp(-(X ** 2 + Y ** 2) / (300e-6) ** 2)[None, None]
+    a0 = O.farfield_aeff(E, 128, 128, 20e-6)[0]
+    assert O.farfield_aeff(np.roll(E, (5, -3), axis=(-2, -1)), 128, 128, 20e-6)[0] == pytest.approx(a0, rel=1e-5)
+    for alpha in (-2e6, 2e6, 6e6):
+        q = np.exp(1j * alpha * (X ** 2 + Y ** 2))[None, None]
+        assert O.farfield_aeff(E * q, 128, 128, 20e-6)[0] > a0 * 1.01
+
+
+def test_quadratic_vertex_exact_for_parabola():
+    x = np.array([-1.0, 0.3, 2.0, 2.5])
+    assert O.quadratic_vertex(x, 3 * (x - 0.7) ** 2 + 1) == pytest.approx(0.7, abs=1e-12)
+    assert math.isnan(O.quadratic_vertex([0.0, 1.0, 2.0], [1.0, 2.0, float("nan")]))
+
+
+def test_autoalign_estimate_lands_near_the_rendered_parameters():
+    """The estimate stage alone (reset, tilt search, window analysis, focus
+    search) lands near the rendered beam: tilt within 0.02 deg, centre within a
+    pixel, waist within 5 %, defocus on the right side."""
+    spec = _estimate_spec()
+    frames, _, _, _ = synth.make_batch(spec, 30, seed=5)
+    cfg = small_cfg(tilt_x=[0.9, 0.9], tilt_y=[-0.4, -0.4], waist=[250e-6, 250e-6], fill_factor=1)
+    c, info = O.autoalign_estimate(frames, cfg)
+    assert abs(c.tilt_x[0] - 1.52) < 0.02 and abs(c.tilt_y[0] - 1.47) < 0.02
+    assert abs(c.beam_centre_x[0] - 30e-6) < 20e-6 and abs(c.beam_centre_y[0] + 20e-6) < 20e-6
+    assert abs(c.waist[0] / 200e-6 - 1) < 0.05
+    assert 0.0 < c.defocus[0] < 1.0
+    assert c.aperture_deg == pytest.approx(math.hypot(c.tilt_x[0], c.tilt_y[0]) / 3, rel=1e-5)
