@@ -142,6 +142,8 @@ SIGNATURES: Dict[str, Tuple[object, List[object]]] = {
     "digHoloGetWindowPath": (c_int, [c_int]),
     "digHoloConfigSetStageTiming": (c_int, [c_int, c_int]),
     "digHoloGetStageTimes": (c_int, [c_int, P_float]),
+    "digHoloAutoAlignGradient": (c_int, [c_int, c_int, P_float, P_float]),
+    "digHoloAutoAlignGradientSearch": (c_float, [c_int, c_int]),
 }
 
 # constants of the header
@@ -332,3 +334,37 @@ def get_fields16(lib, h):
     x = np.ctypeslib.as_array(xp, shape=(W,)).copy()
     y = np.ctypeslib.as_array(yp, shape=(H,)).copy()
     return R, I, scale, x, y
+
+
+GRAD_PARAMS = ("waist", "defocus", "tilt_x", "beam_centre_x", "tilt_y", "beam_centre_y")
+
+
+def set_align_params(lib, h, params) -> None:
+    """Write a [pol, 6] array of alignment parameters (the AutoAlign order of
+    GRAD_PARAMS, API units) through the C API setters."""
+    params = np.asarray(params, dtype=np.float64)
+    for p in range(params.shape[0]):
+        w, d, tx, cx, ty, cy = (float(v) for v in params[p])
+        lib.digHoloConfigSetBasisWaist(h, p, w)
+        lib.digHoloConfigSetDefocus(h, p, d)
+        lib.digHoloConfigSetTilt(h, 0, p, tx)
+        lib.digHoloConfigSetBeamCentre(h, 0, p, cx)
+        lib.digHoloConfigSetTilt(h, 1, p, ty)
+        lib.digHoloConfigSetBeamCentre(h, 1, p, cy)
+
+
+def get_align_params(lib, h, P: int) -> np.ndarray:
+    return np.array([[lib.digHoloConfigGetBasisWaist(h, p), lib.digHoloConfigGetDefocus(h, p),
+                      lib.digHoloConfigGetTilt(h, 0, p), lib.digHoloConfigGetBeamCentre(h, 0, p),
+                      lib.digHoloConfigGetTilt(h, 1, p), lib.digHoloConfigGetBeamCentre(h, 1, p)]
+                     for p in range(P)], dtype=np.float64)
+
+
+def alignment_gradient(lib, h, hold: bool = False):
+    """digHoloAutoAlignGradient: (loss, grad [2, 6] float64)."""
+    loss = c_float()
+    grad = (c_float * 12)()
+    rc = lib.digHoloAutoAlignGradient(h, 1 if hold else 0, C.byref(loss), grad)
+    if rc != DIGHOLO_ERROR_SUCCESS:
+        raise RuntimeError(f"digHoloAutoAlignGradient returned {rc}")
+    return float(loss.value), np.array(list(grad), dtype=np.float64).reshape(2, 6)

@@ -207,6 +207,14 @@ struct Handle {
   DevBuf<int> snapIdx;
   DevBuf<float2> ffBuf;
 
+  // alignment gradient: waist-derivative basis, -V R and conj(A) R, their
+  // inverse transforms, per-frame partial sums; the placement of the last call
+  DevBuf<float> dhgx, dhgy;
+  DevBuf<float2> gq1, gq2, gx1, gx2;
+  DevBuf<double> gPart, gSum, gAxes;
+  Derived gradHeld;
+  bool gradHeldValid = false;
+
   bool stageTiming = false;
   std::vector<std::array<hipEvent_t, 6>> evPool;
   size_t evUsed = 0;
@@ -2011,6 +2019,242 @@ int autoalign_snap(Handle& h, bool doFinal, float* steps) {
   return DIGHOLO_ERROR_SUCCESS;
 }
 
+// ---------------------------------------------------------------------------
+// Alignment-loss gradient (no counterpart in the reference; dh_grad.hip holds
+// the adjoint).  L = -(1/B) sum |c|^2 (minus the linear insertion loss without
+// the SVD); gradient with respect to [pol][AA_WAIST..AA_CY] in API units (m,
+// dioptre, degree, m, degree, m).  The window placement is a step function of
+// centre and tilt: the gradient is that of the smooth piece the current
+// placement belongs to; hold keeps the previous call's placement.
+// ---------------------------------------------------------------------------
+int alignment_gradient(Handle& h, bool hold, double& loss, double grad[2][AA_PARS]) {
+  Config& c = h.cfg;
+  if (c.maxMG <= 0) return DIGHOLO_ERROR_INVALIDARGUMENT;
+  const bool useHold = hold && h.gradHeldValid;
+  const bool hc = h.holdCrop, hw = h.holdWin;
+  if (useHold) {
+    h.heldCrop = h.gradHeld;
+    h.heldWin = h.gradHeld;
+    h.holdCrop = h.holdWin = true;
+  }
+  int e = enqueue_batch(h);
+  h.holdCrop = hc;
+  h.holdWin = hw;
+  if (e) return e;
+  h.gradHeld = h.cur;
+  h.gradHeldValid = true;
+  const Derived& d = h.cur;
+  const dh::Geometry& g = d.g;
+  const int B = g.B, P = g.P, fw = g.fw, fh = g.fh, N = fw * fh;
+  dh::Geometry gF = g;
+  gF.w = fw;
+  gF.h = fh;
+  const dh::FFTPlan* pIx = fft_plan(h, fw);
+  const dh::FFTPlan* pIy = fft_plan(h, fh);
+  if (!pIx || !pIy) return DIGHOLO_ERROR_MEMORYALLOCATION;
+  const size_t n = (size_t)B * P * N;
+  DH_TRY(h.dhgx.ensure((size_t)P * g.G * fw), "alloc");
+  DH_TRY(h.dhgy.ensure((size_t)P * g.G * fh), "alloc");
+  DH_TRY(h.gq1.ensure(n), "alloc");
+  DH_TRY(h.gq2.ensure(n), "alloc");
+  DH_TRY(h.gx1.ensure(n), "alloc");
+  DH_TRY(h.gx2.ensure(n), "alloc");
+  DH_TRY(h.gPart.ensure((size_t)B * P * 8), "alloc");
+  DH_TRY(h.gSum.ensure((size_t)P * 8), "alloc");
+  // per pol: {pxx, pxy}, then the absolute kx of the field plane's columns and
+  // ky of its rows (window order in low-res mode, FFT order in full resolution)
+  const bool lowRes = fw == g.w && fh == g.h && c.resolutionIdx == 1;
+  const double px = (double)c.pixelSize;
+  const double dkx = 2.0 * PI_D / (px * g.W), dky = 2.0 * PI_D / (px * g.H);
+  std::vector<double> ax(2 * P + (size_t)P * (fw + fh));
+  double* kxT = ax.data() + 2 * P;
+  double* kyT = kxT + (size_t)P * fw;
+  for (int p = 0; p < P; ++p) {
+    ax[2 * p] = d.tab.pp[p].pxx;
+    ax[2 * p + 1] = d.tab.pp[p].pxy;
+    for (int i = 0; i < fw; ++i) {
+      const int col = lowRes ? g.c0[p] + i : (i <= g.W / 2 ? i : i - g.W);
+      kxT[(size_t)p * fw + i] = col * dkx;
+    }
+    for (int j = 0; j < fh; ++j) {
+      int row = lowRes ? imod(j + g.SY[p] + g.H / 2, g.H) : j;
+      if (row > g.H / 2) row -= g.H;
+      kyT[(size_t)p * fh + j] = row * dky;
+    }
+  }
+  DH_TRY(h.gAxes.ensure(ax.size()), "alloc");
+  DH_TRY(hipMemcpyAsync(h.gAxes.p, ax.data(), ax.size() * sizeof(double), hipMemcpyHostToDevice, h.stream),
+         "axes upload");
+  DH_TRY(dh::launch_hg_dwaist(d.tab, h.dhgx.p, h.dhgy.p, h.stream), "waist derivative basis");
+  DH_TRY(dh::launch_grad_backproject(gF, 1.0 / B, h.coefs.p, h.hgx.p, h.hgy.p, h.dhgx.p, h.dhgy.p, h.fields.p,
+                                     h.ref.p, h.gAxes.p, h.gq1.p, h.gq2.p, h.gPart.p, h.stream), "back-projection");
+  DH_TRY(dh::launch_ifft_rows(gF, *pIx, h.gq1.p, h.stream), "adjoint IFFT rows");
+  DH_TRY(dh::launch_ifft_cols_tilt(gF, *pIy, h.gq1.p, nullptr, h.gx1.p, h.stream), "adjoint IFFT columns");
+  DH_TRY(dh::launch_ifft_rows(gF, *pIx, h.gq2.p, h.stream), "adjoint IFFT rows");
+  DH_TRY(dh::launch_ifft_cols_tilt(gF, *pIy, h.gq2.p, nullptr, h.gx2.p, h.stream), "adjoint IFFT columns");
+  DH_TRY(dh::launch_grad_kspace(gF, h.gx1.p, h.gx2.p, kxT - ax.data() + h.gAxes.p, kyT - ax.data() + h.gAxes.p,
+                                h.gPart.p, h.stream), "k-space sums");
+  DH_TRY(dh::launch_grad_reduce(B, P, h.gPart.p, h.gSum.p, h.stream), "batch reduction");
+  double S[2][8] = {};
+  DH_TRY(hipMemcpyAsync(S, h.gSum.p, (size_t)P * 8 * sizeof(double), hipMemcpyDeviceToHost, h.stream), "D2H");
+  DH_TRY(hipStreamSynchronize(h.stream), "sync");
+
+  const double lam = (double)c.wavelengthCentre, k0 = 2.0 * PI_D / lam, deg = PI_D / 180.0;
+  for (int p = 0; p < 2; ++p)
+    for (int k = 0; k < AA_PARS; ++k) grad[p][k] = 0.0;
+  loss = 0.0;
+  for (int p = 0; p < P; ++p) {
+    const int tp = (c.polLockTilt || c.polLockDefocus) ? 0 : p;
+    const int ip = c.polLockDefocus ? 0 : p;
+    const int wp = c.polLockWaist ? 0 : p;
+    const double S0 = S[p][0], Sx = S[p][1], Sy = S[p][2], Sr2 = S[p][3], Sw = S[p][4];
+    const double Tbx = S[p][6] / N, Tby = S[p][7] / N;
+    const double thx = (double)c.tiltX[tp] * deg, thy = (double)c.tiltY[tp] * deg;
+    const double bcx = (double)c.beamCX[p], bcy = (double)c.beamCY[p];
+    grad[tp][AA_TILTX] += -2.0 * k0 * (Sx + bcx * S0) * deg;
+    grad[tp][AA_TILTY] += -2.0 * k0 * (Sy + bcy * S0) * deg;
+    grad[ip][AA_DEFOCUS] += -2.0 * (PI_D / lam) * Sr2;
+    grad[p][AA_CX] += -2.0 * Tbx - 2.0 * k0 * thx * S0;
+    grad[p][AA_CY] += -2.0 * Tby - 2.0 * k0 * thy * S0;
+    grad[wp][AA_WAIST] += 2.0 * Sw;
+    loss -= S[p][5] / B;
+  }
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+// Gradient-based alignment (no counterpart in the reference): BFGS on the
+// enabled parameters (the AutoAlign enable flags and pol locks), each scaled by
+// the step the tweak search would start with, Armijo backtracking, placement
+// re-derived at every trial point.  Ends at the best point with the metrics of
+// a final evaluation stored; returns the goal metric in dB.
+int autoalign_gradient_search(Handle& h, int maxIter, float& goalDb) {
+  Config& c = h.cfg;
+  const int P = c.polCount;
+  HoldGuard guard(h);
+  if (c.aaFourierRadius) c.apertureSize = window_radius_rec(c);
+  struct Var {
+    int p, par;
+    double scale;
+  };
+  std::vector<Var> vars;
+  const float w0 = c.waist[0];
+  for (int par = 0; par < AA_PARS; ++par) {
+    const bool on = (par == AA_TILTX || par == AA_TILTY) ? c.aaTilt != 0
+                    : (par == AA_CX || par == AA_CY)     ? c.aaCentre != 0
+                    : par == AA_WAIST                    ? c.aaWaist != 0
+                                                         : c.aaDefocus != 0;
+    if (!on) continue;
+    const bool lock = ((par == AA_TILTX || par == AA_TILTY) && c.polLockTilt) ||
+                      (par == AA_DEFOCUS && c.polLockDefocus) || (par == AA_WAIST && c.polLockWaist);
+    double sc = (par == AA_TILTX || par == AA_TILTY) ? 0.1 * c.apertureSize
+                : par == AA_DEFOCUS ? std::max(0.4, 0.1 * std::fabs((double)c.defocus[0]))
+                                    : 0.1 * w0;
+    for (int p = 0; p < (lock ? 1 : P); ++p) vars.push_back({p, par, sc});
+  }
+  const int nv = (int)vars.size();
+  std::vector<double> x0(nv);
+  for (int i = 0; i < nv; ++i) x0[i] = aa_param(c, vars[i].p, vars[i].par);
+  double L0 = 0.0, G0[2][AA_PARS];
+  auto apply = [&](const std::vector<double>& z) {
+    for (int i = 0; i < nv; ++i) aa_param(c, vars[i].p, vars[i].par) = (float)(x0[i] + z[i] * vars[i].scale);
+  };
+  double norm = 1.0;
+  auto eval = [&](const std::vector<double>& z, double& f, std::vector<double>& gz) -> int {
+    apply(z);
+    double L, Gr[2][AA_PARS];
+    if (int e = alignment_gradient(h, false, L, Gr)) return e;
+    f = L / norm;
+    gz.resize(nv);
+    for (int i = 0; i < nv; ++i) gz[i] = Gr[vars[i].p][vars[i].par] * vars[i].scale / norm;
+    return 0;
+  };
+  if (int e = alignment_gradient(h, false, L0, G0)) return e;
+  norm = std::max(std::fabs(L0), 1e-30);
+  std::vector<double> z(nv, 0.0), gz, zBest = z;
+  double f = 0, fBest;
+  if (nv > 0) {
+    if (int e = eval(z, f, gz)) return e;
+  }
+  fBest = f;
+  std::vector<double> Hi((size_t)nv * nv, 0.0);
+  for (int i = 0; i < nv; ++i) Hi[(size_t)i * nv + i] = 1.0;
+  int quiet = 0;
+  for (int it = 0; it < maxIter && nv > 0; ++it) {
+    std::vector<double> dir(nv, 0.0);
+    double gd = 0.0, dn = 0.0;
+    for (int i = 0; i < nv; ++i) {
+      for (int j = 0; j < nv; ++j) dir[i] -= Hi[(size_t)i * nv + j] * gz[j];
+      gd += gz[i] * dir[i];
+      dn += dir[i] * dir[i];
+    }
+    if (gd >= 0) {   // not a descent direction: restart from steepest descent
+      std::fill(Hi.begin(), Hi.end(), 0.0);
+      for (int i = 0; i < nv; ++i) Hi[(size_t)i * nv + i] = 1.0;
+      gd = dn = 0.0;
+      for (int i = 0; i < nv; ++i) {
+        dir[i] = -gz[i];
+        gd += gz[i] * dir[i];
+        dn += dir[i] * dir[i];
+      }
+    }
+    dn = std::sqrt(dn);
+    double t = dn > 2.0 ? 2.0 / dn : 1.0;   // at most two start steps per iteration
+    std::vector<double> zn(nv), gn;
+    double fn = 0;
+    bool ok = false;
+    for (int ls = 0; ls < 12; ++ls) {
+      for (int i = 0; i < nv; ++i) zn[i] = z[i] + t * dir[i];
+      if (int e = eval(zn, fn, gn)) return e;
+      if (fn <= f + 1e-4 * t * gd) {
+        ok = true;
+        break;
+      }
+      t *= 0.5;
+    }
+    if (!ok) break;
+    std::vector<double> sv(nv), yv(nv);
+    double sy = 0.0;
+    for (int i = 0; i < nv; ++i) {
+      sv[i] = zn[i] - z[i];
+      yv[i] = gn[i] - gz[i];
+      sy += sv[i] * yv[i];
+    }
+    if (sy > 1e-300) {   // BFGS update of the inverse Hessian
+      std::vector<double> Hy(nv, 0.0);
+      double yHy = 0.0;
+      for (int i = 0; i < nv; ++i) {
+        for (int j = 0; j < nv; ++j) Hy[i] += Hi[(size_t)i * nv + j] * yv[j];
+        yHy += yv[i] * Hy[i];
+      }
+      const double r = 1.0 / sy;
+      for (int i = 0; i < nv; ++i)
+        for (int j = 0; j < nv; ++j)
+          Hi[(size_t)i * nv + j] += (1.0 + yHy * r) * r * sv[i] * sv[j] - r * (Hy[i] * sv[j] + sv[i] * Hy[j]);
+    }
+    const double df = f - fn;
+    z = zn;
+    gz = gn;
+    f = fn;
+    if (f < fBest) {
+      fBest = f;
+      zBest = z;
+    }
+    if (c.verbosity >= DIGHOLO_VERBOSITY_DEBUG) {
+      fprintf(stdout, "Gradient align iteration %d  loss %.9f  step %.3g\n", it, f, t * dn);
+      fflush(stdout);
+    }
+    quiet = df < 1e-9 ? quiet + 1 : 0;
+    if (quiet >= 2) break;
+  }
+  apply(zBest);
+  if (int e = enqueue_batch(h)) return e;
+  PowerMeterOut o;
+  if (int e = power_meter(h, -1, c.aaPolIndependence != 0, true, true, true, c.aaMulConjTrans != 0, o)) return e;
+  store_metrics_db(h, o);
+  goalDb = h.aaMetrics[(size_t)c.aaGoalIdx * (h.aaLambdaCount + 1) + h.aaLambdaCount];
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
 void destroy_handle(Handle* h) {
   if (!h) return;
   if (h->devInit) {
@@ -2757,6 +3001,30 @@ EXT_C float digHoloAutoAlign(int handleIdx) {
   }
   if (e) {
     fprintf(stderr, "digHolo: AutoAlign failed (error %d)\n", e);
+    return -FMAX;
+  }
+  return goal;
+}
+EXT_C int digHoloAutoAlignGradient(int handleIdx, int holdPlacement, float* loss, float* grad) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (!loss || !grad) return DIGHOLO_ERROR_NULLPOINTER;
+  if (!h->frames) return DIGHOLO_ERROR_NULLPOINTER;
+  double L = 0.0, gr[2][AA_PARS] = {};
+  const int e = alignment_gradient(*h, holdPlacement != 0, L, gr);
+  if (e) return e;
+  *loss = (float)L;
+  for (int p = 0; p < 2; ++p)
+    for (int k = 0; k < AA_PARS; ++k) grad[p * AA_PARS + k] = (float)gr[p][k];
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C float digHoloAutoAlignGradientSearch(int handleIdx, int maxIterations) {
+  H_OR(0.f);
+  const float FMAX = std::numeric_limits<float>::max();
+  if (!h->frames || CFG.maxMG <= 0) return -FMAX;
+  float goal = -FMAX;
+  const int e = autoalign_gradient_search(*h, maxIterations > 0 ? maxIterations : 30, goal);
+  if (e) {
+    fprintf(stderr, "digHolo: gradient alignment failed (error %d)\n", e);
     return -FMAX;
   }
   return goal;

@@ -177,6 +177,22 @@ size_t snap_work_elems(int Ny, int Nx);
 hipError_t launch_snap_tilt_search(const float* map, int Ny, int Nx, float px, float wx2, float wy2, const float* kx,
                                    const float* ky, float ad, double* work, int* outIdx, hipStream_t s);
 
+// Alignment-loss gradient (dh_grad.hip).  dhgx/dhgy: d/dwaist of the HG
+// tables, same layout.  Back-projection per (frame, pol): partial[bp][0..5] =
+// {sum Im(AV), sum x Im(AV), sum y Im(AV), sum r^2 Im(AV), sum Re(Aw V),
+// sum |c|^2}, q1 = -V R, q2 = conj(A) R; pxy2[p] = {pxx, pxy}.  kspace:
+// partial[bp][6..7] = sum_k {kx, ky} Im(X1 conj X2) with kx [P][fw], ky [P][fh].
+hipError_t launch_hg_dwaist(const TableArgs& a, float* dhgx, float* dhgy, hipStream_t s);
+size_t grad_backproject_lds(const Geometry& g);
+hipError_t launch_grad_backproject(const Geometry& g, double invB, const float2* coefs, const float* hgx,
+                                   const float* hgy, const float* dhgx, const float* dhgy, const float2* fields,
+                                   const float2* ref, const double* pxy2, float2* q1, float2* q2, double* partial,
+                                   hipStream_t s);
+hipError_t launch_grad_kspace(const Geometry& g, const float2* x1, const float2* x2, const double* kx,
+                              const double* ky, double* partial, hipStream_t s);
+// out[p][k] = sum over frames of partial[b][p][k], k < 8
+hipError_t launch_grad_reduce(int B, int P, const double* partial, double* out, hipStream_t s);
+
 // Separable HG overlap on MFMA: split = 0 plain FP32, 1 split-FP16 (hi+lo).
 hipError_t launch_overlap_mfma(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
                                float2* coefs, int split, hipStream_t s);

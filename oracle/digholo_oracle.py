@@ -291,6 +291,7 @@ class PolTables:
     ref: np.ndarray        # [fh, fw] complex reference wave exp(i(phix+phiy)) on the field plane
     hgx: np.ndarray        # [G, fw]
     hgy: np.ndarray        # [G, fh]
+    mask: np.ndarray = None    # [h, w] bool, the circular Fourier window
 
 
 def _pol_index(cfg: Config, p: int) -> Tuple[int, int]:
@@ -305,7 +306,7 @@ def _pol_index(cfg: Config, p: int) -> Tuple[int, int]:
     return tilt_p, idx_p
 
 
-def pol_tables(cfg: Config, plan: Plan, p: int) -> PolTables:
+def pol_tables(cfg: Config, plan: Plan, p: int, mask=None) -> PolTables:
     W, H, w, h = plan.W, plan.H, plan.w, plan.h
     px32 = F32(cfg.pixel_size)
     px = float(px32)
@@ -338,6 +339,8 @@ def pol_tables(cfg: Config, plan: Plan, p: int) -> PolTables:
     dky = (ky32 - (ky0 + shift).astype(np.float32)).astype(np.float32)
     dr2 = (dkx * dkx)[None, :] + (dky * dky)[:, None]
     inside = (dr2.astype(np.float32) <= r02)
+    if mask is not None:            # a held Fourier window (see Hold)
+        inside = np.asarray(mask, dtype=bool)
 
     # continuous parts in FP64
     dk_x = 2.0 * math.pi / (px * W)
@@ -396,7 +399,7 @@ def pol_tables(cfg: Config, plan: Plan, p: int) -> PolTables:
         wp = float(F32(400e-6))
     hgx = hg_modes_1d(G, x, wp, pxx)
     hgy = hg_modes_1d(G, y, wp, pxy)
-    return PolTables(col=col, row=row, factor=factor, ref=ref, hgx=hgx, hgy=hgy)
+    return PolTables(col=col, row=row, factor=factor, ref=ref, hgx=hgx, hgy=hgy, mask=inside)
 
 
 def fft_plane(window: np.ndarray) -> np.ndarray:
@@ -436,8 +439,24 @@ def average_windows(o: np.ndarray) -> np.ndarray:
     return W
 
 
+@dataclass
+class Hold:
+    """Window placement held fixed: FFT crops, Fourier-window position and size
+    (the plan) and the circular Fourier mask of each pol, as AutoAlign holds them
+    between re-placements (digHolo.cpp:12620-12645).  With them held the path is
+    a smooth function of centre, tilt, defocus and waist."""
+    plan: Plan
+    masks: List[np.ndarray]
+
+
+def hold_of(cfg: Config) -> Hold:
+    c = apply_pol_locks(cfg)
+    plan = make_plan(c)
+    return Hold(plan, [pol_tables(c, plan, p).mask for p in range(c.pol_count)])
+
+
 def process_batch(frames: np.ndarray, cfg: Config, emulate_int16: bool = False, avg_count: int = 1,
-                  avg_mode: int = 0, batch_cal=None, transpose: bool = False):
+                  avg_mode: int = 0, batch_cal=None, transpose: bool = False, hold: Hold = None):
     """Full ``digHoloProcessBatch`` path.
 
     frames: [B * avg_count, frameH, frameW] (uint16 or float).  Returns (fields,
@@ -452,11 +471,11 @@ def process_batch(frames: np.ndarray, cfg: Config, emulate_int16: bool = False, 
     A = max(1, int(avg_count))
     B = frames.shape[0] // A
     cfg = apply_pol_locks(cfg)
-    plan = make_plan(cfg)
+    plan = hold.plan if hold is not None else make_plan(cfg)
     P = cfg.pol_count
     G = cfg.group_count
     Mi, Ni = hg_indices(G)
-    tabs = [pol_tables(cfg, plan, p) for p in range(P)]
+    tabs = [pol_tables(cfg, plan, p, mask=hold.masks[p] if hold is not None else None) for p in range(P)]
     low = cfg.resolution_mode == 1
     fh, fw = (plan.h, plan.w) if low else (plan.H, plan.W)
     fields = np.zeros((B, P, fh, fw), dtype=np.complex128)
@@ -1019,3 +1038,113 @@ def autofocus(frames, cfg: Config, scale, centre_x, centre_y):
             done = done and conv[p]
         st += 1
     return best, trials
+
+
+# ---------------------------------------------------------------------------
+# Gradient of the alignment loss (no counterpart in the reference; the FP64
+# restatement of the library's adjoint, pinned by central finite differences
+# with the window placement held)
+# ---------------------------------------------------------------------------
+
+GRAD_WAIST, GRAD_DEFOCUS, GRAD_TILTX, GRAD_CX, GRAD_TILTY, GRAD_CY = range(6)
+
+
+def hg_modes_1d_dwaist(nmax: int, x: np.ndarray, waist: float, dx: float) -> np.ndarray:
+    """d/dw of psi_n(sqrt(2) x / w) sqrt(dx sqrt(2) / w): with u = sqrt(2) x / w,
+    psi_n'(u) = sqrt(n/2) psi_{n-1} - sqrt((n+1)/2) psi_{n+1}, du/dw = -u/w and the
+    norm contributes -1/(2w)."""
+    u = math.sqrt(2.0) * np.asarray(x, dtype=np.float64) / waist
+    psi = hermite_functions(nmax + 1, u)
+    nrm = math.sqrt(dx * math.sqrt(2.0) / waist)
+    out = np.zeros((nmax,) + u.shape)
+    for n in range(nmax):
+        dpsi = -math.sqrt((n + 1) / 2.0) * psi[n + 1]
+        if n > 0:
+            dpsi = dpsi + math.sqrt(n / 2.0) * psi[n - 1]
+        out[n] = nrm * (dpsi * (-u / waist) - psi[n] / (2.0 * waist))
+    return out
+
+
+def alignment_loss(coefs: np.ndarray) -> float:
+    """L = -(1/B) sum |c|^2: minus the linear insertion loss of the batch without
+    the SVD (digHolo.cpp:11046-11063, IL = sum |A|^2 / B)."""
+    c = np.asarray(coefs)
+    return -float((np.abs(c) ** 2).sum()) / c.shape[0]
+
+
+def alignment_grad(frames: np.ndarray, cfg: Config, hold: Hold):
+    """Loss and its gradient with respect to [pol][waist, defocus (dioptre),
+    tilt x (degree), centre x (m), tilt y (degree), centre y (m)] at the
+    configuration, with the placement `hold` fixed.
+
+    With g = dL/dc (Wirtinger, -conj(c)/B) and the adjoint field
+    A(x, y) = sum_mn g_mn HGx_m(x) HGy_n(y):
+      tilt:    dV/dtheta_x = i k0 (x + bcX) V          -> -2 k0 sum (x + bcX) Im(A V)
+      defocus: dV/dD = i (pi/lambda) (x^2 + y^2) V      -> -2 (pi/lambda) sum r^2 Im(A V)
+      waist:   2 Re sum_mn g_mn sum (dHGx HGy + HGx dHGy) V
+      centre:  the window ramp exp(i kx bcx) shifts the field, dE/dbcx = IFFT(i kx W);
+               -2 sum_k kx Im(X1 conj X2) / N with X1 = IFFT(-V R), X2 = IFFT(conj(A) R),
+               plus the reference-wave piston -2 k0 theta_x sum Im(A V).
+    """
+    cl = apply_pol_locks(cfg)
+    fields, coefs, plan = process_batch(frames, cfg, hold=hold)
+    B, P = fields.shape[:2]
+    loss = alignment_loss(coefs)
+    g = -np.conj(coefs) / B
+    low = cl.resolution_mode == 1
+    W, H = plan.W, plan.H
+    fw, fh = (plan.w, plan.h) if low else (W, H)
+    N = fw * fh
+    px = float(F32(cl.pixel_size))
+    lam = float(F32(cl.wavelength))
+    k0 = 2.0 * math.pi / lam
+    dkx, dky = 2.0 * math.pi / (px * W), 2.0 * math.pi / (px * H)
+    pxx, pxy = px * W / fw, px * H / fh
+    x = (np.arange(fw) - fw // 2) * pxx
+    y = (np.arange(fh) - fh // 2) * pxy
+    G = cl.group_count
+    Mi, Ni = hg_indices(G)
+    grad = np.zeros((P, 6))
+    for p in range(P):
+        t = pol_tables(cl, plan, p, mask=hold.masks[p])
+        tp, ip = _pol_index(cl, p)
+        wp = 0 if cl.pol_lock_waist else p
+        waist = float(F32(cl.waist[wp]))
+        dhx = hg_modes_1d_dwaist(G, x, waist, pxx)
+        dhy = hg_modes_1d_dwaist(G, y, waist, pxy)
+        gm = np.zeros((B, G, G), dtype=np.complex128)          # g as [b, m, n]
+        gm[:, Mi, Ni] = g[:, p]
+        A = np.einsum("bmn,mx,ny->byx", gm, t.hgx, t.hgy)
+        Aw = np.einsum("bmn,mx,ny->byx", gm, dhx, t.hgy) + np.einsum("bmn,mx,ny->byx", gm, t.hgx, dhy)
+        V = fields[:, p]
+        im = np.imag(A * V)
+        S0 = im.sum()
+        Sx = (im * x[None, None, :]).sum()
+        Sy = (im * y[None, :, None]).sum()
+        Sr2 = (im * (x[None, None, :] ** 2 + y[None, :, None] ** 2)).sum()
+        Sw = np.real(Aw * V).sum()
+        R = t.ref[None]
+        X1 = np.fft.ifft2(-V * R, axes=(-2, -1)) * N
+        X2 = np.fft.ifft2(np.conj(A) * R, axes=(-2, -1)) * N
+        if low:
+            kxv = (plan.SX[p] - W // 2 + np.arange(fw)) * dkx
+            rows = np.mod(np.arange(fh) + plan.SY[p] + H // 2, H)
+            kyv = np.where(rows <= H // 2, rows, rows - H) * dky
+        else:
+            kxv = np.where(np.arange(W) <= W // 2, np.arange(W), np.arange(W) - W) * dkx
+            kyv = np.where(np.arange(H) <= H // 2, np.arange(H), np.arange(H) - H) * dky
+        imx = np.imag(X1 * np.conj(X2))
+        Tbx = (imx * kxv[None, None, :]).sum() / N
+        Tby = (imx * kyv[None, :, None]).sum() / N
+        thx = float(F32(cl.tilt_x[tp])) * math.pi / 180.0
+        thy = float(F32(cl.tilt_y[tp])) * math.pi / 180.0
+        bcx = float(F32(cl.beam_centre_x[p]))
+        bcy = float(F32(cl.beam_centre_y[p]))
+        deg = math.pi / 180.0
+        grad[tp, GRAD_TILTX] += -2.0 * k0 * (Sx + bcx * S0) * deg
+        grad[tp, GRAD_TILTY] += -2.0 * k0 * (Sy + bcy * S0) * deg
+        grad[ip, GRAD_DEFOCUS] += -2.0 * (math.pi / lam) * Sr2
+        grad[p, GRAD_CX] += -2.0 * Tbx - 2.0 * k0 * thx * S0
+        grad[p, GRAD_CY] += -2.0 * Tby - 2.0 * k0 * thy * S0
+        grad[wp, GRAD_WAIST] += 2.0 * Sw
+    return loss, grad

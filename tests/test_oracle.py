@@ -596,3 +596,71 @@ def test_autoalign_estimate_lands_near_the_rendered_parameters():
     assert abs(c.waist[0] / 200e-6 - 1) < 0.05
     assert 0.0 < c.defocus[0] < 1.0
     assert c.aperture_deg == pytest.approx(math.hypot(c.tilt_x[0], c.tilt_y[0]) / 3, rel=1e-5)
+
+
+# ---------------------------------------------------------------------------
+# Alignment-loss gradient (the adjoint the library computes on the GPU)
+# ---------------------------------------------------------------------------
+
+def _f32(v):
+    return float(np.float32(v))
+
+
+# central-difference steps: powers of two, so base +- step is exact in float32
+# (the configuration is float32, as in the reference)
+_GRAD_STEPS = (2.0 ** -28, 2.0 ** -14, 2.0 ** -16, 2.0 ** -30, 2.0 ** -16, 2.0 ** -30)
+_GRAD_FIELDS = ("waist", "defocus", "tilt_x", "beam_centre_x", "tilt_y", "beam_centre_y")
+
+
+def _fd_grad(frames, cfg, hold, p):
+    import copy
+    out = np.zeros(6)
+    for k, name in enumerate(_GRAD_FIELDS):
+        vals = []
+        for sgn in (1, -1):
+            c = copy.deepcopy(cfg)
+            arr = getattr(c, name)
+            arr[p] = arr[p] + sgn * _GRAD_STEPS[k]
+            assert np.float32(arr[p]) == arr[p]
+            _, coefs, _ = O.process_batch(frames, c, hold=hold)
+            vals.append(O.alignment_loss(coefs))
+        out[k] = (vals[0] - vals[1]) / (2 * _GRAD_STEPS[k])
+    return out
+
+
+@pytest.mark.parametrize("resolution_mode", [1, 0])
+def test_alignment_gradient_matches_central_differences(resolution_mode):
+    """The FP64 adjoint (tilt/defocus phase, window-ramp shift, waist basis
+    derivative) equals central differences of the loss with the placement held."""
+    spec = synth.spec_small()
+    spec.tilt_x_deg, spec.tilt_y_deg = [1.52, 1.52], [1.47, 1.47]
+    spec.beam_centre_x, spec.beam_centre_y, spec.defocus = [30e-6, 0], [-20e-6, 0], [0.5, 0]
+    frames, _, _, _ = synth.make_batch(spec, 6, seed=11)
+    cfg = small_cfg(tilt_x=[_f32(1.51)] * 2, tilt_y=[_f32(1.48)] * 2, beam_centre_x=[_f32(25e-6), 0.0],
+                    beam_centre_y=[_f32(-15e-6), 0.0], defocus=[_f32(0.3), 0.0], waist=[_f32(210e-6)] * 2,
+                    fill_factor=1, resolution_mode=resolution_mode)
+    hold = O.hold_of(cfg)
+    _, g = O.alignment_grad(frames, cfg, hold)
+    fd = _fd_grad(frames, cfg, hold, 0)
+    assert np.allclose(g[0], fd, rtol=2e-6, atol=0), (g[0], fd)
+
+
+def test_alignment_gradient_dual_pol_locks():
+    """Dual polarisation with tilt and waist locked: pol 0 collects both pols'
+    terms of a locked parameter; the unlocked ones are per pol."""
+    spec = synth.HologramSpec(frame_width=320, frame_height=256, pol_count=2, waist=150e-6, group_count=6)
+    spec.beam_centre_x, spec.beam_centre_y = [-1.6e-3, 1.6e-3], [10e-6, -10e-6]
+    spec.tilt_x_deg, spec.tilt_y_deg, spec.defocus = [1.5, 1.5], [1.5, 1.5], [0.2, -0.3]
+    frames, _, _, _ = synth.make_batch(spec, 4, seed=2)
+    cfg = O.Config(frame_width=320, frame_height=256, pol_count=2, fft_window_x=128, fft_window_y=128,
+                   aperture_deg=0.7, tilt_x=[_f32(1.49), _f32(1.49)], tilt_y=[_f32(1.51), _f32(1.51)],
+                   beam_centre_x=[_f32(-1.59e-3), _f32(1.605e-3)], beam_centre_y=[_f32(15e-6), _f32(-5e-6)],
+                   defocus=[_f32(0.1), _f32(-0.2)], waist=[_f32(160e-6)] * 2, group_count=6, pol_lock_tilt=1,
+                   pol_lock_waist=1)
+    hold = O.hold_of(cfg)
+    _, g = O.alignment_grad(frames, cfg, hold)
+    for p in range(2):
+        fd = _fd_grad(frames, cfg, hold, p)
+        if p == 1:   # locked: pol 1's own values are not read
+            fd[[0, 2, 4]] = 0.0
+        assert np.allclose(g[p], fd, rtol=2e-6, atol=1e-9 * np.abs(g).max()), (p, g[p], fd)
