@@ -142,8 +142,8 @@ SIGNATURES: Dict[str, Tuple[object, List[object]]] = {
     "digHoloGetWindowPath": (c_int, [c_int]),
     "digHoloConfigSetStageTiming": (c_int, [c_int, c_int]),
     "digHoloGetStageTimes": (c_int, [c_int, P_float]),
-    "digHoloAutoAlignGradient": (c_int, [c_int, c_int, P_float, P_float]),
-    "digHoloAutoAlignGradientSearch": (c_float, [c_int, c_int]),
+    "digHoloAutoAlignGradient": (c_int, [c_int, c_int, c_float, P_float, P_float]),
+    "digHoloAutoAlignGradientSearch": (c_float, [c_int, c_int, c_float]),
 }
 
 # constants of the header
@@ -360,11 +360,11 @@ def get_align_params(lib, h, P: int) -> np.ndarray:
                      for p in range(P)], dtype=np.float64)
 
 
-def alignment_gradient(lib, h, hold: bool = False):
+def alignment_gradient(lib, h, hold: bool = False, mdl_weight: float = 0.0):
     """digHoloAutoAlignGradient: (loss, grad [2, 6] float64)."""
     loss = c_float()
     grad = (c_float * 12)()
-    rc = lib.digHoloAutoAlignGradient(h, 1 if hold else 0, C.byref(loss), grad)
+    rc = lib.digHoloAutoAlignGradient(h, 1 if hold else 0, mdl_weight, C.byref(loss), grad)
     if rc != DIGHOLO_ERROR_SUCCESS:
         raise RuntimeError(f"digHoloAutoAlignGradient returned {rc}")
     return float(loss.value), np.array(list(grad), dtype=np.float64).reshape(2, 6)

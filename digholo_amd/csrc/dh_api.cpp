@@ -212,6 +212,7 @@ struct Handle {
   DevBuf<float> dhgx, dhgy;
   DevBuf<float2> gq1, gq2, gx1, gx2;
   DevBuf<double> gPart, gSum, gAxes;
+  DevBuf<float2> gExtra;
   Derived gradHeld;
   bool gradHeldValid = false;
 
@@ -2027,7 +2028,85 @@ int autoalign_snap(Handle& h, bool doFinal, float* steps) {
 // centre and tilt: the gradient is that of the smooth piece the current
 // placement belongs to; hold keeps the previous call's placement.
 // ---------------------------------------------------------------------------
-int alignment_gradient(Handle& h, bool hold, double& loss, double grad[2][AA_PARS]) {
+// Mode-dependent-loss term mu (ln s_max^2 - ln s_min^2) of the coefficient
+// matrix the power meter takes the MDL of (digHolo.cpp:10796-11089: frames as
+// rows, both pols side by side unless independent), and its seed
+// dterm/dc (Wirtinger) into h.gExtra: with (u, s, v) a singular triplet,
+// d ln s^2 = 2 Re(u^H dA v) / s, i.e. conj(u_b) v_j / s.  The extreme
+// eigenpairs of the smaller FP64 Gram matrix come from the GPU Gram and the
+// host eigen-solver; the other singular vector is A v / s or A^H u / s.
+int mdl_seed(Handle& h, double mu, double& term) {
+  const Config& c = h.cfg;
+  const dh::Geometry& g = h.cur.g;
+  const int B = g.B, P = g.P, Mpol = g.M;
+  const bool indep = c.aaPolIndependence != 0 || P == 1;
+  const int S = (P == 2 && indep) ? 2 : 1;
+  const int Mm = (P == 2 && !indep) ? 2 * Mpol : Mpol;
+  const int rowStride = P * Mpol;
+  term = 0.0;
+  const size_t total = (size_t)B * P * Mpol;
+  DH_TRY(h.gExtra.ensure(total), "alloc");
+  std::vector<float2> A(total);
+  DH_TRY(hipMemcpyAsync(A.data(), h.coefs.p, total * sizeof(float2), hipMemcpyDeviceToHost, h.stream), "D2H");
+  const bool rowsVec = B <= Mm;
+  const int dim = rowsVec ? B : Mm;
+  struct Pick {
+    int s = -1;
+    double l = 0;
+    std::vector<std::complex<double>> x;
+  } lo, hi;
+  for (int s = 0; s < S; ++s) {
+    DH_TRY(h.gram.ensure((size_t)dim * dim), "alloc");
+    DH_TRY(dh::launch_gram_f64(h.coefs.p, B, Mm, rowStride, s, rowsVec ? 1 : 0, h.gram.p, h.stream), "gram");
+    std::vector<std::complex<double>> G((size_t)dim * dim);
+    DH_TRY(hipMemcpyAsync(G.data(), h.gram.p, G.size() * sizeof(double2), hipMemcpyDeviceToHost, h.stream), "D2H");
+    DH_TRY(hipStreamSynchronize(h.stream), "sync");
+    dh::HermitianExtremePairs r = dh::hermitian_extreme_pairs(dim, G);
+    if (lo.s < 0 || r.lmin < lo.l) lo = Pick{s, r.lmin, r.vmin};
+    if (hi.s < 0 || r.lmax > hi.l) hi = Pick{s, r.lmax, r.vmax};
+  }
+  std::vector<float2> gx(total, make_float2(0.f, 0.f));
+  if (dim >= 2 && mu != 0.0) {
+    if (!(lo.l > 0) || !(hi.l > 0)) return DIGHOLO_ERROR_INVALIDARGUMENT;   // MDL undefined
+    term = mu * (std::log(hi.l) - std::log(lo.l));
+    for (const Pick* pk : {&hi, &lo}) {
+      const double sig = std::sqrt(pk->l);
+      const double coef = (pk == &hi ? mu : -mu) / sig;
+      std::vector<std::complex<double>> u(B), v(Mm);
+      auto at = [&](int b, int j) {
+        const float2 a = A[(size_t)b * rowStride + (size_t)pk->s * Mpol + j];
+        return std::complex<double>(a.x, a.y);
+      };
+      if (rowsVec) {
+        u = pk->x;
+        for (int j = 0; j < Mm; ++j) {
+          std::complex<double> t = 0;
+          for (int b = 0; b < B; ++b) t += std::conj(at(b, j)) * u[b];
+          v[j] = t / sig;
+        }
+      } else {
+        v = pk->x;
+        for (int b = 0; b < B; ++b) {
+          std::complex<double> t = 0;
+          for (int j = 0; j < Mm; ++j) t += at(b, j) * v[j];
+          u[b] = t / sig;
+        }
+      }
+      for (int b = 0; b < B; ++b)
+        for (int j = 0; j < Mm; ++j) {
+          const std::complex<double> d = coef * std::conj(u[b]) * v[j];
+          float2& o = gx[(size_t)b * rowStride + (size_t)pk->s * Mpol + j];
+          o.x += (float)d.real();
+          o.y += (float)d.imag();
+        }
+    }
+  }
+  DH_TRY(hipMemcpyAsync(h.gExtra.p, gx.data(), total * sizeof(float2), hipMemcpyHostToDevice, h.stream), "upload");
+  DH_TRY(hipStreamSynchronize(h.stream), "sync");   // gx leaves scope
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+int alignment_gradient(Handle& h, bool hold, double mdlWeight, double& loss, double grad[2][AA_PARS]) {
   Config& c = h.cfg;
   if (c.maxMG <= 0) return DIGHOLO_ERROR_INVALIDARGUMENT;
   const bool useHold = hold && h.gradHeldValid;
@@ -2085,8 +2164,14 @@ int alignment_gradient(Handle& h, bool hold, double& loss, double grad[2][AA_PAR
   DH_TRY(h.gAxes.ensure(ax.size()), "alloc");
   DH_TRY(hipMemcpyAsync(h.gAxes.p, ax.data(), ax.size() * sizeof(double), hipMemcpyHostToDevice, h.stream),
          "axes upload");
+  double mdlTerm = 0.0;
+  const float2* gExtra = nullptr;
+  if (mdlWeight != 0.0) {
+    if (int em = mdl_seed(h, mdlWeight, mdlTerm)) return em;
+    gExtra = h.gExtra.p;
+  }
   DH_TRY(dh::launch_hg_dwaist(d.tab, h.dhgx.p, h.dhgy.p, h.stream), "waist derivative basis");
-  DH_TRY(dh::launch_grad_backproject(gF, 1.0 / B, h.coefs.p, h.hgx.p, h.hgy.p, h.dhgx.p, h.dhgy.p, h.fields.p,
+  DH_TRY(dh::launch_grad_backproject(gF, 1.0 / B, h.coefs.p, gExtra, h.hgx.p, h.hgy.p, h.dhgx.p, h.dhgy.p, h.fields.p,
                                      h.ref.p, h.gAxes.p, h.gq1.p, h.gq2.p, h.gPart.p, h.stream), "back-projection");
   DH_TRY(dh::launch_ifft_rows(gF, *pIx, h.gq1.p, h.stream), "adjoint IFFT rows");
   DH_TRY(dh::launch_ifft_cols_tilt(gF, *pIy, h.gq1.p, nullptr, h.gx1.p, h.stream), "adjoint IFFT columns");
@@ -2102,7 +2187,7 @@ int alignment_gradient(Handle& h, bool hold, double& loss, double grad[2][AA_PAR
   const double lam = (double)c.wavelengthCentre, k0 = 2.0 * PI_D / lam, deg = PI_D / 180.0;
   for (int p = 0; p < 2; ++p)
     for (int k = 0; k < AA_PARS; ++k) grad[p][k] = 0.0;
-  loss = 0.0;
+  loss = mdlTerm;
   for (int p = 0; p < P; ++p) {
     const int tp = (c.polLockTilt || c.polLockDefocus) ? 0 : p;
     const int ip = c.polLockDefocus ? 0 : p;
@@ -2127,7 +2212,7 @@ int alignment_gradient(Handle& h, bool hold, double& loss, double grad[2][AA_PAR
 // the step the tweak search would start with, Armijo backtracking, placement
 // re-derived at every trial point.  Ends at the best point with the metrics of
 // a final evaluation stored; returns the goal metric in dB.
-int autoalign_gradient_search(Handle& h, int maxIter, float& goalDb) {
+int autoalign_gradient_search(Handle& h, int maxIter, double mdlWeight, float& goalDb) {
   Config& c = h.cfg;
   const int P = c.polCount;
   HoldGuard guard(h);
@@ -2162,13 +2247,13 @@ int autoalign_gradient_search(Handle& h, int maxIter, float& goalDb) {
   auto eval = [&](const std::vector<double>& z, double& f, std::vector<double>& gz) -> int {
     apply(z);
     double L, Gr[2][AA_PARS];
-    if (int e = alignment_gradient(h, false, L, Gr)) return e;
+    if (int e = alignment_gradient(h, false, mdlWeight, L, Gr)) return e;
     f = L / norm;
     gz.resize(nv);
     for (int i = 0; i < nv; ++i) gz[i] = Gr[vars[i].p][vars[i].par] * vars[i].scale / norm;
     return 0;
   };
-  if (int e = alignment_gradient(h, false, L0, G0)) return e;
+  if (int e = alignment_gradient(h, false, mdlWeight, L0, G0)) return e;
   norm = std::max(std::fabs(L0), 1e-30);
   std::vector<double> z(nv, 0.0), gz, zBest = z;
   double f = 0, fBest;
@@ -3005,24 +3090,24 @@ EXT_C float digHoloAutoAlign(int handleIdx) {
   }
   return goal;
 }
-EXT_C int digHoloAutoAlignGradient(int handleIdx, int holdPlacement, float* loss, float* grad) {
+EXT_C int digHoloAutoAlignGradient(int handleIdx, int holdPlacement, float mdlWeight, float* loss, float* grad) {
   H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
   if (!loss || !grad) return DIGHOLO_ERROR_NULLPOINTER;
   if (!h->frames) return DIGHOLO_ERROR_NULLPOINTER;
   double L = 0.0, gr[2][AA_PARS] = {};
-  const int e = alignment_gradient(*h, holdPlacement != 0, L, gr);
+  const int e = alignment_gradient(*h, holdPlacement != 0, (double)mdlWeight, L, gr);
   if (e) return e;
   *loss = (float)L;
   for (int p = 0; p < 2; ++p)
     for (int k = 0; k < AA_PARS; ++k) grad[p * AA_PARS + k] = (float)gr[p][k];
   return DIGHOLO_ERROR_SUCCESS;
 }
-EXT_C float digHoloAutoAlignGradientSearch(int handleIdx, int maxIterations) {
+EXT_C float digHoloAutoAlignGradientSearch(int handleIdx, int maxIterations, float mdlWeight) {
   H_OR(0.f);
   const float FMAX = std::numeric_limits<float>::max();
   if (!h->frames || CFG.maxMG <= 0) return -FMAX;
   float goal = -FMAX;
-  const int e = autoalign_gradient_search(*h, maxIterations > 0 ? maxIterations : 30, goal);
+  const int e = autoalign_gradient_search(*h, maxIterations > 0 ? maxIterations : 30, (double)mdlWeight, goal);
   if (e) {
     fprintf(stderr, "digHolo: gradient alignment failed (error %d)\n", e);
     return -FMAX;

@@ -1,11 +1,13 @@
 // Adjoint of the batch path: the gradient of the alignment loss
-//   L = -(1/B) sum_{b,p,mode} |c|^2
+//   L = -(1/B) sum_{b,p,mode} |c|^2  [+ mu (ln s_max^2 - ln s_min^2)]
 // with respect to each pol's waist, defocus, tilt and beam centre, with the
 // window placement (FFT crops, Fourier-window position, size and mask) held.
 // No counterpart in the reference, whose AutoAlign searches without
 // derivatives (digHolo.cpp:12315-12885).
 //
-// With g = dL/dc = -conj(c)/B (Wirtinger) and the adjoint field
+// With g = dL/dc = -conj(c)/B (Wirtinger; plus the mode-dependent-loss
+// term's seed gExtra, formed on the host from the extreme singular triplets
+// of the coefficient matrix) and the adjoint field
 // A(x, y) = sum_mn g_mn HGx_m(x) HGy_n(y) (the transposed overlap), the
 // field V = -conj(E R) responds to
 //   tilt:    dV/dtheta_x = i k0 (x + bcX) V          (reference-wave slope and piston)
@@ -101,7 +103,8 @@ __device__ __forceinline__ void block_sum(double (&v)[NS], double* sh, double* o
 // One workgroup per (pol, frame).  partial[bp][8] = {sum Im(AV), sum x Im(AV),
 // sum y Im(AV), sum r^2 Im(AV), sum Re(Aw V), sum |c|^2, -, -}.
 __global__ __launch_bounds__(kGThreads) void k_grad_backproject(
-    Geometry g, double invB, const float2* __restrict__ coefs, const float* __restrict__ hgx,
+    Geometry g, double invB, const float2* __restrict__ coefs, const float2* __restrict__ gExtra,
+    const float* __restrict__ hgx,
     const float* __restrict__ hgy, const float* __restrict__ dhgx, const float* __restrict__ dhgy,
     const float2* __restrict__ fields, const float2* __restrict__ ref, const double* __restrict__ pxy2,
     float2* __restrict__ q1, float2* __restrict__ q2, double* __restrict__ partial) {
@@ -117,7 +120,13 @@ __global__ __launch_bounds__(kGThreads) void k_grad_backproject(
   for (int e = threadIdx.x; e < M; e += kGThreads) {
     const float2 v = c[e];
     acc[5] += (double)v.x * v.x + (double)v.y * v.y;
-    gs[e] = make_float2((float)(-v.x * invB), (float)(v.y * invB));   // -conj(c) / B
+    float2 gv = make_float2((float)(-v.x * invB), (float)(v.y * invB));   // -conj(c) / B
+    if (gExtra) {
+      const float2 x = gExtra[bp * M + e];
+      gv.x += x.x;
+      gv.y += x.y;
+    }
+    gs[e] = gv;
   }
   const float* hx = hgx + (size_t)p * G * fw;
   const float* hy = hgy + (size_t)p * G * fh;
@@ -224,7 +233,8 @@ size_t grad_backproject_lds(const Geometry& g) {
   return (size_t)g.M * sizeof(float2) + 2 * (size_t)g.G * kGXC * sizeof(float2);
 }
 
-hipError_t launch_grad_backproject(const Geometry& g, double invB, const float2* coefs, const float* hgx,
+hipError_t launch_grad_backproject(const Geometry& g, double invB, const float2* coefs, const float2* gExtra,
+                                   const float* hgx,
                                    const float* hgy, const float* dhgx, const float* dhgy, const float2* fields,
                                    const float2* ref, const double* pxy2, float2* q1, float2* q2, double* partial,
                                    hipStream_t s) {
@@ -235,7 +245,7 @@ hipError_t launch_grad_backproject(const Geometry& g, double invB, const float2*
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  k_grad_backproject<<<dim3(g.P, g.B), kGThreads, lds, s>>>(g, invB, coefs, hgx, hgy, dhgx, dhgy, fields, ref,
+  k_grad_backproject<<<dim3(g.P, g.B), kGThreads, lds, s>>>(g, invB, coefs, gExtra, hgx, hgy, dhgx, dhgy, fields, ref,
                                                             pxy2, q1, q2, partial);
   return hipGetLastError();
 }

@@ -184,7 +184,9 @@ hipError_t launch_snap_tilt_search(const float* map, int Ny, int Nx, float px, f
 // partial[bp][6..7] = sum_k {kx, ky} Im(X1 conj X2) with kx [P][fw], ky [P][fh].
 hipError_t launch_hg_dwaist(const TableArgs& a, float* dhgx, float* dhgy, hipStream_t s);
 size_t grad_backproject_lds(const Geometry& g);
-hipError_t launch_grad_backproject(const Geometry& g, double invB, const float2* coefs, const float* hgx,
+// gExtra (nullable) [B][P][M] is added to the seed -conj(c)/B.
+hipError_t launch_grad_backproject(const Geometry& g, double invB, const float2* coefs, const float2* gExtra,
+                                   const float* hgx,
                                    const float* hgy, const float* dhgx, const float* dhgy, const float2* fields,
                                    const float2* ref, const double* pxy2, float2* q1, float2* q2, double* partial,
                                    hipStream_t s);

@@ -275,20 +275,23 @@ EXT_C int digHoloConfigSetStageTiming(int handleIdx, int enabled);
 EXT_C int digHoloGetStageTimes(int handleIdx, float* ms);
 
 /* Alignment-loss gradient.  Processes the batch at the current configuration
- * and returns loss = -(1/B) sum |c|^2 (minus the linear insertion loss without
- * the SVD) and grad[pol * 6 + k] = d loss / d parameter k of pol, k in the
+ * and returns loss = -(1/B) sum |c|^2 + mdlWeight (ln s_max^2 - ln s_min^2)
+ * (minus the linear insertion loss without the SVD, plus a mode-dependent-loss
+ * term over the singular values s of the coefficient matrix the MDL metric
+ * uses) and grad[pol * 6 + k] = d loss / d parameter k of pol, k in the
  * AutoAlign order waist (m), defocus (dioptre), tilt x (degree), centre x (m),
  * tilt y (degree), centre y (m), from the adjoint of the path (exact for the
  * current window placement; pol-locked parameters collect both pols' terms on
  * pol 0).  holdPlacement != 0 keeps the FFT crops and Fourier window (position,
  * size, mask) of the previous call, so the loss is a smooth function of the
  * parameters between calls (for finite differences and line searches). */
-EXT_C int digHoloAutoAlignGradient(int handleIdx, int holdPlacement, float* loss, float* grad);
+EXT_C int digHoloAutoAlignGradient(int handleIdx, int holdPlacement, float mdlWeight, float* loss, float* grad);
 /* Gradient-based alignment beside digHoloAutoAlign's search: BFGS on the
- * parameters the AutoAlign enable flags select (pol locks respected), from the
- * current values, at most maxIterations iterations (<= 0: 30).  Ends processed
+ * loss above (mdlWeight as there) over the parameters the AutoAlign enable
+ * flags select (pol locks respected), from the current values, at most
+ * maxIterations iterations (<= 0: 30).  Ends processed
  * at the best point with its metrics stored (AutoAlignGetMetric); returns the
  * goal metric in dB. */
-EXT_C float digHoloAutoAlignGradientSearch(int handleIdx, int maxIterations);
+EXT_C float digHoloAutoAlignGradientSearch(int handleIdx, int maxIterations, float mdlWeight);
 
 #endif

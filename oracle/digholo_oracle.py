@@ -1065,14 +1065,51 @@ def hg_modes_1d_dwaist(nmax: int, x: np.ndarray, waist: float, dx: float) -> np.
     return out
 
 
-def alignment_loss(coefs: np.ndarray) -> float:
-    """L = -(1/B) sum |c|^2: minus the linear insertion loss of the batch without
-    the SVD (digHolo.cpp:11046-11063, IL = sum |A|^2 / B)."""
+def _mdl_matrices(coefs: np.ndarray, independent: bool):
+    """The matrices the MDL metric takes singular values of (frames as rows, both
+    pols side by side unless independent, digHolo.cpp:10796-11089), with the
+    (pol offset) of their columns."""
+    c = np.asarray(coefs, dtype=np.complex128)
+    B, P, M = c.shape
+    if P == 2 and not independent:
+        return [(c.reshape(B, 2 * M), 0)]
+    return [(c[:, p, :], p) for p in range(P)]
+
+
+def alignment_loss(coefs: np.ndarray, mdl_weight: float = 0.0, independent: bool = False) -> float:
+    """L = -(1/B) sum |c|^2 + mu (ln s_max^2 - ln s_min^2): minus the linear
+    insertion loss of the batch without the SVD (digHolo.cpp:11046-11063,
+    IL = sum |A|^2 / B), plus a mode-dependent-loss term over the singular values
+    of the matrices the MDL metric uses."""
     c = np.asarray(coefs)
-    return -float((np.abs(c) ** 2).sum()) / c.shape[0]
+    L = -float((np.abs(c) ** 2).sum()) / c.shape[0]
+    if mdl_weight:
+        sv = np.concatenate([np.linalg.svd(A, compute_uv=False) for A, _ in _mdl_matrices(c, independent)])
+        L += mdl_weight * (math.log(sv.max() ** 2) - math.log(sv.min() ** 2))
+    return L
 
 
-def alignment_grad(frames: np.ndarray, cfg: Config, hold: Hold):
+def _mdl_seed(coefs: np.ndarray, mdl_weight: float, independent: bool) -> np.ndarray:
+    """dL_mdl/dc (Wirtinger): d ln s^2 = 2 Re(u^H dA v) / s -> conj(u_b) v_j / s."""
+    c = np.asarray(coefs, dtype=np.complex128)
+    B, P, M = c.shape
+    out = np.zeros_like(c)
+    trip = []
+    for A, off in _mdl_matrices(c, independent):
+        U, S, Vh = np.linalg.svd(A, full_matrices=False)
+        trip.append((S[0], U[:, 0], np.conj(Vh[0]), off, A.shape[1]))
+        trip.append((S[-1], U[:, -1], np.conj(Vh[-1]), off, A.shape[1]))
+    hi = max(trip, key=lambda t: t[0])
+    lo = min(trip, key=lambda t: t[0])
+    for (s, u, v, off, n), sign in ((hi, 1.0), (lo, -1.0)):
+        gm = sign * mdl_weight * np.outer(np.conj(u), v) / s          # [B, n]
+        flat = out.reshape(B, P * M)
+        flat[:, off * M:off * M + n] += gm
+    return out
+
+
+def alignment_grad(frames: np.ndarray, cfg: Config, hold: Hold, mdl_weight: float = 0.0,
+                   independent: bool = False):
     """Loss and its gradient with respect to [pol][waist, defocus (dioptre),
     tilt x (degree), centre x (m), tilt y (degree), centre y (m)] at the
     configuration, with the placement `hold` fixed.
@@ -1089,8 +1126,10 @@ def alignment_grad(frames: np.ndarray, cfg: Config, hold: Hold):
     cl = apply_pol_locks(cfg)
     fields, coefs, plan = process_batch(frames, cfg, hold=hold)
     B, P = fields.shape[:2]
-    loss = alignment_loss(coefs)
+    loss = alignment_loss(coefs, mdl_weight, independent)
     g = -np.conj(coefs) / B
+    if mdl_weight:
+        g = g + _mdl_seed(coefs, mdl_weight, independent)
     low = cl.resolution_mode == 1
     W, H = plan.W, plan.H
     fw, fh = (plan.w, plan.h) if low else (W, H)

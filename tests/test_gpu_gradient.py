@@ -75,22 +75,27 @@ def oracle_cfg(cfg, start):
                     pol_lock_waist=cfg.get("pol_lock_waist", 1))
 
 
-@pytest.mark.parametrize("case", ["small", "small_fullres", "dual"])
-def test_gradient_matches_oracle(lib, case):
+@pytest.mark.parametrize("case,mdl,B", [("small", 0, 6), ("small_fullres", 0, 6), ("dual", 0, 6),
+                                         ("dual", 1, 6), ("small", 1, 60)])
+def test_gradient_matches_oracle(lib, case, mdl, B):
+    """mdl = 1 adds the mode-dependent-loss term with a weight of 5 % of the
+    power term (B = 6 < modes: Gram of the frames; B = 60 > 55 modes: Gram of
+    the modes)."""
     spec, cfg, start = dual_case() if case == "dual" else small_case(0 if case == "small_fullres" else 1)
-    B = 6
     frames = np.ascontiguousarray(synth.make_batch(spec, B, seed=11)[0])
     P = cfg["pol_count"]
-    print(f"case {case}: {frames.shape}", flush=True)
+    print(f"case {case}: {frames.shape} mdl {mdl}", flush=True)
+    oc = oracle_cfg(cfg, start)
+    hold = O.hold_of(oc)
+    mu = 0.05 * abs(O.alignment_grad(frames, oc, hold)[0]) if mdl else 0.0
     h = lib.digHoloCreate()
     try:
         api.configure(lib, h, **{**cfg, **start})
         assert lib.digHoloSetBatchUint16(h, B, frames.ctypes.data, 0) == 0
-        loss, grad = api.alignment_gradient(lib, h)
+        loss, grad = api.alignment_gradient(lib, h, mdl_weight=mu)
     finally:
         lib.digHoloDestroy(h)
-    oc = oracle_cfg(cfg, start)
-    want_loss, want = O.alignment_grad(frames, oc, O.hold_of(oc))
+    want_loss, want = O.alignment_grad(frames, oc, hold, mdl_weight=mu)
     print("library", loss, grad[:P], "\noracle ", want_loss, want, flush=True)
     assert loss == pytest.approx(want_loss, rel=2e-6)
     a = grad[:P] * SCALE / abs(want_loss)
@@ -165,7 +170,7 @@ def test_gradient_search_matches_tweak(lib):
             assert lib.digHoloSetBatchUint16(h, B, frames.ctypes.data, 0) == 0
             api.process_batch(lib, h, frames)
             t0 = time.perf_counter()
-            ret = lib.digHoloAutoAlign(h) if method == "tweak" else lib.digHoloAutoAlignGradientSearch(h, 40)
+            ret = lib.digHoloAutoAlign(h) if method == "tweak" else lib.digHoloAutoAlignGradientSearch(h, 40, 0.0)
             dt = time.perf_counter() - t0
             il = lib.digHoloAutoAlignGetMetric(h, O.METRIC_IL)
             out[method] = (ret, il, dt, api.get_align_params(lib, h, 2))

@@ -664,3 +664,35 @@ def test_alignment_gradient_dual_pol_locks():
         if p == 1:   # locked: pol 1's own values are not read
             fd[[0, 2, 4]] = 0.0
         assert np.allclose(g[p], fd, rtol=2e-6, atol=1e-9 * np.abs(g).max()), (p, g[p], fd)
+
+
+@pytest.mark.parametrize("independent", [False, True])
+def test_alignment_gradient_with_mdl_term(independent):
+    """The mode-dependent-loss term's seed (extreme singular triplets) carried
+    through the same adjoint equals central differences of the combined loss."""
+    import copy
+    spec = synth.HologramSpec(frame_width=320, frame_height=256, pol_count=2, waist=150e-6, group_count=5)
+    spec.beam_centre_x, spec.beam_centre_y = [-1.6e-3, 1.6e-3], [10e-6, -10e-6]
+    spec.defocus = [0.2, -0.3]
+    frames, _, _, _ = synth.make_batch(spec, 5, seed=4)
+    cfg = O.Config(frame_width=320, frame_height=256, pol_count=2, fft_window_x=128, fft_window_y=128,
+                   aperture_deg=0.7, tilt_x=[_f32(1.49), _f32(1.52)], tilt_y=[_f32(1.51), _f32(1.48)],
+                   beam_centre_x=[_f32(-1.59e-3), _f32(1.605e-3)], beam_centre_y=[_f32(15e-6), _f32(-5e-6)],
+                   defocus=[_f32(0.1), _f32(-0.2)], waist=[_f32(160e-6)] * 2, group_count=5, pol_lock_waist=0)
+    hold = O.hold_of(cfg)
+    mu = 3e7
+    _, g = O.alignment_grad(frames, cfg, hold, mdl_weight=mu, independent=independent)
+    for p in range(2):
+        fd = np.zeros(6)
+        for k, name in enumerate(_GRAD_FIELDS):
+            vals = []
+            for sgn in (1, -1):
+                c = copy.deepcopy(cfg)
+                getattr(c, name)[p] += sgn * _GRAD_STEPS[k]
+                _, coefs, _ = O.process_batch(frames, c, hold=hold)
+                vals.append(O.alignment_loss(coefs, mu, independent))
+            fd[k] = (vals[0] - vals[1]) / (2 * _GRAD_STEPS[k])
+        assert np.allclose(g[p], fd, rtol=1e-5, atol=1e-8 * np.abs(g).max()), (p, g[p], fd)
+    # the MDL term moves the gradient: it is not the power-only one
+    _, g0 = O.alignment_grad(frames, cfg, hold)
+    assert np.abs(g - g0).max() > 1e-3 * np.abs(g0).max()
