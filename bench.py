@@ -61,21 +61,21 @@ def measured_traffic(kernel, fused_overlap):
     return ent.get("bytes_per_launch") if isinstance(ent, dict) else None
 
 
-def make_frames(n_unique=100, seed=1):
-    """n_unique rendered holograms; every one of the B frames gets its own read noise."""
+def make_frames(n_unique=100, seed=1, spec=None, batch=B):
+    """n_unique rendered holograms; every one of the batch frames gets its own read noise."""
     import numpy as np
     import torch
     from digholo_amd import synth
-    spec = synth.spec_dual_640()
+    spec = spec or synth.spec_dual_640()
     rng = np.random.default_rng(seed)
     coefs = synth.random_coefs(spec, n_unique, rng)
     clean, _, _ = synth.render(spec, coefs, quantise=False)
     base = torch.from_numpy(clean.astype(np.float32)).cuda()
-    out = torch.empty((B, FRAME_H, FRAME_W), dtype=torch.int16, device="cuda")
+    out = torch.empty((batch, FRAME_H, FRAME_W), dtype=torch.int16, device="cuda")
     g = torch.Generator(device="cuda")
     g.manual_seed(seed)
-    for b0 in range(0, B, n_unique):
-        n = min(n_unique, B - b0)
+    for b0 in range(0, batch, n_unique):
+        n = min(n_unique, batch - b0)
         noisy = base[:n] + spec.noise_counts * torch.randn(base[:n].shape, generator=g, device="cuda")
         out[b0:b0 + n] = noisy.round().clamp(0, 4095).to(torch.int16)
     torch.cuda.synchronize()
@@ -104,12 +104,62 @@ def cpu_baseline(min_s=10.0, max_s=30.0):
                       f"(single-threaded pocketfft + einsum), {el:.1f} s"}
 
 
+def autoalign_comparison(lib, seed=3, batch=500):
+    """Alignment workload: 640x512 dual-pol frames, batch 500, 45 groups, with the
+    true tilt (+-0.2 deg), defocus (+-0.5 dioptre), waist (+-10 %) and beam
+    centre (+-50 um) of each pol drawn from the seed and hidden.  Both searches
+    start from the nominal values (tilt 1.5 deg, centres -/+1.6 mm, defocus 0,
+    waist 300 um): the reference's tweak search (digHoloAutoAlign, TWEAK) and
+    the gradient search (digHoloAutoAlignGradientSearch), each timed on the
+    host around the call; the insertion loss and MDL each ends on are those of
+    its final evaluation."""
+    import numpy as np
+    from digholo_amd import api, synth
+    rng = np.random.default_rng(seed)
+    spec = synth.spec_dual_640()
+    spec.tilt_x_deg = list(1.5 + rng.uniform(-0.2, 0.2, 2))
+    spec.tilt_y_deg = list(1.5 + rng.uniform(-0.2, 0.2, 2))
+    spec.defocus = list(rng.uniform(-0.5, 0.5, 2))
+    spec.waist = WAIST * (1 + rng.uniform(-0.1, 0.1))
+    spec.beam_centre_x = [BEAM_X[p] + rng.uniform(-50e-6, 50e-6) for p in range(2)]
+    spec.beam_centre_y = list(rng.uniform(-50e-6, 50e-6, 2))
+    frames = make_frames(n_unique=50, seed=seed, spec=spec, batch=batch)
+    res = {}
+    for method in ("tweak", "gradient"):
+        h = lib.digHoloCreate()
+        try:
+            api.configure(lib, h, frame_width=FRAME_W, frame_height=FRAME_H, pol_count=POLS,
+                          fft_window=(WIN, WIN), waist=WAIST, group_count=GROUPS, beam_centre_x=BEAM_X)
+            lib.digHoloConfigSetAutoAlignMode(h, api.DIGHOLO_AUTOALIGNMODE_TWEAK)
+            assert lib.digHoloSetBatchUint16(h, batch, C.c_void_p(frames.data_ptr()), 0) == 0
+            b, m, p = C.c_int(), C.c_int(), C.c_int()
+            lib.digHoloProcessBatchDevice(h, C.byref(b), C.byref(m), C.byref(p))   # tables, warm-up
+            lib.digHoloSynchronize(h)
+            t0 = time.perf_counter()
+            ret = lib.digHoloAutoAlign(h) if method == "tweak" else lib.digHoloAutoAlignGradientSearch(h, 40, 0.0)
+            dt = time.perf_counter() - t0
+            prm = api.get_align_params(lib, h, 2)
+            res[method] = {"seconds": round(dt, 4), "goal_il_db": round(float(ret), 4),
+                           "mdl_db": round(float(lib.digHoloAutoAlignGetMetric(h, 1)), 4),
+                           "tilt_err_deg": round(float(np.abs(prm[:, [2, 4]] - np.array(
+                               [spec.tilt_x_deg, spec.tilt_y_deg]).T).max()), 5),
+                           "defocus_err": round(float(np.abs(prm[:, 1] - np.array(spec.defocus)).max()), 4)}
+        finally:
+            lib.digHoloDestroy(h)
+    res["workload"] = (f"640x512 uint16 dual-pol, batch {batch}, 45 HG groups, hidden tilt/defocus/waist/centre "
+                       f"(seed {seed}), IL goal, both from the nominal start")
+    res["gradient_speedup"] = round(res["tweak"]["seconds"] / res["gradient"]["seconds"], 2)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autoalign", action="store_true",
+                    help="skip the gradient-vs-tweak alignment comparison (batch 500)")
     ap.add_argument("--precision", choices=["fp32", "split16"], default="split16")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for rocprofv3 --pmc: two batches with the fused split overlap, two with the FP32 "
@@ -233,6 +283,11 @@ def main():
                                      / HBM_PEAK_GBS, 4),
         "host_wall_s": round(wall, 3),
     }
+    if not args.no_autoalign:
+        lib.digHoloDestroy(h)
+        del frames
+        torch.cuda.empty_cache()
+        out["autoalign"] = autoalign_comparison(lib)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

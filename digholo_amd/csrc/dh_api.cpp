@@ -2332,6 +2332,9 @@ int autoalign_gradient_search(Handle& h, int maxIter, double mdlWeight, float& g
     if (quiet >= 2) break;
   }
   apply(zBest);
+  // the window radius follows the final tilt, as the tweak search leaves it
+  // (a third of the smallest tilt, digHolo.cpp:12560-12575)
+  if (c.aaFourierRadius) c.apertureSize = window_radius_rec(c);
   if (int e = enqueue_batch(h)) return e;
   PowerMeterOut o;
   if (int e = power_meter(h, -1, c.aaPolIndependence != 0, true, true, true, c.aaMulConjTrans != 0, o)) return e;
