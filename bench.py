@@ -71,7 +71,7 @@ def make_frames(n_unique=100, seed=1, spec=None, batch=B):
     coefs = synth.random_coefs(spec, n_unique, rng)
     clean, _, _ = synth.render(spec, coefs, quantise=False)
     base = torch.from_numpy(clean.astype(np.float32)).cuda()
-    out = torch.empty((batch, FRAME_H, FRAME_W), dtype=torch.int16, device="cuda")
+    out = torch.empty((batch, spec.frame_height, spec.frame_width), dtype=torch.int16, device="cuda")
     g = torch.Generator(device="cuda")
     g.manual_seed(seed)
     for b0 in range(0, batch, n_unique):
@@ -152,6 +152,75 @@ def autoalign_comparison(lib, seed=3, batch=500):
     return res
 
 
+UNEVEN = [  # frame W, H, FFT window, HG groups, frames
+    (320, 256, 128, 10, 3000),
+    (640, 512, 192, 30, 1500),
+    (1024, 1024, 256, 45, 500),
+]
+
+
+def uneven_batching(lib, reps=20):
+    """Uneven batches: 5000 dual-pol uint16 frames across three handle
+    configurations (UNEVEN), frames from the single-pol generator's settings
+    (200 um waist, 1.5 deg tilt) at each size, kernels chosen by the autotuner.
+    Timed with HIP events: one graph launch of all three
+    (digHoloProcessBatchMultiDevice, on the first handle's stream) against the
+    sum of the three batches' own times (digHoloProcessBatchDevice each, back to
+    back on its handle's stream)."""
+    import torch
+    from digholo_amd import api, synth
+    hs, frames = [], []
+    for i, (W, H, win, G, n) in enumerate(UNEVEN):
+        spec = synth.HologramSpec(frame_width=W, frame_height=H, pol_count=2, waist=200e-6, group_count=min(G, 10),
+                                  beam_centre_x=[-W * 5e-6, W * 5e-6])
+        fr = make_frames(n_unique=10 if W >= 1024 else 25, seed=10 + i, spec=spec, batch=n)
+        frames.append(fr)
+        h = lib.digHoloCreate()
+        api.configure(lib, h, frame_width=W, frame_height=H, pol_count=2, fft_window=(win, win), waist=200e-6,
+                      group_count=G, beam_centre_x=(-W * 5e-6, W * 5e-6))
+        lib.digHoloConfigSetOverlapPrecision(h, api.DIGHOLO_PRECISION_SPLIT16)
+        lib.digHoloConfigSetAutoTune(h, 1)
+        assert lib.digHoloSetBatchUint16(h, n, C.c_void_p(fr.data_ptr()), 0) == 0
+        hs.append(h)
+    b, m, p = C.c_int(), C.c_int(), C.c_int()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+    for _ in range(2):          # tunes each shape once, then warm
+        for h in hs:
+            assert lib.digHoloProcessBatchDevice(h, C.byref(b), C.byref(m), C.byref(p))
+            lib.digHoloSynchronize(h)
+    seq_ms = 0.0                # each batch on its own, back to back on its stream
+    for h in hs:
+        st = torch.cuda.ExternalStream(lib.digHoloGetStream(h))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            assert lib.digHoloProcessBatchDevice(h, C.byref(b), C.byref(m), C.byref(p))
+        e1.record(st)
+        e1.synchronize()
+        seq_ms += e0.elapsed_time(e1) / reps
+    ids = (C.c_int * len(hs))(*hs)
+    stream = torch.cuda.ExternalStream(lib.digHoloGetStream(hs[0]))   # exists once the handle has run
+    for _ in range(2):
+        assert lib.digHoloProcessBatchMultiDevice(len(hs), ids) == 0
+    lib.digHoloSynchronize(hs[0])
+    ev[0].record(stream)
+    for _ in range(reps):
+        assert lib.digHoloProcessBatchMultiDevice(len(hs), ids) == 0
+    ev[1].record(stream)
+    ev[1].synchronize()
+    graph_ms = ev[0].elapsed_time(ev[1]) / reps
+    paths = [lib.digHoloGetWindowPath(h) for h in hs]
+    for h in hs:
+        lib.digHoloDestroy(h)
+    total = sum(u[4] for u in UNEVEN)
+    return {"workload": "5000 dual-pol uint16 frames: 3000 x 320x256/128/10 groups, 1500 x 640x512/192/30, "
+                        "500 x 1024x1024/256/45, split-fp16 overlap, autotuned kernels",
+            "graph_ms": round(graph_ms, 4), "graph_frames_s": round(total / (graph_ms * 1e-3), 1),
+            "sequential_ms": round(seq_ms, 4), "sequential_frames_s": round(total / (seq_ms * 1e-3), 1),
+            "graph_speedup": round(seq_ms / graph_ms, 3), "window_paths": paths}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,6 +229,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autoalign", action="store_true",
                     help="skip the gradient-vs-tweak alignment comparison (batch 500)")
+    ap.add_argument("--no-uneven", action="store_true",
+                    help="skip the three-configuration graph-launch measurement (5000 frames)")
     ap.add_argument("--precision", choices=["fp32", "split16"], default="split16")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for rocprofv3 --pmc: two batches with the fused split overlap, two with the FP32 "
@@ -283,11 +354,14 @@ def main():
                                      / HBM_PEAK_GBS, 4),
         "host_wall_s": round(wall, 3),
     }
+    lib.digHoloDestroy(h)
+    del frames
+    torch.cuda.empty_cache()
     if not args.no_autoalign:
-        lib.digHoloDestroy(h)
-        del frames
-        torch.cuda.empty_cache()
         out["autoalign"] = autoalign_comparison(lib)
+    if not args.no_uneven:
+        os.environ.setdefault("DIGHOLO_TUNE_CACHE", os.path.join(ROOT, "gpurun_out", "tune_cache.txt"))
+        out["uneven_batching"] = uneven_batching(lib)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -142,6 +142,9 @@ SIGNATURES: Dict[str, Tuple[object, List[object]]] = {
     "digHoloGetWindowPath": (c_int, [c_int]),
     "digHoloConfigSetStageTiming": (c_int, [c_int, c_int]),
     "digHoloGetStageTimes": (c_int, [c_int, P_float]),
+    "digHoloConfigSetAutoTune": (c_int, [c_int, c_int]),
+    "digHoloConfigGetAutoTune": (c_int, [c_int]),
+    "digHoloProcessBatchMultiDevice": (c_int, [c_int, P_int]),
     "digHoloAutoAlignGradient": (c_int, [c_int, c_int, c_float, P_float, P_float]),
     "digHoloAutoAlignGradientSearch": (c_float, [c_int, c_int, c_float]),
 }

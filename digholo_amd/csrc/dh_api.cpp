@@ -12,11 +12,13 @@
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -215,6 +217,13 @@ struct Handle {
   DevBuf<float2> gExtra;
   Derived gradHeld;
   bool gradHeldValid = false;
+
+  // autotuning: 0 off, 1 tune each new shape once, 2 remembered choices only;
+  // ovTpw: stage-2 tiles per wave of the separate overlap kernel (0: fewest)
+  int autoTune = 0;
+  int ovTpw = 0;
+  bool tuning = false, capturing = false;
+  unsigned tabVersion = 0;  // incremented whenever the device tables are rebuilt
 
   bool stageTiming = false;
   std::vector<std::array<hipEvent_t, 6>> evPool;
@@ -631,6 +640,139 @@ int ensure_row_twiddles(Handle& h, int W) {
   return DIGHOLO_ERROR_SUCCESS;
 }
 
+// ---------------------------------------------------------------------------
+// Autotuner (no counterpart in the reference).  For each batch shape -- frame
+// and window sizes, pols, groups, precision, input type, batch size rounded to
+// a power of two -- the applicable launch variants are timed on the batch
+// itself with HIP events: window path (size-specialised / MFMA columns / FFT
+// columns / staged), the split overlap fused into the window kernel or as its
+// own kernel, and the separate overlap kernel's stage-2 tiles per wave (the
+// fewest that cover the triangle, or twice that).  The fastest is remembered in
+// memory and in a text cache file ($DIGHOLO_TUNE_CACHE, else
+// $HOME/.cache/digholo_amd_tune.txt), one "key path fused tpw ms" line per
+// shape.
+// ---------------------------------------------------------------------------
+struct TuneChoice {
+  int windowMode = 3, fused = 1, tpw = 0;
+  float ms = 0.f;
+};
+std::mutex g_tuneMu;
+std::map<std::string, TuneChoice> g_tune;
+bool g_tuneLoaded = false;
+
+std::string tune_path() {
+  if (const char* e = std::getenv("DIGHOLO_TUNE_CACHE")) return e;
+  if (const char* home = std::getenv("HOME")) return std::string(home) + "/.cache/digholo_amd_tune.txt";
+  return "digholo_amd_tune.txt";
+}
+
+void tune_load_locked() {
+  if (g_tuneLoaded) return;
+  g_tuneLoaded = true;
+  FILE* f = std::fopen(tune_path().c_str(), "r");
+  if (!f) return;
+  char key[512];
+  TuneChoice t;
+  while (std::fscanf(f, "%511s %d %d %d %f", key, &t.windowMode, &t.fused, &t.tpw, &t.ms) == 5) g_tune[key] = t;
+  std::fclose(f);
+}
+
+void tune_save_locked() {
+  const std::string path = tune_path();
+  const size_t slash = path.find_last_of('/');
+  if (slash != std::string::npos) {
+    const std::string dir = path.substr(0, slash);
+    (void)std::system(("mkdir -p '" + dir + "' 2>/dev/null").c_str());
+  }
+  const std::string tmp = path + ".tmp";
+  FILE* f = std::fopen(tmp.c_str(), "w");
+  if (!f) return;
+  for (const auto& kv : g_tune)
+    std::fprintf(f, "%s %d %d %d %.6f\n", kv.first.c_str(), kv.second.windowMode, kv.second.fused, kv.second.tpw,
+                 kv.second.ms);
+  std::fclose(f);
+  std::rename(tmp.c_str(), path.c_str());
+}
+
+std::string tune_key(const Handle& h, const Derived& d) {
+  const Config& c = h.cfg;
+  const dh::Geometry& g = d.g;
+  int bb = 1;
+  while (bb < g.B) bb <<= 1;
+  char buf[256];
+  std::snprintf(buf, sizeof(buf), "f%dx%d_p%d_W%dx%d_w%dx%d_f%dx%d_G%d_pr%d_u%d_t%d_a%d_c%d_B%d", g.frameW, g.frameH,
+                g.P, g.W, g.H, g.w, g.h, g.fw, g.fh, g.G, c.precision, h.frameU16, h.transpose, c.avgCount,
+                c.calEnabled, bb);
+  return buf;
+}
+
+int enqueue_batch(Handle& h, int step);
+
+int autotune(Handle& h) {
+  Config& c = h.cfg;
+  Derived d;
+  int err;
+  if (!derive(c, c.batchCount, d, err, h.holdCrop ? &h.heldCrop : nullptr, h.holdWin ? &h.heldWin : nullptr))
+    return DIGHOLO_ERROR_SUCCESS;   // enqueue_batch reports the error
+  const std::string key = tune_key(h, d);
+  {
+    std::lock_guard<std::mutex> lk(g_tuneMu);
+    tune_load_locked();
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      h.windowMode = it->second.windowMode;
+      h.fusedOverlap = it->second.fused != 0;
+      h.ovTpw = it->second.tpw;
+      return DIGHOLO_ERROR_SUCCESS;
+    }
+  }
+  if (h.autoTune != 1) return DIGHOLO_ERROR_SUCCESS;
+  if (int e = ensure_device(h)) return e;
+  h.tuning = true;
+  const int minTpw = d.g.G > 0 ? dh::overlap_min_tiles_per_wave(d.g) : 0;
+  hipEvent_t e0, e1;
+  DH_TRY(hipEventCreate(&e0), "event create");
+  DH_TRY(hipEventCreate(&e1), "event create");
+  TuneChoice best;
+  best.ms = std::numeric_limits<float>::max();
+  std::vector<std::array<int, 3>> seen;   // (actual path, fused, tpw) already timed
+  int rc = DIGHOLO_ERROR_SUCCESS;
+  for (int wm = 3; wm >= 0 && !rc; --wm)
+    for (int fu = 1; fu >= 0 && !rc; --fu)
+      for (int tp : {0, 2 * minTpw}) {
+        if (tp > 32 || (tp && minTpw == 0)) continue;
+        h.windowMode = wm;
+        h.fusedOverlap = fu != 0;
+        h.ovTpw = tp;
+        if ((rc = enqueue_batch(h, 0))) break;
+        const std::array<int, 3> sig{h.lastPath, h.lastFusedOverlap ? 1 : 0, h.lastFusedOverlap ? 0 : tp};
+        if (std::find(seen.begin(), seen.end(), sig) != seen.end()) continue;
+        seen.push_back(sig);
+        constexpr int reps = 3;
+        (void)hipEventRecord(e0, h.stream);
+        for (int r = 0; r < reps && !rc; ++r) rc = enqueue_batch(h, 0);
+        (void)hipEventRecord(e1, h.stream);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms /= reps;
+        if (c.verbosity >= DIGHOLO_VERBOSITY_DEBUG)
+          fprintf(stdout, "autotune %s: path %d fused %d tpw %d  %.4f ms\n", key.c_str(), sig[0], sig[1], tp, ms);
+        if (ms < best.ms) best = TuneChoice{wm, fu, tp, ms};
+      }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  h.tuning = false;
+  if (rc) return rc;
+  h.windowMode = best.windowMode;
+  h.fusedOverlap = best.fused != 0;
+  h.ovTpw = best.tpw;
+  std::lock_guard<std::mutex> lk(g_tuneMu);
+  g_tune[key] = best;
+  tune_save_locked();
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
 // Queues one full batch on the handle's stream.  On return the coefficients and
 // fields are (or will be, once the stream drains) in device memory.
 void publish_batch(Handle& h, const Derived& d, const void* dframes, bool u16);
@@ -652,6 +794,9 @@ int enqueue_batch(Handle& h, int step = 0) {
   }
   if (c.polLockDefocus) c.defocus[1] = c.defocus[0];
   if (c.polLockWaist) c.waist[1] = c.waist[0];
+  if (h.autoTune && step == 0 && !h.tuning && !h.capturing) {
+    if (int et = autotune(h)) return et;
+  }
 
   Derived d;
   int err;
@@ -700,7 +845,7 @@ int enqueue_batch(Handle& h, int step = 0) {
   }
   if (const int et = ensure_row_twiddles(h, g.W)) return et;
 
-  const bool timing = h.stageTiming;
+  const bool timing = h.stageTiming && !h.capturing;
   hipEvent_t* ev = nullptr;
   if (timing) {
     if (h.evUsed == h.evPool.size()) {
@@ -734,6 +879,7 @@ int enqueue_batch(Handle& h, int step = 0) {
     DH_TRY(hipStreamSynchronize(h.stream), "tables sync");   // the host mask vector goes out of scope
     h.tabKey = d;
     h.tablesValid = true;
+    ++h.tabVersion;
   }
   if (timing) DH_TRY(hipEventRecord(ev[1], h.stream), "event");
   if (step == 1) {
@@ -811,7 +957,7 @@ int enqueue_batch(Handle& h, int step = 0) {
   }
   if (g.G > 0 && !h.lastFusedOverlap) {
     DH_TRY(dh::launch_overlap_mfma(gF, h.fields.p, h.hgx.p, h.hgy.p, h.coefs.p,
-                                   c.precision == DIGHOLO_PRECISION_SPLIT16, h.stream), "overlap");
+                                   c.precision == DIGHOLO_PRECISION_SPLIT16, h.stream, h.ovTpw), "overlap");
   }
   if (timing) DH_TRY(hipEventRecord(ev[5], h.stream), "event");
 
@@ -2343,8 +2489,157 @@ int autoalign_gradient_search(Handle& h, int maxIter, double mdlWeight, float& g
   return DIGHOLO_ERROR_SUCCESS;
 }
 
+// ---------------------------------------------------------------------------
+// Several handles' batches in one graph launch (no counterpart in the
+// reference).  Each handle's launch sequence is captured on its own stream as
+// a branch forked from and joined to the first handle's stream, so batches of
+// different shapes run concurrently; the executable graph is kept per handle
+// list and replayed while every handle's launch signature (window geometry,
+// frames, precision, chosen kernels, tables) is unchanged.
+// ---------------------------------------------------------------------------
+struct MultiGraph {
+  std::vector<Handle*> hs;
+  std::vector<std::string> sigs;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t stream = nullptr;
+};
+std::map<std::vector<int>, MultiGraph> g_multi;
+std::mutex g_multiMu;
+
+void multi_release(MultiGraph& m) {
+  if (m.exec) (void)hipGraphExecDestroy(m.exec);
+  if (m.graph) (void)hipGraphDestroy(m.graph);
+  m.exec = nullptr;
+  m.graph = nullptr;
+}
+
+// launch signature of the handle's next batch, from host state only
+bool launch_signature(Handle& h, std::string& sig) {
+  Config c = h.cfg;
+  if (c.batchCount < 1) c.batchCount = 1;
+  if (c.polLockTilt) {
+    c.tiltX[1] = c.tiltX[0];
+    c.tiltY[1] = c.tiltY[0];
+  }
+  if (c.polLockDefocus) c.defocus[1] = c.defocus[0];
+  if (c.polLockWaist) c.waist[1] = c.waist[0];
+  Derived d;
+  int err;
+  if (!derive(c, c.batchCount, d, err)) return false;
+  std::string s((const char*)&d.g, sizeof(d.g));
+  s.append((const char*)d.tab.pp, sizeof(d.tab.pp));
+  s.append((const char*)d.mask.data(), d.mask.size());
+  char buf[256];
+  std::snprintf(buf, sizeof(buf), "|%p|%d|%d|%d|%d|%d|%d|%d|%u|%u|%d|%d|%p", h.frames, h.frameU16, h.transpose,
+                c.precision, h.windowMode, h.fusedOverlap ? 1 : 0, h.ovTpw, c.avgCount * 8 + c.avgMode,
+                c.calEnabled ? c.calVersion : 0u, h.tabVersion, h.autoTune, c.calEnabled, (void*)h.stream);
+  sig = s + buf;
+  return true;
+}
+
+int process_multi(const std::vector<Handle*>& hs, const std::vector<int>& ids) {
+  for (Handle* h : hs) {
+    if (!h->frames) return DIGHOLO_ERROR_NULLPOINTER;
+    if (!is_device_pointer(h->frames)) {
+      fprintf(stderr, "digHolo: ProcessBatchMultiDevice needs device-resident frames\n");
+      return DIGHOLO_ERROR_INVALIDARGUMENT;
+    }
+    if (int e = ensure_device(*h)) return e;
+  }
+  std::lock_guard<std::mutex> lk(g_multiMu);
+  MultiGraph& m = g_multi[ids];
+  std::vector<std::string> sigs(hs.size());
+  bool same = m.exec != nullptr && m.stream == hs[0]->stream && m.sigs.size() == hs.size();
+  for (size_t i = 0; i < hs.size(); ++i) {
+    if (!launch_signature(*hs[i], sigs[i])) return DIGHOLO_ERROR_INVALIDDIMENSION;
+    if (same && sigs[i] != m.sigs[i]) same = false;
+  }
+  Handle& h0 = *hs[0];
+  if (!same) {
+    multi_release(m);
+    // one eager batch per handle: tables, twiddles, buffers and tuning settle
+    // outside the capture, then the launch sequence is captured
+    for (Handle* h : hs)
+      if (int e = enqueue_batch(*h)) return e;
+    for (Handle* h : hs) {
+      hipError_t e = hipStreamSynchronize(h->stream);
+      if (e != hipSuccess) {
+        log_err(h, "synchronize", e);
+        return DIGHOLO_ERROR_ERROR;
+      }
+    }
+    for (size_t i = 0; i < hs.size(); ++i)
+      if (!launch_signature(*hs[i], sigs[i])) return DIGHOLO_ERROR_INVALIDDIMENSION;
+    hipStream_t s0 = h0.stream;
+    std::vector<hipEvent_t> evs(hs.size() + 1, nullptr);
+    for (auto& e : evs)
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return DIGHOLO_ERROR_ERROR;
+    auto cleanup = [&]() {
+      for (auto& e : evs)
+        if (e) (void)hipEventDestroy(e);
+      for (Handle* h : hs) h->capturing = false;
+    };
+    int rc = DIGHOLO_ERROR_SUCCESS;
+    if (hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      cleanup();
+      return DIGHOLO_ERROR_ERROR;
+    }
+    (void)hipEventRecord(evs[0], s0);
+    for (size_t i = 1; i < hs.size(); ++i)
+      if (hs[i]->stream != s0) (void)hipStreamWaitEvent(hs[i]->stream, evs[0], 0);
+    for (size_t i = 0; i < hs.size() && !rc; ++i) {
+      hs[i]->capturing = true;
+      rc = enqueue_batch(*hs[i]);
+      hs[i]->capturing = false;
+    }
+    for (size_t i = 1; i < hs.size(); ++i)
+      if (hs[i]->stream != s0) {
+        (void)hipEventRecord(evs[i], hs[i]->stream);
+        (void)hipStreamWaitEvent(s0, evs[i], 0);
+      }
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s0, &graph);
+    cleanup();
+    if (rc || ec != hipSuccess || !graph) {
+      if (graph) (void)hipGraphDestroy(graph);
+      fprintf(stderr, "digHolo: graph capture failed (%s)\n", hipGetErrorString(ec));
+      return rc ? rc : DIGHOLO_ERROR_ERROR;
+    }
+    hipGraphExec_t exec = nullptr;
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return DIGHOLO_ERROR_ERROR;
+    }
+    m.graph = graph;
+    m.exec = exec;
+    m.stream = s0;
+    m.sigs = sigs;
+    m.hs = hs;
+  }
+  if (hipGraphLaunch(m.exec, h0.stream) != hipSuccess) return DIGHOLO_ERROR_ERROR;
+  for (Handle* h : hs) {   // every handle's outputs are now those of the replay on h0's stream
+    h->coefsHostValid = h->fieldsHostValid = false;
+    h->analysisValid[0] = h->analysisValid[1] = false;
+    h->f16Valid = false;
+  }
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
 void destroy_handle(Handle* h) {
   if (!h) return;
+  {
+    std::lock_guard<std::mutex> lk(g_multiMu);
+    for (auto it = g_multi.begin(); it != g_multi.end();) {
+      const bool uses = std::find(it->second.hs.begin(), it->second.hs.end(), h) != it->second.hs.end();
+      if (uses) {
+        multi_release(it->second);
+        it = g_multi.erase(it);
+      } else {
+        ++it;
+      }
+    }
+  }
   if (h->devInit) {
     (void)hipStreamSynchronize(h->stream);
     h->staging.release();
@@ -2974,6 +3269,34 @@ EXT_C int digHoloConfigSetOverlapPrecision(int handleIdx, int precision) {
 EXT_C int digHoloConfigGetOverlapPrecision(int handleIdx) {
   H_OR(0);
   return CFG.precision;
+}
+
+EXT_C int digHoloConfigSetAutoTune(int handleIdx, int mode) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (mode < 0 || mode > 2) return DIGHOLO_ERROR_INVALIDARGUMENT;
+  h->autoTune = mode;
+  if (!mode) {   // back to the default kernels
+    h->windowMode = 3;
+    h->fusedOverlap = true;
+    h->ovTpw = 0;
+  }
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C int digHoloConfigGetAutoTune(int handleIdx) {
+  H_OR(0);
+  return h->autoTune;
+}
+EXT_C int digHoloProcessBatchMultiDevice(int count, const int* handleIdx) {
+  if (count < 1 || !handleIdx) return DIGHOLO_ERROR_INVALIDARGUMENT;
+  std::vector<Handle*> hs;
+  std::vector<int> ids(handleIdx, handleIdx + count);
+  for (int i = 0; i < count; ++i) {
+    Handle* h = get(ids[i]);
+    if (!h) return DIGHOLO_ERROR_INVALIDHANDLE;
+    if (std::find(hs.begin(), hs.end(), h) != hs.end()) return DIGHOLO_ERROR_INVALIDARGUMENT;
+    hs.push_back(h);
+  }
+  return process_multi(hs, ids);
 }
 
 EXT_C complex64* digHoloProcessBatchDevice(int handleIdx, int* batchCount, int* modeCount, int* polCount) {

@@ -196,7 +196,10 @@ hipError_t launch_grad_kspace(const Geometry& g, const float2* x1, const float2*
 hipError_t launch_grad_reduce(int B, int P, const double* partial, double* out, hipStream_t s);
 
 // Separable HG overlap on MFMA: split = 0 plain FP32, 1 split-FP16 (hi+lo).
+// tpw: stage-2 tiles per wave of the launch (0: the fewest that cover the
+// triangle, overlap_min_tiles_per_wave; larger values give each wave more).
 hipError_t launch_overlap_mfma(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
-                               float2* coefs, int split, hipStream_t s);
+                               float2* coefs, int split, hipStream_t s, int tpw = 0);
+int overlap_min_tiles_per_wave(const Geometry& g);
 
 }  // namespace dh

@@ -289,8 +289,8 @@ static hipError_t ov_launch(const Geometry& g, const float2* fields, const float
 
 template <bool SPLIT>
 static hipError_t ov_dispatch(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
-                              float2* coefs, hipStream_t s) {
-  const int tpw = tiles_per_wave(g);
+                              float2* coefs, int tpwWanted, hipStream_t s) {
+  const int tpw = std::max(tiles_per_wave(g), tpwWanted);
   if (tpw <= 2) return ov_launch<SPLIT, 2>(g, fields, hgx, hgy, coefs, s);
   if (tpw <= 4) return ov_launch<SPLIT, 4>(g, fields, hgx, hgy, coefs, s);
   if (tpw <= 8) return ov_launch<SPLIT, 8>(g, fields, hgx, hgy, coefs, s);
@@ -299,12 +299,14 @@ static hipError_t ov_dispatch(const Geometry& g, const float2* fields, const flo
   return hipErrorInvalidValue;
 }
 
+int overlap_min_tiles_per_wave(const Geometry& g) { return tiles_per_wave(g); }
+
 hipError_t launch_overlap_mfma(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
-                               float2* coefs, int split, hipStream_t s) {
+                               float2* coefs, int split, hipStream_t s, int tpw) {
   if (g.w % 16 || g.h % kChunk || g.G <= 0) return hipErrorInvalidValue;
   if (overlap_lds(g, split != 0) > 160 * 1024) return hipErrorInvalidValue;
-  return split ? ov_dispatch<true>(g, fields, hgx, hgy, coefs, s)
-               : ov_dispatch<false>(g, fields, hgx, hgy, coefs, s);
+  return split ? ov_dispatch<true>(g, fields, hgx, hgy, coefs, tpw, s)
+               : ov_dispatch<false>(g, fields, hgx, hgy, coefs, tpw, s);
 }
 
 }  // namespace dh

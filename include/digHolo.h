@@ -274,6 +274,26 @@ EXT_C int digHoloGetWindowPath(int handleIdx);
 EXT_C int digHoloConfigSetStageTiming(int handleIdx, int enabled);
 EXT_C int digHoloGetStageTimes(int handleIdx, float* ms);
 
+/* Autotuning of the launch variants per batch shape (frame and window sizes,
+ * pols, groups, precision, input type, batch size rounded up to a power of
+ * two): window path, split overlap fused or separate, overlap tiles per wave,
+ * each timed on the batch with HIP events.  mode 0 (default): default kernels;
+ * 1: tune each new shape on first use and remember the fastest (in memory and
+ * in the cache file $DIGHOLO_TUNE_CACHE, else $HOME/.cache/digholo_amd_tune.txt);
+ * 2: use remembered choices only. */
+EXT_C int digHoloConfigSetAutoTune(int handleIdx, int mode);
+EXT_C int digHoloConfigGetAutoTune(int handleIdx);
+
+/* Processes the current batch of each listed handle (distinct handles, device-
+ * resident frames) in one hipGraph launch on the first handle's stream: each
+ * handle's kernels form a branch of the graph, so batches of different shapes
+ * run concurrently.  The graph is captured on the first call for a handle list
+ * and replayed while every handle's geometry, frames, precision and kernels are
+ * unchanged (otherwise re-captured).  Outputs as digHoloProcessBatchDevice /
+ * digHoloGetFieldsDevice of each handle; synchronise the first handle's stream
+ * (digHoloSynchronize) before reading them. */
+EXT_C int digHoloProcessBatchMultiDevice(int count, const int* handleIdx);
+
 /* Alignment-loss gradient.  Processes the batch at the current configuration
  * and returns loss = -(1/B) sum |c|^2 + mdlWeight (ln s_max^2 - ln s_min^2)
  * (minus the linear insertion loss without the SVD, plus a mode-dependent-loss

@@ -1,0 +1,158 @@
+"""Several handles' batches in one graph launch (digHoloProcessBatchMultiDevice)
+and the launch autotuner (digHoloConfigSetAutoTune); no counterpart in the
+reference.
+
+Three handle configurations of the uneven-batch workload (320x256 / 128 window
+/ 10 groups, 640x512 / 192 / 30, 1024x1024 / 256 / 45; dual pol, uint16 frames
+in HBM), at small batch counts:
+* every replay of the captured graph reproduces, bit for bit, the coefficients
+  and fields each handle computes on its own (outputs cleared between
+  replays, eager torch work and synchronisations in between);
+* a configuration change re-captures, and the replay again matches;
+* with autotuning on, the chosen kernels reproduce the default ones to 1e-5
+  relative L2, and the choice lands in the cache file.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from digholo_amd import api, synth
+from oracle import digholo_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [  # frame W, H, window, groups, batch
+    (320, 256, 128, 10, 12),
+    (640, 512, 192, 30, 6),
+    (1024, 1024, 256, 45, 3),
+]
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("needs a GPU")
+    return api.load()
+
+
+class DevArray:
+    """torch view of library-owned device memory (float32 pairs)."""
+
+    def __init__(self, ptr, n):
+        self.__cuda_array_interface__ = {"shape": (n, 2), "typestr": "<f4", "data": (ptr, False), "version": 3}
+
+
+def frames_for(W, H, G, B, seed):
+    import torch
+    spec = synth.HologramSpec(frame_width=W, frame_height=H, pol_count=2, waist=200e-6, group_count=min(G, 10),
+                              beam_centre_x=[-W * 5e-6, W * 5e-6])
+    f, _, _, _ = synth.make_batch(spec, B, seed=seed)
+    return torch.from_numpy(f.astype(np.int16)).cuda()
+
+
+def make_handle(lib, W, H, win, G, B, frames):
+    h = lib.digHoloCreate()
+    api.configure(lib, h, frame_width=W, frame_height=H, pol_count=2, fft_window=(win, win), waist=200e-6,
+                  group_count=G, beam_centre_x=(-W * 5e-6, W * 5e-6))
+    assert lib.digHoloSetBatchUint16(h, B, C.c_void_p(frames.data_ptr()), 0) == 0
+    return h
+
+
+def outputs(lib, h):
+    import torch
+    b, p, w, hh = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    fptr = lib.digHoloGetFieldsDevice(h, C.byref(b), C.byref(p), C.byref(w), C.byref(hh))
+    nf = b.value * p.value * w.value * hh.value
+    fields = torch.as_tensor(DevArray(fptr, nf), device="cuda")
+    return fields, (b.value, p.value, hh.value, w.value)
+
+
+def individual(lib, h):
+    import torch
+    b, m, p = C.c_int(), C.c_int(), C.c_int()
+    cptr = lib.digHoloProcessBatchDevice(h, C.byref(b), C.byref(m), C.byref(p))
+    assert cptr
+    assert lib.digHoloSynchronize(h) == 0
+    n = b.value * p.value * m.value
+    coefs = torch.as_tensor(DevArray(cptr, n), device="cuda")
+    fields, shape = outputs(lib, h)
+    return coefs, fields
+
+
+def test_multi_graph_replays_match_individual_batches(lib):
+    import torch
+    hs, frames, ref, views = [], [], [], []
+    try:
+        for i, (W, H, win, G, B) in enumerate(SHAPES):
+            fr = frames_for(W, H, G, B, seed=i)
+            frames.append(fr)
+            h = make_handle(lib, W, H, win, G, B, fr)
+            hs.append(h)
+            c, f = individual(lib, h)
+            views.append((c, f))
+            ref.append((c.clone(), f.clone()))
+            print(f"handle {h}: {W}x{H} window {win} G {G} B {B}: coefs {tuple(c.shape)} fields {tuple(f.shape)}",
+                  flush=True)
+        ids = (C.c_int * len(hs))(*hs)
+        for rep in range(4):
+            for c, f in views:          # clear the outputs so a replay that did nothing fails
+                c.zero_()
+                f.zero_()
+            torch.cuda.synchronize()
+            x = torch.randn(1 << 20, device="cuda").square().sum()   # eager work between replays
+            assert lib.digHoloProcessBatchMultiDevice(len(hs), ids) == 0
+            assert lib.digHoloSynchronize(hs[0]) == 0
+            for (c, f), (cr, fr_) in zip(views, ref):
+                assert torch.equal(c, cr), f"replay {rep}: coefficients differ"
+                assert torch.equal(f, fr_), f"replay {rep}: fields differ"
+            print(f"replay {rep} matches ({x.item():.1f})", flush=True)
+        # a changed configuration re-captures
+        lib.digHoloConfigSetTilt(hs[1], 0, 0, 1.48)
+        c1, f1 = individual(lib, hs[1])
+        want = (c1.clone(), f1.clone())
+        for c, f in views:
+            c.zero_()
+            f.zero_()
+        torch.cuda.synchronize()
+        assert lib.digHoloProcessBatchMultiDevice(len(hs), ids) == 0
+        assert lib.digHoloSynchronize(hs[0]) == 0
+        assert torch.equal(views[1][0], want[0]) and torch.equal(views[1][1], want[1])
+        assert torch.equal(views[0][0], ref[0][0]) and torch.equal(views[2][1], ref[2][1])
+    finally:
+        for h in hs:
+            lib.digHoloDestroy(h)
+
+
+def test_autotune_choice_and_cache(lib, tmp_path, monkeypatch):
+    import torch
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("DIGHOLO_TUNE_CACHE", str(cache))
+    W, H, win, G, B = SHAPES[1]
+    fr = frames_for(W, H, G, 8, seed=5)
+    h = make_handle(lib, W, H, win, G, 8, fr)
+    try:
+        for prec in (api.DIGHOLO_PRECISION_FP32, api.DIGHOLO_PRECISION_SPLIT16):
+            lib.digHoloConfigSetOverlapPrecision(h, prec)
+            lib.digHoloConfigSetAutoTune(h, 0)
+            c0, f0 = individual(lib, h)
+            c0, f0 = c0.clone(), f0.clone()
+            assert lib.digHoloConfigSetAutoTune(h, 1) == 0
+            c1, f1 = individual(lib, h)
+            path = lib.digHoloGetWindowPath(h)
+            print(f"precision {prec}: tuned window path {path}", flush=True)
+            ec = O.relative_l2(c1.cpu().numpy(), c0.cpu().numpy())
+            ef = O.relative_l2(f1.cpu().numpy(), f0.cpu().numpy())
+            assert ec <= 1e-5 and ef <= 1e-5, (ec, ef)
+        text = cache.read_text()
+        print(text, flush=True)
+        lines = [ln for ln in text.splitlines() if ln.strip()]
+        assert len(lines) == 2 and all(len(ln.split()) == 5 for ln in lines)
+        # mode 2 reuses the remembered choice without timing anything new
+        assert lib.digHoloConfigSetAutoTune(h, 2) == 0
+        individual(lib, h)
+        assert cache.read_text() == text
+    finally:
+        lib.digHoloDestroy(h)
