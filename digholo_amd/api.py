@@ -115,6 +115,9 @@ SIGNATURES: Dict[str, Tuple[object, List[object]]] = {
     "digHoloConfigSetBasisWaist": (c_int, [c_int, c_int, c_float]),
     "digHoloConfigGetBasisWaist": (c_float, [c_int, c_int]),
     "digHoloConfigSetBasisTypeHG": (c_int, [c_int]),
+    "digHoloConfigSetBasisTypeLG": (c_int, [c_int]),
+    "digHoloConfigSetBasisTypeCustom": (c_int, [c_int, c_int, c_int, c_void_p]),
+    "digHoloBasisGetFields": (c_void_p, [c_int, P_int, P_int, C.POINTER(P_float), C.POINTER(P_float), P_int, P_int]),
     "digHoloConfigSetBasisType": (c_int, [c_int, c_int]),
     "digHoloConfigGetBasisType": (c_int, [c_int]),
     "digHoloProcessBatch": (c_void_p, [c_int, P_int, P_int, P_int]),

@@ -64,6 +64,10 @@ struct Config {
   std::vector<float2> cal;
   int calP = 0, calB = 0, calEnabled = 0;
   unsigned calVersion = 0;
+  // custom basis (digHolo.cpp:4750-4765): column-major basisOut x basisIn
+  std::vector<float2> basisFull;
+  int basisIn = 0, basisOut = 0;
+  unsigned basisVersion = 0;
 };
 
 template <typename T>
@@ -217,6 +221,18 @@ struct Handle {
   DevBuf<float2> gExtra;
   Derived gradHeld;
   bool gradHeldValid = false;
+
+  // basis transform of the HG coefficients (LG / custom): the coefficients the
+  // caller and the power meter see are coefsT with Mout modes per pol when
+  // outT, else the HG ones
+  DevBuf<float2> coefsT, lgTab, customT, basisW, basisFieldsDev;
+  PinnedBuf<float2> hBasisFields;
+  int lgTabG = -1;
+  unsigned customTVersion = ~0u;
+  bool outT = false;
+  int Mout = 0;
+  const float2* outCoefs() const { return outT ? coefsT.p : coefs.p; }
+  float2* outCoefs() { return outT ? coefsT.p : coefs.p; }
 
   // autotuning: 0 off, 1 tune each new shape once, 2 remembered choices only;
   // ovTpw: stage-2 tiles per wave of the separate overlap kernel (0: fewest)
@@ -773,6 +789,85 @@ int autotune(Handle& h) {
   return DIGHOLO_ERROR_SUCCESS;
 }
 
+// ---------------------------------------------------------------------------
+// Basis transforms (digHolo.cpp:9469-9605).  LG: per HG group g the unitary
+// U_g with LG[k] = sum_c U_g[k][c] HG[c] (digHoloLGtoHGcoef :9819-9906): LG
+// mode k of the group has the index pair of HG mode k (m = g-k, n = k),
+// l = m - n, p = (g - |l|)/2, (a, b) = (p, p - l) for l < 0 and (l + p, p)
+// otherwise; for HG (hm = g - c, hn = c)
+//   U_g[k][c] = i^(g + hm) (-1)^[g odd, l > 0] 2^(-g/2) sqrt(hm! hn! / (a! b!))
+//               sum_t (-1)^t C(b, t) C(a, hm - t),
+// the sum in exact 128-bit integers (every term is at most C(g, hm)).  Stored
+// row-major per block, blocks one after another.  Custom: the caller's
+// column-major matrix, the min(M, in) x min(M, out) block used.
+// ---------------------------------------------------------------------------
+std::vector<float2> lg_table(int G) {
+  using i128 = __int128;
+  std::vector<std::vector<i128>> C(G + 1, std::vector<i128>(G + 1, 0));
+  for (int n = 0; n <= G; ++n) {
+    C[n][0] = 1;
+    for (int k = 1; k <= n; ++k) C[n][k] = C[n - 1][k - 1] + (k <= n - 1 ? C[n - 1][k] : 0);
+  }
+  std::vector<long double> lf(G + 1, 0.0L);
+  for (int n = 1; n <= G; ++n) lf[n] = lf[n - 1] + std::log((long double)n);
+  std::vector<float2> t;
+  for (int g = 0; g < G; ++g)
+    for (int k = 0; k <= g; ++k) {
+      const int l = (g - k) - k;
+      const int p = (g - std::abs(l)) / 2;
+      const int a = l < 0 ? p : l + p, b = l < 0 ? -l + p : p;
+      for (int c = 0; c <= g; ++c) {
+        const int hm = g - c, hn = c;
+        i128 sum = 0;
+        for (int tt = 0; tt <= std::min(hm, b); ++tt)
+          if (hm - tt <= a) sum += ((tt & 1) ? -1 : 1) * C[b][tt] * C[a][hm - tt];
+        long double v = (long double)sum * std::exp(0.5L * (lf[hm] + lf[hn] - lf[a] - lf[b]) - 0.5L * g * std::log(2.0L));
+        if ((g & 1) && l > 0) v = -v;
+        const int ph = (g + hm) & 3;   // i^ph
+        const float re = ph == 0 ? (float)v : ph == 2 ? (float)-v : 0.f;
+        const float im = ph == 1 ? (float)v : ph == 3 ? (float)-v : 0.f;
+        t.push_back(make_float2(re, im));
+      }
+    }
+  return t;
+}
+
+// After the overlap: the coefficients in the configured basis (an unsupported
+// type, or a custom basis without a matrix, falls back to HG, :9474-9481).
+int apply_basis(Handle& h) {
+  const Config& c = h.cfg;
+  const dh::Geometry& g = h.cur.g;
+  h.outT = false;
+  h.Mout = g.M;
+  if (g.G <= 0 || g.M <= 0) return DIGHOLO_ERROR_SUCCESS;
+  const size_t rows = (size_t)g.B * g.P;
+  if (c.basisType == DIGHOLO_BASISTYPE_LG) {
+    if (h.lgTabG != g.G) {
+      const std::vector<float2> t = lg_table(g.G);
+      DH_TRY(h.lgTab.ensure(t.size()), "alloc");
+      DH_TRY(hipMemcpy(h.lgTab.p, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice), "LG table");
+      h.lgTabG = g.G;
+    }
+    DH_TRY(h.coefsT.ensure(rows * g.M), "alloc");
+    DH_TRY(dh::launch_lg_apply(h.coefs.p, h.coefsT.p, rows, g.M, h.lgTab.p, h.stream), "LG transform");
+    h.outT = true;
+  } else if (c.basisType == DIGHOLO_BASISTYPE_CUSTOM && !c.basisFull.empty() && c.basisIn > 0 && c.basisOut > 0) {
+    if (h.customTVersion != c.basisVersion) {
+      DH_TRY(h.customT.ensure(c.basisFull.size()), "alloc");
+      DH_TRY(hipMemcpy(h.customT.p, c.basisFull.data(), c.basisFull.size() * sizeof(float2), hipMemcpyHostToDevice),
+             "custom basis");
+      h.customTVersion = c.basisVersion;
+    }
+    const int nIn = std::min(g.M, c.basisIn), nOut = std::min(g.M, c.basisOut);
+    DH_TRY(h.coefsT.ensure(rows * nOut), "alloc");
+    DH_TRY(dh::launch_custom_apply(h.coefs.p, g.M, (int)rows, nIn, h.customT.p, c.basisOut, nOut, h.coefsT.p,
+                                   h.stream), "custom transform");
+    h.outT = true;
+    h.Mout = nOut;
+  }
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
 // Queues one full batch on the handle's stream.  On return the coefficients and
 // fields are (or will be, once the stream drains) in device memory.
 void publish_batch(Handle& h, const Derived& d, const void* dframes, bool u16);
@@ -783,10 +878,6 @@ int enqueue_batch(Handle& h, int step = 0) {
   Config& c = h.cfg;
   if (!h.frames) return DIGHOLO_ERROR_NULLPOINTER;
   if (c.batchCount < 1) c.batchCount = 1;
-  if (c.basisType != DIGHOLO_BASISTYPE_HG && c.maxMG > 0) {
-    fprintf(stderr, "digHolo: only the HG basis is implemented in this build\n");
-    return DIGHOLO_ERROR_INVALIDARGUMENT;
-  }
   // pol locks write pol 0's values into pol 1 (digHolo.cpp:10296-10318, :10527-10531)
   if (c.polLockTilt) {
     c.tiltX[1] = c.tiltX[0];
@@ -962,6 +1053,7 @@ int enqueue_batch(Handle& h, int step = 0) {
   if (timing) DH_TRY(hipEventRecord(ev[5], h.stream), "event");
 
   publish_batch(h, d, dframes, h.frameU16);
+  if (int eb = apply_basis(h)) return eb;
   h.coefsDevValid = true;
   return DIGHOLO_ERROR_SUCCESS;
 }
@@ -1049,6 +1141,7 @@ int step_extract_coefs(Handle& h) {
   DH_TRY(h.coefs.ensure((size_t)g.B * g.P * g.M), "alloc");
   DH_TRY(dh::launch_overlap_mfma(g, h.fields.p, h.hgx.p, h.hgy.p, h.coefs.p,
                                  h.cfg.precision == DIGHOLO_PRECISION_SPLIT16, h.stream), "overlap");
+  if (int eb = apply_basis(h)) return eb;
   h.coefsDevValid = true;
   h.coefsHostValid = false;
   return DIGHOLO_ERROR_SUCCESS;
@@ -1079,9 +1172,9 @@ int download_coefs(Handle& h) {
   if (!h.coefsDevValid) return DIGHOLO_ERROR_NULLPOINTER;
   if (h.coefsHostValid) return DIGHOLO_ERROR_SUCCESS;
   const dh::Geometry& g = h.cur.g;
-  const size_t n = (size_t)g.B * g.P * g.M;
+  const size_t n = (size_t)g.B * g.P * h.Mout;
   DH_TRY(h.hCoefs.ensure(std::max<size_t>(n, 1)), "pinned alloc");
-  if (n) DH_TRY(hipMemcpyAsync(h.hCoefs.p, h.coefs.p, n * sizeof(float2), hipMemcpyDeviceToHost, h.stream), "D2H");
+  if (n) DH_TRY(hipMemcpyAsync(h.hCoefs.p, h.outCoefs(), n * sizeof(float2), hipMemcpyDeviceToHost, h.stream), "D2H");
   DH_TRY(hipStreamSynchronize(h.stream), "sync");
   h.coefsHostValid = true;
   return DIGHOLO_ERROR_SUCCESS;
@@ -1256,7 +1349,7 @@ struct PowerMeterOut {
 int power_meter(Handle& h, int polIdx, bool independent, bool doIL, bool doMDL, bool doXtalk, bool mulConj,
                 PowerMeterOut& o) {
   const dh::Geometry& g = h.cur.g;
-  const int B = g.B, P = g.P, Mpol = g.M;
+  const int B = g.B, P = g.P, Mpol = h.Mout;                // the coefficients of the configured basis
   const float FMAX = std::numeric_limits<float>::max();
   if (Mpol == 0) return DIGHOLO_ERROR_INVALIDARGUMENT;      // needs the Fourier-plane power (not built)
   if (B > 1) {
@@ -1265,14 +1358,14 @@ int power_meter(Handle& h, int polIdx, bool independent, bool doIL, bool doMDL, 
     int M = (P == 2 && !indep) ? 2 * Mpol : Mpol;          // svdM
     const int N = B;                                       // svdN
     int rowStride = P * Mpol;
-    const float2* A = h.coefs.p;
+    const float2* A = h.outCoefs();
     if (polIdx >= 0 && P == 2) {
       S = 1;
       if (indep) {
-        A = h.coefs.p + (size_t)polIdx * Mpol;
+        A = h.outCoefs() + (size_t)polIdx * Mpol;
       } else {
         DH_TRY(h.coefsMix.ensure((size_t)B * 2 * Mpol), "alloc");
-        DH_TRY(hipMemcpy2DAsync(h.coefsMix.p, 2 * Mpol * sizeof(float2), h.coefs.p + (size_t)polIdx * Mpol,
+        DH_TRY(hipMemcpy2DAsync(h.coefsMix.p, 2 * Mpol * sizeof(float2), h.outCoefs() + (size_t)polIdx * Mpol,
                                 rowStride * sizeof(float2), Mpol * sizeof(float2), B, hipMemcpyDeviceToDevice,
                                 h.stream), "mix");
         DH_TRY(hipMemcpy2DAsync(h.coefsMix.p + Mpol, 2 * Mpol * sizeof(float2),
@@ -1385,7 +1478,8 @@ int power_meter(Handle& h, int polIdx, bool independent, bool doIL, bool doMDL, 
   } else {
     // one frame: total power and the power of mode 0 (digHolo.cpp:11093-11124)
     std::vector<float2> c((size_t)P * Mpol);
-    DH_TRY(hipMemcpyAsync(c.data(), h.coefs.p, c.size() * sizeof(float2), hipMemcpyDeviceToHost, h.stream), "D2H");
+    DH_TRY(hipMemcpyAsync(c.data(), h.outCoefs(), c.size() * sizeof(float2), hipMemcpyDeviceToHost, h.stream),
+           "D2H");
     DH_TRY(hipStreamSynchronize(h.stream), "sync");
     double tot = 0, diag = 0;
     for (int p = 0; p < P; ++p)
@@ -1560,9 +1654,10 @@ int autoalign_tweak(Handle& h, float& goalDb, const float* snapSteps) {
   // device copies of coefficients (digHolo.cpp:12632-12637, :12692-12699)
   auto backup_all = [&]() -> int {
     const dh::Geometry& g = h.cur.g;
-    const size_t n = (size_t)g.B * g.P * g.M;
+    const size_t n = (size_t)g.B * g.P * h.Mout;
     DH_TRY(h.coefsBackup.ensure(std::max<size_t>(n, 1)), "alloc");
-    DH_TRY(hipMemcpyAsync(h.coefsBackup.p, h.coefs.p, n * sizeof(float2), hipMemcpyDeviceToDevice, h.stream), "copy");
+    DH_TRY(hipMemcpyAsync(h.coefsBackup.p, h.outCoefs(), n * sizeof(float2), hipMemcpyDeviceToDevice, h.stream),
+           "copy");
     return 0;
   };
   auto process = [&]() -> int {
@@ -1577,9 +1672,9 @@ int autoalign_tweak(Handle& h, float& goalDb, const float* snapSteps) {
   };
   auto backup_pol = [&](int p) -> int {
     const dh::Geometry& g = h.cur.g;
-    const size_t pitch = (size_t)g.P * g.M * sizeof(float2);
-    DH_TRY(hipMemcpy2DAsync(h.coefsBackup.p + (size_t)p * g.M, pitch, h.coefs.p + (size_t)p * g.M, pitch,
-                            (size_t)g.M * sizeof(float2), g.B, hipMemcpyDeviceToDevice, h.stream), "copy");
+    const size_t pitch = (size_t)g.P * h.Mout * sizeof(float2);
+    DH_TRY(hipMemcpy2DAsync(h.coefsBackup.p + (size_t)p * h.Mout, pitch, h.outCoefs() + (size_t)p * h.Mout, pitch,
+                            (size_t)h.Mout * sizeof(float2), g.B, hipMemcpyDeviceToDevice, h.stream), "copy");
     return 0;
   };
 
@@ -3115,10 +3210,30 @@ EXT_C int digHoloConfigSetBasisTypeHG(int handleIdx) {
   CFG.basisType = DIGHOLO_BASISTYPE_HG;
   return DIGHOLO_ERROR_SUCCESS;
 }
+// digHolo.cpp:14736-14742 stores any value and returns it; processing treats
+// an unsupported type as HG (:9474-9481)
 EXT_C int digHoloConfigSetBasisType(int handleIdx, int basisType) {
   H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
-  if (basisType < 0 || basisType > DIGHOLO_BASISTYPE_CUSTOM) return DIGHOLO_ERROR_INVALIDARGUMENT;
   CFG.basisType = basisType;
+  return basisType;
+}
+EXT_C int digHoloConfigSetBasisTypeLG(int handleIdx) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  CFG.basisType = DIGHOLO_BASISTYPE_LG;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+// digHolo.cpp:14774-14803, :4750-4765: the matrix is copied
+EXT_C int digHoloConfigSetBasisTypeCustom(int handleIdx, int modeCountIn, int modeCountOut, complex64* transform) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (!transform) return DIGHOLO_ERROR_NULLPOINTER;
+  if (modeCountIn <= 0 || modeCountOut <= 0) return DIGHOLO_ERROR_INVALIDDIMENSION;
+  CFG.basisType = DIGHOLO_BASISTYPE_CUSTOM;
+  const size_t n = (size_t)modeCountIn * modeCountOut;
+  CFG.basisFull.resize(n);
+  std::memcpy(CFG.basisFull.data(), transform, n * sizeof(float2));
+  CFG.basisIn = modeCountIn;
+  CFG.basisOut = modeCountOut;
+  ++CFG.basisVersion;
   return DIGHOLO_ERROR_SUCCESS;
 }
 EXT_C int digHoloConfigGetBasisType(int handleIdx) {
@@ -3147,8 +3262,8 @@ EXT_C complex64* digHoloProcessBatch(int handleIdx, int* batchCount, int* modeCo
   const dh::Geometry& g = h->cur.g;
   *batchCount = g.B;
   *polCount = g.P;
-  *modeCount = g.M;
-  return g.M > 0 ? reinterpret_cast<complex64*>(h->hCoefs.p) : nullptr;   // G=0: no decomposition
+  *modeCount = h->Mout;
+  return h->Mout > 0 ? reinterpret_cast<complex64*>(h->hCoefs.p) : nullptr;   // G=0: no decomposition
 }
 
 EXT_C complex64* digHoloBasisGetCoefs(int handleIdx, int* batchCount, int* modeCount, int* polCount) {
@@ -3162,8 +3277,78 @@ EXT_C complex64* digHoloBasisGetCoefs(int handleIdx, int* batchCount, int* modeC
   const dh::Geometry& g = h->cur.g;
   *batchCount = g.B;
   *polCount = g.P;
-  *modeCount = g.M;
+  *modeCount = h->Mout;
   return reinterpret_cast<complex64*>(h->hCoefs.p);
+}
+
+// Basis fields of the rows of W (modes x M HG weights) on the field plane g
+// into the pinned host buffer h.hBasisFields, [modes][pol][g.h][g.w].
+int synth_basis_fields(Handle& h, const dh::Geometry& g, const std::vector<float2>& W, int modes, int M) {
+  const size_t nOut = (size_t)modes * g.P * g.h * g.w;
+  DH_TRY(h.basisW.ensure(W.size()), "alloc");
+  DH_TRY(h.basisFieldsDev.ensure(nOut), "alloc");
+  DH_TRY(h.hBasisFields.ensure(nOut), "pinned alloc");
+  DH_TRY(hipMemcpyAsync(h.basisW.p, W.data(), W.size() * sizeof(float2), hipMemcpyHostToDevice, h.stream), "upload");
+  DH_TRY(dh::launch_basis_synth(g, h.basisW.p, modes, M, h.hgx.p, h.hgy.p, h.basisFieldsDev.p, h.stream),
+         "basis fields");
+  DH_TRY(hipMemcpyAsync(h.hBasisFields.p, h.basisFieldsDev.p, nOut * sizeof(float2), hipMemcpyDeviceToHost, h.stream),
+         "D2H");
+  DH_TRY(hipStreamSynchronize(h.stream), "sync");
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+// digHolo.cpp:5233-5296, :15378-15394.  The fields of the output basis modes
+// on each pol's field plane of the last batch, [modes][pol][height][width]:
+// HG modes, LG modes (rows conj(U_g), the inverse transform of the identity)
+// or the custom matrix's rows T(j, :) as given (unconjugated, as the reference
+// builds them), synthesised on the GPU from the FP64-derived HG tables.
+EXT_C complex64* digHoloBasisGetFields(int handleIdx, int* modeCount, int* polCount, float** x, float** y,
+                                       int* width, int* height) {
+  Handle* h = get(handleIdx);
+  auto fail = [&]() -> complex64* {
+    *modeCount = *polCount = *width = *height = 0;
+    *x = *y = nullptr;
+    return nullptr;
+  };
+  if (!h || !h->haveBatch || !h->tablesValid || h->cur.g.G <= 0) return fail();
+  const Config& c = h->cfg;
+  dh::Geometry g = h->cur.g;
+  g.w = g.fw;
+  g.h = g.fh;
+  const int M = g.M;
+  int Mo = M;
+  std::vector<float2> W;
+  if (c.basisType == DIGHOLO_BASISTYPE_LG) {
+    W.assign((size_t)M * M, make_float2(0.f, 0.f));
+    const std::vector<float2> t = lg_table(g.G);
+    size_t off = 0;
+    for (int gg = 0; gg < g.G; ++gg) {
+      const int g0 = gg * (gg + 1) / 2;
+      for (int k = 0; k <= gg; ++k)
+        for (int cc = 0; cc <= gg; ++cc) {
+          const float2 u = t[off + (size_t)k * (gg + 1) + cc];
+          W[(size_t)(g0 + k) * M + g0 + cc] = make_float2(u.x, -u.y);
+        }
+      off += (size_t)(gg + 1) * (gg + 1);
+    }
+  } else if (c.basisType == DIGHOLO_BASISTYPE_CUSTOM && !c.basisFull.empty() && c.basisIn > 0 && c.basisOut > 0) {
+    const int nIn = std::min(M, c.basisIn);
+    Mo = std::min(M, c.basisOut);
+    W.assign((size_t)Mo * M, make_float2(0.f, 0.f));
+    for (int j = 0; j < Mo; ++j)
+      for (int i = 0; i < nIn; ++i) W[(size_t)j * M + i] = c.basisFull[(size_t)i * c.basisOut + j];
+  } else {
+    W.assign((size_t)M * M, make_float2(0.f, 0.f));
+    for (int i = 0; i < M; ++i) W[(size_t)i * M + i] = make_float2(1.f, 0.f);
+  }
+  if (synth_basis_fields(*h, g, W, Mo, M)) return fail();
+  *modeCount = Mo;
+  *polCount = g.P;
+  *width = g.fw;
+  *height = g.fh;
+  *x = h->xAxis.data();
+  *y = h->yAxis.data();
+  return reinterpret_cast<complex64*>(h->hBasisFields.p);
 }
 
 EXT_C complex64* digHoloGetFields(int handleIdx, int* batchCount, int* polCount, float** x, float** y, int* width,
@@ -3310,8 +3495,8 @@ EXT_C complex64* digHoloProcessBatchDevice(int handleIdx, int* batchCount, int* 
   const dh::Geometry& g = h->cur.g;
   *batchCount = g.B;
   *polCount = g.P;
-  *modeCount = g.M;
-  return reinterpret_cast<complex64*>(h->coefs.p);
+  *modeCount = h->Mout;
+  return reinterpret_cast<complex64*>(h->outCoefs());
 }
 
 EXT_C complex64* digHoloGetFieldsDevice(int handleIdx, int* batchCount, int* polCount, int* width, int* height) {

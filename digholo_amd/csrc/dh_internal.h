@@ -177,6 +177,18 @@ size_t snap_work_elems(int Ny, int Nx);
 hipError_t launch_snap_tilt_search(const float* map, int Ny, int Nx, float px, float wx2, float wy2, const float* kx,
                                    const float* ky, float ad, double* work, int* outIdx, hipStream_t s);
 
+// Basis transforms and basis fields (dh_basis.hip).  LG: tab holds the
+// per-group blocks U_g row-major one after another; out[r][g0 + k] =
+// sum_c U_g[k][c] in[r][g0 + c] for rows r < rows.  Custom: out[r][j] =
+// sum_{i < nIn} in[r * ldIn + i] T[i * ldT + j], j < nOut.  Basis fields:
+// out[mode][p][y][x] = sum_i W[mode * ldW + i] HGx_{m_i}(x) HGy_{n_i}(y) on the
+// g.w x g.h plane.
+hipError_t launch_lg_apply(const float2* in, float2* out, size_t rows, int M, const float2* tab, hipStream_t s);
+hipError_t launch_custom_apply(const float2* in, int ldIn, int rows, int nIn, const float2* T, int ldT, int nOut,
+                               float2* out, hipStream_t s);
+hipError_t launch_basis_synth(const Geometry& g, const float2* W, int modes, int ldW, const float* hgx,
+                              const float* hgy, float2* out, hipStream_t s);
+
 // Alignment-loss gradient (dh_grad.hip).  dhgx/dhgy: d/dwaist of the HG
 // tables, same layout.  Back-projection per (frame, pol): partial[bp][0..5] =
 // {sum Im(AV), sum x Im(AV), sum y Im(AV), sum r^2 Im(AV), sum Re(Aw V),

@@ -191,6 +191,9 @@ EXT_C int digHoloConfigGetBasisGroupCount(int handleIdx);                       
 EXT_C int digHoloConfigSetBasisWaist(int handleIdx, int polIdx, float waist);     /* digHolo.h:1404 */
 EXT_C float digHoloConfigGetBasisWaist(int handleIdx, int polIdx);                /* digHolo.h:1416 */
 EXT_C int digHoloConfigSetBasisTypeHG(int handleIdx);                             /* digHolo.h:1424 */
+EXT_C int digHoloConfigSetBasisTypeLG(int handleIdx);                             /* digHolo.h:1435 */
+EXT_C int digHoloConfigSetBasisTypeCustom(int handleIdx, int modeCountIn, int modeCountOut,
+                                          complex64* transform);                 /* digHolo.h:1453 */
 EXT_C int digHoloConfigSetBasisType(int handleIdx, int basisType);                /* digHolo.h:1464 */
 EXT_C int digHoloConfigGetBasisType(int handleIdx);                               /* digHolo.h:1472 */
 
@@ -198,6 +201,8 @@ EXT_C int digHoloConfigGetBasisType(int handleIdx);                             
 EXT_C complex64* digHoloProcessBatch(int handleIdx, int* batchCount, int* modeCount, int* polCount); /* digHolo.h:1494 */
 EXT_C complex64* digHoloGetFields(int handleIdx, int* batchCount, int* polCount, float** x, float** y, int* width, int* height); /* digHolo.h:1549 */
 EXT_C complex64* digHoloBasisGetCoefs(int handleIdx, int* batchCount, int* modeCount, int* polCount); /* digHolo.h:1603 */
+EXT_C complex64* digHoloBasisGetFields(int handleIdx, int* modeCount, int* polCount, float** x, float** y,
+                                       int* width, int* height);                 /* digHolo.h:1587 */
 /* int16 fields R, I [batch][pol][h][w] with per (pol, batch) scale 32767 / max|V| */
 EXT_C int digHoloGetFields16(int handleIdx, int* batchCount, int* polCount, short** fieldR, short** fieldI,
                              float** fieldScale, float** x, float** y, int* width, int* height); /* digHolo.h:1568 */

@@ -1187,3 +1187,95 @@ def alignment_grad(frames: np.ndarray, cfg: Config, hold: Hold, mdl_weight: floa
         grad[p, GRAD_CY] += -2.0 * Tby - 2.0 * k0 * thy * S0
         grad[wp, GRAD_WAIST] += 2.0 * Sw
     return loss, grad
+
+
+# ---------------------------------------------------------------------------
+# Basis transforms (digHolo.cpp:9469-9561, :9819-10030, :4999-5040) and the
+# basis fields (:5233-5296)
+# ---------------------------------------------------------------------------
+
+def lg_blocks(G: int) -> List[np.ndarray]:
+    """Per HG group g (size g+1) the matrix U_g with LG[k] = sum_c U_g[k, c] HG[c]
+    (digHoloLGInit :9909-10031 and digHoloLGtoHGcoef :9819-9906): LG mode k of
+    group g has the index pair of HG mode k (m = g-k, n = k), l = m - n,
+    p = (g - |l|)/2; with (a, b) = (p, p - l) for l < 0 (so a = p, b = -l + p)
+    and (a, b) = (l + p, p) otherwise, the entry for HG (hm = g - c, hn = c) is
+      i^(g + hm) (-1)^[g odd and l > 0] 2^(-g/2) sqrt(hm! hn! / (a! b!))
+        sum_t (-1)^t C(b, t) C(a, hm - t)."""
+    from math import comb, factorial
+    blocks = []
+    for g in range(G):
+        U = np.zeros((g + 1, g + 1), dtype=np.complex128)
+        for k in range(g + 1):
+            m_, n_ = g - k, k
+            l = m_ - n_
+            p = (g - abs(l)) // 2
+            if l < 0:
+                a, b = p, -l + p
+            else:
+                b, a = p, l + p
+            for c in range(g + 1):
+                hm, hn = g - c, c
+                s = sum((-1) ** t * comb(b, t) * comb(a, hm - t) for t in range(0, min(hm, b) + 1) if hm - t <= a)
+                val = s * math.sqrt(factorial(hm) * factorial(hn) / (factorial(a) * factorial(b))) * 2.0 ** (-g / 2)
+                if g % 2 and l > 0:
+                    val = -val
+                U[k, c] = val * (1j ** ((g + hm) % 4))
+        blocks.append(U)
+    return blocks
+
+
+def lg_transform(coefs: np.ndarray, G: int) -> np.ndarray:
+    """HG -> LG coefficients, group by group (HGtoLG :4999-5040)."""
+    c = np.asarray(coefs, dtype=np.complex128)
+    out = np.zeros_like(c)
+    off = 0
+    for g, U in enumerate(lg_blocks(G)):
+        out[..., off:off + g + 1] = c[..., off:off + g + 1] @ U.T
+        off += g + 1
+    return out
+
+
+def custom_transform(coefs: np.ndarray, T: np.ndarray, n_in: int, n_out: int) -> np.ndarray:
+    """Custom basis (:9519-9561, :9566-9605): T is the caller's column-major
+    n_out x n_in matrix (element (j, i) at T[i * n_out + j]); the part used is
+    min(M, n_in) inputs by min(M, n_out) outputs, c_out[j] = sum_i T(j, i) c_HG[i]."""
+    c = np.asarray(coefs, dtype=np.complex128)
+    M = c.shape[-1]
+    di, do = min(M, n_in), min(M, n_out)
+    Tm = np.asarray(T, dtype=np.complex128).reshape(n_in, n_out)[:di, :do]    # [i, j]
+    return c[..., :di] @ Tm
+
+
+def basis_field_weights(G: int, basis: int, T=None, n_in: int = 0, n_out: int = 0) -> np.ndarray:
+    """Rows of HG weights the basis fields are built from (GetFieldsBasis
+    :5233-5296): identity (HG), conj(U_g) rows (LG, the inverse transform of the
+    identity), T(j, :) unconjugated (custom)."""
+    M = G * (G + 1) // 2
+    if basis == 1:
+        Wt = np.zeros((M, M), dtype=np.complex128)
+        off = 0
+        for g, U in enumerate(lg_blocks(G)):
+            Wt[off:off + g + 1, off:off + g + 1] = np.conj(U)
+            off += g + 1
+        return Wt
+    if basis == 2:
+        di, do = min(M, n_in), min(M, n_out)
+        Tm = np.asarray(T, dtype=np.complex128).reshape(n_in, n_out)[:di, :do]
+        Wt = np.zeros((do, M), dtype=np.complex128)
+        Wt[:, :di] = Tm.T
+        return Wt
+    return np.eye(M, dtype=np.complex128)
+
+
+def basis_fields(cfg: Config, weights: np.ndarray) -> np.ndarray:
+    """Basis fields [modes, pol, fh, fw]: sum_i w_i HGx_{m_i}(x) HGy_{n_i}(y)
+    on each pol's field plane."""
+    cfg = apply_pol_locks(cfg)
+    plan = make_plan(cfg)
+    Mi, Ni = hg_indices(cfg.group_count)
+    out = []
+    for p in range(cfg.pol_count):
+        t = pol_tables(cfg, plan, p)
+        out.append(np.einsum("ki,ix,iy->kyx", weights, t.hgx[Mi], t.hgy[Ni]))
+    return np.stack(out, axis=1)

@@ -696,3 +696,71 @@ def test_alignment_gradient_with_mdl_term(independent):
     # the MDL term moves the gradient: it is not the power-only one
     _, g0 = O.alignment_grad(frames, cfg, hold)
     assert np.abs(g - g0).max() > 1e-3 * np.abs(g0).max()
+
+
+# ---------------------------------------------------------------------------
+# Basis transforms
+# ---------------------------------------------------------------------------
+
+def test_lg_blocks_are_unitary():
+    for g, U in enumerate(O.lg_blocks(12)):
+        assert np.allclose(U @ U.conj().T, np.eye(g + 1), atol=1e-12), g
+
+
+def test_lg_basis_fields_are_laguerre_gauss_modes():
+    """conj(U_g) rows applied to continuous HG modes give the textbook LG_{p,l}
+    (up to one phase per mode), with l = m - n of the mode's index pair and one
+    handedness convention for every mode."""
+    from scipy.special import eval_genlaguerre
+    G, w = 7, 1.0
+    x = np.linspace(-6, 6, 241)
+    X, Y = np.meshgrid(x, x)
+    dx = x[1] - x[0]
+    hx = O.hg_modes_1d(G, x, w, dx)
+    Mi, Ni = O.hg_indices(G)
+    Wt = O.basis_field_weights(G, 1)
+    R2, PHI = X ** 2 + Y ** 2, np.arctan2(Y, X)
+    signs = set()
+    for k in range(len(Mi)):
+        field = np.einsum("i,ix,iy->yx", Wt[k], hx[Mi], hx[Ni])
+        l = Mi[k] - Ni[k]
+        g = Mi[k] + Ni[k]
+        p = (g - abs(l)) // 2
+        best = 0.0
+        for s in (1, -1):
+            lg = ((np.sqrt(2 * R2) / w) ** abs(l) * eval_genlaguerre(p, abs(l), 2 * R2 / w ** 2) *
+                  np.exp(-R2 / w ** 2) * np.exp(1j * s * l * PHI))
+            lg = lg / np.sqrt((np.abs(lg) ** 2).sum())
+            ov = abs(np.vdot(lg, field)) / np.sqrt((np.abs(field) ** 2).sum())
+            if ov > best:
+                best, bs = ov, s
+        assert best > 1 - 1e-9, (k, l, p, best)
+        if l != 0:
+            signs.add(bs)
+    assert len(signs) == 1
+
+
+def test_transformed_coefficients_are_the_basis_field_overlaps():
+    """c_LG[k] = sum conj(LG field_k) E and c_custom[j] = sum field_j E with the
+    same fields the library exports (HG overlap without conjugation)."""
+    spec = synth.spec_small()
+    spec.group_count = 6
+    frames, _, _, _ = synth.make_batch(spec, 2, seed=1)
+    cfg = small_cfg(group_count=6)
+    fields, coefs, _ = O.process_batch(frames, cfg)
+    G, M = 6, 21
+    lg = O.lg_transform(coefs, G)
+    F = O.basis_fields(cfg, O.basis_field_weights(G, 1))
+    proj = np.einsum("kpyx,bpyx->bpk", np.conj(F), fields)
+    assert np.allclose(proj, lg, rtol=1e-9, atol=1e-9 * np.abs(lg).max())
+    rng = np.random.default_rng(0)
+    n_in, n_out = 30, 12                   # more inputs than modes: the extra rows are not used
+    T = rng.standard_normal(n_in * n_out) + 1j * rng.standard_normal(n_in * n_out)
+    cu = O.custom_transform(coefs, T, n_in, n_out)
+    assert cu.shape[-1] == n_out
+    F2 = O.basis_fields(cfg, O.basis_field_weights(G, 2, T, n_in, n_out))
+    proj2 = np.einsum("kpyx,bpyx->bpk", F2, fields)
+    assert np.allclose(proj2, cu, rtol=1e-9, atol=1e-9 * np.abs(cu).max())
+    # identity transform leaves the HG coefficients
+    eye = np.eye(M).reshape(-1)
+    assert np.allclose(O.custom_transform(coefs, eye, M, M), coefs)
