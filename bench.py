@@ -221,6 +221,73 @@ def uneven_batching(lib, reps=20):
             "graph_speedup": round(seq_ms / graph_ms, 3), "window_paths": paths}
 
 
+def large_sensor(lib, batch=256, steps=10):
+    """High-order decomposition: 1024x1024 float32 frames, dual pol, batch 256,
+    FFT window 512x512 (IFFT 160x160), 80 HG groups (3240 modes per pol), group
+    power ~ exp(-g/20).  Times the FP32 and the split-fp16 overlap (HIP events on
+    the handle's stream, stage times from the library) and reports the
+    coefficient error of each against the FP64 oracle on two frames."""
+    import numpy as np
+    import torch
+    from digholo_amd import api, synth
+    from oracle import digholo_oracle as O
+    W, Hh, win, G = 1024, 1024, 512, 80
+    bx = (-2.56e-3, 2.56e-3)
+    spec = synth.HologramSpec(frame_width=W, frame_height=Hh, pol_count=2, waist=300e-6, group_count=G,
+                              beam_centre_x=list(bx), group_decay=20.0)
+    fr16 = make_frames(n_unique=8, seed=5, spec=spec, batch=batch)
+    frames = fr16.to(torch.float32).contiguous()
+    del fr16
+    h = lib.digHoloCreate()
+    api.configure(lib, h, frame_width=W, frame_height=Hh, pol_count=2, fft_window=(win, win), waist=300e-6,
+                  group_count=G, beam_centre_x=bx)
+    assert lib.digHoloSetBatch(h, batch, C.c_void_p(frames.data_ptr())) == 0
+    b, m, p = C.c_int(), C.c_int(), C.c_int()
+    res = {}
+    sample = frames[:2].cpu().numpy().astype(np.float64)
+    oc = O.Config(frame_width=W, frame_height=Hh, pol_count=2, fft_window_x=win, fft_window_y=win, aperture_deg=0.7,
+                  tilt_x=[1.5, 1.5], tilt_y=[1.5, 1.5], group_count=G, waist=[300e-6] * 2, beam_centre_x=list(bx))
+    _, c_ref, _ = O.process_batch(sample, oc)
+    geo = api.window_geometry(lib, h)
+    for name, prec in (("fp32", api.DIGHOLO_PRECISION_FP32), ("split16", api.DIGHOLO_PRECISION_SPLIT16)):
+        lib.digHoloConfigSetOverlapPrecision(h, prec)
+        for _ in range(2):
+            assert lib.digHoloProcessBatchDevice(h, C.byref(b), C.byref(m), C.byref(p))
+        lib.digHoloSynchronize(h)
+        stream = torch.cuda.ExternalStream(lib.digHoloGetStream(h))
+        lib.digHoloConfigSetStageTiming(h, 1)
+        st = (C.c_float * 5)()
+        lib.digHoloGetStageTimes(h, st)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            assert lib.digHoloProcessBatchDevice(h, C.byref(b), C.byref(m), C.byref(p))
+        e1.record(stream)
+        e1.synchronize()
+        lib.digHoloGetStageTimes(h, st)
+        lib.digHoloConfigSetStageTiming(h, 0)
+        ms = e0.elapsed_time(e1) / steps
+        coefs = api.get_coefs(lib, h)
+        err = O.relative_l2(coefs[:2], c_ref)
+        ov_ms = float(st[4])
+        flops = overlap_flops_per_frame(2, G, geo["w"], geo["h"]) * batch
+        fft_ms = float(st[0] + st[1] + st[2])
+        fft_bytes = (2 * win * win * 4 + 2 * geo["w"] * geo["h"] * 8) * batch
+        res[name] = {"ms_per_batch": round(ms, 4), "frames_s": round(batch / (ms * 1e-3), 1),
+                     "overlap_ms": round(ov_ms, 4),
+                     "overlap_tflops": round(flops / (ov_ms * 1e-3) / 1e12, 2) if ov_ms > 0 else None,
+                     "overlap_mfma_frac": round(flops / (ov_ms * 1e-3) / 1e12 /
+                                                (FP32_PEAK_TFLOPS if prec == 0 else 2500.0), 4) if ov_ms > 0 else None,
+                     "fft_stages_ms": round(fft_ms, 4),
+                     "fft_stages_hbm_frac": round(fft_bytes / (fft_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "coef_rel_l2_vs_fp64": float(f"{err:.3e}"), "window_path": lib.digHoloGetWindowPath(h)}
+    lib.digHoloDestroy(h)
+    res["workload"] = ("1024x1024 float32 dual-pol, batch 256, FFT 512x512 -> IFFT 160x160, 80 HG groups "
+                       "(3240 modes/pol), group power ~ exp(-g/20)")
+    res["split_vs_fp32_overlap"] = round(res["fp32"]["overlap_ms"] / res["split16"]["overlap_ms"], 3)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -231,6 +298,8 @@ def main():
                     help="skip the gradient-vs-tweak alignment comparison (batch 500)")
     ap.add_argument("--no-uneven", action="store_true",
                     help="skip the three-configuration graph-launch measurement (5000 frames)")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the 1024x1024, 80-group decomposition measurement")
     ap.add_argument("--precision", choices=["fp32", "split16"], default="split16")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for rocprofv3 --pmc: two batches with the fused split overlap, two with the FP32 "
@@ -362,6 +431,9 @@ def main():
     if not args.no_uneven:
         os.environ.setdefault("DIGHOLO_TUNE_CACHE", os.path.join(ROOT, "gpurun_out", "tune_cache.txt"))
         out["uneven_batching"] = uneven_batching(lib)
+    if not args.no_large:
+        torch.cuda.empty_cache()
+        out["large_sensor"] = large_sensor(lib)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

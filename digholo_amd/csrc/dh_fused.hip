@@ -724,7 +724,7 @@ struct W3Shape {
 
 // LDS bytes of the aliased region: the row phase or the field plane
 __host__ __device__ inline int w3_region(int ebytes, int bbytes, int H, int L, int M, int w, int h) {
-  const int row = ebytes + bbytes + 8 * H + 8 * L * (M + 1);
+  const int row = ebytes + bbytes + 8 * H + 8 * L * (M + 1) + 8 * H;
   const int pl = 8 * h * (w + 4);
   return ((row > pl ? row : pl) + 15) & ~15;
 }
@@ -964,6 +964,7 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
   constexpr int BSTRIDE = 2 * (w + 1) * PK;
   uint2* twH = reinterpret_cast<uint2*>(w3_smem + Sh::EBYTES + Sh::BBYTES);     // [H] {cos, sin} hi, lo
   float2* twA = reinterpret_cast<float2*>(twH + H);                            // [L][M + 1]
+  uint2* twH1 = reinterpret_cast<uint2*>(twA + L * (M + 1));                   // [H] {-sin, cos} hi, lo
   float2* fbuf = reinterpret_cast<float2*>(w3_smem);                            // [h][WP] after the rows
   float2* twix = reinterpret_cast<float2*>(w3_smem + a.region);                 // [w]
   float2* twiy = twix + w;                                                      // [h]
@@ -976,7 +977,13 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
 #pragma unroll
   for (int m = 0; m < M; ++m) pre[m] = __builtin_nontemporal_load(src + L * m);
 
-  for (int e = tid; e < H; e += NT) twH[e] = reinterpret_cast<const uint2*>(tabs)[e];
+  for (int e = tid; e < H; e += NT) {
+    const uint2 t = reinterpret_cast<const uint2*>(tabs)[e];
+    twH[e] = t;
+    // the imaginary-part operand (-sin, cos): halves swapped, then the low one negated
+    twH1[e] = make_uint2(__builtin_amdgcn_alignbit(t.x, t.x, 16) ^ 0x8000u,
+                         __builtin_amdgcn_alignbit(t.y, t.y, 16) ^ 0x8000u);
+  }
   for (int e = tid; e < L * (M + 1); e += NT) twA[e] = tabs[H + e];
   for (int e = tid; e < w; e += NT) twix[e] = a.ix.tw[e];
   for (int e = tid; e < h; e += NT) twiy[e] = a.iy.tw[e];
@@ -992,13 +999,16 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
     wlo[s] = tabs[H + L * (M + 1) + alo[s]];
     whi[s] = tabs[H + L * (M + 1) + ahi[s]];
   }
-  // window rows of this lane's MFMA units
-  int ky[NU];
+  // window rows of this lane's MFMA units; the twiddle index ky * y mod H of
+  // the lane's first k-slot, carried from k-step to k-step (y advances by 16)
+  int ky[NU], mrun[NU], ky16[NU];
 #pragma unroll
   for (int i = 0; i < NU; ++i) {
     const int jt = (wv + 4 * i) >> 1;
     int t = (16 * jt + lr + g.SY[p] + H / 2) % H;
     ky[i] = t < 0 ? t + H : t;
+    mrun[i] = (ky[i] * 4 * q) % H;
+    ky16[i] = (ky[i] * 16) % H;
   }
 
   floatx4 acc[NU][U];
@@ -1088,35 +1098,31 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
     }
   };
 
-  // F += D . X over the rows of round rd (B buffer Bs)
+  // F += D . X over the rows of round rd (B buffer Bs); rounds run in order,
+  // so mrun[i] is ky * (rd RR + 16 ks + 4 q) mod H on entry to each k-step
+  const uint2* twU = (wv & 1) ? twH1 : twH;   // every unit of a wave has the wave's part
   auto gemm = [&](int rd, const _Float16* Bs) {
+    (void)rd;
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
       half8 ah[NU], al[NU];
-      const int y0 = rd * RR + 16 * ks + 4 * q;
 #pragma unroll
       for (int i = 0; i < NU; ++i) {
         const int unit = wv + 4 * i;
         if (unit < 2 * J) {
-          const unsigned qq = (unsigned)(ky[i] * y0);
-          int mm = (int)(qq - __umulhi(qq, a.magH) * (unsigned)H);
+          int mm = mrun[i];
+          mrun[i] += ky16[i];
+          if (mrun[i] >= H) mrun[i] -= H;
           uint4 dh, dl;   // k-slot pairs (Re, Im) of A: part 0 (cos, sin), part 1 (-sin, cos)
           unsigned* ph = reinterpret_cast<unsigned*>(&dh);
           unsigned* pl = reinterpret_cast<unsigned*>(&dl);
 #pragma unroll
           for (int e2 = 0; e2 < 4; ++e2) {
-            const uint2 t = twH[mm];
+            const uint2 t = twU[mm];
             mm += ky[i];
             if (mm >= H) mm -= H;
             ph[e2] = t.x;
             pl[e2] = t.y;
-          }
-          if (unit & 1) {
-#pragma unroll
-            for (int e2 = 0; e2 < 4; ++e2) {
-              ph[e2] = __builtin_amdgcn_alignbit(ph[e2], ph[e2], 16) ^ 0x8000u;
-              pl[e2] = __builtin_amdgcn_alignbit(pl[e2], pl[e2], 16) ^ 0x8000u;
-            }
           }
           ah[i] = __builtin_bit_cast(half8, dh);
           al[i] = __builtin_bit_cast(half8, dl);
