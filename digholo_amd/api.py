@@ -69,6 +69,13 @@ SIGNATURES: Dict[str, Tuple[object, List[object]]] = {
     "digHoloConfigGetIFFTResolutionMode": (c_int, [c_int]),
     "digHoloConfigSetWavelengthCentre": (c_int, [c_int, c_float]),
     "digHoloConfigGetWavelengthCentre": (c_float, [c_int]),
+    "digHoloConfigSetWavelengths": (c_int, [c_int, P_float, c_int]),
+    "digHoloConfigSetWavelengthsLinearFrequency": (c_int, [c_int, c_float, c_float, c_int]),
+    "digHoloConfigGetWavelengths": (P_float, [c_int, P_int]),
+    "digHoloConfigSetWavelengthOrdering": (c_int, [c_int, c_int, c_int]),
+    "digHoloConfigGetWavelengthOrdering": (c_int, [c_int, c_int]),
+    "digHoloProcessBatchFrequencySweepLinear": (c_void_p, [c_int, P_int, P_int, P_int, c_float, c_float, c_int]),
+    "digHoloProcessBatchWavelengthSweepArbitrary": (c_void_p, [c_int, P_int, P_int, P_int, P_float, c_int]),
     "digHoloConfigSetAutoAlignBeamCentre": (c_int, [c_int, c_int]),
     "digHoloConfigGetAutoAlignBeamCentre": (c_int, [c_int]),
     "digHoloConfigSetAutoAlignDefocus": (c_int, [c_int, c_int]),
@@ -168,6 +175,10 @@ DIGHOLO_METRIC_COUNT = 9
 DIGHOLO_ANALYSIS_COUNT = 7
 DIGHOLO_ANALYSIS_MAXABS = 3
 DIGHOLO_ANALYSIS_MAXABSIDX = 4
+DIGHOLO_WAVELENGTHORDER_INPUT = 0
+DIGHOLO_WAVELENGTHORDER_OUTPUT = 1
+DIGHOLO_WAVELENGTHORDER_FAST = 0
+DIGHOLO_WAVELENGTHORDER_SLOW = 1
 DIGHOLO_PRECISION_FP32 = 0
 DIGHOLO_PRECISION_SPLIT16 = 1
 
@@ -296,6 +307,22 @@ def process_batch(lib, h, frames: np.ndarray):
     x = np.ctypeslib.as_array(xp, shape=(w.value,)).copy()
     y = np.ctypeslib.as_array(yp, shape=(hh.value,)).copy()
     return coefs, fields, x, y
+
+
+def set_wavelengths(lib, h, wavelengths) -> None:
+    """digHoloConfigSetWavelengths from a sequence of wavelengths (m)."""
+    lam = np.ascontiguousarray(wavelengths, dtype=np.float32)
+    rc = lib.digHoloConfigSetWavelengths(h, lam.ctypes.data_as(P_float), lam.size)
+    if rc != DIGHOLO_ERROR_SUCCESS:
+        raise ValueError(f"digHoloConfigSetWavelengths returned {rc}")
+
+
+def get_wavelengths(lib, h) -> np.ndarray:
+    n = c_int()
+    ptr = lib.digHoloConfigGetWavelengths(h, C.byref(n))
+    if not ptr or n.value <= 0:
+        raise RuntimeError("digHoloConfigGetWavelengths failed")
+    return np.ctypeslib.as_array(ptr, shape=(n.value,)).copy()
 
 
 def batch_summary(lib, h, plane: int):

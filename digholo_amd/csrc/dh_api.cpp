@@ -47,6 +47,10 @@ struct Config {
   float beamCX[2] = {0.f, 0.f}, beamCY[2] = {0.f, 0.f};
   int batchCount = 0, avgCount = 1, avgMode = 0;
   float wavelengthCentre = 1565e-9f;
+  // wavelength sweep (digHolo.cpp:6578-6680): the axis the next batch uses;
+  // fewer than two entries means the centre wavelength alone (:7855-7863)
+  std::vector<float> wavelengths;
+  int wavelengthOrdering[2] = {0, 0};   // input frames, output fields/coefficients
   int polLockTilt = 0, polLockWaist = 1, polLockDefocus = 0;
   int polCount = 1;
   int aaTilt = 1, aaDefocus = 1, aaCentre = 1, aaWaist = 1, aaFourierRadius = 1;
@@ -118,10 +122,16 @@ struct FFTEntry {
 // Everything derived from the configuration that the kernels consume.
 struct Derived {
   dh::Geometry g{};
-  dh::TableArgs tab{};
+  dh::TableArgs tab{};                  // wavelength 0
+  std::vector<float> lam;               // the wavelength axis, L entries
+  std::vector<dh::TableArgs> tabLam;    // every wavelength, tabLam[0] == tab
   int SX[2]{}, SY[2]{};
   std::vector<uint8_t> mask;
   float pxx = 0.f, pxy = 0.f;  // float32 reconstructed pixel pitch (axes export)
+};
+
+struct PowerMeterOut {
+  double v[DIGHOLO_METRIC_COUNT];
 };
 
 struct Handle {
@@ -159,6 +169,8 @@ struct Handle {
   DevBuf<float2> coefsBackup, coefsMix;   // AutoAlign: best-so-far coefficients per pol
   std::vector<float> aaMetrics;   // DIGHOLO_METRIC_COUNT x (lambdaCount + 1)
   int aaLambdaCount = 0;
+  std::vector<PowerMeterOut> pmLam;   // the last power meter call, per wavelength
+  std::vector<float> lamAxis;         // wavelength axis handed out by digHoloConfigGetWavelengths
   bool coefsHostValid = false, fieldsHostValid = false;
   bool coefsDevValid = false;   // coefs belong to the published fields
   // pipeline run step by step (ProcessFFT -> ProcessIFFT -> ProcessRemoveTilt ->
@@ -326,10 +338,25 @@ bool derive(const Config& c, int B, Derived& d, int& err, const Derived* crop = 
     cx[p] = crop->g.cx[p];
     cy[p] = crop->g.cy[p];
   }
-  // --- IFFT window, digHolo.cpp:7868-8048 ---
-  const float lam = c.wavelengthCentre;
-  const float k0max = 2.0f * PI_F / lam;
-  const float k0min = k0max;
+  // --- wavelength axis (digHolo.cpp:7823-7865): a single entry is the centre ---
+  d.lam = c.wavelengths.size() > 1 ? c.wavelengths : std::vector<float>{c.wavelengthCentre};
+  const int L = (int)d.lam.size();
+  for (float v : d.lam)
+    if (!(v > 0.f) || !std::isfinite(v)) {
+      err = DIGHOLO_ERROR_INVALIDARGUMENT;
+      return false;
+    }
+  const int ordIn = c.wavelengthOrdering[0] ? 1 : 0, ordOut = c.wavelengthOrdering[1] ? 1 : 0;
+  // a slow ordering needs whole sweeps: otherwise frames would share output
+  // positions or index past the axis
+  if (L > 1 && (ordIn || ordOut) && B % L != 0) {
+    err = DIGHOLO_ERROR_INVALIDDIMENSION;
+    return false;
+  }
+  // --- IFFT window, digHolo.cpp:7868-8048: wide enough for both ends of the axis ---
+  const float lamA = d.lam[0], lamB = d.lam[L - 1];
+  const float k0max = 2 * PI_F / std::fmax(lamA, lamB);
+  const float k0min = 2 * PI_F / std::fmin(lamA, lamB);
   const float kfx = k_factor(W, px), kfy = k_factor(H, px);
   const float dkx = k_axis(1, W, kfx) - k_axis(0, W, kfx);
   const float dky = k_axis(1, H, kfy) - k_axis(0, H, kfy);
@@ -398,13 +425,19 @@ bool derive(const Config& c, int B, Derived& d, int& err, const Derived* crop = 
   }
   g.G = c.maxMG;
   g.M = g.G * (g.G + 1) / 2;
+  g.L = L;
+  g.ordIn = ordIn;
+  g.ordOut = ordOut;
+  g.subB = B / L;
 
-  // --- Fourier-window mask in float32 (digHolo.cpp:1950-2003, :8647-8656) ---
-  d.mask.assign((size_t)P * h * w, 0);
+  // --- Fourier-window mask in float32 (digHolo.cpp:1950-2003, :8647-8656),
+  // [L][P][h][w]: the aperture moves with each wavelength, the window does not ---
+  d.mask.assign((size_t)L * P * h * w, 0);
   if (win) d.mask = win->mask;
   const float fullShift = std::fabs((k_axis(1, H, kfy) - k_axis(0, H, kfy)) * (float)H);
-  for (int p = 0; p < P && !win; ++p) {
-    const float k0 = 2.0f * PI_F / lam;
+  for (int lp = 0; lp < L * P && !win; ++lp) {
+    const int l = lp / P, p = lp - l * P;
+    const float k0 = 2 * PI_F / d.lam[l];
     const float tx = c.tiltX[p] * UNIT_ANGLE_F;
     const float ty = c.tiltY[p] * UNIT_ANGLE_F;
     const float kx0 = k0 * (float)std::sin((double)tx);
@@ -422,17 +455,19 @@ bool derive(const Config& c, int B, Derived& d, int& err, const Derived* crop = 
         const float dk_x = k_axis(col, W, kfx) - kx0;
         const float dkx2 = dk_x * dk_x;
         const float dr2 = dkx2 + dky2;
-        d.mask[((size_t)p * h + j) * w + i] = dr2 <= r02 ? 1 : 0;
+        d.mask[((size_t)lp * h + j) * w + i] = dr2 <= r02 ? 1 : 0;
       }
     }
   }
 
-  // --- FP64 parameters of the window factor, reference wave and basis ---
-  dh::TableArgs& t = d.tab;
-  t = dh::TableArgs{};
-  t.g = g;
+  // --- FP64 parameters of the window factor, reference wave and basis, per
+  // wavelength (digHolo.cpp:10386-10420: k0 of each wavelength) ---
+  d.tabLam.assign(L, dh::TableArgs{});
   const double pxd = (double)px;
-  const double lamd = (double)lam;
+  for (int l = 0; l < L; ++l) {
+  dh::TableArgs& t = d.tabLam[l];
+  t.g = g;
+  const double lamd = (double)d.lam[l];
   const double k0d = 2.0 * PI_D / lamd;
   for (int p = 0; p < P; ++p) {
     dh::PolParams& q = t.pp[p];
@@ -464,6 +499,8 @@ bool derive(const Config& c, int B, Derived& d, int& err, const Derived* crop = 
     if (wst == 0.f) wst = 400e-6f;
     q.waist = (double)wst;
   }
+  }
+  d.tab = d.tabLam[0];
   d.pxx = (px * W) / g.fw;              // digHolo.cpp:8184-8185 (float)
   d.pxy = (px * H) / g.fh;
   return true;
@@ -476,7 +513,10 @@ bool same_tables(const Derived& a, const Derived& b) {
     return false;
   for (int p = 0; p < x.P; ++p)
     if (x.c0[p] != y.c0[p] || x.SY[p] != y.SY[p]) return false;
-  if (std::memcmp(a.tab.pp, b.tab.pp, sizeof(a.tab.pp)) != 0) return false;
+  if (x.L != y.L || x.ordIn != y.ordIn || x.ordOut != y.ordOut || x.subB != y.subB) return false;
+  if (a.tabLam.size() != b.tabLam.size()) return false;
+  for (size_t l = 0; l < a.tabLam.size(); ++l)
+    if (std::memcmp(a.tabLam[l].pp, b.tabLam[l].pp, sizeof(a.tabLam[l].pp)) != 0) return false;
   return a.mask == b.mask;
 }
 
@@ -902,6 +942,10 @@ int enqueue_batch(Handle& h, int step = 0) {
   const size_t B = g.B, P = g.P;
   const int A = std::max(1, c.avgCount);      // frames averaged into each batch element
   const size_t nFrames = B * (size_t)A;
+  if (A > 1 && g.L > 1) {
+    fprintf(stderr, "digHolo: frame averaging is not available together with a wavelength sweep\n");
+    return DIGHOLO_ERROR_INVALIDARGUMENT;
+  }
   const void* dframes = nullptr;
   if (const int ef = device_frames(h, nFrames, dframes)) return ef;
   dh::BatchCal cal;
@@ -948,14 +992,18 @@ int enqueue_batch(Handle& h, int step = 0) {
     DH_TRY(hipEventRecord(ev[0], h.stream), "event");
   }
   if (!h.tablesValid || !same_tables(d, h.tabKey)) {
-    const size_t T = P * g.h * g.w;
-    DH_TRY(h.mask.ensure(T), "alloc");
-    DH_TRY(h.factor.ensure(T), "alloc");
-    DH_TRY(h.ref.ensure(P * g.fh * g.fw), "alloc");
+    // window factor [L][P][h][w] and reference wave [L][P][fh][fw]; the HG
+    // basis does not depend on the wavelength
+    const size_t T = P * g.h * g.w, TR = P * (size_t)g.fh * g.fw, L = d.tabLam.size();
+    DH_TRY(h.mask.ensure(T * L), "alloc");
+    DH_TRY(h.factor.ensure(T * L), "alloc");
+    DH_TRY(h.ref.ensure(TR * L), "alloc");
     DH_TRY(h.hgx.ensure(std::max<size_t>(P * g.G * g.fw, 1)), "alloc");
     DH_TRY(h.hgy.ensure(std::max<size_t>(P * g.G * g.fh, 1)), "alloc");
-    DH_TRY(hipMemcpyAsync(h.mask.p, d.mask.data(), T, hipMemcpyHostToDevice, h.stream), "mask upload");
-    DH_TRY(dh::launch_tables(d.tab, h.mask.p, h.factor.p, h.ref.p, h.hgx.p, h.hgy.p, h.stream), "tables");
+    DH_TRY(hipMemcpyAsync(h.mask.p, d.mask.data(), T * L, hipMemcpyHostToDevice, h.stream), "mask upload");
+    for (size_t l = 0; l < L; ++l)
+      DH_TRY(dh::launch_tables(d.tabLam[l], h.mask.p + T * l, h.factor.p + T * l, h.ref.p + TR * l,
+                               l ? nullptr : h.hgx.p, l ? nullptr : h.hgy.p, h.stream), "tables");
     DH_TRY(h.colTab.ensure(dh::coldft_table_elems(g)), "alloc");
     DH_TRY(dh::launch_coldft_table(g, h.colTab.p, h.stream), "column table");
     if (const size_t n3 = dh::window_v3_table_elems(g)) {
@@ -1338,25 +1386,27 @@ int compute_fields16(Handle& h) {
 // matrix of the last batch.  pols: -1 both polarisations (the only form the
 // C API's AutoAlignCalcMetrics uses).
 // ---------------------------------------------------------------------------
-struct PowerMeterOut {
-  double v[DIGHOLO_METRIC_COUNT];
-};
 
 // polIdx < 0: both polarisations of the last batch.  polIdx = p: the metric
 // seen while optimising pol p alone (digHolo.cpp:10770-10860): its own
 // coefficients only when the pols are independent, else its coefficients
 // beside the other pol's best-so-far ones from h.coefsBackup.
-int power_meter(Handle& h, int polIdx, bool independent, bool doIL, bool doMDL, bool doXtalk, bool mulConj,
-                PowerMeterOut& o) {
+// One wavelength's metrics over its B / L batch elements: output positions
+// lam, lam + L, ... (wavelength the fast axis) or lam B/L ... (slow),
+// digHolo.cpp:10770-10790.
+int power_meter_lambda(Handle& h, int lam, int polIdx, bool independent, bool doIL, bool doMDL, bool doXtalk,
+                       bool mulConj, PowerMeterOut& o) {
   const dh::Geometry& g = h.cur.g;
   const int B = g.B, P = g.P, Mpol = h.Mout;                // the coefficients of the configured basis
+  const int L = std::max(1, g.L);
   const float FMAX = std::numeric_limits<float>::max();
-  if (Mpol == 0) return DIGHOLO_ERROR_INVALIDARGUMENT;      // needs the Fourier-plane power (not built)
   if (B > 1) {
     const bool indep = independent || P == 1;
     int S = (P == 2 && indep) ? 2 : 1;
     int M = (P == 2 && !indep) ? 2 * Mpol : Mpol;          // svdM
-    const int N = B;                                       // svdN
+    const int N = B / L;                                   // svdN
+    const size_t first = g.ordOut ? (size_t)lam * N : (size_t)lam;   // first batch element of this wavelength
+    const int step = g.ordOut ? 1 : L;                     // batch elements between its rows
     int rowStride = P * Mpol;
     const float2* A = h.outCoefs();
     if (polIdx >= 0 && P == 2) {
@@ -1375,6 +1425,8 @@ int power_meter(Handle& h, int polIdx, bool independent, bool doIL, bool doMDL, 
         rowStride = 2 * Mpol;
       }
     }
+    A += first * rowStride;
+    rowStride *= step;
     DH_TRY(h.metricRows.ensure((size_t)S * N * 5), "alloc");
     DH_TRY(dh::launch_metric_rows(A, N, M, Mpol, rowStride, S, h.metricRows.p, h.stream), "metric rows");
     std::vector<double> rows((size_t)S * N * 5);
@@ -1502,16 +1554,42 @@ int power_meter(Handle& h, int polIdx, bool independent, bool doIL, bool doMDL, 
   return DIGHOLO_ERROR_SUCCESS;
 }
 
-// metrics in dB, one wavelength: out[m][0] and the wavelength average out[m][1]
-// (digHolo.cpp:12183-12191)
+// Metrics of every wavelength into h.pmLam (float, as the reference keeps
+// them) and their linear average into o (digHolo.cpp:12172-12191).
+int power_meter(Handle& h, int polIdx, bool independent, bool doIL, bool doMDL, bool doXtalk, bool mulConj,
+                PowerMeterOut& o) {
+  const dh::Geometry& g = h.cur.g;
+  const int L = std::max(1, g.L);
+  if (h.Mout == 0) return DIGHOLO_ERROR_INVALIDARGUMENT;   // needs the Fourier-plane power (not built)
+  if (g.B < L) return DIGHOLO_ERROR_INVALIDDIMENSION;      // no wavelength has a batch element
+  h.pmLam.assign(L, PowerMeterOut{});
+  float sum[DIGHOLO_METRIC_COUNT] = {};
+  for (int l = 0; l < L; ++l) {
+    PowerMeterOut& ol = h.pmLam[l];
+    if (int e = power_meter_lambda(h, l, polIdx, independent, doIL, doMDL, doXtalk, mulConj, ol)) return e;
+    for (int m = 0; m < DIGHOLO_METRIC_COUNT; ++m) {
+      ol.v[m] = (float)ol.v[m];
+      sum[m] += (float)ol.v[m];
+    }
+  }
+  for (int m = 0; m < DIGHOLO_METRIC_COUNT; ++m) o.v[m] = L == 1 ? h.pmLam[0].v[m] : (double)(sum[m] / L);
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+// metrics in dB: out[m][l] per wavelength and the wavelength average
+// out[m][L] (digHolo.cpp:12183-12191); h.pmLam holds the last power_meter call
 void store_metrics_db(Handle& h, const PowerMeterOut& o) {
-  const int L = 1;
+  const int L = std::max<int>(1, (int)h.pmLam.size());
   h.aaLambdaCount = L;
   h.aaMetrics.assign((size_t)DIGHOLO_METRIC_COUNT * (L + 1), 0.f);
   for (int m = 0; m < DIGHOLO_METRIC_COUNT; ++m) {
-    const float lin = (float)o.v[m];
-    h.aaMetrics[(size_t)m * (L + 1) + 0] = 10.f * std::log10(lin);
-    h.aaMetrics[(size_t)m * (L + 1) + L] = 10.f * std::log10(lin / L);
+    float sum = 0.f;
+    for (int l = 0; l < L; ++l) {
+      const float lin = h.pmLam.empty() ? (float)o.v[m] : (float)h.pmLam[l].v[m];
+      sum += lin;
+      h.aaMetrics[(size_t)m * (L + 1) + l] = 10.f * std::log10(lin);
+    }
+    h.aaMetrics[(size_t)m * (L + 1) + L] = 10.f * std::log10(sum / L);
   }
 }
 
@@ -2350,6 +2428,10 @@ int mdl_seed(Handle& h, double mu, double& term) {
 int alignment_gradient(Handle& h, bool hold, double mdlWeight, double& loss, double grad[2][AA_PARS]) {
   Config& c = h.cfg;
   if (c.maxMG <= 0) return DIGHOLO_ERROR_INVALIDARGUMENT;
+  if (c.wavelengths.size() > 1) {   // the adjoint is written for one reference wave per pol
+    fprintf(stderr, "digHolo: the alignment gradient covers single-wavelength batches\n");
+    return DIGHOLO_ERROR_INVALIDARGUMENT;
+  }
   const bool useHold = hold && h.gradHeldValid;
   const bool hc = h.holdCrop, hw = h.holdWin;
   if (useHold) {
@@ -2623,7 +2705,7 @@ bool launch_signature(Handle& h, std::string& sig) {
   int err;
   if (!derive(c, c.batchCount, d, err)) return false;
   std::string s((const char*)&d.g, sizeof(d.g));
-  s.append((const char*)d.tab.pp, sizeof(d.tab.pp));
+  for (const dh::TableArgs& t : d.tabLam) s.append((const char*)t.pp, sizeof(t.pp));
   s.append((const char*)d.mask.data(), d.mask.size());
   char buf[256];
   std::snprintf(buf, sizeof(buf), "|%p|%d|%d|%d|%d|%d|%d|%d|%u|%u|%d|%d|%p", h.frames, h.frameU16, h.transpose,
@@ -3073,6 +3155,77 @@ EXT_C float digHoloConfigGetWavelengthCentre(int handleIdx) {
   return CFG.wavelengthCentre;
 }
 
+// Wavelength axis (digHolo.cpp:6578-6680, :14611-14700).  The axis applies
+// from the next batch on; one entry means the centre wavelength.
+EXT_C int digHoloConfigSetWavelengths(int handleIdx, float* wavelengths, int wavelengthCount) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (wavelengthCount <= 0) return DIGHOLO_ERROR_INVALIDARGUMENT;
+  if (!wavelengths) return DIGHOLO_ERROR_NULLPOINTER;
+  CFG.wavelengths.assign(wavelengths, wavelengths + wavelengthCount);
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+namespace {
+// Linear in optical frequency f = c0 / lambda, in float32 as the reference
+// evaluates it (digHolo.cpp:6577-6650; c0 an int converted to float; a
+// wavelength above 1 is taken in the unit DIGHOLO_UNIT_LAMBDA).
+int set_wavelengths_linear_frequency(Config& c, float lamStart, float lamStop, int count) {
+  if (count <= 0) return DIGHOLO_ERROR_INVALIDDIMENSION;
+  const float c0 = (float)299792458;
+  if (lamStart > 1) lamStart = lamStart * DIGHOLO_UNIT_LAMBDA;
+  if (lamStop > 1) lamStop = lamStop * DIGHOLO_UNIT_LAMBDA;
+  float fStart = c0 / lamStart;
+  const float fStop = c0 / lamStop;
+  float df = (fStop - fStart) / (float)(count - 1);
+  if (count <= 1) {
+    // one step: the start wavelength's double frequency when only it is valid
+    // (either single-ended case, as the reference writes both), the centre
+    // wavelength when neither is, else the mean of the two
+    if (lamStop <= 0 && lamStart > 0) fStart = c0 / (lamStart / 2);
+    else if (lamStart <= 0 && lamStop > 0) fStart = c0 / (lamStart / 2);
+    else if (lamStart <= 0 && lamStop <= 0) fStart = c0 / (float)(c.wavelengthCentre * DIGHOLO_UNIT_LAMBDA);
+    else fStart = c0 / ((lamStop + lamStart) / 2);
+    df = 0;
+  }
+  c.wavelengths.resize(count);
+  for (int i = 0; i < count; ++i) {
+    c.wavelengths[i] = c0 / (fStart + df * (float)i);   // built without FMA contraction
+  }
+  return DIGHOLO_ERROR_SUCCESS;
+}
+}  // namespace
+
+EXT_C int digHoloConfigSetWavelengthsLinearFrequency(int handleIdx, float wavelengthStart, float wavelengthStop,
+                                                      int wavelengthCount) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (!wavelengthCount) return DIGHOLO_ERROR_INVALIDARGUMENT;
+  return set_wavelengths_linear_frequency(CFG, wavelengthStart, wavelengthStop, wavelengthCount);
+}
+
+// The axis the next batch uses (the centre wavelength alone for fewer than two entries).
+EXT_C float* digHoloConfigGetWavelengths(int handleIdx, int* wavelengthCount) {
+  Handle* h = get(handleIdx);
+  if (!h) return nullptr;
+  h->lamAxis = h->cfg.wavelengths.size() > 1 ? h->cfg.wavelengths : std::vector<float>{h->cfg.wavelengthCentre};
+  if (wavelengthCount) *wavelengthCount = (int)h->lamAxis.size();
+  return h->lamAxis.data();
+}
+
+EXT_C int digHoloConfigSetWavelengthOrdering(int handleIdx, int inoutIdx, int ordering) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if ((inoutIdx != DIGHOLO_WAVELENGTHORDER_INPUT && inoutIdx != DIGHOLO_WAVELENGTHORDER_OUTPUT) ||
+      (ordering != DIGHOLO_WAVELENGTHORDER_FAST && ordering != DIGHOLO_WAVELENGTHORDER_SLOW))
+    return DIGHOLO_ERROR_INVALIDARGUMENT;
+  CFG.wavelengthOrdering[inoutIdx] = ordering;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+EXT_C int digHoloConfigGetWavelengthOrdering(int handleIdx, int inoutIdx) {
+  H_OR(0);
+  if (inoutIdx != DIGHOLO_WAVELENGTHORDER_INPUT && inoutIdx != DIGHOLO_WAVELENGTHORDER_OUTPUT) return 0;
+  return CFG.wavelengthOrdering[inoutIdx];
+}
+
 // ---- AutoAlign switches (digHolo.cpp:13542-13735): non-zero stored as 1 ----
 #define AA_FLAG(Name, field)                                         \
   EXT_C int digHoloConfigSet##Name(int handleIdx, int v) {           \
@@ -3264,6 +3417,22 @@ EXT_C complex64* digHoloProcessBatch(int handleIdx, int* batchCount, int* modeCo
   *polCount = g.P;
   *modeCount = h->Mout;
   return h->Mout > 0 ? reinterpret_cast<complex64*>(h->hCoefs.p) : nullptr;   // G=0: no decomposition
+}
+
+// A batch whose frames follow a wavelength axis: the axis is set first (its
+// error, if any, leaves the previous axis in place), then the batch runs
+// (digHolo.cpp:5046-5076, :15194-15236).
+EXT_C complex64* digHoloProcessBatchFrequencySweepLinear(int handleIdx, int* batchCount, int* modeCount, int* polCount,
+                                                         float lambdaStart, float lambdaStop, int lambdaCount) {
+  if (Handle* h = get(handleIdx)) (void)set_wavelengths_linear_frequency(h->cfg, lambdaStart, lambdaStop, lambdaCount);
+  return digHoloProcessBatch(handleIdx, batchCount, modeCount, polCount);
+}
+
+EXT_C complex64* digHoloProcessBatchWavelengthSweepArbitrary(int handleIdx, int* batchCount, int* modeCount,
+                                                             int* polCount, float* wavelengths, int lambdaCount) {
+  if (Handle* h = get(handleIdx))
+    if (lambdaCount > 0 && wavelengths) h->cfg.wavelengths.assign(wavelengths, wavelengths + lambdaCount);
+  return digHoloProcessBatch(handleIdx, batchCount, modeCount, polCount);
 }
 
 EXT_C complex64* digHoloBasisGetCoefs(int handleIdx, int* batchCount, int* modeCount, int* polCount) {

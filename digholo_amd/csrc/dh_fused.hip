@@ -247,7 +247,7 @@ __global__ __launch_bounds__(kFThreads, 4) void k_fused_window(FusedArgs a, Fuse
     __syncthreads();
   }
   // window factor, then the field plane in LDS
-  const float2* fac = factor + (size_t)p * hw;
+  const float2* fac = factor + (size_t)(lam_in(g, b) * g.P + p) * hw;
 #pragma unroll
   for (int s = 0; s < FPT; ++s) {
     const int e = tid + s * kFThreads;
@@ -256,8 +256,8 @@ __global__ __launch_bounds__(kFThreads, 4) void k_fused_window(FusedArgs a, Fuse
   __syncthreads();
   lds_fft_ip<true>(fbuf, h, w, 1, false, a.ix, twix, a.chunkIx);   // along x
   lds_fft_ip<true>(fbuf, w, 1, w, true, a.iy, twiy, a.chunkIy);    // along y
-  const float2* rp = ref + (size_t)p * hw;
-  float2* out = fields + (size_t)(b * g.P + p) * hw;
+  const float2* rp = ref + (size_t)(lam_in(g, b) * g.P + p) * hw;
+  float2* out = fields + (size_t)(out_pos(g, b) * g.P + p) * hw;
   for (int e = tid; e < hw; e += kFThreads) {
     const float2 t = cmul(fbuf[e], rp[e]);
     out[e] = make_float2(-t.x, t.y);
@@ -463,13 +463,13 @@ __global__ __launch_bounds__(kFThreads, 4) void k_window_mfma(FusedArgs a, Fused
     }
   }
   __syncthreads();
-  const float2* fac = factor + (size_t)p * hw;
+  const float2* fac = factor + (size_t)(lam_in(g, b) * g.P + p) * hw;
   for (int e = tid; e < hw; e += kFThreads) fbuf[e] = cmul(fbuf[e], fac[e]);
   __syncthreads();
   lds_fft_ip<true>(fbuf, h, w, 1, false, a.ix, twix, a.chunkIx);   // along x
   lds_fft_ip<true>(fbuf, w, 1, w, true, a.iy, twiy, a.chunkIy);    // along y
-  const float2* rp = ref + (size_t)p * hw;
-  float2* out = fields + (size_t)(b * g.P + p) * hw;
+  const float2* rp = ref + (size_t)(lam_in(g, b) * g.P + p) * hw;
+  float2* out = fields + (size_t)(out_pos(g, b) * g.P + p) * hw;
   for (int e = tid; e < hw; e += kFThreads) {
     const float2 t = cmul(fbuf[e], rp[e]);
     out[e] = make_float2(-t.x, t.y);
@@ -1177,7 +1177,7 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
   // inverse transforms along x (rows), window factor on load
   {
     constexpr int MM = w / 4, US = MM / 4;
-    const float2* fac = factor + (size_t)p * (h * w);
+    const float2* fac = factor + (size_t)(lam_in(g, b) * g.P + p) * (h * w);
     const float2 cb = a.cal.at(p, b);
     for (int ps = wave; ps < J; ps += 4) {
       const int j = 16 * ps + t16;
@@ -1217,8 +1217,8 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
   float* red = reinterpret_cast<float*>(twiy + h);   // [4] per-wave peaks
   {
     constexpr int MM = h / 4, US = MM / 4;
-    const float2* rp = ref + (size_t)p * (h * w);
-    float2* out = fields + (size_t)(b * g.P + p) * (h * w);
+    const float2* rp = ref + (size_t)(lam_in(g, b) * g.P + p) * (h * w);
+    float2* out = fields + (size_t)(out_pos(g, b) * g.P + p) * (h * w);
     for (int ps = wave; ps < U; ps += 4) {
       const int c = 16 * ps + t16;
       float2* col = fbuf + c;
@@ -1258,7 +1258,7 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
       }
     }
   }
-  if (ovl) w3_overlap<U, J>(a, fbuf, vmax, red, w3_smem + a.ovlOff, p, b, wave, lane);
+  if (ovl) w3_overlap<U, J>(a, fbuf, vmax, red, w3_smem + a.ovlOff, p, out_pos(g, b), wave, lane);
 }
 
 // Tables of the v3 kernel, from FP64: [H] cos and sin of 2 pi m / H as fp16

@@ -35,7 +35,35 @@ struct Geometry {
   int c0[kMaxPol];            // first r2c column read by the IFFT window (SX - W/2)
   int SY[kMaxPol];            // IFFT window row start (shifted-spectrum coordinates)
   int G, M;                   // HG groups and modes per pol
+  // wavelength sweep: L wavelengths; frame and output ordering, 0 wavelength
+  // the fast axis, 1 the slow one; subB = B / L frames per wavelength
+  int L, ordIn, ordOut, subB;
 };
+
+// Wavelength index of input frame b, the batch position its field and
+// coefficients take, and the wavelength index of output position o
+// (digHolo.cpp:2656-2700).  One wavelength: index 0, position b.  The host
+// admits a slow ordering only when L divides B, so positions form a permutation.
+__host__ __device__ __forceinline__ int lam_in(const Geometry& g, int b) {
+  if (g.L <= 1) return 0;
+  return g.ordIn ? b / g.subB : b % g.L;
+}
+__host__ __device__ __forceinline__ int out_pos(const Geometry& g, int b) {
+  if (g.L <= 1) return b;
+  int lam, sub;
+  if (g.ordIn) {
+    sub = b % g.subB;
+    lam = b / g.subB;
+  } else {
+    lam = b % g.L;
+    sub = b / g.L;
+  }
+  return g.ordOut ? lam * g.subB + sub : sub * g.L + lam;
+}
+__host__ __device__ __forceinline__ int lam_out(const Geometry& g, int o) {
+  if (g.L <= 1) return 0;
+  return g.ordOut ? o / g.subB : o % g.L;
+}
 
 // Per-frame complex calibration of the Fourier window (digHolo.cpp:4469-4500,
 // :1831-1840): cal[p % P][b % B]; p == nullptr means none.
@@ -66,6 +94,7 @@ struct TableArgs {
 };
 
 // Launchers (defined in dh_kernels.hip).  All enqueue on `s`.
+// One wavelength's window factor and reference wave; d_hgx == nullptr skips the HG basis.
 hipError_t launch_tables(const TableArgs& a, const uint8_t* d_mask, float2* d_factor, float2* d_ref,
                          float* d_hgx, float* d_hgy, hipStream_t s);
 hipError_t launch_row_fft(const Geometry& g, const FFTPlan& pl, const float2* twW, const void* frames,

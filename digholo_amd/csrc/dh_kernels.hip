@@ -163,8 +163,8 @@ __global__ __launch_bounds__(kThreads) void k_col_fft_window(Geometry g, FFTPlan
   }
   __syncthreads();
   const float2* F = lds_fft<false>(A, Bf, cols, pl);
-  const float2* fac = factor + (size_t)p * g.h * g.w + i0;
-  float2* out = win + (size_t)(b * g.P + p) * g.h * g.w + i0;
+  const float2* fac = factor + (size_t)(lam_in(g, b) * g.P + p) * g.h * g.w + i0;
+  float2* out = win + (size_t)(out_pos(g, b) * g.P + p) * g.h * g.w + i0;
   const int sy = g.SY[p] + n / 2;
   const float2 cb = cal.at(p, b);
   for (int e = threadIdx.x; e < g.h * cols; e += blockDim.x) {
@@ -244,7 +244,7 @@ __global__ __launch_bounds__(kThreads) void k_ifft_cols_tilt(Geometry g, FFTPlan
     }
     return;
   }
-  const float2* rp = ref + (size_t)p * g.h * g.w + i0;
+  const float2* rp = ref + (size_t)(lam_out(g, b) * g.P + p) * g.h * g.w + i0;
   for (int e = threadIdx.x; e < n * cols; e += blockDim.x) {
     const int j = e / cols, c = e - j * cols;
     const float2 t = cmul(E[c * n + j], rp[(size_t)j * g.w + c]);
@@ -255,8 +255,8 @@ __global__ __launch_bounds__(kThreads) void k_ifft_cols_tilt(Geometry g, FFTPlan
 // ---------------------------------------------------------------------------
 // Step kernels (the pipeline run one stage at a time)
 // ---------------------------------------------------------------------------
-// win[b][p][j][c] = spec[b][p][row(j)][c] * factor[p][j][c], spec the column-
-// transformed band of the w window columns ([B][P][H][w]).
+// win[out(b)][p][j][c] = spec[b][p][row(j)][c] * factor[lam(b)][p][j][c], spec
+// the column-transformed band of the w window columns ([B][P][H][w]).
 __global__ void k_select_window(Geometry g, const float2* __restrict__ spec, const float2* __restrict__ factor,
                                 float2* __restrict__ win, BatchCal cal) {
   const size_t hw = (size_t)g.h * g.w;
@@ -268,7 +268,9 @@ __global__ void k_select_window(Geometry g, const float2* __restrict__ spec, con
   const int j = rem / g.w, c = rem - j * g.w;
   int row = (j + g.SY[p] + g.H / 2) % g.H;
   if (row < 0) row += g.H;
-  win[e] = cmul(cmul(spec[(bp * g.H + row) * g.w + c], factor[(size_t)p * hw + rem]), cal.at(p, (int)(bp / g.P)));
+  const int b = (int)(bp / g.P);
+  const float2 fac = factor[(size_t)(lam_in(g, b) * g.P + p) * hw + rem];
+  win[((size_t)out_pos(g, b) * g.P + p) * hw + rem] = cmul(cmul(spec[(bp * g.H + row) * g.w + c], fac), cal.at(p, b));
 }
 
 // Full-resolution IFFT input (digHolo.cpp:1855-1872, :1938-1950, :2042-2065):
@@ -380,13 +382,16 @@ hipError_t launch_average_windows(const Geometry& g, int A, int mode, int frames
   return hipGetLastError();
 }
 
-// fields = -conj(fields * ref), in place
+// fields = -conj(fields * ref), in place; fields in output order, ref of the
+// position's wavelength
 __global__ void k_apply_tilt(Geometry g, const float2* __restrict__ ref, float2* __restrict__ fields) {
   const size_t hw = (size_t)g.h * g.w;
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (size_t)g.B * g.P * hw) return;
-  const int p = (int)((e / hw) % g.P);
-  const float2 t = cmul(fields[e], ref[(size_t)p * hw + e % hw]);
+  const size_t bp = e / hw;
+  const int p = (int)(bp % g.P);
+  const int lam = lam_out(g, (int)(bp / g.P));
+  const float2 t = cmul(fields[e], ref[((size_t)lam * g.P + p) * hw + e % hw]);
   fields[e] = make_float2(-t.x, t.y);
 }
 
@@ -515,7 +520,7 @@ hipError_t launch_tables(const TableArgs& a, const uint8_t* d_mask, float2* d_fa
   const int total = a.g.P * (a.g.h * a.g.w + a.g.fh * a.g.fw);
   k_tables<<<(total + 255) / 256, 256, 0, s>>>(a, d_mask, d_factor, d_ref);
   hipError_t e = hipGetLastError();
-  if (e != hipSuccess || a.g.G == 0) return e;
+  if (e != hipSuccess || a.g.G == 0 || !d_hgx) return e;
   const int t2 = a.g.P * (a.g.fw + a.g.fh);
   k_hg_basis<<<(t2 + 63) / 64, 64, 0, s>>>(a, d_hgx, d_hgy);
   return hipGetLastError();

@@ -23,7 +23,7 @@ In dual polarisation, pol p is rendered in the x-half p of the sensor.
 from __future__ import annotations
 
 import math
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 from typing import List, Optional
 
 import numpy as np
@@ -174,6 +174,25 @@ def make_batch(spec: HologramSpec, batch: int, seed: int = 0, quantise: bool = T
     coefs = random_coefs(spec, batch, rng)
     frames, scale, amps = render(spec, coefs, rng, quantise=quantise)
     return frames, coefs, scale, amps
+
+
+def make_sweep_batch(spec: HologramSpec, wavelengths, batch: int, seed: int = 0, order_in: int = 0,
+                     quantise: bool = True):
+    """Frames of a wavelength sweep: frame b is rendered at wavelengths[l(b)],
+    l(b) = b mod L (wavelength the fast axis) or b // (batch / L) (slow).
+    Returns (frames, coefs, scales [B], amps [B][pol])."""
+    rng = np.random.default_rng(seed)
+    coefs = random_coefs(spec, batch, rng)
+    L = len(wavelengths)
+    frames, scales, amps = [], [], []
+    for b in range(batch):
+        lam = b % L if not order_in else b // max(1, batch // L)
+        s = replace(spec, wavelength=float(wavelengths[lam]))
+        f, sc, am = render(s, coefs[b:b + 1], rng, quantise=quantise)
+        frames.append(f[0])
+        scales.append(sc)
+        amps.append(am)
+    return np.stack(frames), coefs, np.array(scales), amps
 
 
 # The configurations the benchmark and parity tests are quoted on.

@@ -41,6 +41,13 @@ typedef float complex64[2];                       /* digHolo.h:163 */
 #define DIGHOLO_AVGMODE_SEQUENTIALSWEEP 2
 #define DIGHOLO_AVGMODE_COUNT 3
 
+/* Wavelength ordering: digHolo.h:874-878; wavelength unit: digHolo.h:191 */
+#define DIGHOLO_WAVELENGTHORDER_INPUT 0
+#define DIGHOLO_WAVELENGTHORDER_OUTPUT 1
+#define DIGHOLO_WAVELENGTHORDER_FAST 0
+#define DIGHOLO_WAVELENGTHORDER_SLOW 1
+#define DIGHOLO_UNIT_LAMBDA 1
+
 /* AutoAlign modes: digHolo.h:925-928 */
 #define DIGHOLO_AUTOALIGNMODE_FULL 0
 #define DIGHOLO_AUTOALIGNMODE_TWEAK 1
@@ -138,6 +145,16 @@ EXT_C int digHoloConfigSetIFFTResolutionMode(int handleIdx, int resolutionModeId
 EXT_C int digHoloConfigGetIFFTResolutionMode(int handleIdx);                      /* digHolo.h:797 */
 EXT_C int digHoloConfigSetWavelengthCentre(int handleIdx, float lambda0);         /* digHolo.h:814 */
 EXT_C float digHoloConfigGetWavelengthCentre(int handleIdx);                      /* digHolo.h:822 */
+/* Wavelength sweeps: frames at L wavelengths, wavelength the fast (0) or slow
+   (1) axis of the input frames and of the output fields/coefficients.  A slow
+   ordering requires batchCount to be a multiple of L; frame averaging is not
+   available together with a sweep of more than one wavelength. */
+EXT_C int digHoloConfigSetWavelengths(int handleIdx, float* wavelengths, int wavelengthCount); /* digHolo.h:832 */
+EXT_C int digHoloConfigSetWavelengthsLinearFrequency(int handleIdx, float wavelengthStart, float wavelengthStop,
+                                                      int wavelengthCount);          /* digHolo.h:843 */
+EXT_C float* digHoloConfigGetWavelengths(int handleIdx, int* wavelengthCount);     /* digHolo.h:854 */
+EXT_C int digHoloConfigSetWavelengthOrdering(int handleIdx, int inoutIdx, int ordering); /* digHolo.h:886 */
+EXT_C int digHoloConfigGetWavelengthOrdering(int handleIdx, int inoutIdx);         /* digHolo.h:895 */
 
 /* ---- AutoAlign switches ---- */
 EXT_C int digHoloConfigSetAutoAlignBeamCentre(int handleIdx, int enable);         /* digHolo.h:938 */
@@ -199,6 +216,12 @@ EXT_C int digHoloConfigGetBasisType(int handleIdx);                             
 
 /* ---- processing ---- */
 EXT_C complex64* digHoloProcessBatch(int handleIdx, int* batchCount, int* modeCount, int* polCount); /* digHolo.h:1494 */
+EXT_C complex64* digHoloProcessBatchFrequencySweepLinear(int handleIdx, int* batchCount, int* modeCount, int* polCount,
+                                                         float lambdaStart, float lambdaStop,
+                                                         int lambdaCount);           /* digHolo.h:1510 */
+EXT_C complex64* digHoloProcessBatchWavelengthSweepArbitrary(int handleIdx, int* batchCount, int* modeCount,
+                                                             int* polCount, float* wavelengths,
+                                                             int lambdaCount);       /* digHolo.h:1525 */
 EXT_C complex64* digHoloGetFields(int handleIdx, int* batchCount, int* polCount, float** x, float** y, int* width, int* height); /* digHolo.h:1549 */
 EXT_C complex64* digHoloBasisGetCoefs(int handleIdx, int* batchCount, int* modeCount, int* polCount); /* digHolo.h:1603 */
 EXT_C complex64* digHoloBasisGetFields(int handleIdx, int* modeCount, int* polCount, float** x, float** y,

@@ -19,6 +19,7 @@ What it restates (all citations are digHolo.cpp unless noted):
 * HG indices and FP64 Hermite basis ............... :1245-1484, :10044-10645
 * separable overlap c = sum_y HGy sum_x HGx V ...... :2149-2341, :9348-9467
 * power-meter metrics (IL, MDL, diag, SNR) ........ :10714-11129
+* wavelength sweeps (axis, ordering, per-lambda) .. :2656-2700, :6577-6680, :7823-7865, :8644-8670, :12172-12191
 * plane analysis (power, COM, peak, Aeff) ......... :2354-2654, :8990-9188
 
 Numerical contract
@@ -84,6 +85,10 @@ class Config:
     pol_lock_tilt: int = 0
     pol_lock_defocus: int = 0
     pol_lock_waist: int = 1
+    # wavelength sweep (digHolo.cpp:3361, :6578-6680): None or one entry means
+    # the centre wavelength alone; ordering (input, output), 0 fast, 1 slow
+    wavelengths: List[float] = None
+    wavelength_ordering: Tuple[int, int] = (0, 0)
 
 
 # ---------------------------------------------------------------------------
@@ -144,11 +149,55 @@ def tilt_rad_f32(deg: float) -> np.float32:
     return F32(F32(deg) * UNIT_ANGLE_F32)
 
 
+def wavelength_axis(cfg: Config) -> np.ndarray:
+    """The wavelengths a batch uses (float32): the sweep, or the centre
+    wavelength when the sweep has fewer than two entries (digHolo.cpp:7823-7865)."""
+    if cfg.wavelengths is not None and len(cfg.wavelengths) > 1:
+        return np.asarray(cfg.wavelengths, dtype=np.float32)
+    return np.array([cfg.wavelength], dtype=np.float32)
+
+
+def wavelength_ordering_calc(idx: int, L: int, B: int, ord_in: int, ord_out: int) -> Tuple[int, int, int]:
+    """(output position, wavelength index, sub-batch index) of input element idx
+    (digHolo.cpp:2656-2700): wavelength the fast axis (0) or the slow one (1)."""
+    sub_count = B // L
+    if not ord_in:
+        lam, sub = idx % L, idx // L
+    elif sub_count:
+        sub, lam = idx % sub_count, idx // sub_count
+    else:
+        lam = sub = 0
+    out = sub * L + lam if not ord_out else lam * sub_count + sub
+    return out, lam, sub
+
+
+def frequency_sweep_linear(lam_start: float, lam_stop: float, count: int, centre: float) -> np.ndarray:
+    """Wavelengths linear in optical frequency, float32 as digHolo.cpp:6577-6650
+    evaluates them (c0 = 299792458 converted to float; one step takes the mean of
+    valid end points, or the centre wavelength when neither is valid)."""
+    c0 = F32(299792458)
+    a, b = F32(lam_start), F32(lam_stop)     # DIGHOLO_UNIT_LAMBDA = 1
+    f_start = F32(c0 / a) if a != 0 else F32(np.inf)
+    f_stop = F32(c0 / b) if b != 0 else F32(np.inf)
+    with np.errstate(all="ignore"):
+        df = F32(F32(f_stop - f_start) / F32(count - 1))
+    if count <= 1:
+        if b <= 0 < a or a <= 0 < b:
+            f_start = F32(c0 / F32(a / F32(2)))
+        elif a <= 0 and b <= 0:
+            f_start = F32(c0 / F32(centre))
+        else:
+            f_start = F32(c0 / F32(F32(b + a) / F32(2)))
+        df = F32(0)
+    return np.array([F32(c0 / F32(f_start + F32(df * F32(i)))) for i in range(count)], dtype=np.float32)
+
+
 def ifft_window(cfg: Config, W: int, H: int) -> Tuple[int, int, List[int], List[int]]:
-    """IFFT window size and per-pol corner (low-res, r2c).  digHolo.cpp:7868-8048."""
-    lam = F32(cfg.wavelength)
-    k0max = F32(F32(2.0) * PI_F32 / lam)
-    k0min = k0max
+    """IFFT window size and per-pol corner (low-res, r2c).  digHolo.cpp:7868-8048:
+    wide enough for the longest and the shortest wavelength of the axis."""
+    lam = wavelength_axis(cfg)
+    k0max = F32(F32(2.0) * PI_F32 / max(lam[0], lam[-1]))
+    k0min = F32(F32(2.0) * PI_F32 / min(lam[0], lam[-1]))
     kx = k_axis_f32(W, cfg.pixel_size)
     ky = k_axis_f32(H, cfg.pixel_size)
     dkx = F32(kx[1] - kx[0])
@@ -306,11 +355,13 @@ def _pol_index(cfg: Config, p: int) -> Tuple[int, int]:
     return tilt_p, idx_p
 
 
-def pol_tables(cfg: Config, plan: Plan, p: int, mask=None) -> PolTables:
+def pol_tables(cfg: Config, plan: Plan, p: int, mask=None, wavelength=None) -> PolTables:
+    """Tables of pol p at one wavelength (default the centre): the aperture and the
+    reference wave follow k0 = 2 pi / lambda (digHolo.cpp:8644-8656, :10386-10420)."""
     W, H, w, h = plan.W, plan.H, plan.w, plan.h
     px32 = F32(cfg.pixel_size)
     px = float(px32)
-    lam32 = F32(cfg.wavelength)
+    lam32 = F32(cfg.wavelength if wavelength is None else wavelength)
     lam = float(lam32)
 
     # --- Fourier-window copy, low-res r2c branch (digHolo.cpp:1845-2065) ---
@@ -475,7 +526,18 @@ def process_batch(frames: np.ndarray, cfg: Config, emulate_int16: bool = False, 
     P = cfg.pol_count
     G = cfg.group_count
     Mi, Ni = hg_indices(G)
-    tabs = [pol_tables(cfg, plan, p, mask=hold.masks[p] if hold is not None else None) for p in range(P)]
+    lams = wavelength_axis(cfg)
+    L = len(lams)
+    if L > 1 and (A > 1 or hold is not None):
+        raise ValueError("a wavelength sweep is restated without averaging or a held window")
+    # frame b uses wavelength lam_b and lands at output position out_b
+    # (digHolo.cpp:8644-8670, applyTilt :9329-9340 in output order)
+    pos = [wavelength_ordering_calc(b, L, B, *cfg.wavelength_ordering) for b in range(B)]
+    out_b = np.array([q[0] for q in pos])
+    lam_b = np.array([q[1] for q in pos])
+    tabs_l = [[pol_tables(cfg, plan, p, mask=hold.masks[p] if hold is not None else None, wavelength=lams[l])
+               for p in range(P)] for l in range(L)]
+    tabs = tabs_l[0]
     low = cfg.resolution_mode == 1
     fh, fw = (plan.h, plan.w) if low else (plan.H, plan.W)
     fields = np.zeros((B, P, fh, fw), dtype=np.complex128)
@@ -484,7 +546,10 @@ def process_batch(frames: np.ndarray, cfg: Config, emulate_int16: bool = False, 
         t = tabs[p]
         win = frames[:, plan.cy[p]:plan.cy[p] + plan.H, plan.cx[p]:plan.cx[p] + plan.W]
         F = np.fft.rfft2(win.astype(np.float64), axes=(-2, -1))
-        Fw = F[:, t.row[:, None], t.col[None, :]] * t.factor[None]
+        factor = np.stack([tabs_l[l][p].factor for l in range(L)])      # the window itself is the same
+        ref = np.stack([tabs_l[l][p].ref for l in range(L)])
+        Fw = F[:B * A, t.row[:, None], t.col[None, :]]
+        Fw = Fw * (factor[lam_b] if A == 1 else factor[0][None])
         if batch_cal is not None:
             cal = np.asarray(batch_cal, dtype=np.complex128)
             cal_b = lambda b: cal[p % cal.shape[0], b % cal.shape[1]]
@@ -502,14 +567,14 @@ def process_batch(frames: np.ndarray, cfg: Config, emulate_int16: bool = False, 
             full[:, t.row[:, None], t.col[None, :]] = Fw
             Fw = full
         E = np.fft.ifft2(Fw, axes=(-2, -1)) * (fw * fh)              # unnormalised e^{+i}
-        V = -np.conj(E * t.ref[None])
+        V = -np.conj(E * ref[lam_b])
         if emulate_int16:
             V = quantise_int16(V)
-        fields[:, p] = V
+        fields[out_b, p] = V
         if G > 0:
             T = np.einsum("mx,byx->bmy", t.hgx, V)       # stage 1: x contraction
             C = np.einsum("ny,bmy->bmn", t.hgy, T)       # stage 2: y contraction
-            coefs[:, p] = C[:, Mi, Ni]
+            coefs[out_b, p] = C[:, Mi, Ni]
     return fields, coefs, plan
 
 
@@ -708,6 +773,23 @@ def power_meter(coefs: np.ndarray, independent: bool = False, mul_conj: bool = F
     out[METRIC_SNRWORST] = min_snr
     out[METRIC_SNRMG] = sig_mg / noi_mg if do_xtalk else float("nan")
     return out
+
+
+def power_meter_sweep(coefs: np.ndarray, L: int, ord_out: int, **kw) -> np.ndarray:
+    """Linear metrics per wavelength, [L, 9] (digHolo.cpp:10770-10790): wavelength
+    l owns B // L output positions, l, l + L, ... (fast) or l B/L, ... (slow)."""
+    c = np.asarray(coefs)
+    n = c.shape[0] // L
+    rows = [np.arange(n) * L + l if not ord_out else l * n + np.arange(n) for l in range(L)]
+    return np.stack([power_meter(c[r], **kw) for r in rows])
+
+
+def sweep_metrics_db(lin_l: np.ndarray) -> np.ndarray:
+    """[9, L + 1] in dB: every wavelength, then their linear average
+    (digHolo.cpp:12172-12191)."""
+    lin_l = np.asarray(lin_l, dtype=np.float64)
+    with np.errstate(divide="ignore"):
+        return 10.0 * np.log10(np.concatenate([lin_l.T, lin_l.mean(axis=0)[:, None]], axis=1))
 
 
 def metrics_db(lin: np.ndarray) -> np.ndarray:

@@ -174,3 +174,89 @@ def test_autoalign_modes_without_estimate_stage(lib):
     assert lib.digHoloConfigSetBasisGroupCount(h, 0) == 0
     assert lib.digHoloAutoAlign(h) == -fmax
     lib.digHoloDestroy(h)
+
+
+# ---- wavelength sweeps (digHolo.cpp:6577-6680, :14611-14700) ----
+
+def test_wavelength_axis_setters_match_oracle(lib):
+    h = lib.digHoloCreate()
+    rng = np.random.default_rng(7)
+    for _ in range(40):
+        a, b = (float(v) for v in rng.uniform(0.6e-6, 1.9e-6, 2))
+        n = int(rng.integers(2, 200))
+        assert lib.digHoloConfigSetWavelengthsLinearFrequency(h, a, b, n) == 0
+        got = api.get_wavelengths(lib, h)
+        assert np.array_equal(got, O.frequency_sweep_linear(a, b, n, 1565e-9))   # bit-exact float32
+    # fewer than two entries: the centre wavelength (digHolo.cpp:7855-7863)
+    assert lib.digHoloConfigSetWavelengthCentre(h, C.c_float(1.31e-6)) == 0
+    assert lib.digHoloConfigSetWavelengthsLinearFrequency(h, 1.5e-6, 1.6e-6, 1) == 0
+    assert np.array_equal(api.get_wavelengths(lib, h), np.array([1.31e-6], np.float32))
+    lam = np.array([1.5e-6, 1.52e-6, 1.58e-6], np.float32)
+    api.set_wavelengths(lib, h, lam)
+    assert np.array_equal(api.get_wavelengths(lib, h), lam)
+    # error conventions
+    assert lib.digHoloConfigSetWavelengths(h, lam.ctypes.data_as(api.P_float), 0) == api.DIGHOLO_ERROR_INVALIDARGUMENT
+    assert lib.digHoloConfigSetWavelengths(h, None, 3) == api.DIGHOLO_ERROR_NULLPOINTER
+    assert lib.digHoloConfigSetWavelengthsLinearFrequency(h, 1.5e-6, 1.6e-6, 0) == api.DIGHOLO_ERROR_INVALIDARGUMENT
+    assert lib.digHoloConfigSetWavelengthsLinearFrequency(h, 1.5e-6, 1.6e-6, -2) == api.DIGHOLO_ERROR_INVALIDDIMENSION
+    assert np.array_equal(api.get_wavelengths(lib, h), lam)      # failed calls keep the axis
+    for io in (0, 1):
+        assert lib.digHoloConfigGetWavelengthOrdering(h, io) == 0
+        assert lib.digHoloConfigSetWavelengthOrdering(h, io, 1) == 0
+        assert lib.digHoloConfigGetWavelengthOrdering(h, io) == 1
+    assert lib.digHoloConfigSetWavelengthOrdering(h, 2, 0) == api.DIGHOLO_ERROR_INVALIDARGUMENT
+    assert lib.digHoloConfigSetWavelengthOrdering(h, 0, 2) == api.DIGHOLO_ERROR_INVALIDARGUMENT
+    assert lib.digHoloConfigGetWavelengthOrdering(h, 5) == 0
+    assert not lib.digHoloConfigGetWavelengths(10 ** 6, None)       # NULL for an invalid handle
+    lib.digHoloDestroy(h)
+
+
+def test_sweep_window_geometry_matches_oracle(lib):
+    rng = np.random.default_rng(99)
+    checked = 0
+    for _ in range(150):
+        d = random_case(rng)
+        L = int(rng.integers(2, 9))
+        lam = np.sort(rng.uniform(0.5e-6, 2e-6, L)).astype(np.float32)
+        if rng.integers(0, 2):
+            lam = lam[::-1].copy()
+        h = lib.digHoloCreate()
+        try:
+            api.configure(lib, h, frame_width=d["fw"], frame_height=d["fh"], pol_count=d["P"], pixel_size=d["px"],
+                          fft_window=d["win"], aperture_deg=d["r"], wavelength=d["lam"],
+                          beam_centre_x=d["bcx"], beam_centre_y=d["bcy"], tilt_x=d["tx"], tilt_y=d["ty"],
+                          group_count=d["G"], pol_lock_tilt=d["lock"])
+        except ValueError:
+            continue
+        api.set_wavelengths(lib, h, lam)
+        cfg = oracle_cfg(d)
+        cfg.wavelengths = list(lam)
+        if d["lock"]:
+            cfg = O.apply_pol_locks(cfg)
+        plan = O.make_plan(cfg)
+        out = (C.c_int * 12)()
+        rc = lib.digHoloGetWindowGeometry(h, out)
+        lib.digHoloDestroy(h)
+        if plan.w > plan.W // 2 or plan.h > plan.H:     # window beyond the r2c half plane
+            assert rc == api.DIGHOLO_ERROR_INVALIDDIMENSION
+            continue
+        assert rc == 0
+        assert (out[0], out[1], out[2], out[3]) == (plan.W, plan.H, plan.w, plan.h), d
+        for p in range(d["P"]):
+            assert (out[8 + 2 * p], out[9 + 2 * p]) == (plan.SX[p], plan.SY[p]), d
+        checked += 1
+    assert checked > 50
+
+
+def test_slow_wavelength_ordering_needs_whole_sweeps(lib):
+    h = lib.digHoloCreate()
+    api.configure(lib, h, frame_width=320, frame_height=256, fft_window=(128, 128))
+    api.set_wavelengths(lib, h, [1.55e-6, 1.56e-6, 1.57e-6])
+    out = (C.c_int * 12)()
+    assert lib.digHoloConfigSetBatchCount(h, 7) == 0
+    assert lib.digHoloGetWindowGeometry(h, out) == 0                # fast/fast: any batch
+    assert lib.digHoloConfigSetWavelengthOrdering(h, 1, 1) == 0
+    assert lib.digHoloGetWindowGeometry(h, out) == api.DIGHOLO_ERROR_INVALIDDIMENSION
+    assert lib.digHoloConfigSetBatchCount(h, 9) == 0
+    assert lib.digHoloGetWindowGeometry(h, out) == 0
+    lib.digHoloDestroy(h)

@@ -764,3 +764,117 @@ def test_transformed_coefficients_are_the_basis_field_overlaps():
     # identity transform leaves the HG coefficients
     eye = np.eye(M).reshape(-1)
     assert np.allclose(O.custom_transform(coefs, eye, M, M), coefs)
+
+
+# ---------------------------------------------------------------------------
+# wavelength sweeps (digHolo.cpp:2656-2700, :6577-6680, :7823-7865, :8644-8670)
+# ---------------------------------------------------------------------------
+
+def test_wavelength_ordering_is_the_sweep_transpose():
+    L, n = 3, 4
+    B = L * n
+    # fast -> slow: element (sub, lam) of a [n, L] array moves to (lam, sub)
+    grid = np.arange(B).reshape(n, L)
+    out = np.array([O.wavelength_ordering_calc(b, L, B, 0, 1)[0] for b in range(B)])
+    assert np.array_equal(grid.T.reshape(-1)[out], np.arange(B))   # slot out[b] holds input b
+    lam = [O.wavelength_ordering_calc(b, L, B, 0, 0)[1] for b in range(B)]
+    assert lam == [b % L for b in range(B)]
+    lam_s = [O.wavelength_ordering_calc(b, L, B, 1, 1)[1] for b in range(B)]
+    assert lam_s == [b // n for b in range(B)]
+    for oi in (0, 1):   # equal orderings keep every element in place
+        assert [O.wavelength_ordering_calc(b, L, B, oi, oi)[0] for b in range(B)] == list(range(B))
+    # slow -> fast is the inverse of fast -> slow
+    back = [O.wavelength_ordering_calc(o, L, B, 1, 0)[0] for o in out]
+    assert back == list(range(B))
+
+
+def test_frequency_sweep_linear_is_linear_in_frequency():
+    lam = O.frequency_sweep_linear(1530e-9, 1600e-9, 8, 1565e-9)
+    f = 299792458.0 / lam.astype(np.float64)
+    df = np.diff(f)
+    assert np.allclose(df, df[0], rtol=0, atol=3e-7 * f[0])   # float32 wavelengths: ~1e-7 of f each
+    assert abs(lam[0] - 1530e-9) < 2e-15 and abs(lam[-1] - 1600e-9) < 2e-15
+    assert np.all(np.diff(lam) > 0)
+    # one step: the mean of the two ends, or the centre when neither is valid
+    assert np.isclose(O.frequency_sweep_linear(1500e-9, 1600e-9, 1, 1565e-9)[0], 1550e-9, rtol=1e-6)
+    assert O.frequency_sweep_linear(0, 0, 1, 1565e-9)[0] == np.float32(1565e-9)
+
+
+def _sweep_cfg(lams, order=(0, 0), **kw):
+    return small_cfg(wavelengths=list(lams), wavelength_ordering=order, defocus=[0.2, 0.2], **kw)
+
+
+def _sweep_residuals(frames, c_true, cfg):
+    _, coefs, _ = O.process_batch(frames, cfg)
+    L = len(O.wavelength_axis(cfg))
+    B = frames.shape[0]
+    errs = []
+    for b in range(B):
+        o = O.wavelength_ordering_calc(b, L, B, *cfg.wavelength_ordering)[0]
+        ct, cr = c_true[b, 0], coefs[o, 0]
+        alpha = np.vdot(ct, cr) / np.vdot(ct, ct)
+        errs.append(np.linalg.norm(cr - alpha * ct) / np.linalg.norm(cr))
+    return np.array(errs)
+
+
+@pytest.mark.parametrize("order", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_sweep_recovers_frames_rendered_at_each_wavelength(order):
+    lams = [1530e-9, 1565e-9, 1600e-9]
+    spec = synth.spec_small()
+    spec.defocus = [0.2, 0.2]
+    frames, c, _, _ = synth.make_sweep_batch(spec, lams, 6, seed=5, order_in=order[0], quantise=False)
+    errs = _sweep_residuals(frames, c, _sweep_cfg(lams, order))
+    assert errs.max() < 1e-5
+    # the centre wavelength alone leaves a tilt and a curvature on the off-centre frames
+    errs_c = _sweep_residuals(frames, c, small_cfg(defocus=[0.2, 0.2]))
+    lam_b = [b % 3 if not order[0] else b // 2 for b in range(6)]
+    assert all(errs_c[b] > 1e-2 for b in range(6) if lam_b[b] != 1)
+
+
+def test_sweep_aperture_follows_k0_and_the_window_spans_the_axis():
+    # the circular aperture of each wavelength is centred at k0 sin(tilt) with
+    # radius k0 sin(r): centroid and pixel count against the circle
+    def stats(lams):
+        cfg = _sweep_cfg(lams)
+        plan = O.make_plan(cfg)
+        kx = O.k_axis_f32(plan.W, cfg.pixel_size)
+        dk = float(kx[1] - kx[0])
+        out = []
+        for lam in lams:
+            t = O.pol_tables(O.apply_pol_locks(cfg), plan, 0, wavelength=lam)
+            k0 = 2 * math.pi / lam
+            cols = kx[t.col].astype(np.float64)
+            centre = (t.mask * cols[None, :]).sum() / t.mask.sum() / (k0 * math.sin(math.radians(1.5)))
+            area = t.mask.sum() / (math.pi * (k0 * math.sin(math.radians(0.7))) ** 2 / dk ** 2)
+            out.append((centre, area))
+        return plan, out
+    plan, st = stats([1540e-9, 1565e-9, 1590e-9])
+    for centre, area in st:
+        assert abs(centre - 1) < 2e-3 and abs(area - 1) < 1e-2
+    # the window is wider than either end wavelength's own window
+    for lam in (1540e-9, 1590e-9):
+        p1 = O.make_plan(small_cfg(wavelength=lam))
+        assert plan.w >= p1.w and plan.h >= p1.h
+    # the reference sizes the outer edge with the longest wavelength's k0 and the
+    # inner edge with the shortest's (digHolo.cpp:7913-7955), so a wide sweep
+    # clips the shortest wavelength's aperture on the outer side
+    _, st = stats([1450e-9, 1565e-9, 1700e-9])
+    assert st[0][1] < 0.95 and st[0][0] < 0.99
+    assert abs(st[2][1] - 1) < 1e-2
+
+
+def test_power_meter_sweep_known_answer():
+    # wavelength l: an identity transfer scaled by a_l -> IL_l = a_l^2, MDL_l = 1
+    L, M = 2, 6
+    a = np.array([0.5, 0.9])
+    c = np.zeros((L * M, 1, M), dtype=np.complex128)
+    for l in range(L):
+        for k in range(M):
+            c[k * L + l, 0, k] = a[l]          # wavelength the fast axis
+    lin = O.power_meter_sweep(c, L, 0)
+    assert np.allclose(lin[:, O.METRIC_IL], a ** 2)
+    assert np.allclose(lin[:, O.METRIC_MDL], 1.0)
+    db = O.sweep_metrics_db(lin)
+    assert np.isclose(db[O.METRIC_IL, L], 10 * np.log10(np.mean(a ** 2)))
+    c_slow = c.reshape(M, L, 1, M).transpose(1, 0, 2, 3).reshape(L * M, 1, M)
+    assert np.allclose(O.power_meter_sweep(c_slow, L, 1), lin)
