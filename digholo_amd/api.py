@@ -50,6 +50,12 @@ SIGNATURES: Dict[str, Tuple[object, List[object]]] = {
     "digHoloConfigSetBatchCalibration": (c_int, [c_int, c_void_p, c_int, c_int]),
     "digHoloConfigGetBatchCalibration": (c_void_p, [c_int, P_int, P_int]),
     "digHoloConfigSetBatchCalibrationFromFile": (c_int, [c_int, C.c_char_p, c_int, c_int]),
+    "digHoloConfigSetRefCalibrationIntensity": (c_int, [c_int, c_void_p, c_int, c_int, c_int]),
+    "digHoloConfigSetRefCalibrationField": (c_int, [c_int, c_void_p, c_int, c_int, c_int]),
+    "digHoloConfigSetRefCalibrationFromFile": (c_int, [c_int, C.c_char_p, c_int, c_int, c_int]),
+    "digHoloConfigSetRefCalibrationEnabled": (c_int, [c_int, c_int]),
+    "digHoloConfigGetRefCalibrationEnabled": (c_int, [c_int]),
+    "digHoloConfigGetRefCalibrationFields": (c_void_p, [c_int, P_int, P_int, PP_float, PP_float, P_int, P_int]),
     "digHoloConfigSetBatchCalibrationEnabled": (c_int, [c_int, c_int]),
     "digHoloConfigGetBatchCalibrationEnabled": (c_int, [c_int]),
     "digHoloConfigGetBatchCount": (c_int, [c_int]),
@@ -168,6 +174,9 @@ DIGHOLO_ERROR_INVALIDDIMENSION = 5
 DIGHOLO_ERROR_INVALIDPOLARISATION = 6
 DIGHOLO_ERROR_INVALIDAXIS = 7
 DIGHOLO_ERROR_INVALIDARGUMENT = 8
+DIGHOLO_ERROR_MEMORYALLOCATION = 9
+DIGHOLO_ERROR_FILENOTCREATED = 10
+DIGHOLO_ERROR_FILENOTFOUND = 11
 DIGHOLO_AUTOALIGNMODE_FULL = 0
 DIGHOLO_AUTOALIGNMODE_TWEAK = 1
 DIGHOLO_AUTOALIGNMODE_ESTIMATE = 2
@@ -323,6 +332,22 @@ def get_wavelengths(lib, h) -> np.ndarray:
     if not ptr or n.value <= 0:
         raise RuntimeError("digHoloConfigGetWavelengths failed")
     return np.ctypeslib.as_array(ptr, shape=(n.value,)).copy()
+
+
+def get_ref_calibration_fields(lib, h):
+    """digHoloConfigGetRefCalibrationFields -> ([L, pol, h, w] complex64, x, y) or None."""
+    L, P, w, hh = c_int(), c_int(), c_int(), c_int()
+    xp, yp = P_float(), P_float()
+    ptr = lib.digHoloConfigGetRefCalibrationFields(h, C.byref(L), C.byref(P), C.byref(xp), C.byref(yp), C.byref(w),
+                                                   C.byref(hh))
+    if not ptr:
+        return None
+    n = L.value * P.value * hh.value * w.value
+    f = np.ctypeslib.as_array(C.cast(ptr, P_float), shape=(2 * n,)).view(np.complex64)
+    f = f.reshape(L.value, P.value, hh.value, w.value).copy()
+    x = np.ctypeslib.as_array(xp, shape=(w.value,)).copy()
+    y = np.ctypeslib.as_array(yp, shape=(hh.value,)).copy()
+    return f, x, y
 
 
 def batch_summary(lib, h, plane: int):

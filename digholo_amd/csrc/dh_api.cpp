@@ -72,12 +72,22 @@ struct Config {
   std::vector<float2> basisFull;
   int basisIn = 0, basisOut = 0;
   unsigned basisVersion = 0;
+  // reference-wave calibration (digHolo.cpp:5309-5520): [L][height][width],
+  // stored inverted and normalised the way the setters leave it
+  std::vector<float2> refCal;
+  int refCalW = 0, refCalH = 0, refCalL = 0, refCalEnabled = 0;
+  unsigned refCalVersion = 0;
 };
 
+// Device and pinned buffers free themselves when their handle is deleted.
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
   hipError_t ensure(size_t count) {
     if (count <= n) return hipSuccess;
     if (p) (void)hipFree(p);
@@ -98,6 +108,10 @@ template <typename T>
 struct PinnedBuf {
   T* p = nullptr;
   size_t n = 0;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  ~PinnedBuf() { release(); }
   hipError_t ensure(size_t count) {
     if (count <= n) return hipSuccess;
     if (p) (void)hipHostFree(p);
@@ -125,6 +139,16 @@ struct Derived {
   dh::TableArgs tab{};                  // wavelength 0
   std::vector<float> lam;               // the wavelength axis, L entries
   std::vector<dh::TableArgs> tabLam;    // every wavelength, tabLam[0] == tab
+  // reference-wave calibration plane (active only when enabled and sized like the frame)
+  struct Cal {
+    bool active = false;
+    unsigned version = 0;
+    int Lc = 0;
+    int SX = 0, SY = 0;                 // Fourier window of the calibration (zero tilt)
+    std::vector<int> col[2], row[2];    // transform columns/rows each window pixel reads
+    std::vector<float> wt;              // [Lc][P][h][w] mask * envelope * norm
+    std::vector<int> outPos;            // plane position of calibration frame c
+  } cal;
   int SX[2]{}, SY[2]{};
   std::vector<uint8_t> mask;
   float pxx = 0.f, pxy = 0.f;  // float32 reconstructed pixel pitch (axes export)
@@ -253,6 +277,18 @@ struct Handle {
   bool tuning = false, capturing = false;
   unsigned tabVersion = 0;  // incremented whenever the device tables are rebuilt
 
+  // reference-wave calibration: the stored frames on the device, transform
+  // work space, the calibration plane [Lc][P][fh][fw] of the current tables
+  DevBuf<float2> refCalSrc, calA, calY, calT, calT2, calPlane;
+  DevBuf<int> calIdx;
+  DevBuf<float> calWt;
+  DevBuf<double2> calTw;
+  unsigned refCalSrcVersion = ~0u;
+  std::vector<int> calIdxHost;
+  bool calPlaneValid = false;
+  int calPlaneL = 0;
+  PinnedBuf<float2> hCal;
+
   bool stageTiming = false;
   std::vector<std::array<hipEvent_t, 6>> evPool;
   size_t evUsed = 0;
@@ -322,6 +358,104 @@ bool fft_window(const Config& c, int& Wout, int& Hout, int cx[2], int cy[2], int
   Wout = W;
   Hout = H;
   return true;
+}
+
+double xcsc_d(double t) { return t == 0.0 ? 1.0 : t / std::sin(t); }
+
+// (output position, wavelength index) of element idx of a batch of B over L
+// wavelengths (digHolo.cpp:2656-2700)
+int wavelength_ordering_calc(int idx, int L, int B, int ordIn, int ordOut, int& lam) {
+  const int subB = L > 0 ? B / L : 0;
+  int sub;
+  if (!ordIn) {
+    lam = idx % L;
+    sub = idx / L;
+  } else if (subB) {
+    sub = idx % subB;
+    lam = idx / subB;
+  } else {
+    lam = sub = 0;
+  }
+  return ordOut ? lam * subB + sub : sub * L + lam;
+}
+
+// Reference-wave calibration plan (digHolo.cpp:6792-6850, :8253-8292): the
+// calibration's Fourier window is placed for zero tilt over the calibration's
+// wavelength range with its own size (:7868-8048, complex spectrum: no
+// half-plane clamp), then the signal window's size is what is copied; each
+// window pixel is weighted with the aperture of its frame's wavelength, the
+// fill-factor envelope and the FFT/IFFT norm (:1808-2122 with no beam centre).
+void derive_cal(const Config& c, Derived& d, float kfx, float kfy) {
+  Derived::Cal& k = d.cal;
+  k = Derived::Cal{};
+  const dh::Geometry& g = d.g;
+  if (!c.refCalEnabled || c.refCalL <= 0 || c.refCalW != c.frameWidth || c.refCalH != c.frameHeight) return;
+  const int W = g.W, H = g.H, w = g.w, h = g.h, P = g.P, L = g.L;
+  k.active = true;
+  k.version = c.refCalVersion;
+  k.Lc = c.refCalL;
+  const float lamA = k.Lc == 1 ? c.wavelengthCentre : d.lam[0];
+  const float lamB = k.Lc == 1 ? c.wavelengthCentre : d.lam[L - 1];
+  const float k0max = 2 * PI_F / std::fmax(lamA, lamB);
+  const float k0min = 2 * PI_F / std::fmin(lamA, lamB);
+  const float dkx = k_axis(1, W, kfx) - k_axis(0, W, kfx);
+  const float dky = k_axis(1, H, kfy) - k_axis(0, H, kfy);
+  const float r0 = c.apertureSize * UNIT_ANGLE_F;
+  const float zero = 0.f;
+  const float maxEX = (float)((double)((zero + r0) * k0max / dkx) + W / 2.0);
+  const float maxEY = (float)((double)((zero + r0) * k0max / dky) + H / 2.0);
+  const float minSX = (float)((double)((zero - r0) * k0min / dkx) + W / 2.0);
+  const float minSY = (float)((double)((zero - r0) * k0min / dky) + H / 2.0);
+  const int wc = (int)std::round((double)(maxEX + minSX) / 4.0) * 2;
+  const int hc = (int)std::round((double)(maxEY + minSY) / 4.0) * 2;
+  int wcal = (int)(maxEX - minSX + 1.0f);
+  int hcal = (int)(maxEY - minSY + 1.0f);
+  wcal = std::max(16, (int)(16 * std::ceil((1.0f * wcal) / 16)));
+  hcal = std::max(16, (int)(16 * std::ceil((1.0f * hcal) / 16)));
+  int sx = (int)(wc - wcal / 2.0);
+  if (sx < 0) sx = 0;
+  if (sx + wcal > W) sx = W - wcal;
+  const int sy = (int)(hc - hcal / 2.0);
+  k.SX = sx;
+  k.SY = sy;
+  const float fullShift = std::fabs((k_axis(1, H, kfy) - k_axis(0, H, kfy)) * (float)H);
+  const bool lowRes = g.fw == w && g.fh == h && c.resolutionIdx == 1;
+  const float norm0 = std::sqrt((float)((1.0 * g.fw * g.fh) / (1.0 * W * H)));
+  const double ifftLen = lowRes ? (double)w * h : (double)W * H;
+  const float norm = (float)(norm0 / std::sqrt((double)W * H * ifftLen));
+  const double hp = 0.5 * (double)c.pixelSize;
+  for (int p = 0; p < P; ++p) {
+    k.col[p].resize(w);
+    k.row[p].resize(h);
+    for (int i = 0; i < w; ++i) k.col[p][i] = imod(i + sx - W / 2, W);
+    for (int j = 0; j < h; ++j) k.row[p][j] = imod(j + sy + H / 2, H);
+  }
+  k.wt.assign((size_t)k.Lc * P * h * w, 0.f);
+  k.outPos.resize(k.Lc);
+  for (int cc = 0; cc < k.Lc; ++cc) {
+    int lam = 0;
+    k.outPos[cc] = wavelength_ordering_calc(cc, L, k.Lc, g.ordIn, g.ordOut, lam);
+    if (k.outPos[cc] < 0 || k.outPos[cc] >= k.Lc) k.outPos[cc] = cc;
+    const float k0 = 2 * PI_F / d.lam[std::min(lam, L - 1)];
+    const float kr = k0 * (float)std::sin((double)r0);
+    const float r02 = kr * kr;
+    for (int p = 0; p < P; ++p)
+      for (int j = 0; j < h; ++j) {
+        const int jSY = j + sy;
+        const float shift = jSY < 0 ? fullShift : (jSY > H ? -fullShift : 0.0f);
+        const float ky = k_axis(k.row[p][j], H, kfy);
+        const float dk_y = ky - (zero + shift);
+        const float dky2 = dk_y * dk_y;
+        for (int i = 0; i < w; ++i) {
+          const float kx = k_axis(k.col[p][i], W, kfx);
+          const float dk_x = kx - zero;
+          const float dr2 = dk_x * dk_x + dky2;
+          if (!(dr2 <= r02)) continue;
+          const double env = c.fillFactor ? xcsc_d((double)kx * hp) * xcsc_d((double)ky * hp) : 1.0;
+          k.wt[(((size_t)cc * P + p) * h + j) * w + i] = (float)(env * norm);
+        }
+      }
+  }
 }
 
 // crop: keep the camera-plane FFT windows of an earlier derivation; win: keep
@@ -503,6 +637,7 @@ bool derive(const Config& c, int B, Derived& d, int& err, const Derived* crop = 
   d.tab = d.tabLam[0];
   d.pxx = (px * W) / g.fw;              // digHolo.cpp:8184-8185 (float)
   d.pxy = (px * H) / g.fh;
+  derive_cal(c, d, kfx, kfy);
   return true;
 }
 
@@ -517,6 +652,15 @@ bool same_tables(const Derived& a, const Derived& b) {
   if (a.tabLam.size() != b.tabLam.size()) return false;
   for (size_t l = 0; l < a.tabLam.size(); ++l)
     if (std::memcmp(a.tabLam[l].pp, b.tabLam[l].pp, sizeof(a.tabLam[l].pp)) != 0) return false;
+  const Derived::Cal &ca = a.cal, &cb = b.cal;
+  if (ca.active != cb.active) return false;
+  if (ca.active) {
+    if (ca.version != cb.version || ca.Lc != cb.Lc || ca.SX != cb.SX || ca.SY != cb.SY || ca.wt != cb.wt ||
+        ca.outPos != cb.outPos)
+      return false;
+    for (int p = 0; p < x.P; ++p)
+      if (x.cx[p] != y.cx[p] || x.cy[p] != y.cy[p]) return false;   // the calibration frame's crop
+  }
   return a.mask == b.mask;
 }
 
@@ -908,6 +1052,75 @@ int apply_basis(Handle& h) {
   return DIGHOLO_ERROR_SUCCESS;
 }
 
+// Calibration plane of derivation d on the device (digHolo.cpp:6729-6850,
+// :8253-8292), then folded into the reference wave: the reference multiplies
+// the inverse-transformed field by the plane of its wavelength before the tilt
+// is removed (:8736-8758, :8770-8785), and V = -conj(E C R) = -conj(E (C R)).
+int build_ref_calibration(Handle& h, const Derived& d) {
+  const Config& c = h.cfg;
+  const dh::Geometry& g = d.g;
+  const Derived::Cal& k = d.cal;
+  const int P = g.P, W = g.W, H = g.H, w = g.w, hh = g.h, fw = g.fw, fh = g.fh, Lc = k.Lc;
+  const size_t frame = (size_t)c.frameWidth * c.frameHeight;
+  if (h.refCalSrcVersion != c.refCalVersion) {
+    DH_TRY(h.refCalSrc.ensure(c.refCal.size()), "alloc");
+    DH_TRY(hipMemcpyAsync(h.refCalSrc.p, c.refCal.data(), c.refCal.size() * sizeof(float2), hipMemcpyHostToDevice,
+                          h.stream), "calibration upload");
+    DH_TRY(hipStreamSynchronize(h.stream), "calibration upload");
+    h.refCalSrcVersion = c.refCalVersion;
+  }
+  // column lists [P][w], row lists [P][h]
+  h.calIdxHost.assign((size_t)P * (w + hh), 0);
+  for (int p = 0; p < P; ++p) {
+    std::copy(k.col[p].begin(), k.col[p].end(), h.calIdxHost.begin() + (size_t)p * w);
+    std::copy(k.row[p].begin(), k.row[p].end(), h.calIdxHost.begin() + (size_t)P * w + (size_t)p * hh);
+  }
+  DH_TRY(h.calIdx.ensure(h.calIdxHost.size()), "alloc");
+  DH_TRY(h.calWt.ensure(k.wt.size()), "alloc");
+  DH_TRY(hipMemcpyAsync(h.calIdx.p, h.calIdxHost.data(), h.calIdxHost.size() * sizeof(int), hipMemcpyHostToDevice,
+                        h.stream), "calibration upload");
+  DH_TRY(hipMemcpyAsync(h.calWt.p, k.wt.data(), k.wt.size() * sizeof(float), hipMemcpyHostToDevice, h.stream),
+         "calibration upload");
+  // twiddles of the four transform lengths W, H, fw, fh
+  const int Ns[4] = {W, H, fw, fh};
+  size_t off[4], tot = 0;
+  for (int q = 0; q < 4; ++q) {
+    off[q] = tot;
+    tot += Ns[q];
+  }
+  DH_TRY(h.calTw.ensure(tot), "alloc");
+  for (int q = 0; q < 4; ++q) DH_TRY(dh::launch_twiddle_f64(Ns[q], h.calTw.p + off[q], h.stream), "twiddles");
+  const size_t nPlane = (size_t)fh * fw;
+  DH_TRY(h.calA.ensure((size_t)H * w), "alloc");
+  DH_TRY(h.calY.ensure((size_t)hh * w), "alloc");
+  DH_TRY(h.calT.ensure(nPlane), "alloc");
+  DH_TRY(h.calT2.ensure(nPlane), "alloc");
+  DH_TRY(h.calPlane.ensure((size_t)Lc * P * nPlane), "alloc");
+  DH_TRY(dh::launch_zero_complex(h.calPlane.p, (size_t)Lc * P * nPlane, h.stream), "calibration plane");
+  const bool lowRes = c.resolutionIdx == 1;
+  for (int cc = 0; cc < Lc; ++cc)
+    for (int p = 0; p < P; ++p) {
+      const float2* src = h.refCalSrc.p + cc * frame + (size_t)g.cy[p] * c.frameWidth + g.cx[p];
+      const int* cols = h.calIdx.p + (size_t)p * w;
+      const int* rows = h.calIdx.p + (size_t)P * w + (size_t)p * hh;
+      DH_TRY(dh::launch_cdft_axis(src, c.frameWidth, 1, h.calA.p, w, 1, H, W, w, cols, -1, h.calTw.p + off[0],
+                                  h.stream), "calibration transform");
+      DH_TRY(dh::launch_cdft_axis(h.calA.p, 1, w, h.calY.p, 1, w, w, H, hh, rows, -1, h.calTw.p + off[1], h.stream),
+             "calibration transform");
+      DH_TRY(dh::launch_cal_place(h.calY.p, h.calWt.p + ((size_t)cc * P + p) * hh * w, w, hh, fw, fh, lowRes ? 1 : 0,
+                                  k.SX, k.SY, W, H, h.calT.p, h.stream), "calibration window");
+      DH_TRY(dh::launch_cdft_axis(h.calT.p, fw, 1, h.calT2.p, fw, 1, fh, fw, fw, nullptr, +1, h.calTw.p + off[2],
+                                  h.stream), "calibration inverse transform");
+      float2* dst = h.calPlane.p + ((size_t)k.outPos[cc] * P + p) * nPlane;
+      DH_TRY(dh::launch_cdft_axis(h.calT2.p, 1, fw, dst, 1, fw, fw, fh, fh, nullptr, +1, h.calTw.p + off[3], h.stream),
+             "calibration inverse transform");
+    }
+  DH_TRY(dh::launch_cmul_planes(h.ref.p, h.calPlane.p, nPlane, g.L * P, Lc * P, h.stream), "calibrated reference");
+  h.calPlaneValid = true;
+  h.calPlaneL = Lc;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
 // Queues one full batch on the handle's stream.  On return the coefficients and
 // fields are (or will be, once the stream drains) in device memory.
 void publish_batch(Handle& h, const Derived& d, const void* dframes, bool u16);
@@ -1004,6 +1217,9 @@ int enqueue_batch(Handle& h, int step = 0) {
     for (size_t l = 0; l < L; ++l)
       DH_TRY(dh::launch_tables(d.tabLam[l], h.mask.p + T * l, h.factor.p + T * l, h.ref.p + TR * l,
                                l ? nullptr : h.hgx.p, l ? nullptr : h.hgy.p, h.stream), "tables");
+    h.calPlaneValid = false;
+    if (d.cal.active)
+      if (const int ec = build_ref_calibration(h, d)) return ec;
     DH_TRY(h.colTab.ensure(dh::coldft_table_elems(g)), "alloc");
     DH_TRY(dh::launch_coldft_table(g, h.colTab.p, h.stream), "column table");
     if (const size_t n3 = dh::window_v3_table_elems(g)) {
@@ -3083,6 +3299,158 @@ EXT_C int digHoloConfigSetBatchCalibrationEnabled(int handleIdx, int enabled) {
 EXT_C int digHoloConfigGetBatchCalibrationEnabled(int handleIdx) {
   H_OR(0);
   return CFG.calEnabled;
+}
+
+// ---- reference-wave calibration (digHolo.cpp:5309-5520, :15361-15550) ----
+// Intensity: a = sqrt(1 / (I / max I)), zero where that is not finite.
+EXT_C int digHoloConfigSetRefCalibrationIntensity(int handleIdx, unsigned short* cal, int wavelengthCount, int width,
+                                                  int height) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  Config& c = CFG;
+  if (!cal) {
+    c.refCalEnabled = 0;
+    return DIGHOLO_ERROR_NULLPOINTER;
+  }
+  if (width <= 0 || height <= 0 || wavelengthCount <= 0) {
+    c.refCalEnabled = 0;
+    return DIGHOLO_ERROR_INVALIDARGUMENT;
+  }
+  const size_t n = (size_t)width * height * wavelengthCount;
+  int maxV = 0;
+  for (size_t i = 0; i < n; ++i) maxV = std::max<int>(maxV, cal[i]);
+  const float maxScale = 1.0f / (float)maxV;
+  c.refCal.resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    float v = 1.0f / ((float)cal[i] * maxScale);
+    if (std::isnan(v) || std::isinf(v)) v = 0;
+    c.refCal[i] = make_float2(std::sqrt(v), 0.f);
+  }
+  c.refCalW = width;
+  c.refCalH = height;
+  c.refCalL = wavelengthCount;
+  ++c.refCalVersion;
+  c.refCalEnabled = 1;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+// Field: normalised to a peak magnitude of 1, then each part inverted as the
+// reference does it, (1 / re, -1 / im), zero where either is not finite.
+EXT_C int digHoloConfigSetRefCalibrationField(int handleIdx, complex64* cal, int wavelengthCount, int width,
+                                              int height) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  Config& c = CFG;
+  if (!cal) {
+    c.refCalEnabled = 0;
+    return DIGHOLO_ERROR_NULLPOINTER;
+  }
+  if (width <= 0 || height <= 0 || wavelengthCount <= 0) {
+    c.refCalEnabled = 0;
+    return DIGHOLO_ERROR_INVALIDARGUMENT;
+  }
+  const size_t n = (size_t)width * height * wavelengthCount;
+  const float2* z = reinterpret_cast<const float2*>(cal);
+  float maxV = 0;
+  for (size_t i = 0; i < n; ++i) maxV = std::max(maxV, z[i].x * z[i].x + z[i].y * z[i].y);
+  maxV = std::sqrt(maxV);
+  const float maxScale = 1.0f / maxV;
+  c.refCal.resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    float vR = 1.0f / (z[i].x * maxScale);
+    float vI = -1.0f / (z[i].y * maxScale);
+    if (std::isnan(vR) || std::isinf(vR) || std::isnan(vI) || std::isinf(vI)) vR = vI = 0;
+    c.refCal[i] = make_float2(vR, vI);
+  }
+  c.refCalW = width;
+  c.refCalH = height;
+  c.refCalL = wavelengthCount;
+  ++c.refCalVersion;
+  c.refCalEnabled = 1;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
+// The file holds the intensity (uint16) or the field (complex64) form; which
+// one follows from its size (digHolo.cpp:5320-5417, including its rules for a
+// file of one frame).
+EXT_C int digHoloConfigSetRefCalibrationFromFile(int handleIdx, const char* fname, int wavelengthCount, int width,
+                                                 int height) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  FILE* f = fname ? std::fopen(fname, "rb") : nullptr;
+  if (!f) {
+    CFG.refCalEnabled = 0;
+    return DIGHOLO_ERROR_FILENOTFOUND;
+  }
+  std::fseek(f, 0, SEEK_END);
+  const long long bytes = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  int L = wavelengthCount;
+  const long long frame16 = (long long)width * height * (long long)sizeof(unsigned short);
+  const long long frame32 = (long long)width * height * (long long)sizeof(float2);
+  long long total16 = frame16 * L, total32 = frame32 * L;
+  if (bytes == frame16) {
+    L = 1;
+    total16 = frame16;
+  } else if (bytes == frame32 && L != 2) {
+    L = 1;
+    total32 = frame32;
+  }
+  int e = DIGHOLO_ERROR_SUCCESS;
+  if (bytes == total16 && bytes > 0) {
+    std::vector<unsigned short> buf((size_t)(bytes / 2));
+    const size_t got = std::fread(buf.data(), 2, buf.size(), f);
+    e = got == buf.size() ? digHoloConfigSetRefCalibrationIntensity(handleIdx, buf.data(), L, width, height)
+                          : DIGHOLO_ERROR_INVALIDDIMENSION;
+  } else if (bytes == total32 && bytes > 0) {
+    std::vector<float2> buf((size_t)(bytes / 8));
+    const size_t got = std::fread(buf.data(), 8, buf.size(), f);
+    e = got == buf.size()
+            ? digHoloConfigSetRefCalibrationField(handleIdx, reinterpret_cast<complex64*>(buf.data()), L, width, height)
+            : DIGHOLO_ERROR_INVALIDDIMENSION;
+  } else {
+    e = DIGHOLO_ERROR_INVALIDDIMENSION;
+  }
+  std::fclose(f);
+  if (e) CFG.refCalEnabled = 0;
+  return e;
+}
+
+EXT_C int digHoloConfigSetRefCalibrationEnabled(int handleIdx, int enabled) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  CFG.refCalEnabled = enabled ? 1 : 0;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C int digHoloConfigGetRefCalibrationEnabled(int handleIdx) {
+  H_OR(0);
+  return CFG.refCalEnabled;
+}
+
+// The calibration plane the last batch applied, [wavelengthCount][pol][height][width]
+// on the field plane's axes; null before a calibrated batch.
+EXT_C complex64* digHoloConfigGetRefCalibrationFields(int handleIdx, int* wavelengthCount, int* polCount, float** x,
+                                                      float** y, int* width, int* height) {
+  Handle* h = get(handleIdx);
+  auto none = [&]() -> complex64* {
+    if (wavelengthCount) *wavelengthCount = 0;
+    if (polCount) *polCount = 0;
+    if (x) *x = nullptr;
+    if (y) *y = nullptr;
+    if (width) *width = 0;
+    if (height) *height = 0;
+    return nullptr;
+  };
+  if (!h || !h->calPlaneValid || !h->haveBatch || !h->cur.cal.active) return none();
+  const dh::Geometry& g = h->cur.g;
+  const size_t n = (size_t)h->calPlaneL * g.P * g.fh * g.fw;
+  if (h->hCal.ensure(n) != hipSuccess) return none();
+  if (hipMemcpyAsync(h->hCal.p, h->calPlane.p, n * sizeof(float2), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+      hipStreamSynchronize(h->stream) != hipSuccess)
+    return none();
+  if (wavelengthCount) *wavelengthCount = h->calPlaneL;
+  if (polCount) *polCount = g.P;
+  if (x) *x = h->xAxis.data();
+  if (y) *y = h->yAxis.data();
+  if (width) *width = g.fw;
+  if (height) *height = g.fh;
+  return reinterpret_cast<complex64*>(h->hCal.p);
 }
 
 // ---- windows (digHolo.cpp:14234-14330, :13518-13540, :14414-14436, :14589-14611) ----

@@ -243,4 +243,19 @@ hipError_t launch_overlap_mfma(const Geometry& g, const float2* fields, const fl
                                float2* coefs, int split, hipStream_t s, int tpw = 0);
 int overlap_min_tiles_per_wave(const Geometry& g);
 
+// Reference-wave calibration plane (dh_refcal.hip).
+// tw[m] = exp(-2 pi i m / N), FP64, m < N
+hipError_t launch_twiddle_f64(int N, double2* tw, hipStream_t s);
+// out(o, k) = sum_{n < N} in(o, n) exp(sign 2 pi i f_k n / N) with in(o, n) at
+// in[o inO + n inN], out(o, k) at out[o outO + k outK]; fk == nullptr: f_k = k
+hipError_t launch_cdft_axis(const float2* in, size_t inO, size_t inN, float2* out, size_t outO, size_t outK,
+                            int nOuter, int N, int K, const int* fk, int sign, const double2* tw, hipStream_t s);
+// weighted window Y [h][w] * wt [h][w] into a zeroed fh x fw field plane
+hipError_t launch_cal_place(const float2* Y, const float* wt, int w, int h, int fw, int fh, int lowRes, int SX, int SY,
+                            int W, int H, float2* plane, hipStream_t s);
+// a[0..n) = 0 (a kernel, so it may be captured in a graph)
+hipError_t launch_zero_complex(float2* a, size_t n, hipStream_t s);
+// a[q] *= b[q % nb] for nq planes of n elements
+hipError_t launch_cmul_planes(float2* a, const float2* b, size_t n, int nq, int nb, hipStream_t s);
+
 }  // namespace dh

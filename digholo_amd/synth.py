@@ -72,6 +72,9 @@ class HologramSpec:
     # Gauss-Legendre points per axis for integrating the intensity over each
     # pixel (100% fill factor); 0 samples the pixel centre
     pixel_quadrature: int = 0
+    # reference-wave amplitude envelope exp(-r^2 / w_r^2) about the frame centre
+    # (a non-uniform reference, the case a reference calibration corrects); None = flat
+    ref_envelope_waist: Optional[float] = None
 
 
 def random_coefs(spec: HologramSpec, batch: int, rng: np.random.Generator) -> np.ndarray:
@@ -149,6 +152,7 @@ def _render_points(spec, coefs, x, y, half, chunk, amps_in):
         ry = (y - spec.beam_centre_y[p]) ** 2
         carrier = (np.exp(1j * (k0 * thy * y + curv * ry))[:, None] *
                    np.exp(1j * (k0 * thx * xs + curv * rx))[None, :])
+        carrier = carrier * reference_envelope(spec, xs, y)
         amp_p = None if amps_in is None else amps_in[p]
         for b0 in range(0, B, chunk):
             b1 = min(B, b0 + chunk)
@@ -166,6 +170,24 @@ def _render_points(spec, coefs, x, y, half, chunk, amps_in):
             intens[b0:b1, :, p * half:(p + 1) * half] = np.abs(tot) ** 2
         amps.append(amp_p)
     return intens, amps
+
+
+def reference_envelope(spec: HologramSpec, x: np.ndarray, y: np.ndarray) -> np.ndarray:
+    """Reference-wave amplitude [len(y), len(x)] (1 without an envelope)."""
+    if spec.ref_envelope_waist is None:
+        return np.ones((len(y), len(x)))
+    wr2 = spec.ref_envelope_waist ** 2
+    return np.exp(-(y[:, None] ** 2 + x[None, :] ** 2) / wr2)
+
+
+def reference_intensity_frame(spec: HologramSpec, peak: float = 4000.0) -> np.ndarray:
+    """The reference wave alone on the camera as uint16 counts (peak counts at its
+    maximum): the frame a reference-intensity calibration is taken from."""
+    Fw, Fh, px = spec.frame_width, spec.frame_height, spec.pixel_size
+    x = (np.arange(Fw) - Fw // 2) * px
+    y = (np.arange(Fh) - Fh // 2) * px
+    I = reference_envelope(spec, x, y) ** 2
+    return np.clip(np.rint(I * (peak / I.max())), 0, 65535).astype(np.uint16)
 
 
 def make_batch(spec: HologramSpec, batch: int, seed: int = 0, quantise: bool = True):

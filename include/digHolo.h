@@ -126,6 +126,20 @@ EXT_C complex64* digHoloConfigGetBatchCalibration(int handleIdx, int* polCount, 
 EXT_C int digHoloConfigSetBatchCalibrationFromFile(int handleIdx, const char* fname, int polCount, int batchCount); /* digHolo.h:624 */
 EXT_C int digHoloConfigSetBatchCalibrationEnabled(int handleIdx, int enabled);    /* digHolo.h:635 */
 EXT_C int digHoloConfigGetBatchCalibrationEnabled(int handleIdx);                 /* digHolo.h:645 */
+/* Reference-wave calibration: frames (wavelengthCount x height x width) of the
+   reference wave alone, as intensity or field; applied, low-passed through the
+   field's own Fourier window, to every reconstructed field whose frame has the
+   same dimensions.  A calibration of one frame serves every wavelength. */
+EXT_C int digHoloConfigSetRefCalibrationIntensity(int handleIdx, unsigned short* cal, int wavelengthCount, int width,
+                                                  int height);                      /* digHolo.h:369 */
+EXT_C int digHoloConfigSetRefCalibrationField(int handleIdx, complex64* cal, int wavelengthCount, int width,
+                                              int height);                          /* digHolo.h:384 */
+EXT_C int digHoloConfigSetRefCalibrationFromFile(int handleIdx, const char* fname, int wavelengthCount, int width,
+                                                 int height);                       /* digHolo.h:400 */
+EXT_C int digHoloConfigSetRefCalibrationEnabled(int handleIdx, int enabled);      /* digHolo.h:409 */
+EXT_C int digHoloConfigGetRefCalibrationEnabled(int handleIdx);                   /* digHolo.h:417 */
+EXT_C complex64* digHoloConfigGetRefCalibrationFields(int handleIdx, int* wavelengthCount, int* polCount, float** x,
+                                                      float** y, int* width, int* height); /* digHolo.h:437 */
 EXT_C int digHoloConfigGetBatchCount(int handleIdx);                              /* digHolo.h:559 */
 EXT_C int digHoloConfigSetBatchAvgCount(int handleIdx, int avgCount);             /* digHolo.h:570 */
 EXT_C int digHoloConfigGetBatchAvgCount(int handleIdx);                           /* digHolo.h:580 */

@@ -20,6 +20,7 @@ What it restates (all citations are digHolo.cpp unless noted):
 * separable overlap c = sum_y HGy sum_x HGx V ...... :2149-2341, :9348-9467
 * power-meter metrics (IL, MDL, diag, SNR) ........ :10714-11129
 * wavelength sweeps (axis, ordering, per-lambda) .. :2656-2700, :6577-6680, :7823-7865, :8644-8670, :12172-12191
+* reference-wave calibration ...................... :5309-5555, :6729-6850, :8253-8292, :8736-8785
 * plane analysis (power, COM, peak, Aeff) ......... :2354-2654, :8990-9188
 
 Numerical contract
@@ -192,12 +193,12 @@ def frequency_sweep_linear(lam_start: float, lam_stop: float, count: int, centre
     return np.array([F32(c0 / F32(f_start + F32(df * F32(i)))) for i in range(count)], dtype=np.float32)
 
 
-def ifft_window(cfg: Config, W: int, H: int) -> Tuple[int, int, List[int], List[int]]:
-    """IFFT window size and per-pol corner (low-res, r2c).  digHolo.cpp:7868-8048:
-    wide enough for the longest and the shortest wavelength of the axis."""
-    lam = wavelength_axis(cfg)
-    k0max = F32(F32(2.0) * PI_F32 / max(lam[0], lam[-1]))
-    k0min = F32(F32(2.0) * PI_F32 / min(lam[0], lam[-1]))
+def _window_bounds(cfg: Config, W: int, H: int, lam_a, lam_b, tilts, r2c: bool):
+    """IFFT window size and per-pol corner, digHolo.cpp:7868-8048: the outer edge
+    at k0 of the longer end wavelength, the inner edge at k0 of the shorter;
+    tilts [(tx_deg, ty_deg)] per pol; r2c keeps the window in the kx >= 0 half."""
+    k0max = F32(F32(2.0) * PI_F32 / max(F32(lam_a), F32(lam_b)))
+    k0min = F32(F32(2.0) * PI_F32 / min(F32(lam_a), F32(lam_b)))
     kx = k_axis_f32(W, cfg.pixel_size)
     ky = k_axis_f32(H, cfg.pixel_size)
     dkx = F32(kx[1] - kx[0])
@@ -206,10 +207,9 @@ def ifft_window(cfg: Config, W: int, H: int) -> Tuple[int, int, List[int], List[
     wc, hc = [], []
     w = 0
     h = 0
-    for p in range(cfg.pol_count):
-        pp = 0 if cfg.pol_lock_tilt else p
-        tx = tilt_rad_f32(cfg.tilt_x[pp])
-        ty = tilt_rad_f32(cfg.tilt_y[pp])
+    for tx_deg, ty_deg in tilts:
+        tx = tilt_rad_f32(tx_deg)
+        ty = tilt_rad_f32(ty_deg)
         maxEX = F32(float(F32(F32(tx + r0) * k0max) / dkx) + W / 2.0)
         maxEY = F32(float(F32(F32(ty + r0) * k0max) / dky) + H / 2.0)
         minSX = F32(float(F32(F32(tx - r0) * k0min) / dkx) + W / 2.0)
@@ -225,9 +225,9 @@ def ifft_window(cfg: Config, W: int, H: int) -> Tuple[int, int, List[int], List[
     w = max(w, 16)
     h = max(h, 16)
     SX, SY = [], []
-    for p in range(cfg.pol_count):
+    for p in range(len(tilts)):
         sx = int(wc[p] - w / 2.0)
-        if sx < W // 2:
+        if r2c and sx < W // 2:
             sx = W // 2
         if sx < 0:
             sx = 0
@@ -237,6 +237,15 @@ def ifft_window(cfg: Config, W: int, H: int) -> Tuple[int, int, List[int], List[
         SX.append(sx)
         SY.append(sy)
     return w, h, SX, SY
+
+
+def ifft_window(cfg: Config, W: int, H: int) -> Tuple[int, int, List[int], List[int]]:
+    """IFFT window size and per-pol corner (low-res, r2c).  digHolo.cpp:7868-8048:
+    wide enough for the longest and the shortest wavelength of the axis."""
+    lam = wavelength_axis(cfg)
+    tilts = [(cfg.tilt_x[0 if cfg.pol_lock_tilt else p], cfg.tilt_y[0 if cfg.pol_lock_tilt else p])
+             for p in range(cfg.pol_count)]
+    return _window_bounds(cfg, W, H, lam[0], lam[-1], tilts, r2c=True)
 
 
 @dataclass
@@ -453,6 +462,101 @@ def pol_tables(cfg: Config, plan: Plan, p: int, mask=None, wavelength=None) -> P
     return PolTables(col=col, row=row, factor=factor, ref=ref, hgx=hgx, hgy=hgy, mask=inside)
 
 
+# ---------------------------------------------------------------------------
+# Reference-wave calibration (digHolo.cpp:5309-5555, :6729-6850, :8253-8292,
+# :8736-8785 and the complex branch of the window copy, :1808-2122 with r2c = 0)
+# ---------------------------------------------------------------------------
+
+def ref_cal_from_intensity(cal: np.ndarray) -> np.ndarray:
+    """Stored form of an intensity calibration (:5423-5480), float32:
+    sqrt(1 / (I / max I)), zero where that is not finite."""
+    I = np.asarray(cal, dtype=np.uint16).astype(np.float32)
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        scale = F32(1.0) / F32(int(I.max()))
+        v = F32(1.0) / (I * scale)
+    v = np.where(np.isfinite(v), v, F32(0)).astype(np.float32)
+    return np.sqrt(v).astype(np.complex64)
+
+
+def ref_cal_from_field(cal: np.ndarray) -> np.ndarray:
+    """Stored form of a field calibration (:5484-5550), float32: normalised to a
+    peak magnitude of 1, then (1 / re, -1 / im), zero where either is not finite."""
+    z = np.asarray(cal, dtype=np.complex64)
+    re, im = z.real.astype(np.float32), z.imag.astype(np.float32)
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        mx = F32(np.sqrt(F32((re * re + im * im).max())))
+        scale = F32(1.0) / mx
+        vr = F32(1.0) / (re * scale)
+        vi = F32(-1.0) / (im * scale)
+    bad = ~(np.isfinite(vr) & np.isfinite(vi))
+    vr[bad] = 0
+    vi[bad] = 0
+    return (vr + 1j * vi).astype(np.complex64)
+
+
+def ref_cal_planes(src: np.ndarray, cfg: Config, plan: Plan) -> np.ndarray:
+    """Calibration planes [Lc, pol, fh, fw] on the field plane.  Each stored frame
+    is cropped to the pol's FFT window, transformed (complex, unnormalised), its
+    Fourier window -- placed for zero tilt with its own size over the
+    calibration's wavelength range, copied with the field window's size -- is
+    weighted with the aperture of the frame's wavelength, the fill-factor
+    envelope and 1 / (W H), shifted to zero frequency (low resolution) or kept
+    at its own frequencies (full resolution) and inverse transformed."""
+    src = np.asarray(src)
+    Lc = src.shape[0]
+    W, H, w, h = plan.W, plan.H, plan.w, plan.h
+    P = cfg.pol_count
+    lams = wavelength_axis(cfg)
+    L = len(lams)
+    lam_a, lam_b = (cfg.wavelength, cfg.wavelength) if Lc == 1 else (lams[0], lams[-1])
+    _, _, SXc, SYc = _window_bounds(cfg, W, H, lam_a, lam_b, [(0.0, 0.0)] * P, r2c=False)
+    sx, sy = SXc[0], SYc[0]
+    low = cfg.resolution_mode == 1
+    fw, fh = (w, h) if low else (W, H)
+    kxa = k_axis_f32(W, cfg.pixel_size)
+    kya = k_axis_f32(H, cfg.pixel_size)
+    col = np.mod(np.arange(w) + sx - W // 2, W)
+    jSY = np.arange(h) + sy
+    row = np.mod(jSY + H // 2, H)
+    full_shift = F32(abs(F32(F32(kya[1] - kya[0]) * F32(H))))
+    shift = np.where(jSY < 0, full_shift, np.where(jSY > H, -full_shift, F32(0))).astype(np.float32)
+    kx = kxa[col]
+    ky = kya[row]
+    dkx = (kx - F32(0)).astype(np.float32)
+    dky = (ky - (F32(0) + shift)).astype(np.float32)
+    dr2 = ((dkx * dkx)[None, :] + (dky * dky)[:, None]).astype(np.float32)
+    r32 = F32(F32(cfg.aperture_deg) * UNIT_ANGLE_F32)
+    px = float(F32(cfg.pixel_size))
+    if cfg.fill_factor:
+        def xcsc(t):
+            out = np.ones_like(t)
+            nz = t != 0
+            out[nz] = t[nz] / np.sin(t[nz])
+            return out
+        env = xcsc(ky.astype(np.float64) * px / 2)[:, None] * xcsc(kx.astype(np.float64) * px / 2)[None, :]
+    else:
+        env = np.ones((h, w))
+    norm = 1.0 / (W * H)
+    planes = np.zeros((Lc, P, fh, fw), dtype=np.complex128)
+    for c in range(Lc):
+        out, lam_i, _ = wavelength_ordering_calc(c, L, Lc, *cfg.wavelength_ordering)
+        k0 = F32(F32(2.0) * PI_F32 / lams[min(lam_i, L - 1)])
+        kr = F32(k0 * F32(math.sin(float(r32))))
+        inside = dr2 <= F32(kr * kr)
+        wt = inside * env * norm
+        for p in range(P):
+            crop = src[c, plan.cy[p]:plan.cy[p] + H, plan.cx[p]:plan.cx[p] + W].astype(np.complex128)
+            F = np.fft.fft2(crop)
+            Y = F[row[:, None], col[None, :]] * wt
+            Pl = np.zeros((fh, fw), dtype=np.complex128)
+            if low:
+                Pl[np.mod(np.arange(h) + h // 2, h)[:, None], np.mod(np.arange(w) + w // 2, w)[None, :]] = Y
+            else:
+                Pl[np.mod(jSY + H // 2, H)[:, None], np.mod(np.arange(w) + sx + W // 2, W)[None, :]] = Y
+            planes[out if 0 <= out < Lc else c, p] = np.fft.ifft2(Pl) * (fw * fh)
+    return planes
+
+
 def fft_plane(window: np.ndarray) -> np.ndarray:
     """Unnormalised forward 2-D R2C DFT (FFTW r2c convention, digHolo.cpp:7502)."""
     return np.fft.rfft2(np.asarray(window, dtype=np.float64))
@@ -507,14 +611,18 @@ def hold_of(cfg: Config) -> Hold:
 
 
 def process_batch(frames: np.ndarray, cfg: Config, emulate_int16: bool = False, avg_count: int = 1,
-                  avg_mode: int = 0, batch_cal=None, transpose: bool = False, hold: Hold = None):
+                  avg_mode: int = 0, batch_cal=None, transpose: bool = False, hold: Hold = None,
+                  ref_cal=None):
     """Full ``digHoloProcessBatch`` path.
 
     frames: [B * avg_count, frameH, frameW] (uint16 or float).  Returns (fields,
     coefs, plan) with fields [B, pol, h, w] complex128 and coefs [B, pol, M]
     complex128.  batch_cal [calP, calB] complex multiplies the Fourier window of
     batch element b, pol p by batch_cal[p % calP, b % calB] (digHolo.cpp:1831-1840,
-    :8639-8642); transpose applies the uint16 transposed-input conversion.
+    :8639-8642); transpose applies the uint16 transposed-input conversion.  ref_cal
+    [Lc, frameH, frameW] complex is a stored reference-wave calibration
+    (ref_cal_from_*): the inverse-transformed field of wavelength l is multiplied
+    by its plane l mod Lc before the tilt is removed (digHolo.cpp:8736-8758).
     """
     frames = np.asarray(frames)
     if transpose:
@@ -538,6 +646,9 @@ def process_batch(frames: np.ndarray, cfg: Config, emulate_int16: bool = False, 
     tabs_l = [[pol_tables(cfg, plan, p, mask=hold.masks[p] if hold is not None else None, wavelength=lams[l])
                for p in range(P)] for l in range(L)]
     tabs = tabs_l[0]
+    cal_planes = None
+    if ref_cal is not None and np.asarray(ref_cal).shape[1:] == (cfg.frame_height, cfg.frame_width):
+        cal_planes = ref_cal_planes(ref_cal, cfg, plan)
     low = cfg.resolution_mode == 1
     fh, fw = (plan.h, plan.w) if low else (plan.H, plan.W)
     fields = np.zeros((B, P, fh, fw), dtype=np.complex128)
@@ -548,6 +659,8 @@ def process_batch(frames: np.ndarray, cfg: Config, emulate_int16: bool = False, 
         F = np.fft.rfft2(win.astype(np.float64), axes=(-2, -1))
         factor = np.stack([tabs_l[l][p].factor for l in range(L)])      # the window itself is the same
         ref = np.stack([tabs_l[l][p].ref for l in range(L)])
+        if cal_planes is not None:
+            ref = ref * np.stack([cal_planes[l % cal_planes.shape[0], p] for l in range(L)])
         Fw = F[:B * A, t.row[:, None], t.col[None, :]]
         Fw = Fw * (factor[lam_b] if A == 1 else factor[0][None])
         if batch_cal is not None:

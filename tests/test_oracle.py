@@ -878,3 +878,70 @@ def test_power_meter_sweep_known_answer():
     assert np.isclose(db[O.METRIC_IL, L], 10 * np.log10(np.mean(a ** 2)))
     c_slow = c.reshape(M, L, 1, M).transpose(1, 0, 2, 3).reshape(L * M, 1, M)
     assert np.allclose(O.power_meter_sweep(c_slow, L, 1), lin)
+
+
+# ---------------------------------------------------------------------------
+# reference-wave calibration (digHolo.cpp:5309-5555, :6729-6850, :8253-8292)
+# ---------------------------------------------------------------------------
+
+def test_ref_cal_stored_forms():
+    # intensity: sqrt(max / I), zero where I = 0
+    st = O.ref_cal_from_intensity(np.array([[[0, 100, 400]]], dtype=np.uint16))
+    assert np.allclose(st, [[[0, 2, 1]]])
+    # field: peak magnitude 1, then (1 / re, -1 / im); a zero part zeroes the pixel
+    z = np.array([[[3 + 4j, 1 + 0j, 0.5 - 0.5j]]], dtype=np.complex64)
+    st = O.ref_cal_from_field(z)
+    assert np.allclose(st[0, 0, 0], 5 / 3 - 1j * 5 / 4)
+    assert st[0, 0, 1] == 0
+    assert np.allclose(st[0, 0, 2], 10 - 1j * (-10))
+
+
+@pytest.mark.parametrize("resolution_mode", [1, 0])
+def test_uniform_ref_calibration_is_the_identity(resolution_mode):
+    cfg = small_cfg(resolution_mode=resolution_mode)
+    plan = O.make_plan(cfg)
+    flat = O.ref_cal_from_intensity(np.full((1, 256, 320), 1234, np.uint16))
+    planes = O.ref_cal_planes(flat, cfg, plan)
+    assert np.allclose(planes, 1.0, atol=1e-12)          # only the zero frequency survives, with gain 1
+    spec = synth.spec_small()
+    frames, _, _, _ = synth.make_batch(spec, 2, seed=3, quantise=False)
+    f0, c0, _ = O.process_batch(frames, cfg)
+    f1, c1, _ = O.process_batch(frames, cfg, ref_cal=flat)
+    assert O.relative_l2(c1, c0) < 1e-12 and O.relative_l2(f1, f0) < 1e-12
+
+
+@pytest.mark.parametrize("resolution_mode", [1, 0])
+def test_intensity_calibration_undoes_a_nonuniform_reference(resolution_mode):
+    spec = synth.spec_small()
+    spec.ref_envelope_waist = 4e-3
+    cfg = small_cfg(resolution_mode=resolution_mode)
+    frames, c_true, _, _ = synth.make_batch(spec, 3, seed=8, quantise=False)
+    cal = O.ref_cal_from_intensity(synth.reference_intensity_frame(spec)[None])
+
+    def worst(coefs):
+        errs = []
+        for b in range(3):
+            ct, cr = c_true[b, 0], coefs[b, 0]
+            alpha = np.vdot(ct, cr) / np.vdot(ct, ct)
+            errs.append(np.linalg.norm(cr - alpha * ct) / np.linalg.norm(cr))
+        return max(errs)
+    e_raw = worst(O.process_batch(frames, cfg)[1])
+    e_cal = worst(O.process_batch(frames, cfg, ref_cal=cal)[1])
+    print(e_raw, e_cal)
+    assert e_raw > 1e-3 and e_cal < e_raw / 20
+    # multiplying by the envelope instead of dividing by it makes things worse
+    wrong = (1.0 / np.where(np.abs(cal) > 0, cal, 1)).astype(np.complex64)
+    assert worst(O.process_batch(frames, cfg, ref_cal=wrong)[1]) > e_raw
+
+
+def test_constant_field_calibration_scales_by_the_conjugate():
+    cfg = small_cfg()
+    z = np.full((1, 256, 320), 0.6 + 0.8j, np.complex64)
+    cal = O.ref_cal_from_field(z)
+    C = complex(cal[0, 0, 0])
+    assert np.isclose(C, 1 / 0.6 - 1j / 0.8, rtol=1e-6)
+    spec = synth.spec_small()
+    frames, _, _, _ = synth.make_batch(spec, 2, seed=4, quantise=False)
+    _, c0, _ = O.process_batch(frames, cfg)
+    _, c1, _ = O.process_batch(frames, cfg, ref_cal=cal)
+    assert np.allclose(c1, np.conj(C) * c0, rtol=1e-9, atol=1e-12 * np.abs(c0).max())
