@@ -984,7 +984,10 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
     twH1[e] = make_uint2(__builtin_amdgcn_alignbit(t.x, t.x, 16) ^ 0x8000u,
                          __builtin_amdgcn_alignbit(t.y, t.y, 16) ^ 0x8000u);
   }
-  for (int e = tid; e < L * (M + 1); e += NT) twA[e] = tabs[H + e];
+  // the inter-pass twiddles carry the power-of-two scale of the B operand, so
+  // pass B and the unpack produce scaled values (v[0] is scaled on its own)
+  const float sc = 0.5f * a.xscale;           // ev + w od = 2 X
+  for (int e = tid; e < L * (M + 1); e += NT) twA[e] = cscale(tabs[H + e], sc);
   for (int e = tid; e < w; e += NT) twix[e] = a.ix.tw[e];
   for (int e = tid; e < h; e += NT) twiy[e] = a.iy.tw[e];
 
@@ -1017,7 +1020,6 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
 #pragma unroll
     for (int u = 0; u < U; ++u) acc[i][u] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  const float sc = 0.5f * a.xscale;           // ev + w od = 2 X
   const int rho = wave * RPW + r;             // this lane's row within a round
   auto unpack = [](float2 zk, float2 zp, float2 wk) {
     const float2 ev = make_float2(zk.x + zp.x, zk.y - zp.y);
@@ -1032,7 +1034,7 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
     auto put = [&](int k, float2 X) {
       const unsigned c0k = (unsigned)(k - c0);
       const int c = c0k < (unsigned)w ? (int)c0k : w;
-      const float xr = X.x * sc, xi = X.y * sc;
+      const float xr = X.x, xi = X.y;
       const _Float16 rh = (_Float16)xr, ih = (_Float16)xi;
       const half2v hi = {rh, ih};
       const half2v lo = {(_Float16)(xr - (float)rh), (_Float16)(xi - (float)ih)};
@@ -1049,6 +1051,7 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
     }
     // pass A and the inter-pass twiddle w_NR^(l a)
     dft_any<M, false>(v);
+    v[0] = cscale(v[0], sc);
 #pragma unroll
     for (int aa = 1; aa < M; ++aa) v[aa] = cmul(v[aa], twA[l * (M + 1) + aa]);
     // transpose through the wave-private rows, low then high half of a
@@ -1084,6 +1087,9 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
       const bool selfpair = (l * S + s) == 0;   // a = 0 and a = M/2 pair within their own group
       sfor<0, L>([&](auto bc) {
         constexpr int bb = decltype(bc)::value;
+        // bins M bb .. M bb + M - 1; a block wholly outside the window is skipped
+        // (c0 is uniform, so this is a scalar branch)
+        if (M * bb >= c0 + w || M * bb + M <= c0) return;
         // partners as values first: a select between two array elements would
         // become a dynamically indexed (scratch) load
         const float2 zlr = zl[s][(L - bb) % L], zhr = zh[s][L - 1 - bb], zlm = zl[s][L - 1 - bb];
@@ -1093,8 +1099,10 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
         put(ahi[s] + M * bb, unpack(zh[s][bb], ph, tw_mul<bb, 2 * L, false>(whi[s])));
       });
       // Nyquist column k = NR from Z[0] (other lanes discard)
-      const float2 z0 = zl[s][0];
-      put(selfpair ? NR : c0 - 1, make_float2(2.f * (z0.x - z0.y), 0.f));
+      if (NR >= c0 && NR < c0 + w) {
+        const float2 z0 = zl[s][0];
+        put(selfpair ? NR : c0 - 1, make_float2(2.f * (z0.x - z0.y), 0.f));
+      }
     }
   };
 
@@ -1177,14 +1185,24 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
   // inverse transforms along x (rows), window factor on load
   {
     constexpr int MM = w / 4, US = MM / 4;
-    const float2* fac = factor + (size_t)(lam_in(g, b) * g.P + p) * (h * w);
+    // factor rows through one uniform base and 32-bit lane offsets; the pass's
+    // loads are all issued before the first is used
+    const char* fac = reinterpret_cast<const char*>(factor + (size_t)(lam_in(g, b) * g.P + p) * (h * w));
+    const bool hasCal = a.cal.p != nullptr;
     const float2 cb = a.cal.at(p, b);
     for (int ps = wave; ps < J; ps += 4) {
       const int j = 16 * ps + t16;
       float2* row = fbuf + j * WP;
       float2 v[MM];
+      const unsigned fo = (unsigned)(j * w + l4) * 8u;
 #pragma unroll
-      for (int m = 0; m < MM; ++m) v[m] = cmul(cmul(row[l4 + 4 * m], fac[j * w + l4 + 4 * m]), cb);
+      for (int m = 0; m < MM; ++m) v[m] = *reinterpret_cast<const float2*>(fac + fo + 32u * m);
+#pragma unroll
+      for (int m = 0; m < MM; ++m) v[m] = cmul(row[l4 + 4 * m], v[m]);
+      if (hasCal) {
+#pragma unroll
+        for (int m = 0; m < MM; ++m) v[m] = cmul(v[m], cb);
+      }
       dft_any<MM, true>(v);
 #pragma unroll
       for (int aa = 1; aa < MM; ++aa) {
@@ -1217,11 +1235,20 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
   float* red = reinterpret_cast<float*>(twiy + h);   // [4] per-wave peaks
   {
     constexpr int MM = h / 4, US = MM / 4;
-    const float2* rp = ref + (size_t)(lam_in(g, b) * g.P + p) * (h * w);
-    float2* out = fields + (size_t)(out_pos(g, b) * g.P + p) * (h * w);
+    // reference wave and fields through uniform bases and 32-bit lane offsets;
+    // the pass's reference values are loaded before its transforms
+    const char* rp = reinterpret_cast<const char*>(ref + (size_t)(lam_in(g, b) * g.P + p) * (h * w));
+    char* out = reinterpret_cast<char*>(fields + (size_t)(out_pos(g, b) * g.P + p) * (h * w));
     for (int ps = wave; ps < U; ps += 4) {
       const int c = 16 * ps + t16;
       float2* col = fbuf + c;
+      const unsigned ro = (unsigned)(l4 * US * w + c) * 8u;   // element (k, c) at ro + (k - l4 US) w 8
+      float2 rv[US][4];
+#pragma unroll
+      for (int s2 = 0; s2 < US; ++s2)
+#pragma unroll
+        for (int bb = 0; bb < 4; ++bb)
+          rv[s2][bb] = *reinterpret_cast<const float2*>(rp + ro + (unsigned)((s2 + MM * bb) * w) * 8u);
       float2 v[MM];
 #pragma unroll
       for (int m = 0; m < MM; ++m) v[m] = col[(l4 + 4 * m) * WP];
@@ -1247,9 +1274,9 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
 #pragma unroll
         for (int bb = 0; bb < 4; ++bb) {
           const int k = l4 * US + s2 + MM * bb;
-          const float2 tt = cmul(z[s2][bb], rp[k * w + c]);
+          const float2 tt = cmul(z[s2][bb], rv[s2][bb]);
           const float2 V = make_float2(-tt.x, tt.y);
-          out[k * w + c] = V;
+          *reinterpret_cast<float2*>(out + ro + (unsigned)((s2 + MM * bb) * w) * 8u) = V;
           if (ovl) {   // the field stays in the plane for the overlap
             col[k * WP] = V;
             vmax = fmaxf(vmax, fmaxf(fabsf(V.x), fabsf(V.y)));
