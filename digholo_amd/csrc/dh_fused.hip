@@ -959,12 +959,15 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
   const int lr = lane & 15, q = lane >> 4;    // MFMA operand coordinates
   const int c0 = g.c0[p];
 
-  float2* E = reinterpret_cast<float2*>(w3_smem) + (wave * RPW + r) * RPc;      // this lane's exchange row
-  _Float16* Bs0 = reinterpret_cast<_Float16*>(w3_smem + Sh::EBYTES);            // [2 buffers][2][w + 1][PK]
+  // row-phase layout from LDS address 0 (so the column-DFT table reads need no
+  // base add): twT [part][hi, lo][H] (part 0 {cos, sin}, part 1 {-sin, cos}, fp16
+  // pairs), twA [L][M + 1], the exchange rows, the two B buffers
+  unsigned* twT = reinterpret_cast<unsigned*>(w3_smem);
+  float2* twA = reinterpret_cast<float2*>(w3_smem + 16 * H);
+  unsigned char* rowBase = w3_smem + 16 * H + 8 * L * (M + 1);
+  float2* E = reinterpret_cast<float2*>(rowBase) + (wave * RPW + r) * RPc;      // this lane's exchange row
+  _Float16* Bs0 = reinterpret_cast<_Float16*>(rowBase + Sh::EBYTES);           // [2 buffers][2][w + 1][PK]
   constexpr int BSTRIDE = 2 * (w + 1) * PK;
-  uint2* twH = reinterpret_cast<uint2*>(w3_smem + Sh::EBYTES + Sh::BBYTES);     // [H] {cos, sin} hi, lo
-  float2* twA = reinterpret_cast<float2*>(twH + H);                            // [L][M + 1]
-  uint2* twH1 = reinterpret_cast<uint2*>(twA + L * (M + 1));                   // [H] {-sin, cos} hi, lo
   float2* fbuf = reinterpret_cast<float2*>(w3_smem);                            // [h][WP] after the rows
   float2* twix = reinterpret_cast<float2*>(w3_smem + a.region);                 // [w]
   float2* twiy = twix + w;                                                      // [h]
@@ -979,10 +982,11 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
 
   for (int e = tid; e < H; e += NT) {
     const uint2 t = reinterpret_cast<const uint2*>(tabs)[e];
-    twH[e] = t;
+    twT[e] = t.x;
+    twT[H + e] = t.y;
     // the imaginary-part operand (-sin, cos): halves swapped, then the low one negated
-    twH1[e] = make_uint2(__builtin_amdgcn_alignbit(t.x, t.x, 16) ^ 0x8000u,
-                         __builtin_amdgcn_alignbit(t.y, t.y, 16) ^ 0x8000u);
+    twT[2 * H + e] = __builtin_amdgcn_alignbit(t.x, t.x, 16) ^ 0x8000u;
+    twT[3 * H + e] = __builtin_amdgcn_alignbit(t.y, t.y, 16) ^ 0x8000u;
   }
   // the inter-pass twiddles carry the power-of-two scale of the B operand, so
   // pass B and the unpack produce scaled values (v[0] is scaled on its own)
@@ -1004,14 +1008,23 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
   }
   // window rows of this lane's MFMA units; the twiddle index ky * y mod H of
   // the lane's first k-slot, carried from k-step to k-step (y advances by 16)
+  // With H a power of two the indices run unreduced in bytes and are masked at
+  // use, the part's table selected by a disjoint bit: (t & (8H - 8)) | 8H.
+  const bool hp2 = (H & (H - 1)) == 0;
   int ky[NU], mrun[NU], ky16[NU];
 #pragma unroll
   for (int i = 0; i < NU; ++i) {
     const int jt = (wv + 4 * i) >> 1;
     int t = (16 * jt + lr + g.SY[p] + H / 2) % H;
     ky[i] = t < 0 ? t + H : t;
-    mrun[i] = (ky[i] * 4 * q) % H;
-    ky16[i] = (ky[i] * 16) % H;
+    if (hp2) {   // in bytes of the 4-byte table words
+      mrun[i] = 16 * ky[i] * q;
+      ky16[i] = 64 * ky[i];
+      ky[i] *= 4;
+    } else {
+      mrun[i] = (ky[i] * 4 * q) % H;
+      ky16[i] = (ky[i] * 16) % H;
+    }
   }
 
   floatx4 acc[NU][U];
@@ -1108,7 +1121,13 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
 
   // F += D . X over the rows of round rd (B buffer Bs); rounds run in order,
   // so mrun[i] is ky * (rd RR + 16 ks + 4 q) mod H on entry to each k-step
-  const uint2* twU = (wv & 1) ? twH1 : twH;   // every unit of a wave has the wave's part
+  const unsigned* twU = twT + ((wv & 1) ? 2 * H : 0);   // every unit of a wave has the wave's part
+  // power-of-two H: LDS byte address of the hi word (t & 4(H - 1)) + the part's
+  // table base, the lo word 4H further on
+  using lds_u32 = __attribute__((address_space(3))) const unsigned;
+  const unsigned m4 = 4u * (unsigned)(H - 1), lo4 = 4u * (unsigned)H;
+  const unsigned selB = (unsigned)(size_t)((__attribute__((address_space(3))) unsigned char*)w3_smem) +
+                        ((wv & 1) ? 8u * (unsigned)H : 0u);
   auto gemm = [&](int rd, const _Float16* Bs) {
     (void)rd;
 #pragma unroll
@@ -1118,19 +1137,30 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
       for (int i = 0; i < NU; ++i) {
         const int unit = wv + 4 * i;
         if (unit < 2 * J) {
-          int mm = mrun[i];
-          mrun[i] += ky16[i];
-          if (mrun[i] >= H) mrun[i] -= H;
           uint4 dh, dl;   // k-slot pairs (Re, Im) of A: part 0 (cos, sin), part 1 (-sin, cos)
           unsigned* ph = reinterpret_cast<unsigned*>(&dh);
           unsigned* pl = reinterpret_cast<unsigned*>(&dl);
+          if (hp2) {
+            unsigned tb = (unsigned)mrun[i];
+            mrun[i] += ky16[i];
 #pragma unroll
-          for (int e2 = 0; e2 < 4; ++e2) {
-            const uint2 t = twU[mm];
-            mm += ky[i];
-            if (mm >= H) mm -= H;
-            ph[e2] = t.x;
-            pl[e2] = t.y;
+            for (int e2 = 0; e2 < 4; ++e2) {
+              const unsigned ad = (tb & m4) + selB;
+              ph[e2] = *reinterpret_cast<lds_u32*>((size_t)ad);
+              pl[e2] = *reinterpret_cast<lds_u32*>((size_t)(ad + lo4));
+              tb += (unsigned)ky[i];
+            }
+          } else {
+            int mm = mrun[i];
+            mrun[i] += ky16[i];
+            if (mrun[i] >= H) mrun[i] -= H;
+#pragma unroll
+            for (int e2 = 0; e2 < 4; ++e2) {
+              ph[e2] = twU[mm];
+              pl[e2] = twU[H + mm];
+              mm += ky[i];
+              if (mm >= H) mm -= H;
+            }
           }
           ah[i] = __builtin_bit_cast(half8, dh);
           al[i] = __builtin_bit_cast(half8, dl);
