@@ -3,8 +3,10 @@
 //   stage 1:  T[(y,c)][m] = sum_x V_c[y][x] HGx[m][x]        (c = re/im)
 //   stage 2:  C[(m,c)][n] += sum_y T[(y,c)][m] HGy[n][y]      -> coef(m,n), m+n < G
 //
-// One workgroup (4 waves) owns one (frame, pol) field and walks it in chunks
-// of 16 field rows.  Per chunk, stage 1 is a (32 x Gp x w) product whose
+// One workgroup (4 waves) per pol and frame slot walks the frames b =
+// slot, slot + slots, ... (the basis stays in LDS for all of them) and each
+// field in chunks of 16 field rows; the next chunk's field values are loaded
+// into registers while the current chunk is multiplied.  Per chunk, stage 1 is a (32 x Gp x w) product whose
 // output is re-laid out in LDS as the A operand of stage 2, a (2Gp x Gp x 16)
 // product accumulated in registers across chunks.  Tiles of stage 2 that hold
 // no mode with m+n < G are skipped.
@@ -55,7 +57,7 @@ __global__ __launch_bounds__(kOvThreads) void k_overlap_mfma(Geometry g, const f
                                                              float2* __restrict__ coefs) {
   using E = typename OvT<SPLIT>::E;
   extern __shared__ __align__(16) unsigned char ov_smem[];
-  const int p = blockIdx.x, b = blockIdx.y;
+  const int p = blockIdx.x;
   const int G = g.G, w = g.w, h = g.h, M = g.M;
   const int Gp = (G + 15) & ~15;
   const int pw = pitch_of(w), ph = pitch_of(h), pk = pitch_of(kChunk);
@@ -68,28 +70,7 @@ __global__ __launch_bounds__(kOvThreads) void k_overlap_mfma(Geometry g, const f
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, q = lane >> 4;
 
-  const float2* V = fields + (size_t)(b * g.P + p) * h * w;
-
-  // --- per-field power-of-two scale (split form only) ---
-  float scale = 1.f;
-  if (SPLIT) {
-    float mx = 0.f;
-    for (int e = tid; e < h * w; e += kOvThreads) {
-      const float2 v = V[e];
-      mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)));
-    }
-    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-    if (lane == 0) s_red[wave] = mx;
-    __syncthreads();
-    mx = fmaxf(fmaxf(s_red[0], s_red[1]), fmaxf(s_red[2], s_red[3]));
-    if (mx > 0.f) {
-      int ex;
-      frexpf(mx, &ex);                 // mx in [2^(ex-1), 2^ex)
-      scale = ldexpf(1.f, 8 - ex);     // max * scale in [128, 256)
-    }
-  }
-
-  // --- basis into LDS (rows m >= G are zero) ---
+  // --- basis into LDS (rows m >= G are zero), once for every frame of the slot ---
   for (int e = tid; e < Gp * w; e += kOvThreads) {
     const int m = e / w, x = e - m * w;
     const float v = m < G ? hgx[((size_t)p * G + m) * w + x] : 0.f;
@@ -128,17 +109,48 @@ __global__ __launch_bounds__(kOvThreads) void k_overlap_mfma(Geometry g, const f
       ++cnt;
     }
   }
+  const int rt1 = 2, ct1 = Gp / 16;
+  // this thread's elements of a chunk: e = tid + 256 j < 16 w; held in registers
+  // one chunk ahead when w <= 128 (j < 8), else loaded as they are stored
+  constexpr int kPerT = 8;
+  const int nPer = (kChunk * w - tid + kOvThreads - 1) / kOvThreads;
+  const bool ahead = kChunk * w <= kPerT * kOvThreads;
+  for (int b = blockIdx.y; b < g.B; b += gridDim.y) {
+  const float2* V = fields + (size_t)(b * g.P + p) * h * w;
+
+  // --- per-field power-of-two scale (split form only) ---
+  float scale = 1.f;
+  if (SPLIT) {
+    float mx = 0.f;
+    for (int e = tid; e < h * w; e += kOvThreads) {
+      const float2 v = V[e];
+      mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)));
+    }
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    __syncthreads();   // s_red of the previous frame has been read
+    if (lane == 0) s_red[wave] = mx;
+    __syncthreads();
+    mx = fmaxf(fmaxf(s_red[0], s_red[1]), fmaxf(s_red[2], s_red[3]));
+    if (mx > 0.f) {
+      int ex;
+      frexpf(mx, &ex);                 // mx in [2^(ex-1), 2^ex)
+      scale = ldexpf(1.f, 8 - ex);     // max * scale in [128, 256)
+    }
+  }
+
   floatx4 acc[TPW];
 #pragma unroll
   for (int i = 0; i < TPW; ++i) acc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  const int rt1 = 2, ct1 = Gp / 16;
+  float2 nxt[kPerT];   // the next chunk's values, in flight during the current chunk
+#pragma unroll
+  for (int j = 0; j < kPerT; ++j)
+    if (ahead && j < nPer) nxt[j] = V[tid + kOvThreads * j];
   for (int y0 = 0; y0 < h; y0 += kChunk) {
     __syncthreads();   // previous chunk's A1/A2 readers are done (and basis/scale ready)
     // V chunk -> A1 rows (yy, c)
-    for (int e = tid; e < kChunk * w; e += kOvThreads) {
+    auto store_a1 = [&](int e, float2 v) {
       const int yy = e / w, x = e - yy * w;
-      const float2 v = V[(size_t)(y0 + yy) * w + x];
       if (SPLIT) {
         _Float16 hi, lo;
         split16(v.x * scale, hi, lo);
@@ -151,6 +163,19 @@ __global__ __launch_bounds__(kOvThreads) void k_overlap_mfma(Geometry g, const f
         A1[(2 * yy) * pw + x] = v.x;
         A1[(2 * yy + 1) * pw + x] = v.y;
       }
+    };
+    if (ahead) {
+#pragma unroll
+      for (int j = 0; j < kPerT; ++j) {
+        if (j < nPer) {
+          const int e = tid + kOvThreads * j;
+          const float2 v = nxt[j];
+          if (y0 + kChunk < h) nxt[j] = V[(size_t)(y0 + kChunk) * w + e];
+          store_a1(e, v);
+        }
+      }
+    } else {
+      for (int e = tid; e < kChunk * w; e += kOvThreads) store_a1(e, V[(size_t)y0 * w + e]);
     }
     __syncthreads();
     // stage 1: tiles (rt in 0..1) x (ct in 0..Gp/16)
@@ -179,8 +204,18 @@ __global__ __launch_bounds__(kOvThreads) void k_overlap_mfma(Geometry g, const f
           d = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, d, 0, 0, 0);
         }
       } else {
-        for (int k0 = 0; k0 < w; k0 += 4) {
-          d = __builtin_amdgcn_mfma_f32_16x16x4f32((float)arow[k0 + q], (float)brow[k0 + q], d, 0, 0, 0);
+        // lane group q takes the k-slots q w/4 ... (q + 1) w/4 - 1 (the same for A and B),
+        // four per 16-byte load
+        const int kq = w >> 2;
+        const float* ar = reinterpret_cast<const float*>(arow) + q * kq;
+        const float* br = reinterpret_cast<const float*>(brow) + q * kq;
+        for (int s4 = 0; s4 < kq; s4 += 4) {
+          const float4 a4 = *reinterpret_cast<const float4*>(ar + s4);
+          const float4 b4 = *reinterpret_cast<const float4*>(br + s4);
+          d = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.x, b4.x, d, 0, 0, 0);
+          d = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.y, b4.y, d, 0, 0, 0);
+          d = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.z, b4.z, d, 0, 0, 0);
+          d = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.w, b4.w, d, 0, 0, 0);
         }
       }
       // D[r][m], r = rt*16 + 4q + i = (yy, c)  ->  A2[(m, c)][yy]
@@ -225,9 +260,13 @@ __global__ __launch_bounds__(kOvThreads) void k_overlap_mfma(Geometry g, const f
           acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc[i], 0, 0, 0);
           acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc[i], 0, 0, 0);
         } else {
-#pragma unroll
-          for (int k0 = 0; k0 < kChunk; k0 += 4)
-            acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)arow[k0 + q], (float)brow[k0 + q], acc[i], 0, 0, 0);
+          // lane group q takes the k-slots 4q .. 4q + 3 of the chunk
+          const float4 a4 = *reinterpret_cast<const float4*>(reinterpret_cast<const float*>(arow) + 4 * q);
+          const float4 b4 = *reinterpret_cast<const float4*>(reinterpret_cast<const float*>(brow) + 4 * q);
+          acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.x, b4.x, acc[i], 0, 0, 0);
+          acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.y, b4.y, acc[i], 0, 0, 0);
+          acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.z, b4.z, acc[i], 0, 0, 0);
+          acc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.w, b4.w, acc[i], 0, 0, 0);
         }
       }
     }
@@ -252,6 +291,7 @@ __global__ __launch_bounds__(kOvThreads) void k_overlap_mfma(Geometry g, const f
       }
     }
   }
+  }   // frames of this slot
 }
 
 static size_t overlap_lds(const Geometry& g, bool split) {
@@ -282,7 +322,9 @@ static hipError_t ov_launch(const Geometry& g, const float2* fields, const float
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  dim3 grid(g.P, g.B);
+  // about two resident workgroups per CU (256 CUs), each walking several frames
+  const int slots = std::max(1, std::min(g.B, 512 / std::max(1, g.P)));
+  dim3 grid(g.P, slots);
   k_overlap_mfma<SPLIT, TPW><<<grid, kOvThreads, lds, s>>>(g, fields, hgx, hgy, coefs);
   return hipGetLastError();
 }
