@@ -19,6 +19,7 @@
 // kernels of dh_kernels.hip.
 #include <algorithm>
 #include <type_traits>
+#include <type_traits>
 
 #include "dh_fft.h"
 #include "dh_internal.h"
@@ -1128,6 +1129,7 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
   const unsigned m4 = 4u * (unsigned)(H - 1), lo4 = 4u * (unsigned)H;
   const unsigned selB = (unsigned)(size_t)((__attribute__((address_space(3))) unsigned char*)w3_smem) +
                         ((wv & 1) ? 8u * (unsigned)H : 0u);
+  const int nUnits = (2 * J - wv + 3) / 4;   // units wv, wv + 4, ... below 2 J: NU or NU - 1
   auto gemm = [&](int rd, const _Float16* Bs) {
     (void)rd;
 #pragma unroll
@@ -1166,20 +1168,45 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
           al[i] = __builtin_bit_cast(half8, dl);
         }
       }
+      // one product (hi hi, hi lo, lo hi) at a time over the units and a pair of
+      // n-tiles, so consecutive MFMAs write different accumulators; the wave's
+      // unit count is a compile-time branch, so each sequence has no branch inside
+      auto mma = [&](auto ni) {
+        constexpr int NI = decltype(ni)::value;
+        constexpr int UC = 2;
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const _Float16* bq = Bs + (16 * u + lr) * PK + 32 * ks + 8 * q;
-        const half8 bh = *reinterpret_cast<const half8*>(bq);
-        const half8 bl = *reinterpret_cast<const half8*>(bq + (w + 1) * PK);
+        for (int u0 = 0; u0 < U; u0 += UC) {
+          half8 bh[UC], bl[UC];
 #pragma unroll
-        for (int i = 0; i < NU; ++i) {
-          if (wv + 4 * i < 2 * J) {
-            acc[i][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh, acc[i][u], 0, 0, 0);
-            acc[i][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl, acc[i][u], 0, 0, 0);
-            acc[i][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh, acc[i][u], 0, 0, 0);
+          for (int uu = 0; uu < UC; ++uu) {
+            if (u0 + uu < U) {
+              const _Float16* bq = Bs + (16 * (u0 + uu) + lr) * PK + 32 * ks + 8 * q;
+              bh[uu] = *reinterpret_cast<const half8*>(bq);
+              bl[uu] = *reinterpret_cast<const half8*>(bq + (w + 1) * PK);
+            }
           }
+#pragma unroll
+          for (int uu = 0; uu < UC; ++uu)
+#pragma unroll
+            for (int i = 0; i < NI; ++i)
+              if (u0 + uu < U)
+                acc[i][u0 + uu] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh[uu], acc[i][u0 + uu], 0, 0, 0);
+#pragma unroll
+          for (int uu = 0; uu < UC; ++uu)
+#pragma unroll
+            for (int i = 0; i < NI; ++i)
+              if (u0 + uu < U)
+                acc[i][u0 + uu] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[uu], acc[i][u0 + uu], 0, 0, 0);
+#pragma unroll
+          for (int uu = 0; uu < UC; ++uu)
+#pragma unroll
+            for (int i = 0; i < NI; ++i)
+              if (u0 + uu < U)
+                acc[i][u0 + uu] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[uu], acc[i][u0 + uu], 0, 0, 0);
         }
-      }
+      };
+      if (nUnits == NU) mma(std::integral_constant<int, NU>{});
+      else mma(std::integral_constant<int, NU - 1>{});
     }
   };
 
