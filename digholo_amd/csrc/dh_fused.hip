@@ -843,43 +843,55 @@ __device__ __forceinline__ void w3_overlap(const W3Args& a, const float2* plane,
   for (int ch = 0; ch < J; ++ch) {
     _Float16* A2 = A2b + (ch & 1) * bufH;
     load_b2(ch);
-    // stage 1: T_part[yy][m] = sum_x V_part[16 ch + yy][x] HGx[m][x]
+    // stage 1: T_part[yy][m] = sum_x V_part[16 ch + yy][x] HGx[m][x]; a wave's
+    // units share its part (u1 = wave + 4 i), so the field operand is split once
+    // per k-step for all of them
+    {
+      const int part = wave & 1;
+      floatx4 d[NU1];
 #pragma unroll
-    for (int i = 0; i < NU1; ++i) {
-      const int u1 = wave + 4 * i;
-      if (u1 < 2 * ct1) {
-        const int mt = u1 >> 1, part = u1 & 1;
-        floatx4 d = {0.f, 0.f, 0.f, 0.f};
+      for (int i = 0; i < NU1; ++i) d[i] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int ks = 0; ks < KS1; ++ks) {
-          const int kb = 32 * ks + 8 * q;
-          half8 ah, al;
-          if (kb < w) {   // a ragged last step (w % 32 == 16) zero-fills the upper lane groups
-            const float4* vp = reinterpret_cast<const float4*>(plane + (16 * ch + lr) * WP + kb);
-            float vv[8];
+      for (int ks = 0; ks < KS1; ++ks) {
+        const int kb = 32 * ks + 8 * q;
+        half8 ah, al;
+        if (kb < w) {   // a ragged last step (w % 32 == 16) zero-fills the upper lane groups
+          const float4* vp = reinterpret_cast<const float4*>(plane + (16 * ch + lr) * WP + kb);
+          float vv[8];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              const float4 t = vp[e];
-              vv[2 * e] = part ? t.y : t.x;
-              vv[2 * e + 1] = part ? t.w : t.z;
-            }
-            split8(vv, ah, al);
-          } else {
-#pragma unroll
-            for (int e = 0; e < 8; ++e) ah[e] = al[e] = (_Float16)0.f;
+          for (int e = 0; e < 4; ++e) {
+            const float4 t = vp[e];
+            vv[2 * e] = part ? t.y : t.x;
+            vv[2 * e + 1] = part ? t.w : t.z;
           }
-          d = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, b1h[i][ks], d, 0, 0, 0);
-          d = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, b1l[i][ks], d, 0, 0, 0);
-          d = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, b1h[i][ks], d, 0, 0, 0);
-        }
-        // D[yy = 4q + e][m = 16 mt + lr] -> A2[hi/lo][(m, part)][yy]
-        const int row = 2 * (16 * mt + lr) + part;
+          split8(vv, ah, al);
+        } else {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int yy = 4 * q + e;
-          const _Float16 dh = (_Float16)d[e];
-          A2[row * kOvPK + yy] = dh;
-          A2[2 * Gp * kOvPK + row * kOvPK + yy] = (_Float16)(d[e] - (float)dh);
+          for (int e = 0; e < 8; ++e) ah[e] = al[e] = (_Float16)0.f;
+        }
+#pragma unroll
+        for (int i = 0; i < NU1; ++i) {
+          if (wave + 4 * i < 2 * ct1) {
+            d[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, b1h[i][ks], d[i], 0, 0, 0);
+            d[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, b1l[i][ks], d[i], 0, 0, 0);
+            d[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, b1h[i][ks], d[i], 0, 0, 0);
+          }
+        }
+      }
+      // D[yy = 4q + e][m = 16 mt + lr] -> A2[hi/lo][(m, part)][yy]
+#pragma unroll
+      for (int i = 0; i < NU1; ++i) {
+        const int u1 = wave + 4 * i;
+        if (u1 < 2 * ct1) {
+          const int mt = u1 >> 1;
+          const int row = 2 * (16 * mt + lr) + part;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int yy = 4 * q + e;
+            const _Float16 dh = (_Float16)d[i][e];
+            A2[row * kOvPK + yy] = dh;
+            A2[2 * Gp * kOvPK + row * kOvPK + yy] = (_Float16)(d[i][e] - (float)dh);
+          }
         }
       }
     }
