@@ -155,6 +155,7 @@ SIGNATURES: Dict[str, Tuple[object, List[object]]] = {
     "digHoloGetStream": (c_void_p, [c_int]),
     "digHoloGetWindowGeometry": (c_int, [c_int, P_int]),
     "digHoloConfigSetFusedWindow": (c_int, [c_int, c_int]),
+    "digHoloConfigSetFusedOverlap": (c_int, [c_int, c_int]),
     "digHoloGetWindowPath": (c_int, [c_int]),
     "digHoloConfigSetStageTiming": (c_int, [c_int, c_int]),
     "digHoloGetStageTimes": (c_int, [c_int, P_float]),

@@ -239,7 +239,9 @@ struct Handle {
   DevBuf<float2> calDev;    // batch calibration on the device
   unsigned calDevVersion = ~0u;
   DevBuf<uint16_t> hgx16, hgy16;   // split-fp16 basis of the fused overlap
-  bool fusedOverlap = true; // path 3 computes split-fp16 coefficients in the window kernel
+  // path 3 computes the coefficients in the window kernel: -1 for the split
+  // form only (the faster choice on the measured shapes), 0 never, 1 in either precision
+  int fusedOverlap = -1;
   bool lastFusedOverlap = false;
 
   // AutoAlign parameter estimate: batch-summed Fourier maps [P][Ny][Nx],
@@ -921,7 +923,7 @@ int autotune(Handle& h) {
     auto it = g_tune.find(key);
     if (it != g_tune.end()) {
       h.windowMode = it->second.windowMode;
-      h.fusedOverlap = it->second.fused != 0;
+      h.fusedOverlap = it->second.fused;
       h.ovTpw = it->second.tpw;
       return DIGHOLO_ERROR_SUCCESS;
     }
@@ -942,7 +944,7 @@ int autotune(Handle& h) {
       for (int tp : {0, 2 * minTpw}) {
         if (tp > 32 || (tp && minTpw == 0)) continue;
         h.windowMode = wm;
-        h.fusedOverlap = fu != 0;
+        h.fusedOverlap = fu;
         h.ovTpw = tp;
         if ((rc = enqueue_batch(h, 0))) break;
         const std::array<int, 3> sig{h.lastPath, h.lastFusedOverlap ? 1 : 0, h.lastFusedOverlap ? 0 : tp};
@@ -965,7 +967,7 @@ int autotune(Handle& h) {
   h.tuning = false;
   if (rc) return rc;
   h.windowMode = best.windowMode;
-  h.fusedOverlap = best.fused != 0;
+  h.fusedOverlap = best.fused;
   h.ovTpw = best.tpw;
   std::lock_guard<std::mutex> lk(g_tuneMu);
   g_tune[key] = best;
@@ -1262,13 +1264,16 @@ int enqueue_batch(Handle& h, int step = 0) {
   else if (h.windowMode >= 1 && plan_fused(h, g, fa)) path = 1;
   if (path) {
     // one kernel from camera window to field
-    const bool fuseOv = path == 3 && h.fusedOverlap && c.precision == DIGHOLO_PRECISION_SPLIT16 &&
+    // the overlap runs inside the specialised window kernel in either precision
+    const bool split = c.precision == DIGHOLO_PRECISION_SPLIT16;
+    const bool fuseOv = path == 3 && (h.fusedOverlap > 0 || (h.fusedOverlap < 0 && split)) &&
                         dh::window_v3_overlap_fits(g);
     h.lastFusedOverlap = fuseOv;
     if (path == 3)
       DH_TRY(dh::launch_window_v3(fa, dframes, h.w3Tab.p, h.factor.p, h.ref.p, h.fields.p,
-                                  fuseOv ? h.hgx16.p : nullptr, fuseOv ? h.hgy16.p : nullptr,
-                                  fuseOv ? h.coefs.p : nullptr, cal, h.stream),
+                                  fuseOv && split ? h.hgx16.p : nullptr, fuseOv && split ? h.hgy16.p : nullptr,
+                                  fuseOv ? h.coefs.p : nullptr, cal, h.stream, fuseOv && !split ? h.hgx.p : nullptr,
+                                  fuseOv && !split ? h.hgy.p : nullptr),
              "fused window (register rows)");
     else if (path == 2)
       DH_TRY(dh::launch_window_mfma(fa, h.twW.p, dframes, h.colTab.p, h.factor.p, h.ref.p, h.fields.p, h.stream),
@@ -2925,7 +2930,7 @@ bool launch_signature(Handle& h, std::string& sig) {
   s.append((const char*)d.mask.data(), d.mask.size());
   char buf[256];
   std::snprintf(buf, sizeof(buf), "|%p|%d|%d|%d|%d|%d|%d|%d|%u|%u|%d|%d|%p", h.frames, h.frameU16, h.transpose,
-                c.precision, h.windowMode, h.fusedOverlap ? 1 : 0, h.ovTpw, c.avgCount * 8 + c.avgMode,
+                c.precision, h.windowMode, h.fusedOverlap, h.ovTpw, c.avgCount * 8 + c.avgMode,
                 c.calEnabled ? c.calVersion : 0u, h.tabVersion, h.autoTune, c.calEnabled, (void*)h.stream);
   sig = s + buf;
   return true;
@@ -3999,7 +4004,7 @@ EXT_C int digHoloConfigSetAutoTune(int handleIdx, int mode) {
   h->autoTune = mode;
   if (!mode) {   // back to the default kernels
     h->windowMode = 3;
-    h->fusedOverlap = true;
+    h->fusedOverlap = -1;
     h->ovTpw = 0;
   }
   return DIGHOLO_ERROR_SUCCESS;
@@ -4260,6 +4265,12 @@ EXT_C int digHoloConfigSetFusedWindow(int handleIdx, int enabled) {
   H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
   if (enabled < 0 || enabled > 3) return DIGHOLO_ERROR_INVALIDARGUMENT;
   h->windowMode = enabled;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C int digHoloConfigSetFusedOverlap(int handleIdx, int mode) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (mode < -1 || mode > 1) return DIGHOLO_ERROR_INVALIDARGUMENT;
+  h->fusedOverlap = mode;
   return DIGHOLO_ERROR_SUCCESS;
 }
 EXT_C int digHoloGetWindowPath(int handleIdx) {

@@ -736,6 +736,9 @@ struct W3Args {
   // fused split-fp16 HG overlap (null hgx16: coefficients computed elsewhere)
   const _Float16* hgx16;              // [P][hi/lo][Gp][w], rows m >= G zero
   const _Float16* hgy16;              // [P][hi/lo][Gp][h]
+  // fused FP32 HG overlap instead (null: none): the basis [P][G][w], [P][G][h]
+  const float* hgx32;
+  const float* hgy32;
   float2* coefs;                      // [B][P][M]
   BatchCal cal;                       // per-frame window calibration
   int region;         // bytes of the aliased row-phase / field-plane region
@@ -934,6 +937,120 @@ __device__ __forceinline__ void w3_overlap(const W3Args& a, const float2* plane,
         if (n < G && m < G && m + n < G) {
           const int gs = m + n;
           out[gs * (gs + 1) / 2 + n] = make_float2(cacc[i][ii] * inv, cacc[i][ii + 1] * inv);
+        }
+      }
+    }
+  }
+}
+
+// FP32 form of the fused overlap (v_mfma_f32_16x16x4_f32, FP32 operands and
+// accumulation; no scale): the same two stages and tiles as above, lane group q
+// of every MFMA taking the contiguous k-slots q K/4 .. (q + 1) K/4 - 1 of A and B.
+// Stage-1 output buffers at st1: 2 x [2 Gp][kOvPK32] floats.
+constexpr int kOvPK32 = 20;   // floats per stage-1 output row (16 field rows + pad)
+template <int U, int J>
+__device__ __forceinline__ void w3_overlap32(const W3Args& a, const float2* plane, unsigned char* st1, int p, int b,
+                                             int wave, int lane) {
+  constexpr int w = 16 * U, h = 16 * J, WP = w + 4, KQ = w / 4;
+  const Geometry& g = a.g;
+  const int G = g.G, Gp = (G + 15) & ~15, ct1 = Gp / 16;
+  const int lr = lane & 15, q = lane >> 4;
+  const float* bx = a.hgx32 + (size_t)p * G * w;
+  const float* by = a.hgy32 + (size_t)p * G * h;
+  float* A2b = reinterpret_cast<float*>(st1);
+  const int bufF = 2 * Gp * kOvPK32;   // floats per buffer
+  const int rt2 = (2 * Gp) / 16, ct2 = Gp / 16;
+  int myTile[kOvMaxTiles];
+  int nMine = 0;
+  {
+    int cnt = 0;
+    for (int t = 0; t < rt2 * ct2; ++t) {
+      const int mt = t / ct2, nt = t - mt * ct2;
+      if (8 * mt + 16 * nt >= G) continue;   // no (m, n) of the tile has m + n < G
+      if (cnt % 4 == wave && nMine < kOvMaxTiles) myTile[nMine++] = t;
+      ++cnt;
+    }
+  }
+  floatx4 cacc[kOvMaxTiles];
+#pragma unroll
+  for (int i = 0; i < kOvMaxTiles; ++i) cacc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
+  // stage-1 basis operands of the wave's units (m-tile, part), every chunk
+  constexpr int NU1 = 2;
+  const int part = wave & 1;   // a wave's units u1 = wave + 4 i share it
+  float b1[NU1][KQ];
+#pragma unroll
+  for (int i = 0; i < NU1; ++i) {
+    const int u1 = wave + 4 * i;
+    const int m = 16 * (u1 >> 1) + lr;
+#pragma unroll
+    for (int s4 = 0; s4 < KQ; ++s4)
+      b1[i][s4] = (u1 < 2 * ct1 && m < G) ? bx[(size_t)m * w + q * KQ + s4] : 0.f;
+  }
+  float4 b2[kOvMaxTiles];
+  for (int ch = 0; ch < J; ++ch) {
+    float* A2 = A2b + (ch & 1) * bufF;
+#pragma unroll
+    for (int i = 0; i < kOvMaxTiles; ++i) {
+      if (i < nMine) {
+        const int n = (myTile[i] % ct2) * 16 + lr;
+        b2[i] = n < G ? *reinterpret_cast<const float4*>(by + (size_t)n * h + 16 * ch + 4 * q)
+                      : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    }
+    // stage 1: T_part[yy][m] = sum_x V_part[16 ch + yy][x] HGx[m][x]
+    floatx4 d[NU1];
+#pragma unroll
+    for (int i = 0; i < NU1; ++i) d[i] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const float2* vrow = plane + (16 * ch + lr) * WP + q * KQ;
+#pragma unroll
+    for (int s4 = 0; s4 < KQ; s4 += 2) {
+      const float4 t = *reinterpret_cast<const float4*>(vrow + s4);
+      const float a0 = part ? t.y : t.x, a1 = part ? t.w : t.z;
+#pragma unroll
+      for (int i = 0; i < NU1; ++i) {
+        if (wave + 4 * i < 2 * ct1) {
+          d[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1[i][s4], d[i], 0, 0, 0);
+          d[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1[i][s4 + 1], d[i], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NU1; ++i) {
+      const int u1 = wave + 4 * i;
+      if (u1 < 2 * ct1) {
+        const int row = 2 * (16 * (u1 >> 1) + lr) + part;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) A2[row * kOvPK32 + 4 * q + e] = d[i][e];
+      }
+    }
+    __syncthreads();   // this chunk's A2 is complete (the other buffer is free, as above)
+    // stage 2: C[(m, part)][n] += sum_yy A2[(m, part)][yy] HGy[n][16 ch + yy]
+#pragma unroll
+    for (int i = 0; i < kOvMaxTiles; ++i) {
+      if (i < nMine) {
+        const int mt = myTile[i] / ct2;
+        const float4 av = *reinterpret_cast<const float4*>(A2 + (mt * 16 + lr) * kOvPK32 + 4 * q);
+        cacc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, b2[i].x, cacc[i], 0, 0, 0);
+        cacc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, b2[i].y, cacc[i], 0, 0, 0);
+        cacc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, b2[i].z, cacc[i], 0, 0, 0);
+        cacc[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, b2[i].w, cacc[i], 0, 0, 0);
+      }
+    }
+  }
+  // the triangle m + n < G
+  float2* out = a.coefs + (size_t)(b * g.P + p) * g.M;
+#pragma unroll
+  for (int i = 0; i < kOvMaxTiles; ++i) {
+    if (i < nMine) {
+      const int t = myTile[i];
+      const int mt = t / ct2, nt = t - mt * ct2;
+      const int n = nt * 16 + lr;
+#pragma unroll
+      for (int ii = 0; ii < 4; ii += 2) {
+        const int m = (mt * 16 + 4 * q + ii) >> 1;   // rows ii, ii + 1 are (m, re), (m, im)
+        if (n < G && m < G && m + n < G) {
+          const int gs = m + n;
+          out[gs * (gs + 1) / 2 + n] = make_float2(cacc[i][ii], cacc[i][ii + 1]);
         }
       }
     }
@@ -1299,7 +1416,7 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
   }
   __syncthreads();
   // inverse transforms along y (columns), tilt on the way out
-  const bool ovl = a.hgx16 != nullptr;
+  const bool ovl = a.hgx16 != nullptr || a.hgx32 != nullptr;
   float vmax = 0.f;
   float* red = reinterpret_cast<float*>(twiy + h);   // [4] per-wave peaks
   {
@@ -1354,7 +1471,11 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
       }
     }
   }
-  if (ovl) w3_overlap<U, J>(a, fbuf, vmax, red, w3_smem + a.ovlOff, p, out_pos(g, b), wave, lane);
+  if (a.hgx16) w3_overlap<U, J>(a, fbuf, vmax, red, w3_smem + a.ovlOff, p, out_pos(g, b), wave, lane);
+  else if (a.hgx32) {
+    __syncthreads();   // the whole plane is written before any wave reads it
+    w3_overlap32<U, J>(a, fbuf, w3_smem + a.ovlOff, p, out_pos(g, b), wave, lane);
+  }
 }
 
 // Tables of the v3 kernel, from FP64: [H] cos and sin of 2 pi m / H as fp16
@@ -1425,7 +1546,10 @@ bool window_v3_overlap_fits(const Geometry& g) {
   return g.G > 0 && ov_tiles(g.G) <= 4 * kOvMaxTiles && ((g.G + 15) & ~15) <= 64;
 }
 
-static int w3_ovl_bytes(const Geometry& g) { return 2 * (2 * 2 * ((g.G + 15) & ~15) * kOvPK * 2); }
+static int w3_ovl_bytes(const Geometry& g) {
+  const int Gp = (g.G + 15) & ~15;
+  return std::max(2 * (2 * 2 * Gp * kOvPK * 2), 2 * (2 * Gp * kOvPK32 * 4));   // split or FP32 buffers
+}
 
 template <int L, int M, int U, int J>
 static hipError_t w3_launch(W3Args a, const void* frames, const float2* tab, const float2* factor,
@@ -1435,7 +1559,7 @@ static hipError_t w3_launch(W3Args a, const void* frames, const float2* tab, con
   if (g.H % Sh::RR) return hipErrorNotSupported;
   a.region = w3_region(Sh::EBYTES, Sh::BBYTES, g.H, L, M, Sh::w, Sh::h);
   a.ovlOff = 8 * Sh::h * Sh::WP;
-  if (a.hgx16) a.region = std::max(a.region, a.ovlOff + w3_ovl_bytes(g));
+  if (a.hgx16 || a.hgx32) a.region = std::max(a.region, a.ovlOff + w3_ovl_bytes(g));
   const size_t lds = w3_lds<L, M, U, J>(a);
   if (lds > 160 * 1024) return hipErrorNotSupported;
   if (lds > 64 * 1024) {
@@ -1493,9 +1617,9 @@ bool window_v3_supported(const FusedArgs& a) {
 
 hipError_t launch_window_v3(const FusedArgs& fa, const void* frames_u16, const float2* tab, const float2* factor,
                             const float2* ref, float2* fields, const void* hgx16, const void* hgy16, float2* coefs,
-                            const BatchCal& cal, hipStream_t s) {
+                            const BatchCal& cal, hipStream_t s, const float* hgx32, const float* hgy32) {
   if (!window_v3_supported(fa)) return hipErrorNotSupported;
-  if (hgx16 && !window_v3_overlap_fits(fa.g)) return hipErrorNotSupported;
+  if ((hgx16 || hgx32) && !window_v3_overlap_fits(fa.g)) return hipErrorNotSupported;
   const Geometry& g = fa.g;
   int L, M;
   w3_lm(g.W / 2, L, M);
@@ -1505,6 +1629,8 @@ hipError_t launch_window_v3(const FusedArgs& fa, const void* frames_u16, const f
   a.iy = fa.iy;
   a.hgx16 = static_cast<const _Float16*>(hgx16);
   a.hgy16 = static_cast<const _Float16*>(hgy16);
+  a.hgx32 = hgx16 ? nullptr : hgx32;
+  a.hgy32 = hgx16 ? nullptr : hgy32;
   a.coefs = coefs;
   a.cal = cal;
   a.magH = magic_div((unsigned)g.H);

@@ -154,10 +154,12 @@ bool window_v3_supported(const FusedArgs& a);
 size_t window_v3_table_elems(const Geometry& g);
 hipError_t launch_window_v3_tables(const Geometry& g, float2* tab, hipStream_t s);
 // With hgx16/hgy16 (split basis from launch_hg_split) and coefs set, the same
-// launch also computes the split-fp16 HG overlap into coefs.
+// launch also computes the split-fp16 HG overlap into coefs; with hgx32/hgy32
+// (the FP32 basis) instead, the FP32 overlap.
 hipError_t launch_window_v3(const FusedArgs& a, const void* frames_u16, const float2* tab, const float2* factor,
                             const float2* ref, float2* fields, const void* hgx16, const void* hgy16, float2* coefs,
-                            const BatchCal& cal, hipStream_t s);
+                            const BatchCal& cal, hipStream_t s, const float* hgx32 = nullptr,
+                            const float* hgy32 = nullptr);
 bool window_v3_overlap_fits(const Geometry& g);
 // Split-fp16 basis [P][hi/lo][Gp][n] (Gp = G rounded up to 16) from hgx/hgy.
 size_t hg_split_elems(const Geometry& g, int n);

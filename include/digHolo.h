@@ -306,6 +306,9 @@ EXT_C int digHoloGetWindowGeometry(int handleIdx, int* out);
  * fit falls back to the next lower one.  GetWindowPath reports the path of the
  * last batch (3, 2, 1, 0; -1 none yet). */
 EXT_C int digHoloConfigSetFusedWindow(int handleIdx, int enabled);
+/* Overlap inside the specialised window kernel: -1 (default) for the split-fp16
+ * form only, 0 never (the separate overlap kernel), 1 in either precision. */
+EXT_C int digHoloConfigSetFusedOverlap(int handleIdx, int mode);
 EXT_C int digHoloGetWindowPath(int handleIdx);
 
 /* Per-stage GPU time in ms, measured with HIP events on the handle's stream

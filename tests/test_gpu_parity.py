@@ -203,6 +203,30 @@ def test_split16_overlap_parity(lib, name, B):
     print(f"{name}: coef rel-L2 vs FP64  split16 {e16:.2e}  fp32 {e32:.2e}", flush=True)
 
 
+@pytest.mark.parametrize("name,B", [("small", 6), ("dual640", 3)])
+def test_fused_overlap_modes(lib, name, B):
+    # the overlap inside the specialised window kernel, both precisions, against the
+    # separate overlap kernel and the oracle
+    frames, cfg = make_case(name, B, seed=23)
+    h = lib.digHoloCreate()
+    api.configure(lib, h, **cfg)
+    out = {}
+    for prec in (api.DIGHOLO_PRECISION_FP32, api.DIGHOLO_PRECISION_SPLIT16):
+        assert lib.digHoloConfigSetOverlapPrecision(h, prec) == 0
+        for mode in (1, 0):
+            assert lib.digHoloConfigSetFusedOverlap(h, mode) == 0
+            coefs, fields, _, _, _ = run_gpu(lib, frames, cfg, h=h)
+            assert lib.digHoloGetWindowPath(h) == 3
+            check(frames, cfg, coefs, fields)
+            out[prec, mode] = coefs
+    for prec in (api.DIGHOLO_PRECISION_FP32, api.DIGHOLO_PRECISION_SPLIT16):
+        d = O.relative_l2(out[prec, 1], out[prec, 0])
+        print(f"{name} precision {prec}: fused vs separate {d:.2e}", flush=True)
+        assert d < 2e-6
+    assert lib.digHoloConfigSetFusedOverlap(h, 2) == api.DIGHOLO_ERROR_INVALIDARGUMENT
+    lib.digHoloDestroy(h)
+
+
 def test_large_window_high_order(lib):
     frames, cfg = make_case("large1024", 1, seed=12)
     coefs, fields, _, _, h = run_gpu(lib, frames, cfg)
