@@ -1287,6 +1287,20 @@ int enqueue_batch(Handle& h, int step = 0) {
       DH_TRY(hipEventRecord(ev[3], h.stream), "event");
       DH_TRY(hipEventRecord(ev[4], h.stream), "event");
     }
+  } else if (A == 1 && !fullRes && dh::staged_ip_supported(g, *pRow, *pCol, *pIx, *pIy)) {
+    // staged kernels on in-place transforms: rows, then columns + window + inverse along y,
+    // then inverse along x + reference wave
+    DH_TRY(h.rowspec.ensure(nFrames * P * g.H * g.w), "alloc");
+    DH_TRY(h.win.ensure(B * P * g.h * g.w), "alloc");
+    DH_TRY(dh::launch_row_fft_ip(gA, *pRow, h.twW.p, dframes, h.frameU16, h.rowspec.p, h.stream), "row FFT");
+    if (timing) DH_TRY(hipEventRecord(ev[2], h.stream), "event");
+    DH_TRY(dh::launch_col_window_iy(g, *pCol, *pIy, h.rowspec.p, h.factor.p, h.win.p, cal, h.stream),
+           "column FFT");
+    if (timing) DH_TRY(hipEventRecord(ev[3], h.stream), "event");
+    DH_TRY(dh::launch_ifft_rows_tilt(gF, *pIx, h.win.p, h.ref.p, h.fields.p, h.stream), "IFFT rows");
+    if (timing) DH_TRY(hipEventRecord(ev[4], h.stream), "event");
+    h.lastPath = 0;
+    h.lastFusedOverlap = false;
   } else {
     // staged kernels through HBM intermediates (any window size; averaging)
     DH_TRY(h.rowspec.ensure(nFrames * P * g.H * g.w), "alloc");

@@ -19,93 +19,15 @@
 // kernels of dh_kernels.hip.
 #include <algorithm>
 #include <type_traits>
-#include <type_traits>
 
-#include "dh_fft.h"
-#include "dh_internal.h"
+#include "dh_fft_ip.h"
 
 namespace dh {
 
 constexpr int kFThreads = 512;
-constexpr int kMaxV = 8;                    // complex values per thread in one stage
 constexpr int kPre = 8;                     // prefetched row elements per thread
 constexpr size_t kFusedLdsMax = 80 * 1024;  // two workgroups per CU
 constexpr int kMaxFPT = 16;                 // window pixels accumulated per thread
-
-__device__ __forceinline__ int fdiv(int q, unsigned mag) {
-  return mag ? (int)__umulhi((unsigned)q, mag) : q;
-}
-
-// In-place radix-R Stockham stage over nb transforms of length n; element
-// (t, k) lives at buf[t*ts + k*es].  tmajor: consecutive threads take
-// consecutive transforms (use when ts == 1), else consecutive butterflies.
-template <int R, bool INV, int NT = kFThreads>
-__device__ __forceinline__ void stage_ip(float2* buf, int n, int nb, int ts, int es, int Ns, unsigned magNbf,
-                                         unsigned magNs, unsigned magNb, bool tmajor, const float2* tw) {
-  constexpr int BMAX = kMaxV / R > 0 ? kMaxV / R : 1;
-  const int nbf = n / R;
-  const int total = nb * nbf;
-  const int step = n / (Ns * R);
-  float2 v[BMAX][R];
-  int wb[BMAX];
-#pragma unroll
-  for (int bb = 0; bb < BMAX; ++bb) {
-    const int q = threadIdx.x + bb * NT;
-    wb[bb] = -1;
-    if (q < total) {
-      int t, j;
-      if (tmajor) {
-        j = fdiv(q, magNb);
-        t = q - j * nb;
-      } else {
-        t = fdiv(q, magNbf);
-        j = q - t * nbf;
-      }
-      const int jm = j - fdiv(j, magNs) * Ns;
-      const int tb = t * ts;
-#pragma unroll
-      for (int r = 0; r < R; ++r) v[bb][r] = buf[tb + (j + r * nbf) * es];
-      if (jm) {
-#pragma unroll
-        for (int r = 1; r < R; ++r) v[bb][r] = cmul(v[bb][r], twiddle<INV>(tw, jm * r * step));
-      }
-      dftR<R, INV>(v[bb]);
-      wb[bb] = tb + ((j - jm) * R + jm) * es;
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int bb = 0; bb < BMAX; ++bb) {
-    if (wb[bb] >= 0) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) buf[wb[bb] + r * Ns * es] = v[bb][r];
-    }
-  }
-  __syncthreads();
-}
-
-// nb transforms, at most `chunk` per in-place pass.
-template <bool INV, int NT = kFThreads>
-__device__ void lds_fft_ip(float2* buf, int nb, int ts, int es, bool tmajor, const FFTPlan& pl, const float2* tw,
-                           int chunk) {
-  for (int t0 = 0; t0 < nb; t0 += chunk) {
-    const int cnt = min(chunk, nb - t0);
-    const unsigned magNb = tmajor ? magic_div((unsigned)cnt) : 0u;
-    float2* b = buf + t0 * ts;
-    int Ns = 1;
-    for (int s = 0; s < pl.nstages; ++s) {
-      const int R = pl.radix[s];
-      switch (R) {
-        case 8: stage_ip<8, INV, NT>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
-        case 4: stage_ip<4, INV, NT>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
-        case 2: stage_ip<2, INV, NT>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
-        case 3: stage_ip<3, INV, NT>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
-        default: stage_ip<5, INV, NT>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw); break;
-      }
-      Ns *= R;
-    }
-  }
-}
 
 struct FusedLayout {
   int regionX;   // float2 elements shared by (row buffer + band) and the field
@@ -216,7 +138,7 @@ __global__ __launch_bounds__(kFThreads, 4) void k_fused_window(FusedArgs a, Fuse
         if (nr < R) fetch_rows(pre, frames, u16, fbase, g.frameW, R, nr, nb * RB, min(RB, Hc - nb * RB), n, mg.n);
       }
       __syncthreads();
-      lds_fft_ip<false>(rowbuf, rows, n, 1, false, a.row, twn, RB);
+      lds_fft_ip<false, kFThreads>(rowbuf, rows, n, 1, false, a.row, twn, RB);
       // unpack the packed real transform at the window columns
       for (int e = tid; e < rows * w; e += kFThreads) {
         const int tt = fdiv(e, mg.w), i = e - tt * w;
@@ -231,7 +153,7 @@ __global__ __launch_bounds__(kFThreads, 4) void k_fused_window(FusedArgs a, Fuse
       __syncthreads();
     }
     // Hc-point FFT down each of the w band columns
-    lds_fft_ip<false>(band, w, 1, w, true, a.col, twc, a.chunkCol);
+    lds_fft_ip<false, kFThreads>(band, w, 1, w, true, a.col, twc, a.chunkCol);
     // accumulate this residue into the window pixels each thread owns
 #pragma unroll
     for (int s = 0; s < FPT; ++s) {
@@ -255,8 +177,8 @@ __global__ __launch_bounds__(kFThreads, 4) void k_fused_window(FusedArgs a, Fuse
     if (e < hw) fbuf[e] = cmul(F[s], fac[e]);
   }
   __syncthreads();
-  lds_fft_ip<true>(fbuf, h, w, 1, false, a.ix, twix, a.chunkIx);   // along x
-  lds_fft_ip<true>(fbuf, w, 1, w, true, a.iy, twiy, a.chunkIy);    // along y
+  lds_fft_ip<true, kFThreads>(fbuf, h, w, 1, false, a.ix, twix, a.chunkIx);   // along x
+  lds_fft_ip<true, kFThreads>(fbuf, w, 1, w, true, a.iy, twiy, a.chunkIy);    // along y
   const float2* rp = ref + (size_t)(lam_in(g, b) * g.P + p) * hw;
   float2* out = fields + (size_t)(out_pos(g, b) * g.P + p) * hw;
   for (int e = tid; e < hw; e += kFThreads) {
@@ -265,26 +187,14 @@ __global__ __launch_bounds__(kFThreads, 4) void k_fused_window(FusedArgs a, Fuse
   }
 }
 
-// Most transforms of plan pl one in-place pass of nt threads can hold (0:
-// unsupported radix).
-static int ip_chunk(const FFTPlan& pl, int nt = kFThreads) {
-  int best = 1 << 30;
-  for (int s = 0; s < pl.nstages; ++s) {
-    const int R = pl.radix[s];
-    if (R != 2 && R != 3 && R != 4 && R != 5 && R != 8) return 0;
-    const int bmax = kMaxV / R > 0 ? kMaxV / R : 1;
-    best = std::min(best, bmax * nt * R / pl.n);
-  }
-  return best;
-}
-
 static int fpt_of(const FusedArgs& a) { return (a.g.h * a.g.w + kFThreads - 1) / kFThreads; }
 
 bool fused_prepare(FusedArgs& a) {
   if (a.g.fw != a.g.w || a.g.fh != a.g.h) return false;
   if (a.R < 1 || a.Hc * a.R != a.g.H) return false;
   if (fpt_of(a) > kMaxFPT) return false;
-  const int cr = ip_chunk(a.row), cc = ip_chunk(a.col), cx = ip_chunk(a.ix), cy = ip_chunk(a.iy);
+  const int cr = ip_chunk(a.row, kFThreads), cc = ip_chunk(a.col, kFThreads);
+  const int cx = ip_chunk(a.ix, kFThreads), cy = ip_chunk(a.iy, kFThreads);
   if (cr < 1 || cc < 1 || cx < 1 || cy < 1) return false;
   a.RB = std::min(std::min(a.Hc, cr), kPre * kFThreads / a.row.n);
   if (a.RB < 1) return false;
@@ -399,7 +309,7 @@ __global__ __launch_bounds__(kFThreads, 4) void k_window_mfma(FusedArgs a, Fused
     }
     if (ch + 1 < nch) fetch_rows(pre, frames, 1, fbase, g.frameW, 1, 0, (ch + 1) * RB, RB, n, mg.n);
     __syncthreads();
-    lds_fft_ip<false>(rowbuf, RB, n, 1, false, a.row, twn, RB);
+    lds_fft_ip<false, kFThreads>(rowbuf, RB, n, 1, false, a.row, twn, RB);
     // unpack the window columns of the packed real transform, scale, split into B
     for (int e = tid; e < RB * w; e += kFThreads) {
       const int tt = fdiv(e, mg.w), i = e - tt * w;
@@ -467,8 +377,8 @@ __global__ __launch_bounds__(kFThreads, 4) void k_window_mfma(FusedArgs a, Fused
   const float2* fac = factor + (size_t)(lam_in(g, b) * g.P + p) * hw;
   for (int e = tid; e < hw; e += kFThreads) fbuf[e] = cmul(fbuf[e], fac[e]);
   __syncthreads();
-  lds_fft_ip<true>(fbuf, h, w, 1, false, a.ix, twix, a.chunkIx);   // along x
-  lds_fft_ip<true>(fbuf, w, 1, w, true, a.iy, twiy, a.chunkIy);    // along y
+  lds_fft_ip<true, kFThreads>(fbuf, h, w, 1, false, a.ix, twix, a.chunkIx);   // along x
+  lds_fft_ip<true, kFThreads>(fbuf, w, 1, w, true, a.iy, twiy, a.chunkIy);    // along y
   const float2* rp = ref + (size_t)(lam_in(g, b) * g.P + p) * hw;
   float2* out = fields + (size_t)(out_pos(g, b) * g.P + p) * hw;
   for (int e = tid; e < hw; e += kFThreads) {
@@ -524,7 +434,7 @@ bool window_mfma_prepare(FusedArgs& a) {
   const bool inst = (cpw == 1 && (nu == 3 || nu == 4 || nu == 5 || nu == 6 || nu == 8)) ||
                     (cpw == 2 && (nu == 5 || nu == 6 || nu == 8));
   if (!inst) return false;
-  const int cr = ip_chunk(a.row), cx = ip_chunk(a.ix), cy = ip_chunk(a.iy);
+  const int cr = ip_chunk(a.row, kFThreads), cx = ip_chunk(a.ix, kFThreads), cy = ip_chunk(a.iy, kFThreads);
   if (cr < 1 || cx < 1 || cy < 1) return false;
   int RB = std::min(32, std::min(cr, kPre * kFThreads / a.row.n));
   RB -= RB % 16;

@@ -102,6 +102,17 @@ hipError_t launch_row_fft(const Geometry& g, const FFTPlan& pl, const float2* tw
 hipError_t launch_col_fft_window(const Geometry& g, const FFTPlan& pl, const float2* rowspec,
                                  const float2* factor, float2* win, const BatchCal& cal, hipStream_t s);
 hipError_t launch_ifft_rows(const Geometry& g, const FFTPlan& pl, float2* win, hipStream_t s);
+// Staged kernels on in-place transforms (radix 2/3/4/5/8 plans only): row
+// FFT, then column FFT + window + inverse FFT along y, then inverse FFT along x
+// + reference wave.  staged_ip_supported says whether the three apply.
+bool staged_ip_supported(const Geometry& g, const FFTPlan& row, const FFTPlan& col, const FFTPlan& ix,
+                         const FFTPlan& iy);
+hipError_t launch_row_fft_ip(const Geometry& g, const FFTPlan& pl, const float2* twW, const void* frames,
+                             int frame_is_u16, float2* rowspec, hipStream_t s);
+hipError_t launch_col_window_iy(const Geometry& g, const FFTPlan& pc, const FFTPlan& py, const float2* rowspec,
+                                const float2* factor, float2* win, const BatchCal& cal, hipStream_t s);
+hipError_t launch_ifft_rows_tilt(const Geometry& g, const FFTPlan& pl, const float2* win, const float2* ref,
+                                 float2* fields, hipStream_t s);
 // ref == nullptr: the inverse transform only (field before tilt removal)
 hipError_t launch_ifft_cols_tilt(const Geometry& g, const FFTPlan& pl, const float2* win,
                                  const float2* ref, float2* fields, hipStream_t s);

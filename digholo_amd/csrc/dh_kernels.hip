@@ -17,8 +17,7 @@
 // (radix 8/4/2/3/5 butterflies, any other prime by a direct DFT).
 #include <algorithm>
 
-#include "dh_fft.h"
-#include "dh_internal.h"
+#include "dh_fft_ip.h"
 
 namespace dh {
 
@@ -250,6 +249,195 @@ __global__ __launch_bounds__(kThreads) void k_ifft_cols_tilt(Geometry g, FFTPlan
     const float2 t = cmul(E[c * n + j], rp[(size_t)j * g.w + c]);
     fields[base + (size_t)j * g.w + c] = make_float2(-t.x, t.y);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Staged path on in-place transforms (every radix 2/3/4/5/8).  Each block
+// stages its transforms' twiddles in LDS, index arithmetic is multiply-high
+// division, and the inverse transform along y runs in the column kernel, so
+// the window plane makes one trip through HBM fewer than in the kernels above:
+//   frames  --k_row_fft_ip-----> rowspec[B][P][H][w]
+//   rowspec --k_col_window_iy--> win[B][P][h][w]     column FFT, window rows,
+//                                                    factor, inverse FFT along y
+//   win     --k_ifft_rows_tilt-> fields[B][P][h][w]  inverse FFT along x,
+//                                                    V = -conj(E ref)
+// ---------------------------------------------------------------------------
+constexpr int kColThreads = 512;   // column kernel: one stage pass covers twice the columns
+constexpr int kSel = 8;            // window values per thread held across the row selection
+constexpr int kLd = 8;             // global loads per thread in flight in the copy loops
+
+// for e in [0, cnt): put(e, get(e)), kLd loads issued before the first is used
+template <int NT = kThreads, class Get, class Put>
+__device__ __forceinline__ void batched(int cnt, Get&& get, Put&& put) {
+  for (int e0 = threadIdx.x; e0 < cnt; e0 += kLd * NT) {
+    float2 v[kLd];
+#pragma unroll
+    for (int k = 0; k < kLd; ++k) {
+      const int e = e0 + k * NT;
+      if (e < cnt) v[k] = get(e);
+    }
+#pragma unroll
+    for (int k = 0; k < kLd; ++k) {
+      const int e = e0 + k * NT;
+      if (e < cnt) put(e, v[k]);
+    }
+  }
+}
+
+// grid (ceil(H/RB), P, B); rows as in k_row_fft
+__global__ __launch_bounds__(kThreads) void k_row_fft_ip(Geometry g, FFTPlan pl, const float2* __restrict__ twW,
+                                                         const void* __restrict__ frames, int u16,
+                                                         float2* __restrict__ rowspec, int RB, int chunk,
+                                                         unsigned magN, unsigned magW) {
+  extern __shared__ float2 smem[];
+  const int n = pl.n, w = g.w;
+  float2* tw = smem;          // [n] transform twiddles
+  float2* tu = smem + n;      // [w] unpack twiddles exp(-2 pi i k / W) of the window columns
+  float2* A = tu + w;         // [RB][LD], point k of row r at r LD + pad32(k)
+  const int LD = pad_stride(n);
+  const int b = blockIdx.z, p = blockIdx.y, y0 = blockIdx.x * RB;
+  const int rows = min(RB, g.H - y0);
+  const int c0 = g.c0[p];
+  for (int e = threadIdx.x; e < n; e += kThreads) tw[e] = pl.tw[e];
+  for (int e = threadIdx.x; e < w; e += kThreads) tu[e] = twW[c0 + e];
+  const size_t fbase = (size_t)b * g.frameH * g.frameW + (size_t)(g.cy[p] + y0) * g.frameW + g.cx[p];
+  auto put = [&](int e, float2 v) {
+    const int r = fdiv(e, magN);
+    A[r * LD + pad32(e - r * n)] = v;
+  };
+  if (u16) {
+    const unsigned short* f = static_cast<const unsigned short*>(frames) + fbase;
+    batched(rows * n, [&](int e) {
+      const int r = fdiv(e, magN), k = e - r * n;
+      const ushort2 v = *reinterpret_cast<const ushort2*>(f + (size_t)r * g.frameW + 2 * k);
+      return make_float2((float)v.x, (float)v.y);
+    }, put);
+  } else {
+    const float* f = static_cast<const float*>(frames) + fbase;
+    batched(rows * n, [&](int e) {
+      const int r = fdiv(e, magN), k = e - r * n;
+      return *reinterpret_cast<const float2*>(f + (size_t)r * g.frameW + 2 * k);
+    }, put);
+  }
+  __syncthreads();
+  lds_fft_ip<false, kThreads, true>(A, rows, LD, 1, false, pl, tw, chunk);
+  float2* out = rowspec + ((size_t)(b * g.P + p) * g.H + y0) * w;
+  for (int e = threadIdx.x; e < rows * w; e += kThreads) {
+    const int r = fdiv(e, magW), kk = e - r * w;
+    const int k = c0 + kk;                   // k <= n (= W/2, the Nyquist column)
+    const float2 zk = A[r * LD + pad32(k == n ? 0 : k)];
+    float2 zc = A[r * LD + pad32((k == 0 || k == n) ? 0 : n - k)];
+    zc.y = -zc.y;
+    const float2 ev = cscale(cadd(zk, zc), 0.5f);
+    const float2 od = mul_mi<false>(cscale(csub(zk, zc), 0.5f));   // (zk - zc)/(2i)
+    out[e] = cadd(ev, cmul(tu[kk], od));
+  }
+}
+
+// grid (ceil(w/CB), P, B).  Loads and stores move the strip's CB * 8
+// contiguous bytes per row; in LDS the strip is column-major, each column
+// padded.  h * CB <= kThreads * kSel (the selected window is held in registers while
+// it moves to the top of the strip).
+__global__ __launch_bounds__(kColThreads) void k_col_window_iy(Geometry g, FFTPlan pc, FFTPlan py,
+                                                            const float2* __restrict__ rowspec,
+                                                            const float2* __restrict__ factor,
+                                                            float2* __restrict__ win, int CB, int chunkC,
+                                                            int chunkY, BatchCal cal) {
+  extern __shared__ float2 smem[];
+  const int H = g.H, h = g.h, w = g.w;
+  float2* twc = smem;
+  float2* twy = smem + H;
+  float2* A = twy + h;        // [CB][LD], point y of column c at c LD + pad32(y)
+  const int LD = pad_stride(H);
+  const int b = blockIdx.z, p = blockIdx.y, i0 = blockIdx.x * CB;
+  const int cols = min(CB, w - i0);
+  const unsigned magC = magic_div((unsigned)cols);
+  for (int e = threadIdx.x; e < H; e += kColThreads) twc[e] = pc.tw[e];
+  for (int e = threadIdx.x; e < h; e += kColThreads) twy[e] = py.tw[e];
+  const float2* in = rowspec + (size_t)(b * g.P + p) * H * w + i0;
+  batched<kColThreads>(H * cols, [&](int e) {
+    const int y = fdiv(e, magC), c = e - y * cols;
+    return in[(size_t)y * w + c];
+  }, [&](int e, float2 v) {
+    const int y = fdiv(e, magC), c = e - y * cols;
+    A[c * LD + pad32(y)] = v;
+  });
+  // window row j is spectrum row (j + SY + H/2) mod H; its factor is loaded
+  // while the column transform runs
+  int sy = (g.SY[p] + H / 2) % H;
+  if (sy < 0) sy += H;
+  const float2* fac = factor + (size_t)(lam_in(g, b) * g.P + p) * h * w + i0;
+  float2 sel[kSel];
+#pragma unroll
+  for (int s = 0; s < kSel; ++s) {
+    const int e = threadIdx.x + s * kColThreads;
+    if (e < h * cols) {
+      const int j = fdiv(e, magC), c = e - j * cols;
+      sel[s] = fac[(size_t)j * w + c];
+    }
+  }
+  __syncthreads();
+  lds_fft_ip<false, kColThreads, true>(A, cols, LD, 1, false, pc, twc, chunkC);
+  const float2 cb = cal.at(p, b);
+#pragma unroll
+  for (int s = 0; s < kSel; ++s) {
+    const int e = threadIdx.x + s * kColThreads;
+    if (e < h * cols) {
+      const int j = fdiv(e, magC), c = e - j * cols;
+      int row = j + sy;
+      if (row >= H) row -= H;
+      sel[s] = cmul(cmul(A[c * LD + pad32(row)], sel[s]), cb);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < kSel; ++s) {
+    const int e = threadIdx.x + s * kColThreads;
+    if (e < h * cols) {
+      const int j = fdiv(e, magC), c = e - j * cols;
+      A[c * LD + pad32(j)] = sel[s];
+    }
+  }
+  __syncthreads();
+  lds_fft_ip<true, kColThreads, true>(A, cols, LD, 1, false, py, twy, chunkY);
+  float2* out = win + (size_t)(out_pos(g, b) * g.P + p) * h * w + i0;
+  for (int e = threadIdx.x; e < h * cols; e += kColThreads) {
+    const int j = fdiv(e, magC), c = e - j * cols;
+    out[(size_t)j * w + c] = A[c * LD + pad32(j)];
+  }
+}
+
+// grid (ceil(h/RB), P, B); win holds the plane already transformed along y.
+// ref null: the field before the reference wave is removed.
+__global__ __launch_bounds__(kThreads) void k_ifft_rows_tilt(Geometry g, FFTPlan pl, const float2* __restrict__ win,
+                                                             const float2* __restrict__ ref,
+                                                             float2* __restrict__ fields, int RB, int chunk) {
+  extern __shared__ float2 smem[];
+  const int n = pl.n;   // = w
+  float2* tw = smem;
+  float2* A = smem + n;        // [RB][LD], point k of row r at r LD + pad32(k)
+  const int LD = pad_stride(n);
+  const unsigned magN = magic_div((unsigned)n);
+  auto at = [&](int e) -> float2& {
+    const int r = fdiv(e, magN);
+    return A[r * LD + pad32(e - r * n)];
+  };
+  const int b = blockIdx.z, p = blockIdx.y, y0 = blockIdx.x * RB;
+  const int rows = min(RB, g.h - y0);
+  for (int e = threadIdx.x; e < n; e += kThreads) tw[e] = pl.tw[e];
+  const size_t base = ((size_t)(b * g.P + p) * g.h + y0) * n;
+  batched(rows * n, [&](int e) { return win[base + e]; }, [&](int e, float2 v) { at(e) = v; });
+  __syncthreads();
+  lds_fft_ip<true, kThreads, true>(A, rows, LD, 1, false, pl, tw, chunk);
+  if (!ref) {
+    for (int e = threadIdx.x; e < rows * n; e += kThreads) fields[base + e] = at(e);
+    return;
+  }
+  const float2* rp = ref + ((size_t)(lam_out(g, b) * g.P + p) * g.h + y0) * n;
+  batched(rows * n, [&](int e) { return rp[e]; }, [&](int e, float2 r) {
+    const float2 t = cmul(at(e), r);
+    fields[base + e] = make_float2(-t.x, t.y);
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -565,6 +753,68 @@ hipError_t launch_ifft_rows(const Geometry& g, const FFTPlan& pl, float2* win, h
   if (e != hipSuccess) return e;
   dim3 grid((g.h + RB - 1) / RB, g.P, g.B);
   k_ifft_rows<<<grid, kThreads, lds, s>>>(g, pl, win, RB);
+  return hipGetLastError();
+}
+
+// In-place staged kernels: rows per block of the row transforms, columns per
+// block of the column kernel (0 when a transform has no in-place butterfly or
+// the strip does not fit).
+static int ip_rows_per_block(const FFTPlan& pl, int total) {
+  const int chunk = ip_chunk(pl, kThreads);
+  if (chunk < 1) return 0;
+  return std::min(total, std::max(chunk, 4096 / pl.n));
+}
+
+static int ip_cols_per_block(const Geometry& g, const FFTPlan& pc, const FFTPlan& py) {
+  if (ip_chunk(pc, kColThreads) < 1 || ip_chunk(py, kColThreads) < 1) return 0;
+  for (int cb = std::min(16, g.w); cb >= 1; --cb) {
+    const size_t lds = sizeof(float2) * ((size_t)g.H + g.h + (size_t)pad_stride(g.H) * cb);
+    if (g.h * cb <= kColThreads * kSel && lds <= 80 * 1024) return cb;
+  }
+  return 0;
+}
+
+bool staged_ip_supported(const Geometry& g, const FFTPlan& row, const FFTPlan& col, const FFTPlan& ix,
+                         const FFTPlan& iy) {
+  return ip_rows_per_block(row, g.H) > 0 && ip_rows_per_block(ix, g.h) > 0 && ix.n == g.w && iy.n == g.h &&
+         ip_cols_per_block(g, col, iy) > 0;
+}
+
+hipError_t launch_row_fft_ip(const Geometry& g, const FFTPlan& pl, const float2* twW, const void* frames,
+                             int frame_is_u16, float2* rowspec, hipStream_t s) {
+  const int RB = ip_rows_per_block(pl, g.H);
+  if (RB < 1) return hipErrorNotSupported;
+  const size_t lds = sizeof(float2) * ((size_t)pl.n + g.w + (size_t)pad_stride(pl.n) * RB);
+  hipError_t e = set_lds(k_row_fft_ip, lds);
+  if (e != hipSuccess) return e;
+  dim3 grid((g.H + RB - 1) / RB, g.P, g.B);
+  k_row_fft_ip<<<grid, kThreads, lds, s>>>(g, pl, twW, frames, frame_is_u16, rowspec, RB, ip_chunk(pl, kThreads),
+                                           magic_div((unsigned)pl.n), magic_div((unsigned)g.w));
+  return hipGetLastError();
+}
+
+hipError_t launch_col_window_iy(const Geometry& g, const FFTPlan& pc, const FFTPlan& py, const float2* rowspec,
+                                const float2* factor, float2* win, const BatchCal& cal, hipStream_t s) {
+  const int CB = ip_cols_per_block(g, pc, py);
+  if (CB < 1 || py.n != g.h) return hipErrorNotSupported;
+  const size_t lds = sizeof(float2) * ((size_t)g.H + g.h + (size_t)pad_stride(g.H) * CB);
+  hipError_t e = set_lds(k_col_window_iy, lds);
+  if (e != hipSuccess) return e;
+  dim3 grid((g.w + CB - 1) / CB, g.P, g.B);
+  k_col_window_iy<<<grid, kColThreads, lds, s>>>(g, pc, py, rowspec, factor, win, CB, ip_chunk(pc, kColThreads),
+                                                 ip_chunk(py, kColThreads), cal);
+  return hipGetLastError();
+}
+
+hipError_t launch_ifft_rows_tilt(const Geometry& g, const FFTPlan& pl, const float2* win, const float2* ref,
+                                 float2* fields, hipStream_t s) {
+  const int RB = ip_rows_per_block(pl, g.h);
+  if (RB < 1 || pl.n != g.w) return hipErrorNotSupported;
+  const size_t lds = sizeof(float2) * ((size_t)pl.n + (size_t)pad_stride(pl.n) * RB);
+  hipError_t e = set_lds(k_ifft_rows_tilt, lds);
+  if (e != hipSuccess) return e;
+  dim3 grid((g.h + RB - 1) / RB, g.P, g.B);
+  k_ifft_rows_tilt<<<grid, kThreads, lds, s>>>(g, pl, win, ref, fields, RB, ip_chunk(pl, kThreads));
   return hipGetLastError();
 }
 

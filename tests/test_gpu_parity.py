@@ -50,6 +50,10 @@ def make_case(name, B, seed, quantise=True, **over):
                                   group_count=80, beam_centre_x=[-2.56e-3, 2.56e-3], group_decay=20.0)
         cfg = dict(frame_width=1024, frame_height=1024, pol_count=2, fft_window=(512, 512), waist=300e-6,
                    group_count=80, beam_centre_x=(-2.56e-3, 2.56e-3))
+    elif name == "radix7":
+        spec = synth.HologramSpec(frame_width=224, frame_height=224, pol_count=1, waist=120e-6, group_count=8)
+        cfg = dict(frame_width=224, frame_height=224, pol_count=1, fft_window=(112, 112), waist=120e-6,
+                   group_count=8)
     else:
         raise KeyError(name)
     for k, v in over.pop("spec", {}).items():
@@ -232,6 +236,22 @@ def test_large_window_high_order(lib):
     coefs, fields, _, _, h = run_gpu(lib, frames, cfg)
     assert lib.digHoloGetWindowPath(h) == 0          # 160x160 field: staged kernels
     check(frames, cfg, coefs, fields)
+
+
+@pytest.mark.parametrize("name,B", [("radix7", 3), ("small", 3)])
+def test_staged_kernels_any_radix(lib, name, B):
+    """The staged kernels: a 56-point row transform (radix 7) has no in-place
+    butterfly and takes the ping-pong kernels, the 128-point one the in-place
+    kernels; both against the oracle."""
+    frames, cfg = make_case(name, B, seed=4)
+    print(f"case {name} B={B}", flush=True)
+    h = lib.digHoloCreate()
+    api.configure(lib, h, **cfg)
+    assert lib.digHoloConfigSetFusedWindow(h, 0) == 0
+    coefs, fields, _, _, h = run_gpu(lib, frames, cfg, h=h)
+    assert lib.digHoloGetWindowPath(h) == 0
+    check(frames, cfg, coefs, fields)
+    lib.digHoloDestroy(h)
 
 
 @pytest.mark.parametrize("name,B", [("small", 4), ("dual640", 2)])
