@@ -931,7 +931,8 @@ int autotune(Handle& h) {
   if (h.autoTune != 1) return DIGHOLO_ERROR_SUCCESS;
   if (int e = ensure_device(h)) return e;
   h.tuning = true;
-  const int minTpw = d.g.G > 0 ? dh::overlap_min_tiles_per_wave(d.g) : 0;
+  const int minTpw =
+      d.g.G > 0 ? dh::overlap_min_tiles_per_wave(d.g, c.precision == DIGHOLO_PRECISION_SPLIT16) : 0;
   hipEvent_t e0, e1;
   DH_TRY(hipEventCreate(&e0), "event create");
   DH_TRY(hipEventCreate(&e1), "event create");

@@ -254,7 +254,7 @@ hipError_t launch_grad_reduce(int B, int P, const double* partial, double* out, 
 // triangle, overlap_min_tiles_per_wave; larger values give each wave more).
 hipError_t launch_overlap_mfma(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
                                float2* coefs, int split, hipStream_t s, int tpw = 0);
-int overlap_min_tiles_per_wave(const Geometry& g);
+int overlap_min_tiles_per_wave(const Geometry& g, int split);
 
 // Reference-wave calibration plane (dh_refcal.hip).
 // tw[m] = exp(-2 pi i m / N), FP64, m < N

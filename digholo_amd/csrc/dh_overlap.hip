@@ -3,7 +3,8 @@
 //   stage 1:  T[(y,c)][m] = sum_x V_c[y][x] HGx[m][x]        (c = re/im)
 //   stage 2:  C[(m,c)][n] += sum_y T[(y,c)][m] HGy[n][y]      -> coef(m,n), m+n < G
 //
-// One workgroup (4 waves) per pol and frame slot walks the frames b =
+// One workgroup (4 waves; 8 when its LDS leaves room for only one workgroup
+// per CU) per pol and frame slot walks the frames b =
 // slot, slot + slots, ... (the basis stays in LDS for all of them) and each
 // field in chunks of 16 field rows; the next chunk's field values are loaded
 // into registers while the current chunk is multiplied.  Per chunk, stage 1 is a (32 x Gp x w) product whose
@@ -33,7 +34,6 @@ namespace dh {
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-constexpr int kOvThreads = 256;
 constexpr int kChunk = 16;          // field rows per chunk (K of stage 2)
 
 template <bool SPLIT>
@@ -50,8 +50,8 @@ __device__ __forceinline__ void split16(float x, _Float16& hi, _Float16& lo) {
 // spread 16 consecutive rows over the banks
 __host__ __device__ constexpr int pitch_of(int K) { return K + 8; }
 
-template <bool SPLIT, int TPW>
-__global__ __launch_bounds__(kOvThreads) void k_overlap_mfma(Geometry g, const float2* __restrict__ fields,
+template <bool SPLIT, int TPW, int NW>
+__global__ __launch_bounds__(64 * NW) void k_overlap_mfma(Geometry g, const float2* __restrict__ fields,
                                                              const float* __restrict__ hgx,
                                                              const float* __restrict__ hgy,
                                                              float2* __restrict__ coefs) {
@@ -66,12 +66,13 @@ __global__ __launch_bounds__(kOvThreads) void k_overlap_mfma(Geometry g, const f
   E* By = Bx + nL * Gp * pw;                      // [nL][Gp][ph]   HGy[n][y]
   E* A1 = By + nL * Gp * ph;                      // [nL][32][pw]   V chunk, rows (yy, c)
   E* A2 = A1 + nL * 32 * pw;                      // [nL][2Gp][pk]  T chunk, rows (m, c)
-  __shared__ float s_red[kOvThreads / 64];
+  constexpr int NT = 64 * NW;
+  __shared__ float s_red[NW];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, q = lane >> 4;
 
   // --- basis into LDS (rows m >= G are zero), once for every frame of the slot ---
-  for (int e = tid; e < Gp * w; e += kOvThreads) {
+  for (int e = tid; e < Gp * w; e += NT) {
     const int m = e / w, x = e - m * w;
     const float v = m < G ? hgx[((size_t)p * G + m) * w + x] : 0.f;
     if (SPLIT) {
@@ -83,7 +84,7 @@ __global__ __launch_bounds__(kOvThreads) void k_overlap_mfma(Geometry g, const f
       Bx[m * pw + x] = v;
     }
   }
-  for (int e = tid; e < Gp * h; e += kOvThreads) {
+  for (int e = tid; e < Gp * h; e += NT) {
     const int n = e / h, y = e - n * h;
     const float v = n < G ? hgy[((size_t)p * G + n) * h + y] : 0.f;
     if (SPLIT) {
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(kOvThreads) void k_overlap_mfma(Geometry g, const f
     for (int t = 0; t < rt2 * ct2; ++t) {
       const int mt = t / ct2, nt = t - mt * ct2;
       if (8 * mt + 16 * nt >= G) continue;          // every (m, n) in the tile has m + n >= G
-      if (cnt % 4 == wave && nMine < TPW) myTile[nMine++] = t;
+      if (cnt % NW == wave && nMine < TPW) myTile[nMine++] = t;
       ++cnt;
     }
   }
@@ -113,8 +114,8 @@ __global__ __launch_bounds__(kOvThreads) void k_overlap_mfma(Geometry g, const f
   // this thread's elements of a chunk: e = tid + 256 j < 16 w; held in registers
   // one chunk ahead when w <= 128 (j < 8), else loaded as they are stored
   constexpr int kPerT = 8;
-  const int nPer = (kChunk * w - tid + kOvThreads - 1) / kOvThreads;
-  const bool ahead = kChunk * w <= kPerT * kOvThreads;
+  const int nPer = (kChunk * w - tid + NT - 1) / NT;
+  const bool ahead = kChunk * w <= kPerT * NT;
   for (int b = blockIdx.y; b < g.B; b += gridDim.y) {
   const float2* V = fields + (size_t)(b * g.P + p) * h * w;
 
@@ -122,7 +123,7 @@ __global__ __launch_bounds__(kOvThreads) void k_overlap_mfma(Geometry g, const f
   float scale = 1.f;
   if (SPLIT) {
     float mx = 0.f;
-    for (int e = tid; e < h * w; e += kOvThreads) {
+    for (int e = tid; e < h * w; e += NT) {
       const float2 v = V[e];
       mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)));
     }
@@ -130,7 +131,9 @@ __global__ __launch_bounds__(kOvThreads) void k_overlap_mfma(Geometry g, const f
     __syncthreads();   // s_red of the previous frame has been read
     if (lane == 0) s_red[wave] = mx;
     __syncthreads();
-    mx = fmaxf(fmaxf(s_red[0], s_red[1]), fmaxf(s_red[2], s_red[3]));
+    mx = s_red[0];
+#pragma unroll
+    for (int i = 1; i < NW; ++i) mx = fmaxf(mx, s_red[i]);
     if (mx > 0.f) {
       int ex;
       frexpf(mx, &ex);                 // mx in [2^(ex-1), 2^ex)
@@ -145,7 +148,7 @@ __global__ __launch_bounds__(kOvThreads) void k_overlap_mfma(Geometry g, const f
   float2 nxt[kPerT];   // the next chunk's values, in flight during the current chunk
 #pragma unroll
   for (int j = 0; j < kPerT; ++j)
-    if (ahead && j < nPer) nxt[j] = V[tid + kOvThreads * j];
+    if (ahead && j < nPer) nxt[j] = V[tid + NT * j];
   for (int y0 = 0; y0 < h; y0 += kChunk) {
     __syncthreads();   // previous chunk's A1/A2 readers are done (and basis/scale ready)
     // V chunk -> A1 rows (yy, c)
@@ -168,18 +171,18 @@ __global__ __launch_bounds__(kOvThreads) void k_overlap_mfma(Geometry g, const f
 #pragma unroll
       for (int j = 0; j < kPerT; ++j) {
         if (j < nPer) {
-          const int e = tid + kOvThreads * j;
+          const int e = tid + NT * j;
           const float2 v = nxt[j];
           if (y0 + kChunk < h) nxt[j] = V[(size_t)(y0 + kChunk) * w + e];
           store_a1(e, v);
         }
       }
     } else {
-      for (int e = tid; e < kChunk * w; e += kOvThreads) store_a1(e, V[(size_t)y0 * w + e]);
+      for (int e = tid; e < kChunk * w; e += NT) store_a1(e, V[(size_t)y0 * w + e]);
     }
     __syncthreads();
     // stage 1: tiles (rt in 0..1) x (ct in 0..Gp/16)
-    for (int t = wave; t < rt1 * ct1; t += 4) {
+    for (int t = wave; t < rt1 * ct1; t += NW) {
       const int rt = t / ct1, ct = t - rt * ct1;
       floatx4 d = {0.f, 0.f, 0.f, 0.f};
       const E* arow = A1 + (rt * 16 + lr) * pw;
@@ -302,7 +305,10 @@ static size_t overlap_lds(const Geometry& g, bool split) {
                    2 * (size_t)Gp * pitch_of(kChunk));
 }
 
-static int tiles_per_wave(const Geometry& g) {
+// waves per workgroup: 8 when the LDS holds a single workgroup per CU
+static int ov_waves(const Geometry& g, bool split) { return overlap_lds(g, split) > 80 * 1024 ? 8 : 4; }
+
+static int tiles_per_wave(const Geometry& g, int nw) {
   const int Gp = (g.G + 15) & ~15;
   const int rt2 = (2 * Gp) / 16, ct2 = Gp / 16;
   int cnt = 0;
@@ -310,38 +316,45 @@ static int tiles_per_wave(const Geometry& g) {
     const int mt = t / ct2, nt = t - mt * ct2;
     if (8 * mt + 16 * nt < g.G) ++cnt;
   }
-  return (cnt + 3) / 4;
+  return (cnt + nw - 1) / nw;
 }
 
-template <bool SPLIT, int TPW>
+template <bool SPLIT, int TPW, int NW>
 static hipError_t ov_launch(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
                             float2* coefs, hipStream_t s) {
   const size_t lds = overlap_lds(g, SPLIT);
   if (lds > 64 * 1024) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_overlap_mfma<SPLIT, TPW>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_overlap_mfma<SPLIT, TPW, NW>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
   // about two resident workgroups per CU (256 CUs), each walking several frames
   const int slots = std::max(1, std::min(g.B, 512 / std::max(1, g.P)));
   dim3 grid(g.P, slots);
-  k_overlap_mfma<SPLIT, TPW><<<grid, kOvThreads, lds, s>>>(g, fields, hgx, hgy, coefs);
+  k_overlap_mfma<SPLIT, TPW, NW><<<grid, 64 * NW, lds, s>>>(g, fields, hgx, hgy, coefs);
   return hipGetLastError();
+}
+
+template <bool SPLIT, int NW>
+static hipError_t ov_dispatch_nw(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
+                                 float2* coefs, int tpwWanted, hipStream_t s) {
+  const int tpw = std::max(tiles_per_wave(g, NW), tpwWanted);
+  if (tpw <= 2) return ov_launch<SPLIT, 2, NW>(g, fields, hgx, hgy, coefs, s);
+  if (tpw <= 4) return ov_launch<SPLIT, 4, NW>(g, fields, hgx, hgy, coefs, s);
+  if (tpw <= 8) return ov_launch<SPLIT, 8, NW>(g, fields, hgx, hgy, coefs, s);
+  if (tpw <= 16) return ov_launch<SPLIT, 16, NW>(g, fields, hgx, hgy, coefs, s);
+  if (tpw <= 32) return ov_launch<SPLIT, 32, NW>(g, fields, hgx, hgy, coefs, s);
+  return hipErrorInvalidValue;
 }
 
 template <bool SPLIT>
 static hipError_t ov_dispatch(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
                               float2* coefs, int tpwWanted, hipStream_t s) {
-  const int tpw = std::max(tiles_per_wave(g), tpwWanted);
-  if (tpw <= 2) return ov_launch<SPLIT, 2>(g, fields, hgx, hgy, coefs, s);
-  if (tpw <= 4) return ov_launch<SPLIT, 4>(g, fields, hgx, hgy, coefs, s);
-  if (tpw <= 8) return ov_launch<SPLIT, 8>(g, fields, hgx, hgy, coefs, s);
-  if (tpw <= 16) return ov_launch<SPLIT, 16>(g, fields, hgx, hgy, coefs, s);
-  if (tpw <= 32) return ov_launch<SPLIT, 32>(g, fields, hgx, hgy, coefs, s);
-  return hipErrorInvalidValue;
+  return ov_waves(g, SPLIT) == 8 ? ov_dispatch_nw<SPLIT, 8>(g, fields, hgx, hgy, coefs, tpwWanted, s)
+                                 : ov_dispatch_nw<SPLIT, 4>(g, fields, hgx, hgy, coefs, tpwWanted, s);
 }
 
-int overlap_min_tiles_per_wave(const Geometry& g) { return tiles_per_wave(g); }
+int overlap_min_tiles_per_wave(const Geometry& g, int split) { return tiles_per_wave(g, ov_waves(g, split != 0)); }
 
 hipError_t launch_overlap_mfma(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
                                float2* coefs, int split, hipStream_t s, int tpw) {
