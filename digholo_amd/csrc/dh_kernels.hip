@@ -262,7 +262,8 @@ __global__ __launch_bounds__(kThreads) void k_ifft_cols_tilt(Geometry g, FFTPlan
 //   win     --k_ifft_rows_tilt-> fields[B][P][h][w]  inverse FFT along x,
 //                                                    V = -conj(E ref)
 // ---------------------------------------------------------------------------
-constexpr int kColThreads = 512;   // column kernel: one stage pass covers twice the columns
+constexpr int kRowThreads = 512;   // row and column kernels: one stage pass covers twice the
+constexpr int kColThreads = 512;   // transforms of a 256-thread block
 constexpr int kSel = 8;            // window values per thread held across the row selection
 constexpr int kLd = 8;             // global loads per thread in flight in the copy loops
 
@@ -285,7 +286,7 @@ __device__ __forceinline__ void batched(int cnt, Get&& get, Put&& put) {
 }
 
 // grid (ceil(H/RB), P, B); rows as in k_row_fft
-__global__ __launch_bounds__(kThreads) void k_row_fft_ip(Geometry g, FFTPlan pl, const float2* __restrict__ twW,
+__global__ __launch_bounds__(kRowThreads) void k_row_fft_ip(Geometry g, FFTPlan pl, const float2* __restrict__ twW,
                                                          const void* __restrict__ frames, int u16,
                                                          float2* __restrict__ rowspec, int RB, int chunk,
                                                          unsigned magN, unsigned magW) {
@@ -298,8 +299,8 @@ __global__ __launch_bounds__(kThreads) void k_row_fft_ip(Geometry g, FFTPlan pl,
   const int b = blockIdx.z, p = blockIdx.y, y0 = blockIdx.x * RB;
   const int rows = min(RB, g.H - y0);
   const int c0 = g.c0[p];
-  for (int e = threadIdx.x; e < n; e += kThreads) tw[e] = pl.tw[e];
-  for (int e = threadIdx.x; e < w; e += kThreads) tu[e] = twW[c0 + e];
+  for (int e = threadIdx.x; e < n; e += kRowThreads) tw[e] = pl.tw[e];
+  for (int e = threadIdx.x; e < w; e += kRowThreads) tu[e] = twW[c0 + e];
   const size_t fbase = (size_t)b * g.frameH * g.frameW + (size_t)(g.cy[p] + y0) * g.frameW + g.cx[p];
   auto put = [&](int e, float2 v) {
     const int r = fdiv(e, magN);
@@ -307,22 +308,22 @@ __global__ __launch_bounds__(kThreads) void k_row_fft_ip(Geometry g, FFTPlan pl,
   };
   if (u16) {
     const unsigned short* f = static_cast<const unsigned short*>(frames) + fbase;
-    batched(rows * n, [&](int e) {
+    batched<kRowThreads>(rows * n, [&](int e) {
       const int r = fdiv(e, magN), k = e - r * n;
       const ushort2 v = *reinterpret_cast<const ushort2*>(f + (size_t)r * g.frameW + 2 * k);
       return make_float2((float)v.x, (float)v.y);
     }, put);
   } else {
     const float* f = static_cast<const float*>(frames) + fbase;
-    batched(rows * n, [&](int e) {
+    batched<kRowThreads>(rows * n, [&](int e) {
       const int r = fdiv(e, magN), k = e - r * n;
       return *reinterpret_cast<const float2*>(f + (size_t)r * g.frameW + 2 * k);
     }, put);
   }
   __syncthreads();
-  lds_fft_ip<false, kThreads, true>(A, rows, LD, 1, false, pl, tw, chunk);
+  lds_fft_ip<false, kRowThreads, true>(A, rows, LD, 1, false, pl, tw, chunk);
   float2* out = rowspec + ((size_t)(b * g.P + p) * g.H + y0) * w;
-  for (int e = threadIdx.x; e < rows * w; e += kThreads) {
+  for (int e = threadIdx.x; e < rows * w; e += kRowThreads) {
     const int r = fdiv(e, magW), kk = e - r * w;
     const int k = c0 + kk;                   // k <= n (= W/2, the Nyquist column)
     const float2 zk = A[r * LD + pad32(k == n ? 0 : k)];
@@ -759,8 +760,8 @@ hipError_t launch_ifft_rows(const Geometry& g, const FFTPlan& pl, float2* win, h
 // In-place staged kernels: rows per block of the row transforms, columns per
 // block of the column kernel (0 when a transform has no in-place butterfly or
 // the strip does not fit).
-static int ip_rows_per_block(const FFTPlan& pl, int total) {
-  const int chunk = ip_chunk(pl, kThreads);
+static int ip_rows_per_block(const FFTPlan& pl, int total, int nt = kThreads) {
+  const int chunk = ip_chunk(pl, nt);
   if (chunk < 1) return 0;
   return std::min(total, std::max(chunk, 4096 / pl.n));
 }
@@ -776,20 +777,21 @@ static int ip_cols_per_block(const Geometry& g, const FFTPlan& pc, const FFTPlan
 
 bool staged_ip_supported(const Geometry& g, const FFTPlan& row, const FFTPlan& col, const FFTPlan& ix,
                          const FFTPlan& iy) {
-  return ip_rows_per_block(row, g.H) > 0 && ip_rows_per_block(ix, g.h) > 0 && ix.n == g.w && iy.n == g.h &&
+  return ip_rows_per_block(row, g.H, kRowThreads) > 0 && ip_rows_per_block(ix, g.h) > 0 && ix.n == g.w && iy.n == g.h &&
          ip_cols_per_block(g, col, iy) > 0;
 }
 
 hipError_t launch_row_fft_ip(const Geometry& g, const FFTPlan& pl, const float2* twW, const void* frames,
                              int frame_is_u16, float2* rowspec, hipStream_t s) {
-  const int RB = ip_rows_per_block(pl, g.H);
+  const int RB = ip_rows_per_block(pl, g.H, kRowThreads);
   if (RB < 1) return hipErrorNotSupported;
   const size_t lds = sizeof(float2) * ((size_t)pl.n + g.w + (size_t)pad_stride(pl.n) * RB);
   hipError_t e = set_lds(k_row_fft_ip, lds);
   if (e != hipSuccess) return e;
   dim3 grid((g.H + RB - 1) / RB, g.P, g.B);
-  k_row_fft_ip<<<grid, kThreads, lds, s>>>(g, pl, twW, frames, frame_is_u16, rowspec, RB, ip_chunk(pl, kThreads),
-                                           magic_div((unsigned)pl.n), magic_div((unsigned)g.w));
+  k_row_fft_ip<<<grid, kRowThreads, lds, s>>>(g, pl, twW, frames, frame_is_u16, rowspec, RB,
+                                              ip_chunk(pl, kRowThreads), magic_div((unsigned)pl.n),
+                                              magic_div((unsigned)g.w));
   return hipGetLastError();
 }
 
