@@ -221,6 +221,44 @@ def uneven_batching(lib, reps=20):
             "graph_speedup": round(seq_ms / graph_ms, 3), "window_paths": paths}
 
 
+LARGE_W, LARGE_WIN, LARGE_G, LARGE_BX = 1024, 512, 80, (-2.56e-3, 2.56e-3)
+
+
+def large_setup(lib, batch):
+    """Handle and resident float32 frames of the 1024x1024, 80-group workload."""
+    import torch
+    from digholo_amd import api, synth
+    spec = synth.HologramSpec(frame_width=LARGE_W, frame_height=LARGE_W, pol_count=2, waist=300e-6,
+                              group_count=LARGE_G, beam_centre_x=list(LARGE_BX), group_decay=20.0)
+    fr16 = make_frames(n_unique=8, seed=5, spec=spec, batch=batch)
+    frames = fr16.to(torch.float32).contiguous()
+    del fr16
+    h = lib.digHoloCreate()
+    api.configure(lib, h, frame_width=LARGE_W, frame_height=LARGE_W, pol_count=2, fft_window=(LARGE_WIN, LARGE_WIN),
+                  waist=300e-6, group_count=LARGE_G, beam_centre_x=LARGE_BX)
+    assert lib.digHoloSetBatch(h, batch, C.c_void_p(frames.data_ptr())) == 0
+    return h, frames
+
+
+def traffic_probe_large(lib, batch=256):
+    import torch
+    from digholo_amd import api
+    h, frames = large_setup(lib, batch)
+    b, m, p = C.c_int(), C.c_int(), C.c_int()
+    for prec in (api.DIGHOLO_PRECISION_FP32, api.DIGHOLO_PRECISION_FP32,
+                 api.DIGHOLO_PRECISION_SPLIT16, api.DIGHOLO_PRECISION_SPLIT16):
+        assert lib.digHoloConfigSetOverlapPrecision(h, prec) == 0
+        assert lib.digHoloProcessBatchDevice(h, C.byref(b), C.byref(m), C.byref(p))
+        assert lib.digHoloSynchronize(h) == 0
+    x = torch.empty(128 * 1024 * 1024, dtype=torch.int32, device="cuda").fill_(1)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(f"traffic probe (large) done, window path {lib.digHoloGetWindowPath(h)}", flush=True)
+    lib.digHoloDestroy(h)
+
+
 def large_sensor(lib, batch=256, steps=10):
     """High-order decomposition: 1024x1024 float32 frames, dual pol, batch 256,
     FFT window 512x512 (IFFT 160x160), 80 HG groups (3240 modes per pol), group
@@ -229,19 +267,10 @@ def large_sensor(lib, batch=256, steps=10):
     coefficient error of each against the FP64 oracle on two frames."""
     import numpy as np
     import torch
-    from digholo_amd import api, synth
+    from digholo_amd import api
     from oracle import digholo_oracle as O
-    W, Hh, win, G = 1024, 1024, 512, 80
-    bx = (-2.56e-3, 2.56e-3)
-    spec = synth.HologramSpec(frame_width=W, frame_height=Hh, pol_count=2, waist=300e-6, group_count=G,
-                              beam_centre_x=list(bx), group_decay=20.0)
-    fr16 = make_frames(n_unique=8, seed=5, spec=spec, batch=batch)
-    frames = fr16.to(torch.float32).contiguous()
-    del fr16
-    h = lib.digHoloCreate()
-    api.configure(lib, h, frame_width=W, frame_height=Hh, pol_count=2, fft_window=(win, win), waist=300e-6,
-                  group_count=G, beam_centre_x=bx)
-    assert lib.digHoloSetBatch(h, batch, C.c_void_p(frames.data_ptr())) == 0
+    W, Hh, win, G, bx = LARGE_W, LARGE_W, LARGE_WIN, LARGE_G, LARGE_BX
+    h, frames = large_setup(lib, batch)
     b, m, p = C.c_int(), C.c_int(), C.c_int()
     res = {}
     sample = frames[:2].cpu().numpy().astype(np.float64)
@@ -304,6 +333,9 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="for rocprofv3 --pmc: two batches with the fused split overlap, two with the FP32 "
                          "overlap, then a 512 MiB device copy that scales the counters; prints no JSON")
+    ap.add_argument("--traffic-probe-large", action="store_true",
+                    help="the same for the 1024x1024, 80-group workload (staged kernels): two FP32 batches, "
+                         "two split batches, then the 512 MiB copy; prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -316,6 +348,9 @@ def main():
 
     lib = api.load()
     torch.cuda.set_device(0)
+    if args.traffic_probe_large:
+        traffic_probe_large(lib)
+        return
     frames = make_frames()
     h = lib.digHoloCreate()
     api.configure(lib, h, frame_width=FRAME_W, frame_height=FRAME_H, pol_count=POLS, fft_window=(WIN, WIN),
