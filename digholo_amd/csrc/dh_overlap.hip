@@ -3,8 +3,7 @@
 //   stage 1:  T[(y,c)][m] = sum_x V_c[y][x] HGx[m][x]        (c = re/im)
 //   stage 2:  C[(m,c)][n] += sum_y T[(y,c)][m] HGy[n][y]      -> coef(m,n), m+n < G
 //
-// One workgroup (4 waves; 8 when its LDS leaves room for only one workgroup
-// per CU) per pol and frame slot walks the frames b =
+// One workgroup (8 waves) per pol and frame slot walks the frames b =
 // slot, slot + slots, ... (the basis stays in LDS for all of them) and each
 // field in chunks of 16 field rows; the next chunk's field values are loaded
 // into registers while the current chunk is multiplied.  Per chunk, stage 1 is a (32 x Gp x w) product whose
@@ -305,8 +304,9 @@ static size_t overlap_lds(const Geometry& g, bool split) {
                    2 * (size_t)Gp * pitch_of(kChunk));
 }
 
-// waves per workgroup: 8 when the LDS holds a single workgroup per CU
-static int ov_waves(const Geometry& g, bool split) { return overlap_lds(g, split) > 80 * 1024 ? 8 : 4; }
+// waves per workgroup (8 measured faster than 4 on both the 80-point and the
+// 160-point fields: more waves per SIMD to cover the LDS and barrier latency)
+constexpr int kOvWaves = 8;
 
 static int tiles_per_wave(const Geometry& g, int nw) {
   const int Gp = (g.G + 15) & ~15;
@@ -350,11 +350,13 @@ static hipError_t ov_dispatch_nw(const Geometry& g, const float2* fields, const 
 template <bool SPLIT>
 static hipError_t ov_dispatch(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
                               float2* coefs, int tpwWanted, hipStream_t s) {
-  return ov_waves(g, SPLIT) == 8 ? ov_dispatch_nw<SPLIT, 8>(g, fields, hgx, hgy, coefs, tpwWanted, s)
-                                 : ov_dispatch_nw<SPLIT, 4>(g, fields, hgx, hgy, coefs, tpwWanted, s);
+  return ov_dispatch_nw<SPLIT, kOvWaves>(g, fields, hgx, hgy, coefs, tpwWanted, s);
 }
 
-int overlap_min_tiles_per_wave(const Geometry& g, int split) { return tiles_per_wave(g, ov_waves(g, split != 0)); }
+int overlap_min_tiles_per_wave(const Geometry& g, int split) {
+  (void)split;
+  return tiles_per_wave(g, kOvWaves);
+}
 
 hipError_t launch_overlap_mfma(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
                                float2* coefs, int split, hipStream_t s, int tpw) {

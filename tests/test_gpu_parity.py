@@ -238,13 +238,17 @@ def test_large_window_high_order(lib):
     check(frames, cfg, coefs, fields)
 
 
-@pytest.mark.parametrize("name,B", [("radix7", 3), ("small", 3)])
-def test_staged_kernels_any_radix(lib, name, B):
+@pytest.mark.parametrize("name,B,quantise,over", [
+    ("radix7", 3, True, {}), ("small", 3, True, {}), ("small", 3, False, {}),
+    ("dual640", 2, False, {"fft_window": (256, 192), "beam_centre_y": (30e-6, -50e-6)}),
+    ("dual640w192", 2, True, {"fft_window": (192, 256)})])
+def test_staged_kernels_any_radix(lib, name, B, quantise, over):
     """The staged kernels: a 56-point row transform (radix 7) has no in-place
-    butterfly and takes the ping-pong kernels, the 128-point one the in-place
-    kernels; both against the oracle."""
-    frames, cfg = make_case(name, B, seed=4)
-    print(f"case {name} B={B}", flush=True)
+    butterfly and takes the ping-pong kernels, the others the in-place kernels
+    (uint16 and float32 frames, rectangular FFT windows, ragged column strips);
+    all against the oracle."""
+    frames, cfg = make_case(name, B, seed=4, quantise=quantise, **over)
+    print(f"case {name} B={B} quantise={quantise} {over}", flush=True)
     h = lib.digHoloCreate()
     api.configure(lib, h, **cfg)
     assert lib.digHoloConfigSetFusedWindow(h, 0) == 0
