@@ -26,10 +26,10 @@ __host__ __device__ constexpr int pad_stride(int n) { return (pad32(n - 1) + 1) 
 // (t, k) lives at buf[t*ts + k*es] (PAD: at buf[t*ts + pad32(k)], es unused).
 // tmajor: consecutive threads take consecutive transforms (use when ts == 1),
 // else consecutive butterflies.
-template <int R, bool INV, int NT, bool PAD = false>
+template <int R, bool INV, int NT, bool PAD = false, int VMAX = kMaxV>
 __device__ __forceinline__ void stage_ip(float2* buf, int n, int nb, int ts, int es, int Ns, unsigned magNbf,
                                          unsigned magNs, unsigned magNb, bool tmajor, const float2* tw) {
-  constexpr int BMAX = kMaxV / R > 0 ? kMaxV / R : 1;
+  constexpr int BMAX = VMAX / R > 0 ? VMAX / R : 1;
   const int nbf = n / R;
   const int total = nb * nbf;
   const int step = n / (Ns * R);
@@ -83,7 +83,7 @@ __device__ __forceinline__ void stage_ip(float2* buf, int n, int nb, int ts, int
 }
 
 // nb transforms, at most `chunk` per in-place pass.
-template <bool INV, int NT, bool PAD = false>
+template <bool INV, int NT, bool PAD = false, int VMAX = kMaxV>
 __device__ void lds_fft_ip(float2* buf, int nb, int ts, int es, bool tmajor, const FFTPlan& pl, const float2* tw,
                            int chunk) {
   for (int t0 = 0; t0 < nb; t0 += chunk) {
@@ -95,19 +95,19 @@ __device__ void lds_fft_ip(float2* buf, int nb, int ts, int es, bool tmajor, con
       const int R = pl.radix[s];
       switch (R) {
         case 8:
-          stage_ip<8, INV, NT, PAD>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw);
+          stage_ip<8, INV, NT, PAD, VMAX>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw);
           break;
         case 4:
-          stage_ip<4, INV, NT, PAD>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw);
+          stage_ip<4, INV, NT, PAD, VMAX>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw);
           break;
         case 2:
-          stage_ip<2, INV, NT, PAD>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw);
+          stage_ip<2, INV, NT, PAD, VMAX>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw);
           break;
         case 3:
-          stage_ip<3, INV, NT, PAD>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw);
+          stage_ip<3, INV, NT, PAD, VMAX>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw);
           break;
         default:
-          stage_ip<5, INV, NT, PAD>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw);
+          stage_ip<5, INV, NT, PAD, VMAX>(b, pl.n, cnt, ts, es, Ns, pl.magNbf[s], pl.magNs[s], magNb, tmajor, tw);
           break;
       }
       Ns *= R;
@@ -115,14 +115,93 @@ __device__ void lds_fft_ip(float2* buf, int nb, int ts, int es, bool tmajor, con
   }
 }
 
+// ---------------------------------------------------------------------------
+// The same transform for a compile-time power-of-two length N (radix 8
+// stages, then one of radix 4 or 2), padded, transforms at stride ts: every
+// division is a shift, and the LDS offsets of a butterfly's points are
+// constants added to one computed base wherever the stride is a multiple of 32.
+// ---------------------------------------------------------------------------
+template <int N, int R, int NS, bool INV, int NT, int VMAX>
+__device__ __forceinline__ void stage_pow2(float2* buf, int nb, int ts, const float2* tw) {
+  constexpr int NBF = N / R;
+  constexpr int STEP = N / (NS * R);
+  constexpr int BMAX = VMAX / R > 0 ? VMAX / R : 1;
+  const int total = nb * NBF;
+  float2 v[BMAX][R];
+  int wb[BMAX], wk[BMAX];
+#pragma unroll
+  for (int bb = 0; bb < BMAX; ++bb) {
+    const int q = threadIdx.x + bb * NT;
+    wb[bb] = -1;
+    wk[bb] = 0;
+    if (q < total) {
+      const int t = q / NBF, j = q % NBF, jm = j % NS;
+      const int tb = t * ts;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if constexpr (NBF % 32 == 0)
+          v[bb][r] = buf[tb + pad32(j) + r * (NBF + NBF / 32)];
+        else
+          v[bb][r] = buf[tb + pad32(j + r * NBF)];
+      }
+      if (NS > 1 && jm) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[bb][r] = cmul(v[bb][r], twiddle<INV>(tw, jm * r * STEP));
+      }
+      dftR<R, INV>(v[bb]);
+      wb[bb] = tb;
+      wk[bb] = (j - jm) * R + jm;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int bb = 0; bb < BMAX; ++bb) {
+    if (wb[bb] >= 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if constexpr (NS % 32 == 0)
+          buf[wb[bb] + pad32(wk[bb]) + r * (NS + NS / 32)] = v[bb][r];
+        else
+          buf[wb[bb] + pad32(wk[bb] + r * NS)] = v[bb][r];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+template <int N, int NS, bool INV, int NT, int VMAX>
+__device__ __forceinline__ void stages_pow2(float2* buf, int nb, int ts, const float2* tw) {
+  if constexpr (NS < N) {
+    constexpr int R = N / NS >= 8 ? 8 : N / NS;
+    stage_pow2<N, R, NS, INV, NT, VMAX>(buf, nb, ts, tw);
+    stages_pow2<N, NS * R, INV, NT, VMAX>(buf, nb, ts, tw);
+  }
+}
+
+// Most transforms of length N one pass of NT threads takes.
+template <int N, int NT, int VMAX>
+__host__ __device__ constexpr int pow2_chunk() {
+  // radix 8 stages bound it (radix 4 / 2 hold 2x / 4x the butterflies per thread)
+  return (VMAX / 8 > 0 ? VMAX / 8 : 1) * NT * 8 / N;
+}
+
+// nb transforms of length N at buf + t * ts (padded points), in passes of
+// pow2_chunk transforms.  Called by the whole block; ends with a barrier.
+template <int N, bool INV, int NT, int VMAX = kMaxV>
+__device__ void lds_fft_pow2(float2* buf, int nb, int ts, const float2* tw) {
+  constexpr int CH = pow2_chunk<N, NT, VMAX>();
+  static_assert(CH >= 1, "transform too long for one pass of the block");
+  for (int t0 = 0; t0 < nb; t0 += CH) stages_pow2<N, 1, INV, NT, VMAX>(buf + t0 * ts, min(CH, nb - t0), ts, tw);
+}
+
 // Transforms one in-place pass can take with nt threads (each thread holds at
-// most kMaxV values per stage); 0 when a radix has no in-place butterfly.
-inline int ip_chunk(const FFTPlan& pl, int nt) {
+// most vmax values per stage); 0 when a radix has no in-place butterfly.
+inline int ip_chunk(const FFTPlan& pl, int nt, int vmax = kMaxV) {
   int best = 1 << 30;
   for (int s = 0; s < pl.nstages; ++s) {
     const int R = pl.radix[s];
     if (R != 2 && R != 3 && R != 4 && R != 5 && R != 8) return 0;
-    const int bmax = kMaxV / R > 0 ? kMaxV / R : 1;
+    const int bmax = vmax / R > 0 ? vmax / R : 1;
     best = std::min(best, bmax * nt * R / pl.n);
   }
   return best;

@@ -265,6 +265,7 @@ __global__ __launch_bounds__(kThreads) void k_ifft_cols_tilt(Geometry g, FFTPlan
 constexpr int kRowThreads = 512;   // row and column kernels: one stage pass covers twice the
 constexpr int kColThreads = 512;   // transforms of a 256-thread block
 constexpr int kSel = 8;            // window values per thread held across the row selection
+constexpr int kColV = 16;          // column kernel: values per thread in one transform stage
 constexpr int kLd = 8;             // global loads per thread in flight in the copy loops
 
 // for e in [0, cnt): put(e, get(e)), kLd loads issued before the first is used
@@ -285,13 +286,15 @@ __device__ __forceinline__ void batched(int cnt, Get&& get, Put&& put) {
   }
 }
 
-// grid (ceil(H/RB), P, B); rows as in k_row_fft
+// grid (ceil(H/RB), P, B); rows as in k_row_fft.  NC: the transform length
+// when it is a power of two with a compile-time transform, else 0.
+template <int NC>
 __global__ __launch_bounds__(kRowThreads) void k_row_fft_ip(Geometry g, FFTPlan pl, const float2* __restrict__ twW,
                                                          const void* __restrict__ frames, int u16,
                                                          float2* __restrict__ rowspec, int RB, int chunk,
                                                          unsigned magN, unsigned magW) {
   extern __shared__ float2 smem[];
-  const int n = pl.n, w = g.w;
+  const int n = NC ? NC : pl.n, w = g.w;
   float2* tw = smem;          // [n] transform twiddles
   float2* tu = smem + n;      // [w] unpack twiddles exp(-2 pi i k / W) of the window columns
   float2* A = tu + w;         // [RB][LD], point k of row r at r LD + pad32(k)
@@ -302,26 +305,30 @@ __global__ __launch_bounds__(kRowThreads) void k_row_fft_ip(Geometry g, FFTPlan 
   for (int e = threadIdx.x; e < n; e += kRowThreads) tw[e] = pl.tw[e];
   for (int e = threadIdx.x; e < w; e += kRowThreads) tu[e] = twW[c0 + e];
   const size_t fbase = (size_t)b * g.frameH * g.frameW + (size_t)(g.cy[p] + y0) * g.frameW + g.cx[p];
+  auto rowof = [&](int e) { return NC ? (int)((unsigned)e / (unsigned)NC) : fdiv(e, magN); };
   auto put = [&](int e, float2 v) {
-    const int r = fdiv(e, magN);
+    const int r = rowof(e);
     A[r * LD + pad32(e - r * n)] = v;
   };
   if (u16) {
     const unsigned short* f = static_cast<const unsigned short*>(frames) + fbase;
     batched<kRowThreads>(rows * n, [&](int e) {
-      const int r = fdiv(e, magN), k = e - r * n;
+      const int r = rowof(e), k = e - r * n;
       const ushort2 v = *reinterpret_cast<const ushort2*>(f + (size_t)r * g.frameW + 2 * k);
       return make_float2((float)v.x, (float)v.y);
     }, put);
   } else {
     const float* f = static_cast<const float*>(frames) + fbase;
     batched<kRowThreads>(rows * n, [&](int e) {
-      const int r = fdiv(e, magN), k = e - r * n;
+      const int r = rowof(e), k = e - r * n;
       return *reinterpret_cast<const float2*>(f + (size_t)r * g.frameW + 2 * k);
     }, put);
   }
   __syncthreads();
-  lds_fft_ip<false, kRowThreads, true>(A, rows, LD, 1, false, pl, tw, chunk);
+  if constexpr (NC > 0)
+    lds_fft_pow2<NC, false, kRowThreads>(A, rows, LD, tw);
+  else
+    lds_fft_ip<false, kRowThreads, true>(A, rows, LD, 1, false, pl, tw, chunk);
   float2* out = rowspec + ((size_t)(b * g.P + p) * g.H + y0) * w;
   for (int e = threadIdx.x; e < rows * w; e += kRowThreads) {
     const int r = fdiv(e, magW), kk = e - r * w;
@@ -337,15 +344,17 @@ __global__ __launch_bounds__(kRowThreads) void k_row_fft_ip(Geometry g, FFTPlan 
 
 // grid (ceil(w/CB), P, B).  Loads and stores move the strip's CB * 8
 // contiguous bytes per row; in LDS the strip is column-major, each column
-// padded.  h * CB <= kThreads * kSel (the selected window is held in registers while
-// it moves to the top of the strip).
+// padded.  h * CB <= kColThreads * kSel (the selected window is held in
+// registers while it moves to the top of the strip).  NC: H when the column
+// transform has a compile-time form (a power of two), else 0.
+template <int NC>
 __global__ __launch_bounds__(kColThreads) void k_col_window_iy(Geometry g, FFTPlan pc, FFTPlan py,
                                                             const float2* __restrict__ rowspec,
                                                             const float2* __restrict__ factor,
                                                             float2* __restrict__ win, int CB, int chunkC,
                                                             int chunkY, BatchCal cal) {
   extern __shared__ float2 smem[];
-  const int H = g.H, h = g.h, w = g.w;
+  const int H = NC ? NC : g.H, h = g.h, w = g.w;
   float2* twc = smem;
   float2* twy = smem + H;
   float2* A = twy + h;        // [CB][LD], point y of column c at c LD + pad32(y)
@@ -378,7 +387,10 @@ __global__ __launch_bounds__(kColThreads) void k_col_window_iy(Geometry g, FFTPl
     }
   }
   __syncthreads();
-  lds_fft_ip<false, kColThreads, true>(A, cols, LD, 1, false, pc, twc, chunkC);
+  if constexpr (NC > 0)
+    lds_fft_pow2<NC, false, kColThreads, kColV>(A, cols, LD, twc);
+  else
+    lds_fft_ip<false, kColThreads, true, kColV>(A, cols, LD, 1, false, pc, twc, chunkC);
   const float2 cb = cal.at(p, b);
 #pragma unroll
   for (int s = 0; s < kSel; ++s) {
@@ -400,7 +412,7 @@ __global__ __launch_bounds__(kColThreads) void k_col_window_iy(Geometry g, FFTPl
     }
   }
   __syncthreads();
-  lds_fft_ip<true, kColThreads, true>(A, cols, LD, 1, false, py, twy, chunkY);
+  lds_fft_ip<true, kColThreads, true, kColV>(A, cols, LD, 1, false, py, twy, chunkY);
   float2* out = win + (size_t)(out_pos(g, b) * g.P + p) * h * w + i0;
   for (int e = threadIdx.x; e < h * cols; e += kColThreads) {
     const int j = fdiv(e, magC), c = e - j * cols;
@@ -767,7 +779,7 @@ static int ip_rows_per_block(const FFTPlan& pl, int total, int nt = kThreads) {
 }
 
 static int ip_cols_per_block(const Geometry& g, const FFTPlan& pc, const FFTPlan& py) {
-  if (ip_chunk(pc, kColThreads) < 1 || ip_chunk(py, kColThreads) < 1) return 0;
+  if (ip_chunk(pc, kColThreads, kColV) < 1 || ip_chunk(py, kColThreads, kColV) < 1) return 0;
   for (int cb = std::min(16, g.w); cb >= 1; --cb) {
     const size_t lds = sizeof(float2) * ((size_t)g.H + g.h + (size_t)pad_stride(g.H) * cb);
     if (g.h * cb <= kColThreads * kSel && lds <= 80 * 1024) return cb;
@@ -786,10 +798,19 @@ hipError_t launch_row_fft_ip(const Geometry& g, const FFTPlan& pl, const float2*
   const int RB = ip_rows_per_block(pl, g.H, kRowThreads);
   if (RB < 1) return hipErrorNotSupported;
   const size_t lds = sizeof(float2) * ((size_t)pl.n + g.w + (size_t)pad_stride(pl.n) * RB);
-  hipError_t e = set_lds(k_row_fft_ip, lds);
+  auto kern = k_row_fft_ip<0>;
+  switch (pl.n) {
+    case 64: kern = k_row_fft_ip<64>; break;
+    case 128: kern = k_row_fft_ip<128>; break;
+    case 256: kern = k_row_fft_ip<256>; break;
+    case 512: kern = k_row_fft_ip<512>; break;
+    case 1024: kern = k_row_fft_ip<1024>; break;
+    default: break;
+  }
+  hipError_t e = set_lds(kern, lds);
   if (e != hipSuccess) return e;
   dim3 grid((g.H + RB - 1) / RB, g.P, g.B);
-  k_row_fft_ip<<<grid, kRowThreads, lds, s>>>(g, pl, twW, frames, frame_is_u16, rowspec, RB,
+  kern<<<grid, kRowThreads, lds, s>>>(g, pl, twW, frames, frame_is_u16, rowspec, RB,
                                               ip_chunk(pl, kRowThreads), magic_div((unsigned)pl.n),
                                               magic_div((unsigned)g.w));
   return hipGetLastError();
@@ -800,11 +821,21 @@ hipError_t launch_col_window_iy(const Geometry& g, const FFTPlan& pc, const FFTP
   const int CB = ip_cols_per_block(g, pc, py);
   if (CB < 1 || py.n != g.h) return hipErrorNotSupported;
   const size_t lds = sizeof(float2) * ((size_t)g.H + g.h + (size_t)pad_stride(g.H) * CB);
-  hipError_t e = set_lds(k_col_window_iy, lds);
+  auto kern = k_col_window_iy<0>;
+  switch (pc.n) {
+    case 64: kern = k_col_window_iy<64>; break;
+    case 128: kern = k_col_window_iy<128>; break;
+    case 256: kern = k_col_window_iy<256>; break;
+    case 512: kern = k_col_window_iy<512>; break;
+    case 1024: kern = k_col_window_iy<1024>; break;
+    default: break;
+  }
+  hipError_t e = set_lds(kern, lds);
   if (e != hipSuccess) return e;
   dim3 grid((g.w + CB - 1) / CB, g.P, g.B);
-  k_col_window_iy<<<grid, kColThreads, lds, s>>>(g, pc, py, rowspec, factor, win, CB, ip_chunk(pc, kColThreads),
-                                                 ip_chunk(py, kColThreads), cal);
+  kern<<<grid, kColThreads, lds, s>>>(g, pc, py, rowspec, factor, win, CB,
+                                                 ip_chunk(pc, kColThreads, kColV), ip_chunk(py, kColThreads, kColV),
+                                                 cal);
   return hipGetLastError();
 }
 
