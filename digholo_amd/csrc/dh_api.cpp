@@ -175,6 +175,7 @@ struct Handle {
   bool holdCrop = false, holdWin = false;   // AutoAlign: keep FFT crops / Fourier window
   Derived heldCrop, heldWin;
   DevBuf<float2> rowspec, win, fields, coefs, factor, ref, twW;
+  DevBuf<unsigned> fieldPeak;   // [B][P] largest |Re|, |Im| of each field (staged path), float bits
   DevBuf<float> hgx, hgy;
   DevBuf<uint8_t> mask;
   std::map<int, std::unique_ptr<FFTEntry>> ffts;
@@ -1257,6 +1258,7 @@ int enqueue_batch(Handle& h, int step = 0) {
   }
   h.stepState = 0;
   dh::FusedArgs fa;
+  const unsigned* peak = nullptr;   // field peaks left by the staged kernels for the split overlap
   int path = 0;
   if (A > 1) path = 0;   // averaging runs on the staged kernels
   else if (h.windowMode >= 3 && h.frameU16 && dh::window_v3_table_elems(g) && plan_window_v3(h, g, fa)) path = 3;
@@ -1295,10 +1297,16 @@ int enqueue_batch(Handle& h, int step = 0) {
     DH_TRY(h.win.ensure(B * P * g.h * g.w), "alloc");
     DH_TRY(dh::launch_row_fft_ip(gA, *pRow, h.twW.p, dframes, h.frameU16, h.rowspec.p, h.stream), "row FFT");
     if (timing) DH_TRY(hipEventRecord(ev[2], h.stream), "event");
-    DH_TRY(dh::launch_col_window_iy(g, *pCol, *pIy, h.rowspec.p, h.factor.p, h.win.p, cal, h.stream),
+    // the split overlap's per-field scale comes from the field kernel
+    if (g.G > 0 && c.precision == DIGHOLO_PRECISION_SPLIT16) {
+      DH_TRY(h.fieldPeak.ensure(B * P), "alloc");
+      peak = h.fieldPeak.p;
+    }
+    unsigned* peakW = peak ? h.fieldPeak.p : nullptr;
+    DH_TRY(dh::launch_col_window_iy(g, *pCol, *pIy, h.rowspec.p, h.factor.p, h.win.p, cal, h.stream, peakW),
            "column FFT");
     if (timing) DH_TRY(hipEventRecord(ev[3], h.stream), "event");
-    DH_TRY(dh::launch_ifft_rows_tilt(gF, *pIx, h.win.p, h.ref.p, h.fields.p, h.stream), "IFFT rows");
+    DH_TRY(dh::launch_ifft_rows_tilt(gF, *pIx, h.win.p, h.ref.p, h.fields.p, h.stream, peakW), "IFFT rows");
     if (timing) DH_TRY(hipEventRecord(ev[4], h.stream), "event");
     h.lastPath = 0;
     h.lastFusedOverlap = false;
@@ -1332,7 +1340,8 @@ int enqueue_batch(Handle& h, int step = 0) {
   }
   if (g.G > 0 && !h.lastFusedOverlap) {
     DH_TRY(dh::launch_overlap_mfma(gF, h.fields.p, h.hgx.p, h.hgy.p, h.coefs.p,
-                                   c.precision == DIGHOLO_PRECISION_SPLIT16, h.stream, h.ovTpw), "overlap");
+                                   c.precision == DIGHOLO_PRECISION_SPLIT16, h.stream, h.ovTpw, peak),
+           "overlap");
   }
   if (timing) DH_TRY(hipEventRecord(ev[5], h.stream), "event");
 

@@ -109,10 +109,14 @@ bool staged_ip_supported(const Geometry& g, const FFTPlan& row, const FFTPlan& c
                          const FFTPlan& iy);
 hipError_t launch_row_fft_ip(const Geometry& g, const FFTPlan& pl, const float2* twW, const void* frames,
                              int frame_is_u16, float2* rowspec, hipStream_t s);
+// peak (optional, [B][P] float bits): the column kernel clears it, the row
+// kernel leaves there the largest |Re|, |Im| of each output field, for the
+// split overlap's scale.
 hipError_t launch_col_window_iy(const Geometry& g, const FFTPlan& pc, const FFTPlan& py, const float2* rowspec,
-                                const float2* factor, float2* win, const BatchCal& cal, hipStream_t s);
+                                const float2* factor, float2* win, const BatchCal& cal, hipStream_t s,
+                                unsigned* peak = nullptr);
 hipError_t launch_ifft_rows_tilt(const Geometry& g, const FFTPlan& pl, const float2* win, const float2* ref,
-                                 float2* fields, hipStream_t s);
+                                 float2* fields, hipStream_t s, unsigned* peak = nullptr);
 // ref == nullptr: the inverse transform only (field before tilt removal)
 hipError_t launch_ifft_cols_tilt(const Geometry& g, const FFTPlan& pl, const float2* win,
                                  const float2* ref, float2* fields, hipStream_t s);
@@ -252,8 +256,11 @@ hipError_t launch_grad_reduce(int B, int P, const double* partial, double* out, 
 // Separable HG overlap on MFMA: split = 0 plain FP32, 1 split-FP16 (hi+lo).
 // tpw: stage-2 tiles per wave of the launch (0: the fewest that cover the
 // triangle, overlap_min_tiles_per_wave; larger values give each wave more).
+// peak: per-field largest |Re|, |Im| as float bits ([B][P]) when the producing
+// kernel has left it, else null (the split form then finds it itself).
 hipError_t launch_overlap_mfma(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
-                               float2* coefs, int split, hipStream_t s, int tpw = 0);
+                               float2* coefs, int split, hipStream_t s, int tpw = 0,
+                               const unsigned* peak = nullptr);
 int overlap_min_tiles_per_wave(const Geometry& g, int split);
 
 // Reference-wave calibration plane (dh_refcal.hip).

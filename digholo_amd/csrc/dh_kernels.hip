@@ -352,7 +352,7 @@ __global__ __launch_bounds__(kColThreads) void k_col_window_iy(Geometry g, FFTPl
                                                             const float2* __restrict__ rowspec,
                                                             const float2* __restrict__ factor,
                                                             float2* __restrict__ win, int CB, int chunkC,
-                                                            int chunkY, BatchCal cal) {
+                                                            int chunkY, BatchCal cal, unsigned* __restrict__ peak) {
   extern __shared__ float2 smem[];
   const int H = NC ? NC : g.H, h = g.h, w = g.w;
   float2* twc = smem;
@@ -362,6 +362,7 @@ __global__ __launch_bounds__(kColThreads) void k_col_window_iy(Geometry g, FFTPl
   const int b = blockIdx.z, p = blockIdx.y, i0 = blockIdx.x * CB;
   const int cols = min(CB, w - i0);
   const unsigned magC = magic_div((unsigned)cols);
+  if (peak && blockIdx.x == 0 && threadIdx.x == 0) peak[out_pos(g, b) * g.P + p] = 0u;
   for (int e = threadIdx.x; e < H; e += kColThreads) twc[e] = pc.tw[e];
   for (int e = threadIdx.x; e < h; e += kColThreads) twy[e] = py.tw[e];
   const float2* in = rowspec + (size_t)(b * g.P + p) * H * w + i0;
@@ -424,7 +425,8 @@ __global__ __launch_bounds__(kColThreads) void k_col_window_iy(Geometry g, FFTPl
 // ref null: the field before the reference wave is removed.
 __global__ __launch_bounds__(kThreads) void k_ifft_rows_tilt(Geometry g, FFTPlan pl, const float2* __restrict__ win,
                                                              const float2* __restrict__ ref,
-                                                             float2* __restrict__ fields, int RB, int chunk) {
+                                                             float2* __restrict__ fields, int RB, int chunk,
+                                                             unsigned* __restrict__ peak) {
   extern __shared__ float2 smem[];
   const int n = pl.n;   // = w
   float2* tw = smem;
@@ -447,10 +449,16 @@ __global__ __launch_bounds__(kThreads) void k_ifft_rows_tilt(Geometry g, FFTPlan
     return;
   }
   const float2* rp = ref + ((size_t)(lam_out(g, b) * g.P + p) * g.h + y0) * n;
+  float mx = 0.f;
   batched(rows * n, [&](int e) { return rp[e]; }, [&](int e, float2 r) {
     const float2 t = cmul(at(e), r);
     fields[base + e] = make_float2(-t.x, t.y);
+    mx = fmaxf(mx, fmaxf(fabsf(t.x), fabsf(t.y)));
   });
+  if (peak) {   // non-negative floats order as their bit patterns
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(peak + b * g.P + p, __float_as_uint(mx));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -817,7 +825,8 @@ hipError_t launch_row_fft_ip(const Geometry& g, const FFTPlan& pl, const float2*
 }
 
 hipError_t launch_col_window_iy(const Geometry& g, const FFTPlan& pc, const FFTPlan& py, const float2* rowspec,
-                                const float2* factor, float2* win, const BatchCal& cal, hipStream_t s) {
+                                const float2* factor, float2* win, const BatchCal& cal, hipStream_t s,
+                                unsigned* peak) {
   const int CB = ip_cols_per_block(g, pc, py);
   if (CB < 1 || py.n != g.h) return hipErrorNotSupported;
   const size_t lds = sizeof(float2) * ((size_t)g.H + g.h + (size_t)pad_stride(g.H) * CB);
@@ -835,19 +844,20 @@ hipError_t launch_col_window_iy(const Geometry& g, const FFTPlan& pc, const FFTP
   dim3 grid((g.w + CB - 1) / CB, g.P, g.B);
   kern<<<grid, kColThreads, lds, s>>>(g, pc, py, rowspec, factor, win, CB,
                                                  ip_chunk(pc, kColThreads, kColV), ip_chunk(py, kColThreads, kColV),
-                                                 cal);
+                                                 cal, peak);
   return hipGetLastError();
 }
 
 hipError_t launch_ifft_rows_tilt(const Geometry& g, const FFTPlan& pl, const float2* win, const float2* ref,
-                                 float2* fields, hipStream_t s) {
+                                 float2* fields, hipStream_t s, unsigned* peak) {
   const int RB = ip_rows_per_block(pl, g.h);
   if (RB < 1 || pl.n != g.w) return hipErrorNotSupported;
   const size_t lds = sizeof(float2) * ((size_t)pl.n + (size_t)pad_stride(pl.n) * RB);
   hipError_t e = set_lds(k_ifft_rows_tilt, lds);
   if (e != hipSuccess) return e;
   dim3 grid((g.h + RB - 1) / RB, g.P, g.B);
-  k_ifft_rows_tilt<<<grid, kThreads, lds, s>>>(g, pl, win, ref, fields, RB, ip_chunk(pl, kThreads));
+  k_ifft_rows_tilt<<<grid, kThreads, lds, s>>>(g, pl, win, ref, fields, RB, ip_chunk(pl, kThreads),
+                                               ref ? peak : nullptr);
   return hipGetLastError();
 }
 

@@ -53,7 +53,8 @@ template <bool SPLIT, int TPW, int NW>
 __global__ __launch_bounds__(64 * NW) void k_overlap_mfma(Geometry g, const float2* __restrict__ fields,
                                                              const float* __restrict__ hgx,
                                                              const float* __restrict__ hgy,
-                                                             float2* __restrict__ coefs) {
+                                                             float2* __restrict__ coefs,
+                                                             const unsigned* __restrict__ peak) {
   using E = typename OvT<SPLIT>::E;
   extern __shared__ __align__(16) unsigned char ov_smem[];
   const int p = blockIdx.x;
@@ -122,17 +123,21 @@ __global__ __launch_bounds__(64 * NW) void k_overlap_mfma(Geometry g, const floa
   float scale = 1.f;
   if (SPLIT) {
     float mx = 0.f;
-    for (int e = tid; e < h * w; e += NT) {
-      const float2 v = V[e];
-      mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)));
-    }
-    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-    __syncthreads();   // s_red of the previous frame has been read
-    if (lane == 0) s_red[wave] = mx;
-    __syncthreads();
-    mx = s_red[0];
+    if (peak) {   // left by the kernel that wrote the field
+      mx = __uint_as_float(peak[b * g.P + p]);
+    } else {
+      for (int e = tid; e < h * w; e += NT) {
+        const float2 v = V[e];
+        mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)));
+      }
+      for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+      __syncthreads();   // s_red of the previous frame has been read
+      if (lane == 0) s_red[wave] = mx;
+      __syncthreads();
+      mx = s_red[0];
 #pragma unroll
-    for (int i = 1; i < NW; ++i) mx = fmaxf(mx, s_red[i]);
+      for (int i = 1; i < NW; ++i) mx = fmaxf(mx, s_red[i]);
+    }
     if (mx > 0.f) {
       int ex;
       frexpf(mx, &ex);                 // mx in [2^(ex-1), 2^ex)
@@ -321,7 +326,7 @@ static int tiles_per_wave(const Geometry& g, int nw) {
 
 template <bool SPLIT, int TPW, int NW>
 static hipError_t ov_launch(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
-                            float2* coefs, hipStream_t s) {
+                            float2* coefs, hipStream_t s, const unsigned* peak) {
   const size_t lds = overlap_lds(g, SPLIT);
   if (lds > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_overlap_mfma<SPLIT, TPW, NW>),
@@ -331,26 +336,26 @@ static hipError_t ov_launch(const Geometry& g, const float2* fields, const float
   // about two resident workgroups per CU (256 CUs), each walking several frames
   const int slots = std::max(1, std::min(g.B, 512 / std::max(1, g.P)));
   dim3 grid(g.P, slots);
-  k_overlap_mfma<SPLIT, TPW, NW><<<grid, 64 * NW, lds, s>>>(g, fields, hgx, hgy, coefs);
+  k_overlap_mfma<SPLIT, TPW, NW><<<grid, 64 * NW, lds, s>>>(g, fields, hgx, hgy, coefs, peak);
   return hipGetLastError();
 }
 
 template <bool SPLIT, int NW>
 static hipError_t ov_dispatch_nw(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
-                                 float2* coefs, int tpwWanted, hipStream_t s) {
+                                 float2* coefs, int tpwWanted, hipStream_t s, const unsigned* peak) {
   const int tpw = std::max(tiles_per_wave(g, NW), tpwWanted);
-  if (tpw <= 2) return ov_launch<SPLIT, 2, NW>(g, fields, hgx, hgy, coefs, s);
-  if (tpw <= 4) return ov_launch<SPLIT, 4, NW>(g, fields, hgx, hgy, coefs, s);
-  if (tpw <= 8) return ov_launch<SPLIT, 8, NW>(g, fields, hgx, hgy, coefs, s);
-  if (tpw <= 16) return ov_launch<SPLIT, 16, NW>(g, fields, hgx, hgy, coefs, s);
-  if (tpw <= 32) return ov_launch<SPLIT, 32, NW>(g, fields, hgx, hgy, coefs, s);
+  if (tpw <= 2) return ov_launch<SPLIT, 2, NW>(g, fields, hgx, hgy, coefs, s, peak);
+  if (tpw <= 4) return ov_launch<SPLIT, 4, NW>(g, fields, hgx, hgy, coefs, s, peak);
+  if (tpw <= 8) return ov_launch<SPLIT, 8, NW>(g, fields, hgx, hgy, coefs, s, peak);
+  if (tpw <= 16) return ov_launch<SPLIT, 16, NW>(g, fields, hgx, hgy, coefs, s, peak);
+  if (tpw <= 32) return ov_launch<SPLIT, 32, NW>(g, fields, hgx, hgy, coefs, s, peak);
   return hipErrorInvalidValue;
 }
 
 template <bool SPLIT>
 static hipError_t ov_dispatch(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
-                              float2* coefs, int tpwWanted, hipStream_t s) {
-  return ov_dispatch_nw<SPLIT, kOvWaves>(g, fields, hgx, hgy, coefs, tpwWanted, s);
+                              float2* coefs, int tpwWanted, hipStream_t s, const unsigned* peak) {
+  return ov_dispatch_nw<SPLIT, kOvWaves>(g, fields, hgx, hgy, coefs, tpwWanted, s, peak);
 }
 
 int overlap_min_tiles_per_wave(const Geometry& g, int split) {
@@ -359,11 +364,11 @@ int overlap_min_tiles_per_wave(const Geometry& g, int split) {
 }
 
 hipError_t launch_overlap_mfma(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
-                               float2* coefs, int split, hipStream_t s, int tpw) {
+                               float2* coefs, int split, hipStream_t s, int tpw, const unsigned* peak) {
   if (g.w % 16 || g.h % kChunk || g.G <= 0) return hipErrorInvalidValue;
   if (overlap_lds(g, split != 0) > 160 * 1024) return hipErrorInvalidValue;
-  return split ? ov_dispatch<true>(g, fields, hgx, hgy, coefs, tpw, s)
-               : ov_dispatch<false>(g, fields, hgx, hgy, coefs, tpw, s);
+  return split ? ov_dispatch<true>(g, fields, hgx, hgy, coefs, tpw, s, peak)
+               : ov_dispatch<false>(g, fields, hgx, hgy, coefs, tpw, s, nullptr);
 }
 
 }  // namespace dh

@@ -231,11 +231,18 @@ def test_fused_overlap_modes(lib, name, B):
     lib.digHoloDestroy(h)
 
 
-def test_large_window_high_order(lib):
-    frames, cfg = make_case("large1024", 1, seed=12)
-    coefs, fields, _, _, h = run_gpu(lib, frames, cfg)
+@pytest.mark.parametrize("precision", [0, 1])
+def test_large_window_high_order(lib, precision):
+    """1024x1024 sensor, 512x512 windows, 80 groups on the staged kernels; the
+    split overlap takes its per-field scale from the field kernel."""
+    frames, cfg = make_case("large1024", 2, seed=12)
+    h = lib.digHoloCreate()
+    api.configure(lib, h, **cfg)
+    assert lib.digHoloConfigSetOverlapPrecision(h, precision) == 0
+    coefs, fields, _, _, h = run_gpu(lib, frames, cfg, h=h)
     assert lib.digHoloGetWindowPath(h) == 0          # 160x160 field: staged kernels
     check(frames, cfg, coefs, fields)
+    lib.digHoloDestroy(h)
 
 
 @pytest.mark.parametrize("name,B,quantise,over", [
