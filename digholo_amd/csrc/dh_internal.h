@@ -8,6 +8,9 @@ namespace dh {
 
 constexpr int kMaxPol = 2;
 constexpr int kMaxStages = 12;
+// Frames of one batch (averaged frames included): kernels index frames by
+// blockIdx.y or blockIdx.z, whose grid extent is limited to 65535.
+constexpr int kMaxBatchFrames = 65535;
 
 // One 1-D FFT length factored for the Stockham kernels.
 struct FFTPlan {

@@ -324,3 +324,18 @@ def test_full_resolution_steps_and_averaging(lib):
     f_ref, c_ref, _ = O.process_batch(buf, oracle_for(cfg), avg_count=A, avg_mode=0)
     assert O.relative_l2(fields, f_ref) <= TOL and O.relative_l2(coefs, c_ref) <= TOL
     lib.digHoloDestroy(h)
+
+
+def test_batch_frame_limit(lib):
+    """A batch beyond the grid's frame extent (65535) is refused before any
+    frame is read, and the handle still processes a valid batch afterwards."""
+    frames, cfg = make_case("small", 2, seed=3)
+    h = lib.digHoloCreate()
+    api.configure(lib, h, **cfg)
+    buf = np.ascontiguousarray(frames)
+    assert lib.digHoloSetBatchUint16(h, 70000, buf.ctypes.data_as(C.c_void_p), 0) == 0
+    b, m, p = C.c_int(), C.c_int(), C.c_int()
+    assert not lib.digHoloProcessBatch(h, C.byref(b), C.byref(m), C.byref(p))
+    coefs, fields, _, _, h = run_gpu(lib, frames, cfg, h=h)
+    check(frames, cfg, coefs, fields)
+    lib.digHoloDestroy(h)
