@@ -979,6 +979,30 @@ __global__ void k_hg_split(const float* __restrict__ hg, int P, int G, int n, _F
   dst[((size_t)pp * 2 + 1) * Gp * n + rem] = (_Float16)(v - (float)hi);
 }
 
+// (pol, frame) of this workgroup in a one-dimensional grid of P B.  With two
+// pols, workgroups 16 q + r and 16 q + r + 8 are the two pols of frame
+// 8 q + r: the dispatcher deals workgroups to the eight XCDs in turn, so both
+// land on the same XCD and the camera rows their windows share are read from
+// HBM once and then from that XCD's L2.  The frames past the last full group
+// of eight are mapped plainly.
+__device__ __forceinline__ void w3_item(int P, int B, int& p, int& b) {
+  const int id = blockIdx.x;
+  if (P != 2) {
+    p = id % P;
+    b = id / P;
+    return;
+  }
+  const int full = (B >> 3) << 4;
+  if (id < full) {
+    const int q = id >> 4, r = id & 15;
+    p = r >> 3;
+    b = 8 * q + (r & 7);
+  } else {
+    p = (id - full) & 1;
+    b = ((B >> 3) << 3) + ((id - full) >> 1);
+  }
+}
+
 template <int L, int M, int U, int J>
 __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const unsigned short* __restrict__ frames,
                                                           const float2* __restrict__ tabs,
@@ -991,7 +1015,8 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
   extern __shared__ __align__(16) unsigned char w3_smem[];
   const Geometry& g = a.g;
   const int H = g.H;
-  const int p = blockIdx.x, b = blockIdx.y;
+  int p, b;
+  w3_item(a.g.P, a.g.B, p, b);
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wv = (wave + b) & 3;              // units rotate with the frame so every SIMD gets its share
@@ -1477,7 +1502,7 @@ static hipError_t w3_launch(W3Args a, const void* frames, const float2* tab, con
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  dim3 grid(g.P, g.B);
+  dim3 grid(g.P * g.B);
   k_window_v3<L, M, U, J><<<grid, kW3Threads, lds, s>>>(a, static_cast<const unsigned short*>(frames), tab, factor,
                                                         ref, fields);
   return hipGetLastError();
