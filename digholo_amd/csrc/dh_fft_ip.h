@@ -116,10 +116,11 @@ __device__ void lds_fft_ip(float2* buf, int nb, int ts, int es, bool tmajor, con
 }
 
 // ---------------------------------------------------------------------------
-// The same transform for a compile-time power-of-two length N (radix 8
-// stages, then one of radix 4 or 2), padded, transforms at stride ts: every
-// division is a shift, and the LDS offsets of a butterfly's points are
-// constants added to one computed base wherever the stride is a multiple of 32.
+// The same transform for a compile-time length N (power of two: radix 8
+// stages, then one of radix 4 or 2; else radices 8/4/2/3/5 as the run-time
+// plans), padded, transforms at stride ts: divisions by constants (shifts for
+// powers of two), and the LDS offsets of a butterfly's points are constants
+// added to one computed base wherever the stride is a multiple of 32.
 // ---------------------------------------------------------------------------
 template <int N, int R, int NS, bool INV, int NT, int VMAX>
 __device__ __forceinline__ void stage_pow2(float2* buf, int nb, int ts, const float2* tw) {
@@ -169,20 +170,46 @@ __device__ __forceinline__ void stage_pow2(float2* buf, int nb, int ts, const fl
   __syncthreads();
 }
 
+// radix of the next stage for the m points still to factor: 8, 4, 2, 3, 5 in
+// that order of preference (the run-time plans use the same order); 0 when m
+// has another prime factor
+__host__ __device__ constexpr int ct_radix(int m) {
+  return m % 8 == 0 ? 8 : m % 4 == 0 ? 4 : m % 2 == 0 ? 2 : m % 3 == 0 ? 3 : m % 5 == 0 ? 5 : 0;
+}
+
+// true when N factors into 2, 3 and 5
+__host__ __device__ constexpr bool ct_ok(int N) {
+  for (int m = N; m > 1;) {
+    const int r = ct_radix(m);
+    if (!r) return false;
+    m /= r;
+  }
+  return N > 1;
+}
+
 template <int N, int NS, bool INV, int NT, int VMAX>
 __device__ __forceinline__ void stages_pow2(float2* buf, int nb, int ts, const float2* tw) {
   if constexpr (NS < N) {
-    constexpr int R = N / NS >= 8 ? 8 : N / NS;
+    constexpr int R = ct_radix(N / NS);
+    static_assert(R > 0, "length with a prime factor above 5");
     stage_pow2<N, R, NS, INV, NT, VMAX>(buf, nb, ts, tw);
     stages_pow2<N, NS * R, INV, NT, VMAX>(buf, nb, ts, tw);
   }
 }
 
-// Most transforms of length N one pass of NT threads takes.
+// Most transforms of length N one pass of NT threads takes (each thread holds
+// at most VMAX values, at least one butterfly, per stage).
 template <int N, int NT, int VMAX>
 __host__ __device__ constexpr int pow2_chunk() {
-  // radix 8 stages bound it (radix 4 / 2 hold 2x / 4x the butterflies per thread)
-  return (VMAX / 8 > 0 ? VMAX / 8 : 1) * NT * 8 / N;
+  int best = 1 << 30;
+  for (int m = N; m > 1;) {
+    const int r = ct_radix(m);
+    const int bmax = VMAX / r > 0 ? VMAX / r : 1;
+    const int c = bmax * NT * r / N;
+    best = c < best ? c : best;
+    m /= r;
+  }
+  return best;
 }
 
 // nb transforms of length N at buf + t * ts (padded points), in passes of
@@ -192,6 +219,27 @@ __device__ void lds_fft_pow2(float2* buf, int nb, int ts, const float2* tw) {
   constexpr int CH = pow2_chunk<N, NT, VMAX>();
   static_assert(CH >= 1, "transform too long for one pass of the block");
   for (int t0 = 0; t0 < nb; t0 += CH) stages_pow2<N, 1, INV, NT, VMAX>(buf + t0 * ts, min(CH, nb - t0), ts, tw);
+}
+
+// nb padded transforms of plan pl at buf + t * ts: the compile-time form for
+// the lengths listed (inverse-window sizes in multiples of 16), else the
+// run-time one (chunk: ip_chunk of pl).  Called by the whole block.
+template <bool INV, int NT, int VMAX = kMaxV>
+__device__ void lds_fft_any(float2* buf, int nb, int ts, const FFTPlan& pl, const float2* tw, int chunk) {
+  switch (pl.n) {
+    case 32: lds_fft_pow2<32, INV, NT, VMAX>(buf, nb, ts, tw); break;
+    case 48: lds_fft_pow2<48, INV, NT, VMAX>(buf, nb, ts, tw); break;
+    case 64: lds_fft_pow2<64, INV, NT, VMAX>(buf, nb, ts, tw); break;
+    case 80: lds_fft_pow2<80, INV, NT, VMAX>(buf, nb, ts, tw); break;
+    case 96: lds_fft_pow2<96, INV, NT, VMAX>(buf, nb, ts, tw); break;
+    case 128: lds_fft_pow2<128, INV, NT, VMAX>(buf, nb, ts, tw); break;
+    case 144: lds_fft_pow2<144, INV, NT, VMAX>(buf, nb, ts, tw); break;
+    case 160: lds_fft_pow2<160, INV, NT, VMAX>(buf, nb, ts, tw); break;
+    case 192: lds_fft_pow2<192, INV, NT, VMAX>(buf, nb, ts, tw); break;
+    case 240: lds_fft_pow2<240, INV, NT, VMAX>(buf, nb, ts, tw); break;
+    case 256: lds_fft_pow2<256, INV, NT, VMAX>(buf, nb, ts, tw); break;
+    default: lds_fft_ip<INV, NT, true, VMAX>(buf, nb, ts, 1, false, pl, tw, chunk); break;
+  }
 }
 
 // Transforms one in-place pass can take with nt threads (each thread holds at

@@ -443,7 +443,7 @@ __global__ __launch_bounds__(kThreads) void k_ifft_rows_tilt(Geometry g, FFTPlan
   const size_t base = ((size_t)(b * g.P + p) * g.h + y0) * n;
   batched(rows * n, [&](int e) { return win[base + e]; }, [&](int e, float2 v) { at(e) = v; });
   __syncthreads();
-  lds_fft_ip<true, kThreads, true>(A, rows, LD, 1, false, pl, tw, chunk);
+  lds_fft_any<true, kThreads>(A, rows, LD, pl, tw, chunk);
   if (!ref) {
     for (int e = threadIdx.x; e < rows * n; e += kThreads) fields[base + e] = at(e);
     return;
