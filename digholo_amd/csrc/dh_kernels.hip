@@ -413,7 +413,7 @@ __global__ __launch_bounds__(kColThreads) void k_col_window_iy(Geometry g, FFTPl
     }
   }
   __syncthreads();
-  lds_fft_ip<true, kColThreads, true, kColV>(A, cols, LD, 1, false, py, twy, chunkY);
+  lds_fft_any<true, kColThreads>(A, cols, LD, py, twy, chunkY);
   float2* out = win + (size_t)(out_pos(g, b) * g.P + p) * h * w + i0;
   for (int e = threadIdx.x; e < h * cols; e += kColThreads) {
     const int j = fdiv(e, magC), c = e - j * cols;
@@ -787,7 +787,7 @@ static int ip_rows_per_block(const FFTPlan& pl, int total, int nt = kThreads) {
 }
 
 static int ip_cols_per_block(const Geometry& g, const FFTPlan& pc, const FFTPlan& py) {
-  if (ip_chunk(pc, kColThreads, kColV) < 1 || ip_chunk(py, kColThreads, kColV) < 1) return 0;
+  if (ip_chunk(pc, kColThreads, kColV) < 1 || ip_chunk(py, kColThreads) < 1) return 0;
   for (int cb = std::min(16, g.w); cb >= 1; --cb) {
     const size_t lds = sizeof(float2) * ((size_t)g.H + g.h + (size_t)pad_stride(g.H) * cb);
     if (g.h * cb <= kColThreads * kSel && lds <= 80 * 1024) return cb;
@@ -843,7 +843,7 @@ hipError_t launch_col_window_iy(const Geometry& g, const FFTPlan& pc, const FFTP
   if (e != hipSuccess) return e;
   dim3 grid((g.w + CB - 1) / CB, g.P, g.B);
   kern<<<grid, kColThreads, lds, s>>>(g, pc, py, rowspec, factor, win, CB,
-                                                 ip_chunk(pc, kColThreads, kColV), ip_chunk(py, kColThreads, kColV),
+                                                 ip_chunk(pc, kColThreads, kColV), ip_chunk(py, kColThreads),
                                                  cal, peak);
   return hipGetLastError();
 }
