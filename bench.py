@@ -5,8 +5,12 @@ Workload: 640x512 uint16 frames, dual polarisation, batch 1000, FFT window
 step = digHoloProcessBatchDevice over the whole batch with the frames already
 resident in HBM, fields and coefficients left in HBM.
 
-Prints one JSON line (see DESIGN.md for the definitions of the roofline and
-CPU-baseline fields).
+Prints one JSON line.  "roofline" is the dominant kernel (the window kernel):
+achieved = algorithmic bytes per launch (window pixels in, fields and
+coefficients out) / its average launch time from HIP events on the handle's
+stream, against 8 TB/s; "traffic" is its measured HBM bytes per launch from the
+committed rocprofv3 FETCH_SIZE / WRITE_SIZE record.  "cpu_baseline" is the FP64
+oracle on a bounded sample of the same workload, single-threaded.
 """
 from __future__ import annotations
 
@@ -14,6 +18,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import tempfile
 import sys
 import time
 
@@ -464,7 +469,8 @@ def main():
     if not args.no_autoalign:
         out["autoalign"] = autoalign_comparison(lib)
     if not args.no_uneven:
-        os.environ.setdefault("DIGHOLO_TUNE_CACHE", os.path.join(ROOT, "gpurun_out", "tune_cache.txt"))
+        # the autotuner remembers its choices for the three shapes across runs here
+        os.environ.setdefault("DIGHOLO_TUNE_CACHE", os.path.join(tempfile.gettempdir(), "digholo_tune_cache.txt"))
         out["uneven_batching"] = uneven_batching(lib)
     if not args.no_large:
         torch.cuda.empty_cache()

@@ -3,9 +3,8 @@
 The reference ships no tests or golden vectors and cannot be compiled here, so
 each oracle function is pinned by mathematics chosen so that a plausible
 mistake (dropped term, wrong sign, swapped index, wrong normalisation) fails.
-The IFFT-window sizes are the ones the survey measured by running the
-reference itself (320x256/128 -> 48, 640x512/192 -> 64, /256 -> 80,
-1024x1024/512 -> 160).
+The IFFT-window sizes are the ones the reference itself produces for these
+windows (320x256/128 -> 48, 640x512/192 -> 64, /256 -> 80, 1024x1024/512 -> 160).
 """
 import math
 
