@@ -71,31 +71,37 @@ __global__ __launch_bounds__(64 * NW) void k_overlap_mfma(Geometry g, const floa
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, q = lane >> 4;
 
-  // --- basis into LDS (rows m >= G are zero), once for every frame of the slot ---
-  for (int e = tid; e < Gp * w; e += NT) {
-    const int m = e / w, x = e - m * w;
-    const float v = m < G ? hgx[((size_t)p * G + m) * w + x] : 0.f;
-    if (SPLIT) {
-      _Float16 hi, lo;
-      split16(v, hi, lo);
-      Bx[m * pw + x] = hi;
-      Bx[Gp * pw + m * pw + x] = lo;
-    } else {
-      Bx[m * pw + x] = v;
+  // --- basis into LDS (rows m >= G are zero), once for every frame of the slot;
+  //     eight loads per thread in flight before the first is stored ---
+  auto stage_basis = [&](const float* src, E* dst, int len, int pitch) {
+    constexpr int kB = 8;
+    for (int e0 = tid; e0 < Gp * len; e0 += kB * NT) {
+      float v[kB];
+#pragma unroll
+      for (int k = 0; k < kB; ++k) {
+        const int e = e0 + k * NT;
+        const int m = e / len, x = e - m * len;
+        v[k] = (e < Gp * len && m < G) ? src[((size_t)p * G + m) * len + x] : 0.f;
+      }
+#pragma unroll
+      for (int k = 0; k < kB; ++k) {
+        const int e = e0 + k * NT;
+        if (e < Gp * len) {
+          const int m = e / len, x = e - m * len;
+          if (SPLIT) {
+            _Float16 hi, lo;
+            split16(v[k], hi, lo);
+            dst[m * pitch + x] = hi;
+            dst[Gp * pitch + m * pitch + x] = lo;
+          } else {
+            dst[m * pitch + x] = v[k];
+          }
+        }
+      }
     }
-  }
-  for (int e = tid; e < Gp * h; e += NT) {
-    const int n = e / h, y = e - n * h;
-    const float v = n < G ? hgy[((size_t)p * G + n) * h + y] : 0.f;
-    if (SPLIT) {
-      _Float16 hi, lo;
-      split16(v, hi, lo);
-      By[n * ph + y] = hi;
-      By[Gp * ph + n * ph + y] = lo;
-    } else {
-      By[n * ph + y] = v;
-    }
-  }
+  };
+  stage_basis(hgx, Bx, w, pw);
+  stage_basis(hgy, By, h, ph);
 
   // --- stage-2 tiles owned by this wave ---
   const int rt2 = (2 * Gp) / 16, ct2 = Gp / 16;
@@ -333,8 +339,10 @@ static hipError_t ov_launch(const Geometry& g, const float2* fields, const float
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  // about two resident workgroups per CU (256 CUs), each walking several frames
-  const int slots = std::max(1, std::min(g.B, 512 / std::max(1, g.P)));
+  // as many workgroups as stay resident (two per CU on 256 CUs, one when the LDS
+  // holds only one), each walking several frames with the basis staged once
+  const int perCU = lds > 80 * 1024 ? 1 : 2;
+  const int slots = std::max(1, std::min(g.B, 256 * perCU / std::max(1, g.P)));
   dim3 grid(g.P, slots);
   k_overlap_mfma<SPLIT, TPW, NW><<<grid, 64 * NW, lds, s>>>(g, fields, hgx, hgy, coefs, peak);
   return hipGetLastError();
