@@ -339,3 +339,14 @@ def test_batch_frame_limit(lib):
     coefs, fields, _, _, h = run_gpu(lib, frames, cfg, h=h)
     check(frames, cfg, coefs, fields)
     lib.digHoloDestroy(h)
+
+
+def test_window_kernel_grid_pairs_pols(lib):
+    """The window kernel's one-dimensional grid pairs the two pols of frames
+    8q .. 8q + 7 sixteen workgroups at a time and maps the frames after the
+    last full group plainly: 19 frames cover both parts."""
+    frames, cfg = make_case("dual640", 19, seed=21)
+    coefs, fields, _, _, h = run_gpu(lib, frames, cfg)
+    assert lib.digHoloGetWindowPath(h) == 3
+    check(frames, cfg, coefs, fields)
+    lib.digHoloDestroy(h)
