@@ -5,12 +5,18 @@ Workload: 640x512 uint16 frames, dual polarisation, batch 1000, FFT window
 step = digHoloProcessBatchDevice over the whole batch with the frames already
 resident in HBM, fields and coefficients left in HBM.
 
-Prints one JSON line.  "roofline" is the dominant kernel (the window kernel):
-achieved = algorithmic bytes per launch (window pixels in, fields and
-coefficients out) / its average launch time from HIP events on the handle's
-stream, against 8 TB/s; "traffic" is its measured HBM bytes per launch from the
-committed rocprofv3 FETCH_SIZE / WRITE_SIZE record.  "cpu_baseline" is the FP64
-oracle on a bounded sample of the same workload, single-threaded.
+Prints one JSON line.  "value" is frames/s over K timed steps with nothing
+but the batches on the stream (HIP events around the K steps).  "roofline" is
+the dominant kernel (the window kernel): achieved = algorithmic bytes per
+launch (window pixels in, fields and coefficients out) / its average launch
+time, from HIP events on the handle's stream around that kernel alone over a
+second set of K steps; "traffic" is its measured HBM bytes per launch from the
+committed rocprofv3 FETCH_SIZE / WRITE_SIZE record.  After timing, frames
+0, B/2 and B-1 of the timed batch are checked against the FP64 oracle.
+"host_inclusive" is the rate with host frames in and coefficients and fields
+back on the host (digHoloProcessBatch + digHoloGetFields).  "cpu_baseline" is
+the FP64 oracle on a bounded sample of the same workload, single-threaded,
+with the reference's own CPU figure (measured in another container) beside it.
 """
 from __future__ import annotations
 
@@ -106,18 +112,31 @@ def cpu_baseline(min_s=10.0, max_s=30.0):
             break
     return {"value": done / el, "unit": "frames/s", "cores": 1, "kind": "port",
             "sample": f"{done} frames of the same workload through the FP64 numpy oracle "
-                      f"(single-threaded pocketfft + einsum), {el:.1f} s"}
+                      f"(single-threaded pocketfft + einsum), {el:.1f} s",
+            "reference_cross_box": REFERENCE_CPU}
 
 
-def autoalign_comparison(lib, seed=3, batch=500):
+# The reference's own CPU path on this workload (digHoloBenchmark end-to-end
+# rate, 640x512 dual pol, batch 1000, FFT 256 -> IFFT 80, 45 groups), measured
+# in another container: it cannot be built where the GPU is.
+REFERENCE_CPU = {"value": 7712.0, "unit": "frames/s", "cores": 8, "kind": "reference",
+                 "sample": "digHoloBenchmark end-to-end 7.712 batches/s x 1000 frames, float32 frames, "
+                           "8 threads, Intel Xeon 2.70 GHz AVX2/FMA, FFTs from MKL's FFTW interface; "
+                           "measured in a different container from the GPU host"}
+
+
+def autoalign_comparison(lib, seed=3, batch=500, mdl_rel=0.05):
     """Alignment workload: 640x512 dual-pol frames, batch 500, 45 groups, with the
     true tilt (+-0.2 deg), defocus (+-0.5 dioptre), waist (+-10 %) and beam
     centre (+-50 um) of each pol drawn from the seed and hidden.  Both searches
     start from the nominal values (tilt 1.5 deg, centres -/+1.6 mm, defocus 0,
-    waist 300 um): the reference's tweak search (digHoloAutoAlign, TWEAK) and
-    the gradient search (digHoloAutoAlignGradientSearch), each timed on the
-    host around the call; the insertion loss and MDL each ends on are those of
-    its final evaluation."""
+    waist 300 um) and aim at the mode-dependent loss: the reference's tweak
+    search (digHoloAutoAlign, TWEAK, goal MDL: an insertion-loss pass, then the
+    MDL pass) and the gradient search (digHoloAutoAlignGradientSearch) on
+    -IL_lin + mu (ln s_max^2 - ln s_min^2) with mu = mdl_rel x |IL_lin| at the
+    start, each timed on the host around the call; the insertion loss and MDL
+    each ends on are those of its final evaluation.  Also the cost of one
+    gradient evaluation with and without the MDL term."""
     import numpy as np
     from digholo_amd import api, synth
     rng = np.random.default_rng(seed)
@@ -130,29 +149,46 @@ def autoalign_comparison(lib, seed=3, batch=500):
     spec.beam_centre_y = list(rng.uniform(-50e-6, 50e-6, 2))
     frames = make_frames(n_unique=50, seed=seed, spec=spec, batch=batch)
     res = {}
-    for method in ("tweak", "gradient"):
+    mu = None
+    for method in ("gradient_cost", "tweak", "gradient"):
         h = lib.digHoloCreate()
         try:
             api.configure(lib, h, frame_width=FRAME_W, frame_height=FRAME_H, pol_count=POLS,
                           fft_window=(WIN, WIN), waist=WAIST, group_count=GROUPS, beam_centre_x=BEAM_X)
             lib.digHoloConfigSetAutoAlignMode(h, api.DIGHOLO_AUTOALIGNMODE_TWEAK)
+            lib.digHoloConfigSetAutoAlignGoalIdx(h, 1)   # DIGHOLO_METRIC_MDL
             assert lib.digHoloSetBatchUint16(h, batch, C.c_void_p(frames.data_ptr()), 0) == 0
             b, m, p = C.c_int(), C.c_int(), C.c_int()
             lib.digHoloProcessBatchDevice(h, C.byref(b), C.byref(m), C.byref(p))   # tables, warm-up
             lib.digHoloSynchronize(h)
+            if method == "gradient_cost":
+                loss0, _ = api.alignment_gradient(lib, h)
+                mu = mdl_rel * abs(loss0)
+                cost = {}
+                for name, w in (("power_only", 0.0), ("with_mdl", mu)):
+                    api.alignment_gradient(lib, h, mdl_weight=w)
+                    t0 = time.perf_counter()
+                    for _ in range(5):
+                        api.alignment_gradient(lib, h, mdl_weight=w)
+                    cost[name] = round((time.perf_counter() - t0) / 5 * 1e3, 3)
+                res["gradient_eval_ms"] = cost
+                continue
             t0 = time.perf_counter()
-            ret = lib.digHoloAutoAlign(h) if method == "tweak" else lib.digHoloAutoAlignGradientSearch(h, 40, 0.0)
+            ret = lib.digHoloAutoAlign(h) if method == "tweak" else lib.digHoloAutoAlignGradientSearch(h, 40, mu)
             dt = time.perf_counter() - t0
             prm = api.get_align_params(lib, h, 2)
-            res[method] = {"seconds": round(dt, 4), "goal_il_db": round(float(ret), 4),
+            res[method] = {"seconds": round(dt, 4), "goal_mdl_db": round(float(ret), 4),
+                           "il_db": round(float(lib.digHoloAutoAlignGetMetric(h, 0)), 4),
                            "mdl_db": round(float(lib.digHoloAutoAlignGetMetric(h, 1)), 4),
                            "tilt_err_deg": round(float(np.abs(prm[:, [2, 4]] - np.array(
                                [spec.tilt_x_deg, spec.tilt_y_deg]).T).max()), 5),
                            "defocus_err": round(float(np.abs(prm[:, 1] - np.array(spec.defocus)).max()), 4)}
         finally:
             lib.digHoloDestroy(h)
+    res["mdl_weight"] = mu
     res["workload"] = (f"640x512 uint16 dual-pol, batch {batch}, 45 HG groups, hidden tilt/defocus/waist/centre "
-                       f"(seed {seed}), IL goal, both from the nominal start")
+                       f"(seed {seed}), MDL goal (gradient: power + MDL term, weight {mdl_rel} of the power "
+                       f"term), both from the nominal start")
     res["gradient_speedup"] = round(res["tweak"]["seconds"] / res["gradient"]["seconds"], 2)
     return res
 
@@ -289,18 +325,22 @@ def large_sensor(lib, batch=256, steps=10):
             assert lib.digHoloProcessBatchDevice(h, C.byref(b), C.byref(m), C.byref(p))
         lib.digHoloSynchronize(h)
         stream = torch.cuda.ExternalStream(lib.digHoloGetStream(h))
-        lib.digHoloConfigSetStageTiming(h, 1)
-        st = (C.c_float * 5)()
-        lib.digHoloGetStageTimes(h, st)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         for _ in range(steps):
             assert lib.digHoloProcessBatchDevice(h, C.byref(b), C.byref(m), C.byref(p))
         e1.record(stream)
         e1.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        # kernel times from the library's stage events, on a second set of batches
+        lib.digHoloConfigSetStageTiming(h, 1)
+        st = (C.c_float * 5)()
+        lib.digHoloGetStageTimes(h, st)
+        for _ in range(steps):
+            assert lib.digHoloProcessBatchDevice(h, C.byref(b), C.byref(m), C.byref(p))
+        lib.digHoloSynchronize(h)
         lib.digHoloGetStageTimes(h, st)
         lib.digHoloConfigSetStageTiming(h, 0)
-        ms = e0.elapsed_time(e1) / steps
         coefs = api.get_coefs(lib, h)
         err = O.relative_l2(coefs[:2], c_ref)
         ov_ms = float(st[4])
@@ -319,6 +359,50 @@ def large_sensor(lib, batch=256, steps=10):
     res["workload"] = ("1024x1024 float32 dual-pol, batch 256, FFT 512x512 -> IFFT 160x160, 80 HG groups "
                        "(3240 modes/pol), group power ~ exp(-g/20)")
     res["split_vs_fp32_overlap"] = round(res["fp32"]["overlap_ms"] / res["split16"]["overlap_ms"], 3)
+    return res
+
+
+def host_inclusive(lib, frames_dev, reps=3):
+    """Frames/s with the frames in host memory and the results on the host:
+    digHoloSetBatchUint16 on a host buffer, digHoloProcessBatch (coefficients
+    copied back by the call) and digHoloGetFields (fields copied back), timed on
+    the host around whole calls.  Pageable memory as a caller's numpy array,
+    and page-locked memory allocated for the purpose."""
+    import numpy as np
+    import torch
+    from digholo_amd import api
+    res = {}
+    for kind in ("pageable", "pinned"):
+        if kind == "pageable":
+            host = frames_dev.cpu().numpy().view(np.uint16)
+            ptr = host.ctypes.data
+        else:
+            host = torch.empty(frames_dev.shape, dtype=torch.int16, pin_memory=True)
+            host.copy_(frames_dev)
+            ptr = host.data_ptr()
+        h = lib.digHoloCreate()
+        try:
+            api.configure(lib, h, frame_width=FRAME_W, frame_height=FRAME_H, pol_count=POLS, fft_window=(WIN, WIN),
+                          waist=WAIST, group_count=GROUPS, beam_centre_x=BEAM_X)
+            lib.digHoloConfigSetOverlapPrecision(h, api.DIGHOLO_PRECISION_SPLIT16)
+            b, m, p, w, hh = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+            xp, yp = C.POINTER(C.c_float)(), C.POINTER(C.c_float)()
+
+            def call():
+                assert lib.digHoloSetBatchUint16(h, B, C.c_void_p(ptr), 0) == 0
+                assert lib.digHoloProcessBatch(h, C.byref(b), C.byref(m), C.byref(p))
+                assert lib.digHoloGetFields(h, C.byref(b), C.byref(p), C.byref(xp), C.byref(yp), C.byref(w),
+                                            C.byref(hh))
+            call()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                call()
+            dt = (time.perf_counter() - t0) / reps
+        finally:
+            lib.digHoloDestroy(h)
+        res[kind] = {"frames_s": round(B / dt, 1), "ms_per_batch": round(dt * 1e3, 3)}
+    res["note"] = ("host uint16 frames (655 MB per batch) copied in, coefficients and fields (119 MB) copied "
+                   "out on every call: bound by the host link, not by the kernels")
     return res
 
 
@@ -387,34 +471,60 @@ def main():
     # time on the stream the kernels run on (the handle's own HIP stream)
     stream = torch.cuda.ExternalStream(lib.digHoloGetStream(h))
 
-    def timed(precision):
+    def timed(precision, fused=-1):
+        """K steps with nothing else on the stream (the rate), then K steps with
+        the library's stage events around each kernel (the kernel times)."""
         assert lib.digHoloConfigSetOverlapPrecision(h, precision) == 0
+        assert lib.digHoloConfigSetFusedOverlap(h, fused) == 0
         for _ in range(args.warmup):
             step()
         lib.digHoloSynchronize(h)
-        lib.digHoloConfigSetStageTiming(h, 1)
-        stage = (C.c_float * 5)()
-        lib.digHoloGetStageTimes(h, stage)          # drop anything collected so far
         start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         start.record(stream)
-        t0 = time.perf_counter()
         for _ in range(args.steps):
             step()
         end.record(stream)
         end.synchronize()
-        wall = time.perf_counter() - t0
+        ms = start.elapsed_time(end) / args.steps
+        lib.digHoloConfigSetStageTiming(h, 1)
+        stage = (C.c_float * 5)()
+        lib.digHoloGetStageTimes(h, stage)          # drop anything collected so far
+        for _ in range(args.steps):
+            step()
+        lib.digHoloSynchronize(h)
         lib.digHoloGetStageTimes(h, stage)
         lib.digHoloConfigSetStageTiming(h, 0)
-        return start.elapsed_time(end) / args.steps, [float(x) for x in stage], wall
+        return ms, [float(x) for x in stage]
 
-    ms32, st32, _ = timed(api.DIGHOLO_PRECISION_FP32)
-    ms16, st16, wall = timed(api.DIGHOLO_PRECISION_SPLIT16)
+    ms32, st32 = timed(api.DIGHOLO_PRECISION_FP32, 0)            # window kernel + FP32 overlap kernel
+    ms16s, st16s = timed(api.DIGHOLO_PRECISION_SPLIT16, 0)       # window kernel + split overlap kernel
+    ms16, st16 = timed(api.DIGHOLO_PRECISION_SPLIT16, -1)        # split overlap inside the window kernel
     headline = api.DIGHOLO_PRECISION_SPLIT16 if args.precision == "split16" else api.DIGHOLO_PRECISION_FP32
+    if headline == api.DIGHOLO_PRECISION_FP32:
+        lib.digHoloConfigSetOverlapPrecision(h, headline)
+        lib.digHoloConfigSetFusedOverlap(h, 0)
+        step()
     ms_per_step, st = (ms16, st16) if headline == api.DIGHOLO_PRECISION_SPLIT16 else (ms32, st32)
+    path = lib.digHoloGetWindowPath(h)
+
+    # the timed batch's own outputs against the FP64 oracle: first, middle, last frame
+    lib.digHoloSynchronize(h)
+    from oracle import digholo_oracle as O
+    picks = [0, B // 2, B - 1]
+    sample = frames[picks].cpu().numpy().astype(np.uint16)
+    oc = O.Config(frame_width=FRAME_W, frame_height=FRAME_H, pol_count=POLS, fft_window_x=WIN, fft_window_y=WIN,
+                  aperture_deg=0.7, tilt_x=[1.5, 1.5], tilt_y=[1.5, 1.5], group_count=GROUPS,
+                  waist=[WAIST, WAIST], beam_centre_x=list(BEAM_X))
+    f_ref, c_ref, _ = O.process_batch(sample, oc)
+    coefs_all = api.get_coefs(lib, h)
+    fields_all = api.get_fields(lib, h)[0]
+    check = {"frames": picks,
+             "coef_rel_l2": float(f"{O.relative_l2(coefs_all[picks], c_ref):.3e}"),
+             "field_rel_l2": float(f"{O.relative_l2(fields_all[picks], f_ref):.3e}")}
+    assert check["coef_rel_l2"] <= 1e-5 and check["field_rel_l2"] <= 1e-5, check
 
     Wd, Hd, w, hh, M = geo["W"], geo["H"], geo["w"], geo["h"], m.value
     frames_s = B / (ms_per_step * 1e-3)
-    path = lib.digHoloGetWindowPath(h)
     fused_overlap = headline == api.DIGHOLO_PRECISION_SPLIT16 and path == 3 and st[4] < 0.05 * st[0]
     fft_ms = st[0] + st[1] + st[2]          # camera window -> fields (one kernel when fused)
     if fused_overlap:   # the window kernel also writes the coefficients
@@ -426,8 +536,13 @@ def main():
     kernel = {3: "k_window_v3", 2: "k_window_mfma", 1: "k_fused_window"}.get(
         path, "k_row_fft + k_col_fft_window + k_ifft_rows + k_ifft_cols_tilt")
     traffic = measured_traffic(kernel, fused_overlap)
-    # split overlap cost: its own kernel, or what fusing it adds to the window kernel
-    ov16 = st16[4] if st16[4] >= 0.05 * st16[0] else max(st16[0] - st32[0], 0.0)
+    ov_bytes = (POLS * w * hh * 8 + POLS * M * 8) * B
+
+    def overlap_stats(ov_ms, peak):
+        return {"ms": round(ov_ms, 4), "tflops": round(ov_flops / (ov_ms * 1e-3) / 1e12, 2),
+                "mfma_frac": round(ov_flops / (ov_ms * 1e-3) / 1e12 / peak, 4),
+                "hbm_frac": round(ov_bytes / (ov_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
     out = {
         "metric": METRIC,
         "value": round(frames_s, 1),
@@ -439,7 +554,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32+split-f16",
+        "dtype_note": "FP32 data and accumulation; the column DFT and the overlap multiply split-fp16 (hi+lo) "
+                      "operands on MFMA with FP32 accumulation (22 significant bits per operand)",
         "data": "synthetic (seeded HG holograms, 12-bit uint16 with read noise)",
         "config": {"workload": "640x512 uint16 dual-pol, batch 1000, FFT 256x256 -> IFFT 80x80, "
                                "45 HG groups (1035 modes/pol), frames resident in HBM",
@@ -450,19 +567,23 @@ def main():
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": fft_bytes, "avg_launch_ms": round(fft_ms, 4)},
+        "timed_batch_vs_fp64": check,
         "fp32_overlap": {"frames_s": round(B / (ms32 * 1e-3), 1), "ms_per_step": round(ms32, 4),
-                         "overlap_ms": round(st32[4], 4)},
-        "split16_overlap": {"frames_s": round(B / (ms16 * 1e-3), 1), "ms_per_step": round(ms16, 4),
-                            "overlap_ms": round(ov16, 4), "fused_into_window_kernel": fused_overlap},
-        "split_vs_fp32_end_to_end": round(ms32 / ms16, 3),
-        "split_vs_fp32_overlap_stage": round(st32[4] / ov16, 3) if ov16 > 0 else None,
+                         "window_ms": round(st32[0], 4), "overlap": overlap_stats(st32[4], FP32_PEAK_TFLOPS)},
+        "split16_separate_overlap": {"frames_s": round(B / (ms16s * 1e-3), 1), "ms_per_step": round(ms16s, 4),
+                                     "window_ms": round(st16s[0], 4),
+                                     "overlap": overlap_stats(st16s[4], 2500.0)},
+        "split16_fused_overlap": {"frames_s": round(B / (ms16 * 1e-3), 1), "ms_per_step": round(ms16, 4),
+                                  "window_with_overlap_ms": round(st16[0], 4),
+                                  "fused_into_window_kernel": fused_overlap},
+        "split_vs_fp32_end_to_end": round(ms32 / ms_per_step, 3),
+        "split_vs_fp32_overlap_kernel": round(st32[4] / st16s[4], 3) if st16s[4] > 0 else None,
         "stage_ms": {"window": st[0], "col_fft_window": st[1], "ifft_tilt": st[2], "tables": st[3],
                      "overlap": st[4]},
-        "overlap_tflops": round(ov_flops / (ov16 * 1e-3) / 1e12, 3) if ov16 > 0 else None,
         "end_to_end_hbm_frac": round(algorithmic_bytes_per_frame(POLS, Wd, Hd, w, hh, M) * frames_s / 1e9
                                      / HBM_PEAK_GBS, 4),
-        "host_wall_s": round(wall, 3),
     }
+    out["host_inclusive"] = host_inclusive(lib, frames)
     lib.digHoloDestroy(h)
     del frames
     torch.cuda.empty_cache()

@@ -22,6 +22,9 @@
 #include <thread>
 #include <vector>
 
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include "../../include/digHolo.h"
 #include "dh_eigen.h"
 #include "dh_internal.h"
@@ -258,6 +261,7 @@ struct Handle {
   DevBuf<float2> gq1, gq2, gx1, gx2;
   DevBuf<double> gPart, gSum, gAxes;
   DevBuf<float2> gExtra;
+  DevBuf<double2> mdlX, mdlY;   // MDL seed: Gram eigenvectors and their images
   Derived gradHeld;
   bool gradHeldValid = false;
 
@@ -869,6 +873,16 @@ std::string tune_path() {
   return "digholo_amd_tune.txt";
 }
 
+// Tune-cache lines carry this tag: choices written by a build whose kernels
+// differ (other tile counts, other paths) are not reused.
+constexpr const char* kTuneVersion = "v2";
+
+// A cached choice is applied only if the library could have made it itself.
+bool tune_choice_valid(const TuneChoice& t) {
+  return t.windowMode >= 0 && t.windowMode <= 3 && (t.fused == 0 || t.fused == 1) && t.tpw >= 0 && t.tpw <= 32 &&
+         std::isfinite(t.ms) && t.ms >= 0.f;
+}
+
 void tune_load_locked() {
   if (g_tuneLoaded) return;
   g_tuneLoaded = true;
@@ -876,25 +890,34 @@ void tune_load_locked() {
   if (!f) return;
   char key[512];
   TuneChoice t;
-  while (std::fscanf(f, "%511s %d %d %d %f", key, &t.windowMode, &t.fused, &t.tpw, &t.ms) == 5) g_tune[key] = t;
+  while (std::fscanf(f, "%511s %d %d %d %f", key, &t.windowMode, &t.fused, &t.tpw, &t.ms) == 5)
+    if (tune_choice_valid(t) && std::strncmp(key, kTuneVersion, std::strlen(kTuneVersion)) == 0) g_tune[key] = t;
   std::fclose(f);
+}
+
+// mkdir -p without a shell: every missing directory of the path, in order
+void make_dirs(const std::string& dir) {
+  for (size_t i = 1; i <= dir.size(); ++i)
+    if (i == dir.size() || dir[i] == '/') (void)::mkdir(dir.substr(0, i).c_str(), 0755);
 }
 
 void tune_save_locked() {
   const std::string path = tune_path();
   const size_t slash = path.find_last_of('/');
-  if (slash != std::string::npos) {
-    const std::string dir = path.substr(0, slash);
-    (void)std::system(("mkdir -p '" + dir + "' 2>/dev/null").c_str());
-  }
-  const std::string tmp = path + ".tmp";
+  if (slash != std::string::npos && slash > 0) make_dirs(path.substr(0, slash));
+  // a temporary name of this process's own, renamed over the file in one step,
+  // so concurrent writers never interleave their lines
+  const std::string tmp = path + ".tmp" + std::to_string((long)::getpid());
   FILE* f = std::fopen(tmp.c_str(), "w");
   if (!f) return;
   for (const auto& kv : g_tune)
     std::fprintf(f, "%s %d %d %d %.6f\n", kv.first.c_str(), kv.second.windowMode, kv.second.fused, kv.second.tpw,
                  kv.second.ms);
-  std::fclose(f);
-  std::rename(tmp.c_str(), path.c_str());
+  if (std::fclose(f) == 0) {
+    if (std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
+  } else {
+    std::remove(tmp.c_str());
+  }
 }
 
 std::string tune_key(const Handle& h, const Derived& d) {
@@ -903,8 +926,8 @@ std::string tune_key(const Handle& h, const Derived& d) {
   int bb = 1;
   while (bb < g.B) bb <<= 1;
   char buf[256];
-  std::snprintf(buf, sizeof(buf), "f%dx%d_p%d_W%dx%d_w%dx%d_f%dx%d_G%d_pr%d_u%d_t%d_a%d_c%d_B%d", g.frameW, g.frameH,
-                g.P, g.W, g.H, g.w, g.h, g.fw, g.fh, g.G, c.precision, h.frameU16, h.transpose, c.avgCount,
+  std::snprintf(buf, sizeof(buf), "%s_f%dx%d_p%d_W%dx%d_w%dx%d_f%dx%d_G%d_pr%d_u%d_t%d_a%d_c%d_B%d", kTuneVersion,
+                g.frameW, g.frameH, g.P, g.W, g.H, g.w, g.h, g.fw, g.fh, g.G, c.precision, h.frameU16, h.transpose, c.avgCount,
                 c.calEnabled, bb);
   return buf;
 }
@@ -2616,8 +2639,7 @@ int mdl_seed(Handle& h, double mu, double& term) {
   term = 0.0;
   const size_t total = (size_t)B * P * Mpol;
   DH_TRY(h.gExtra.ensure(total), "alloc");
-  std::vector<float2> A(total);
-  DH_TRY(hipMemcpyAsync(A.data(), h.coefs.p, total * sizeof(float2), hipMemcpyDeviceToHost, h.stream), "D2H");
+  DH_TRY(dh::launch_zero_complex(h.gExtra.p, total, h.stream), "clear");
   const bool rowsVec = B <= Mm;
   const int dim = rowsVec ? B : Mm;
   struct Pick {
@@ -2625,54 +2647,40 @@ int mdl_seed(Handle& h, double mu, double& term) {
     double l = 0;
     std::vector<std::complex<double>> x;
   } lo, hi;
+  // the Gram of each matrix on the device, its extreme eigenpairs on the host
+  DH_TRY(h.gram.ensure((size_t)dim * dim), "alloc");
+  std::vector<std::complex<double>> G((size_t)dim * dim);
   for (int s = 0; s < S; ++s) {
-    DH_TRY(h.gram.ensure((size_t)dim * dim), "alloc");
     DH_TRY(dh::launch_gram_f64(h.coefs.p, B, Mm, rowStride, s, rowsVec ? 1 : 0, h.gram.p, h.stream), "gram");
-    std::vector<std::complex<double>> G((size_t)dim * dim);
     DH_TRY(hipMemcpyAsync(G.data(), h.gram.p, G.size() * sizeof(double2), hipMemcpyDeviceToHost, h.stream), "D2H");
     DH_TRY(hipStreamSynchronize(h.stream), "sync");
     dh::HermitianExtremePairs r = dh::hermitian_extreme_pairs(dim, G);
     if (lo.s < 0 || r.lmin < lo.l) lo = Pick{s, r.lmin, r.vmin};
     if (hi.s < 0 || r.lmax > hi.l) hi = Pick{s, r.lmax, r.vmax};
   }
-  std::vector<float2> gx(total, make_float2(0.f, 0.f));
   if (dim >= 2 && mu != 0.0) {
     if (!(lo.l > 0) || !(hi.l > 0)) return DIGHOLO_ERROR_INVALIDARGUMENT;   // MDL undefined
     term = mu * (std::log(hi.l) - std::log(lo.l));
+    // the projections and the rank-one seeds run on the device over the
+    // resident coefficients; only the eigenvectors travel
+    DH_TRY(h.mdlX.ensure(2 * (size_t)dim), "alloc");
+    DH_TRY(h.mdlY.ensure(2 * (size_t)(rowsVec ? Mm : B)), "alloc");
+    std::vector<std::complex<double>> xs(2 * (size_t)dim);
+    std::copy(hi.x.begin(), hi.x.end(), xs.begin());
+    std::copy(lo.x.begin(), lo.x.end(), xs.begin() + dim);
+    DH_TRY(hipMemcpyAsync(h.mdlX.p, xs.data(), xs.size() * sizeof(double2), hipMemcpyHostToDevice, h.stream),
+           "upload");
+    const size_t ny = rowsVec ? Mm : B;
+    int k = 0;
     for (const Pick* pk : {&hi, &lo}) {
       const double sig = std::sqrt(pk->l);
       const double coef = (pk == &hi ? mu : -mu) / sig;
-      std::vector<std::complex<double>> u(B), v(Mm);
-      auto at = [&](int b, int j) {
-        const float2 a = A[(size_t)b * rowStride + (size_t)pk->s * Mpol + j];
-        return std::complex<double>(a.x, a.y);
-      };
-      if (rowsVec) {
-        u = pk->x;
-        for (int j = 0; j < Mm; ++j) {
-          std::complex<double> t = 0;
-          for (int b = 0; b < B; ++b) t += std::conj(at(b, j)) * u[b];
-          v[j] = t / sig;
-        }
-      } else {
-        v = pk->x;
-        for (int b = 0; b < B; ++b) {
-          std::complex<double> t = 0;
-          for (int j = 0; j < Mm; ++j) t += at(b, j) * v[j];
-          u[b] = t / sig;
-        }
-      }
-      for (int b = 0; b < B; ++b)
-        for (int j = 0; j < Mm; ++j) {
-          const std::complex<double> d = coef * std::conj(u[b]) * v[j];
-          float2& o = gx[(size_t)b * rowStride + (size_t)pk->s * Mpol + j];
-          o.x += (float)d.real();
-          o.y += (float)d.imag();
-        }
+      DH_TRY(dh::launch_mdl_seed_pair(h.coefs.p, B, Mm, rowStride, pk->s, rowsVec ? 1 : 0, h.mdlX.p + k * dim, sig,
+                                      coef, h.mdlY.p + k * ny, h.gExtra.p, h.stream), "MDL seed");
+      ++k;
     }
+    DH_TRY(hipStreamSynchronize(h.stream), "sync");   // xs leaves scope
   }
-  DH_TRY(hipMemcpyAsync(h.gExtra.p, gx.data(), total * sizeof(float2), hipMemcpyHostToDevice, h.stream), "upload");
-  DH_TRY(hipStreamSynchronize(h.stream), "sync");   // gx leaves scope
   return DIGHOLO_ERROR_SUCCESS;
 }
 
@@ -2931,6 +2939,7 @@ struct MultiGraph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;   // recorded after each launch; the other handles' streams wait on it
 };
 std::map<std::vector<int>, MultiGraph> g_multi;
 std::mutex g_multiMu;
@@ -2938,8 +2947,10 @@ std::mutex g_multiMu;
 void multi_release(MultiGraph& m) {
   if (m.exec) (void)hipGraphExecDestroy(m.exec);
   if (m.graph) (void)hipGraphDestroy(m.graph);
+  if (m.done) (void)hipEventDestroy(m.done);
   m.exec = nullptr;
   m.graph = nullptr;
+  m.done = nullptr;
 }
 
 // launch signature of the handle's next batch, from host state only
@@ -3042,10 +3053,16 @@ int process_multi(const std::vector<Handle*>& hs, const std::vector<int>& ids) {
     m.graph = graph;
     m.exec = exec;
     m.stream = s0;
+    if (hipEventCreateWithFlags(&m.done, hipEventDisableTiming) != hipSuccess) m.done = nullptr;
     m.sigs = sigs;
     m.hs = hs;
   }
   if (hipGraphLaunch(m.exec, h0.stream) != hipSuccess) return DIGHOLO_ERROR_ERROR;
+  // later work on any handle's own stream (result downloads, the next batch)
+  // follows the replay
+  if (m.done && hipEventRecord(m.done, h0.stream) == hipSuccess)
+    for (Handle* h : hs)
+      if (h->stream != h0.stream) (void)hipStreamWaitEvent(h->stream, m.done, 0);
   for (Handle* h : hs) {   // every handle's outputs are now those of the replay on h0's stream
     h->coefsHostValid = h->fieldsHostValid = false;
     h->analysisValid[0] = h->analysisValid[1] = false;
@@ -3966,53 +3983,105 @@ EXT_C int digHoloConfigGetVerbosity(int handleIdx) {
   return CFG.verbosity;
 }
 
-// Per-stage rates in the reference's info layout (digHolo.h:1783-1788): calls
-// per second of each stage and frames per second end to end, measured with
-// HIP events on repeated batches for about goalDuration seconds.
+// digHoloBenchmark (digHolo.cpp:13241-13408): after an untimed warm-up batch,
+// repeated batches for about goalDuration seconds.  Returns batches per second
+// end to end (the reference's hzTotalMeas, :13345, :13396) and fills info[0..6]
+// (7 floats, as the reference writes, :13374-13382): calls per second of the
+// FFT, IFFT, ApplyTilt and overlap stages; info[3] the rate at which the HG
+// basis is regenerated with a changing waist (:13318-13333); info[5] the rate
+// the stage times add up to, 1 / sum of each kernel's time per batch
+// (:13348); info[6] the measured end-to-end rate again.  Stages are HIP-event
+// intervals on the handle's stream, taken on a second set of batches so the
+// end-to-end time carries no event records between kernels.  Where one kernel
+// runs several stages (the fused window kernel: FFT, window, IFFT and tilt,
+// and with the fused overlap the coefficients too), each of those stages is
+// reported at that kernel's rate and the kernel is counted once in info[5].
 EXT_C float digHoloBenchmark(int handleIdx, float goalDuration, float* info) {
   Handle* h = get(handleIdx);
-  if (!h) return 0.f;
+  if (!h || !(goalDuration > 0.f)) return 0.f;
   const bool prevTiming = h->stageTiming;
-  h->stageTiming = true;
+  h->stageTiming = false;
   if (enqueue_batch(*h) || hipStreamSynchronize(h->stream) != hipSuccess) {
     h->stageTiming = prevTiming;
     return 0.f;
   }
-  collect_stage_times(*h);
-  double acc[5] = {0, 0, 0, 0, 0};
-  int iters = 0;
   hipEvent_t a, b;
-  (void)hipEventCreate(&a);
-  (void)hipEventCreate(&b);
-  double wall = 0;
-  while (wall < goalDuration || iters < 3) {
-    (void)hipEventRecord(a, h->stream);
-    if (enqueue_batch(*h)) break;
-    (void)hipEventRecord(b, h->stream);
-    (void)hipEventSynchronize(b);
+  if (hipEventCreate(&a) != hipSuccess) return 0.f;
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return 0.f;
+  }
+  // the first batch sizes the run: about goalDuration seconds in total
+  float ms1 = 0.f;
+  (void)hipEventRecord(a, h->stream);
+  int rc = enqueue_batch(*h);
+  (void)hipEventRecord(b, h->stream);
+  (void)hipEventSynchronize(b);
+  (void)hipEventElapsedTime(&ms1, a, b);
+  const double est = std::max(1e-6, ms1 * 1e-3);
+  const int iters = (int)std::min(1e6, std::max(3.0, std::ceil(goalDuration / est)));
+  float msAll = 0.f;
+  (void)hipEventRecord(a, h->stream);
+  for (int i = 0; i < iters && !rc; ++i) rc = enqueue_batch(*h);
+  (void)hipEventRecord(b, h->stream);
+  (void)hipEventSynchronize(b);
+  (void)hipEventElapsedTime(&msAll, a, b);
+  const double hzTotal = (!rc && msAll > 0.f) ? iters / (msAll * 1e-3) : 0.0;
+  // stage intervals on a smaller set of batches (at most a tenth of the run)
+  double st[5] = {0, 0, 0, 0, 0};
+  const int sIters = std::max(3, std::min(iters, 200));
+  if (!rc) {
+    h->stageTiming = true;
+    collect_stage_times(*h);   // drop anything pending
+    for (int i = 0; i < sIters && !rc; ++i) rc = enqueue_batch(*h);
     collect_stage_times(*h);
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, a, b);
-    wall += ms * 1e-3;
-    for (int k = 0; k < 5; ++k) acc[k] += h->stageMs[k];
-    ++iters;
-    if (iters > 100000) break;
+    for (int k = 0; k < 5; ++k) st[k] = h->stageMs[k];
+  }
+  h->stageTiming = prevTiming;
+  // HG basis regeneration with the waist moving by 1e-5 of itself per call, as
+  // the reference's loop does, into work buffers (the handle's tables stay)
+  double hzBasis = 0.0;
+  const dh::Geometry& g = h->cur.g;
+  if (!rc && g.G > 0) {
+    DevBuf<float> bx, by;
+    if (bx.ensure((size_t)g.P * g.G * g.fw) == hipSuccess && by.ensure((size_t)g.P * g.G * g.fh) == hipSuccess) {
+      dh::TableArgs ta = h->cur.tab;
+      const int n2 = std::max(3, std::min(iters * 100, 20000));
+      const double step = ta.pp[0].waist / 100000.0;
+      float msB = 0.f;
+      (void)hipEventRecord(a, h->stream);
+      for (int i = 0; i < n2; ++i) {
+        for (int p = 0; p < g.P; ++p) ta.pp[p].waist += step;
+        if (dh::launch_hg_basis(ta, bx.p, by.p, h->stream) != hipSuccess) break;
+      }
+      (void)hipEventRecord(b, h->stream);
+      (void)hipEventSynchronize(b);
+      (void)hipEventElapsedTime(&msB, a, b);
+      hzBasis = msB > 0.f ? n2 / (msB * 1e-3) : 0.0;
+    }
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
-  h->stageTiming = prevTiming;
-  const double B = h->cur.g.B;
-  const double fps = wall > 0 ? B * iters / wall : 0;
+  if (rc) return 0.f;
   if (info) {
-    const double fft = acc[0] + acc[1];
-    info[DIGHOLO_BENCHMARK_FFT] = fft > 0 ? (float)(iters * 1e3 / fft) : 0.f;
-    info[DIGHOLO_BENCHMARK_IFFT] = acc[2] > 0 ? (float)(iters * 1e3 / acc[2]) : 0.f;
-    info[DIGHOLO_BENCHMARK_APPLYTILT] = acc[2] > 0 ? (float)(iters * 1e3 / acc[2]) : 0.f;   // fused with IFFT
-    info[DIGHOLO_BENCHMARK_BASIS] = acc[3] > 0 ? (float)(iters * 1e3 / acc[3]) : 0.f;
-    info[DIGHOLO_BENCHMARK_OVERLAP] = acc[4] > 0 ? (float)(iters * 1e3 / acc[4]) : 0.f;
-    info[DIGHOLO_BENCHMARK_TOTAL] = (float)fps;
+    // stage k of stageMs: row FFT (or the fused window kernel), column FFT +
+    // window, IFFT + tilt, tables, overlap
+    const bool fused = h->lastPath > 0;
+    const bool fusedOv = fused && h->lastFusedOverlap;
+    const double fftMs = fused ? st[0] : st[0] + st[1];
+    const double ifftMs = fused ? st[0] : st[2];
+    const double ovMs = fusedOv ? st[0] : st[4];
+    const double sumMs = fused ? st[0] + (fusedOv ? 0.0 : st[4]) : st[0] + st[1] + st[2] + st[4];
+    auto hz = [](double ms) { return ms > 0 ? (float)(1e3 / ms) : 0.f; };
+    info[DIGHOLO_BENCHMARK_FFT] = hz(fftMs);
+    info[DIGHOLO_BENCHMARK_IFFT] = hz(ifftMs);
+    info[DIGHOLO_BENCHMARK_APPLYTILT] = hz(ifftMs);   // the tilt runs in the IFFT kernel on every path
+    info[DIGHOLO_BENCHMARK_BASIS] = (float)hzBasis;
+    info[DIGHOLO_BENCHMARK_OVERLAP] = g.G > 0 ? hz(ovMs) : 0.f;
+    info[DIGHOLO_BENCHMARK_TOTAL] = hz(sumMs);
+    info[DIGHOLO_BENCHMARK_TOTAL + 1] = (float)hzTotal;
   }
-  return (float)fps;
+  return (float)hzTotal;
 }
 
 // ---- extensions ----

@@ -100,6 +100,8 @@ struct TableArgs {
 // One wavelength's window factor and reference wave; d_hgx == nullptr skips the HG basis.
 hipError_t launch_tables(const TableArgs& a, const uint8_t* d_mask, float2* d_factor, float2* d_ref,
                          float* d_hgx, float* d_hgy, hipStream_t s);
+// The HG basis alone ([P][G][fw] and [P][G][fh]), as launch_tables builds it.
+hipError_t launch_hg_basis(const TableArgs& a, float* d_hgx, float* d_hgy, hipStream_t s);
 hipError_t launch_row_fft(const Geometry& g, const FFTPlan& pl, const float2* twW, const void* frames,
                           int frame_is_u16, float2* rowspec, hipStream_t s);
 hipError_t launch_col_fft_window(const Geometry& g, const FFTPlan& pl, const float2* rowspec,
@@ -191,6 +193,12 @@ hipError_t launch_metric_rows(const float2* coefs, int N, int M, int Mpol, int r
 // FP64 Gram of matrix s: rows as vectors (N x N) or columns as vectors (M x M).
 hipError_t launch_gram_f64(const float2* coefs, int N, int M, int rowStride, int s, int rowsAreVectors,
                            double2* G, hipStream_t st);
+// MDL seed of the alignment gradient for one extreme singular triplet of
+// matrix s: x is the Gram eigenvector (length N when rowsAreVectors, else M),
+// y receives its image (A^H x or A x, over sig; length M or N), and
+// g[b][s M + j] += coef conj(u[b]) v[j] with (u, v) = (x, y) or (y, x).
+hipError_t launch_mdl_seed_pair(const float2* coefs, int N, int M, int rowStride, int s, int rowsAreVectors,
+                                const double2* x, double sig, double coef, double2* y, float2* g, hipStream_t st);
 
 // Plane analysis (dh_analysis.hip).  Statistics of Nx x Ny planes: columns
 // [x0, x1) of rows other than skip0/skip1 are visited; include (Ny x Nx bytes,

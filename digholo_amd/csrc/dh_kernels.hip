@@ -735,6 +735,13 @@ hipError_t launch_tables(const TableArgs& a, const uint8_t* d_mask, float2* d_fa
   return hipGetLastError();
 }
 
+hipError_t launch_hg_basis(const TableArgs& a, float* d_hgx, float* d_hgy, hipStream_t s) {
+  if (a.g.G == 0) return hipSuccess;
+  const int t2 = a.g.P * (a.g.fw + a.g.fh);
+  k_hg_basis<<<(t2 + 63) / 64, 64, 0, s>>>(a, d_hgx, d_hgy);
+  return hipGetLastError();
+}
+
 hipError_t launch_row_fft(const Geometry& g, const FFTPlan& pl, const float2* twW, const void* frames,
                           int frame_is_u16, float2* rowspec, hipStream_t s) {
   const int RB = std::min(g.H, batch_per_block(pl.n, 2048));

@@ -97,6 +97,67 @@ __global__ __launch_bounds__(256) void k_gram_f64(const float2* __restrict__ coe
   if (a < dim && c < dim) G[(size_t)a * dim + c] = make_double2(re, im);
 }
 
+// MDL term of the alignment-loss gradient: for the extreme singular triplet
+// (sigma, u, v) of A_s, d sigma^2 / d conj(A) = sigma u v^H, so the seed of
+// mu (ln s_max^2 - ln s_min^2) is coef conj(u[b]) v[j] with coef = +-mu/sigma
+// (the conjugate because the adjoint takes dL/dc as conj(...)).  One of u, v is
+// the Gram eigenvector x; the other is its image under A^H (rowsAreVectors, x
+// of length N) or A (x of length M), over sigma.
+// k_mdl_image: y[j] = sum_b conj(A[b][j]) x[b] / sig (rows), one thread per j;
+// or y[b] = sum_j A[b][j] x[j] / sig (columns), one 256-thread block per b.
+__global__ __launch_bounds__(256) void k_mdl_image(const float2* __restrict__ coefs, int N, int M, int rowStride,
+                                                   int s, int rowsAreVectors, const double2* __restrict__ x,
+                                                   double invSig, double2* __restrict__ y) {
+  const float2* A = coefs + (size_t)s * M;
+  if (rowsAreVectors) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= M) return;
+    double re = 0, im = 0;
+    for (int b = 0; b < N; ++b) {
+      const float2 a = A[(size_t)b * rowStride + j];
+      const double2 u = x[b];
+      re += a.x * u.x + a.y * u.y;   // conj(a) u
+      im += a.x * u.y - a.y * u.x;
+    }
+    y[j] = make_double2(re * invSig, im * invSig);
+    return;
+  }
+  const int b = blockIdx.x;
+  double re = 0, im = 0;
+  for (int j = threadIdx.x; j < M; j += blockDim.x) {
+    const float2 a = A[(size_t)b * rowStride + j];
+    const double2 v = x[j];
+    re += a.x * v.x - a.y * v.y;
+    im += a.x * v.y + a.y * v.x;
+  }
+  __shared__ double red[2][256];
+  red[0][threadIdx.x] = re;
+  red[1][threadIdx.x] = im;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + o];
+      red[1][threadIdx.x] += red[1][threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) y[b] = make_double2(red[0][0] * invSig, red[1][0] * invSig);
+}
+
+// g[b][s M + j] += coef conj(u[b]) v[j] for b < N, j < M
+__global__ __launch_bounds__(256) void k_mdl_outer(float2* __restrict__ g, int N, int M, int rowStride, int s,
+                                                   const double2* __restrict__ u, const double2* __restrict__ v,
+                                                   double coef) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)N * M) return;
+  const int b = (int)(e / M), j = (int)(e - (size_t)b * M);
+  const double2 ub = u[b], vj = v[j];
+  const double re = coef * (ub.x * vj.x + ub.y * vj.y), im = coef * (ub.x * vj.y - ub.y * vj.x);
+  float2& o = g[(size_t)b * rowStride + (size_t)s * M + j];
+  o.x += (float)re;
+  o.y += (float)im;
+}
+
 hipError_t launch_metric_rows(const float2* coefs, int N, int M, int Mpol, int rowStride, int S, double* out,
                               hipStream_t st) {
   dim3 grid(N, S);
@@ -109,6 +170,19 @@ hipError_t launch_gram_f64(const float2* coefs, int N, int M, int rowStride, int
   const int dim = rowsAreVectors ? N : M;
   dim3 grid((dim + 15) / 16, (dim + 15) / 16);
   k_gram_f64<<<grid, 256, 0, st>>>(coefs, N, M, rowStride, s, rowsAreVectors, G);
+  return hipGetLastError();
+}
+
+hipError_t launch_mdl_seed_pair(const float2* coefs, int N, int M, int rowStride, int s, int rowsAreVectors,
+                                const double2* x, double sig, double coef, double2* y, float2* g, hipStream_t st) {
+  if (rowsAreVectors) k_mdl_image<<<(M + 255) / 256, 256, 0, st>>>(coefs, N, M, rowStride, s, 1, x, 1.0 / sig, y);
+  else k_mdl_image<<<N, 256, 0, st>>>(coefs, N, M, rowStride, s, 0, x, 1.0 / sig, y);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const double2* u = rowsAreVectors ? x : y;
+  const double2* v = rowsAreVectors ? y : x;
+  const size_t n = (size_t)N * M;
+  k_mdl_outer<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(g, N, M, rowStride, s, u, v, coef);
   return hipGetLastError();
 }
 

@@ -265,6 +265,10 @@ EXT_C int digHoloProcessBasisExtractCoefs(int handleIdx);                       
 
 EXT_C int digHoloConfigSetThreadCount(int handleIdx, int threadCount);            /* digHolo.h:1749 */
 EXT_C int digHoloConfigGetThreadCount(int handleIdx);                             /* digHolo.h:1757 */
+/* Returns batches per second end to end (digHolo.cpp:13345, :13396); info
+ * must hold 7 floats: per-stage calls per second at the indices below, [5] the
+ * rate the stage times add up to, [6] the measured end-to-end rate
+ * (digHolo.cpp:13374-13382). */
 EXT_C float digHoloBenchmark(int handleIdx, float goalDuration, float* info);     /* digHolo.h:1776 */
 EXT_C int digHoloConfigSetVerbosity(int handleIdx, int verbosity);                /* digHolo.h:1924 */
 EXT_C int digHoloConfigGetVerbosity(int handleIdx);                               /* digHolo.h:1937 */

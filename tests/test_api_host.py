@@ -45,6 +45,12 @@ def test_invalid_handle_conventions(lib):
     assert lib.digHoloDestroy(h) == 0
     assert lib.digHoloConfigSetPolCount(h, 1) == api.DIGHOLO_ERROR_INVALIDHANDLE
     assert lib.digHoloCreate() == h2 + 1                  # handles are never reused
+    # digHoloBenchmark: 0 for an invalid handle or a non-positive duration
+    # (digHolo.cpp:13248, :13404), before touching any device
+    info = (C.c_float * 7)(*([9.0] * 7))
+    assert lib.digHoloBenchmark(-1, C.c_float(1.0), info) == 0.0
+    assert lib.digHoloBenchmark(h2, C.c_float(0.0), info) == 0.0
+    assert list(info) == [9.0] * 7
 
 
 def test_setter_validation(lib):

@@ -179,3 +179,133 @@ def test_gradient_search_matches_tweak(lib):
         print(method, out[method][:3], "\n", out[method][3], flush=True)
     assert np.isfinite(out["gradient"][0])
     assert out["gradient"][1] >= out["tweak"][1] - 0.01
+
+
+def c3_case(B=6, seed=3):
+    """The differentiable-alignment workload: 640x512 dual-pol frames, 45 groups,
+    true tilt (+-0.2 deg), defocus (+-0.5 dioptre), waist (+-10 %) and beam
+    centre (+-50 um) drawn from the seed; the gradient is taken at the nominal
+    start (tilt 1.5 deg, centres -/+1.6 mm, defocus 0, waist 300 um) with every
+    pol lock off, so each pol has its own 6 parameters."""
+    rng = np.random.default_rng(seed)
+    spec = synth.spec_dual_640()
+    spec.tilt_x_deg = list(1.5 + rng.uniform(-0.2, 0.2, 2))
+    spec.tilt_y_deg = list(1.5 + rng.uniform(-0.2, 0.2, 2))
+    spec.defocus = list(rng.uniform(-0.5, 0.5, 2))
+    spec.waist = 300e-6 * (1 + rng.uniform(-0.1, 0.1))
+    spec.beam_centre_x = [(-1.6e-3, 1.6e-3)[p] + rng.uniform(-50e-6, 50e-6) for p in range(2)]
+    spec.beam_centre_y = list(rng.uniform(-50e-6, 50e-6, 2))
+    frames = np.ascontiguousarray(synth.make_batch(spec, B, seed=seed)[0])
+    cfg = dict(frame_width=640, frame_height=512, pol_count=2, fft_window=(256, 256), group_count=45,
+               pol_lock_tilt=0, pol_lock_defocus=0, pol_lock_waist=0)
+    start = dict(tilt_x=(1.5, 1.5), tilt_y=(1.5, 1.5), beam_centre_x=(-1.6e-3, 1.6e-3),
+                 beam_centre_y=(0.0, 0.0), defocus=(0.0, 0.0), waist=300e-6)
+    return frames, cfg, start
+
+
+def oracle_params(oc):
+    """[pol][waist, defocus, tilt x, centre x, tilt y, centre y] of an oracle Config."""
+    return np.array([[oc.waist[p], oc.defocus[p], oc.tilt_x[p], oc.beam_centre_x[p], oc.tilt_y[p],
+                      oc.beam_centre_y[p]] for p in range(2)], dtype=np.float64)
+
+
+def oracle_with(oc, prm):
+    import dataclasses
+    return dataclasses.replace(oc, waist=[float(prm[0, 0]), float(prm[1, 0])],
+                               defocus=[float(prm[0, 1]), float(prm[1, 1])],
+                               tilt_x=[float(prm[0, 2]), float(prm[1, 2])],
+                               beam_centre_x=[float(prm[0, 3]), float(prm[1, 3])],
+                               tilt_y=[float(prm[0, 4]), float(prm[1, 4])],
+                               beam_centre_y=[float(prm[0, 5]), float(prm[1, 5])])
+
+
+def oracle_central_differences(frames, oc, hold, mu, steps):
+    """FP64 central differences of the oracle loss with the placement held,
+    Richardson-extrapolated (h and h/2, error O(h^4)).  Each perturbed value is
+    rounded to float32 first (the oracle reads its parameters as float32) and
+    the difference quotient uses the rounded values."""
+    base = oracle_params(oc)
+    out = np.zeros((2, 6))
+
+    def loss_at(prm):
+        _, coefs, _ = O.process_batch(frames, oracle_with(oc, prm), hold=hold)
+        return O.alignment_loss(coefs, mu)
+
+    for p in range(2):
+        for k in range(6):
+            d = []
+            for hstep in (steps[k], steps[k] / 2):
+                vals, xs = [], []
+                for sgn in (1, -1):
+                    prm = base.copy()
+                    prm[p, k] = float(np.float32(base[p, k] + sgn * hstep))
+                    xs.append(prm[p, k])
+                    vals.append(loss_at(prm))
+                d.append((vals[0] - vals[1]) / (xs[0] - xs[1]))
+            out[p, k] = (4 * d[1] - d[0]) / 3
+    return out
+
+
+def test_c3_gradient_g45_mdl_vs_fp64_central_differences(lib):
+    """Differentiable-alignment bar: at 640x512 dual pol, 45 groups, with the
+    mode-dependent-loss term on (weight 5 % of the power term), every
+    component of the GPU adjoint gradient (12 = 2 pols x 6 parameters) is within
+    1e-3 relative of FP64 central differences of the oracle's loss with the
+    placement held.  The same gradient against central differences of the
+    library's own FP32 loss is printed beside it (FP32 differencing noise sets
+    that figure, so its bar is 1e-2)."""
+    frames, cfg, start = c3_case()
+    oc = O.Config(frame_width=640, frame_height=512, pol_count=2, fft_window_x=256, fft_window_y=256,
+                  aperture_deg=0.7, beam_centre_x=list(start["beam_centre_x"]),
+                  beam_centre_y=list(start["beam_centre_y"]), tilt_x=list(start["tilt_x"]),
+                  tilt_y=list(start["tilt_y"]), defocus=list(start["defocus"]), waist=[start["waist"]] * 2,
+                  group_count=45, pol_lock_tilt=0, pol_lock_defocus=0, pol_lock_waist=0)
+    hold = O.hold_of(oc)
+    mu = 0.05 * abs(O.alignment_grad(frames, oc, hold)[0])
+    h = lib.digHoloCreate()
+    try:
+        api.configure(lib, h, **{**cfg, **start})
+        assert lib.digHoloSetBatchUint16(h, frames.shape[0], frames.ctypes.data, 0) == 0
+        loss, grad = api.alignment_gradient(lib, h, mdl_weight=mu)
+        # the library's own FP32 loss differenced with the placement held; the
+        # C API's waist setter writes every pol (digHolo.cpp:14058-14063), so
+        # the waist is differenced for both pols at once (d/dw0 + d/dw1)
+        base = api.get_align_params(lib, h, 2)
+        fsteps = np.array([2e-6, 0.02, 2e-3, 1e-6, 2e-3, 1e-6])
+        fd32 = np.zeros((2, 6))
+        for p in range(2):
+            for k in range(6):
+                if k == 0 and p == 1:
+                    continue
+                vals, xs = [], []
+                for sgn in (1, -1):
+                    prm = base.copy()
+                    cols = [0, 1] if k == 0 else [p]
+                    for q in cols:
+                        prm[q, k] = float(np.float32(base[q, k] + sgn * fsteps[k]))
+                    xs.append(prm[p, k])
+                    api.set_align_params(lib, h, prm)
+                    vals.append(api.alignment_gradient(lib, h, hold=True, mdl_weight=mu)[0])
+                fd32[p, k] = (vals[0] - vals[1]) / (xs[0] - xs[1])
+                api.set_align_params(lib, h, base)
+    finally:
+        lib.digHoloDestroy(h)
+    want_loss, want_adj = O.alignment_grad(frames, oc, hold, mdl_weight=mu)
+    fd64 = oracle_central_differences(frames, oc, hold, mu, np.array([1e-6, 0.01, 1e-3, 0.5e-6, 1e-3, 0.5e-6]))
+    print("loss", loss, want_loss, flush=True)
+    print("GPU adjoint\n", grad, "\nFP64 central differences\n", fd64, "\nFP64 adjoint\n", want_adj,
+          "\nFP32 central differences (library)\n", fd32, flush=True)
+    rel64 = np.abs(grad - fd64) / np.abs(fd64)
+    g32, f32w = grad.copy(), fd32.copy()
+    g32[0, 0] = grad[0, 0] + grad[1, 0]        # both waists moved together
+    g32[1, 0] = f32w[1, 0] = 0.0
+    ref32 = np.abs(fd64).copy()
+    ref32[0, 0] = abs(fd64[0, 0] + fd64[1, 0])
+    ref32[1, 0] = 1.0
+    rel32 = np.abs(g32 - f32w) / ref32
+    print(f"max relative error per component: vs FP64 central differences {rel64.max():.2e}, "
+          f"vs FP32 central differences {rel32.max():.2e}, oracle adjoint vs FP64 differences "
+          f"{(np.abs(want_adj - fd64) / np.abs(fd64)).max():.2e}", flush=True)
+    assert loss == pytest.approx(want_loss, rel=2e-6)
+    assert rel64.max() <= 1e-3, rel64
+    assert rel32.max() <= 1e-2, rel32

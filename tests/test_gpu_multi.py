@@ -156,3 +156,38 @@ def test_autotune_choice_and_cache(lib, tmp_path, monkeypatch):
         assert cache.read_text() == text
     finally:
         lib.digHoloDestroy(h)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_multi_graph_outputs_match_oracle(lib, precision):
+    """The uneven-batch workload's three handle shapes (320x256 / 128 / 10
+    groups, 640x512 / 192 / 30, 1024x1024 / 256 / 45; dual pol) through one graph
+    launch: each handle's fields and coefficients, read after the replay, are
+    within 1e-5 relative L2 of the FP64 oracle (FP32 and split-fp16 overlap)."""
+    hs, frames, host = [], [], []
+    try:
+        for i, (W, H, win, G, B) in enumerate(SHAPES):
+            fr = frames_for(W, H, G, B, seed=30 + i)
+            frames.append(fr)
+            host.append(fr.cpu().numpy().astype(np.uint16))
+            h = make_handle(lib, W, H, win, G, B, fr)
+            assert lib.digHoloConfigSetOverlapPrecision(h, precision) == 0
+            hs.append(h)
+        ids = (C.c_int * len(hs))(*hs)
+        for rep in range(2):        # the first launch captures, the second replays
+            assert lib.digHoloProcessBatchMultiDevice(len(hs), ids) == 0
+        assert lib.digHoloSynchronize(hs[0]) == 0
+        for (W, H, win, G, B), h, fr in zip(SHAPES, hs, host):
+            coefs = api.get_coefs(lib, h)
+            fields = api.get_fields(lib, h)[0]
+            oc = O.Config(frame_width=W, frame_height=H, pol_count=2, fft_window_x=win, fft_window_y=win,
+                          aperture_deg=0.7, beam_centre_x=[-W * 5e-6, W * 5e-6], tilt_x=[1.5, 1.5],
+                          tilt_y=[1.5, 1.5], group_count=G, waist=[200e-6, 200e-6])
+            f_ref, c_ref, _ = O.process_batch(fr, oc)
+            ef, ec = O.relative_l2(fields, f_ref), O.relative_l2(coefs, c_ref)
+            print(f"{W}x{H} window {win} G {G} B {B} path {lib.digHoloGetWindowPath(h)}: "
+                  f"fields {ef:.2e} coefs {ec:.2e}", flush=True)
+            assert ef <= 1e-5 and ec <= 1e-5
+    finally:
+        for h in hs:
+            lib.digHoloDestroy(h)

@@ -245,6 +245,22 @@ def test_large_window_high_order(lib, precision):
     lib.digHoloDestroy(h)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_large_sensor_float32_frames(lib, precision):
+    """The large-sensor workload as specified: 1024x1024 float32 frames (not
+    quantised), dual pol, 512x512 windows, 80 groups with group power
+    ~ exp(-g/20), FP32 and split-fp16 overlap, against the oracle."""
+    frames, cfg = make_case("large1024", 2, seed=14, quantise=False)
+    assert frames.dtype == np.float32
+    h = lib.digHoloCreate()
+    api.configure(lib, h, **cfg)
+    assert lib.digHoloConfigSetOverlapPrecision(h, precision) == 0
+    coefs, fields, _, _, h = run_gpu(lib, frames, cfg, h=h)
+    assert lib.digHoloGetWindowPath(h) == 0
+    check(frames, cfg, coefs, fields)
+    lib.digHoloDestroy(h)
+
+
 @pytest.mark.parametrize("name,B,quantise,over", [
     ("radix7", 3, True, {}), ("small", 3, True, {}), ("small", 3, False, {}),
     ("dual640", 2, False, {"fft_window": (256, 192), "beam_centre_y": (30e-6, -50e-6)}),
