@@ -526,7 +526,9 @@ def main():
     Wd, Hd, w, hh, M = geo["W"], geo["H"], geo["w"], geo["h"], m.value
     frames_s = B / (ms_per_step * 1e-3)
     fused_overlap = headline == api.DIGHOLO_PRECISION_SPLIT16 and path == 3 and st[4] < 0.05 * st[0]
-    fft_ms = st[0] + st[1] + st[2]          # camera window -> fields (one kernel when fused)
+    # camera window -> fields: the one window kernel when fused (the other two
+    # stage intervals are then empty event pairs), else the three staged kernels
+    fft_ms = st[0] if path > 0 else st[0] + st[1] + st[2]
     if fused_overlap:   # the window kernel also writes the coefficients
         fft_bytes = algorithmic_bytes_per_frame(POLS, Wd, Hd, w, hh, M) * B
     else:

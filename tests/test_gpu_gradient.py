@@ -251,9 +251,11 @@ def test_c3_gradient_g45_mdl_vs_fp64_central_differences(lib):
     mode-dependent-loss term on (weight 5 % of the power term), every
     component of the GPU adjoint gradient (12 = 2 pols x 6 parameters) is within
     1e-3 relative of FP64 central differences of the oracle's loss with the
-    placement held.  The same gradient against central differences of the
-    library's own FP32 loss is printed beside it (FP32 differencing noise sets
-    that figure, so its bar is 1e-2)."""
+    placement held (the FP64 oracle's adjoint meets the same bar).  The same
+    gradient against central differences of the library's own loss is printed
+    beside it: that loss comes back as a float32 (about 4.8e9 here, one unit in
+    the last place = 512), so differencing noise and step truncation set that
+    figure, and its bar is 3e-2."""
     frames, cfg, start = c3_case()
     oc = O.Config(frame_width=640, frame_height=512, pol_count=2, fft_window_x=256, fft_window_y=256,
                   aperture_deg=0.7, beam_centre_x=list(start["beam_centre_x"]),
@@ -308,4 +310,5 @@ def test_c3_gradient_g45_mdl_vs_fp64_central_differences(lib):
           f"{(np.abs(want_adj - fd64) / np.abs(fd64)).max():.2e}", flush=True)
     assert loss == pytest.approx(want_loss, rel=2e-6)
     assert rel64.max() <= 1e-3, rel64
-    assert rel32.max() <= 1e-2, rel32
+    assert (np.abs(want_adj - fd64) / np.abs(fd64)).max() <= 1e-3
+    assert rel32.max() <= 3e-2, rel32
