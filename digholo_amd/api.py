@@ -149,6 +149,7 @@ SIGNATURES: Dict[str, Tuple[object, List[object]]] = {
     "digHoloConfigSetOverlapPrecision": (c_int, [c_int, c_int]),
     "digHoloConfigGetOverlapPrecision": (c_int, [c_int]),
     "digHoloProcessBatchDevice": (c_void_p, [c_int, P_int, P_int, P_int]),
+    "digHoloProcessBatchDeviceNoAlloc": (c_int, [c_int, c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "digHoloGetFieldsDevice": (c_void_p, [c_int, P_int, P_int, P_int, P_int]),
     "digHoloSynchronize": (c_int, [c_int]),
     "digHoloSetStream": (c_int, [c_int, c_void_p]),

@@ -28,7 +28,7 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-D__HIP_PLATFO
               "-I" + os.path.join(ROCM, "include"), "-Wall", "-Wno-unused-result"]
 
 KERNEL_SOURCES = ["dh_kernels.hip", "dh_fused.hip", "dh_overlap.hip", "dh_metrics.hip", "dh_analysis.hip",
-                  "dh_align.hip", "dh_grad.hip", "dh_basis.hip", "dh_refcal.hip"]
+                  "dh_align.hip", "dh_grad.hip", "dh_basis.hip", "dh_refcal.hip", "dh_noalloc.hip"]
 HOST_SOURCES = ["dh_api.cpp", "dh_eigen.cpp"]
 
 

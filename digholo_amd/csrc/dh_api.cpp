@@ -4125,6 +4125,95 @@ EXT_C int digHoloProcessBatchMultiDevice(int count, const int* handleIdx) {
   return process_multi(hs, ids);
 }
 
+// Allocation-free form of digHoloProcessBatchDevice (see the header): the
+// staged kernels with their intermediates in the per-GPU scratch array, on the
+// tables and plans of the handle's last processed configuration.
+EXT_C int digHoloProcessBatchDeviceNoAlloc(int handleIdx, const void* frames, int batchCount, int frameIsUint16,
+                                           complex64* fieldsOut, complex64* coefsOut) {
+  Handle* hp = get(handleIdx);
+  if (!hp) return DIGHOLO_ERROR_INVALIDHANDLE;
+  Handle& h = *hp;
+  if (!frames || !fieldsOut) return DIGHOLO_ERROR_NULLPOINTER;
+  if (batchCount < 1 || batchCount > dh::kMaxBatchFrames) return DIGHOLO_ERROR_INVALIDARGUMENT;
+  if (!h.devInit || !h.tablesValid) {
+    fprintf(stderr, "digHolo: ProcessBatchDeviceNoAlloc needs the handle's tables: process one batch with this "
+                    "configuration first\n");
+    return DIGHOLO_ERROR_NULLPOINTER;
+  }
+  const Derived& d = h.tabKey;
+  dh::Geometry g = d.g;
+  if (g.L != 1 || g.fw != g.w || g.fh != g.h || h.cfg.basisType != DIGHOLO_BASISTYPE_HG) {
+    fprintf(stderr, "digHolo: ProcessBatchDeviceNoAlloc covers one wavelength, IFFTResolutionMode 1 and the HG "
+                    "basis\n");
+    return DIGHOLO_ERROR_INVALIDARGUMENT;
+  }
+  if (g.G > 0 && !coefsOut) return DIGHOLO_ERROR_NULLPOINTER;
+  if (!is_device_pointer(frames) || !is_device_pointer(fieldsOut) || (coefsOut && !is_device_pointer(coefsOut)) ||
+      (reinterpret_cast<uintptr_t>(frames) % 8) != 0) {
+    fprintf(stderr, "digHolo: ProcessBatchDeviceNoAlloc takes device frames (8-byte aligned) and device outputs\n");
+    return DIGHOLO_ERROR_INVALIDARGUMENT;
+  }
+  // the plans and twiddles the handle's last batch made (looked up, never created)
+  auto plan = [&](int n) -> const dh::FFTPlan* {
+    auto it = h.ffts.find(n);
+    return it == h.ffts.end() ? nullptr : &it->second->plan;
+  };
+  const dh::FFTPlan *pRow = plan(g.W / 2), *pCol = plan(g.H), *pIx = plan(g.w), *pIy = plan(g.h);
+  if (!pRow || !pCol || !pIx || !pIy || h.twWn != g.W) {
+    fprintf(stderr, "digHolo: ProcessBatchDeviceNoAlloc needs the handle's FFT plans: process one batch with this "
+                    "configuration first\n");
+    return DIGHOLO_ERROR_NULLPOINTER;
+  }
+  if (!dh::staged_ip_supported(g, *pRow, *pCol, *pIx, *pIy)) {
+    fprintf(stderr, "digHolo: ProcessBatchDeviceNoAlloc: transform lengths outside the staged kernels' radices\n");
+    return DIGHOLO_ERROR_INVALIDDIMENSION;
+  }
+  g.B = batchCount;
+  g.subB = batchCount;
+  // scratch layout: row-spectrum band [B][P][H][w], window [B][P][h][w], peaks [B][P]
+  auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t BP = (size_t)g.B * g.P;
+  const size_t bandBytes = up(BP * g.H * g.w * sizeof(float2));
+  const size_t winBytes = up(BP * g.h * g.w * sizeof(float2));
+  const size_t peakBytes = up(BP * sizeof(unsigned));
+  if (bandBytes + winBytes + peakBytes > dh::kNoAllocScratchBytes) {
+    const size_t per = (size_t)g.P * ((size_t)g.H * g.w + (size_t)g.h * g.w) * sizeof(float2) + g.P * 4;
+    fprintf(stderr, "digHolo: ProcessBatchDeviceNoAlloc: %d frames need %zu bytes of scratch, more than the %zu "
+                    "available (at most about %zu frames of this shape per call)\n",
+            batchCount, bandBytes + winBytes + peakBytes, dh::kNoAllocScratchBytes, dh::kNoAllocScratchBytes / per);
+    return DIGHOLO_ERROR_MEMORYALLOCATION;
+  }
+  void* base = nullptr;
+  DH_TRY(dh::noalloc_scratch(&base), "scratch address");
+  char* sb = static_cast<char*>(base);
+  float2* band = reinterpret_cast<float2*>(sb);
+  float2* win = reinterpret_cast<float2*>(sb + bandBytes);
+  unsigned* peak = reinterpret_cast<unsigned*>(sb + bandBytes + winBytes);
+  const Config& c = h.cfg;
+  dh::BatchCal cal;
+  if (c.calEnabled && c.calP > 0 && c.calB > 0) {
+    if (h.calDevVersion != c.calVersion || !h.calDev.p) {
+      fprintf(stderr, "digHolo: ProcessBatchDeviceNoAlloc: the batch calibration is not on the device yet\n");
+      return DIGHOLO_ERROR_NULLPOINTER;
+    }
+    cal.p = h.calDev.p;
+    cal.P = c.calP;
+    cal.B = c.calB;
+  }
+  const bool split = c.precision == DIGHOLO_PRECISION_SPLIT16;
+  float2* fields = reinterpret_cast<float2*>(fieldsOut);
+  // the column kernel clears the peaks before the row-IFFT kernel raises them
+  unsigned* peakW = (g.G > 0 && split) ? peak : nullptr;
+  DH_TRY(dh::launch_row_fft_ip(g, *pRow, h.twW.p, frames, frameIsUint16 ? 1 : 0, band, h.stream), "row FFT");
+  DH_TRY(dh::launch_col_window_iy(g, *pCol, *pIy, band, h.factor.p, win, cal, h.stream, peakW), "column FFT");
+  DH_TRY(dh::launch_ifft_rows_tilt(g, *pIx, win, h.ref.p, fields, h.stream, peakW), "IFFT rows");
+  if (g.G > 0)
+    DH_TRY(dh::launch_overlap_mfma(g, fields, h.hgx.p, h.hgy.p, reinterpret_cast<float2*>(coefsOut), split ? 1 : 0,
+                                   h.stream, h.ovTpw, peakW),
+           "overlap");
+  return DIGHOLO_ERROR_SUCCESS;
+}
+
 EXT_C complex64* digHoloProcessBatchDevice(int handleIdx, int* batchCount, int* modeCount, int* polCount) {
   Handle* h = get(handleIdx);
   if (!h || enqueue_batch(*h)) {

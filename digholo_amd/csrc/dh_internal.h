@@ -12,6 +12,9 @@ constexpr int kMaxStages = 12;
 // blockIdx.y or blockIdx.z, whose grid extent is limited to 65535.
 constexpr int kMaxBatchFrames = 65535;
 
+// Bytes of the per-GPU scratch array of the allocation-free batch entry point.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+
 // One 1-D FFT length factored for the Stockham kernels.
 struct FFTPlan {
   int n;                      // transform length (complex points)
@@ -263,6 +266,10 @@ hipError_t launch_grad_kspace(const Geometry& g, const float2* x1, const float2*
                               const double* ky, double* partial, hipStream_t s);
 // out[p][k] = sum over frames of partial[b][p][k], k < 8
 hipError_t launch_grad_reduce(int B, int P, const double* partial, double* out, hipStream_t s);
+
+// Device address of the current GPU's scratch array (kNoAllocScratchBytes,
+// 256-byte aligned), looked up on the first call per device (dh_noalloc.hip).
+hipError_t noalloc_scratch(void** ptr);
 
 // Separable HG overlap on MFMA: split = 0 plain FP32, 1 split-FP16 (hi+lo).
 // tpw: stage-2 tiles per wave of the launch (0: the fewest that cover the

@@ -343,6 +343,24 @@ EXT_C int digHoloConfigGetAutoTune(int handleIdx);
  * (digHoloSynchronize) before reading them. */
 EXT_C int digHoloProcessBatchMultiDevice(int count, const int* handleIdx);
 
+/* Allocation-free batch (no counterpart in the reference), for callers that
+ * cannot allocate device memory: code under HIP graph capture, tight serving
+ * loops.  Processes batchCount device frames (uint16 when frameIsUint16, else
+ * float32; frameWidth x frameHeight each, 8-byte aligned) with the
+ * configuration whose tables the handle built last (one earlier
+ * digHoloProcessBatch* call with it is required; settings changed since are not
+ * seen) into the caller's device buffers: fields [B][pol][h][w] and, with a
+ * basis, HG coefficients [B][pol][modes] (the configured overlap precision).
+ * Asynchronous on the handle's stream.  It calls no allocator: the staged
+ * kernels keep their intermediates in a fixed 64 MiB device array per GPU
+ * (B x pol x (H + h) x w x 8 bytes; a larger batch returns
+ * DIGHOLO_ERROR_MEMORYALLOCATION and is not processed).  Every call on one GPU
+ * shares that array: calls that overlap on one GPU, from any handle or stream,
+ * race.  One wavelength, IFFTResolutionMode 1, HG basis, no frame averaging or
+ * transposition. */
+EXT_C int digHoloProcessBatchDeviceNoAlloc(int handleIdx, const void* frames, int batchCount, int frameIsUint16,
+                                           complex64* fields, complex64* coefs);
+
 /* Alignment-loss gradient.  Processes the batch at the current configuration
  * and returns loss = -(1/B) sum |c|^2 + mdlWeight (ln s_max^2 - ln s_min^2)
  * (minus the linear insertion loss without the SVD, plus a mode-dependent-loss
