@@ -614,6 +614,20 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 
+// (xr, xi) split into fp16 pairs hi = (fp16(xr), fp16(xi)) and
+// lo = (fp16(xr - hi.x), fp16(xi - hi.y)), both rounded to nearest even: the
+// residuals are exact in FP32, and v_fma_mix forms them from the fp16 halves
+// of hi and rounds them to fp16 in one instruction each (the same values as
+// converting, subtracting and converting again, in three instructions
+// instead of eight).
+__device__ __forceinline__ void split_f16x2(float xr, float xi, unsigned& hi, unsigned& lo) {
+  asm("v_cvt_pk_f16_f32 %0, %2, %3\n\t"
+      "v_fma_mixlo_f16 %1, %0, -1.0, %2 op_sel_hi:[1,0,0]\n\t"
+      "v_fma_mixhi_f16 %1, %0, -1.0, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+      : "=&v"(hi), "=&v"(lo)
+      : "v"(xr), "v"(xi));
+}
+
 constexpr int kW3Threads = 256;
 
 template <int L, int M, int U, int J>
@@ -703,13 +717,13 @@ __device__ __forceinline__ void w3_overlap(const W3Args& a, const float2* plane,
 #pragma unroll
   for (int i = 0; i < kOvMaxTiles; ++i) cacc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
   auto split8 = [&](const float* v, half8& hi, half8& lo) {
+    uint4 h4, l4;
+    unsigned* hw = reinterpret_cast<unsigned*>(&h4);
+    unsigned* lw = reinterpret_cast<unsigned*>(&l4);
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const float x = v[e] * scale;
-      const _Float16 hh = (_Float16)x;
-      hi[e] = hh;
-      lo[e] = (_Float16)(x - (float)hh);
-    }
+    for (int e = 0; e < 4; ++e) split_f16x2(v[2 * e] * scale, v[2 * e + 1] * scale, hw[e], lw[e]);
+    hi = __builtin_bit_cast(half8, h4);
+    lo = __builtin_bit_cast(half8, l4);
   };
   // stage-1 units (m-tile, part) of this wave and their basis operands, which
   // are the same for every chunk
@@ -1003,7 +1017,9 @@ __device__ __forceinline__ void w3_item(int P, int B, int& p, int& b) {
   }
 }
 
-template <int L, int M, int U, int J>
+// SQ: a square FFT window (H = W = 2 L M), so the column transform's length
+// and its table offsets are compile-time constants.
+template <int L, int M, int U, int J, bool SQ>
 __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const unsigned short* __restrict__ frames,
                                                           const float2* __restrict__ tabs,
                                                           const float2* __restrict__ factor,
@@ -1014,7 +1030,7 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
   constexpr int NU = Sh::NU, LP = Sh::LP, RPc = Sh::RPc, PK = Sh::PK, NR = Sh::NR, WP = Sh::WP;
   extern __shared__ __align__(16) unsigned char w3_smem[];
   const Geometry& g = a.g;
-  const int H = g.H;
+  const int H = SQ ? 2 * NR : g.H;
   int p, b;
   w3_item(a.g.P, a.g.B, p, b);
   const int tid = threadIdx.x, lane = tid & 63;
@@ -1112,12 +1128,10 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
     auto put = [&](int k, float2 X) {
       const unsigned c0k = (unsigned)(k - c0);
       const int c = c0k < (unsigned)w ? (int)c0k : w;
-      const float xr = X.x, xi = X.y;
-      const _Float16 rh = (_Float16)xr, ih = (_Float16)xi;
-      const half2v hi = {rh, ih};
-      const half2v lo = {(_Float16)(xr - (float)rh), (_Float16)(xi - (float)ih)};
-      *reinterpret_cast<half2v*>(Bs + c * PK + 2 * rho) = hi;
-      *reinterpret_cast<half2v*>(Bs + (w + 1 + c) * PK + 2 * rho) = lo;
+      unsigned hi, lo;
+      split_f16x2(X.x, X.y, hi, lo);
+      *reinterpret_cast<unsigned*>(Bs + c * PK + 2 * rho) = hi;
+      *reinterpret_cast<unsigned*>(Bs + (w + 1 + c) * PK + 2 * rho) = lo;
     };
     float2 v[M];
 #pragma unroll
@@ -1191,8 +1205,10 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
   // table base, the lo word 4H further on
   using lds_u32 = __attribute__((address_space(3))) const unsigned;
   const unsigned m4 = 4u * (unsigned)(H - 1), lo4 = 4u * (unsigned)H;
-  const unsigned selB = (unsigned)(size_t)((__attribute__((address_space(3))) unsigned char*)w3_smem) +
-                        ((wv & 1) ? 8u * (unsigned)H : 0u);
+  // the part's table: a bit above every masked index (8H), OR-ed in; the
+  // tables start at LDS address 0 (ldsBase folds to that constant)
+  const unsigned partB = (wv & 1) ? 8u * (unsigned)H : 0u;
+  const unsigned ldsBase = (unsigned)(size_t)((__attribute__((address_space(3))) unsigned char*)w3_smem);
   const int nUnits = (2 * J - wv + 3) / 4;   // units wv, wv + 4, ... below 2 J: NU or NU - 1
   auto gemm = [&](int rd, const _Float16* Bs) {
     (void)rd;
@@ -1211,7 +1227,7 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
             mrun[i] += ky16[i];
 #pragma unroll
             for (int e2 = 0; e2 < 4; ++e2) {
-              const unsigned ad = (tb & m4) + selB;
+              const unsigned ad = ((tb & m4) | partB) + ldsBase;
               ph[e2] = *reinterpret_cast<lds_u32*>((size_t)ad);
               pl[e2] = *reinterpret_cast<lds_u32*>((size_t)(ad + lo4));
               tb += (unsigned)ky[i];
@@ -1497,14 +1513,14 @@ static hipError_t w3_launch(W3Args a, const void* frames, const float2* tab, con
   if (a.hgx16 || a.hgx32) a.region = std::max(a.region, a.ovlOff + w3_ovl_bytes(g));
   const size_t lds = w3_lds<L, M, U, J>(a);
   if (lds > 160 * 1024) return hipErrorNotSupported;
+  auto kern = g.H == 2 * L * M ? k_window_v3<L, M, U, J, true> : k_window_v3<L, M, U, J, false>;
   if (lds > 64 * 1024) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_window_v3<L, M, U, J>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
   dim3 grid(g.P * g.B);
-  k_window_v3<L, M, U, J><<<grid, kW3Threads, lds, s>>>(a, static_cast<const unsigned short*>(frames), tab, factor,
-                                                        ref, fields);
+  kern<<<grid, kW3Threads, lds, s>>>(a, static_cast<const unsigned short*>(frames), tab, factor, ref, fields);
   return hipGetLastError();
 }
 
