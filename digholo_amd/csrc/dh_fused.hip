@@ -684,18 +684,7 @@ __device__ __forceinline__ void w3_overlap(const W3Args& a, const float2* plane,
   const Geometry& g = a.g;
   const int G = g.G, Gp = (G + 15) & ~15, ct1 = Gp / 16;
   const int lr = lane & 15, q = lane >> 4;
-  // per-field power-of-two scale: peak magnitude into [128, 256)
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o));
-  if (lane == 0) red[wave] = vmax;
-  __syncthreads();
-  vmax = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-  float scale = 1.f;
-  if (vmax > 0.f) {
-    int ex;
-    frexpf(vmax, &ex);
-    scale = ldexpf(1.f, 8 - ex);
-  }
+  float scale = 1.f;   // per-field power of two, set below once the basis loads are in flight
   const _Float16* bx = a.hgx16 + (size_t)p * 2 * Gp * w;
   const _Float16* by = a.hgy16 + (size_t)p * 2 * Gp * h;
   _Float16* A2b = reinterpret_cast<_Float16*>(st1);
@@ -749,27 +738,41 @@ __device__ __forceinline__ void w3_overlap(const W3Args& a, const float2* plane,
       }
     }
   }
-  // stage-2 basis operands of chunk ch (loaded before the chunk's stage 1)
+  // stage-2 basis operands of chunk ch, loaded one chunk ahead of their use
   half8 b2h[kOvMaxTiles], b2l[kOvMaxTiles];
-  auto load_b2 = [&](int ch) {
+  auto load_b2 = [&](int ch, half8 (&dh)[kOvMaxTiles], half8 (&dl)[kOvMaxTiles]) {
 #pragma unroll
     for (int i = 0; i < kOvMaxTiles; ++i) {
       if (i < nMine) {
         const int nt = myTile[i] % ct2;
         if (q < 2) {
           const _Float16* brow = by + (nt * 16 + lr) * h + 16 * ch + 8 * q;
-          b2h[i] = *reinterpret_cast<const half8*>(brow);
-          b2l[i] = *reinterpret_cast<const half8*>(brow + Gp * h);
+          dh[i] = *reinterpret_cast<const half8*>(brow);
+          dl[i] = *reinterpret_cast<const half8*>(brow + Gp * h);
         } else {
 #pragma unroll
-          for (int e = 0; e < 8; ++e) b2h[i][e] = b2l[i][e] = (_Float16)0.f;
+          for (int e = 0; e < 8; ++e) dh[i][e] = dl[i][e] = (_Float16)0.f;
         }
       }
     }
   };
+  load_b2(0, b2h, b2l);
+  // per-field power-of-two scale: peak magnitude into [128, 256)
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o));
+  if (lane == 0) red[wave] = vmax;
+  __syncthreads();
+  vmax = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+  if (vmax > 0.f) {
+    int ex;
+    frexpf(vmax, &ex);
+    scale = ldexpf(1.f, 8 - ex);
+  }
   for (int ch = 0; ch < J; ++ch) {
     _Float16* A2 = A2b + (ch & 1) * bufH;
-    load_b2(ch);
+    half8 b2nh[kOvMaxTiles], b2nl[kOvMaxTiles];
+    const bool more = ch + 1 < J;
+    if (more) load_b2(ch + 1, b2nh, b2nl);
     // stage 1: T_part[yy][m] = sum_x V_part[16 ch + yy][x] HGx[m][x]; a wave's
     // units share its part (u1 = wave + 4 i), so the field operand is split once
     // per k-step for all of them
@@ -845,6 +848,13 @@ __device__ __forceinline__ void w3_overlap(const W3Args& a, const float2* plane,
     // No barrier here: the next chunk writes the other buffer, and this one is
     // rewritten only after the next chunk's barrier, which every wave reaches
     // after finishing these reads.
+    if (more) {
+#pragma unroll
+      for (int i = 0; i < kOvMaxTiles; ++i) {
+        b2h[i] = b2nh[i];
+        b2l[i] = b2nl[i];
+      }
+    }
   }
   // the triangle m + n < G
   const float inv = 1.f / scale;
@@ -1303,6 +1313,34 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
     __syncthreads();
   }
 
+  // The epilogue's first row pass needs the window factor and its first
+  // column pass the reference wave (both from L2): their loads are issued
+  // here, so their latency passes while the plane is written.
+  const int t16 = lane >> 2, l4 = lane & 3;   // transform within a wave pass, lane within the transform
+  constexpr int MMx = w / 4, MMy = h / 4, USy = MMy / 4;
+  const char* fac = reinterpret_cast<const char*>(factor + (size_t)(lam_in(g, b) * g.P + p) * (h * w));
+  const char* rp = reinterpret_cast<const char*>(ref + (size_t)(lam_in(g, b) * g.P + p) * (h * w));
+  auto load_factor = [&](int ps, float2 (&f)[MMx]) {
+    const unsigned fo = (unsigned)((16 * ps + t16) * w + l4) * 8u;
+#pragma unroll
+    for (int m = 0; m < MMx; ++m) f[m] = *reinterpret_cast<const float2*>(fac + fo + 32u * m);
+  };
+  // column passes deal lanes l4-major (lane = 16 l4 + t16): a 64-bit access to
+  // one column's rows l4 + 4 m then touches each LDS bank at most twice per
+  // half wave on the plane's pitch (t16-major: four times)
+  const int tc = lane & 15, lc = lane >> 4;
+  auto load_ref = [&](int ps, float2 (&r)[USy][4]) {
+    const unsigned ro = (unsigned)(lc * USy * w + 16 * ps + tc) * 8u;
+#pragma unroll
+    for (int s2 = 0; s2 < USy; ++s2)
+#pragma unroll
+      for (int bb = 0; bb < 4; ++bb)
+        r[s2][bb] = *reinterpret_cast<const float2*>(rp + ro + (unsigned)((s2 + MMy * bb) * w) * 8u);
+  };
+  float2 fcur[MMx], rcur[USy][4];
+  if (wave < J) load_factor(wave, fcur);
+  if (wave < U) load_ref(wave, rcur);
+
   // window plane: F / xscale
   const float inv = 1.f / a.xscale;
   float* fl = reinterpret_cast<float*>(fbuf);
@@ -1318,24 +1356,21 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
     }
   }
   __syncthreads();
-  const int t16 = lane >> 2, l4 = lane & 3;   // transform within a wave pass, lane within the transform
-  // inverse transforms along x (rows), window factor on load
+  // inverse transforms along x (rows), window factor on load; a wave with a
+  // second pass loads its factor rows at the start of the first
   {
-    constexpr int MM = w / 4, US = MM / 4;
-    // factor rows through one uniform base and 32-bit lane offsets; the pass's
-    // loads are all issued before the first is used
-    const char* fac = reinterpret_cast<const char*>(factor + (size_t)(lam_in(g, b) * g.P + p) * (h * w));
+    constexpr int MM = MMx, US = MM / 4;
     const bool hasCal = a.cal.p != nullptr;
     const float2 cb = a.cal.at(p, b);
     for (int ps = wave; ps < J; ps += 4) {
       const int j = 16 * ps + t16;
       float2* row = fbuf + j * WP;
+      float2 fnx[MM];
+      const bool more = ps + 4 < J;
+      if (more) load_factor(ps + 4, fnx);
       float2 v[MM];
-      const unsigned fo = (unsigned)(j * w + l4) * 8u;
 #pragma unroll
-      for (int m = 0; m < MM; ++m) v[m] = *reinterpret_cast<const float2*>(fac + fo + 32u * m);
-#pragma unroll
-      for (int m = 0; m < MM; ++m) v[m] = cmul(row[l4 + 4 * m], v[m]);
+      for (int m = 0; m < MM; ++m) v[m] = cmul(row[l4 + 4 * m], fcur[m]);
       if (hasCal) {
 #pragma unroll
         for (int m = 0; m < MM; ++m) v[m] = cmul(v[m], cb);
@@ -1363,6 +1398,10 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
 #pragma unroll
         for (int bb = 0; bb < 4; ++bb) row[l4 * US + s2 + MM * bb] = z[s2][bb];
       }
+      if (more) {
+#pragma unroll
+        for (int m = 0; m < MM; ++m) fcur[m] = fnx[m];
+      }
     }
   }
   __syncthreads();
@@ -1371,47 +1410,43 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
   float vmax = 0.f;
   float* red = reinterpret_cast<float*>(twiy + h);   // [4] per-wave peaks
   {
-    constexpr int MM = h / 4, US = MM / 4;
-    // reference wave and fields through uniform bases and 32-bit lane offsets;
-    // the pass's reference values are loaded before its transforms
-    const char* rp = reinterpret_cast<const char*>(ref + (size_t)(lam_in(g, b) * g.P + p) * (h * w));
+    constexpr int MM = MMy, US = USy;
+    // reference wave (loaded a pass ahead) and fields through uniform bases
+    // and 32-bit lane offsets
     char* out = reinterpret_cast<char*>(fields + (size_t)(out_pos(g, b) * g.P + p) * (h * w));
     for (int ps = wave; ps < U; ps += 4) {
-      const int c = 16 * ps + t16;
+      const int c = 16 * ps + tc;
       float2* col = fbuf + c;
-      const unsigned ro = (unsigned)(l4 * US * w + c) * 8u;   // element (k, c) at ro + (k - l4 US) w 8
-      float2 rv[US][4];
-#pragma unroll
-      for (int s2 = 0; s2 < US; ++s2)
-#pragma unroll
-        for (int bb = 0; bb < 4; ++bb)
-          rv[s2][bb] = *reinterpret_cast<const float2*>(rp + ro + (unsigned)((s2 + MM * bb) * w) * 8u);
+      const unsigned ro = (unsigned)(lc * US * w + c) * 8u;   // element (k, c) at ro + (k - lc US) w 8
+      float2 rnx[US][4];
+      const bool more = ps + 4 < U;
+      if (more) load_ref(ps + 4, rnx);
       float2 v[MM];
 #pragma unroll
-      for (int m = 0; m < MM; ++m) v[m] = col[(l4 + 4 * m) * WP];
+      for (int m = 0; m < MM; ++m) v[m] = col[(lc + 4 * m) * WP];
       dft_any<MM, true>(v);
 #pragma unroll
       for (int aa = 1; aa < MM; ++aa) {
-        const float2 tw = twiy[l4 * aa];
+        const float2 tw = twiy[lc * aa];
         v[aa] = cmul(v[aa], make_float2(tw.x, -tw.y));
       }
       wave_lds_sync();
 #pragma unroll
-      for (int aa = 0; aa < MM; ++aa) col[(l4 * MM + aa) * WP] = v[aa];
+      for (int aa = 0; aa < MM; ++aa) col[(lc * MM + aa) * WP] = v[aa];
       wave_lds_sync();
       float2 z[US][4];
 #pragma unroll
       for (int s2 = 0; s2 < US; ++s2)
 #pragma unroll
-        for (int ll = 0; ll < 4; ++ll) z[s2][ll] = col[(ll * MM + l4 * US + s2) * WP];
+        for (int ll = 0; ll < 4; ++ll) z[s2][ll] = col[(ll * MM + lc * US + s2) * WP];
       wave_lds_sync();
 #pragma unroll
       for (int s2 = 0; s2 < US; ++s2) {
         dftR<4, true>(z[s2]);
 #pragma unroll
         for (int bb = 0; bb < 4; ++bb) {
-          const int k = l4 * US + s2 + MM * bb;
-          const float2 tt = cmul(z[s2][bb], rv[s2][bb]);
+          const int k = lc * US + s2 + MM * bb;
+          const float2 tt = cmul(z[s2][bb], rcur[s2][bb]);
           const float2 V = make_float2(-tt.x, tt.y);
           *reinterpret_cast<float2*>(out + ro + (unsigned)((s2 + MM * bb) * w) * 8u) = V;
           if (ovl) {   // the field stays in the plane for the overlap
@@ -1419,6 +1454,12 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
             vmax = fmaxf(vmax, fmaxf(fabsf(V.x), fabsf(V.y)));
           }
         }
+      }
+      if (more) {
+#pragma unroll
+        for (int s2 = 0; s2 < US; ++s2)
+#pragma unroll
+          for (int bb = 0; bb < 4; ++bb) rcur[s2][bb] = rnx[s2][bb];
       }
     }
   }
