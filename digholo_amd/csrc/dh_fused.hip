@@ -671,40 +671,31 @@ struct W3Args {
   float xscale;       // power of two keeping the row spectra inside fp16
 };
 
-constexpr int kOvPK = 24;   // halfs per stage-1 output row (16 field rows + pad)
 constexpr int kOvMaxTiles = 4;   // stage-2 tiles per wave (G up to ~50)
 
 // Split-fp16 separable overlap of the h x w field in the LDS plane (pitch
-// w + 4), 256 threads; stage-1 buffers at st1 (2 x [hi/lo][2 Gp][kOvPK] halfs).
-// Writes coefs[b][p][mode], modes ordered by group g, k = 0..g (m = g - k, n = k).
+// w + 4), 256 threads.  Writes coefs[b][p][mode], modes ordered by group g,
+// k = 0..g (m = g - k, n = k).
+//
+// Stage 1 per 16-row chunk of the field: T_part[m][yy] = sum_x V_part[yy][x]
+// HGx[m][x] on v_mfma_f32_16x16x32_f16, one unit (16 modes m, real or
+// imaginary part) per output tile; lane (m, q) receives T[m][4q .. 4q + 3],
+// which is exactly the A operand of v_mfma_f32_16x16x16_f16 (row m, k-slots
+// 4q .. 4q + 3).  Stage 2, C_part[m][n] += sum_yy T_part[m][yy] HGy[n][yy],
+// therefore runs on the same wave from registers with K = 16 = the chunk's
+// rows: no exchange through LDS and no barrier per chunk.
+typedef _Float16 half4 __attribute__((ext_vector_type(4)));
 template <int U, int J>
-__device__ __forceinline__ void w3_overlap(const W3Args& a, const float2* plane, float vmax, float* red,
-                                           unsigned char* st1, int p, int b, int wave, int lane) {
+__device__ __forceinline__ void w3_overlap(const W3Args& a, const float2* plane, float vmax, float* red, int p,
+                                           int b, int wave, int lane) {
   constexpr int w = 16 * U, h = 16 * J, WP = w + 4;
   const Geometry& g = a.g;
   const int G = g.G, Gp = (G + 15) & ~15, ct1 = Gp / 16;
   const int lr = lane & 15, q = lane >> 4;
+  const int part = wave & 1;   // the wave's units u1 = wave + 4 i all have this part
   float scale = 1.f;   // per-field power of two, set below once the basis loads are in flight
   const _Float16* bx = a.hgx16 + (size_t)p * 2 * Gp * w;
   const _Float16* by = a.hgy16 + (size_t)p * 2 * Gp * h;
-  _Float16* A2b = reinterpret_cast<_Float16*>(st1);
-  const int bufH = 2 * 2 * Gp * kOvPK;   // halfs per buffer
-  // stage-2 tiles of this wave: rows (m, part) in 16s, columns n in 16s
-  const int rt2 = (2 * Gp) / 16, ct2 = Gp / 16;
-  int myTile[kOvMaxTiles];
-  int nMine = 0;
-  {
-    int cnt = 0;
-    for (int t = 0; t < rt2 * ct2; ++t) {
-      const int mt = t / ct2, nt = t - mt * ct2;
-      if (8 * mt + 16 * nt >= G) continue;   // no (m, n) of the tile has m + n < G
-      if (cnt % 4 == wave && nMine < kOvMaxTiles) myTile[nMine++] = t;
-      ++cnt;
-    }
-  }
-  floatx4 cacc[kOvMaxTiles];
-#pragma unroll
-  for (int i = 0; i < kOvMaxTiles; ++i) cacc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
   auto split8 = [&](const float* v, half8& hi, half8& lo) {
     uint4 h4, l4;
     unsigned* hw = reinterpret_cast<unsigned*>(&h4);
@@ -718,6 +709,7 @@ __device__ __forceinline__ void w3_overlap(const W3Args& a, const float2* plane,
   // are the same for every chunk
   constexpr int KS1 = (w + 31) / 32;
   constexpr int NU1 = 2;                       // units per wave: 2 ct1 <= 8 (Gp <= 64)
+  constexpr int NT2 = 4;                       // stage-2 n-tiles: Gp / 16 <= 4
   half8 b1h[NU1][KS1], b1l[NU1][KS1];
 #pragma unroll
   for (int i = 0; i < NU1; ++i) {
@@ -738,30 +730,30 @@ __device__ __forceinline__ void w3_overlap(const W3Args& a, const float2* plane,
       }
     }
   }
-  // stage-2 basis operands of chunk ch, loaded one chunk ahead of their use
-  half8 b2h[kOvMaxTiles], b2l[kOvMaxTiles];
-  auto load_b2 = [&](int ch, half8 (&dh)[kOvMaxTiles], half8 (&dl)[kOvMaxTiles]) {
+  // stage-2 basis operands of chunk ch (lane (n, q): HGy[16 nt + n][16 ch + 4q ..
+  // + 3], rows n >= G zero), loaded one chunk ahead of their use
+  auto load_b2 = [&](int ch, half4 (&dh)[NT2], half4 (&dl)[NT2]) {
 #pragma unroll
-    for (int i = 0; i < kOvMaxTiles; ++i) {
-      if (i < nMine) {
-        const int nt = myTile[i] % ct2;
-        if (q < 2) {
-          const _Float16* brow = by + (nt * 16 + lr) * h + 16 * ch + 8 * q;
-          dh[i] = *reinterpret_cast<const half8*>(brow);
-          dl[i] = *reinterpret_cast<const half8*>(brow + Gp * h);
-        } else {
-#pragma unroll
-          for (int e = 0; e < 8; ++e) dh[i][e] = dl[i][e] = (_Float16)0.f;
-        }
+    for (int nt = 0; nt < NT2; ++nt) {
+      if (nt < Gp / 16) {
+        const _Float16* brow = by + (nt * 16 + lr) * h + 16 * ch + 4 * q;
+        dh[nt] = *reinterpret_cast<const half4*>(brow);
+        dl[nt] = *reinterpret_cast<const half4*>(brow + Gp * h);
       }
     }
   };
+  half4 b2h[NT2], b2l[NT2];
   load_b2(0, b2h, b2l);
+  floatx4 cacc[NU1][NT2];
+#pragma unroll
+  for (int i = 0; i < NU1; ++i)
+#pragma unroll
+    for (int nt = 0; nt < NT2; ++nt) cacc[i][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
   // per-field power-of-two scale: peak magnitude into [128, 256)
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o));
   if (lane == 0) red[wave] = vmax;
-  __syncthreads();
+  __syncthreads();   // also: the whole field plane is written
   vmax = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
   if (vmax > 0.f) {
     int ex;
@@ -769,108 +761,87 @@ __device__ __forceinline__ void w3_overlap(const W3Args& a, const float2* plane,
     scale = ldexpf(1.f, 8 - ex);
   }
   for (int ch = 0; ch < J; ++ch) {
-    _Float16* A2 = A2b + (ch & 1) * bufH;
-    half8 b2nh[kOvMaxTiles], b2nl[kOvMaxTiles];
+    half4 b2nh[NT2], b2nl[NT2];
     const bool more = ch + 1 < J;
     if (more) load_b2(ch + 1, b2nh, b2nl);
-    // stage 1: T_part[yy][m] = sum_x V_part[16 ch + yy][x] HGx[m][x]; a wave's
-    // units share its part (u1 = wave + 4 i), so the field operand is split once
-    // per k-step for all of them
-    {
-      const int part = wave & 1;
-      floatx4 d[NU1];
+    // stage 1: the field operand is split once per k-step for both units
+    floatx4 d[NU1];
 #pragma unroll
-      for (int i = 0; i < NU1; ++i) d[i] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < NU1; ++i) d[i] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int ks = 0; ks < KS1; ++ks) {
-        const int kb = 32 * ks + 8 * q;
-        half8 ah, al;
-        if (kb < w) {   // a ragged last step (w % 32 == 16) zero-fills the upper lane groups
-          const float4* vp = reinterpret_cast<const float4*>(plane + (16 * ch + lr) * WP + kb);
-          float vv[8];
+    for (int ks = 0; ks < KS1; ++ks) {
+      const int kb = 32 * ks + 8 * q;
+      half8 ah, al;
+      if (kb < w) {   // a ragged last step (w % 32 == 16) zero-fills the upper lane groups
+        const float4* vp = reinterpret_cast<const float4*>(plane + (16 * ch + lr) * WP + kb);
+        float vv[8];
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const float4 t = vp[e];
-            vv[2 * e] = part ? t.y : t.x;
-            vv[2 * e + 1] = part ? t.w : t.z;
-          }
-          split8(vv, ah, al);
-        } else {
-#pragma unroll
-          for (int e = 0; e < 8; ++e) ah[e] = al[e] = (_Float16)0.f;
+        for (int e = 0; e < 4; ++e) {
+          const float4 t = vp[e];
+          vv[2 * e] = part ? t.y : t.x;
+          vv[2 * e + 1] = part ? t.w : t.z;
         }
+        split8(vv, ah, al);
+      } else {
 #pragma unroll
-        for (int i = 0; i < NU1; ++i) {
-          if (wave + 4 * i < 2 * ct1) {
-            d[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, b1h[i][ks], d[i], 0, 0, 0);
-            d[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, b1l[i][ks], d[i], 0, 0, 0);
-            d[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, b1h[i][ks], d[i], 0, 0, 0);
-          }
-        }
+        for (int e = 0; e < 8; ++e) ah[e] = al[e] = (_Float16)0.f;
       }
-      // D[yy = 4q + e][m = 16 mt + lr] -> A2[hi/lo][(m, part)][yy]
 #pragma unroll
       for (int i = 0; i < NU1; ++i) {
-        const int u1 = wave + 4 * i;
-        if (u1 < 2 * ct1) {
-          const int mt = u1 >> 1;
-          const int row = 2 * (16 * mt + lr) + part;
+        if (wave + 4 * i < 2 * ct1) {
+          d[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, b1h[i][ks], d[i], 0, 0, 0);
+          d[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, b1l[i][ks], d[i], 0, 0, 0);
+          d[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, b1h[i][ks], d[i], 0, 0, 0);
+        }
+      }
+    }
+    // stage 2 from registers: T (split into fp16 hi + lo) times the chunk's HGy
+    // rows, over the n-tiles that hold some m + n < G
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int yy = 4 * q + e;
-            const _Float16 dh = (_Float16)d[i][e];
-            A2[row * kOvPK + yy] = dh;
-            A2[2 * Gp * kOvPK + row * kOvPK + yy] = (_Float16)(d[i][e] - (float)dh);
+    for (int i = 0; i < NU1; ++i) {
+      const int u1 = wave + 4 * i;
+      if (u1 < 2 * ct1) {
+        const int mt = u1 >> 1;
+        uint2 th, tl;
+        split_f16x2(d[i][0], d[i][1], th.x, tl.x);
+        split_f16x2(d[i][2], d[i][3], th.y, tl.y);
+        const half4 ah = __builtin_bit_cast(half4, th), al = __builtin_bit_cast(half4, tl);
+#pragma unroll
+        for (int nt = 0; nt < NT2; ++nt) {
+          if (nt < Gp / 16 && 16 * mt + 16 * nt < G) {
+            cacc[i][nt] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, b2h[nt], cacc[i][nt], 0, 0, 0);
+            cacc[i][nt] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, b2l[nt], cacc[i][nt], 0, 0, 0);
+            cacc[i][nt] = __builtin_amdgcn_mfma_f32_16x16x16f16(al, b2h[nt], cacc[i][nt], 0, 0, 0);
           }
         }
       }
     }
-    __syncthreads();   // this chunk's A2 is complete (the other buffer is free: see below)
-    // stage 2: C[(m, part)][n] += sum_yy A2[(m, part)][yy] HGy[n][16 ch + yy]
-#pragma unroll
-    for (int i = 0; i < kOvMaxTiles; ++i) {
-      if (i < nMine) {
-        const int mt = myTile[i] / ct2;
-        half8 ah, al;
-        if (q < 2) {   // K = 16: lane groups 0, 1 carry yy 0..15, groups 2, 3 are zero
-          const _Float16* arow = A2 + (mt * 16 + lr) * kOvPK + 8 * q;
-          ah = *reinterpret_cast<const half8*>(arow);
-          al = *reinterpret_cast<const half8*>(arow + 2 * Gp * kOvPK);
-        } else {
-#pragma unroll
-          for (int e = 0; e < 8; ++e) ah[e] = al[e] = (_Float16)0.f;
-        }
-        cacc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, b2h[i], cacc[i], 0, 0, 0);
-        cacc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, b2l[i], cacc[i], 0, 0, 0);
-        cacc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, b2h[i], cacc[i], 0, 0, 0);
-      }
-    }
-    // No barrier here: the next chunk writes the other buffer, and this one is
-    // rewritten only after the next chunk's barrier, which every wave reaches
-    // after finishing these reads.
     if (more) {
 #pragma unroll
-      for (int i = 0; i < kOvMaxTiles; ++i) {
-        b2h[i] = b2nh[i];
-        b2l[i] = b2nl[i];
+      for (int nt = 0; nt < NT2; ++nt) {
+        b2h[nt] = b2nh[nt];
+        b2l[nt] = b2nl[nt];
       }
     }
   }
-  // the triangle m + n < G
+  // the triangle m + n < G; lane (n, q) holds m = 16 mt + 4q .. + 3 of this part
   const float inv = 1.f / scale;
-  float2* out = a.coefs + (size_t)(b * g.P + p) * g.M;
+  float* out = reinterpret_cast<float*>(a.coefs + (size_t)(b * g.P + p) * g.M) + part;
 #pragma unroll
-  for (int i = 0; i < kOvMaxTiles; ++i) {
-    if (i < nMine) {
-      const int t = myTile[i];
-      const int mt = t / ct2, nt = t - mt * ct2;
-      const int n = nt * 16 + lr;
+  for (int i = 0; i < NU1; ++i) {
+    const int u1 = wave + 4 * i;
+    if (u1 < 2 * ct1) {
+      const int mt = u1 >> 1;
 #pragma unroll
-      for (int ii = 0; ii < 4; ii += 2) {
-        const int m = (mt * 16 + 4 * q + ii) >> 1;   // rows ii, ii + 1 are (m, re), (m, im)
-        if (n < G && m < G && m + n < G) {
-          const int gs = m + n;
-          out[gs * (gs + 1) / 2 + n] = make_float2(cacc[i][ii] * inv, cacc[i][ii + 1] * inv);
+      for (int nt = 0; nt < NT2; ++nt) {
+        const int n = nt * 16 + lr;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int m = 16 * mt + 4 * q + e;
+          if (nt < Gp / 16 && m + n < G) {
+            const int gs = m + n;
+            out[2 * (gs * (gs + 1) / 2 + n)] = cacc[i][nt][e] * inv;
+          }
         }
       }
     }
@@ -1463,7 +1434,7 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
       }
     }
   }
-  if (a.hgx16) w3_overlap<U, J>(a, fbuf, vmax, red, w3_smem + a.ovlOff, p, out_pos(g, b), wave, lane);
+  if (a.hgx16) w3_overlap<U, J>(a, fbuf, vmax, red, p, out_pos(g, b), wave, lane);
   else if (a.hgx32) {
     __syncthreads();   // the whole plane is written before any wave reads it
     w3_overlap32<U, J>(a, fbuf, w3_smem + a.ovlOff, p, out_pos(g, b), wave, lane);
@@ -1540,7 +1511,7 @@ bool window_v3_overlap_fits(const Geometry& g) {
 
 static int w3_ovl_bytes(const Geometry& g) {
   const int Gp = (g.G + 15) & ~15;
-  return std::max(2 * (2 * 2 * Gp * kOvPK * 2), 2 * (2 * Gp * kOvPK32 * 4));   // split or FP32 buffers
+  return 2 * (2 * Gp * kOvPK32 * 4);   // the FP32 form's double-buffered stage-1 output
 }
 
 template <int L, int M, int U, int J>
@@ -1551,7 +1522,7 @@ static hipError_t w3_launch(W3Args a, const void* frames, const float2* tab, con
   if (g.H % Sh::RR) return hipErrorNotSupported;
   a.region = w3_region(Sh::EBYTES, Sh::BBYTES, g.H, L, M, Sh::w, Sh::h);
   a.ovlOff = 8 * Sh::h * Sh::WP;
-  if (a.hgx16 || a.hgx32) a.region = std::max(a.region, a.ovlOff + w3_ovl_bytes(g));
+  if (a.hgx32) a.region = std::max(a.region, a.ovlOff + w3_ovl_bytes(g));   // the FP32 form's stage-1 buffers
   const size_t lds = w3_lds<L, M, U, J>(a);
   if (lds > 160 * 1024) return hipErrorNotSupported;
   auto kern = g.H == 2 * L * M ? k_window_v3<L, M, U, J, true> : k_window_v3<L, M, U, J, false>;
