@@ -6,15 +6,20 @@
 // One workgroup (8 waves) per pol and frame slot walks the frames b =
 // slot, slot + slots, ... (the basis stays in LDS for all of them) and each
 // field in chunks of 16 field rows; the next chunk's field values are loaded
-// into registers while the current chunk is multiplied.  Per chunk, stage 1 is a (32 x Gp x w) product whose
-// output is re-laid out in LDS as the A operand of stage 2, a (2Gp x Gp x 16)
-// product accumulated in registers across chunks.  Tiles of stage 2 that hold
-// no mode with m+n < G are skipped.
+// into registers while the current chunk is multiplied.  Per chunk, stage 1 is
+// a (32 x Gp x w) product over the chunk's rows grouped by part (rows 0-15 the
+// real parts of field rows 0-15, rows 16-31 the imaginary parts), so each lane
+// of a stage-1 tile receives T_c[m][yy] for four consecutive yy, stored as one
+// packed write into the A operand of stage 2, a (2Gp x Gp x 16) product
+// accumulated in registers across chunks.  Tiles of stage 2 that hold no mode
+// with m+n < G are skipped.
 //
 // Two arithmetic forms:
 //  * FP32:   v_mfma_f32_16x16x4_f32 on float operands;
 //  * SPLIT:  each operand x = hi + lo with hi = fp16(x), lo = fp16(x - hi);
-//            hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_f16, FP32 accumulate.
+//            hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_f16 (stage 1) and
+//            v_mfma_f32_16x16x16_f16 (stage 2, K = the 16 rows of a chunk),
+//            FP32 accumulate.
 //            The field is scaled by a power of two per (frame, pol) so that its
 //            largest magnitude lies in (128, 256]; every partial sum then stays
 //            far inside the fp16 range, and the scale is removed at the end.
@@ -31,7 +36,19 @@
 namespace dh {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef _Float16 half4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// (xr, xi) -> fp16 pairs hi = (fp16(xr), fp16(xi)), lo = fp16 of the exact
+// residuals, both rounded to nearest even (cvt_pk + fma_mix: the values of
+// convert / subtract / convert).
+__device__ __forceinline__ void split_pair(float xr, float xi, unsigned& hi, unsigned& lo) {
+  asm("v_cvt_pk_f16_f32 %0, %2, %3\n\t"
+      "v_fma_mixlo_f16 %1, %0, -1.0, %2 op_sel_hi:[1,0,0]\n\t"
+      "v_fma_mixhi_f16 %1, %0, -1.0, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+      : "=&v"(hi), "=&v"(lo)
+      : "v"(xr), "v"(xi));
+}
 
 constexpr int kChunk = 16;          // field rows per chunk (K of stage 2)
 
@@ -48,6 +65,9 @@ __device__ __forceinline__ void split16(float x, _Float16& hi, _Float16& lo) {
 // row pitch (elements) of an LDS matrix with K columns: pad by 8 elements to
 // spread 16 consecutive rows over the banks
 __host__ __device__ constexpr int pitch_of(int K) { return K + 8; }
+// row pitch of the stage-2 A operand (16 chunk rows): 20 elements keep its
+// 8- and 16-byte accesses aligned and 16 consecutive rows on distinct banks
+constexpr int kPitch2 = 20;
 
 template <bool SPLIT, int TPW, int NW>
 __global__ __launch_bounds__(64 * NW) void k_overlap_mfma(Geometry g, const float2* __restrict__ fields,
@@ -60,11 +80,11 @@ __global__ __launch_bounds__(64 * NW) void k_overlap_mfma(Geometry g, const floa
   const int p = blockIdx.x;
   const int G = g.G, w = g.w, h = g.h, M = g.M;
   const int Gp = (G + 15) & ~15;
-  const int pw = pitch_of(w), ph = pitch_of(h), pk = pitch_of(kChunk);
+  const int pw = pitch_of(w), ph = pitch_of(h), pk = kPitch2;
   const int nL = SPLIT ? 2 : 1;   // hi (+ lo) planes
   E* Bx = reinterpret_cast<E*>(ov_smem);          // [nL][Gp][pw]   HGx[m][x]
   E* By = Bx + nL * Gp * pw;                      // [nL][Gp][ph]   HGy[n][y]
-  E* A1 = By + nL * Gp * ph;                      // [nL][32][pw]   V chunk, rows (yy, c)
+  E* A1 = By + nL * Gp * ph;                      // [nL][32][pw]   V chunk, rows (c, yy)
   E* A2 = A1 + nL * 32 * pw;                      // [nL][2Gp][pk]  T chunk, rows (m, c)
   constexpr int NT = 64 * NW;
   __shared__ float s_red[NW];
@@ -117,13 +137,15 @@ __global__ __launch_bounds__(64 * NW) void k_overlap_mfma(Geometry g, const floa
     }
   }
   const int rt1 = 2, ct1 = Gp / 16;
-  // this thread's elements of a chunk: e = tid + 256 j < 16 w; held in registers
-  // one chunk ahead when w <= 128 (j < 8), else loaded as they are stored
-  constexpr int kPerT = 8;
-  const int nPer = (kChunk * w - tid + NT - 1) / NT;
-  const bool ahead = kChunk * w <= kPerT * NT;
+  // this thread's element pairs of a chunk: (2 e2, 2 e2 + 1), e2 = tid + NT j <
+  // 8 w (w is even, so a pair never straddles rows); held in registers one chunk
+  // ahead when 8 w <= kPerT NT, else loaded as they are stored
+  constexpr int kPerT = 4;
+  const int nPer = (kChunk * w / 2 - tid + NT - 1) / NT;
+  const bool ahead = kChunk * w / 2 <= kPerT * NT;
   for (int b = blockIdx.y; b < g.B; b += gridDim.y) {
   const float2* V = fields + (size_t)(b * g.P + p) * h * w;
+  const float4* V4 = reinterpret_cast<const float4*>(V);
 
   // --- per-field power-of-two scale (split form only) ---
   float scale = 1.f;
@@ -155,40 +177,42 @@ __global__ __launch_bounds__(64 * NW) void k_overlap_mfma(Geometry g, const floa
 #pragma unroll
   for (int i = 0; i < TPW; ++i) acc[i] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  float2 nxt[kPerT];   // the next chunk's values, in flight during the current chunk
+  float4 nxt[kPerT];   // the next chunk's pairs, in flight during the current chunk
 #pragma unroll
   for (int j = 0; j < kPerT; ++j)
-    if (ahead && j < nPer) nxt[j] = V[tid + NT * j];
+    if (ahead && j < nPer) nxt[j] = V4[tid + NT * j];
   for (int y0 = 0; y0 < h; y0 += kChunk) {
     __syncthreads();   // previous chunk's A1/A2 readers are done (and basis/scale ready)
-    // V chunk -> A1 rows (yy, c)
-    auto store_a1 = [&](int e, float2 v) {
+    // V chunk -> A1: real parts in rows yy, imaginary parts in rows 16 + yy;
+    // one 32-bit (fp16 pair) or 64-bit (float pair) write per part and plane
+    auto store_a1 = [&](int e2, float4 v) {
+      const int e = 2 * e2;
       const int yy = e / w, x = e - yy * w;
       if (SPLIT) {
-        _Float16 hi, lo;
-        split16(v.x * scale, hi, lo);
-        A1[(2 * yy) * pw + x] = hi;
-        A1[32 * pw + (2 * yy) * pw + x] = lo;
-        split16(v.y * scale, hi, lo);
-        A1[(2 * yy + 1) * pw + x] = hi;
-        A1[32 * pw + (2 * yy + 1) * pw + x] = lo;
+        unsigned rh, rl, ih, il;
+        split_pair(v.x * scale, v.z * scale, rh, rl);
+        split_pair(v.y * scale, v.w * scale, ih, il);
+        *reinterpret_cast<unsigned*>(A1 + yy * pw + x) = rh;
+        *reinterpret_cast<unsigned*>(A1 + 32 * pw + yy * pw + x) = rl;
+        *reinterpret_cast<unsigned*>(A1 + (16 + yy) * pw + x) = ih;
+        *reinterpret_cast<unsigned*>(A1 + 32 * pw + (16 + yy) * pw + x) = il;
       } else {
-        A1[(2 * yy) * pw + x] = v.x;
-        A1[(2 * yy + 1) * pw + x] = v.y;
+        *reinterpret_cast<float2*>(A1 + yy * pw + x) = make_float2(v.x, v.z);
+        *reinterpret_cast<float2*>(A1 + (16 + yy) * pw + x) = make_float2(v.y, v.w);
       }
     };
     if (ahead) {
 #pragma unroll
       for (int j = 0; j < kPerT; ++j) {
         if (j < nPer) {
-          const int e = tid + NT * j;
-          const float2 v = nxt[j];
-          if (y0 + kChunk < h) nxt[j] = V[(size_t)(y0 + kChunk) * w + e];
-          store_a1(e, v);
+          const int e2 = tid + NT * j;
+          const float4 v = nxt[j];
+          if (y0 + kChunk < h) nxt[j] = V4[(size_t)(y0 + kChunk) * (w / 2) + e2];
+          store_a1(e2, v);
         }
       }
     } else {
-      for (int e = tid; e < kChunk * w; e += NT) store_a1(e, V[(size_t)y0 * w + e]);
+      for (int e2 = tid; e2 < kChunk * w / 2; e2 += NT) store_a1(e2, V4[(size_t)y0 * (w / 2) + e2]);
     }
     __syncthreads();
     // stage 1: tiles (rt in 0..1) x (ct in 0..Gp/16)
@@ -231,21 +255,17 @@ __global__ __launch_bounds__(64 * NW) void k_overlap_mfma(Geometry g, const floa
           d = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.w, b4.w, d, 0, 0, 0);
         }
       }
-      // D[r][m], r = rt*16 + 4q + i = (yy, c)  ->  A2[(m, c)][yy]
+      // D[yy][m] of part c = rt, yy = 4q + i  ->  A2[(m, c)][4q .. 4q + 3], one write
       const int m = ct * 16 + lr;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int r = rt * 16 + 4 * q + i;
-        const int yy = r >> 1, c = r & 1;
-        const int row2 = 2 * m + c;
-        if (SPLIT) {
-          _Float16 hi, lo;
-          split16(d[i], hi, lo);
-          A2[row2 * pk + yy] = hi;
-          A2[2 * Gp * pk + row2 * pk + yy] = lo;
-        } else {
-          A2[row2 * pk + yy] = d[i];
-        }
+      const int row2 = 2 * m + rt;
+      if (SPLIT) {
+        uint2 th, tl;
+        split_pair(d[0], d[1], th.x, tl.x);
+        split_pair(d[2], d[3], th.y, tl.y);
+        *reinterpret_cast<uint2*>(A2 + row2 * pk + 4 * q) = th;
+        *reinterpret_cast<uint2*>(A2 + 2 * Gp * pk + row2 * pk + 4 * q) = tl;
+      } else {
+        *reinterpret_cast<float4*>(A2 + row2 * pk + 4 * q) = make_float4(d[0], d[1], d[2], d[3]);
       }
     }
     __syncthreads();
@@ -258,20 +278,14 @@ __global__ __launch_bounds__(64 * NW) void k_overlap_mfma(Geometry g, const floa
         const E* arow = A2 + (mt * 16 + lr) * pk;
         const E* brow = By + (nt * 16 + lr) * ph + y0;
         if (SPLIT) {
-          // K = 16: lane groups 0,1 carry k 0..15, groups 2,3 are zero
-          half8 ah, al, bh, bl;
-          if (q < 2) {
-            ah = *reinterpret_cast<const half8*>(arow + 8 * q);
-            al = *reinterpret_cast<const half8*>(arow + 2 * Gp * pk + 8 * q);
-            bh = *reinterpret_cast<const half8*>(brow + 8 * q);
-            bl = *reinterpret_cast<const half8*>(brow + Gp * ph + 8 * q);
-          } else {
-#pragma unroll
-            for (int j = 0; j < 8; ++j) ah[j] = al[j] = bh[j] = bl[j] = (_Float16)0.f;
-          }
-          acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc[i], 0, 0, 0);
-          acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc[i], 0, 0, 0);
-          acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc[i], 0, 0, 0);
+          // K = 16 = the chunk's rows: lane group q carries k-slots 4q .. 4q + 3
+          const half4 ah = *reinterpret_cast<const half4*>(arow + 4 * q);
+          const half4 al = *reinterpret_cast<const half4*>(arow + 2 * Gp * pk + 4 * q);
+          const half4 bh = *reinterpret_cast<const half4*>(brow + 4 * q);
+          const half4 bl = *reinterpret_cast<const half4*>(brow + Gp * ph + 4 * q);
+          acc[i] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bh, acc[i], 0, 0, 0);
+          acc[i] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, bl, acc[i], 0, 0, 0);
+          acc[i] = __builtin_amdgcn_mfma_f32_16x16x16f16(al, bh, acc[i], 0, 0, 0);
         } else {
           // lane group q takes the k-slots 4q .. 4q + 3 of the chunk
           const float4 a4 = *reinterpret_cast<const float4*>(reinterpret_cast<const float*>(arow) + 4 * q);
@@ -312,7 +326,7 @@ static size_t overlap_lds(const Geometry& g, bool split) {
   const size_t e = split ? 2 : 4;
   const int nL = split ? 2 : 1;
   return e * nL * ((size_t)Gp * pitch_of(g.w) + (size_t)Gp * pitch_of(g.h) + 32 * (size_t)pitch_of(g.w) +
-                   2 * (size_t)Gp * pitch_of(kChunk));
+                   2 * (size_t)Gp * kPitch2);
 }
 
 // waves per workgroup (8 measured faster than 4 on both the 80-point and the
