@@ -296,6 +296,22 @@ struct Handle {
   int calPlaneL = 0;
   PinnedBuf<float2> hCal;
 
+  // multi-batch launch (ProcessBatchMultiDevice): while `collect` is set, a
+  // batch that takes the size-specialised window kernel (HG basis) records
+  // that launch -- and the separate overlap that follows it -- instead of
+  // issuing them, for one launch of all the handles' window kernels
+  bool collect = false;
+  bool preferV3 = false;   // the specialised window kernel wherever it applies, whatever the tuned choice
+  bool pendValid = false, pendOverlap = false, pendSplit = false;
+  dh::W3Launch pend{};
+  dh::Geometry pendGF{};
+  // afterMulti: recorded on another handle's stream after a multi-batch launch
+  // that wrote this handle's outputs; the next API call on this handle makes
+  // its stream wait on it (after_multi).  fence: marks this stream's own work
+  // for a multi-batch launch on another stream.
+  hipEvent_t afterMulti = nullptr;
+  hipEvent_t fence = nullptr;
+
   bool stageTiming = false;
   std::vector<std::array<hipEvent_t, 6>> evPool;
   size_t evUsed = 0;
@@ -305,11 +321,29 @@ struct Handle {
 
 std::vector<Handle*> g_handles;     // index = handle; nullptr once destroyed
 std::mutex g_registry;
+std::mutex g_multiMu;               // multi-batch launch state (MultiGraph, Handle::afterMulti)
 
-Handle* get(int idx) {
+// Orders the handle's stream after the last multi-batch launch that wrote its
+// outputs from another stream.
+void after_multi(Handle& h) {
+  std::lock_guard<std::mutex> lk(g_multiMu);
+  if (h.afterMulti) (void)hipStreamWaitEvent(h.stream, h.afterMulti, 0);
+  h.afterMulti = nullptr;
+}
+
+// The handle without the ordering that get() applies.
+Handle* get_raw(int idx) {
   std::lock_guard<std::mutex> lk(g_registry);
   if (idx < 0 || idx >= (int)g_handles.size()) return nullptr;
   return g_handles[idx];
+}
+
+// The handle behind an API call: any work the call queues on its stream
+// follows the multi-batch launches that wrote its outputs.
+Handle* get(int idx) {
+  Handle* h = get_raw(idx);
+  if (h && h->afterMulti) after_multi(*h);
+  return h;
 }
 
 void log_err(const Handle*, const char* what, hipError_t e) {
@@ -1290,7 +1324,9 @@ int enqueue_batch(Handle& h, int step = 0) {
   const unsigned* peak = nullptr;   // field peaks left by the staged kernels for the split overlap
   int path = 0;
   if (A > 1) path = 0;   // averaging runs on the staged kernels
-  else if (h.windowMode >= 3 && h.frameU16 && dh::window_v3_table_elems(g) && plan_window_v3(h, g, fa)) path = 3;
+  else if ((h.windowMode >= 3 || (h.preferV3 && !h.tuning)) && h.frameU16 && dh::window_v3_table_elems(g) &&
+           plan_window_v3(h, g, fa))
+    path = 3;
   else if (cal.p) path = 0;   // only the specialised and the staged kernels apply a calibration
   else if (h.windowMode >= 2 && h.frameU16 && plan_window_mfma(h, g, fa)) path = 2;
   else if (h.windowMode >= 1 && plan_fused(h, g, fa)) path = 1;
@@ -1301,13 +1337,28 @@ int enqueue_batch(Handle& h, int step = 0) {
     const bool fuseOv = path == 3 && (h.fusedOverlap > 0 || (h.fusedOverlap < 0 && split)) &&
                         dh::window_v3_overlap_fits(g);
     h.lastFusedOverlap = fuseOv;
-    if (path == 3)
-      DH_TRY(dh::launch_window_v3(fa, dframes, h.w3Tab.p, h.factor.p, h.ref.p, h.fields.p,
-                                  fuseOv && split ? h.hgx16.p : nullptr, fuseOv && split ? h.hgy16.p : nullptr,
-                                  fuseOv ? h.coefs.p : nullptr, cal, h.stream, fuseOv && !split ? h.hgx.p : nullptr,
-                                  fuseOv && !split ? h.hgy.p : nullptr),
-             "fused window (register rows)");
-    else if (path == 2)
+    if (path == 3) {
+      const dh::W3Launch wl{fa,
+                            dframes,
+                            h.w3Tab.p,
+                            h.factor.p,
+                            h.ref.p,
+                            h.fields.p,
+                            fuseOv && split ? h.hgx16.p : nullptr,
+                            fuseOv && split ? h.hgy16.p : nullptr,
+                            fuseOv ? h.coefs.p : nullptr,
+                            cal,
+                            fuseOv && !split ? h.hgx.p : nullptr,
+                            fuseOv && !split ? h.hgy.p : nullptr};
+      if (h.collect && c.basisType == DIGHOLO_BASISTYPE_HG && dh::window_v3_multi_supported(wl)) {
+        h.pend = wl;
+        h.pendValid = true;
+      } else {
+        DH_TRY(dh::launch_window_v3(wl.fa, wl.frames, wl.tab, wl.factor, wl.ref, wl.fields, wl.hgx16, wl.hgy16,
+                                    wl.coefs, wl.cal, h.stream, wl.hgx32, wl.hgy32),
+               "fused window (register rows)");
+      }
+    } else if (path == 2)
       DH_TRY(dh::launch_window_mfma(fa, h.twW.p, dframes, h.colTab.p, h.factor.p, h.ref.p, h.fields.p, h.stream),
              "fused window (MFMA columns)");
     else
@@ -1368,9 +1419,15 @@ int enqueue_batch(Handle& h, int step = 0) {
     h.lastFusedOverlap = false;
   }
   if (g.G > 0 && !h.lastFusedOverlap) {
-    DH_TRY(dh::launch_overlap_mfma(gF, h.fields.p, h.hgx.p, h.hgy.p, h.coefs.p,
-                                   c.precision == DIGHOLO_PRECISION_SPLIT16, h.stream, h.ovTpw, peak),
-           "overlap");
+    if (h.pendValid) {   // follows the collected window kernel
+      h.pendOverlap = true;
+      h.pendGF = gF;
+      h.pendSplit = c.precision == DIGHOLO_PRECISION_SPLIT16;
+    } else {
+      DH_TRY(dh::launch_overlap_mfma(gF, h.fields.p, h.hgx.p, h.hgy.p, h.coefs.p,
+                                     c.precision == DIGHOLO_PRECISION_SPLIT16, h.stream, h.ovTpw, peak),
+             "overlap");
+    }
   }
   if (timing) DH_TRY(hipEventRecord(ev[5], h.stream), "event");
 
@@ -2926,55 +2983,285 @@ int autoalign_gradient_search(Handle& h, int maxIter, double mdlWeight, float& g
 }
 
 // ---------------------------------------------------------------------------
-// Several handles' batches in one graph launch (no counterpart in the
-// reference).  Each handle's launch sequence is captured on its own stream as
-// a branch forked from and joined to the first handle's stream, so batches of
-// different shapes run concurrently; the executable graph is kept per handle
-// list and replayed while every handle's launch signature (window geometry,
-// frames, precision, chosen kernels, tables) is unchanged.
+// Several handles' batches in one launch sequence (no counterpart in the
+// reference).  The first call for a handle list captures it into a graph:
+// each handle's batch is prepared on its own stream, a branch forked from and
+// joined to the first handle's stream; batches on the size-specialised window
+// kernel (HG basis) only record that launch, and the recorded launches are
+// issued on the first handle's stream after the join -- as one launch of the
+// multi-batch window kernel while the batches are small, else one after the
+// other -- followed by the separate overlaps of those batches.  When the graph
+// holds nothing but these launches, later calls issue them directly (a graph
+// launch costs more host time than a few kernel launches); otherwise they
+// replay the graph.  Either is kept per handle list while every handle's
+// configuration and launch state are unchanged.
 // ---------------------------------------------------------------------------
+
+// Every Config field, compared bitwise; the batch-calibration, custom-basis
+// and reference-calibration arrays by their version counters.  The wavelength
+// list is compared by value.
+#define DH_CONFIG_SCALARS(X)                                                                                      \
+  X(frameWidth) X(frameHeight) X(pixelSize) X(apertureSize) X(resolutionIdx) X(waist) X(tiltX) X(tiltY) X(defocus) \
+  X(beamCX) X(beamCY) X(batchCount) X(avgCount) X(avgMode) X(wavelengthCentre) X(wavelengthOrdering)              \
+  X(polLockTilt) X(polLockWaist) X(polLockDefocus) X(polCount) X(aaTilt) X(aaDefocus) X(aaCentre) X(aaWaist)      \
+  X(aaFourierRadius) X(aaPolIndependence) X(aaMulConjTrans) X(aaTol) X(aaMode) X(aaGoalIdx) X(fillFactor)         \
+  X(fftWindowSizeX) X(fftWindowSizeY) X(maxMG) X(basisType) X(verbosity) X(threadCount) X(precision) X(calP)       \
+  X(calB) X(calEnabled) X(calVersion) X(basisIn) X(basisOut) X(basisVersion) X(refCalW) X(refCalH) X(refCalL)     \
+  X(refCalEnabled) X(refCalVersion)
+#define DH_FIELD_BYTES(f) +sizeof(Config::f)
+// a field added to Config has to join DH_CONFIG_SCALARS (or be compared by a
+// version counter like the three arrays)
+constexpr size_t kConfigPad = 12;   // alignment padding ahead of the vector members (x86-64)
+static_assert(sizeof(Config) == (0 DH_CONFIG_SCALARS(DH_FIELD_BYTES)) + 4 * sizeof(std::vector<float>) + kConfigPad,
+              "Config changed: update DH_CONFIG_SCALARS");
+
+bool same_config(const Config& a, const Config& b) {
+#define DH_FIELD_EQ(f) if (std::memcmp(&a.f, &b.f, sizeof(a.f)) != 0) return false;
+  DH_CONFIG_SCALARS(DH_FIELD_EQ)
+#undef DH_FIELD_EQ
+  return a.wavelengths == b.wavelengths;
+}
+
+// what a captured launch sequence depends on besides the configuration
+struct LaunchState {
+  const void* frames;
+  int frameU16, transpose, windowMode, fusedOverlap, ovTpw, autoTune;
+  bool holdCrop, holdWin;
+  unsigned tabVersion;
+  hipStream_t stream;
+  bool operator==(const LaunchState& o) const {
+    return frames == o.frames && frameU16 == o.frameU16 && transpose == o.transpose && windowMode == o.windowMode &&
+           fusedOverlap == o.fusedOverlap && ovTpw == o.ovTpw && autoTune == o.autoTune && holdCrop == o.holdCrop &&
+           holdWin == o.holdWin && tabVersion == o.tabVersion && stream == o.stream;
+  }
+};
+LaunchState launch_state(const Handle& h) {
+  return LaunchState{h.frames,     h.frameU16, h.transpose, h.windowMode, h.fusedOverlap, h.ovTpw,
+                     h.autoTune,   h.holdCrop, h.holdWin,   h.tabVersion, h.stream};
+}
+
+struct MultiOp {
+  int kind = 0;   // 0: multi-batch window kernel over w3[i0, i0 + n); 1: window kernel w3[i0]; 2: overlap
+  int i0 = 0, n = 0;
+  dh::Geometry g{};   // overlap operands
+  const float2* fields = nullptr;
+  const float *hgx = nullptr, *hgy = nullptr;
+  float2* coefs = nullptr;
+  int split = 0, tpw = 0;
+};
+
 struct MultiGraph {
   std::vector<Handle*> hs;
-  std::vector<std::string> sigs;
+  std::vector<Config> cfgs;   // without the three versioned arrays
+  std::vector<LaunchState> states;
+  std::vector<dh::W3Launch> w3;
+  std::vector<MultiOp> ops;
+  bool direct = false;        // the graph is exactly `ops`: issue them without it
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t stream = nullptr;
-  hipEvent_t done = nullptr;   // recorded after each launch; the other handles' streams wait on it
+  hipEvent_t done = nullptr;  // recorded after each launch; the other handles' streams wait on it
 };
 std::map<std::vector<int>, MultiGraph> g_multi;
-std::mutex g_multiMu;
 
+// Called with g_multiMu held: the other handles' streams take their pending
+// wait on m.done before the event goes.
 void multi_release(MultiGraph& m) {
+  for (Handle* h : m.hs)
+    if (m.done && h->afterMulti == m.done) {
+      (void)hipStreamWaitEvent(h->stream, m.done, 0);
+      h->afterMulti = nullptr;
+    }
   if (m.exec) (void)hipGraphExecDestroy(m.exec);
   if (m.graph) (void)hipGraphDestroy(m.graph);
   if (m.done) (void)hipEventDestroy(m.done);
   m.exec = nullptr;
   m.graph = nullptr;
   m.done = nullptr;
+  m.direct = false;
+  m.w3.clear();
+  m.ops.clear();
 }
 
-// launch signature of the handle's next batch, from host state only
-bool launch_signature(Handle& h, std::string& sig) {
-  Config c = h.cfg;
-  if (c.batchCount < 1) c.batchCount = 1;
-  if (c.polLockTilt) {
-    c.tiltX[1] = c.tiltX[0];
-    c.tiltY[1] = c.tiltY[0];
+Config config_sans_arrays(const Config& c) {
+  Config k;
+#define DH_FIELD_CP(f) std::memcpy(&k.f, &c.f, sizeof(c.f));
+  DH_CONFIG_SCALARS(DH_FIELD_CP)
+#undef DH_FIELD_CP
+  k.wavelengths = c.wavelengths;
+  return k;
+}
+
+hipError_t run_multi_ops(const MultiGraph& m, hipStream_t s) {
+  for (const MultiOp& o : m.ops) {
+    hipError_t e = hipSuccess;
+    if (o.kind == 0) {
+      e = dh::launch_window_v3_multi(m.w3.data() + o.i0, o.n, s);
+    } else if (o.kind == 1) {
+      const dh::W3Launch& w = m.w3[o.i0];
+      e = dh::launch_window_v3(w.fa, w.frames, w.tab, w.factor, w.ref, w.fields, w.hgx16, w.hgy16, w.coefs, w.cal, s,
+                               w.hgx32, w.hgy32);
+    } else {
+      e = dh::launch_overlap_mfma(o.g, o.fields, o.hgx, o.hgy, o.coefs, o.split, s, o.tpw, nullptr);
+    }
+    if (e != hipSuccess) return e;
   }
-  if (c.polLockDefocus) c.defocus[1] = c.defocus[0];
-  if (c.polLockWaist) c.waist[1] = c.waist[0];
-  Derived d;
-  int err;
-  if (!derive(c, c.batchCount, d, err)) return false;
-  std::string s((const char*)&d.g, sizeof(d.g));
-  for (const dh::TableArgs& t : d.tabLam) s.append((const char*)t.pp, sizeof(t.pp));
-  s.append((const char*)d.mask.data(), d.mask.size());
-  char buf[256];
-  std::snprintf(buf, sizeof(buf), "|%p|%d|%d|%d|%d|%d|%d|%d|%u|%u|%d|%d|%p", h.frames, h.frameU16, h.transpose,
-                c.precision, h.windowMode, h.fusedOverlap, h.ovTpw, c.avgCount * 8 + c.avgMode,
-                c.calEnabled ? c.calVersion : 0u, h.tabVersion, h.autoTune, c.calEnabled, (void*)h.stream);
-  sig = s + buf;
-  return true;
+  return hipSuccess;
+}
+
+// Batches whose window kernels together hold at most this many (pol, frame)
+// work items share one launch of the multi-batch window kernel: below a few
+// waves of workgroups over the chip, separate launches leave it idle in their
+// tails.  Larger batches keep their own kernels, which run at the occupancy of
+// their own shape rather than that of the largest one in the launch.
+#ifndef DH_MULTI_MERGE_ITEMS
+#define DH_MULTI_MERGE_ITEMS 1024
+#endif
+constexpr long kMultiMergeItems = DH_MULTI_MERGE_ITEMS;
+
+// Called with g_multiMu held.
+int capture_multi(MultiGraph& m, const std::vector<Handle*>& hs) {
+  Handle& h0 = *hs[0];
+  // small batches take the specialised window kernel wherever it applies (a
+  // tuned choice made for the batch on its own does not hold inside one launch)
+  long cfgItems = 0;
+  for (Handle* h : hs) cfgItems += (long)std::max(1, h->cfg.polCount) * std::max(1, h->cfg.batchCount);
+  struct Prefer {
+    const std::vector<Handle*>& hs;
+    ~Prefer() {
+      for (Handle* h : hs) h->preferV3 = false;
+    }
+  } prefer{hs};
+  for (Handle* h : hs) h->preferV3 = cfgItems <= kMultiMergeItems;
+  // one eager batch per handle: tables, twiddles, buffers and tuning settle
+  // outside the capture, then the launch sequence is captured
+  for (Handle* h : hs)
+    if (int e = enqueue_batch(*h)) return e;
+  for (Handle* h : hs) {
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) {
+      log_err(h, "synchronize", e);
+      return DIGHOLO_ERROR_ERROR;
+    }
+  }
+  hipStream_t s0 = h0.stream;
+  std::vector<hipEvent_t> evs(hs.size(), nullptr);
+  for (auto& e : evs)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) e = nullptr;
+  auto cleanup = [&]() {
+    for (auto& e : evs)
+      if (e) (void)hipEventDestroy(e);
+    for (Handle* h : hs) h->capturing = h->collect = h->pendValid = h->pendOverlap = false;
+  };
+  for (auto e : evs)
+    if (!e) {
+      cleanup();
+      return DIGHOLO_ERROR_ERROR;
+    }
+  if (hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    cleanup();
+    return DIGHOLO_ERROR_ERROR;
+  }
+  int rc = DIGHOLO_ERROR_SUCCESS;
+  (void)hipEventRecord(evs[0], s0);
+  for (size_t i = 1; i < hs.size(); ++i)
+    if (hs[i]->stream != s0) (void)hipStreamWaitEvent(hs[i]->stream, evs[0], 0);
+  long items = 0;
+  for (size_t i = 0; i < hs.size() && !rc; ++i) {
+    hs[i]->capturing = true;
+    hs[i]->collect = true;
+    hs[i]->pendValid = hs[i]->pendOverlap = false;
+    rc = enqueue_batch(*hs[i]);
+    if (hs[i]->pendValid) items += (long)hs[i]->pend.fa.g.P * hs[i]->pend.fa.g.B;
+  }
+  for (size_t i = 1; i < hs.size(); ++i)
+    if (hs[i]->stream != s0) {
+      (void)hipEventRecord(evs[i], hs[i]->stream);
+      (void)hipStreamWaitEvent(s0, evs[i], 0);
+    }
+  // the recorded window kernels, then the separate overlaps that follow some of them
+  m.w3.clear();
+  m.ops.clear();
+  for (Handle* h : hs)
+    if (h->pendValid) m.w3.push_back(h->pend);
+  if (items <= kMultiMergeItems) {
+    for (size_t i0 = 0; i0 < m.w3.size(); i0 += dh::kW3MaxBatches) {
+      MultiOp o;
+      o.kind = 0;
+      o.i0 = (int)i0;
+      o.n = (int)std::min<size_t>(dh::kW3MaxBatches, m.w3.size() - i0);
+      m.ops.push_back(o);
+    }
+  } else {
+    for (size_t i = 0; i < m.w3.size(); ++i) {
+      MultiOp o;
+      o.kind = 1;
+      o.i0 = (int)i;
+      m.ops.push_back(o);
+    }
+  }
+  for (Handle* h : hs)
+    if (h->pendValid && h->pendOverlap) {
+      MultiOp o;
+      o.kind = 2;
+      o.g = h->pendGF;
+      o.fields = h->fields.p;
+      o.hgx = h->hgx.p;
+      o.hgy = h->hgy.p;
+      o.coefs = h->coefs.p;
+      o.split = h->pendSplit ? 1 : 0;
+      o.tpw = h->ovTpw;
+      m.ops.push_back(o);
+    }
+  if (!rc) {
+    const hipError_t e = run_multi_ops(m, s0);
+    if (e != hipSuccess) {
+      log_err(&h0, "multi-batch window kernels", e);
+      rc = DIGHOLO_ERROR_ERROR;
+    }
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(s0, &graph);
+  cleanup();
+  if (rc || ec != hipSuccess || !graph) {
+    if (graph) (void)hipGraphDestroy(graph);
+    fprintf(stderr, "digHolo: graph capture failed (%s)\n", hipGetErrorString(ec));
+    return rc ? rc : DIGHOLO_ERROR_ERROR;
+  }
+  size_t nodes = 0;
+  if (hipGraphGetNodes(graph, nullptr, &nodes) == hipSuccess && nodes == m.ops.size()) {
+    std::vector<hipGraphNode_t> list(nodes);
+    bool kernels = hipGraphGetNodes(graph, list.data(), &nodes) == hipSuccess;
+    for (size_t i = 0; i < nodes && kernels; ++i) {
+      hipGraphNodeType t;
+      kernels = hipGraphNodeGetType(list[i], &t) == hipSuccess && t == hipGraphNodeTypeKernel;
+    }
+    m.direct = kernels;
+  }
+  if (m.direct) {
+    (void)hipGraphDestroy(graph);
+  } else {
+    hipGraphExec_t exec = nullptr;
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return DIGHOLO_ERROR_ERROR;
+    }
+    m.graph = graph;
+    m.exec = exec;
+  }
+  m.stream = s0;
+  if (hipEventCreateWithFlags(&m.done, hipEventDisableTiming) != hipSuccess) {
+    multi_release(m);
+    return DIGHOLO_ERROR_ERROR;
+  }
+  m.hs = hs;
+  m.cfgs.clear();
+  m.states.clear();
+  for (Handle* h : hs) {
+    m.cfgs.push_back(config_sans_arrays(h->cfg));
+    m.states.push_back(launch_state(*h));
+  }
+  return DIGHOLO_ERROR_SUCCESS;
 }
 
 int process_multi(const std::vector<Handle*>& hs, const std::vector<int>& ids) {
@@ -2988,82 +3275,58 @@ int process_multi(const std::vector<Handle*>& hs, const std::vector<int>& ids) {
   }
   std::lock_guard<std::mutex> lk(g_multiMu);
   MultiGraph& m = g_multi[ids];
-  std::vector<std::string> sigs(hs.size());
-  bool same = m.exec != nullptr && m.stream == hs[0]->stream && m.sigs.size() == hs.size();
-  for (size_t i = 0; i < hs.size(); ++i) {
-    if (!launch_signature(*hs[i], sigs[i])) return DIGHOLO_ERROR_INVALIDDIMENSION;
-    if (same && sigs[i] != m.sigs[i]) same = false;
-  }
   Handle& h0 = *hs[0];
+  hipStream_t s0 = h0.stream;
+  bool same = (m.exec || m.direct) && m.stream == s0 && m.hs == hs;
+  for (size_t i = 0; i < hs.size() && same; ++i)
+    same = same_config(hs[i]->cfg, m.cfgs[i]) && launch_state(*hs[i]) == m.states[i];
   if (!same) {
     multi_release(m);
-    // one eager batch per handle: tables, twiddles, buffers and tuning settle
-    // outside the capture, then the launch sequence is captured
+    // the eager batches run on every handle's stream: each first follows the
+    // multi-batch launches that wrote its outputs
     for (Handle* h : hs)
-      if (int e = enqueue_batch(*h)) return e;
-    for (Handle* h : hs) {
-      hipError_t e = hipStreamSynchronize(h->stream);
-      if (e != hipSuccess) {
-        log_err(h, "synchronize", e);
+      if (h->afterMulti) {
+        (void)hipStreamWaitEvent(h->stream, h->afterMulti, 0);
+        h->afterMulti = nullptr;
+      }
+    if (int e = capture_multi(m, hs)) {
+      multi_release(m);
+      return e;
+    }
+  } else {
+    // s0 follows the work queued on the other handles' streams since their
+    // last multi-batch launch; one whose stream still waits on this list's
+    // last launch has had none (every API call on it would have taken that wait)
+    if (h0.afterMulti) {
+      (void)hipStreamWaitEvent(s0, h0.afterMulti, 0);
+      h0.afterMulti = nullptr;
+    }
+    for (size_t i = 1; i < hs.size(); ++i) {
+      Handle& h = *hs[i];
+      if (h.stream == s0 || (m.done && h.afterMulti == m.done)) continue;
+      if (h.afterMulti) {
+        (void)hipStreamWaitEvent(h.stream, h.afterMulti, 0);
+        h.afterMulti = nullptr;
+      }
+      if (!h.fence && hipEventCreateWithFlags(&h.fence, hipEventDisableTiming) != hipSuccess) {
+        h.fence = nullptr;
         return DIGHOLO_ERROR_ERROR;
       }
+      if (hipEventRecord(h.fence, h.stream) != hipSuccess || hipStreamWaitEvent(s0, h.fence, 0) != hipSuccess)
+        return DIGHOLO_ERROR_ERROR;
     }
-    for (size_t i = 0; i < hs.size(); ++i)
-      if (!launch_signature(*hs[i], sigs[i])) return DIGHOLO_ERROR_INVALIDDIMENSION;
-    hipStream_t s0 = h0.stream;
-    std::vector<hipEvent_t> evs(hs.size() + 1, nullptr);
-    for (auto& e : evs)
-      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return DIGHOLO_ERROR_ERROR;
-    auto cleanup = [&]() {
-      for (auto& e : evs)
-        if (e) (void)hipEventDestroy(e);
-      for (Handle* h : hs) h->capturing = false;
-    };
-    int rc = DIGHOLO_ERROR_SUCCESS;
-    if (hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-      cleanup();
-      return DIGHOLO_ERROR_ERROR;
-    }
-    (void)hipEventRecord(evs[0], s0);
-    for (size_t i = 1; i < hs.size(); ++i)
-      if (hs[i]->stream != s0) (void)hipStreamWaitEvent(hs[i]->stream, evs[0], 0);
-    for (size_t i = 0; i < hs.size() && !rc; ++i) {
-      hs[i]->capturing = true;
-      rc = enqueue_batch(*hs[i]);
-      hs[i]->capturing = false;
-    }
-    for (size_t i = 1; i < hs.size(); ++i)
-      if (hs[i]->stream != s0) {
-        (void)hipEventRecord(evs[i], hs[i]->stream);
-        (void)hipStreamWaitEvent(s0, evs[i], 0);
-      }
-    hipGraph_t graph = nullptr;
-    const hipError_t ec = hipStreamEndCapture(s0, &graph);
-    cleanup();
-    if (rc || ec != hipSuccess || !graph) {
-      if (graph) (void)hipGraphDestroy(graph);
-      fprintf(stderr, "digHolo: graph capture failed (%s)\n", hipGetErrorString(ec));
-      return rc ? rc : DIGHOLO_ERROR_ERROR;
-    }
-    hipGraphExec_t exec = nullptr;
-    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
-      (void)hipGraphDestroy(graph);
-      return DIGHOLO_ERROR_ERROR;
-    }
-    m.graph = graph;
-    m.exec = exec;
-    m.stream = s0;
-    if (hipEventCreateWithFlags(&m.done, hipEventDisableTiming) != hipSuccess) m.done = nullptr;
-    m.sigs = sigs;
-    m.hs = hs;
   }
-  if (hipGraphLaunch(m.exec, h0.stream) != hipSuccess) return DIGHOLO_ERROR_ERROR;
-  // later work on any handle's own stream (result downloads, the next batch)
-  // follows the replay
-  if (m.done && hipEventRecord(m.done, h0.stream) == hipSuccess)
-    for (Handle* h : hs)
-      if (h->stream != h0.stream) (void)hipStreamWaitEvent(h->stream, m.done, 0);
-  for (Handle* h : hs) {   // every handle's outputs are now those of the replay on h0's stream
+  const hipError_t e = m.direct ? run_multi_ops(m, s0) : hipGraphLaunch(m.exec, s0);
+  if (e != hipSuccess) {
+    log_err(&h0, "multi-batch launch", e);
+    return DIGHOLO_ERROR_ERROR;
+  }
+  // later work on any other handle's own stream (result downloads, the next
+  // batch) follows this launch: the handle's next API call makes its stream wait
+  if (hipEventRecord(m.done, s0) != hipSuccess) return DIGHOLO_ERROR_ERROR;
+  for (size_t i = 1; i < hs.size(); ++i)
+    if (hs[i]->stream != s0) hs[i]->afterMulti = m.done;
+  for (Handle* h : hs) {   // every handle's outputs are now those of this launch on h0's stream
     h->coefsHostValid = h->fieldsHostValid = false;
     h->analysisValid[0] = h->analysisValid[1] = false;
     h->f16Valid = false;
@@ -3084,6 +3347,9 @@ void destroy_handle(Handle* h) {
         ++it;
       }
     }
+    // the stream synchronisation below then covers a launch from another stream
+    if (h->afterMulti) (void)hipStreamWaitEvent(h->stream, h->afterMulti, 0);
+    h->afterMulti = nullptr;
   }
   if (h->devInit) {
     (void)hipStreamSynchronize(h->stream);
@@ -3124,6 +3390,7 @@ void destroy_handle(Handle* h) {
     h->hF16.release();
     for (auto& set : h->evPool)
       for (auto& e : set) (void)hipEventDestroy(e);
+    if (h->fence) (void)hipEventDestroy(h->fence);
     if (h->ownStream) (void)hipStreamDestroy(h->ownStream);
   }
   delete h;
@@ -4117,7 +4384,7 @@ EXT_C int digHoloProcessBatchMultiDevice(int count, const int* handleIdx) {
   std::vector<Handle*> hs;
   std::vector<int> ids(handleIdx, handleIdx + count);
   for (int i = 0; i < count; ++i) {
-    Handle* h = get(ids[i]);
+    Handle* h = get_raw(ids[i]);   // process_multi orders the streams itself
     if (!h) return DIGHOLO_ERROR_INVALIDHANDLE;
     if (std::find(hs.begin(), hs.end(), h) != hs.end()) return DIGHOLO_ERROR_INVALIDARGUMENT;
     hs.push_back(h);

@@ -980,8 +980,7 @@ __global__ void k_hg_split(const float* __restrict__ hg, int P, int G, int n, _F
 // land on the same XCD and the camera rows their windows share are read from
 // HBM once and then from that XCD's L2.  The frames past the last full group
 // of eight are mapped plainly.
-__device__ __forceinline__ void w3_item(int P, int B, int& p, int& b) {
-  const int id = blockIdx.x;
+__device__ __forceinline__ void w3_item(int P, int B, int id, int& p, int& b) {
   if (P != 2) {
     p = id % P;
     b = id / P;
@@ -999,13 +998,12 @@ __device__ __forceinline__ void w3_item(int P, int B, int& p, int& b) {
 }
 
 // SQ: a square FFT window (H = W = 2 L M), so the column transform's length
-// and its table offsets are compile-time constants.
+// and its table offsets are compile-time constants.  item: the (pol, frame)
+// work item of the batch this workgroup processes (w3_item).
 template <int L, int M, int U, int J, bool SQ>
-__global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const unsigned short* __restrict__ frames,
-                                                          const float2* __restrict__ tabs,
-                                                          const float2* __restrict__ factor,
-                                                          const float2* __restrict__ ref,
-                                                          float2* __restrict__ fields) {
+__device__ __forceinline__ void w3_body(const W3Args& a, const unsigned short* __restrict__ frames,
+                                        const float2* __restrict__ tabs, const float2* __restrict__ factor,
+                                        const float2* __restrict__ ref, float2* __restrict__ fields, int item) {
   using Sh = W3Shape<L, M, U, J>;
   constexpr int NT = kW3Threads, RPW = Sh::RPW, RR = Sh::RR, KS = Sh::KS, S = Sh::S, w = Sh::w, h = Sh::h;
   constexpr int NU = Sh::NU, LP = Sh::LP, RPc = Sh::RPc, PK = Sh::PK, NR = Sh::NR, WP = Sh::WP;
@@ -1013,7 +1011,7 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
   const Geometry& g = a.g;
   const int H = SQ ? 2 * NR : g.H;
   int p, b;
-  w3_item(a.g.P, a.g.B, p, b);
+  w3_item(a.g.P, a.g.B, item, p, b);
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wv = (wave + b) & 3;              // units rotate with the frame so every SIMD gets its share
@@ -1441,6 +1439,33 @@ __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const uns
   }
 }
 
+template <int L, int M, int U, int J, bool SQ>
+__global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const unsigned short* __restrict__ frames,
+                                                          const float2* __restrict__ tabs,
+                                                          const float2* __restrict__ factor,
+                                                          const float2* __restrict__ ref,
+                                                          float2* __restrict__ fields) {
+  w3_body<L, M, U, J, SQ>(a, frames, tabs, factor, ref, fields, blockIdx.x);
+}
+
+// One handle's batch in a multi-batch launch: its arguments, its kernel
+// shape (index into W3_SHAPES) and its P B work items.
+struct W3Item {
+  W3Args a;
+  const unsigned short* frames;
+  const float2* tabs;
+  const float2* factor;
+  const float2* ref;
+  float2* fields;
+  int shape;
+  int items;
+};
+struct W3Multi {
+  W3Item it[kW3MaxBatches];
+  int n;       // batches
+  int total;   // work items of all batches
+};
+
 // Tables of the v3 kernel, from FP64: [H] cos and sin of 2 pi m / H as fp16
 // hi/lo pairs {(cos_hi, sin_hi), (cos_lo, sin_lo)}; [L][M + 1]
 // exp(-2 pi i l a / NR); [M] exp(-2 pi i a / W).
@@ -1514,16 +1539,23 @@ static int w3_ovl_bytes(const Geometry& g) {
   return 2 * (2 * Gp * kOvPK32 * 4);   // the FP32 form's double-buffered stage-1 output
 }
 
+// LDS layout of a batch's launch (region, overlap buffers); returns the bytes
+template <int L, int M, int U, int J>
+static size_t w3_layout(W3Args& a) {
+  using Sh = W3Shape<L, M, U, J>;
+  a.region = w3_region(Sh::EBYTES, Sh::BBYTES, a.g.H, L, M, Sh::w, Sh::h);
+  a.ovlOff = 8 * Sh::h * Sh::WP;
+  if (a.hgx32) a.region = std::max(a.region, a.ovlOff + w3_ovl_bytes(a.g));   // the FP32 form's stage-1 buffers
+  return w3_lds<L, M, U, J>(a);
+}
+
 template <int L, int M, int U, int J>
 static hipError_t w3_launch(W3Args a, const void* frames, const float2* tab, const float2* factor,
                             const float2* ref, float2* fields, hipStream_t s) {
   using Sh = W3Shape<L, M, U, J>;
   const Geometry& g = a.g;
   if (g.H % Sh::RR) return hipErrorNotSupported;
-  a.region = w3_region(Sh::EBYTES, Sh::BBYTES, g.H, L, M, Sh::w, Sh::h);
-  a.ovlOff = 8 * Sh::h * Sh::WP;
-  if (a.hgx32) a.region = std::max(a.region, a.ovlOff + w3_ovl_bytes(g));   // the FP32 form's stage-1 buffers
-  const size_t lds = w3_lds<L, M, U, J>(a);
+  const size_t lds = w3_layout<L, M, U, J>(a);
   if (lds > 160 * 1024) return hipErrorNotSupported;
   auto kern = g.H == 2 * L * M ? k_window_v3<L, M, U, J, true> : k_window_v3<L, M, U, J, false>;
   if (lds > 64 * 1024) {
@@ -1578,15 +1610,14 @@ bool window_v3_supported(const FusedArgs& a) {
   return false;
 }
 
-hipError_t launch_window_v3(const FusedArgs& fa, const void* frames_u16, const float2* tab, const float2* factor,
-                            const float2* ref, float2* fields, const void* hgx16, const void* hgy16, float2* coefs,
-                            const BatchCal& cal, hipStream_t s, const float* hgx32, const float* hgy32) {
-  if (!window_v3_supported(fa)) return hipErrorNotSupported;
-  if ((hgx16 || hgx32) && !window_v3_overlap_fits(fa.g)) return hipErrorNotSupported;
+// Kernel arguments of one batch (false: the shape is not specialised).
+static bool w3_args(const FusedArgs& fa, const void* hgx16, const void* hgy16, float2* coefs, const BatchCal& cal,
+                    const float* hgx32, const float* hgy32, W3Args& a, int& L, int& M) {
+  if (!window_v3_supported(fa)) return false;
+  if ((hgx16 || hgx32) && !window_v3_overlap_fits(fa.g)) return false;
   const Geometry& g = fa.g;
-  int L, M;
   w3_lm(g.W / 2, L, M);
-  W3Args a{};
+  a = W3Args{};
   a.g = g;
   a.ix = fa.ix;
   a.iy = fa.iy;
@@ -1597,17 +1628,116 @@ hipError_t launch_window_v3(const FusedArgs& fa, const void* frames_u16, const f
   a.coefs = coefs;
   a.cal = cal;
   a.magH = magic_div((unsigned)g.H);
-  if (a.magH == 0) return hipErrorNotSupported;
+  if (a.magH == 0) return false;
   // |X| <= W * 65535 < 2^(16 + log2 W): scale into [2^14, 2^15) at most
   int lw = 0;
   while ((1 << lw) < g.W) ++lw;
   a.xscale = ldexpf(1.f, 15 - (16 + lw));
+  return true;
+}
+
+hipError_t launch_window_v3(const FusedArgs& fa, const void* frames_u16, const float2* tab, const float2* factor,
+                            const float2* ref, float2* fields, const void* hgx16, const void* hgy16, float2* coefs,
+                            const BatchCal& cal, hipStream_t s, const float* hgx32, const float* hgy32) {
+  W3Args a;
+  int L, M;
+  if (!w3_args(fa, hgx16, hgy16, coefs, cal, hgx32, hgy32, a, L, M)) return hipErrorNotSupported;
+  const Geometry& g = fa.g;
   const int U = g.w / 16, J = g.h / 16;
 #define W3_CALL(LL, MM, UU, JJ) \
   if (L == LL && M == MM && U == UU && J == JJ) return w3_launch<LL, MM, UU, JJ>(a, frames_u16, tab, factor, ref, fields, s);
   W3_SHAPES(W3_CALL)
 #undef W3_CALL
   return hipErrorNotSupported;
+}
+
+// Several batches (different handles, shapes and sizes) in one launch:
+// a one-dimensional grid holds the (pol, frame) work items of the batches one
+// batch after the other, so no workgroup is idle.  Workgroups are dispatched
+// in order, so the batches run back to back with no launch boundary or tail
+// between them.  Square windows and the shapes of
+// W3_MULTI_SHAPES (a kernel holding more of the specialised bodies spills).
+#define W3_MULTI_SHAPES(X) X(4, 16, 3, 3) X(4, 16, 4, 4) X(4, 24, 4, 4) X(8, 16, 4, 4) X(8, 16, 5, 5)
+
+__global__ __launch_bounds__(kW3Threads, 2) void k_window_v3_multi(W3Multi m) {
+  // the batch of this workgroup: the item ranges of the batches follow each
+  // other, [base, end) is that of batch k
+  int k = 0, base = 0, end = m.it[0].items;
+#pragma unroll
+  for (int i = 1; i < kW3MaxBatches; ++i) {
+    if (i < m.n && (int)blockIdx.x >= end) {
+      base = end;
+      end += m.it[i].items;
+      k = i;
+    }
+  }
+  k = __builtin_amdgcn_readfirstlane(k);
+  const W3Item& t = m.it[k];
+  const int item = __builtin_amdgcn_readfirstlane((int)blockIdx.x - base);
+  if (item >= t.items) return;
+  int sid = 0;
+#define W3_MULTI_CASE(LL, MM, UU, JJ)                                                                  \
+  if (t.shape == sid) w3_body<LL, MM, UU, JJ, true>(t.a, t.frames, t.tabs, t.factor, t.ref, t.fields, item); \
+  ++sid;
+  W3_MULTI_SHAPES(W3_MULTI_CASE)
+#undef W3_MULTI_CASE
+}
+
+// The shape index of a batch in W3_MULTI_SHAPES (-1: none), with its LDS bytes.
+static int w3_multi_shape(W3Args& a, int L, int M, size_t& bytes) {
+  const Geometry& g = a.g;
+  if (g.H != g.W) return -1;   // square windows: the compile-time column length
+  const int U = g.w / 16, J = g.h / 16;
+  int sid = 0, shape = -1;
+#define W3_MULTI_SHAPE(LL, MM, UU, JJ)                                                             \
+  if (shape < 0 && L == LL && M == MM && U == UU && J == JJ && g.H % W3Shape<LL, MM, UU, JJ>::RR == 0) { \
+    shape = sid;                                                                                  \
+    bytes = w3_layout<LL, MM, UU, JJ>(a);                                                         \
+  }                                                                                               \
+  ++sid;
+  W3_MULTI_SHAPES(W3_MULTI_SHAPE)
+#undef W3_MULTI_SHAPE
+  return bytes <= 160 * 1024 ? shape : -1;
+}
+
+bool window_v3_multi_supported(const W3Launch& b) {
+  W3Args a;
+  int L, M;
+  size_t bytes = 0;
+  if (!w3_args(b.fa, b.hgx16, b.hgy16, b.coefs, b.cal, b.hgx32, b.hgy32, a, L, M)) return false;
+  return w3_multi_shape(a, L, M, bytes) >= 0;
+}
+
+hipError_t launch_window_v3_multi(const W3Launch* b, int n, hipStream_t s) {
+  static_assert(sizeof(W3Multi) <= 4000, "kernel arguments too large");
+  if (n < 1 || n > kW3MaxBatches) return hipErrorInvalidValue;
+  W3Multi m{};
+  size_t lds = 0;
+  for (int i = 0; i < n; ++i) {
+    W3Item& t = m.it[i];
+    int L, M;
+    size_t bytes = 0;
+    if (!w3_args(b[i].fa, b[i].hgx16, b[i].hgy16, b[i].coefs, b[i].cal, b[i].hgx32, b[i].hgy32, t.a, L, M))
+      return hipErrorNotSupported;
+    t.shape = w3_multi_shape(t.a, L, M, bytes);
+    if (t.shape < 0) return hipErrorNotSupported;
+    lds = std::max(lds, bytes);
+    t.frames = static_cast<const unsigned short*>(b[i].frames);
+    t.tabs = b[i].tab;
+    t.factor = b[i].factor;
+    t.ref = b[i].ref;
+    t.fields = b[i].fields;
+    t.items = t.a.g.P * t.a.g.B;
+    m.total += t.items;
+  }
+  m.n = n;
+  if (lds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_window_v3_multi),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  k_window_v3_multi<<<m.total, kW3Threads, lds, s>>>(m);
+  return hipGetLastError();
 }
 
 hipError_t launch_fused_window(const FusedArgs& a0, const float2* twW, const float2* twH, const void* frames,

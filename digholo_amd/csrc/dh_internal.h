@@ -184,6 +184,24 @@ hipError_t launch_window_v3(const FusedArgs& a, const void* frames_u16, const fl
                             const BatchCal& cal, hipStream_t s, const float* hgx32 = nullptr,
                             const float* hgy32 = nullptr);
 bool window_v3_overlap_fits(const Geometry& g);
+// One batch of a multi-batch launch of the size-specialised window kernel: the
+// arguments launch_window_v3 takes.
+constexpr int kW3MaxBatches = 4;
+struct W3Launch {
+  FusedArgs fa;
+  const void* frames;
+  const float2 *tab, *factor, *ref;
+  float2* fields;
+  const void *hgx16, *hgy16;
+  float2* coefs;
+  BatchCal cal;
+  const float *hgx32, *hgy32;
+};
+// n <= kW3MaxBatches batches with square FFT windows in one launch (batch =
+// blockIdx.y); window_v3_multi_supported says whether a batch's shape is one
+// the multi-batch kernel holds.
+bool window_v3_multi_supported(const W3Launch& b);
+hipError_t launch_window_v3_multi(const W3Launch* batches, int n, hipStream_t s);
 // Split-fp16 basis [P][hi/lo][Gp][n] (Gp = G rounded up to 16) from hgx/hgy.
 size_t hg_split_elems(const Geometry& g, int n);
 hipError_t launch_hg_split(const Geometry& g, const float* hgx, const float* hgy, void* hgx16, void* hgy16,

@@ -334,11 +334,16 @@ EXT_C int digHoloConfigSetAutoTune(int handleIdx, int mode);
 EXT_C int digHoloConfigGetAutoTune(int handleIdx);
 
 /* Processes the current batch of each listed handle (distinct handles, device-
- * resident frames) in one hipGraph launch on the first handle's stream: each
- * handle's kernels form a branch of the graph, so batches of different shapes
- * run concurrently.  The graph is captured on the first call for a handle list
- * and replayed while every handle's geometry, frames, precision and kernels are
- * unchanged (otherwise re-captured).  Outputs as digHoloProcessBatchDevice /
+ * resident frames) on the first handle's stream.  The first call for a handle
+ * list captures the launches into a hipGraph: batches on the size-specialised
+ * window kernel (HG basis, square windows) share one launch of the multi-batch
+ * window kernel when they hold at most 1024 (pol, frame) items together, else
+ * run one after the other; other batches are branches of the graph.  Later
+ * calls repeat the launches (directly when the graph holds nothing else, else
+ * by replaying it) while every handle's configuration, frames and kernels are
+ * unchanged; otherwise they re-capture.  Work queued through the API on another
+ * listed handle's stream before the call runs first, and that handle's next
+ * API call follows it.  Outputs as digHoloProcessBatchDevice /
  * digHoloGetFieldsDevice of each handle; synchronise the first handle's stream
  * (digHoloSynchronize) before reading them. */
 EXT_C int digHoloProcessBatchMultiDevice(int count, const int* handleIdx);

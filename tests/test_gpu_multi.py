@@ -1,13 +1,15 @@
-"""Several handles' batches in one graph launch (digHoloProcessBatchMultiDevice)
+"""Several handles' batches in one launch sequence (digHoloProcessBatchMultiDevice)
 and the launch autotuner (digHoloConfigSetAutoTune); no counterpart in the
 reference.
 
 Three handle configurations of the uneven-batch workload (320x256 / 128 window
 / 10 groups, 640x512 / 192 / 30, 1024x1024 / 256 / 45; dual pol, uint16 frames
-in HBM), at small batch counts:
-* every replay of the captured graph reproduces, bit for bit, the coefficients
+in HBM), at small batch counts (one launch of the multi-batch window kernel):
+* every call after the capture reproduces, bit for bit, the coefficients
   and fields each handle computes on its own (outputs cleared between
-  replays, eager torch work and synchronisations in between);
+  calls, eager torch work and synchronisations in between);
+* above the one-launch limit the handles' own kernels run one after the other,
+  again bit for bit, ordered against work on the handles' own streams;
 * a configuration change re-captures, and the replay again matches;
 * with autotuning on, the chosen kernels reproduce the default ones to 1e-5
   relative L2, and the choice lands in the cache file.
@@ -188,6 +190,64 @@ def test_multi_graph_outputs_match_oracle(lib, precision):
             print(f"{W}x{H} window {win} G {G} B {B} path {lib.digHoloGetWindowPath(h)}: "
                   f"fields {ef:.2e} coefs {ec:.2e}", flush=True)
             assert ef <= 1e-5 and ec <= 1e-5
+    finally:
+        for h in hs:
+            lib.digHoloDestroy(h)
+
+
+def test_multi_large_batches_and_stream_order(lib):
+    """Batches above the one-launch limit (1060 (pol, frame) items: 520 frames of
+    320x256 / 128 / 10 groups and 10 of 640x512 / 192 / 30) run as the handles'
+    own window kernels one after the other on the first handle's stream: every
+    call reproduces each handle's own outputs bit for bit.  Ordering across the
+    handles' streams: work queued on the second handle's stream before the call
+    (a 1 ms spin, then its coefficients cleared) comes first, and a later
+    download on that stream (digHoloBasisGetCoefs) waits for a call held back by
+    a spin on the first handle's stream."""
+    import torch
+    shapes = [(320, 256, 128, 10, 520), (640, 512, 192, 30, 10)]
+    hs, frames, ref, views = [], [], [], []
+    try:
+        for i, (W, H, win, G, B) in enumerate(shapes):
+            fr = frames_for(W, H, G, B, seed=50 + i)
+            frames.append(fr)
+            h = make_handle(lib, W, H, win, G, B, fr)
+            hs.append(h)
+            c, f = individual(lib, h)
+            views.append((c, f))
+            ref.append((c.clone(), f.clone()))
+            print(f"handle {h}: {W}x{H} window {win} G {G} B {B}", flush=True)
+        ids = (C.c_int * len(hs))(*hs)
+        for rep in range(3):
+            for c, f in views:
+                c.zero_()
+                f.zero_()
+            torch.cuda.synchronize()
+            assert lib.digHoloProcessBatchMultiDevice(len(hs), ids) == 0
+            assert lib.digHoloSynchronize(hs[0]) == 0
+            for (c, f), (cr, fr_) in zip(views, ref):
+                assert torch.equal(c, cr), f"call {rep}: coefficients differ"
+                assert torch.equal(f, fr_), f"call {rep}: fields differ"
+            print(f"call {rep} matches", flush=True)
+        s0 = torch.cuda.ExternalStream(lib.digHoloGetStream(hs[0]))
+        s1 = torch.cuda.ExternalStream(lib.digHoloGetStream(hs[1]))
+        # work on the second handle's stream before the call comes first
+        with torch.cuda.stream(s1):
+            torch.cuda._sleep(2_000_000)
+            views[1][0].zero_()
+        assert lib.digHoloProcessBatchMultiDevice(len(hs), ids) == 0
+        assert lib.digHoloSynchronize(hs[0]) == 0
+        assert torch.equal(views[1][0], ref[1][0]), "the call ran ahead of the second handle's stream"
+        # a download on the second handle's stream follows a delayed call
+        views[1][0].zero_()
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s0):
+            torch.cuda._sleep(2_000_000)
+        assert lib.digHoloProcessBatchMultiDevice(len(hs), ids) == 0
+        got = api.get_coefs(lib, hs[1])
+        want = ref[1][0].cpu().numpy().view(np.complex64).reshape(got.shape)
+        assert np.array_equal(got, want), "the download ran ahead of the call"
+        assert lib.digHoloSynchronize(hs[0]) == 0
     finally:
         for h in hs:
             lib.digHoloDestroy(h)
