@@ -31,9 +31,12 @@ path and this oracle pick the same pixels.  Every continuous quantity (DFTs,
 phases, Hermite functions, sinc envelope, normalisation) is evaluated in
 float64 from the float32 user inputs.  The reference's int16 storage of fields,
 reference wave and modes (:2820-2832, :10490-10497, :1444-1471) is *not*
-reproduced: the output here is the exact value those int16 paths approximate
-(their own error is ~6e-5 relative L2).  ``emulate_int16=True`` adds the int16
-field quantisation of ``applyTilt16`` for error-budget tests.
+reproduced by default: the output is the exact value those int16 paths
+approximate.  ``emulate_int16=True`` adds the int16 field quantisation of
+``applyTilt16``; ``emulate_reference=True`` runs the reference's whole int16 tail
+(float32 sincos with its :786 slip, int16 reference wave, int16 fields, int16 HG
+modes with per-mode scales, float32 accumulation of 16-row blocks in the
+overlap) for the error budget between the reference and the exact path.
 
 Pinning: every function here is checked in ``tests/test_oracle.py`` against
 closed forms or brute force (direct DFT, Hermite polynomials, discrete
@@ -612,7 +615,7 @@ def hold_of(cfg: Config) -> Hold:
 
 def process_batch(frames: np.ndarray, cfg: Config, emulate_int16: bool = False, avg_count: int = 1,
                   avg_mode: int = 0, batch_cal=None, transpose: bool = False, hold: Hold = None,
-                  ref_cal=None):
+                  ref_cal=None, emulate_reference: bool = False):
     """Full ``digHoloProcessBatch`` path.
 
     frames: [B * avg_count, frameH, frameW] (uint16 or float).  Returns (fields,
@@ -623,6 +626,10 @@ def process_batch(frames: np.ndarray, cfg: Config, emulate_int16: bool = False, 
     [Lc, frameH, frameW] complex is a stored reference-wave calibration
     (ref_cal_from_*): the inverse-transformed field of wavelength l is multiplied
     by its plane l mod Lc before the tilt is removed (digHolo.cpp:8736-8758).
+    emulate_reference replaces the tilt removal and the overlap by the reference's
+    int16 forms (ref_wave_int16, tilt_int16, hg_int16, overlap16; low-res IFFT, one
+    wavelength, no averaging or calibration): fields are the int16 values times the
+    frame's float32 scale as digHoloGetFields returns them (:4950-4976).
     """
     frames = np.asarray(frames)
     if transpose:
@@ -680,6 +687,27 @@ def process_batch(frames: np.ndarray, cfg: Config, emulate_int16: bool = False, 
             full[:, t.row[:, None], t.col[None, :]] = Fw
             Fw = full
         E = np.fft.ifft2(Fw, axes=(-2, -1)) * (fw * fh)              # unnormalised e^{+i}
+        if emulate_reference:
+            if L > 1 or A > 1 or batch_cal is not None or cal_planes is not None or not low:
+                raise ValueError("the reference emulation covers one wavelength, low-res, no averaging "
+                                 "or calibration")
+            rx, ry = ref_wave_int16(cfg, plan, p)
+            if G > 0:
+                qx, sx = hg_int16(t.hgx)
+                qy, sy = hg_int16(t.hgy)
+            for b in range(B):
+                Eb = E[b].astype(np.complex64)
+                # the frame's scale from the field analysis of E (:2649, :9174)
+                p2 = (Eb.real * Eb.real + Eb.imag * Eb.imag).astype(np.float32)
+                mx = np.sqrt(np.max(p2)).astype(np.float32)
+                maxabs = F32(32767.0 / float(mx))
+                q = tilt_int16(Eb, rx, ry, maxabs)
+                fscale = F32(F32(1.0) / maxabs)
+                fields[out_b[b], p] = (q.real.astype(np.float32) * fscale).astype(np.float64) + \
+                    1j * (q.imag.astype(np.float32) * fscale).astype(np.float64)
+                if G > 0:
+                    coefs[out_b[b], p] = overlap16(q, fscale, qx, sx, qy, sy, G)
+            continue
         V = -np.conj(E * ref[lam_b])
         if emulate_int16:
             V = quantise_int16(V)
@@ -689,6 +717,157 @@ def process_batch(frames: np.ndarray, cfg: Config, emulate_int16: bool = False, 
             C = np.einsum("ny,bmy->bmn", t.hgy, T)       # stage 2: y contraction
             coefs[out_b, p] = C[:, Mi, Ni]
     return fields, coefs, plan
+
+
+# ---------------------------------------------------------------------------
+# Reference emulation: the reference's float32 sincos, its int16 reference
+# wave, int16 fields and int16 HG modes with per-mode scales, and its overlap's
+# integer products with float32 accumulation over 16-row blocks.  Test
+# infrastructure for the error budget between the reference and the exact path
+# (low-res IFFT, one wavelength).
+# ---------------------------------------------------------------------------
+
+def _fma32(a, b, c):
+    """float32 fused multiply-add (the product of two float32 values is exact in float64)."""
+    return (np.asarray(a, np.float64) * np.asarray(b, np.float64) + np.asarray(c, np.float64)).astype(np.float32)
+
+
+def sincos_f32(x0) -> Tuple[np.ndarray, np.ndarray]:
+    """``sincos_ps`` (digHolo.cpp:716-812) in float32: the quadrant y = rint(|x| 2/pi), a
+    three-part Cody-Waite reduction onto [-pi/4, pi/4] with Cephes' split of pi/2,
+    Taylor polynomials to x^9 (sin) and x^8 (cos) in Horner form, then the quadrant's
+    swaps and signs.  The cos polynomial's second step multiplies by x where x^2 is
+    meant (:786), which this restatement keeps: cos carries an error up to ~6e-6."""
+    x0 = np.asarray(x0, dtype=np.float32)
+    xa = np.abs(x0)
+    y = np.rint((xa * F32(2.0 / math.pi)).astype(np.float32)).astype(np.float32)
+    q = y.astype(np.int64)
+    x = _fma32(y, F32(-0.78515625 * 2.0), xa)
+    x = _fma32(y, F32(-2.4187564849853515625e-4 * 2.0), x)
+    x = _fma32(y, F32(-3.77489497744594108e-8 * 2.0), x)
+    x2 = (x * x).astype(np.float32)
+    x3 = (x2 * x).astype(np.float32)
+    fact = [math.factorial(n) for n in range(10)]
+    s = (F32(1.0 / fact[9]) * x2).astype(np.float32)
+    s = _fma32(s, x2, F32(-1.0 / fact[7]))
+    s = _fma32(s, x2, F32(1.0 / fact[5]))
+    s = _fma32(s, x2, F32(-1.0 / fact[3]))
+    s = _fma32(s, x3, x)
+    c = (F32(1.0 / fact[8]) * x2).astype(np.float32)
+    c = _fma32(c, x, F32(-1.0 / fact[6]))          # x, as at :786
+    c = _fma32(c, x2, F32(1.0 / fact[4]))
+    c = _fma32(c, x2, F32(-0.5))
+    c = _fma32(c, x2, F32(1.0))
+    odd = (q & 1) == 1
+    s, c = np.where(odd, c, s), np.where(odd, s, c)
+    s = np.where(x0 < 0, -s, s)
+    s = np.where(((q >> 1) & 1) == 1, -s, s)
+    c = np.where((((q + 1) >> 1) & 1) == 1, -c, c)
+    return s.astype(np.float32), c.astype(np.float32)
+
+
+def ref_wave_int16(cfg: Config, plan: Plan, p: int):
+    """The reference wave as the reference stores it (digHolo.cpp:10289-10497): separable
+    x and y factors exp(i(k r + piston + focal r^2)) evaluated in float32 on the field-plane
+    axes, each component times 16383 rounded to int16.  Returns (rx, ry) complex arrays
+    of the integer values (real and imaginary parts are the stored int16s)."""
+    if cfg.resolution_mode != 1:
+        raise ValueError("the reference emulation covers the low-res IFFT")
+    W, H = plan.W, plan.H
+    tp, ip = _pol_index(cfg, p)
+    lam = float(F32(cfg.wavelength))
+    k0 = F32(2.0 * math.pi / lam)
+    thx = tilt_rad_f32(cfg.tilt_x[tp])
+    thy = tilt_rad_f32(cfg.tilt_y[tp])
+    kxa = k_axis_f32(W, cfg.pixel_size)
+    kya = k_axis_f32(H, cfg.pixel_size)
+    dkx = F32(kxa[1] - kxa[0])
+    dky = F32(kya[1] - kya[0])
+    offx = F32((plan.SX[ip] - W / 2.0) * float(dkx))
+    offy = F32((plan.SY[ip] - H / 2.0) * float(dky))
+    D = F32(cfg.defocus[ip])
+    kx = F32(F32(-k0 * thx) + offx)
+    ky = F32(F32(-k0 * thy) + offy)
+    kx0 = kx if offx == 0 else F32(k0 * F32(-thx))
+    ky0 = ky if offy == 0 else F32(k0 * F32(-thy))
+    focal = F32(-math.pi * float(D) / lam)
+    x, y = field_axes(cfg, plan)
+    out = []
+    for axis, k, k00, centre in ((x, kx, kx0, cfg.beam_centre_x[p]), (y, ky, ky0, cfg.beam_centre_y[p])):
+        piston = F32(F32(centre) * k00)
+        r = axis.astype(np.float32)
+        arg = (_fma32(k, r, piston) + (focal * (r * r).astype(np.float32)).astype(np.float32)).astype(np.float32)
+        sn, cs = sincos_f32(arg)
+        qr = np.clip(np.rint((F32(16383.0) * cs).astype(np.float32)), -32768, 32767)
+        qi = np.clip(np.rint((F32(16383.0) * sn).astype(np.float32)), -32768, 32767)
+        out.append(qr.astype(np.int64) + 1j * qi.astype(np.int64))
+    return out[0], out[1]
+
+
+def hg_int16(modes: np.ndarray):
+    """int16 HG modes with a per-mode scale (digHolo.cpp:1444-1471): scale factor
+    32767 / max|mode| in float64, the float32 mode times its float32 value rounded to
+    int16; the stored scale is float32(1 / factor).  modes [G, n] -> (q [G, n] int64,
+    scales [G] float32)."""
+    modes = np.asarray(modes, dtype=np.float64)
+    q = np.zeros(modes.shape, dtype=np.int64)
+    sc = np.zeros(modes.shape[0], dtype=np.float32)
+    for m in range(modes.shape[0]):
+        mx = float(np.max(np.abs(modes[m])))
+        f = 32767.0 / mx
+        v = (F32(f) * modes[m].astype(np.float32)).astype(np.float32)
+        q[m] = np.clip(np.rint(v), -32768, 32767).astype(np.int64)
+        sc[m] = F32(1.0 / f)
+    return q, sc
+
+
+def tilt_int16(E: np.ndarray, rx: np.ndarray, ry: np.ndarray, scale) -> np.ndarray:
+    """``applyTilt16`` (digHolo.cpp:2701-2837) for one (frame, pol): the reference wave from
+    its int16 factors (integer products, times float32 1/16383^2), V = -conj(E R) in
+    float32, then V times the frame's scale rounded to int16.  Returns the int16 field
+    as a complex array of integers."""
+    E = np.asarray(E).astype(np.complex64)
+    rr = (ry.real[:, None] * rx.real[None, :] - ry.imag[:, None] * rx.imag[None, :]).astype(np.float32)
+    ri = (ry.real[:, None] * rx.imag[None, :] + ry.imag[:, None] * rx.real[None, :]).astype(np.float32)
+    s = F32(1.0 / (16383.0 * 16383.0))
+    ar = (s * rr).astype(np.float32)
+    ai = (s * ri).astype(np.float32)
+    Er, Ei = E.real.astype(np.float32), E.imag.astype(np.float32)
+    vr = ((Ei * ai).astype(np.float32) - (Er * ar).astype(np.float32)).astype(np.float32)
+    vi = ((Er * ai).astype(np.float32) + (Ei * ar).astype(np.float32)).astype(np.float32)
+    scale = F32(scale)
+    qr = np.clip(np.rint((scale * vr).astype(np.float32)), -32768, 32767)
+    qi = np.clip(np.rint((scale * vi).astype(np.float32)), -32768, 32767)
+    return qr.astype(np.int64) + 1j * qi.astype(np.int64)
+
+
+def overlap16(f16: np.ndarray, fscale, qx: np.ndarray, sx: np.ndarray, qy: np.ndarray, sy: np.ndarray,
+              G: int) -> np.ndarray:
+    """``overlapFieldSeparable16`` (digHolo.cpp:2149-2341) for one (frame, pol): exact
+    integer products of the int16 field and x-modes summed over x (T[y][m]), then per
+    block of 16 rows the block's T times the int16 y-mode summed in float64, rounded to
+    float32 and accumulated in float32 over the blocks; at the last block the float32
+    product of the three scales multiplies the sum.  When the field has exactly one
+    block (16 rows) that last-block step never runs (:2313-2329) and the coefficients
+    stay unscaled, as there.  Returns [M] complex64 in mode order."""
+    fh = f16.shape[0]
+    Mi, Ni = hg_indices(G)
+    T = np.einsum("mx,yx->my", qx.astype(np.float64), f16.real.astype(np.float64)), \
+        np.einsum("mx,yx->my", qx.astype(np.float64), f16.imag.astype(np.float64))
+    nb = fh // 16
+    out = np.zeros(len(Mi), dtype=np.complex64)
+    for j, (m, n) in enumerate(zip(Mi, Ni)):
+        acc = [F32(0.0), F32(0.0)]
+        for part in range(2):
+            for blk in range(nb):
+                ys = slice(16 * blk, 16 * blk + 16)
+                ov = F32(float(np.dot(T[part][m, ys], qy[n, ys].astype(np.float64))))
+                acc[part] = ov if blk == 0 else F32(acc[part] + ov)
+            if nb > 1:
+                sc = F32(float(sx[m]) * float(sy[n]) * float(F32(fscale)))
+                acc[part] = F32(acc[part] * sc)
+        out[j] = np.complex64(complex(float(acc[0]), float(acc[1])))
+    return out
 
 
 def quantise_int16(V: np.ndarray) -> np.ndarray:

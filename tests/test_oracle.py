@@ -190,6 +190,96 @@ def test_int16_emulation_error_budget():
     assert e_c < 2e-4
 
 
+def test_sincos_f32_reference_form():
+    # the reference's float32 sincos (digHolo.cpp:716-812): its cos polynomial
+    # multiplies by x where x^2 is meant (:786), an error of 1/8! x^8 (1 - x) that
+    # reaches ~6.4e-6 at a reduced argument of -pi/4 (odd quadrants put it into sin)
+    x = np.linspace(-60.0, 60.0, 240001).astype(np.float32)
+    s, c = O.sincos_f32(x)
+    es = np.max(np.abs(s - np.sin(x.astype(np.float64))))
+    ec = np.max(np.abs(c - np.cos(x.astype(np.float64))))
+    assert 5e-6 < es < 7e-6 and 5e-6 < ec < 7e-6, (es, ec)
+    # first quadrant: the reduced argument is |x| >= 0, where the slip stays below 8e-7
+    xr = np.linspace(0.0, np.pi / 4 * 0.999, 10001).astype(np.float32)
+    s, c = O.sincos_f32(xr)
+    assert np.max(np.abs(s - np.sin(xr.astype(np.float64)))) < 3e-7
+    e1 = np.max(np.abs(c - np.cos(xr.astype(np.float64))))
+    assert 3e-7 < e1 < 8e-7, e1
+    # odd symmetry of sin, even of cos
+    s1, c1 = O.sincos_f32(-x)
+    s0, c0 = O.sincos_f32(x)
+    assert np.array_equal(s1, -s0) and np.array_equal(c1, c0)
+
+
+def test_reference_wave_int16():
+    # int16 factors of the separable reference wave (digHolo.cpp:10386-10497) against
+    # the FP64 wave pinned by the known-answer tests: quantisation (1/(2*16383) per
+    # component) and float32 phases, nothing more
+    cfg = O.apply_pol_locks(small_cfg(fill_factor=1, defocus=[0.3, 0.3], beam_centre_x=[1e-4, 1e-4]))
+    plan = O.make_plan(cfg)
+    t = O.pol_tables(cfg, plan, 0)
+    rx, ry = O.ref_wave_int16(cfg, plan, 0)
+    for f in (rx, ry):
+        assert np.all(np.abs(f.real) <= 16383) and np.all(np.abs(f.imag) <= 16383)
+        assert np.all(f.real == np.rint(f.real)) and np.all(f.imag == np.rint(f.imag))
+    R = ry[:, None] * rx[None, :] / 16383.0 ** 2
+    assert np.max(np.abs(R - t.ref)) < 1.5e-4
+    assert 5e-6 < O.relative_l2(R, t.ref) < 1e-4
+
+
+def test_hg_int16_per_mode_scale():
+    # digHolo.cpp:1444-1471: each mode spans the int16 range, dequantised within half a step
+    cfg = small_cfg()
+    plan = O.make_plan(cfg)
+    t = O.pol_tables(O.apply_pol_locks(cfg), plan, 0)
+    q, sc = O.hg_int16(t.hgx)
+    assert np.all(np.max(np.abs(q), axis=1) == 32767)
+    err = np.abs(q * sc[:, None].astype(np.float64) - t.hgx) / sc[:, None]
+    assert 0.4 < np.max(err) <= 0.51         # rounded to the nearest step
+
+
+def test_overlap16_block_accumulation():
+    # digHolo.cpp:2149-2341: integer products, float64 sums within a 16-row block,
+    # float32 across blocks, the scales applied at the last block -- and with a
+    # single 16-row block the scales are never applied (:2313-2329)
+    rng = np.random.default_rng(7)
+    G, w = 6, 32
+    M = G * (G + 1) // 2
+    Mi, Ni = O.hg_indices(G)
+    qx = rng.integers(-32767, 32768, (G, w))
+    sx = rng.uniform(1e-6, 1e-5, G).astype(np.float32)
+    for h in (48, 16):
+        f = rng.integers(-32767, 32768, (h, w)) + 1j * rng.integers(-32767, 32768, (h, w))
+        qy = rng.integers(-32767, 32768, (G, h))
+        sy = rng.uniform(1e-6, 1e-5, G).astype(np.float32)
+        fs = np.float32(3e-5)
+        c = O.overlap16(f, fs, qx, sx, qy, sy, G)
+        exact_int = np.array([(qy[n] * (f @ qx[m])).sum() for m, n in zip(Mi, Ni)])
+        if h == 16:
+            assert np.array_equal(c, exact_int.astype(np.complex64))
+            continue
+        scale = np.array([float(np.float32(float(sx[m]) * float(sy[n]) * float(fs))) for m, n in zip(Mi, Ni)])
+        exact = exact_int * scale
+        blocks = np.array([[np.abs((qy[n, 16 * b:16 * b + 16] * (f[16 * b:16 * b + 16] @ qx[m])).sum())
+                            for b in range(h // 16)] for m, n in zip(Mi, Ni)]).sum(axis=1)
+        bound = 4 * 2.0 ** -24 * blocks * scale
+        assert np.all(np.abs(c - exact) <= bound + 1e-30)
+        assert len(c) == M
+
+
+def test_reference_emulation_error_budget():
+    # the reference's int16 tail against the exact path on 12-bit holograms: what a
+    # comparison with the reference's own output can be held to
+    spec = synth.spec_small()
+    frames, c, s, A = synth.make_batch(spec, 3, seed=2, quantise=True)
+    cfg = small_cfg(fill_factor=1)
+    f0, c0, _ = O.process_batch(frames, cfg)
+    f1, c1, _ = O.process_batch(frames, cfg, emulate_reference=True)
+    ef, ec = O.relative_l2(f1, f0), O.relative_l2(c1, c0)
+    assert 2e-5 < ef < 2e-4, ef
+    assert 3e-6 < ec < 1e-4, ec
+
+
 def test_linearity():
     spec = synth.spec_small()
     f1, _, _, _ = synth.make_batch(spec, 2, seed=4, quantise=False)
