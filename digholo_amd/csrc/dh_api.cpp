@@ -1425,7 +1425,8 @@ int enqueue_batch(Handle& h, int step = 0) {
       h.pendSplit = c.precision == DIGHOLO_PRECISION_SPLIT16;
     } else {
       DH_TRY(dh::launch_overlap_mfma(gF, h.fields.p, h.hgx.p, h.hgy.p, h.coefs.p,
-                                     c.precision == DIGHOLO_PRECISION_SPLIT16, h.stream, h.ovTpw, peak),
+                                     c.precision == DIGHOLO_PRECISION_SPLIT16, h.stream, h.ovTpw, peak, h.hgx16.p,
+                                     h.hgy16.p),
              "overlap");
     }
   }
@@ -1519,7 +1520,9 @@ int step_extract_coefs(Handle& h) {
   if (g.G <= 0) return DIGHOLO_ERROR_SUCCESS;
   DH_TRY(h.coefs.ensure((size_t)g.B * g.P * g.M), "alloc");
   DH_TRY(dh::launch_overlap_mfma(g, h.fields.p, h.hgx.p, h.hgy.p, h.coefs.p,
-                                 h.cfg.precision == DIGHOLO_PRECISION_SPLIT16, h.stream), "overlap");
+                                 h.cfg.precision == DIGHOLO_PRECISION_SPLIT16, h.stream, 0, nullptr, h.hgx16.p,
+                                 h.hgy16.p),
+         "overlap");
   if (int eb = apply_basis(h)) return eb;
   h.coefsDevValid = true;
   h.coefsHostValid = false;
@@ -3046,6 +3049,7 @@ struct MultiOp {
   dh::Geometry g{};   // overlap operands
   const float2* fields = nullptr;
   const float *hgx = nullptr, *hgy = nullptr;
+  const void *hgx16 = nullptr, *hgy16 = nullptr;
   float2* coefs = nullptr;
   int split = 0, tpw = 0;
 };
@@ -3102,7 +3106,8 @@ hipError_t run_multi_ops(const MultiGraph& m, hipStream_t s) {
       e = dh::launch_window_v3(w.fa, w.frames, w.tab, w.factor, w.ref, w.fields, w.hgx16, w.hgy16, w.coefs, w.cal, s,
                                w.hgx32, w.hgy32);
     } else {
-      e = dh::launch_overlap_mfma(o.g, o.fields, o.hgx, o.hgy, o.coefs, o.split, s, o.tpw, nullptr);
+      e = dh::launch_overlap_mfma(o.g, o.fields, o.hgx, o.hgy, o.coefs, o.split, s, o.tpw, nullptr, o.hgx16,
+                                  o.hgy16);
     }
     if (e != hipSuccess) return e;
   }
@@ -3208,6 +3213,8 @@ int capture_multi(MultiGraph& m, const std::vector<Handle*>& hs) {
       o.fields = h->fields.p;
       o.hgx = h->hgx.p;
       o.hgy = h->hgy.p;
+      o.hgx16 = h->hgx16.p;
+      o.hgy16 = h->hgy16.p;
       o.coefs = h->coefs.p;
       o.split = h->pendSplit ? 1 : 0;
       o.tpw = h->ovTpw;
@@ -4476,7 +4483,7 @@ EXT_C int digHoloProcessBatchDeviceNoAlloc(int handleIdx, const void* frames, in
   DH_TRY(dh::launch_ifft_rows_tilt(g, *pIx, win, h.ref.p, fields, h.stream, peakW), "IFFT rows");
   if (g.G > 0)
     DH_TRY(dh::launch_overlap_mfma(g, fields, h.hgx.p, h.hgy.p, reinterpret_cast<float2*>(coefsOut), split ? 1 : 0,
-                                   h.stream, h.ovTpw, peakW),
+                                   h.stream, h.ovTpw, peakW, h.hgx16.p, h.hgy16.p),
            "overlap");
   return DIGHOLO_ERROR_SUCCESS;
 }

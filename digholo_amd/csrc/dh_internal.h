@@ -294,10 +294,20 @@ hipError_t noalloc_scratch(void** ptr);
 // triangle, overlap_min_tiles_per_wave; larger values give each wave more).
 // peak: per-field largest |Re|, |Im| as float bits ([B][P]) when the producing
 // kernel has left it, else null (the split form then finds it itself).
+// hgx16/hgy16: the fp16 hi/lo basis copies (launch_hg_split); when given, the
+// split form runs on k_overlap_split where that covers the shape.
 hipError_t launch_overlap_mfma(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
                                float2* coefs, int split, hipStream_t s, int tpw = 0,
-                               const unsigned* peak = nullptr);
+                               const unsigned* peak = nullptr, const void* hgx16 = nullptr,
+                               const void* hgy16 = nullptr);
 int overlap_min_tiles_per_wave(const Geometry& g, int split);
+// Split-fp16 overlap with the basis operands held in registers (one wave per
+// 16-mode tile, one (frame, pol) per workgroup) from the fp16 hi/lo basis
+// copies of launch_hg_split; fields with w <= 160 and G <= 96 (w, h multiples of
+// 16), else hipErrorNotSupported.
+bool overlap_split_supported(const Geometry& g);
+hipError_t launch_overlap_split(const Geometry& g, const float2* fields, const void* hgx16, const void* hgy16,
+                                float2* coefs, hipStream_t s, const unsigned* peak);
 
 // Reference-wave calibration plane (dh_refcal.hip).
 // tw[m] = exp(-2 pi i m / N), FP64, m < N

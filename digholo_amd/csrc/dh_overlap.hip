@@ -321,6 +321,251 @@ __global__ __launch_bounds__(64 * NW) void k_overlap_mfma(Geometry g, const floa
   }   // frames of this slot
 }
 
+// float4 pairs of a field chunk (8 w of them, w <= 32 KS1) per thread of a
+// workgroup of NT2 waves: at most 8
+__host__ __device__ constexpr int split_pairs_per_thread(int ks1, int nt2) {
+  return (4 * ks1 + nt2 - 1) / nt2 < 8 ? (4 * ks1 + nt2 - 1) / nt2 : 8;
+}
+
+// Split-fp16 overlap with its basis operands in registers (fields of w <= 160,
+// G <= 96): one wave per m-tile of 16 modes, both parts (re, im) of the field on
+// the same wave, so the wave's HGx operand (KS1 k-steps of hi and lo) is loaded
+// once and serves every chunk of the field and both parts.  Per 16-row chunk the
+// field goes to LDS once (double-buffered, row pitch w + 2 complex: 16 rows of one
+// 16-byte column fall on distinct bank groups), every wave splits its A operand
+// into fp16 hi/lo as it reads it, and stage 2 runs from the stage-1 accumulators
+// (their C layout is the A layout of v_mfma_f32_16x16x16_f16) against HGy
+// fragments read from L2: no exchange through LDS, one barrier per chunk.  One
+// (frame, pol) per workgroup.
+// Workgroups per CU the register budget is bounded for: three (168 VGPRs) where
+// the wave's basis, accumulators and prefetched chunk fit without spilling
+// (8 KS1 + 8 NT2 + 4 pairs <= 80 registers, and fewer than six m-tiles; checked
+// against the compiler's spill counts), else two.
+__host__ __device__ constexpr int split_min_blocks(int ks1, int nt2) {
+  return 8 * ks1 + 8 * nt2 + 4 * split_pairs_per_thread(ks1, nt2) <= 80 && nt2 < 6 ? 3 : 2;
+}
+
+template <int KS1, int NT2>
+__global__ __launch_bounds__(64 * NT2, split_min_blocks(KS1, NT2)) void k_overlap_split(Geometry g, const float2* __restrict__ fields,
+                                                            const _Float16* __restrict__ hgx16,
+                                                            const _Float16* __restrict__ hgy16,
+                                                            float2* __restrict__ coefs,
+                                                            const unsigned* __restrict__ peak) {
+  extern __shared__ __align__(16) unsigned char os_smem[];
+  __shared__ float s_red[NT2];
+  constexpr int NT = 64 * NT2;
+  const int p = blockIdx.x, b = blockIdx.y;
+  const int G = g.G, Gp = (G + 15) & ~15, w = g.w, h = g.h;
+  const int P2 = w + 2;
+  float2* buf = reinterpret_cast<float2*>(os_smem);   // [2][16 P2 + 2]
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int mt = __builtin_amdgcn_readfirstlane(tid >> 6);   // this wave's m-tile
+  const int lr = lane & 15, q = lane >> 4;
+  const _Float16* bx = hgx16 + (size_t)p * 2 * Gp * w;
+  const _Float16* by = hgy16 + (size_t)p * 2 * Gp * h;
+
+  // stage-1 basis operand: HGx rows 16 mt + lr, k-slots 32 ks + 8 q .. + 7
+  half8 b1h[KS1], b1l[KS1];
+#pragma unroll
+  for (int ks = 0; ks < KS1; ++ks) {
+    const int kb = 32 * ks + 8 * q;
+    if (kb < w) {
+      const _Float16* bp = bx + (16 * mt + lr) * w + kb;
+      b1h[ks] = *reinterpret_cast<const half8*>(bp);
+      b1l[ks] = *reinterpret_cast<const half8*>(bp + Gp * w);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) b1h[ks][e] = b1l[ks][e] = (_Float16)0.f;
+    }
+  }
+
+  // complex pairs as native 4-vectors (a struct-typed float4 copy would keep the
+  // prefetch registers in scratch)
+  const floatx4* V4 = reinterpret_cast<const floatx4*>(fields + (size_t)(b * g.P + p) * h * w);
+  // per-field power of two: the peak magnitude into [128, 256)
+  float mx;
+  if (peak) {   // left by the kernel that wrote the field
+    mx = __uint_as_float(peak[b * g.P + p]);
+  } else {
+    mx = 0.f;
+    for (int e = tid; e < h * w / 2; e += NT) {
+      const floatx4 v = V4[e];
+      mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3]))));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    if (lane == 0) s_red[mt] = mx;
+    __syncthreads();
+    mx = s_red[0];
+#pragma unroll
+    for (int i = 1; i < NT2; ++i) mx = fmaxf(mx, s_red[i]);
+  }
+  float scale = 1.f;
+  if (mx > 0.f) {
+    int ex;
+    frexpf(mx, &ex);
+    scale = ldexpf(1.f, 8 - ex);
+  }
+
+  // a chunk is 8 w float4 (complex pairs); this thread's pairs e2 = tid + NT j
+  // (j < kPer, which the launcher checks covers the chunk) and their LDS offsets
+  // are the same for every chunk; the next chunk's pairs are held in registers
+  // (a pair past the chunk reads the chunk's last pair and is stored to a spare
+  // slot after the buffer, so the loops below have no branch)
+  constexpr int kPer = split_pairs_per_thread(KS1, NT2);
+  const int n4 = 8 * w, J = h / 16, BS = 16 * P2 + 2;
+  int lofs[kPer], gofs[kPer];
+  floatx4 nxt[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int e2 = tid + NT * j, yy = (2 * e2) / w;
+    lofs[j] = e2 < n4 ? yy * P2 + 2 * e2 - yy * w : 16 * P2;
+    gofs[j] = e2 < n4 ? e2 : n4 - 1;
+    *reinterpret_cast<floatx4*>(buf + lofs[j]) = V4[gofs[j]];    // chunk 0
+    nxt[j] = V4[(J > 1 ? n4 : 0) + gofs[j]];
+  }
+
+  floatx4 acc0[NT2], acc1[NT2];
+#pragma unroll
+  for (int nt = 0; nt < NT2; ++nt) acc0[nt] = acc1[nt] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int ch = 0; ch < J; ++ch) {
+    __syncthreads();   // chunk ch is in its buffer; the other buffer's readers are done
+    const float2* cb = buf + (ch & 1) * BS;
+    // stage 1, both parts: T_c[yy][m] = sum_x V_c[yy][x] HGx[m][x]
+    floatx4 d0 = {0.f, 0.f, 0.f, 0.f}, d1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < KS1; ++ks) {
+      const int kb = 32 * ks + 8 * q;
+      half8 ahr, alr, ahi, ali;
+      if (kb < w) {   // a ragged last step (w % 32 == 16) zero-fills the upper lane groups
+        const floatx4* vp = reinterpret_cast<const floatx4*>(cb + lr * P2 + kb);
+        uint4 rh, rl, ih, il;
+        unsigned* prh = reinterpret_cast<unsigned*>(&rh);
+        unsigned* prl = reinterpret_cast<unsigned*>(&rl);
+        unsigned* pih = reinterpret_cast<unsigned*>(&ih);
+        unsigned* pil = reinterpret_cast<unsigned*>(&il);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const floatx4 t = vp[e];   // (re, im) of columns kb + 2e, kb + 2e + 1
+          split_pair(t[0] * scale, t[2] * scale, prh[e], prl[e]);
+          split_pair(t[1] * scale, t[3] * scale, pih[e], pil[e]);
+        }
+        ahr = __builtin_bit_cast(half8, rh);
+        alr = __builtin_bit_cast(half8, rl);
+        ahi = __builtin_bit_cast(half8, ih);
+        ali = __builtin_bit_cast(half8, il);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) ahr[e] = alr[e] = ahi[e] = ali[e] = (_Float16)0.f;
+      }
+      d0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahr, b1h[ks], d0, 0, 0, 0);
+      d1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, b1h[ks], d1, 0, 0, 0);
+      d0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahr, b1l[ks], d0, 0, 0, 0);
+      d1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, b1l[ks], d1, 0, 0, 0);
+      d0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(alr, b1h[ks], d0, 0, 0, 0);
+      d1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ali, b1h[ks], d1, 0, 0, 0);
+    }
+    // stage-2 basis of this chunk, HGy[16 nt + lr][16 ch + 4 q .. + 3] (rows >= G
+    // zero), from L2 while the next chunk is stored
+    half4 b2h[NT2], b2l[NT2];
+#pragma unroll
+    for (int nt = 0; nt < NT2; ++nt) {
+      if (16 * mt + 16 * nt < G) {
+        const _Float16* brow = by + (16 * nt + lr) * h + 16 * ch + 4 * q;
+        b2h[nt] = *reinterpret_cast<const half4*>(brow);
+        b2l[nt] = *reinterpret_cast<const half4*>(brow + Gp * h);
+      }
+    }
+    // the next chunk into the other buffer (loaded a chunk ago); the one after in flight
+    if (ch + 1 < J) {
+      float2* nb = buf + ((ch + 1) & 1) * BS;
+      const size_t c2 = (size_t)(ch + 2 < J ? ch + 2 : ch + 1) * n4;
+#pragma unroll
+      for (int j = 0; j < kPer; ++j) {
+        *reinterpret_cast<floatx4*>(nb + lofs[j]) = nxt[j];
+        nxt[j] = V4[c2 + gofs[j]];
+      }
+    }
+    // stage 2 from registers: C_c[m][n] += sum_yy T_c[m][yy] HGy[n][yy]
+    uint2 t0h, t0l, t1h, t1l;
+    split_pair(d0[0], d0[1], t0h.x, t0l.x);
+    split_pair(d0[2], d0[3], t0h.y, t0l.y);
+    split_pair(d1[0], d1[1], t1h.x, t1l.x);
+    split_pair(d1[2], d1[3], t1h.y, t1l.y);
+    const half4 a0h = __builtin_bit_cast(half4, t0h), a0l = __builtin_bit_cast(half4, t0l);
+    const half4 a1h = __builtin_bit_cast(half4, t1h), a1l = __builtin_bit_cast(half4, t1l);
+#pragma unroll
+    for (int nt = 0; nt < NT2; ++nt) {
+      if (16 * mt + 16 * nt < G) {
+        acc0[nt] = __builtin_amdgcn_mfma_f32_16x16x16f16(a0h, b2h[nt], acc0[nt], 0, 0, 0);
+        acc1[nt] = __builtin_amdgcn_mfma_f32_16x16x16f16(a1h, b2h[nt], acc1[nt], 0, 0, 0);
+        acc0[nt] = __builtin_amdgcn_mfma_f32_16x16x16f16(a0h, b2l[nt], acc0[nt], 0, 0, 0);
+        acc1[nt] = __builtin_amdgcn_mfma_f32_16x16x16f16(a1h, b2l[nt], acc1[nt], 0, 0, 0);
+        acc0[nt] = __builtin_amdgcn_mfma_f32_16x16x16f16(a0l, b2h[nt], acc0[nt], 0, 0, 0);
+        acc1[nt] = __builtin_amdgcn_mfma_f32_16x16x16f16(a1l, b2h[nt], acc1[nt], 0, 0, 0);
+      }
+    }
+  }
+  // the triangle m + n < G; lane (n, q) holds m = 16 mt + 4 q .. + 3
+  const float inv = 1.f / scale;
+  float2* out = coefs + (size_t)(b * g.P + p) * g.M;
+#pragma unroll
+  for (int nt = 0; nt < NT2; ++nt) {
+    const int n = 16 * nt + lr;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int m = 16 * mt + 4 * q + e;
+      if (m + n < G) {
+        const int gs = m + n;
+        out[gs * (gs + 1) / 2 + n] = make_float2(acc0[nt][e] * inv, acc1[nt][e] * inv);
+      }
+    }
+  }
+}
+
+template <int KS1, int NT2>
+static hipError_t split_launch(const Geometry& g, const float2* fields, const void* hgx16, const void* hgy16,
+                               float2* coefs, hipStream_t s, const unsigned* peak) {
+  if (8 * g.w > split_pairs_per_thread(KS1, NT2) * 64 * NT2) return hipErrorNotSupported;
+  const size_t lds = (size_t)2 * (16 * (g.w + 2) + 2) * sizeof(float2);
+  dim3 grid(g.P, g.B);
+  k_overlap_split<KS1, NT2><<<grid, 64 * NT2, lds, s>>>(g, fields, static_cast<const _Float16*>(hgx16),
+                                                         static_cast<const _Float16*>(hgy16), coefs, peak);
+  return hipGetLastError();
+}
+
+template <int KS1>
+static hipError_t split_dispatch_nt(const Geometry& g, const float2* fields, const void* hgx16, const void* hgy16,
+                                    float2* coefs, hipStream_t s, const unsigned* peak) {
+  switch ((g.G + 15) / 16) {
+    case 1: return split_launch<KS1, 1>(g, fields, hgx16, hgy16, coefs, s, peak);
+    case 2: return split_launch<KS1, 2>(g, fields, hgx16, hgy16, coefs, s, peak);
+    case 3: return split_launch<KS1, 3>(g, fields, hgx16, hgy16, coefs, s, peak);
+    case 4: return split_launch<KS1, 4>(g, fields, hgx16, hgy16, coefs, s, peak);
+    case 5: return split_launch<KS1, 5>(g, fields, hgx16, hgy16, coefs, s, peak);
+    case 6: return split_launch<KS1, 6>(g, fields, hgx16, hgy16, coefs, s, peak);
+    default: return hipErrorNotSupported;
+  }
+}
+
+bool overlap_split_supported(const Geometry& g) {
+  return g.G > 0 && g.G <= 96 && g.w % 16 == 0 && g.h % 16 == 0 && g.w >= 16 && g.w <= 160 && g.h >= 16 &&
+         (size_t)g.B <= 65535;
+}
+
+hipError_t launch_overlap_split(const Geometry& g, const float2* fields, const void* hgx16, const void* hgy16,
+                                float2* coefs, hipStream_t s, const unsigned* peak) {
+  if (!overlap_split_supported(g) || !hgx16 || !hgy16) return hipErrorNotSupported;
+  switch ((g.w + 31) / 32) {
+    case 1: return split_dispatch_nt<1>(g, fields, hgx16, hgy16, coefs, s, peak);
+    case 2: return split_dispatch_nt<2>(g, fields, hgx16, hgy16, coefs, s, peak);
+    case 3: return split_dispatch_nt<3>(g, fields, hgx16, hgy16, coefs, s, peak);
+    case 4: return split_dispatch_nt<4>(g, fields, hgx16, hgy16, coefs, s, peak);
+    case 5: return split_dispatch_nt<5>(g, fields, hgx16, hgy16, coefs, s, peak);
+    default: return hipErrorNotSupported;
+  }
+}
+
 static size_t overlap_lds(const Geometry& g, bool split) {
   const int Gp = (g.G + 15) & ~15;
   const size_t e = split ? 2 : 4;
@@ -386,8 +631,13 @@ int overlap_min_tiles_per_wave(const Geometry& g, int split) {
 }
 
 hipError_t launch_overlap_mfma(const Geometry& g, const float2* fields, const float* hgx, const float* hgy,
-                               float2* coefs, int split, hipStream_t s, int tpw, const unsigned* peak) {
+                               float2* coefs, int split, hipStream_t s, int tpw, const unsigned* peak,
+                               const void* hgx16, const void* hgy16) {
   if (g.w % 16 || g.h % kChunk || g.G <= 0) return hipErrorInvalidValue;
+  if (split && hgx16 && hgy16) {
+    const hipError_t e = launch_overlap_split(g, fields, hgx16, hgy16, coefs, s, peak);
+    if (e != hipErrorNotSupported) return e;
+  }
   if (overlap_lds(g, split != 0) > 160 * 1024) return hipErrorInvalidValue;
   return split ? ov_dispatch<true>(g, fields, hgx, hgy, coefs, tpw, s, peak)
                : ov_dispatch<false>(g, fields, hgx, hgy, coefs, tpw, s, nullptr);
