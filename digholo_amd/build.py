@@ -13,6 +13,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(os.path.dirname(PKG), "_build")
 LIB = os.path.join(PKG, "libdigholo.so")
+INCLUDE = os.path.join(os.path.dirname(PKG), "include")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
@@ -51,6 +52,15 @@ def build(verbose: bool = True, jobs: int = 0) -> str:
         obj = os.path.join(BUILD, src + ".o")
         cmds.append(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    # the C++-linkage names of every entry point (what a caller of the
+    # reference header links against), generated from include/digHolo.h
+    from digholo_amd.gen_cxx_aliases import generate
+    alias_src = os.path.join(BUILD, "dh_cxx_aliases.cpp")
+    with open(alias_src, "w") as f:
+        f.write(generate())
+    obj = alias_src + ".o"
+    cmds.append(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-I" + INCLUDE, "-c", alias_src, "-o", obj])
+    objs.append(obj)
     jobs = jobs or min(8, os.cpu_count() or 1)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(lambda c: _run(c, verbose), cmds))

@@ -6,14 +6,17 @@
  * reference declaration it replaces; the citation after each one is
  * reference digHolo.h:line.  Unlike the reference on Linux (whose EXT_C is
  * empty, digHolo.h:148-156, so its .so exports C++-mangled names), these are
- * exported with C linkage.
+ * declared with C linkage; libdigholo.so exports every function under both
+ * names, so a C++ program built against the reference header links too.
+ * Defining DIGHOLO_CXX_LINKAGE before this header declares them with C++
+ * linkage, as the reference header does.
  *
  * Extensions with no reference counterpart are grouped at the end and say so.
  */
 #ifndef DIGHOLO_MI355X_H
 #define DIGHOLO_MI355X_H
 
-#ifdef __cplusplus
+#if defined(__cplusplus) && !defined(DIGHOLO_CXX_LINKAGE)
 #define EXT_C extern "C"
 #else
 #define EXT_C
