@@ -2159,8 +2159,12 @@ int autoalign_tweak(Handle& h, float& goalDb, const float* snapSteps) {
             to_db(o, out[0]);
             std::copy(out[0], out[0] + DIGHOLO_METRIC_COUNT, out[1]);
           }
+          double vx0 = NAN;
           for (int p = 0; p < P; ++p) {
             const float pwr = out[p][goal];
+            if (c.verbosity >= DIGHOLO_VERBOSITY_COOKED)
+              fprintf(stdout, "AutoAlign sample  parameter %d  step %d  pol %d  value %.9g  goal %.7f\n", par, step, p,
+                      (double)dTx[p], (double)pwr);
             xs[p][step] = dTx[p];
             ys[p][step] = pwr;
             if (pwr > maxPwr[p] && valid[p]) {
@@ -2174,8 +2178,9 @@ int autoalign_tweak(Handle& h, float& goalDb, const float* snapSteps) {
             }
             if (step >= sampleCount - 1) {
               const double vx = quadratic_vertex(xs[p], ys[p], step + 1);
+              if (p == 0) vx0 = vx;
               double next;
-              if (std::isfinite(vx)) {
+              if (std::isfinite(vx0)) {   // pol 0's fit decides for every pol, as the reference's test (:12752)
                 const double d = vx - best[p][par];
                 next = std::fabs(d) > stepBound[par] ? best[p][par] + (d > 0 ? stepBound[par] : -stepBound[par]) : vx;
               } else {   // no usable fit: a point near the best so far

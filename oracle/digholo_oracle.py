@@ -1415,6 +1415,251 @@ def autofocus(frames, cfg: Config, scale, centre_x, centre_y):
 
 
 # ---------------------------------------------------------------------------
+# AutoAlign tweak search (digHolo.cpp:12315-12885): one parameter at a time,
+# three samples about the current value, then samples at the vertex of the
+# least-squares quadratic through every sample so far, the best kept
+# ---------------------------------------------------------------------------
+
+AA_WAIST, AA_DEFOCUS, AA_TILTX, AA_CX, AA_TILTY, AA_CY = range(6)      # digHolo.cpp:2938-2943
+_AA_FIELD = {AA_WAIST: "waist", AA_DEFOCUS: "defocus", AA_TILTX: "tilt_x", AA_CX: "beam_centre_x",
+             AA_TILTY: "tilt_y", AA_CY: "beam_centre_y"}
+
+
+def quadratic_vertex(x, y) -> float:
+    """``fitToQuadratic`` (digHolo.cpp:3146-3257): Gauss-Newton steps on
+    y = a x^2 + b x + c, solved by least squares (sgels) each step.  The model is
+    linear in (a, b, c), so the steps reach the least-squares coefficients and
+    the returned maximum is -b / (2 a); NaN when a sample is not finite.  Solved
+    here in float64 on centred, scaled abscissae (the vertex is invariant under
+    that change of variable)."""
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    if not (np.all(np.isfinite(x)) and np.all(np.isfinite(y))):
+        return float("nan")
+    xm = x.mean()
+    sc = np.max(np.abs(x - xm))
+    if not sc > 0:
+        return float("nan")
+    u = (x - xm) / sc
+    A = np.stack([u * u, u, np.ones_like(u)], axis=1)
+    coef, _, rank, _ = np.linalg.lstsq(A, y, rcond=None)
+    if rank < 3 or coef[0] == 0:
+        return float("nan")
+    return float(xm + sc * (-coef[1] / (2 * coef[0])))
+
+
+def window_radius_rec(cfg: Config) -> np.float32:
+    """A third of the smallest tilt magnitude over the pols (digHolo.cpp:11978-11993)."""
+    m = min(float(np.sqrt(F32(F32(cfg.tilt_x[p]) * F32(cfg.tilt_x[p]) + F32(cfg.tilt_y[p]) * F32(cfg.tilt_y[p]))))
+            for p in range(cfg.pol_count))
+    return F32(F32(m) / F32(3.0))
+
+
+def window_radius_max(pixel_size: float, wavelength: float) -> np.float32:
+    """Largest Fourier-window radius any tilt can use (digHolo.cpp:11960-11971): a
+    fraction (sqrt(68) - 2) / 16 of the largest resolvable angle lambda / (2 px)."""
+    wc = F32((-2.0 + math.sqrt(68.0)) / 16.0)
+    wmax = F32(float(F32(wavelength)) / (2.0 * float(F32(pixel_size))))
+    return F32(F32(wc * wmax) / UNIT_ANGLE_F32)
+
+
+def autoalign_tweak(frames: np.ndarray, cfg: Config, tol: float = 0.0, goal: int = METRIC_IL,
+                    independent: bool = False, mul_conj: bool = False, align_tilt: bool = True,
+                    align_defocus: bool = True, align_centre: bool = True, align_waist: bool = True,
+                    fourier_radius: bool = True, max_attempts: int = 20, trace=None):
+    """``digHoloAutoAlign`` in TWEAK mode (digHolo.cpp:12276-12290, then the tweak
+    stage :12315-12885) from the parameters in cfg.
+
+    Steps (TWEAK mode, :12303-12320): tilt 10% of the aperture, centre and waist
+    10% of pol 0's waist, defocus max(0.4, 10% of |defocus|); tried values are
+    clamped to the limits of :12336-12359; samples go to the vertex of the
+    least-squares quadratic through every sample of the sweep, held within ten
+    steps of the best value.  The first evaluation fixes the FFT crops; a tilt
+    sweep also holds the Fourier window of the last full evaluation (only the
+    reference wave follows the tilt, :12620-12645); after each parameter the
+    best values are loaded and the batch reprocessed.  Unlocked parameters of
+    two pols are judged with the other pol's best coefficients (:10852-10871).
+    A goal other than IL first runs an IL pass, then the goal with steps a tenth
+    as large (:12463-12497).  Attempts repeat until the best goal value moves by
+    at most tol (0.1 + 10 tol on an IL pre-pass), or 20 attempts.
+
+    Returns (cfg with the final parameters, final metrics in dB [9], attempts
+    per pass)."""
+    import copy
+    c = copy.deepcopy(cfg)
+    P = c.pol_count
+    FMAX = float(np.finfo(np.float32).max)
+    # on entry the window radius opens to the largest usable one when the tilt is
+    # searched, else to a third of the tilt (:12276-12290)
+    if fourier_radius:
+        c.aperture_deg = float(window_radius_max(c.pixel_size, c.wavelength) if align_tilt else window_radius_rec(c))
+    px = F32(c.pixel_size)
+    lam = F32(c.wavelength)
+    base = [0.0] * 6
+    base[AA_TILTX] = base[AA_TILTY] = F32(F32(c.aperture_deg) * F32(0.1))
+    base[AA_CX] = base[AA_CY] = F32(F32(c.waist[0]) * F32(0.1))
+    base[AA_WAIST] = F32(F32(c.waist[0]) * F32(0.1))
+    base[AA_DEFOCUS] = F32(0.4)
+    for p in range(P):
+        base[AA_DEFOCUS] = max(base[AA_DEFOCUS], F32(abs(F32(c.defocus[p])) * F32(0.1)))
+    tilt_max = F32(F32(lam / F32(F32(2) * px)) / UNIT_ANGLE_F32)
+    lim = {AA_TILTX: (F32(0), tilt_max),
+           AA_TILTY: (-(F32(c.aperture_deg) + tilt_max), F32(c.aperture_deg) + tilt_max),
+           AA_DEFOCUS: (F32(-100), F32(100)),
+           AA_WAIST: (F32(px / F32(3)), F32(px * F32(max(c.frame_height, c.frame_width)))),
+           AA_CX: (-F32(F32(c.frame_width / 2.0) * px), F32(F32(c.frame_width / 2.0) * px)),
+           AA_CY: (-F32(F32(c.frame_height / 2.0) * px), F32(F32(c.frame_height / 2.0) * px))}
+    bound = [F32(b * F32(10)) for b in base]
+    get = lambda p, par: F32(getattr(c, _AA_FIELD[par])[p])
+
+    def put(p, par, v):
+        getattr(c, _AA_FIELD[par])[p] = float(F32(v))
+
+    def locked(par):
+        return ((par in (AA_TILTX, AA_TILTY)) and c.pol_lock_tilt) or (par == AA_DEFOCUS and c.pol_lock_defocus) or \
+            (par == AA_WAIST and c.pol_lock_waist)
+
+    best = [[get(p, i) for i in range(6)] for p in range(2 if P > 1 else 1)]
+    if P == 1:
+        best.append(list(best[0]))
+    st = {"crop": None, "win": None, "last": None}
+
+    def process():
+        cfg_l = apply_pol_locks(c)
+        plan = make_plan(cfg_l)
+        if st["crop"] is not None:
+            plan.cx, plan.cy = list(st["crop"].cx), list(st["crop"].cy)
+        masks = [None] * P
+        if st["win"] is not None:
+            wp, wm = st["win"]
+            plan.w, plan.h, plan.SX, plan.SY = wp.w, wp.h, list(wp.SX), list(wp.SY)
+            masks = list(wm)
+        _, coefs, plan = process_batch(frames, c, hold=Hold(plan, masks))
+        if st["crop"] is None:
+            st["crop"] = plan
+        st["last"] = (plan, [pol_tables(cfg_l, plan, p, mask=masks[p]).mask for p in range(P)])
+        return coefs
+
+    def db(lin):
+        return 10.0 * np.log10(np.asarray(lin, dtype=np.float32).astype(np.float64))
+
+    goal_count = 2 if goal else 1
+    last_tol = 0.1 + tol * 10.0 if goal else tol
+    step = list(base)
+    out_best = [[-FMAX] * 9 for _ in range(2)]
+    backup = None
+    coefs = None
+    attempts_log = []
+    for gp in range(goal_count):
+        g = goal if gp else METRIC_IL
+        converged, first = False, True
+        best_pwr = [-FMAX, -FMAX]
+        last_err = [FMAX, FMAX]
+        out_best[0][g] = out_best[1][g] = -FMAX
+        attempts = 0
+        if gp:
+            step = [F32(b / F32(10)) for b in base]
+            last_tol = tol
+        while not converged:
+            enabled = {AA_TILTX: align_tilt, AA_TILTY: align_tilt, AA_CX: align_centre, AA_CY: align_centre,
+                       AA_WAIST: align_waist, AA_DEFOCUS: align_defocus}
+            for par in range(6):
+                if not enabled[par]:
+                    continue
+                lock = locked(par)
+                var0 = [get(0, par), get(0, par) if lock or P == 1 else get(1, par)]
+                max_pwr = [-FMAX, -FMAX]
+                tilt_par = par in (AA_TILTX, AA_TILTY)
+                if tilt_par and st["last"] is not None:
+                    st["win"] = st["last"]
+                dtx = [[0.0, 0.0] for _ in range(7)]
+                xs = [[], []]
+                ys = [[], []]
+                for k in range(6):
+                    valid = [True, True]
+                    for p in range(P):
+                        if k < 3:
+                            dtx[k][p] = F32(var0[p] + ((k - 1.0) * float(step[par]) / 2.0))
+                        if lock:
+                            dtx[k][1] = dtx[k][0]
+                        dtx[k][p] = min(max(F32(dtx[k][p]), lim[par][0]), lim[par][1])
+                        valid[p] = bool(np.isfinite(dtx[k][p]))
+                        if valid[p]:
+                            put(p, par, dtx[k][p])
+                    if fourier_radius:
+                        c.aperture_deg = float(min(F32(c.aperture_deg), window_radius_rec(c)))
+                    coefs = process()
+                    if tilt_par and st["win"] is None:
+                        st["win"] = st["last"]
+                    if first:
+                        backup = coefs.copy()
+                        first = False
+                    if (not lock and P > 1) or independent:
+                        outs = []
+                        for p in range(P):
+                            if independent:
+                                lin = power_meter(coefs[:, p:p + 1], False, mul_conj)
+                            else:
+                                mix = np.stack([coefs[:, p], backup[:, 1 - p]], axis=1)
+                                lin = power_meter(mix, False, mul_conj)
+                            outs.append(db(lin))
+                    else:
+                        o = db(power_meter(coefs, independent, mul_conj))
+                        outs = [o, o]
+                    vx0 = None
+                    for p in range(P):
+                        pwr = float(F32(outs[p][g]))
+                        xs[p].append(float(dtx[k][p]))
+                        ys[p].append(pwr)
+                        if trace is not None:
+                            trace.append(("sample", par, k, p, float(dtx[k][p]), pwr))
+                        if pwr > max_pwr[p] and valid[p]:
+                            max_pwr[p] = pwr
+                            if pwr > best_pwr[p]:
+                                best[p] = [get(p, i) for i in range(6)]
+                                out_best[p] = [float(F32(v)) for v in outs[p]]
+                                backup[:, p] = coefs[:, p]
+                                best_pwr[p] = pwr
+                        if k >= 2:
+                            vx = quadratic_vertex(xs[p], ys[p])
+                            if p == 0:
+                                vx0 = vx
+                            # the reference tests pol 0's vertex for every pol (:12752)
+                            if np.isfinite(vx0):
+                                d = vx - float(best[p][par])
+                                nxt = float(best[p][par]) + (float(bound[par]) if d > 0 else -float(bound[par])) \
+                                    if abs(d) > float(bound[par]) else vx
+                            else:
+                                nxt = float(best[p][par]) + ((k - 2.0) * float(step[par]) / 4.0)
+                            dtx[k + 1][p] = F32(nxt)
+                st["win"] = None
+                for p in range(P):
+                    for i in range(6):
+                        put(p, i, best[p][i])
+                coefs = process()
+            if first:
+                coefs = process()
+                backup = coefs.copy()
+                first = False
+            backup = coefs.copy()
+            err = 0.0
+            for p in range(P):
+                err += abs(float(out_best[p][g]) - last_err[p])
+                last_err[p] = float(out_best[p][g])
+            err /= P
+            attempts += 1
+            if trace is not None:
+                trace.append((gp, attempts, [list(map(float, b)) for b in best], [o[g] for o in out_best], err))
+            if err <= last_tol or attempts == max_attempts:
+                converged = True
+        attempts_log.append(attempts)
+    st["crop"] = st["win"] = None
+    coefs = process()
+    final = db(power_meter(coefs, independent, mul_conj))
+    return c, final, attempts_log
+
+
+# ---------------------------------------------------------------------------
 # Gradient of the alignment loss (no counterpart in the reference; the FP64
 # restatement of the library's adjoint, pinned by central finite differences
 # with the window placement held)

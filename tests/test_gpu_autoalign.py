@@ -12,7 +12,9 @@ search starts from perturbed values.  Checks:
   reference does, so it can stop a few thousandths of a dB short;
 * the metrics the library reports afterwards equal the oracle's power meter
   applied to the oracle's processing of the same frames at the parameters the
-  library ended on (1e-3 dB), so the search leaves the handle consistent.
+  library ended on (1e-3 dB), so the search leaves the handle consistent;
+* the whole search follows the oracle's restatement of the reference's tweak
+  (oracle.autoalign_tweak) to the same parameters and goal, single and dual pol.
 """
 import numpy as np
 import pytest
@@ -145,3 +147,68 @@ def test_autoalign_mdl_goal_two_passes(lib):
     # a goal other than IL runs an IL pass first, then the goal with finer steps
     spec, cfg, start = small_case()
     run_case(lib, spec, cfg, start, B=30, goal=O.METRIC_MDL, tol=1e-3, check_params=False)
+
+
+def oracle_config(cfg, start):
+    P = cfg["pol_count"]
+    two = lambda v: list(v) if isinstance(v, (tuple, list)) else [v, v]
+    return O.Config(frame_width=cfg["frame_width"], frame_height=cfg["frame_height"], pol_count=P,
+                    fft_window_x=cfg["fft_window"][0], fft_window_y=cfg["fft_window"][1],
+                    aperture_deg=start.get("aperture_deg", 0.7), beam_centre_x=two(start["beam_centre_x"]),
+                    beam_centre_y=two(start["beam_centre_y"]), tilt_x=two(start["tilt_x"]),
+                    tilt_y=two(start["tilt_y"]), defocus=two(start["defocus"]), waist=two(start["waist"]),
+                    group_count=cfg["group_count"], pol_lock_tilt=cfg.get("pol_lock_tilt", 0),
+                    pol_lock_defocus=cfg.get("pol_lock_defocus", 0), pol_lock_waist=cfg.get("pol_lock_waist", 1))
+
+
+@pytest.mark.parametrize("case", ["single", "dual"])
+def test_tweak_trajectory_matches_oracle(lib, case):
+    """The tweak search against the oracle's restatement (oracle.autoalign_tweak,
+    digHolo.cpp:12276-12885) from the same start on the same frames: the same
+    final parameters (tilt 1e-3 deg, centre 0.5 um, waist 0.2 %, defocus 0.01 D)
+    and the same goal metric (1e-3 dB).  Every decision of the search
+    (best-so-far, quadratic vertex, step clamp) compares metrics the two compute
+    to ~1e-5 dB, so they follow one path; vertices of the flat peaks move by a
+    fraction of those tolerances."""
+    if case == "single":
+        spec, cfg, start = small_case()
+        B = 12
+    else:
+        spec = synth.spec_dual_640()
+        spec.group_count = 12
+        spec.tilt_x_deg = [1.50, 1.53]
+        spec.tilt_y_deg = [1.50, 1.46]
+        spec.defocus = [0.3, -0.4]
+        cfg = dict(frame_width=640, frame_height=512, pol_count=2, fft_window=(256, 256), group_count=12)
+        start = dict(tilt_x=(1.53, 1.50), tilt_y=(1.48, 1.49), beam_centre_x=(-1.62e-3, 1.585e-3),
+                     beam_centre_y=(-10e-6, 5e-6), defocus=(0.0, 0.0), waist=330e-6)
+        B = 10
+    frames, _, _, _ = synth.make_batch(spec, B, seed=11)
+    P = cfg["pol_count"]
+    h = lib.digHoloCreate()
+    try:
+        api.configure(lib, h, **{**cfg, **start})
+        lib.digHoloConfigSetAutoAlignMode(h, api.DIGHOLO_AUTOALIGNMODE_TWEAK)
+        lib.digHoloConfigSetAutoAlignTol(h, 0.0)
+        api.process_batch(lib, h, frames)
+        ret = lib.digHoloAutoAlign(h)
+        prm = read_params(lib, h, P)
+    finally:
+        lib.digHoloDestroy(h)
+    oc = oracle_config(cfg, start)
+    c, final, attempts = O.autoalign_tweak(frames, oc, tol=0.0)
+    print(f"{case}: GPU goal {ret:.5f}  oracle goal {final[0]:.5f}  oracle attempts {attempts}  "
+          f"window radius {prm['aperture_deg']:.5f} / {c.aperture_deg:.5f}", flush=True)
+    for p in range(P):
+        print(f"  pol {p}: tilt {prm['tilt_x'][p]:.5f} {prm['tilt_y'][p]:.5f} / {c.tilt_x[p]:.5f} {c.tilt_y[p]:.5f}"
+              f"  centre {prm['beam_centre_x'][p] * 1e6:.3f} {prm['beam_centre_y'][p] * 1e6:.3f} / "
+              f"{c.beam_centre_x[p] * 1e6:.3f} {c.beam_centre_y[p] * 1e6:.3f}  waist {prm['waist'][p] * 1e6:.3f} / "
+              f"{c.waist[p] * 1e6:.3f}  defocus {prm['defocus'][p]:.4f} / {c.defocus[p]:.4f}", flush=True)
+    assert abs(ret - final[0]) < 1e-3
+    assert abs(prm["aperture_deg"] - c.aperture_deg) < 1e-4
+    for p in range(P):
+        assert abs(prm["tilt_x"][p] - c.tilt_x[p]) < 1e-3 and abs(prm["tilt_y"][p] - c.tilt_y[p]) < 1e-3
+        assert abs(prm["beam_centre_x"][p] - c.beam_centre_x[p]) < 0.5e-6
+        assert abs(prm["beam_centre_y"][p] - c.beam_centre_y[p]) < 0.5e-6
+        assert abs(prm["waist"][p] / c.waist[p] - 1) < 2e-3
+        assert abs(prm["defocus"][p] - c.defocus[p]) < 0.01

@@ -1034,3 +1034,27 @@ def test_constant_field_calibration_scales_by_the_conjugate():
     _, c0, _ = O.process_batch(frames, cfg)
     _, c1, _ = O.process_batch(frames, cfg, ref_cal=cal)
     assert np.allclose(c1, np.conj(C) * c0, rtol=1e-9, atol=1e-12 * np.abs(c0).max())
+
+
+def test_quadratic_vertex_and_window_radii():
+    # fitToQuadratic (digHolo.cpp:3146-3257) returns the vertex of the
+    # least-squares parabola; any three samples of a parabola give its vertex
+    for x0, scale in ((2.0, 1.0), (1.2e-4, 1e-5), (-0.4, 0.2)):
+        xs = x0 + scale * np.array([-1.3, 0.2, 0.9, 2.1])
+        ys = -3.0 * ((xs - x0) / scale) ** 2 + 5.0
+        assert O.quadratic_vertex(xs[:3], ys[:3]) == pytest.approx(x0, rel=1e-9, abs=1e-12 * scale)
+        assert O.quadratic_vertex(xs, ys) == pytest.approx(x0, rel=1e-9, abs=1e-12 * scale)
+    # least squares through noisy samples: numpy's polyfit on the same points
+    rng = np.random.default_rng(3)
+    xs = np.linspace(-1, 1, 6)
+    ys = -(xs - 0.3) ** 2 + 0.05 * rng.standard_normal(6)
+    a, b, _ = np.polyfit(xs, ys, 2)
+    assert O.quadratic_vertex(xs, ys) == pytest.approx(-b / (2 * a), rel=1e-9)
+    assert np.isnan(O.quadratic_vertex([1.0, 2.0, np.nan], [0.0, 1.0, 0.0]))
+    assert np.isnan(O.quadratic_vertex([1.0, 1.0, 1.0], [0.0, 1.0, 0.0]))
+    # window radii (digHolo.cpp:11960-11993): (sqrt(68) - 2)/16 of lambda/(2 px) in
+    # degrees, and a third of the smallest tilt magnitude
+    r = O.window_radius_max(20e-6, 1565e-9)
+    assert float(r) == pytest.approx((np.sqrt(68) - 2) / 16 * 1565e-9 / 40e-6 * 180 / np.pi, rel=1e-6)
+    cfg = small_cfg(pol_count=2, tilt_x=[3.0, 1.2], tilt_y=[4.0, 0.5])
+    assert float(O.window_radius_rec(cfg)) == pytest.approx(1.3 / 3, rel=1e-6)
