@@ -613,19 +613,24 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+typedef __fp16 fp16x2 __attribute__((ext_vector_type(2)));
 
 // (xr, xi) split into fp16 pairs hi = (fp16(xr), fp16(xi)) and
 // lo = (fp16(xr - hi.x), fp16(xi - hi.y)), both rounded to nearest even: the
 // residuals are exact in FP32, and v_fma_mix forms them from the fp16 halves
 // of hi and rounds them to fp16 in one instruction each (the same values as
 // converting, subtracting and converting again, in three instructions
-// instead of eight).
+// instead of eight).  hi is an ordinary conversion, so the compiler sees it
+// read xr and xi and keeps the wait states an MFMA result needs before a VALU
+// reads it; the inline v_fma_mix depend on hi and so come after it (the
+// compiler does not check the operands of inline assembly for that hazard).
 __device__ __forceinline__ void split_f16x2(float xr, float xi, unsigned& hi, unsigned& lo) {
-  asm("v_cvt_pk_f16_f32 %0, %2, %3\n\t"
-      "v_fma_mixlo_f16 %1, %0, -1.0, %2 op_sel_hi:[1,0,0]\n\t"
-      "v_fma_mixhi_f16 %1, %0, -1.0, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
-      : "=&v"(hi), "=&v"(lo)
-      : "v"(xr), "v"(xi));
+  const half2v hv = {(_Float16)xr, (_Float16)xi};
+  hi = __builtin_bit_cast(unsigned, hv);
+  asm("v_fma_mixlo_f16 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]\n\t"
+      "v_fma_mixhi_f16 %0, %1, -1.0, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+      : "=&v"(lo)
+      : "v"(hi), "v"(xr), "v"(xi));
 }
 
 constexpr int kW3Threads = 256;
@@ -633,8 +638,13 @@ constexpr int kW3Threads = 256;
 template <int L, int M, int U, int J>
 struct W3Shape {
   static constexpr int NR = L * M;            // complex points per row transform
-  static constexpr int RPW = 64 / L;          // rows per wave per round
-  static constexpr int RR = 4 * RPW;          // rows per round
+  // MR: the row transform on MFMA (128-point rows: a 16-point DFT, then per
+  // residue pair one matrix for the 8-point DFT, its twiddles and the r2c
+  // unpack), in rounds of 16 rows; the exchange holds whole rows, grouped by
+  // residue pair, and it and the B buffer are double-buffered
+  static constexpr bool MR = L == 8 && M == 16;
+  static constexpr int RPW = 64 / L;          // rows per wave per round (register row transform)
+  static constexpr int RR = MR ? 16 : 4 * RPW;   // rows per round
   static constexpr int KS = RR / 16;          // MFMA k-steps per round
   static constexpr int S = M / L / 2;         // (a, M - a) pairs per lane
   static constexpr int w = 16 * U, h = 16 * J;
@@ -642,10 +652,28 @@ struct W3Shape {
   static constexpr int LP = M / 2 + 2;        // exchange: complex per lane (half)
   static constexpr int RPc = L * LP + (8 - (L * LP) % 16 + 16) % 16;   // per row, = 8 mod 16
   static constexpr int PK = 2 * RR + 8;       // halfs per B column
-  static constexpr int EBYTES = 4 * RPW * RPc * 8;
+  static constexpr int EPP = 16 * 8 + 16;     // MR exchange: bytes per (row, pair), 16 complex + pad
+  static constexpr int ERP = 8 * EPP + 16;    // MR exchange: bytes per row
+  static constexpr int EBYTES = MR ? 2 * RR * ERP : 4 * RPW * RPc * 8;
   static constexpr int BBYTES = 2 * 2 * (w + 1) * PK * 2;   // two buffers; column w takes discarded values
   static constexpr int WP = w + 4;            // field-plane row pitch (complex)
 };
+
+constexpr int kMrMats = 10;
+constexpr int kMrMatHalfs = 2 * 2 * 64 * 8;
+constexpr double kMrScaleA = 1.0 / 32.0;
+
+// the column k of output o of residue pair v
+__host__ __device__ inline int kmr(int v, int o) {
+  if (v == 0 || v == 8) {
+    if (o == 0) return v == 8 ? 128 : 0;
+    return o < 8 ? 16 * o : 8 + 16 * (o - 8);
+  }
+  return o < 8 ? v + 16 * o : 16 - v + 16 * (o - 8);
+}
+
+// offset (complex elements, 16-byte aligned) of the MR fragments in the table
+__host__ __device__ inline int w3_mr_off(int H, int L, int M) { return (H + L * (M + 1) + M + 1) & ~1; }
 
 // LDS bytes of the aliased region: the row phase or the field plane
 __host__ __device__ inline int w3_region(int ebytes, int bbytes, int H, int L, int M, int w, int h) {
@@ -1037,8 +1065,29 @@ __device__ __forceinline__ void w3_body(const W3Args& a, const unsigned short* _
   const unsigned* src = reinterpret_cast<const unsigned*>(frames + ((size_t)b * g.frameH + g.cy[p]) * g.frameW +
                                                           g.cx[p]) + l + (size_t)(wave * RPW + r) * fw2;
   unsigned pre[M];
+  // MR: pass-A operand words of this wave's two tiles of the round; tile t
+  // holds rows 4 wave + 2t + (0, 1) of the round, tile row lr = 8 r' + l being
+  // sequence l of row r'.  Lane (lr, q) holds the words l + 32 q + 8 e (e =
+  // 0..3) of its row, i.e. z[l + 8 m], m = 4q + e: the 8 lanes of a row read
+  // 32 contiguous bytes.
+  unsigned fr[2][4];
+  const unsigned* srcMR = reinterpret_cast<const unsigned*>(frames + ((size_t)b * g.frameH + g.cy[p]) * g.frameW +
+                                                            g.cx[p]) + 32 * q + (lr & 7) +
+                          (size_t)(4 * wave + (lr >> 3)) * fw2;
+  auto load_mr = [&](int rd) {
 #pragma unroll
-  for (int m = 0; m < M; ++m) pre[m] = __builtin_nontemporal_load(src + L * m);
+    for (int t = 0; t < 2; ++t) {
+      const unsigned* s = srcMR + (size_t)(rd * RR + 2 * t) * fw2;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) fr[t][e] = __builtin_nontemporal_load(s + 8 * e);
+    }
+  };
+  if constexpr (Sh::MR) {
+    load_mr(0);
+  } else {
+#pragma unroll
+    for (int m = 0; m < M; ++m) pre[m] = __builtin_nontemporal_load(src + L * m);
+  }
 
   for (int e = tid; e < H; e += NT) {
     const uint2 t = reinterpret_cast<const uint2*>(tabs)[e];
@@ -1051,20 +1100,23 @@ __device__ __forceinline__ void w3_body(const W3Args& a, const unsigned short* _
   // the inter-pass twiddles carry the power-of-two scale of the B operand, so
   // pass B and the unpack produce scaled values (v[0] is scaled on its own)
   const float sc = 0.5f * a.xscale;           // ev + w od = 2 X
-  for (int e = tid; e < L * (M + 1); e += NT) twA[e] = cscale(tabs[H + e], sc);
+  if constexpr (!Sh::MR)
+    for (int e = tid; e < L * (M + 1); e += NT) twA[e] = cscale(tabs[H + e], sc);
   for (int e = tid; e < w; e += NT) twix[e] = a.ix.tw[e];
   for (int e = tid; e < h; e += NT) twiy[e] = a.iy.tw[e];
 
   // this lane's a values in pass B and exp(-2 pi i a / W) for them
   int alo[S], ahi[S];
   float2 wlo[S], whi[S];
+  if constexpr (!Sh::MR) {
 #pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const int pi = l * S + s;
-    alo[s] = pi;
-    ahi[s] = pi == 0 ? M / 2 : M - pi;
-    wlo[s] = tabs[H + L * (M + 1) + alo[s]];
-    whi[s] = tabs[H + L * (M + 1) + ahi[s]];
+    for (int s = 0; s < S; ++s) {
+      const int pi = l * S + s;
+      alo[s] = pi;
+      ahi[s] = pi == 0 ? M / 2 : M - pi;
+      wlo[s] = tabs[H + L * (M + 1) + alo[s]];
+      whi[s] = tabs[H + L * (M + 1) + ahi[s]];
+    }
   }
   // window rows of this lane's MFMA units; the twiddle index ky * y mod H of
   // the lane's first k-slot, carried from k-step to k-step (y advances by 16)
@@ -1239,7 +1291,10 @@ __device__ __forceinline__ void w3_body(const W3Args& a, const unsigned short* _
 #pragma unroll
           for (int uu = 0; uu < UC; ++uu) {
             if (u0 + uu < U) {
-              const _Float16* bq = Bs + (16 * (u0 + uu) + lr) * PK + 32 * ks + 8 * q;
+              // MR: column c at position (c mod 16) U + c / 16, so the pair stage's
+              // columns (16 apart) are written to consecutive positions
+              const int bcol = Sh::MR ? lr * U + u0 + uu : 16 * (u0 + uu) + lr;
+              const _Float16* bq = Bs + bcol * PK + 32 * ks + 8 * q;
               bh[uu] = *reinterpret_cast<const half8*>(bq);
               bl[uu] = *reinterpret_cast<const half8*>(bq + (w + 1) * PK);
             }
@@ -1269,17 +1324,158 @@ __device__ __forceinline__ void w3_body(const W3Args& a, const unsigned short* _
     }
   };
 
-  // Two B buffers: round rd's rows go to buffer rd & 1, so one barrier per
-  // round separates the writers of a buffer from its readers (a buffer is
-  // rewritten two rounds later, after every wave has passed the barrier that
-  // follows its last reads).
-  __syncthreads();   // tables staged
-  rows(0, Bs0);
-  __syncthreads();
-  for (int rd = 0; rd < nrd; ++rd) {
-    if (rd + 1 < nrd) rows(rd + 1, Bs0 + ((rd + 1) & 1) * BSTRIDE);
-    gemm(rd, Bs0 + (rd & 1) * BSTRIDE);
+  if constexpr (Sh::MR) {
+    // Row transforms on MFMA.  Pass A: Y[l][a] = sum_m z[l + 8m] w16^(a m) for
+    // the 16-row tiles (l, round), the frame samples offset by -2048 so that
+    // 12-bit values are exact in fp16 (a wave whose words hold larger values
+    // adds the exact fp16 remainder as a third product).  Pass A leaves its
+    // output in the exchange, grouped by residue pair; pass B multiplies each
+    // pair's 16 Y values of a row by the pair's matrix (8-point DFT, twiddles,
+    // r2c unpack), which yields the row spectrum at 16 columns, straight in
+    // the B buffer's layout (lane (o, q) holds rows 4q .. 4q + 3 of column o).
+    // Round i's exchange and B buffer are buffer i & 1: one barrier per round.
+    const _Float16* mrf = reinterpret_cast<const _Float16*>(tabs + w3_mr_off(H, L, M));
+    const bool nyq = c0 + w > NR;   // the window holds column NR (then not column 0)
+    half8 bA[2][2], bP[2][2][2];    // fragments: 16-point DFT, the wave's pairs wave and wave + 4
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+      for (int hl = 0; hl < 2; ++hl) bA[nt][hl] = *reinterpret_cast<const half8*>(mrf + (nt * 2 + hl) * 512 + lane * 8);
+    int kc[2], kpos[2];   // window column of this lane's pass-B output, its B buffer position
+    float dca[2];
+    // the offset taken off every sample leaves column 0 short by 2048 W
+    const float dcfix = 2048.f * (float)(2 * NR) * a.xscale;
+#pragma unroll
+    for (int pp = 0; pp < 2; ++pp) {
+      const int v = (wave + 4 * pp == 0 && nyq) ? 8 : wave + 4 * pp;
+      const _Float16* m = mrf + (1 + v) * kMrMatHalfs + lane * 8;
+#pragma unroll
+      for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int hl = 0; hl < 2; ++hl) bP[pp][nt][hl] = *reinterpret_cast<const half8*>(m + (nt * 2 + hl) * 512);
+      const int k = kmr(v, lr);
+      kc[pp] = k - c0;
+      kpos[pp] = (unsigned)kc[pp] < (unsigned)w ? (kc[pp] & 15) * U + (kc[pp] >> 4) : w;   // w: discarded
+      dca[pp] = k == 0 ? dcfix : 0.f;
+    }
+    // this lane's pass-A column a = lr in the exchange: pair and position in it
+    const int pa = (lr == 0 || lr == 8) ? 0 : (lr < 8 ? lr : 16 - lr);
+    const int wa = lr >= 8 ? 1 : 0;
+    constexpr int EBUF = RR * Sh::ERP;
+    const floatx4 zero4 = {0.f, 0.f, 0.f, 0.f};
+
+    // pass A of round rd (fr holds its words; round rd + 1's are loaded here)
+    // into exchange buffer rd & 1
+    auto passA = [&](int rd) {
+      half8 ah[2], al[2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        uint4 hw, lw;
+        unsigned* hp = reinterpret_cast<unsigned*>(&hw);
+        unsigned* lp = reinterpret_cast<unsigned*>(&lw);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const float x0 = (float)(fr[t][e] & 0xffffu) - 2048.f, x1 = (float)(fr[t][e] >> 16) - 2048.f;
+          const fp16x2 hv = __builtin_amdgcn_cvt_pkrtz(x0, x1);
+          hp[e] = __builtin_bit_cast(unsigned, hv);
+          // the remainder: zero for 12-bit samples, else an integer below 32 in
+          // magnitude (exact)
+          const fp16x2 lv = __builtin_amdgcn_cvt_pkrtz(x0 - (float)hv.x, x1 - (float)hv.y);
+          lp[e] = __builtin_bit_cast(unsigned, lv);
+        }
+        ah[t] = __builtin_bit_cast(half8, hw);
+        al[t] = __builtin_bit_cast(half8, lw);
+      }
+      load_mr(rd + 1 < nrd ? rd + 1 : nrd - 1);   // (past the end: a row already read, unused)
+      floatx4 acc2[2][2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) acc2[t][nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[t], bA[nt][0], zero4, 0, 0, 0);
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+          acc2[t][nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[t], bA[nt][1], acc2[t][nt], 0, 0, 0);
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+          acc2[t][nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[t], bA[nt][0], acc2[t][nt], 0, 0, 0);
+      // lane (a, q) holds Y[l][a] of tile rows 4q + e: round row 4 wave + 2t +
+      // (q >> 1), l = 4 (q & 1) + e
+      unsigned char* eb = rowBase + (rd & 1) * EBUF + (4 * wave + (q >> 1)) * Sh::ERP + pa * Sh::EPP + wa * 8 +
+                          64 * (q & 1);
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          *reinterpret_cast<float2*>(eb + 2 * t * Sh::ERP + 16 * e) = make_float2(acc2[t][0][e], acc2[t][1][e]);
+    };
+    // pass B of round rd (exchange buffer rd & 1), pairs wave and wave + 4;
+    // the window columns go to B buffer rd & 1 as fp16 hi / lo
+    auto pairs = [&](int rd) {
+      const unsigned char* eb = rowBase + (rd & 1) * EBUF + lr * Sh::ERP + 32 * q;
+      _Float16* Bs = Bs0 + (rd & 1) * BSTRIDE;
+#pragma unroll
+      for (int pp = 0; pp < 2; ++pp) {
+        const float4* ep = reinterpret_cast<const float4*>(eb + (wave + 4 * pp) * Sh::EPP);
+        const float4 e0 = ep[0], e1 = ep[1];
+        uint4 hw, lw;
+        split_f16x2(e0.x, e0.y, hw.x, lw.x);
+        split_f16x2(e0.z, e0.w, hw.y, lw.y);
+        split_f16x2(e1.x, e1.y, hw.z, lw.z);
+        split_f16x2(e1.z, e1.w, hw.w, lw.w);
+        const half8 ah = __builtin_bit_cast(half8, hw), al = __builtin_bit_cast(half8, lw);
+        floatx4 c[2];
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) c[nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bP[pp][nt][0], zero4, 0, 0, 0);
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) c[nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bP[pp][nt][1], c[nt], 0, 0, 0);
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) c[nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bP[pp][nt][0], c[nt], 0, 0, 0);
+        uint4 xh, xl;
+        unsigned* hp = reinterpret_cast<unsigned*>(&xh);
+        unsigned* lp = reinterpret_cast<unsigned*>(&xl);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) split_f16x2(c[0][e] + dca[pp], c[1][e], hp[e], lp[e]);
+        _Float16* bp = Bs + kpos[pp] * PK + 8 * q;
+        *reinterpret_cast<uint4*>(bp) = xh;
+        *reinterpret_cast<uint4*>(bp + (w + 1) * PK) = xl;
+      }
+    };
+
+    // (nrd >= 2: H is at least 32)
+    __syncthreads();   // tables staged
+    passA(0);
+    passA(1);
     __syncthreads();
+    pairs(0);
+    __syncthreads();
+    for (int rd = 0; rd + 2 < nrd; ++rd) {
+      pairs(rd + 1);
+      gemm(rd, Bs0 + (rd & 1) * BSTRIDE);
+      passA(rd + 2);
+      __syncthreads();
+    }
+    pairs(nrd - 1);
+    gemm(nrd - 2, Bs0 + (nrd & 1) * BSTRIDE);
+    __syncthreads();
+    gemm(nrd - 1, Bs0 + ((nrd - 1) & 1) * BSTRIDE);
+    __syncthreads();
+  } else {
+    // Two B buffers: round rd's rows go to buffer rd & 1, so one barrier per
+    // round separates the writers of a buffer from its readers (a buffer is
+    // rewritten two rounds later, after every wave has passed the barrier that
+    // follows its last reads).
+    __syncthreads();   // tables staged
+    rows(0, Bs0);
+    __syncthreads();
+    for (int rd = 0; rd < nrd; ++rd) {
+      if (rd + 1 < nrd) rows(rd + 1, Bs0 + ((rd + 1) & 1) * BSTRIDE);
+      gemm(rd, Bs0 + (rd & 1) * BSTRIDE);
+      __syncthreads();
+    }
   }
 
   // The epilogue's first row pass needs the window factor and its first
@@ -1494,6 +1690,69 @@ __global__ void k_w3_tables(int H, int L, int M, float2* __restrict__ tab) {
   tab[blockIdx.x * blockDim.x + threadIdx.x] = make_float2((float)cs, (float)sn);
 }
 
+// MFMA fragments of the MR row transform (128-point rows), from FP64, as fp16
+// hi/lo pairs laid out [matrix][n-tile][hi, lo][lane][8 k-slots]: lane
+// (n, q) of a 16x16x32 B operand holds column n, k-slots 8q .. 8q + 7.
+//
+// Matrix 0, the 16-point DFT of pass A (times 2^-5): k-slot 8q + j is part
+// j & 1 (re, im) of z[l + 8 m], m = 4q + (j >> 1); column a of n-tile 0 / 1 is
+// Re / Im of Y[l][a] = sum_m z[l + 8 m] w16^(a m).
+//
+// Matrices 1 + v, pass B of residue pair v (v = 0..7; 8: pair 0 with the
+// Nyquist column in place of column 0): k-slot 8q + j is part j & 1 of
+// Y[l][a], complex index c = 4q + (j >> 1), l = c >> 1, a the pair's residue
+// c & 1 (pair v >= 1: v, 16 - v; pair 0: 0, 8).  Column o of n-tile 0 / 1 is
+// Re / Im of X[k], k = kmr(v, o), with Z[k] = sum_l w128^(l k) Y[l][k mod 16]
+// and X[k] = (Z[k] + conj Z[128 - k]) / 2 - i w256^k (Z[k] - conj Z[128 - k]) / 2,
+// times xscale / 2^-5: the r2c unpack of the 256 real samples packed into
+// z[n] = f[2n] + i f[2n + 1].  (w_N = exp(-2 pi i / N))
+__global__ void k_w3_mr_tables(double pairScale, _Float16* __restrict__ dst) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= kMrMats * 2 * 64 * 8) return;
+  const int j = t & 7, lane = (t >> 3) & 63, nt = (t >> 9) & 1, mat = t >> 10;
+  const int q = lane >> 4, n = lane & 15, pin = j & 1;
+  double v;
+  if (mat == 0) {
+    const int m = 4 * q + (j >> 1);
+    double s, c;
+    sincospi(2.0 * ((n * m) % 16) / 16.0, &s, &c);
+    v = nt == 0 ? (pin ? s : c) : (pin ? c : -s);
+    v *= kMrScaleA;
+  } else {
+    const int pv = mat - 1, p = pv == 8 ? 0 : pv;
+    const int ci = 4 * q + (j >> 1), l = ci >> 1, which = ci & 1;
+    const int ain = p == 0 ? (which ? 8 : 0) : (which ? 16 - p : p);
+    const int k = kmr(pv, n), kp = (128 - k) & 127;
+    // dZ = w128^(l k) Y for the unit Y = 1 (re) or i (im)
+    auto dz = [&](int kk, double& re, double& im) {
+      re = im = 0.0;
+      if ((kk & 15) != ain) return;
+      double s, c;
+      sincospi(-2.0 * ((l * kk) % 128) / 128.0, &s, &c);   // w128^(l kk) = c + i s
+      if (pin == 0) { re = c; im = s; }
+      else { re = -s; im = c; }
+    };
+    double zr, zi, pr, pi;
+    dz(k, zr, zi);
+    dz(kp, pr, pi);
+    pi = -pi;                                   // conj Z[128 - k]
+    const double er = 0.5 * (zr + pr), ei = 0.5 * (zi + pi);
+    const double dr = 0.5 * (zr - pr), di = 0.5 * (zi - pi);
+    // -i w256^k (d): w256^k = c - i s
+    double s, c;
+    sincospi(2.0 * (k % 256) / 256.0, &s, &c);
+    const double wr = c, wi = -s;
+    const double mr = di, mi = -dr;            // -i d
+    const double xr = er + wr * mr - wi * mi, xi = ei + wr * mi + wi * mr;
+    v = (nt == 0 ? xr : xi) * pairScale;
+  }
+  const _Float16 hi = (_Float16)v;
+  const _Float16 lo = (_Float16)(v - (double)hi);
+  _Float16* m = dst + (size_t)mat * kMrMatHalfs + (size_t)nt * (2 * 64 * 8);
+  m[lane * 8 + j] = hi;
+  m[64 * 8 + lane * 8 + j] = lo;
+}
+
 // (L, M) of a row transform of nr complex points; false if not specialised
 static bool w3_lm(int nr, int& L, int& M) {
   if (nr == 128) { L = 8; M = 16; return true; }
@@ -1502,17 +1761,31 @@ static bool w3_lm(int nr, int& L, int& M) {
   return false;
 }
 
+// the power of two keeping the row spectra of uint16 frames inside fp16:
+// |X| <= W 65535 < 2^(16 + log2 W), scaled into [2^14, 2^15) at most
+static float w3_xscale(int W) {
+  int lw = 0;
+  while ((1 << lw) < W) ++lw;
+  return ldexpf(1.f, 15 - (16 + lw));
+}
+
 size_t window_v3_table_elems(const Geometry& g) {
   int L, M;
   if (!w3_lm(g.W / 2, L, M)) return 0;
+  if (L == 8 && M == 16) return (size_t)w3_mr_off(g.H, L, M) + kMrMats * kMrMatHalfs / 4;
   return (size_t)g.H + (size_t)L * (M + 1) + M;
 }
 
 hipError_t launch_window_v3_tables(const Geometry& g, float2* tab, hipStream_t s) {
   int L, M;
   if (!w3_lm(g.W / 2, L, M)) return hipErrorNotSupported;
-  const int n = (int)window_v3_table_elems(g);
+  const int n = g.H + L * (M + 1) + M;
   k_w3_tables<<<(n + 255) / 256, 256, 0, s>>>(g.H, L, M, tab);
+  if (L == 8 && M == 16) {
+    const int nf = kMrMats * 2 * 64 * 8;
+    k_w3_mr_tables<<<(nf + 255) / 256, 256, 0, s>>>((double)w3_xscale(g.W) / kMrScaleA,
+                                                     reinterpret_cast<_Float16*>(tab + w3_mr_off(g.H, L, M)));
+  }
   return hipGetLastError();
 }
 
@@ -1588,6 +1861,7 @@ hipError_t launch_hg_split(const Geometry& g, const float* hgx, const float* hgy
 static bool w3_common(const Geometry& g, int& L, int& M) {
   if (!w3_lm(g.W / 2, L, M)) return false;
   if (g.w % 16 || g.h % 16 || g.H > 1024) return false;
+  if (L == 8 && M == 16 && g.H < 32) return false;   // the MFMA row path runs at least two 16-row rounds
   if ((g.frameW & 1) || (g.cx[0] & 1) || (g.P > 1 && (g.cx[1] & 1))) return false;   // dword row loads
   for (int p = 0; p < g.P; ++p)
     if (g.c0[p] < 0 || g.c0[p] + g.w > g.W / 2 + 1) return false;

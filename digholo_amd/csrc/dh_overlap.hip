@@ -39,15 +39,21 @@ typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef _Float16 half4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+
 // (xr, xi) -> fp16 pairs hi = (fp16(xr), fp16(xi)), lo = fp16 of the exact
 // residuals, both rounded to nearest even (cvt_pk + fma_mix: the values of
-// convert / subtract / convert).
+// convert / subtract / convert).  hi is an ordinary conversion, so the
+// compiler keeps the wait states an MFMA result needs before a VALU reads it;
+// the inline v_fma_mix depend on hi and so come after it (inline assembly
+// operands are not checked for that hazard).
 __device__ __forceinline__ void split_pair(float xr, float xi, unsigned& hi, unsigned& lo) {
-  asm("v_cvt_pk_f16_f32 %0, %2, %3\n\t"
-      "v_fma_mixlo_f16 %1, %0, -1.0, %2 op_sel_hi:[1,0,0]\n\t"
-      "v_fma_mixhi_f16 %1, %0, -1.0, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
-      : "=&v"(hi), "=&v"(lo)
-      : "v"(xr), "v"(xi));
+  const half2v hv = {(_Float16)xr, (_Float16)xi};
+  hi = __builtin_bit_cast(unsigned, hv);
+  asm("v_fma_mixlo_f16 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]\n\t"
+      "v_fma_mixhi_f16 %0, %1, -1.0, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+      : "=&v"(lo)
+      : "v"(hi), "v"(xr), "v"(xi));
 }
 
 constexpr int kChunk = 16;          // field rows per chunk (K of stage 2)

@@ -5,9 +5,12 @@ reference.
 Three handle configurations of the uneven-batch workload (320x256 / 128 window
 / 10 groups, 640x512 / 192 / 30, 1024x1024 / 256 / 45; dual pol, uint16 frames
 in HBM), at small batch counts (one launch of the multi-batch window kernel):
-* every call after the capture reproduces, bit for bit, the coefficients
-  and fields each handle computes on its own (outputs cleared between
-  calls, eager torch work and synchronisations in between);
+* every call after the capture reproduces the coefficients and fields each
+  handle computes on its own (outputs cleared between calls, eager torch work
+  and synchronisations in between): the merged launch runs the same kernel
+  body compiled into another kernel, whose floating-point contraction may
+  differ from the handle's own kernel by an ulp, so it is held to 1e-6 of the
+  largest value, and every replay to the first one bit for bit;
 * above the one-launch limit the handles' own kernels run one after the other,
   again bit for bit, ordered against work on the handles' own streams;
 * a configuration change re-captures, and the replay again matches;
@@ -72,6 +75,11 @@ def outputs(lib, h):
     return fields, (b.value, p.value, hh.value, w.value)
 
 
+def close(a, b, rel=1e-6):
+    """a equals b to FP32 rounding: max |a - b| <= rel max |b|."""
+    return float((a - b).abs().max()) <= rel * float(b.abs().max())
+
+
 def individual(lib, h):
     import torch
     b, m, p = C.c_int(), C.c_int(), C.c_int()
@@ -99,6 +107,7 @@ def test_multi_graph_replays_match_individual_batches(lib):
             print(f"handle {h}: {W}x{H} window {win} G {G} B {B}: coefs {tuple(c.shape)} fields {tuple(f.shape)}",
                   flush=True)
         ids = (C.c_int * len(hs))(*hs)
+        first = None
         for rep in range(4):
             for c, f in views:          # clear the outputs so a replay that did nothing fails
                 c.zero_()
@@ -108,8 +117,12 @@ def test_multi_graph_replays_match_individual_batches(lib):
             assert lib.digHoloProcessBatchMultiDevice(len(hs), ids) == 0
             assert lib.digHoloSynchronize(hs[0]) == 0
             for (c, f), (cr, fr_) in zip(views, ref):
-                assert torch.equal(c, cr), f"replay {rep}: coefficients differ"
-                assert torch.equal(f, fr_), f"replay {rep}: fields differ"
+                assert close(c, cr), f"replay {rep}: coefficients differ"
+                assert close(f, fr_), f"replay {rep}: fields differ"
+            if first is None:
+                first = [(c.clone(), f.clone()) for c, f in views]
+            for (c, f), (c0, f0) in zip(views, first):
+                assert torch.equal(c, c0) and torch.equal(f, f0), f"replay {rep} differs from replay 0"
             print(f"replay {rep} matches ({x.item():.1f})", flush=True)
         # a changed configuration re-captures
         lib.digHoloConfigSetTilt(hs[1], 0, 0, 1.48)
@@ -121,8 +134,8 @@ def test_multi_graph_replays_match_individual_batches(lib):
         torch.cuda.synchronize()
         assert lib.digHoloProcessBatchMultiDevice(len(hs), ids) == 0
         assert lib.digHoloSynchronize(hs[0]) == 0
-        assert torch.equal(views[1][0], want[0]) and torch.equal(views[1][1], want[1])
-        assert torch.equal(views[0][0], ref[0][0]) and torch.equal(views[2][1], ref[2][1])
+        assert close(views[1][0], want[0]) and close(views[1][1], want[1])
+        assert close(views[0][0], ref[0][0]) and close(views[2][1], ref[2][1])
     finally:
         for h in hs:
             lib.digHoloDestroy(h)
