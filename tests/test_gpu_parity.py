@@ -393,3 +393,49 @@ def test_distance_from_reference_emulation(lib, precision):
           f"exact vs emulation fields {ef0:.3e} coefs {ec0:.3e}", flush=True)
     assert ef <= 1e-4 and ec <= 1e-4
     assert abs(ef - ef0) <= 0.02 * ef0 and abs(ec - ec0) <= 0.02 * ec0
+
+
+def _tilt_for_c0(lib, cfg, target_c0):
+    """The x tilt (degrees) whose Fourier window starts at column target_c0 of
+    the r2c half plane (window start SX - W/2), from the host-only plan."""
+    h = lib.digHoloCreate()
+    try:
+        for t in np.arange(0.30, 2.20, 0.001):
+            c = dict(cfg, tilt_x=(float(t), float(t)))
+            api.configure(lib, h, **c)
+            geo = api.window_geometry(lib, h)
+            if all(sx - geo["W"] // 2 == target_c0 for sx in geo["SX"]):
+                return float(t)
+    finally:
+        lib.digHoloDestroy(h)
+    return None
+
+
+@pytest.mark.parametrize("case", ["16bit", "c0=0"])
+def test_parity_mfma_rows_edge_cases(lib, case):
+    """The MFMA row transform of 256-wide windows: samples above 12 bits (an
+    exact fp16 remainder product) and a window that starts at column 0 (the
+    column-0 sum of the -2048 sample offset restored).  (The window planner
+    keeps Fourier windows below the Nyquist column on these geometries: their
+    start is even and at most W/2 - w.)"""
+    _, cfg = make_case("dual640", 2, seed=3)
+    spec_over = {}
+    if case != "16bit":
+        t = _tilt_for_c0(lib, cfg, 0)
+        if t is None:
+            pytest.skip("no tilt gives that window start")
+        cfg["tilt_x"] = (t, t)
+        spec_over = dict(tilt_x_deg=[t, t])
+    frames, cfg = make_case("dual640", 2, seed=3, spec=spec_over, **{k: v for k, v in cfg.items() if k == "tilt_x"})
+    if case == "16bit":
+        frames = (frames.astype(np.uint32) * 16).astype(np.uint16)   # up to 65520: every sample above 4096 possible
+        assert frames.max() > 4096
+    h = lib.digHoloCreate()
+    api.configure(lib, h, **cfg)
+    assert lib.digHoloConfigSetFusedWindow(h, 3) == 0
+    coefs, fields, _, _, h = run_gpu(lib, frames, cfg, h=h)
+    assert lib.digHoloGetWindowPath(h) == 3
+    geo = api.window_geometry(lib, h)
+    print(f"case {case}: tilt {cfg.get('tilt_x')} window starts {[sx - geo['W'] // 2 for sx in geo['SX']]}", flush=True)
+    check(frames, cfg, coefs, fields)
+    lib.digHoloDestroy(h)
