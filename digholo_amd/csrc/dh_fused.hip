@@ -1367,24 +1367,26 @@ __device__ __forceinline__ void w3_body(const W3Args& a, const unsigned short* _
     // pass A of round rd (fr holds its words; round rd + 1's are loaded here)
     // into exchange buffer rd & 1
     auto passA = [&](int rd) {
-      half8 ah[2], al[2];
+      unsigned any = 0;
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) any |= fr[t][e];
+      // a sample >= 4096 in the wave's words: the exact fp16 remainder as a third product
+      const bool big = __builtin_amdgcn_ballot_w64((any & 0xF000F000u) != 0u) != 0;
+      half8 ah[2];
+      float xs[2][4][2];
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
-        uint4 hw, lw;
+        uint4 hw;
         unsigned* hp = reinterpret_cast<unsigned*>(&hw);
-        unsigned* lp = reinterpret_cast<unsigned*>(&lw);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          const float x0 = (float)(fr[t][e] & 0xffffu) - 2048.f, x1 = (float)(fr[t][e] >> 16) - 2048.f;
-          const fp16x2 hv = __builtin_amdgcn_cvt_pkrtz(x0, x1);
-          hp[e] = __builtin_bit_cast(unsigned, hv);
-          // the remainder: zero for 12-bit samples, else an integer below 32 in
-          // magnitude (exact)
-          const fp16x2 lv = __builtin_amdgcn_cvt_pkrtz(x0 - (float)hv.x, x1 - (float)hv.y);
-          lp[e] = __builtin_bit_cast(unsigned, lv);
+          xs[t][e][0] = (float)(fr[t][e] & 0xffffu) - 2048.f;
+          xs[t][e][1] = (float)(fr[t][e] >> 16) - 2048.f;
+          hp[e] = __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(xs[t][e][0], xs[t][e][1]));
         }
         ah[t] = __builtin_bit_cast(half8, hw);
-        al[t] = __builtin_bit_cast(half8, lw);
       }
       load_mr(rd + 1 < nrd ? rd + 1 : nrd - 1);   // (past the end: a row already read, unused)
       floatx4 acc2[2][2];
@@ -1397,11 +1399,23 @@ __device__ __forceinline__ void w3_body(const W3Args& a, const unsigned short* _
 #pragma unroll
         for (int nt = 0; nt < 2; ++nt)
           acc2[t][nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[t], bA[nt][1], acc2[t][nt], 0, 0, 0);
+      if (big) {
 #pragma unroll
-      for (int t = 0; t < 2; ++t)
+        for (int t = 0; t < 2; ++t) {
+          uint4 lw;
+          unsigned* lp = reinterpret_cast<unsigned*>(&lw);
 #pragma unroll
-        for (int nt = 0; nt < 2; ++nt)
-          acc2[t][nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[t], bA[nt][0], acc2[t][nt], 0, 0, 0);
+          for (int e = 0; e < 4; ++e) {   // the remainder: an integer below 32 in magnitude (exact)
+            const fp16x2 hv = __builtin_bit_cast(fp16x2, reinterpret_cast<const unsigned*>(&ah[t])[e]);
+            lp[e] = __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(xs[t][e][0] - (float)hv.x,
+                                                                           xs[t][e][1] - (float)hv.y));
+          }
+          const half8 al = __builtin_bit_cast(half8, lw);
+#pragma unroll
+          for (int nt = 0; nt < 2; ++nt)
+            acc2[t][nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bA[nt][0], acc2[t][nt], 0, 0, 0);
+        }
+      }
       // lane (a, q) holds Y[l][a] of tile rows 4q + e: round row 4 wave + 2t +
       // (q >> 1), l = 4 (q & 1) + e
       unsigned char* eb = rowBase + (rd & 1) * EBUF + (4 * wave + (q >> 1)) * Sh::ERP + pa * Sh::EPP + wa * 8 +
