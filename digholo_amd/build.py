@@ -50,7 +50,9 @@ def build(verbose: bool = True, jobs: int = 0) -> str:
         objs.append(obj)
     for src in HOST_SOURCES:
         obj = os.path.join(BUILD, src + ".o")
-        cmds.append(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        # the FP64 eigen-solve behind mode-dependent loss runs on the host cores
+        omp = ["-fopenmp"] if src == "dh_eigen.cpp" else []
+        cmds.append(["g++", *HOST_FLAGS, *omp, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     # the C++-linkage names of every entry point (what a caller of the
     # reference header links against), generated from include/digHolo.h
@@ -65,7 +67,7 @@ def build(verbose: bool = True, jobs: int = 0) -> str:
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(lambda c: _run(c, verbose), cmds))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", *objs, "-o", tmp], verbose)
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", *objs, "-lgomp", "-o", tmp], verbose)
     os.replace(tmp, LIB)
     return LIB
 

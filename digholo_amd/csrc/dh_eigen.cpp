@@ -17,6 +17,10 @@ namespace dh {
 
 using cd = std::complex<double>;
 
+// trailing blocks at least this tall are reduced with the rows spread over
+// the host's threads (OpenMP; OMP_NUM_THREADS sets how many)
+constexpr int kParallelRows = 96;
+
 void hermitian_tridiagonal(int n, std::vector<cd>& A, std::vector<double>& d, std::vector<double>& e) {
   hermitian_tridiagonal_q(n, A, d, e, nullptr, nullptr);
 }
@@ -47,7 +51,9 @@ void hermitian_tridiagonal_q(int n, std::vector<cd>& A, std::vector<double>& d, 
     const double tau = 2.0 / vn;
     if (refl) refl->push_back(Reflector{k, tau, std::vector<cd>(v.begin(), v.begin() + L)});
     if (sub) (*sub)[k] = -ph * alpha;
-    // p = tau * B v, B the trailing (L x L) block
+    // p = tau * B v, B the trailing (L x L) block (rows in parallel on the
+    // host cores once the block is large enough to pay for the threads)
+#pragma omp parallel for schedule(static) if (L >= kParallelRows)
     for (int i = 0; i < L; ++i) {
       cd s = 0;
       const cd* row = &A[(size_t)(k + 1 + i) * n + k + 1];
@@ -59,6 +65,7 @@ void hermitian_tridiagonal_q(int n, std::vector<cd>& A, std::vector<double>& d, 
     const cd K = 0.5 * tau * vp;
     for (int i = 0; i < L; ++i) p[i] -= K * v[i];          // w
     // B <- B - v w^H - w v^H
+#pragma omp parallel for schedule(static) if (L >= kParallelRows)
     for (int i = 0; i < L; ++i) {
       cd* row = &A[(size_t)(k + 1 + i) * n + k + 1];
       const cd vi = v[i], wi = p[i];
