@@ -329,6 +329,10 @@ __global__ __launch_bounds__(64 * NW) void k_overlap_mfma(Geometry g, const floa
 
 // float4 pairs of a field chunk (8 w of them, w <= 32 KS1) per thread of a
 // workgroup of NT2 waves: at most 8
+// halfs per row of the split overlap's fp16 chunk planes: the field width or
+// the k-steps' width, whichever is larger, + 8 (rows 16 bytes apart mod 256)
+__host__ __device__ constexpr int split_row_halfs(int w, int ks1) { return (w > 32 * ks1 ? w : 32 * ks1) + 8; }
+
 __host__ __device__ constexpr int split_pairs_per_thread(int ks1, int nt2) {
   return (4 * ks1 + nt2 - 1) / nt2 < 8 ? (4 * ks1 + nt2 - 1) / nt2 : 8;
 }
@@ -362,8 +366,6 @@ __global__ __launch_bounds__(64 * NT2, split_min_blocks(KS1, NT2)) void k_overla
   constexpr int NT = 64 * NT2;
   const int p = blockIdx.x, b = blockIdx.y;
   const int G = g.G, Gp = (G + 15) & ~15, w = g.w, h = g.h;
-  const int P2 = w + 2;
-  float2* buf = reinterpret_cast<float2*>(os_smem);   // [2][16 P2 + 2]
   const int tid = threadIdx.x, lane = tid & 63;
   const int mt = __builtin_amdgcn_readfirstlane(tid >> 6);   // this wave's m-tile
   const int lr = lane & 15, q = lane >> 4;
@@ -413,57 +415,62 @@ __global__ __launch_bounds__(64 * NT2, split_min_blocks(KS1, NT2)) void k_overla
     scale = ldexpf(1.f, 8 - ex);
   }
 
-  // a chunk is 8 w float4 (complex pairs); this thread's pairs e2 = tid + NT j
-  // (j < kPer, which the launcher checks covers the chunk) and their LDS offsets
-  // are the same for every chunk; the next chunk's pairs are held in registers
-  // (a pair past the chunk reads the chunk's last pair and is stored to a spare
-  // slot after the buffer, so the loops below have no branch)
+  // Each chunk (16 field rows) is split once into fp16 hi / lo planes [re hi,
+  // re lo, im hi, im lo][16 rows][SP halfs] in LDS (double-buffered; columns
+  // w .. 32 KS1 - 1 zero), so every wave reads its MFMA operands with one
+  // 16-byte load per plane.  This thread's complex pairs e2 = tid + NT j (j <
+  // kPer, which the launcher checks covers the chunk) and their LDS offsets
+  // are the same for every chunk; the next chunk's pairs are held in registers.
   constexpr int kPer = split_pairs_per_thread(KS1, NT2);
-  const int n4 = 8 * w, J = h / 16, BS = 16 * P2 + 2;
+  const int n4 = 8 * w, J = h / 16;
+  const int SP = split_row_halfs(w, KS1), PS = 16 * SP;
+  _Float16* pl = reinterpret_cast<_Float16*>(os_smem);   // [2][4][16][SP]
+  for (int e = tid; e < 2 * 4 * 16 * (32 * KS1 - w); e += NT) {
+    const int c = 32 * KS1 - w, row = e / c;
+    pl[row * SP + w + (e - row * c)] = (_Float16)0.f;
+  }
   int lofs[kPer], gofs[kPer];
   floatx4 nxt[kPer];
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     const int e2 = tid + NT * j, yy = (2 * e2) / w;
-    lofs[j] = e2 < n4 ? yy * P2 + 2 * e2 - yy * w : 16 * P2;
+    lofs[j] = e2 < n4 ? yy * SP + 2 * e2 - yy * w : -1;
     gofs[j] = e2 < n4 ? e2 : n4 - 1;
-    *reinterpret_cast<floatx4*>(buf + lofs[j]) = V4[gofs[j]];    // chunk 0
-    nxt[j] = V4[(J > 1 ? n4 : 0) + gofs[j]];
+    nxt[j] = V4[gofs[j]];   // chunk 0
   }
+  auto stage = [&](int bi) {   // nxt (scaled, split) into buffer bi
+    _Float16* pb = pl + bi * 4 * PS;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      if (lofs[j] >= 0) {
+        unsigned rh, rl, ih, il;
+        split_pair(nxt[j][0] * scale, nxt[j][2] * scale, rh, rl);   // re of columns x, x + 1
+        split_pair(nxt[j][1] * scale, nxt[j][3] * scale, ih, il);   // im
+        *reinterpret_cast<unsigned*>(pb + lofs[j]) = rh;
+        *reinterpret_cast<unsigned*>(pb + PS + lofs[j]) = rl;
+        *reinterpret_cast<unsigned*>(pb + 2 * PS + lofs[j]) = ih;
+        *reinterpret_cast<unsigned*>(pb + 3 * PS + lofs[j]) = il;
+      }
+    }
+  };
+  stage(0);
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) nxt[j] = V4[(J > 1 ? n4 : 0) + gofs[j]];
 
   floatx4 acc0[NT2], acc1[NT2];
 #pragma unroll
   for (int nt = 0; nt < NT2; ++nt) acc0[nt] = acc1[nt] = floatx4{0.f, 0.f, 0.f, 0.f};
   for (int ch = 0; ch < J; ++ch) {
     __syncthreads();   // chunk ch is in its buffer; the other buffer's readers are done
-    const float2* cb = buf + (ch & 1) * BS;
+    const _Float16* cb = pl + (ch & 1) * 4 * PS + lr * SP + 8 * q;
     // stage 1, both parts: T_c[yy][m] = sum_x V_c[yy][x] HGx[m][x]
     floatx4 d0 = {0.f, 0.f, 0.f, 0.f}, d1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int ks = 0; ks < KS1; ++ks) {
-      const int kb = 32 * ks + 8 * q;
-      half8 ahr, alr, ahi, ali;
-      if (kb < w) {   // a ragged last step (w % 32 == 16) zero-fills the upper lane groups
-        const floatx4* vp = reinterpret_cast<const floatx4*>(cb + lr * P2 + kb);
-        uint4 rh, rl, ih, il;
-        unsigned* prh = reinterpret_cast<unsigned*>(&rh);
-        unsigned* prl = reinterpret_cast<unsigned*>(&rl);
-        unsigned* pih = reinterpret_cast<unsigned*>(&ih);
-        unsigned* pil = reinterpret_cast<unsigned*>(&il);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const floatx4 t = vp[e];   // (re, im) of columns kb + 2e, kb + 2e + 1
-          split_pair(t[0] * scale, t[2] * scale, prh[e], prl[e]);
-          split_pair(t[1] * scale, t[3] * scale, pih[e], pil[e]);
-        }
-        ahr = __builtin_bit_cast(half8, rh);
-        alr = __builtin_bit_cast(half8, rl);
-        ahi = __builtin_bit_cast(half8, ih);
-        ali = __builtin_bit_cast(half8, il);
-      } else {
-#pragma unroll
-        for (int e = 0; e < 8; ++e) ahr[e] = alr[e] = ahi[e] = ali[e] = (_Float16)0.f;
-      }
+      const half8 ahr = *reinterpret_cast<const half8*>(cb + 32 * ks);
+      const half8 alr = *reinterpret_cast<const half8*>(cb + PS + 32 * ks);
+      const half8 ahi = *reinterpret_cast<const half8*>(cb + 2 * PS + 32 * ks);
+      const half8 ali = *reinterpret_cast<const half8*>(cb + 3 * PS + 32 * ks);
       d0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahr, b1h[ks], d0, 0, 0, 0);
       d1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, b1h[ks], d1, 0, 0, 0);
       d0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahr, b1l[ks], d0, 0, 0, 0);
@@ -484,13 +491,10 @@ __global__ __launch_bounds__(64 * NT2, split_min_blocks(KS1, NT2)) void k_overla
     }
     // the next chunk into the other buffer (loaded a chunk ago); the one after in flight
     if (ch + 1 < J) {
-      float2* nb = buf + ((ch + 1) & 1) * BS;
+      stage((ch + 1) & 1);
       const size_t c2 = (size_t)(ch + 2 < J ? ch + 2 : ch + 1) * n4;
 #pragma unroll
-      for (int j = 0; j < kPer; ++j) {
-        *reinterpret_cast<floatx4*>(nb + lofs[j]) = nxt[j];
-        nxt[j] = V4[c2 + gofs[j]];
-      }
+      for (int j = 0; j < kPer; ++j) nxt[j] = V4[c2 + gofs[j]];
     }
     // stage 2 from registers: C_c[m][n] += sum_yy T_c[m][yy] HGy[n][yy]
     uint2 t0h, t0l, t1h, t1l;
@@ -533,7 +537,7 @@ template <int KS1, int NT2>
 static hipError_t split_launch(const Geometry& g, const float2* fields, const void* hgx16, const void* hgy16,
                                float2* coefs, hipStream_t s, const unsigned* peak) {
   if (8 * g.w > split_pairs_per_thread(KS1, NT2) * 64 * NT2) return hipErrorNotSupported;
-  const size_t lds = (size_t)2 * (16 * (g.w + 2) + 2) * sizeof(float2);
+  const size_t lds = (size_t)2 * 4 * 16 * split_row_halfs(g.w, KS1) * sizeof(_Float16);
   dim3 grid(g.P, g.B);
   k_overlap_split<KS1, NT2><<<grid, 64 * NT2, lds, s>>>(g, fields, static_cast<const _Float16*>(hgx16),
                                                          static_cast<const _Float16*>(hgy16), coefs, peak);
