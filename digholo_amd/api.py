@@ -157,6 +157,8 @@ SIGNATURES: Dict[str, Tuple[object, List[object]]] = {
     "digHoloGetWindowGeometry": (c_int, [c_int, P_int]),
     "digHoloConfigSetFusedWindow": (c_int, [c_int, c_int]),
     "digHoloConfigSetFusedOverlap": (c_int, [c_int, c_int]),
+    "digHoloConfigSetWindowRowTransform": (c_int, [c_int, c_int]),
+    "digHoloConfigGetWindowRowTransform": (c_int, [c_int]),
     "digHoloGetWindowPath": (c_int, [c_int]),
     "digHoloConfigSetStageTiming": (c_int, [c_int, c_int]),
     "digHoloGetStageTimes": (c_int, [c_int, P_float]),

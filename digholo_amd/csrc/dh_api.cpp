@@ -247,6 +247,8 @@ struct Handle {
   // form only (the faster choice on the measured shapes), 0 never, 1 in either precision
   int fusedOverlap = -1;
   bool lastFusedOverlap = false;
+  // row transform of path 3 on 256-wide windows: 0 MFMA, 1 registers
+  int rowMode = 0;
 
   // AutoAlign parameter estimate: batch-summed Fourier maps [P][Ny][Nx],
   // DCT work space, peak index, axes; far-field copies of the fields and maps
@@ -886,15 +888,16 @@ int ensure_row_twiddles(Handle& h, int W) {
 // and window sizes, pols, groups, precision, input type, batch size rounded to
 // a power of two -- the applicable launch variants are timed on the batch
 // itself with HIP events: window path (size-specialised / MFMA columns / FFT
-// columns / staged), the split overlap fused into the window kernel or as its
-// own kernel, and the separate overlap kernel's stage-2 tiles per wave (the
-// fewest that cover the triangle, or twice that).  The fastest is remembered in
-// memory and in a text cache file ($DIGHOLO_TUNE_CACHE, else
-// $HOME/.cache/digholo_amd_tune.txt), one "key path fused tpw ms" line per
-// shape.
+// columns / staged), the size-specialised kernel's row transform where it has
+// two (MFMA or registers, 256-wide windows), the split overlap fused into the
+// window kernel or as its own kernel, and the separate overlap kernel's
+// stage-2 tiles per wave (the fewest that cover the triangle, or twice that).
+// The fastest is remembered in memory and in a text cache file
+// ($DIGHOLO_TUNE_CACHE, else $HOME/.cache/digholo_amd_tune.txt), one
+// "key path fused tpw rows ms" line per shape.
 // ---------------------------------------------------------------------------
 struct TuneChoice {
-  int windowMode = 3, fused = 1, tpw = 0;
+  int windowMode = 3, fused = 1, tpw = 0, rows = 0;
   float ms = 0.f;
 };
 std::mutex g_tuneMu;
@@ -909,12 +912,12 @@ std::string tune_path() {
 
 // Tune-cache lines carry this tag: choices written by a build whose kernels
 // differ (other tile counts, other paths) are not reused.
-constexpr const char* kTuneVersion = "v2";
+constexpr const char* kTuneVersion = "v3";
 
 // A cached choice is applied only if the library could have made it itself.
 bool tune_choice_valid(const TuneChoice& t) {
   return t.windowMode >= 0 && t.windowMode <= 3 && (t.fused == 0 || t.fused == 1) && t.tpw >= 0 && t.tpw <= 32 &&
-         std::isfinite(t.ms) && t.ms >= 0.f;
+         (t.rows == 0 || t.rows == 1) && std::isfinite(t.ms) && t.ms >= 0.f;
 }
 
 void tune_load_locked() {
@@ -924,7 +927,7 @@ void tune_load_locked() {
   if (!f) return;
   char key[512];
   TuneChoice t;
-  while (std::fscanf(f, "%511s %d %d %d %f", key, &t.windowMode, &t.fused, &t.tpw, &t.ms) == 5)
+  while (std::fscanf(f, "%511s %d %d %d %d %f", key, &t.windowMode, &t.fused, &t.tpw, &t.rows, &t.ms) == 6)
     if (tune_choice_valid(t) && std::strncmp(key, kTuneVersion, std::strlen(kTuneVersion)) == 0) g_tune[key] = t;
   std::fclose(f);
 }
@@ -945,8 +948,8 @@ void tune_save_locked() {
   FILE* f = std::fopen(tmp.c_str(), "w");
   if (!f) return;
   for (const auto& kv : g_tune)
-    std::fprintf(f, "%s %d %d %d %.6f\n", kv.first.c_str(), kv.second.windowMode, kv.second.fused, kv.second.tpw,
-                 kv.second.ms);
+    std::fprintf(f, "%s %d %d %d %d %.6f\n", kv.first.c_str(), kv.second.windowMode, kv.second.fused,
+                 kv.second.tpw, kv.second.rows, kv.second.ms);
   if (std::fclose(f) == 0) {
     if (std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
   } else {
@@ -983,6 +986,7 @@ int autotune(Handle& h) {
       h.windowMode = it->second.windowMode;
       h.fusedOverlap = it->second.fused;
       h.ovTpw = it->second.tpw;
+      h.rowMode = it->second.rows;
       return DIGHOLO_ERROR_SUCCESS;
     }
   }
@@ -996,17 +1000,21 @@ int autotune(Handle& h) {
   DH_TRY(hipEventCreate(&e1), "event create");
   TuneChoice best;
   best.ms = std::numeric_limits<float>::max();
-  std::vector<std::array<int, 3>> seen;   // (actual path, fused, tpw) already timed
+  std::vector<std::array<int, 4>> seen;   // (actual path, fused, tpw, rows) already timed
+  const int rowChoices = dh::window_v3_row_choice(d.g) ? 2 : 1;
   int rc = DIGHOLO_ERROR_SUCCESS;
   for (int wm = 3; wm >= 0 && !rc; --wm)
+    for (int rm = 0; rm < (wm == 3 ? rowChoices : 1) && !rc; ++rm)
     for (int fu = 1; fu >= 0 && !rc; --fu)
       for (int tp : {0, 2 * minTpw}) {
         if (tp > 32 || (tp && minTpw == 0)) continue;
         h.windowMode = wm;
         h.fusedOverlap = fu;
         h.ovTpw = tp;
+        h.rowMode = rm;
         if ((rc = enqueue_batch(h, 0))) break;
-        const std::array<int, 3> sig{h.lastPath, h.lastFusedOverlap ? 1 : 0, h.lastFusedOverlap ? 0 : tp};
+        const std::array<int, 4> sig{h.lastPath, h.lastFusedOverlap ? 1 : 0, h.lastFusedOverlap ? 0 : tp,
+                                     h.lastPath == 3 ? rm : 0};
         if (std::find(seen.begin(), seen.end(), sig) != seen.end()) continue;
         seen.push_back(sig);
         constexpr int reps = 3;
@@ -1018,8 +1026,9 @@ int autotune(Handle& h) {
         (void)hipEventElapsedTime(&ms, e0, e1);
         ms /= reps;
         if (c.verbosity >= DIGHOLO_VERBOSITY_DEBUG)
-          fprintf(stdout, "autotune %s: path %d fused %d tpw %d  %.4f ms\n", key.c_str(), sig[0], sig[1], tp, ms);
-        if (ms < best.ms) best = TuneChoice{wm, fu, tp, ms};
+          fprintf(stdout, "autotune %s: path %d fused %d tpw %d rows %d  %.4f ms\n", key.c_str(), sig[0], sig[1], tp,
+                  sig[3], ms);
+        if (ms < best.ms) best = TuneChoice{wm, fu, tp, rm, ms};
       }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -1028,6 +1037,7 @@ int autotune(Handle& h) {
   h.windowMode = best.windowMode;
   h.fusedOverlap = best.fused;
   h.ovTpw = best.tpw;
+  h.rowMode = best.rows;
   std::lock_guard<std::mutex> lk(g_tuneMu);
   g_tune[key] = best;
   tune_save_locked();
@@ -1349,13 +1359,14 @@ int enqueue_batch(Handle& h, int step = 0) {
                             fuseOv ? h.coefs.p : nullptr,
                             cal,
                             fuseOv && !split ? h.hgx.p : nullptr,
-                            fuseOv && !split ? h.hgy.p : nullptr};
+                            fuseOv && !split ? h.hgy.p : nullptr,
+                            h.rowMode};
       if (h.collect && c.basisType == DIGHOLO_BASISTYPE_HG && dh::window_v3_multi_supported(wl)) {
         h.pend = wl;
         h.pendValid = true;
       } else {
         DH_TRY(dh::launch_window_v3(wl.fa, wl.frames, wl.tab, wl.factor, wl.ref, wl.fields, wl.hgx16, wl.hgy16,
-                                    wl.coefs, wl.cal, h.stream, wl.hgx32, wl.hgy32),
+                                    wl.coefs, wl.cal, h.stream, wl.hgx32, wl.hgy32, wl.rowMode),
                "fused window (register rows)");
       }
     } else if (path == 2)
@@ -3033,19 +3044,19 @@ bool same_config(const Config& a, const Config& b) {
 // what a captured launch sequence depends on besides the configuration
 struct LaunchState {
   const void* frames;
-  int frameU16, transpose, windowMode, fusedOverlap, ovTpw, autoTune;
+  int frameU16, transpose, windowMode, fusedOverlap, ovTpw, autoTune, rowMode;
   bool holdCrop, holdWin;
   unsigned tabVersion;
   hipStream_t stream;
   bool operator==(const LaunchState& o) const {
     return frames == o.frames && frameU16 == o.frameU16 && transpose == o.transpose && windowMode == o.windowMode &&
-           fusedOverlap == o.fusedOverlap && ovTpw == o.ovTpw && autoTune == o.autoTune && holdCrop == o.holdCrop &&
-           holdWin == o.holdWin && tabVersion == o.tabVersion && stream == o.stream;
+           fusedOverlap == o.fusedOverlap && ovTpw == o.ovTpw && autoTune == o.autoTune && rowMode == o.rowMode &&
+           holdCrop == o.holdCrop && holdWin == o.holdWin && tabVersion == o.tabVersion && stream == o.stream;
   }
 };
 LaunchState launch_state(const Handle& h) {
-  return LaunchState{h.frames,     h.frameU16, h.transpose, h.windowMode, h.fusedOverlap, h.ovTpw,
-                     h.autoTune,   h.holdCrop, h.holdWin,   h.tabVersion, h.stream};
+  return LaunchState{h.frames,   h.frameU16, h.transpose, h.windowMode, h.fusedOverlap, h.ovTpw,
+                     h.autoTune, h.rowMode,  h.holdCrop,  h.holdWin,    h.tabVersion,   h.stream};
 }
 
 struct MultiOp {
@@ -3109,7 +3120,7 @@ hipError_t run_multi_ops(const MultiGraph& m, hipStream_t s) {
     } else if (o.kind == 1) {
       const dh::W3Launch& w = m.w3[o.i0];
       e = dh::launch_window_v3(w.fa, w.frames, w.tab, w.factor, w.ref, w.fields, w.hgx16, w.hgy16, w.coefs, w.cal, s,
-                               w.hgx32, w.hgy32);
+                               w.hgx32, w.hgy32, w.rowMode);
     } else {
       e = dh::launch_overlap_mfma(o.g, o.fields, o.hgx, o.hgy, o.coefs, o.split, s, o.tpw, nullptr, o.hgx16,
                                   o.hgy16);
@@ -4743,6 +4754,16 @@ EXT_C int digHoloConfigSetFusedOverlap(int handleIdx, int mode) {
 EXT_C int digHoloGetWindowPath(int handleIdx) {
   H_OR(-1);
   return h->haveBatch ? h->lastPath : -1;
+}
+EXT_C int digHoloConfigSetWindowRowTransform(int handleIdx, int mode) {
+  H_OR(DIGHOLO_ERROR_INVALIDHANDLE);
+  if (mode < 0 || mode > 1) return DIGHOLO_ERROR_INVALIDARGUMENT;
+  h->rowMode = mode;
+  return DIGHOLO_ERROR_SUCCESS;
+}
+EXT_C int digHoloConfigGetWindowRowTransform(int handleIdx) {
+  H_OR(0);
+  return h->rowMode;
 }
 
 EXT_C int digHoloConfigSetStageTiming(int handleIdx, int enabled) {

@@ -635,14 +635,16 @@ __device__ __forceinline__ void split_f16x2(float xr, float xi, unsigned& hi, un
 
 constexpr int kW3Threads = 256;
 
-template <int L, int M, int U, int J>
+// R: the MFMA row transform where it exists (128-point rows); false keeps the
+// register transform (the autotuner times both)
+template <int L, int M, int U, int J, bool R = true>
 struct W3Shape {
   static constexpr int NR = L * M;            // complex points per row transform
   // MR: the row transform on MFMA (128-point rows: a 16-point DFT, then per
   // residue pair one matrix for the 8-point DFT, its twiddles and the r2c
   // unpack), in rounds of 16 rows; the exchange holds whole rows, grouped by
   // residue pair, and it and the B buffer are double-buffered
-  static constexpr bool MR = L == 8 && M == 16;
+  static constexpr bool MR = R && L == 8 && M == 16;
   static constexpr int RPW = 64 / L;          // rows per wave per round (register row transform)
   static constexpr int RR = MR ? 16 : 4 * RPW;   // rows per round
   static constexpr int KS = RR / 16;          // MFMA k-steps per round
@@ -1028,11 +1030,11 @@ __device__ __forceinline__ void w3_item(int P, int B, int id, int& p, int& b) {
 // SQ: a square FFT window (H = W = 2 L M), so the column transform's length
 // and its table offsets are compile-time constants.  item: the (pol, frame)
 // work item of the batch this workgroup processes (w3_item).
-template <int L, int M, int U, int J, bool SQ>
+template <int L, int M, int U, int J, bool SQ, bool R = true>
 __device__ __forceinline__ void w3_body(const W3Args& a, const unsigned short* __restrict__ frames,
                                         const float2* __restrict__ tabs, const float2* __restrict__ factor,
                                         const float2* __restrict__ ref, float2* __restrict__ fields, int item) {
-  using Sh = W3Shape<L, M, U, J>;
+  using Sh = W3Shape<L, M, U, J, R>;
   constexpr int NT = kW3Threads, RPW = Sh::RPW, RR = Sh::RR, KS = Sh::KS, S = Sh::S, w = Sh::w, h = Sh::h;
   constexpr int NU = Sh::NU, LP = Sh::LP, RPc = Sh::RPc, PK = Sh::PK, NR = Sh::NR, WP = Sh::WP;
   extern __shared__ __align__(16) unsigned char w3_smem[];
@@ -1649,13 +1651,13 @@ __device__ __forceinline__ void w3_body(const W3Args& a, const unsigned short* _
   }
 }
 
-template <int L, int M, int U, int J, bool SQ>
+template <int L, int M, int U, int J, bool SQ, bool R>
 __global__ __launch_bounds__(kW3Threads, 2) void k_window_v3(W3Args a, const unsigned short* __restrict__ frames,
                                                           const float2* __restrict__ tabs,
                                                           const float2* __restrict__ factor,
                                                           const float2* __restrict__ ref,
                                                           float2* __restrict__ fields) {
-  w3_body<L, M, U, J, SQ>(a, frames, tabs, factor, ref, fields, blockIdx.x);
+  w3_body<L, M, U, J, SQ, R>(a, frames, tabs, factor, ref, fields, blockIdx.x);
 }
 
 // One handle's batch in a multi-batch launch: its arguments, its kernel
@@ -1827,24 +1829,24 @@ static int w3_ovl_bytes(const Geometry& g) {
 }
 
 // LDS layout of a batch's launch (region, overlap buffers); returns the bytes
-template <int L, int M, int U, int J>
+template <int L, int M, int U, int J, bool R = true>
 static size_t w3_layout(W3Args& a) {
-  using Sh = W3Shape<L, M, U, J>;
+  using Sh = W3Shape<L, M, U, J, R>;
   a.region = w3_region(Sh::EBYTES, Sh::BBYTES, a.g.H, L, M, Sh::w, Sh::h);
   a.ovlOff = 8 * Sh::h * Sh::WP;
   if (a.hgx32) a.region = std::max(a.region, a.ovlOff + w3_ovl_bytes(a.g));   // the FP32 form's stage-1 buffers
   return w3_lds<L, M, U, J>(a);
 }
 
-template <int L, int M, int U, int J>
+template <int L, int M, int U, int J, bool R = true>
 static hipError_t w3_launch(W3Args a, const void* frames, const float2* tab, const float2* factor,
                             const float2* ref, float2* fields, hipStream_t s) {
-  using Sh = W3Shape<L, M, U, J>;
+  using Sh = W3Shape<L, M, U, J, R>;
   const Geometry& g = a.g;
   if (g.H % Sh::RR) return hipErrorNotSupported;
-  const size_t lds = w3_layout<L, M, U, J>(a);
+  const size_t lds = w3_layout<L, M, U, J, R>(a);
   if (lds > 160 * 1024) return hipErrorNotSupported;
-  auto kern = g.H == 2 * L * M ? k_window_v3<L, M, U, J, true> : k_window_v3<L, M, U, J, false>;
+  auto kern = g.H == 2 * L * M ? k_window_v3<L, M, U, J, true, R> : k_window_v3<L, M, U, J, false, R>;
   if (lds > 64 * 1024) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -1924,16 +1926,29 @@ static bool w3_args(const FusedArgs& fa, const void* hgx16, const void* hgy16, f
   return true;
 }
 
+// rowMode 1: the register row transform where the MFMA one exists
+template <int L, int M, int U, int J>
+static hipError_t w3_launch_rows(int rowMode, const W3Args& a, const void* frames, const float2* tab,
+                                 const float2* factor, const float2* ref, float2* fields, hipStream_t s) {
+  if constexpr (L == 8 && M == 16) {
+    if (rowMode == 1) return w3_launch<L, M, U, J, false>(a, frames, tab, factor, ref, fields, s);
+  }
+  return w3_launch<L, M, U, J, true>(a, frames, tab, factor, ref, fields, s);
+}
+
+bool window_v3_row_choice(const Geometry& g) { return g.W == 256; }
+
 hipError_t launch_window_v3(const FusedArgs& fa, const void* frames_u16, const float2* tab, const float2* factor,
                             const float2* ref, float2* fields, const void* hgx16, const void* hgy16, float2* coefs,
-                            const BatchCal& cal, hipStream_t s, const float* hgx32, const float* hgy32) {
+                            const BatchCal& cal, hipStream_t s, const float* hgx32, const float* hgy32, int rowMode) {
   W3Args a;
   int L, M;
   if (!w3_args(fa, hgx16, hgy16, coefs, cal, hgx32, hgy32, a, L, M)) return hipErrorNotSupported;
   const Geometry& g = fa.g;
   const int U = g.w / 16, J = g.h / 16;
 #define W3_CALL(LL, MM, UU, JJ) \
-  if (L == LL && M == MM && U == UU && J == JJ) return w3_launch<LL, MM, UU, JJ>(a, frames_u16, tab, factor, ref, fields, s);
+  if (L == LL && M == MM && U == UU && J == JJ)                                                  \
+    return w3_launch_rows<LL, MM, UU, JJ>(rowMode, a, frames_u16, tab, factor, ref, fields, s);
   W3_SHAPES(W3_CALL)
 #undef W3_CALL
   return hipErrorNotSupported;

@@ -179,10 +179,13 @@ hipError_t launch_window_v3_tables(const Geometry& g, float2* tab, hipStream_t s
 // With hgx16/hgy16 (split basis from launch_hg_split) and coefs set, the same
 // launch also computes the split-fp16 HG overlap into coefs; with hgx32/hgy32
 // (the FP32 basis) instead, the FP32 overlap.
+// rowMode: 0 the MFMA row transform where it exists (256-wide windows), 1 the
+// register one (window_v3_row_choice: whether the geometry has the choice).
 hipError_t launch_window_v3(const FusedArgs& a, const void* frames_u16, const float2* tab, const float2* factor,
                             const float2* ref, float2* fields, const void* hgx16, const void* hgy16, float2* coefs,
                             const BatchCal& cal, hipStream_t s, const float* hgx32 = nullptr,
-                            const float* hgy32 = nullptr);
+                            const float* hgy32 = nullptr, int rowMode = 0);
+bool window_v3_row_choice(const Geometry& g);
 bool window_v3_overlap_fits(const Geometry& g);
 // One batch of a multi-batch launch of the size-specialised window kernel: the
 // arguments launch_window_v3 takes.
@@ -196,6 +199,7 @@ struct W3Launch {
   float2* coefs;
   BatchCal cal;
   const float *hgx32, *hgy32;
+  int rowMode;   // launch_window_v3's row transform choice
 };
 // n <= kW3MaxBatches batches with square FFT windows in one launch (batch =
 // blockIdx.y); window_v3_multi_supported says whether a batch's shape is one

@@ -306,8 +306,9 @@ EXT_C void* digHoloGetStream(int handleIdx);
 EXT_C int digHoloGetWindowGeometry(int handleIdx, int* out);
 
 /* Window path.  mode 3 (default): the size-specialised camera-window-to-field
- * kernel (row transforms in registers, column transform as a split-fp16 GEMM on
- * MFMA; uint16 frames, W/2 in {64, 96, 128}); 2: the general fused kernel with
+ * kernel (row transforms on MFMA for 256-wide FFT windows, in registers for
+ * the others; column transform as a split-fp16 GEMM on MFMA; uint16 frames,
+ * W/2 in {64, 96, 128}); 2: the general fused kernel with
  * the MFMA column transform (uint16 frames); 1: the fused kernel with FFT column
  * transforms; 0: the staged kernels.  A mode the shape or input type does not
  * fit falls back to the next lower one.  GetWindowPath reports the path of the
@@ -317,6 +318,13 @@ EXT_C int digHoloConfigSetFusedWindow(int handleIdx, int enabled);
  * form only, 0 never (the separate overlap kernel), 1 in either precision. */
 EXT_C int digHoloConfigSetFusedOverlap(int handleIdx, int mode);
 EXT_C int digHoloGetWindowPath(int handleIdx);
+/* Row transform of the size-specialised window kernel for 256-wide FFT windows:
+ * 0 (default) on MFMA (a 16-point DFT of the samples, then one matrix per
+ * residue pair for the rest of the transform and the r2c unpack), 1 in
+ * registers (two FFT passes).  Other widths always use the register
+ * transform.  The autotuner times both.  No counterpart in the reference. */
+EXT_C int digHoloConfigSetWindowRowTransform(int handleIdx, int mode);
+EXT_C int digHoloConfigGetWindowRowTransform(int handleIdx);
 
 /* Per-stage GPU time in ms, measured with HIP events on the handle's stream
  * when enabled, averaged over every batch processed since the previous call:

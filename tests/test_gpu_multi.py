@@ -141,11 +141,15 @@ def test_multi_graph_replays_match_individual_batches(lib):
             lib.digHoloDestroy(h)
 
 
-def test_autotune_choice_and_cache(lib, tmp_path, monkeypatch):
+@pytest.mark.parametrize("shape", [SHAPES[1], (640, 512, 256, 45, 8)])
+def test_autotune_choice_and_cache(lib, tmp_path, monkeypatch, shape):
+    """The tuned launch reproduces the default one to 1e-5 and lands in the
+    cache (key path fused tpw rows ms); the 256-wide window also has the row
+    transform choice (MFMA or registers)."""
     import torch
     cache = tmp_path / "tune.txt"
     monkeypatch.setenv("DIGHOLO_TUNE_CACHE", str(cache))
-    W, H, win, G, B = SHAPES[1]
+    W, H, win, G, B = shape
     fr = frames_for(W, H, G, 8, seed=5)
     h = make_handle(lib, W, H, win, G, 8, fr)
     try:
@@ -157,14 +161,19 @@ def test_autotune_choice_and_cache(lib, tmp_path, monkeypatch):
             assert lib.digHoloConfigSetAutoTune(h, 1) == 0
             c1, f1 = individual(lib, h)
             path = lib.digHoloGetWindowPath(h)
-            print(f"precision {prec}: tuned window path {path}", flush=True)
+            rows = lib.digHoloConfigGetWindowRowTransform(h)
+            print(f"precision {prec}: tuned window path {path}, row transform {rows}", flush=True)
+            assert rows in (0, 1) and (rows == 0 or win == 256)
             ec = O.relative_l2(c1.cpu().numpy(), c0.cpu().numpy())
             ef = O.relative_l2(f1.cpu().numpy(), f0.cpu().numpy())
             assert ec <= 1e-5 and ef <= 1e-5, (ec, ef)
         text = cache.read_text()
         print(text, flush=True)
+        # (the library's tuned choices are process-wide: the file holds every
+        # shape tuned so far, two of them this one's)
         lines = [ln for ln in text.splitlines() if ln.strip()]
-        assert len(lines) == 2 and all(len(ln.split()) == 5 for ln in lines)
+        assert all(len(ln.split()) == 6 for ln in lines)
+        assert len([ln for ln in lines if f"_W{win}x{win}_" in ln and f"_B{8}" in ln]) == 2
         # mode 2 reuses the remembered choice without timing anything new
         assert lib.digHoloConfigSetAutoTune(h, 2) == 0
         individual(lib, h)

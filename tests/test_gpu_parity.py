@@ -439,3 +439,24 @@ def test_parity_mfma_rows_edge_cases(lib, case):
     print(f"case {case}: tilt {cfg.get('tilt_x')} window starts {[sx - geo['W'] // 2 for sx in geo['SX']]}", flush=True)
     check(frames, cfg, coefs, fields)
     lib.digHoloDestroy(h)
+
+
+@pytest.mark.parametrize("rows", [0, 1])
+def test_parity_window_row_transforms(lib, rows):
+    """Both row transforms of the size-specialised kernel on the 256-wide
+    window (0: MFMA, 1: registers), split and FP32 overlap."""
+    frames, cfg = make_case("dual640", 3, seed=21)
+    for prec in (api.DIGHOLO_PRECISION_SPLIT16, api.DIGHOLO_PRECISION_FP32):
+        h = lib.digHoloCreate()
+        api.configure(lib, h, **cfg)
+        assert lib.digHoloConfigSetWindowRowTransform(h, rows) == 0
+        assert lib.digHoloConfigGetWindowRowTransform(h) == rows
+        assert lib.digHoloConfigSetOverlapPrecision(h, prec) == 0
+        coefs, fields, _, _, h = run_gpu(lib, frames, cfg, h=h)
+        assert lib.digHoloGetWindowPath(h) == 3
+        print(f"row transform {rows} precision {prec}", flush=True)
+        check(frames, cfg, coefs, fields)
+        lib.digHoloDestroy(h)
+    h = lib.digHoloCreate()
+    assert lib.digHoloConfigSetWindowRowTransform(h, 2) == api.DIGHOLO_ERROR_INVALIDARGUMENT
+    lib.digHoloDestroy(h)
