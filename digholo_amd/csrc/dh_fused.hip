@@ -676,6 +676,10 @@ __host__ __device__ inline int kmr(int v, int o) {
 
 // offset (complex elements, 16-byte aligned) of the MR fragments in the table
 __host__ __device__ inline int w3_mr_off(int H, int L, int M) { return (H + L * (M + 1) + M + 1) & ~1; }
+// MFMA row path: the column product's A operands after the pair matrices
+// (float2 units; 16-byte aligned), [P][H / 16][2 J][hi, lo][64 lanes] uint4
+__host__ __device__ inline int w3_atab_off(int H, int L, int M) { return w3_mr_off(H, L, M) + kMrMats * kMrMatHalfs / 4; }
+__host__ __device__ inline size_t w3_atab_elems(int P, int H, int J) { return (size_t)P * (H / 16) * (2 * J) * 256; }
 
 // LDS bytes of the aliased region: the row phase or the field plane
 __host__ __device__ inline int w3_region(int ebytes, int bbytes, int H, int L, int M, int w, int h) {
@@ -1091,7 +1095,8 @@ __device__ __forceinline__ void w3_body(const W3Args& a, const unsigned short* _
     for (int m = 0; m < M; ++m) pre[m] = __builtin_nontemporal_load(src + L * m);
   }
 
-  for (int e = tid; e < H; e += NT) {
+  // (the MFMA row path reads its column-product operands from a table of their own)
+  for (int e = tid; e < (Sh::MR ? 0 : H); e += NT) {
     const uint2 t = reinterpret_cast<const uint2*>(tabs)[e];
     twT[e] = t.x;
     twT[H + e] = t.y;
@@ -1243,15 +1248,57 @@ __device__ __forceinline__ void w3_body(const W3Args& a, const unsigned short* _
   const unsigned partB = (wv & 1) ? 8u * (unsigned)H : 0u;
   const unsigned ldsBase = (unsigned)(size_t)((__attribute__((address_space(3))) unsigned char*)w3_smem);
   const int nUnits = (2 * J - wv + 3) / 4;   // units wv, wv + 4, ... below 2 J: NU or NU - 1
-  auto gemm = [&](int rd, const _Float16* Bs) {
-    (void)rd;
+  // the wave's unit count NI is a compile-time argument, so a round holds no
+  // branch on it.  mma_units: k-step ks of the product for A operands ah / al
+  // of the wave's units.
+  auto mma_units = [&](auto ni, const half8* ah, const half8* al, const _Float16* Bs, int ks) {
+    constexpr int NI = decltype(ni)::value;
+    // one product (hi hi, hi lo, lo hi) at a time over the units and a pair of
+    // n-tiles, so consecutive MFMAs write different accumulators
+    constexpr int UC = 2;
+#pragma unroll
+    for (int u0 = 0; u0 < U; u0 += UC) {
+      half8 bh[UC], bl[UC];
+#pragma unroll
+      for (int uu = 0; uu < UC; ++uu) {
+        if (u0 + uu < U) {
+          // MR: column c at position (c mod 16) U + c / 16, so the pair stage's
+          // columns (16 apart) are written to consecutive positions
+          const int bcol = Sh::MR ? lr * U + u0 + uu : 16 * (u0 + uu) + lr;
+          const _Float16* bq = Bs + bcol * PK + 32 * ks + 8 * q;
+          bh[uu] = *reinterpret_cast<const half8*>(bq);
+          bl[uu] = *reinterpret_cast<const half8*>(bq + (w + 1) * PK);
+        }
+      }
+#pragma unroll
+      for (int uu = 0; uu < UC; ++uu)
+#pragma unroll
+        for (int i = 0; i < NI; ++i)
+          if (u0 + uu < U)
+            acc[i][u0 + uu] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh[uu], acc[i][u0 + uu], 0, 0, 0);
+#pragma unroll
+      for (int uu = 0; uu < UC; ++uu)
+#pragma unroll
+        for (int i = 0; i < NI; ++i)
+          if (u0 + uu < U)
+            acc[i][u0 + uu] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[uu], acc[i][u0 + uu], 0, 0, 0);
+#pragma unroll
+      for (int uu = 0; uu < UC; ++uu)
+#pragma unroll
+        for (int i = 0; i < NI; ++i)
+          if (u0 + uu < U)
+            acc[i][u0 + uu] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[uu], acc[i][u0 + uu], 0, 0, 0);
+    }
+  };
+  // A operands from the LDS table
+  auto gemm_n = [&](auto ni, const _Float16* Bs) {
+    constexpr int NI = decltype(ni)::value;
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
-      half8 ah[NU], al[NU];
+      half8 ah[NI > 0 ? NI : 1], al[NI > 0 ? NI : 1];
 #pragma unroll
-      for (int i = 0; i < NU; ++i) {
-        const int unit = wv + 4 * i;
-        if (unit < 2 * J) {
+      for (int i = 0; i < NI; ++i) {
+        {
           uint4 dh, dl;   // k-slot pairs (Re, Im) of A: part 0 (cos, sin), part 1 (-sin, cos)
           unsigned* ph = reinterpret_cast<unsigned*>(&dh);
           unsigned* pl = reinterpret_cast<unsigned*>(&dl);
@@ -1281,49 +1328,13 @@ __device__ __forceinline__ void w3_body(const W3Args& a, const unsigned short* _
           al[i] = __builtin_bit_cast(half8, dl);
         }
       }
-      // one product (hi hi, hi lo, lo hi) at a time over the units and a pair of
-      // n-tiles, so consecutive MFMAs write different accumulators; the wave's
-      // unit count is a compile-time branch, so each sequence has no branch inside
-      auto mma = [&](auto ni) {
-        constexpr int NI = decltype(ni)::value;
-        constexpr int UC = 2;
-#pragma unroll
-        for (int u0 = 0; u0 < U; u0 += UC) {
-          half8 bh[UC], bl[UC];
-#pragma unroll
-          for (int uu = 0; uu < UC; ++uu) {
-            if (u0 + uu < U) {
-              // MR: column c at position (c mod 16) U + c / 16, so the pair stage's
-              // columns (16 apart) are written to consecutive positions
-              const int bcol = Sh::MR ? lr * U + u0 + uu : 16 * (u0 + uu) + lr;
-              const _Float16* bq = Bs + bcol * PK + 32 * ks + 8 * q;
-              bh[uu] = *reinterpret_cast<const half8*>(bq);
-              bl[uu] = *reinterpret_cast<const half8*>(bq + (w + 1) * PK);
-            }
-          }
-#pragma unroll
-          for (int uu = 0; uu < UC; ++uu)
-#pragma unroll
-            for (int i = 0; i < NI; ++i)
-              if (u0 + uu < U)
-                acc[i][u0 + uu] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh[uu], acc[i][u0 + uu], 0, 0, 0);
-#pragma unroll
-          for (int uu = 0; uu < UC; ++uu)
-#pragma unroll
-            for (int i = 0; i < NI; ++i)
-              if (u0 + uu < U)
-                acc[i][u0 + uu] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[uu], acc[i][u0 + uu], 0, 0, 0);
-#pragma unroll
-          for (int uu = 0; uu < UC; ++uu)
-#pragma unroll
-            for (int i = 0; i < NI; ++i)
-              if (u0 + uu < U)
-                acc[i][u0 + uu] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[uu], acc[i][u0 + uu], 0, 0, 0);
-        }
-      };
-      if (nUnits == NU) mma(std::integral_constant<int, NU>{});
-      else mma(std::integral_constant<int, NU - 1>{});
+      mma_units(ni, ah, al, Bs, ks);
     }
+  };
+  auto gemm = [&](int rd, const _Float16* Bs) {
+    (void)rd;
+    if (nUnits == NU) gemm_n(std::integral_constant<int, NU>{}, Bs);
+    else gemm_n(std::integral_constant<int, NU - 1>{}, Bs);
   };
 
   if constexpr (Sh::MR) {
@@ -1368,16 +1379,22 @@ __device__ __forceinline__ void w3_body(const W3Args& a, const unsigned short* _
 
     // pass A of round rd (fr holds its words; round rd + 1's are loaded here)
     // into exchange buffer rd & 1
-    auto passA = [&](int rd) {
+    struct PassA {
+      bool big;
+      half8 ah[2];
+      float xs[2][4][2];
+      floatx4 acc2[2][2];
+    };
+    auto passA_main = [&](int rd, PassA& st) {
       unsigned any = 0;
 #pragma unroll
       for (int t = 0; t < 2; ++t)
 #pragma unroll
         for (int e = 0; e < 4; ++e) any |= fr[t][e];
       // a sample >= 4096 in the wave's words: the exact fp16 remainder as a third product
-      const bool big = __builtin_amdgcn_ballot_w64((any & 0xF000F000u) != 0u) != 0;
-      half8 ah[2];
-      float xs[2][4][2];
+      st.big = __builtin_amdgcn_ballot_w64((any & 0xF000F000u) != 0u) != 0;
+      half8 (&ah)[2] = st.ah;
+      float (&xs)[2][4][2] = st.xs;
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
         uint4 hw;
@@ -1391,7 +1408,7 @@ __device__ __forceinline__ void w3_body(const W3Args& a, const unsigned short* _
         ah[t] = __builtin_bit_cast(half8, hw);
       }
       load_mr(rd + 1 < nrd ? rd + 1 : nrd - 1);   // (past the end: a row already read, unused)
-      floatx4 acc2[2][2];
+      floatx4 (&acc2)[2][2] = st.acc2;
 #pragma unroll
       for (int t = 0; t < 2; ++t)
 #pragma unroll
@@ -1401,7 +1418,14 @@ __device__ __forceinline__ void w3_body(const W3Args& a, const unsigned short* _
 #pragma unroll
         for (int nt = 0; nt < 2; ++nt)
           acc2[t][nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[t], bA[nt][1], acc2[t][nt], 0, 0, 0);
-      if (big) {
+    };
+    // a wave whose samples reach 4096: the exact fp16 remainder as a third
+    // product (a branch of its own after the round's other work)
+    auto passA_fix = [&](PassA& st) {
+      if (st.big) {
+        const half8 (&ah)[2] = st.ah;
+        const float (&xs)[2][4][2] = st.xs;
+        floatx4 (&acc2)[2][2] = st.acc2;
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
           uint4 lw;
@@ -1418,8 +1442,11 @@ __device__ __forceinline__ void w3_body(const W3Args& a, const unsigned short* _
             acc2[t][nt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bA[nt][0], acc2[t][nt], 0, 0, 0);
         }
       }
-      // lane (a, q) holds Y[l][a] of tile rows 4q + e: round row 4 wave + 2t +
-      // (q >> 1), l = 4 (q & 1) + e
+    };
+    // lane (a, q) holds Y[l][a] of tile rows 4q + e: round row 4 wave + 2t +
+    // (q >> 1), l = 4 (q & 1) + e
+    auto passA_store = [&](int rd, const PassA& st) {
+      const floatx4 (&acc2)[2][2] = st.acc2;
       unsigned char* eb = rowBase + (rd & 1) * EBUF + (4 * wave + (q >> 1)) * Sh::ERP + pa * Sh::EPP + wa * 8 +
                           64 * (q & 1);
 #pragma unroll
@@ -1427,6 +1454,12 @@ __device__ __forceinline__ void w3_body(const W3Args& a, const unsigned short* _
 #pragma unroll
         for (int e = 0; e < 4; ++e)
           *reinterpret_cast<float2*>(eb + 2 * t * Sh::ERP + 16 * e) = make_float2(acc2[t][0][e], acc2[t][1][e]);
+    };
+    auto passA = [&](int rd) {
+      PassA st;
+      passA_main(rd, st);
+      passA_fix(st);
+      passA_store(rd, st);
     };
     // pass B of round rd (exchange buffer rd & 1), pairs wave and wave + 4;
     // the window columns go to B buffer rd & 1 as fp16 hi / lo
@@ -1461,24 +1494,64 @@ __device__ __forceinline__ void w3_body(const W3Args& a, const unsigned short* _
       }
     };
 
-    // (nrd >= 2: H is at least 32)
-    __syncthreads();   // tables staged
-    passA(0);
-    passA(1);
-    __syncthreads();
-    pairs(0);
-    __syncthreads();
-    for (int rd = 0; rd + 2 < nrd; ++rd) {
-      pairs(rd + 1);
-      gemm(rd, Bs0 + (rd & 1) * BSTRIDE);
-      passA(rd + 2);
+    // The column product's A operands come precomputed per window (every
+    // frame of the batch uses the same ones): unit u of 16-row block yb at
+    // atab + (yb 2 J + u) 128, 16-byte words hi then lo, lane-contiguous, read
+    // one round ahead from L2.
+    const uint4* atab = reinterpret_cast<const uint4*>(tabs + w3_atab_off(H, L, M)) +
+                        (size_t)p * (H / 16) * (2 * J) * 128 + lane;
+    auto load_a = [&](auto ni, int yb, uint4 (&an)[NU][2]) {
+      constexpr int NI = decltype(ni)::value;
+#pragma unroll
+      for (int i = 0; i < NI; ++i) {
+        const uint4* s = atab + (size_t)(yb * 2 * J + wv + 4 * i) * 128;
+        an[i][0] = s[0];
+        an[i][1] = s[64];
+      }
+    };
+    auto gemm_a = [&](auto ni, const uint4 (&an)[NU][2], const _Float16* Bs) {
+      constexpr int NI = decltype(ni)::value;
+      half8 ah[NI > 0 ? NI : 1], al[NI > 0 ? NI : 1];
+#pragma unroll
+      for (int i = 0; i < NI; ++i) {
+        ah[i] = __builtin_bit_cast(half8, an[i][0]);
+        al[i] = __builtin_bit_cast(half8, an[i][1]);
+      }
+      mma_units(ni, ah, al, Bs, 0);
+    };
+    // (nrd >= 2: H is at least 32).  A round's pass B, column product and
+    // pass A touch different buffers; up to pass A's rare correction they form
+    // one block, so their MFMA and VALU work can interleave.
+    auto rounds = [&](auto ni) {
+      uint4 an[NU][2];   // A operands of the next column product
+      load_a(ni, 0, an);
+      __syncthreads();   // tables staged
+      passA(0);
+      passA(1);
       __syncthreads();
-    }
-    pairs(nrd - 1);
-    gemm(nrd - 2, Bs0 + (nrd & 1) * BSTRIDE);
-    __syncthreads();
-    gemm(nrd - 1, Bs0 + ((nrd - 1) & 1) * BSTRIDE);
-    __syncthreads();
+      pairs(0);
+      __syncthreads();
+      for (int rd = 0; rd + 2 < nrd; ++rd) {
+        PassA st;
+        pairs(rd + 1);
+        gemm_a(ni, an, Bs0 + (rd & 1) * BSTRIDE);
+        load_a(ni, rd + 1, an);
+        passA_main(rd + 2, st);
+        // the scheduler's small-GEMM interleave of LDS reads with MFMAs
+        __builtin_amdgcn_iglp_opt(0);
+        passA_fix(st);
+        passA_store(rd + 2, st);
+        __syncthreads();
+      }
+      pairs(nrd - 1);
+      gemm_a(ni, an, Bs0 + (nrd & 1) * BSTRIDE);
+      load_a(ni, nrd - 1, an);
+      __syncthreads();
+      gemm_a(ni, an, Bs0 + ((nrd - 1) & 1) * BSTRIDE);
+      __syncthreads();
+    };
+    if (nUnits == NU) rounds(std::integral_constant<int, NU>{});
+    else rounds(std::integral_constant<int, NU - 1>{});
   } else {
     // Two B buffers: round rd's rows go to buffer rd & 1, so one barrier per
     // round separates the writers of a buffer from its readers (a buffer is
@@ -1518,9 +1591,12 @@ __device__ __forceinline__ void w3_body(const W3Args& a, const unsigned short* _
       for (int bb = 0; bb < 4; ++bb)
         r[s2][bb] = *reinterpret_cast<const float2*>(rp + ro + (unsigned)((s2 + MMy * bb) * w) * 8u);
   };
+  // the epilogue's passes and overlap units rotate with the frame as well (a
+  // wave with a second pass sits on a different SIMD for every other frame)
+  const int we = (wv + 2) & 3;
   float2 fcur[MMx], rcur[USy][4];
-  if (wave < J) load_factor(wave, fcur);
-  if (wave < U) load_ref(wave, rcur);
+  if (we < J) load_factor(we, fcur);
+  if (we < U) load_ref(we, rcur);
 
   // window plane: F / xscale
   const float inv = 1.f / a.xscale;
@@ -1543,7 +1619,7 @@ __device__ __forceinline__ void w3_body(const W3Args& a, const unsigned short* _
     constexpr int MM = MMx, US = MM / 4;
     const bool hasCal = a.cal.p != nullptr;
     const float2 cb = a.cal.at(p, b);
-    for (int ps = wave; ps < J; ps += 4) {
+    for (int ps = we; ps < J; ps += 4) {
       const int j = 16 * ps + t16;
       float2* row = fbuf + j * WP;
       float2 fnx[MM];
@@ -1595,7 +1671,7 @@ __device__ __forceinline__ void w3_body(const W3Args& a, const unsigned short* _
     // reference wave (loaded a pass ahead) and fields through uniform bases
     // and 32-bit lane offsets
     char* out = reinterpret_cast<char*>(fields + (size_t)(out_pos(g, b) * g.P + p) * (h * w));
-    for (int ps = wave; ps < U; ps += 4) {
+    for (int ps = we; ps < U; ps += 4) {
       const int c = 16 * ps + tc;
       float2* col = fbuf + c;
       const unsigned ro = (unsigned)(lc * US * w + c) * 8u;   // element (k, c) at ro + (k - lc US) w 8
@@ -1644,10 +1720,10 @@ __device__ __forceinline__ void w3_body(const W3Args& a, const unsigned short* _
       }
     }
   }
-  if (a.hgx16) w3_overlap<U, J>(a, fbuf, vmax, red, p, out_pos(g, b), wave, lane);
+  if (a.hgx16) w3_overlap<U, J>(a, fbuf, vmax, red, p, out_pos(g, b), we, lane);
   else if (a.hgx32) {
     __syncthreads();   // the whole plane is written before any wave reads it
-    w3_overlap32<U, J>(a, fbuf, w3_smem + a.ovlOff, p, out_pos(g, b), wave, lane);
+    w3_overlap32<U, J>(a, fbuf, w3_smem + a.ovlOff, p, out_pos(g, b), we, lane);
   }
 }
 
@@ -1681,6 +1757,38 @@ struct W3Multi {
 // Tables of the v3 kernel, from FP64: [H] cos and sin of 2 pi m / H as fp16
 // hi/lo pairs {(cos_hi, sin_hi), (cos_lo, sin_lo)}; [L][M + 1]
 // exp(-2 pi i l a / NR); [M] exp(-2 pi i a / W).
+// Column-product A operands of the MFMA row path, per window: unit (jt, part)
+// of 16-row block yb, lane (lr, q) holding window row ky = (16 jt + lr + SY +
+// H / 2) mod H against rows y = 16 yb + 4 q + e (e = 0..3): the staged table's
+// fp16 hi / lo words of w_H^(ky y), part 1 as (-sin, cos).  [P][H / 16][2 J]
+// [hi, lo][64] uint4.  Reads the first H entries of tab (k_w3_tables).
+__global__ void k_w3_atab(Geometry g, const float2* __restrict__ tab, uint4* __restrict__ at) {
+  const int H = g.H, J = g.h / 16, nyb = H / 16;
+  const int n = g.P * nyb * 2 * J * 64;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int lane = e & 63;
+  int r = e >> 6;
+  const int unit = r % (2 * J);
+  r /= 2 * J;
+  const int yb = r % nyb, p = r / nyb;
+  const int lr = lane & 15, q = lane >> 4, jt = unit >> 1, part = unit & 1;
+  int ky = (16 * jt + lr + g.SY[p] + H / 2) % H;
+  if (ky < 0) ky += H;
+  const uint2* t2 = reinterpret_cast<const uint2*>(tab);
+  unsigned hw[4], lw[4];
+#pragma unroll
+  for (int e2 = 0; e2 < 4; ++e2) {
+    const int y = 16 * yb + 4 * q + e2;
+    const uint2 t = t2[(int)(((long long)ky * y) % H)];
+    hw[e2] = part ? (__builtin_amdgcn_alignbit(t.x, t.x, 16) ^ 0x8000u) : t.x;
+    lw[e2] = part ? (__builtin_amdgcn_alignbit(t.y, t.y, 16) ^ 0x8000u) : t.y;
+  }
+  uint4* dst = at + (size_t)((p * nyb + yb) * 2 * J + unit) * 128;
+  dst[lane] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+  dst[64 + lane] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+}
+
 __global__ void k_w3_tables(int H, int L, int M, float2* __restrict__ tab) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   const int NR = L * M;
@@ -1788,7 +1896,7 @@ static float w3_xscale(int W) {
 size_t window_v3_table_elems(const Geometry& g) {
   int L, M;
   if (!w3_lm(g.W / 2, L, M)) return 0;
-  if (L == 8 && M == 16) return (size_t)w3_mr_off(g.H, L, M) + kMrMats * kMrMatHalfs / 4;
+  if (L == 8 && M == 16) return (size_t)w3_atab_off(g.H, L, M) + w3_atab_elems(g.P, g.H, g.h / 16);
   return (size_t)g.H + (size_t)L * (M + 1) + M;
 }
 
@@ -1801,6 +1909,10 @@ hipError_t launch_window_v3_tables(const Geometry& g, float2* tab, hipStream_t s
     const int nf = kMrMats * 2 * 64 * 8;
     k_w3_mr_tables<<<(nf + 255) / 256, 256, 0, s>>>((double)w3_xscale(g.W) / kMrScaleA,
                                                      reinterpret_cast<_Float16*>(tab + w3_mr_off(g.H, L, M)));
+    if (g.H % 16 == 0 && g.h % 16 == 0 && g.h > 0) {
+      const int na = g.P * (g.H / 16) * 2 * (g.h / 16) * 64;
+      k_w3_atab<<<(na + 255) / 256, 256, 0, s>>>(g, tab, reinterpret_cast<uint4*>(tab + w3_atab_off(g.H, L, M)));
+    }
   }
   return hipGetLastError();
 }
