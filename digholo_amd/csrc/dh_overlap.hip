@@ -460,6 +460,20 @@ __global__ __launch_bounds__(64 * NT2, split_min_blocks(KS1, NT2)) void k_overla
   floatx4 acc0[NT2], acc1[NT2];
 #pragma unroll
   for (int nt = 0; nt < NT2; ++nt) acc0[nt] = acc1[nt] = floatx4{0.f, 0.f, 0.f, 0.f};
+  // stage-2 basis of chunk ch, HGy[16 nt + lr][16 ch + 4 q .. + 3] (rows >= G
+  // zero), one chunk ahead of its use
+  half4 b2nh[NT2], b2nl[NT2];
+  auto load_b2 = [&](int ch) {
+#pragma unroll
+    for (int nt = 0; nt < NT2; ++nt) {
+      if (16 * mt + 16 * nt < G) {
+        const _Float16* brow = by + (16 * nt + lr) * h + 16 * ch + 4 * q;
+        b2nh[nt] = *reinterpret_cast<const half4*>(brow);
+        b2nl[nt] = *reinterpret_cast<const half4*>(brow + Gp * h);
+      }
+    }
+  };
+  load_b2(0);
   for (int ch = 0; ch < J; ++ch) {
     __syncthreads();   // chunk ch is in its buffer; the other buffer's readers are done
     const _Float16* cb = pl + (ch & 1) * 4 * PS + lr * SP + 8 * q;
@@ -478,17 +492,14 @@ __global__ __launch_bounds__(64 * NT2, split_min_blocks(KS1, NT2)) void k_overla
       d0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(alr, b1h[ks], d0, 0, 0, 0);
       d1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ali, b1h[ks], d1, 0, 0, 0);
     }
-    // stage-2 basis of this chunk, HGy[16 nt + lr][16 ch + 4 q .. + 3] (rows >= G
-    // zero), from L2 while the next chunk is stored
+    // stage-2 basis of this chunk (loaded a chunk ago); the next chunk's from L2
     half4 b2h[NT2], b2l[NT2];
 #pragma unroll
     for (int nt = 0; nt < NT2; ++nt) {
-      if (16 * mt + 16 * nt < G) {
-        const _Float16* brow = by + (16 * nt + lr) * h + 16 * ch + 4 * q;
-        b2h[nt] = *reinterpret_cast<const half4*>(brow);
-        b2l[nt] = *reinterpret_cast<const half4*>(brow + Gp * h);
-      }
+      b2h[nt] = b2nh[nt];
+      b2l[nt] = b2nl[nt];
     }
+    if (ch + 1 < J) load_b2(ch + 1);
     // the next chunk into the other buffer (loaded a chunk ago); the one after in flight
     if (ch + 1 < J) {
       stage((ch + 1) & 1);
