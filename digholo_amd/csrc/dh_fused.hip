@@ -1368,7 +1368,7 @@ __device__ __forceinline__ void w3_body(const W3Args& a, const unsigned short* _
         for (int hl = 0; hl < 2; ++hl) bP[pp][nt][hl] = *reinterpret_cast<const half8*>(m + (nt * 2 + hl) * 512);
       const int k = kmr(v, lr);
       kc[pp] = k - c0;
-      kpos[pp] = (unsigned)kc[pp] < (unsigned)w ? (kc[pp] & 15) * U + (kc[pp] >> 4) : w;   // w: discarded
+      kpos[pp] = (unsigned)kc[pp] < (unsigned)w ? (kc[pp] & 15) * U + (kc[pp] >> 4) : w;   // w: not written
       dca[pp] = k == 0 ? dcfix : 0.f;
     }
     // this lane's pass-A column a = lr in the exchange: pair and position in it
@@ -1488,9 +1488,13 @@ __device__ __forceinline__ void w3_body(const W3Args& a, const unsigned short* _
         unsigned* lp = reinterpret_cast<unsigned*>(&xl);
 #pragma unroll
         for (int e = 0; e < 4; ++e) split_f16x2(c[0][e] + dca[pp], c[1][e], hp[e], lp[e]);
-        _Float16* bp = Bs + kpos[pp] * PK + 8 * q;
-        *reinterpret_cast<uint4*>(bp) = xh;
-        *reinterpret_cast<uint4*>(bp + (w + 1) * PK) = xl;
+        // only the window's columns are written (lanes outside it would all
+        // hit the spare column, one address: a bank conflict per write)
+        if (kpos[pp] < w) {
+          _Float16* bp = Bs + kpos[pp] * PK + 8 * q;
+          *reinterpret_cast<uint4*>(bp) = xh;
+          *reinterpret_cast<uint4*>(bp + (w + 1) * PK) = xl;
+        }
       }
     };
 
