@@ -113,17 +113,17 @@ __global__ void k_batch_sum(const float* __restrict__ pw, int B, int P, int N, f
 // int16 planes R, I = round(scale * V) with saturation; scale[p][B + 1]
 __global__ void k_fields16(const float2* __restrict__ fields, int B, int P, int hw, const float* __restrict__ scale,
                            short* __restrict__ re, short* __restrict__ im) {
-  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t total = (size_t)B * P * hw;
-  if (e >= total) return;
-  const int bp = (int)(e / hw);
-  const int b = bp / P, p = bp - b * P;
-  const float s = scale[(size_t)p * (B + 1) + b];
-  const float2 v = fields[e];
-  const float r = fminf(fmaxf(rintf(__fmul_rn(s, v.x)), -32768.f), 32767.f);
-  const float i = fminf(fmaxf(rintf(__fmul_rn(s, v.y)), -32768.f), 32767.f);
-  re[e] = (short)r;
-  im[e] = (short)i;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const int bp = (int)(e / hw);
+    const int b = bp / P, p = bp - b * P;
+    const float s = scale[(size_t)p * (B + 1) + b];
+    const float2 v = fields[e];
+    const float r = fminf(fmaxf(rintf(__fmul_rn(s, v.x)), -32768.f), 32767.f);
+    const float i = fminf(fmaxf(rintf(__fmul_rn(s, v.y)), -32768.f), 32767.f);
+    re[e] = (short)r;
+    im[e] = (short)i;
+  }
 }
 
 }  // namespace
@@ -149,7 +149,7 @@ hipError_t launch_fields16(const float2* fields, int B, int P, int hw, const flo
                            hipStream_t s) {
   const size_t total = (size_t)B * P * hw;
   if (total == 0) return hipErrorInvalidValue;
-  k_fields16<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(fields, B, P, hw, scale, re, im);
+  k_fields16<<<stride_blocks(total, 256), 256, 0, s>>>(fields, B, P, hw, scale, re, im);
   return hipGetLastError();
 }
 

@@ -1227,9 +1227,8 @@ int enqueue_batch(Handle& h, int step = 0) {
   const int A = std::max(1, c.avgCount);      // frames averaged into each batch element
   const size_t nFrames = B * (size_t)A;
   if (nFrames > (size_t)dh::kMaxBatchFrames) {
-    // the kernels take the frame index from gridDim.y / gridDim.z (at most 65535)
-    fprintf(stderr, "digHolo: at most %d frames per batch on this device (got %zu); split the batch\n",
-            dh::kMaxBatchFrames, nFrames);
+    fprintf(stderr, "digHolo: at most %d frames per batch (got %zu); split the batch\n", dh::kMaxBatchFrames,
+            nFrames);
     return DIGHOLO_ERROR_INVALIDARGUMENT;
   }
   if (A > 1 && g.L > 1) {

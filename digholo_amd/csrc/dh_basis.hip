@@ -30,18 +30,19 @@ __device__ __forceinline__ int group_of(int k) {
 // one thread per output coefficient; rows = B * P
 __global__ void k_lg_apply(const float2* __restrict__ in, float2* __restrict__ out, size_t rows, int M,
                            const float2* __restrict__ tab) {
-  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= rows * M) return;
-  const size_t r = e / M;
-  const int k = (int)(e - r * M);
-  const int g = group_of(k);
-  const int g0 = g * (g + 1) / 2, kk = k - g0;
-  // sum_{h<g} (h+1)^2 = g (g+1) (2g+1) / 6
-  const float2* U = tab + (size_t)g * (g + 1) * (2 * g + 1) / 6 + (size_t)kk * (g + 1);
-  const float2* x = in + r * M + g0;
-  float2 acc = make_float2(0.f, 0.f);
-  for (int c = 0; c <= g; ++c) acc = cmla(acc, U[c], x[c]);
-  out[e] = acc;
+  const size_t total = rows * M;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = e / M;
+    const int k = (int)(e - r * M);
+    const int g = group_of(k);
+    const int g0 = g * (g + 1) / 2, kk = k - g0;
+    // sum_{h<g} (h+1)^2 = g (g+1) (2g+1) / 6
+    const float2* U = tab + (size_t)g * (g + 1) * (2 * g + 1) / 6 + (size_t)kk * (g + 1);
+    const float2* x = in + r * M + g0;
+    float2 acc = make_float2(0.f, 0.f);
+    for (int c = 0; c <= g; ++c) acc = cmla(acc, U[c], x[c]);
+    out[e] = acc;
+  }
 }
 
 // 64 x 64 output tile per 256-thread workgroup, 4 x 4 per thread, k-steps of 16
@@ -132,16 +133,23 @@ __global__ __launch_bounds__(256) void k_basis_synth(Geometry g, const float2* _
 hipError_t launch_lg_apply(const float2* in, float2* out, size_t rows, int M, const float2* tab, hipStream_t s) {
   const size_t n = rows * M;
   if (n == 0) return hipSuccess;
-  k_lg_apply<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(in, out, rows, M, tab);
+  k_lg_apply<<<stride_blocks(n, 256), 256, 0, s>>>(in, out, rows, M, tab);
   return hipGetLastError();
 }
 
 hipError_t launch_custom_apply(const float2* in, int ldIn, int rows, int nIn, const float2* T, int ldT, int nOut,
                                float2* out, hipStream_t s) {
   if (rows <= 0 || nOut <= 0) return hipSuccess;
-  dim3 grid((nOut + kCT - 1) / kCT, (rows + kCT - 1) / kCT);
-  k_custom_apply<<<grid, 256, 0, s>>>(in, ldIn, rows, nIn, T, ldT, nOut, out);
-  return hipGetLastError();
+  // grid y covers at most 65535 row tiles: larger row counts run in chunks
+  constexpr int kRowsPerLaunch = 65535 * kCT;
+  for (int r0 = 0; r0 < rows; r0 += kRowsPerLaunch) {
+    const int nr = rows - r0 < kRowsPerLaunch ? rows - r0 : kRowsPerLaunch;
+    dim3 grid((nOut + kCT - 1) / kCT, (nr + kCT - 1) / kCT);
+    k_custom_apply<<<grid, 256, 0, s>>>(in + (size_t)r0 * ldIn, ldIn, nr, nIn, T, ldT, nOut, out + (size_t)r0 * nOut);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_basis_synth(const Geometry& g, const float2* W, int modes, int ldW, const float* hgx,

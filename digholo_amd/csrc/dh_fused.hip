@@ -79,7 +79,7 @@ __global__ __launch_bounds__(kFThreads, 4) void k_fused_window(FusedArgs a, Fuse
                                                                float2* __restrict__ fields) {
   extern __shared__ float2 fsm[];
   const Geometry& g = a.g;
-  const int p = blockIdx.x, b = blockIdx.y;
+  const int p = blockIdx.x, b = blockIdx.y + g.b0;
   const int n = a.row.n, Hc = a.Hc, R = a.R, RB = a.RB;
   const int H = g.H, w = g.w, h = g.h, hw = h * w;
   const int tid = threadIdx.x;
@@ -214,9 +214,12 @@ static hipError_t fused_launch(const FusedArgs& a, const FusedMagic& mg, const f
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  dim3 grid(a.g.P, a.g.B);
-  k_fused_window<FPT><<<grid, kFThreads, lds, s>>>(a, mg, twW, twH, frames, u16, factor, ref, fields);
-  return hipGetLastError();
+  return for_frame_chunks(a.g.B, [&](int b0, int nb) {
+    FusedArgs ac = a;
+    ac.g.b0 = b0;
+    k_fused_window<FPT><<<dim3(a.g.P, nb), kFThreads, lds, s>>>(ac, mg, twW, twH, frames, u16, factor, ref, fields);
+    return hipGetLastError();
+  });
 }
 
 // ===========================================================================
@@ -266,7 +269,7 @@ __global__ __launch_bounds__(kFThreads, 4) void k_window_mfma(FusedArgs a, Fused
                                                               float2* __restrict__ fields) {
   extern __shared__ __align__(16) unsigned char wm_smem[];
   const Geometry& g = a.g;
-  const int p = blockIdx.x, b = blockIdx.y;
+  const int p = blockIdx.x, b = blockIdx.y + g.b0;
   const int n = a.row.n, RB = a.RB, H = g.H, w = g.w, h = g.h, hw = h * w;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, q = lane >> 4;
@@ -460,11 +463,15 @@ static hipError_t wm_launch(const FusedArgs& a, const FusedMagic& mg, float xsca
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  dim3 grid(a.g.P, a.g.B);
-  k_window_mfma<CPW, NU><<<grid, kFThreads, lds, s>>>(a, mg, xscale, twW,
-                                                       static_cast<const unsigned short*>(frames),
-                                                       static_cast<const _Float16*>(dtab), factor, ref, fields);
-  return hipGetLastError();
+  return for_frame_chunks(a.g.B, [&](int b0, int nb) {
+    FusedArgs ac = a;
+    ac.g.b0 = b0;
+    k_window_mfma<CPW, NU><<<dim3(a.g.P, nb), kFThreads, lds, s>>>(ac, mg, xscale, twW,
+                                                                  static_cast<const unsigned short*>(frames),
+                                                                  static_cast<const _Float16*>(dtab), factor, ref,
+                                                                  fields);
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_window_mfma(const FusedArgs& a0, const float2* twW, const void* frames_u16, const void* dtab,

@@ -109,7 +109,7 @@ __global__ __launch_bounds__(kGThreads) void k_grad_backproject(
     const float2* __restrict__ fields, const float2* __restrict__ ref, const double* __restrict__ pxy2,
     float2* __restrict__ q1, float2* __restrict__ q2, double* __restrict__ partial) {
   extern __shared__ __align__(16) unsigned char gsm[];
-  const int p = blockIdx.x, b = blockIdx.y;
+  const int p = blockIdx.x, b = blockIdx.y + g.b0;
   const int G = g.G, M = g.M, fw = g.fw, fh = g.fh;
   float2* gs = reinterpret_cast<float2*>(gsm);                 // [M]
   float2* T1 = gs + M;                                          // [G][kGXC]
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(kGThreads) void k_grad_kspace(Geometry g, const flo
                                                            const double* __restrict__ kx,
                                                            const double* __restrict__ ky,
                                                            double* __restrict__ partial) {
-  const int p = blockIdx.x, b = blockIdx.y;
+  const int p = blockIdx.x, b = blockIdx.y + g.b0;
   const int fw = g.fw, fh = g.fh, N = fw * fh;
   const size_t bp = (size_t)b * g.P + p;
   const float2* a = x1 + bp * N;
@@ -245,15 +245,23 @@ hipError_t launch_grad_backproject(const Geometry& g, double invB, const float2*
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
-  k_grad_backproject<<<dim3(g.P, g.B), kGThreads, lds, s>>>(g, invB, coefs, gExtra, hgx, hgy, dhgx, dhgy, fields, ref,
-                                                            pxy2, q1, q2, partial);
-  return hipGetLastError();
+  return for_frame_chunks(g.B, [&](int b0, int nb) {
+    Geometry gc = g;
+    gc.b0 = b0;
+    k_grad_backproject<<<dim3(g.P, nb), kGThreads, lds, s>>>(gc, invB, coefs, gExtra, hgx, hgy, dhgx, dhgy, fields,
+                                                             ref, pxy2, q1, q2, partial);
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_grad_kspace(const Geometry& g, const float2* x1, const float2* x2, const double* kx,
                               const double* ky, double* partial, hipStream_t s) {
-  k_grad_kspace<<<dim3(g.P, g.B), kGThreads, 0, s>>>(g, x1, x2, kx, ky, partial);
-  return hipGetLastError();
+  return for_frame_chunks(g.B, [&](int b0, int nb) {
+    Geometry gc = g;
+    gc.b0 = b0;
+    k_grad_kspace<<<dim3(g.P, nb), kGThreads, 0, s>>>(gc, x1, x2, kx, ky, partial);
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_grad_reduce(int B, int P, const double* partial, double* out, hipStream_t s) {

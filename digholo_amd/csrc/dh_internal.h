@@ -8,9 +8,12 @@ namespace dh {
 
 constexpr int kMaxPol = 2;
 constexpr int kMaxStages = 12;
-// Frames of one batch (averaged frames included): kernels index frames by
-// blockIdx.y or blockIdx.z, whose grid extent is limited to 65535.
-constexpr int kMaxBatchFrames = 65535;
+// Frames of one launch along grid y or z (the grid extent there is limited to
+// 65535): kernels that index frames by blockIdx.y / blockIdx.z add
+// Geometry::b0, and their launchers cover a batch in chunks of this many.
+constexpr int kMaxGridFrames = 65535;
+// Frames of one batch (averaged frames included).
+constexpr int kMaxBatchFrames = 1 << 24;
 
 // Bytes of the per-GPU scratch array of the allocation-free batch entry point.
 constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
@@ -44,7 +47,26 @@ struct Geometry {
   // wavelength sweep: L wavelengths; frame and output ordering, 0 wavelength
   // the fast axis, 1 the slow one; subB = B / L frames per wavelength
   int L, ordIn, ordOut, subB;
+  int b0;                     // first frame of a launch covering part of the batch (grid y / z)
 };
+
+// Workgroups for an elementwise kernel that strides over n elements: one
+// element per thread up to 2^22 workgroups, several per thread beyond (a
+// launch's work-item count in one dimension must stay below 2^32).
+inline unsigned stride_blocks(size_t n, int threads) {
+  const size_t b = (n + threads - 1) / threads;
+  return (unsigned)(b < ((size_t)1 << 22) ? b : ((size_t)1 << 22));
+}
+
+// f(b0, nb) for consecutive chunks of at most kMaxGridFrames frames of n.
+template <class F>
+inline hipError_t for_frame_chunks(int n, F&& f) {
+  for (int b0 = 0; b0 < n; b0 += kMaxGridFrames) {
+    const hipError_t e = f(b0, n - b0 < kMaxGridFrames ? n - b0 : kMaxGridFrames);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
 
 // Wavelength index of input frame b, the batch position its field and
 // coefficients take, and the wavelength index of output position o

@@ -109,7 +109,7 @@ __global__ __launch_bounds__(kThreads) void k_row_fft(Geometry g, FFTPlan pl, co
   const int n = pl.n;
   float2* A = smem;
   float2* Bf = smem + RB * n;
-  const int b = blockIdx.z, p = blockIdx.y, y0 = blockIdx.x * RB;
+  const int b = blockIdx.z + g.b0, p = blockIdx.y, y0 = blockIdx.x * RB;
   const int rows = min(RB, g.H - y0);
   const size_t fbase = (size_t)b * g.frameH * g.frameW + (size_t)(g.cy[p] + y0) * g.frameW + g.cx[p];
   for (int e = threadIdx.x; e < rows * n; e += blockDim.x) {
@@ -153,7 +153,7 @@ __global__ __launch_bounds__(kThreads) void k_col_fft_window(Geometry g, FFTPlan
   const int n = pl.n;
   float2* A = smem;
   float2* Bf = smem + CB * n;
-  const int b = blockIdx.z, p = blockIdx.y, i0 = blockIdx.x * CB;
+  const int b = blockIdx.z + g.b0, p = blockIdx.y, i0 = blockIdx.x * CB;
   const int cols = min(CB, g.w - i0);
   const float2* in = rowspec + (size_t)(b * g.P + p) * g.H * g.w + i0;
   for (int e = threadIdx.x; e < n * cols; e += blockDim.x) {
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(kThreads) void k_col_fft_full(Geometry g, FFTPlan p
   const int n = pl.n;
   float2* A = smem;
   float2* Bf = smem + CB * n;
-  const int b = blockIdx.z, p = blockIdx.y, i0 = blockIdx.x * CB;
+  const int b = blockIdx.z + g.b0, p = blockIdx.y, i0 = blockIdx.x * CB;
   const int cols = min(CB, g.w - i0);
   float2* base = spec + (size_t)(b * g.P + p) * g.H * g.w + i0;
   for (int e = threadIdx.x; e < n * cols; e += blockDim.x) {
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(kThreads) void k_ifft_rows(Geometry g, FFTPlan pl, 
   const int n = pl.n;
   float2* A = smem;
   float2* Bf = smem + RB * n;
-  const int b = blockIdx.z, p = blockIdx.y, y0 = blockIdx.x * RB;
+  const int b = blockIdx.z + g.b0, p = blockIdx.y, y0 = blockIdx.x * RB;
   const int rows = min(RB, g.h - y0);
   float2* base = win + ((size_t)(b * g.P + p) * g.h + y0) * g.w;
   for (int e = threadIdx.x; e < rows * n; e += blockDim.x) A[e] = base[e];
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(kThreads) void k_ifft_cols_tilt(Geometry g, FFTPlan
   const int n = pl.n;
   float2* A = smem;
   float2* Bf = smem + CB * n;
-  const int b = blockIdx.z, p = blockIdx.y, i0 = blockIdx.x * CB;
+  const int b = blockIdx.z + g.b0, p = blockIdx.y, i0 = blockIdx.x * CB;
   const int cols = min(CB, g.w - i0);
   const size_t base = (size_t)(b * g.P + p) * g.h * g.w + i0;
   for (int e = threadIdx.x; e < n * cols; e += blockDim.x) {
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(kRowThreads) void k_row_fft_ip(Geometry g, FFTPlan 
   float2* tu = smem + n;      // [w] unpack twiddles exp(-2 pi i k / W) of the window columns
   float2* A = tu + w;         // [RB][LD], point k of row r at r LD + pad32(k)
   const int LD = pad_stride(n);
-  const int b = blockIdx.z, p = blockIdx.y, y0 = blockIdx.x * RB;
+  const int b = blockIdx.z + g.b0, p = blockIdx.y, y0 = blockIdx.x * RB;
   const int rows = min(RB, g.H - y0);
   const int c0 = g.c0[p];
   for (int e = threadIdx.x; e < n; e += kRowThreads) tw[e] = pl.tw[e];
@@ -359,7 +359,7 @@ __global__ __launch_bounds__(kColThreads) void k_col_window_iy(Geometry g, FFTPl
   float2* twy = smem + H;
   float2* A = twy + h;        // [CB][LD], point y of column c at c LD + pad32(y)
   const int LD = pad_stride(H);
-  const int b = blockIdx.z, p = blockIdx.y, i0 = blockIdx.x * CB;
+  const int b = blockIdx.z + g.b0, p = blockIdx.y, i0 = blockIdx.x * CB;
   const int cols = min(CB, w - i0);
   const unsigned magC = magic_div((unsigned)cols);
   if (peak && blockIdx.x == 0 && threadIdx.x == 0) peak[out_pos(g, b) * g.P + p] = 0u;
@@ -437,7 +437,7 @@ __global__ __launch_bounds__(kThreads) void k_ifft_rows_tilt(Geometry g, FFTPlan
     const int r = fdiv(e, magN);
     return A[r * LD + pad32(e - r * n)];
   };
-  const int b = blockIdx.z, p = blockIdx.y, y0 = blockIdx.x * RB;
+  const int b = blockIdx.z + g.b0, p = blockIdx.y, y0 = blockIdx.x * RB;
   const int rows = min(RB, g.h - y0);
   for (int e = threadIdx.x; e < n; e += kThreads) tw[e] = pl.tw[e];
   const size_t base = ((size_t)(b * g.P + p) * g.h + y0) * n;
@@ -468,54 +468,54 @@ __global__ __launch_bounds__(kThreads) void k_ifft_rows_tilt(Geometry g, FFTPlan
 // the column-transformed band of the w window columns ([B][P][H][w]).
 __global__ void k_select_window(Geometry g, const float2* __restrict__ spec, const float2* __restrict__ factor,
                                 float2* __restrict__ win, BatchCal cal) {
-  const size_t hw = (size_t)g.h * g.w;
-  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)g.B * g.P * hw) return;
-  const size_t bp = e / hw;
-  const int p = (int)(bp % g.P);
-  const int rem = (int)(e - bp * hw);
-  const int j = rem / g.w, c = rem - j * g.w;
-  int row = (j + g.SY[p] + g.H / 2) % g.H;
-  if (row < 0) row += g.H;
-  const int b = (int)(bp / g.P);
-  const float2 fac = factor[(size_t)(lam_in(g, b) * g.P + p) * hw + rem];
-  win[((size_t)out_pos(g, b) * g.P + p) * hw + rem] = cmul(cmul(spec[(bp * g.H + row) * g.w + c], fac), cal.at(p, b));
+  const size_t hw = (size_t)g.h * g.w, total = (size_t)g.B * g.P * hw;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t bp = e / hw;
+    const int p = (int)(bp % g.P);
+    const int rem = (int)(e - bp * hw);
+    const int j = rem / g.w, c = rem - j * g.w;
+    int row = (j + g.SY[p] + g.H / 2) % g.H;
+    if (row < 0) row += g.H;
+    const int b = (int)(bp / g.P);
+    const float2 fac = factor[(size_t)(lam_in(g, b) * g.P + p) * hw + rem];
+    win[((size_t)out_pos(g, b) * g.P + p) * hw + rem] = cmul(cmul(spec[(bp * g.H + row) * g.w + c], fac), cal.at(p, b));
+  }
 }
 
 // Full-resolution IFFT input (digHolo.cpp:1855-1872, :1938-1950, :2042-2065):
 // the window keeps its own frequency position in a zero W x H plane.
 __global__ void k_scatter_full(Geometry g, const float2* __restrict__ win, float2* __restrict__ plane) {
-  const size_t n = (size_t)g.H * g.W;
-  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)g.B * g.P * n) return;
-  const size_t bp = e / n;
-  const int p = (int)(bp % g.P);
-  const int rem = (int)(e - bp * n);
-  const int row = rem / g.W, col = rem - row * g.W;
-  int j = (row - g.SY[p] - g.H / 2) % g.H;
-  if (j < 0) j += g.H;
-  const int i = col - g.c0[p];
-  float2 v = make_float2(0.f, 0.f);
-  if (j < g.h && i >= 0 && i < g.w) v = win[(bp * g.h + j) * g.w + i];
-  plane[e] = v;
+  const size_t n = (size_t)g.H * g.W, total = (size_t)g.B * g.P * n;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t bp = e / n;
+    const int p = (int)(bp % g.P);
+    const int rem = (int)(e - bp * n);
+    const int row = rem / g.W, col = rem - row * g.W;
+    int j = (row - g.SY[p] - g.H / 2) % g.H;
+    if (j < 0) j += g.H;
+    const int i = col - g.c0[p];
+    float2 v = make_float2(0.f, 0.f);
+    if (j < g.h && i >= 0 && i < g.w) v = win[(bp * g.h + j) * g.w + i];
+    plane[e] = v;
+  }
 }
 
 hipError_t launch_scatter_full(const Geometry& g, const float2* win, float2* plane, hipStream_t s) {
   const size_t n = (size_t)g.B * g.P * g.H * g.W;
-  k_scatter_full<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, win, plane);
+  k_scatter_full<<<stride_blocks(n, 256), 256, 0, s>>>(g, win, plane);
   return hipGetLastError();
 }
 
 // uint16 frames given transposed (digHolo.cpp:1206-1229): out[f][i H + j] = in[f][j W + i]
 __global__ void k_u16_transpose(const unsigned short* __restrict__ in, unsigned short* __restrict__ out, int W,
                                 int H, size_t frames) {
-  const size_t n = (size_t)W * H;
-  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= frames * n) return;
-  const size_t f = e / n;
-  const int o = (int)(e - f * n);
-  const int i = o / H, j = o - i * H;
-  out[e] = in[f * n + (size_t)j * W + i];
+  const size_t n = (size_t)W * H, total = frames * n;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t f = e / n;
+    const int o = (int)(e - f * n);
+    const int i = o / H, j = o - i * H;
+    out[e] = in[f * n + (size_t)j * W + i];
+  }
 }
 
 // In-phase averaging of A Fourier windows per (batch element, pol)
@@ -528,7 +528,7 @@ __global__ __launch_bounds__(256) void k_average_windows(Geometry g, int A, int 
                                                          const float2* __restrict__ winAll,
                                                          float2* __restrict__ win, BatchCal cal) {
   __shared__ float2 red[256 / 64];
-  const int p = blockIdx.x, b = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int p = blockIdx.x, b = blockIdx.y + g.b0, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int hw = g.h * g.w;
   const float inv = 1.f / (float)A;
   const float2 cb = cal.at(p, b);
@@ -578,42 +578,45 @@ __global__ __launch_bounds__(256) void k_average_windows(Geometry g, int A, int 
 
 hipError_t launch_u16_transpose(const void* in, void* out, int W, int H, size_t frames, hipStream_t s) {
   const size_t n = frames * W * H;
-  k_u16_transpose<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(static_cast<const unsigned short*>(in),
-                                                               static_cast<unsigned short*>(out), W, H, frames);
+  k_u16_transpose<<<stride_blocks(n, 256), 256, 0, s>>>(static_cast<const unsigned short*>(in),
+                                                        static_cast<unsigned short*>(out), W, H, frames);
   return hipGetLastError();
 }
 
 hipError_t launch_average_windows(const Geometry& g, int A, int mode, int frames, const float2* winAll, float2* win,
                                   const BatchCal& cal, hipStream_t s) {
   if (g.h * g.w > 256 * kAvgPer) return hipErrorNotSupported;
-  dim3 grid(g.P, g.B);
-  k_average_windows<<<grid, 256, 0, s>>>(g, A, mode, frames, winAll, win, cal);
-  return hipGetLastError();
+  return for_frame_chunks(g.B, [&](int b0, int nb) {
+    Geometry gc = g;
+    gc.b0 = b0;
+    k_average_windows<<<dim3(g.P, nb), 256, 0, s>>>(gc, A, mode, frames, winAll, win, cal);
+    return hipGetLastError();
+  });
 }
 
 // fields = -conj(fields * ref), in place; fields in output order, ref of the
 // position's wavelength
 __global__ void k_apply_tilt(Geometry g, const float2* __restrict__ ref, float2* __restrict__ fields) {
-  const size_t hw = (size_t)g.h * g.w;
-  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)g.B * g.P * hw) return;
-  const size_t bp = e / hw;
-  const int p = (int)(bp % g.P);
-  const int lam = lam_out(g, (int)(bp / g.P));
-  const float2 t = cmul(fields[e], ref[((size_t)lam * g.P + p) * hw + e % hw]);
-  fields[e] = make_float2(-t.x, t.y);
+  const size_t hw = (size_t)g.h * g.w, total = (size_t)g.B * g.P * hw;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t bp = e / hw;
+    const int p = (int)(bp % g.P);
+    const int lam = lam_out(g, (int)(bp / g.P));
+    const float2 t = cmul(fields[e], ref[((size_t)lam * g.P + p) * hw + e % hw]);
+    fields[e] = make_float2(-t.x, t.y);
+  }
 }
 
 hipError_t launch_select_window(const Geometry& g, const float2* spec, const float2* factor, float2* win,
                                 const BatchCal& cal, hipStream_t s) {
   const size_t n = (size_t)g.B * g.P * g.h * g.w;
-  k_select_window<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, spec, factor, win, cal);
+  k_select_window<<<stride_blocks(n, 256), 256, 0, s>>>(g, spec, factor, win, cal);
   return hipGetLastError();
 }
 
 hipError_t launch_apply_tilt(const Geometry& g, const float2* ref, float2* fields, hipStream_t s) {
   const size_t n = (size_t)g.B * g.P * g.h * g.w;
-  k_apply_tilt<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(g, ref, fields);
+  k_apply_tilt<<<stride_blocks(n, 256), 256, 0, s>>>(g, ref, fields);
   return hipGetLastError();
 }
 
@@ -748,9 +751,12 @@ hipError_t launch_row_fft(const Geometry& g, const FFTPlan& pl, const float2* tw
   const size_t lds = 2 * (size_t)RB * pl.n * sizeof(float2);
   hipError_t e = set_lds(k_row_fft, lds);
   if (e != hipSuccess) return e;
-  dim3 grid((g.H + RB - 1) / RB, g.P, g.B);
-  k_row_fft<<<grid, kThreads, lds, s>>>(g, pl, twW, frames, frame_is_u16, rowspec, RB);
-  return hipGetLastError();
+  return for_frame_chunks(g.B, [&](int b0, int nb) {
+    Geometry gc = g;
+    gc.b0 = b0;
+    k_row_fft<<<dim3((g.H + RB - 1) / RB, g.P, nb), kThreads, lds, s>>>(gc, pl, twW, frames, frame_is_u16, rowspec, RB);
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_col_fft_window(const Geometry& g, const FFTPlan& pl, const float2* rowspec,
@@ -759,9 +765,12 @@ hipError_t launch_col_fft_window(const Geometry& g, const FFTPlan& pl, const flo
   const size_t lds = 2 * (size_t)CB * pl.n * sizeof(float2);
   hipError_t e = set_lds(k_col_fft_window, lds);
   if (e != hipSuccess) return e;
-  dim3 grid((g.w + CB - 1) / CB, g.P, g.B);
-  k_col_fft_window<<<grid, kThreads, lds, s>>>(g, pl, rowspec, factor, win, CB, cal);
-  return hipGetLastError();
+  return for_frame_chunks(g.B, [&](int b0, int nb) {
+    Geometry gc = g;
+    gc.b0 = b0;
+    k_col_fft_window<<<dim3((g.w + CB - 1) / CB, g.P, nb), kThreads, lds, s>>>(gc, pl, rowspec, factor, win, CB, cal);
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_col_fft_full(const Geometry& g, const FFTPlan& pl, float2* spec, hipStream_t s) {
@@ -769,9 +778,12 @@ hipError_t launch_col_fft_full(const Geometry& g, const FFTPlan& pl, float2* spe
   const size_t lds = 2 * (size_t)CB * pl.n * sizeof(float2);
   hipError_t e = set_lds(k_col_fft_full, lds);
   if (e != hipSuccess) return e;
-  dim3 grid((g.w + CB - 1) / CB, g.P, g.B);
-  k_col_fft_full<<<grid, kThreads, lds, s>>>(g, pl, spec, CB);
-  return hipGetLastError();
+  return for_frame_chunks(g.B, [&](int b0, int nb) {
+    Geometry gc = g;
+    gc.b0 = b0;
+    k_col_fft_full<<<dim3((g.w + CB - 1) / CB, g.P, nb), kThreads, lds, s>>>(gc, pl, spec, CB);
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_ifft_rows(const Geometry& g, const FFTPlan& pl, float2* win, hipStream_t s) {
@@ -779,9 +791,12 @@ hipError_t launch_ifft_rows(const Geometry& g, const FFTPlan& pl, float2* win, h
   const size_t lds = 2 * (size_t)RB * pl.n * sizeof(float2);
   hipError_t e = set_lds(k_ifft_rows, lds);
   if (e != hipSuccess) return e;
-  dim3 grid((g.h + RB - 1) / RB, g.P, g.B);
-  k_ifft_rows<<<grid, kThreads, lds, s>>>(g, pl, win, RB);
-  return hipGetLastError();
+  return for_frame_chunks(g.B, [&](int b0, int nb) {
+    Geometry gc = g;
+    gc.b0 = b0;
+    k_ifft_rows<<<dim3((g.h + RB - 1) / RB, g.P, nb), kThreads, lds, s>>>(gc, pl, win, RB);
+    return hipGetLastError();
+  });
 }
 
 // In-place staged kernels: rows per block of the row transforms, columns per
@@ -824,11 +839,15 @@ hipError_t launch_row_fft_ip(const Geometry& g, const FFTPlan& pl, const float2*
   }
   hipError_t e = set_lds(kern, lds);
   if (e != hipSuccess) return e;
-  dim3 grid((g.H + RB - 1) / RB, g.P, g.B);
-  kern<<<grid, kRowThreads, lds, s>>>(g, pl, twW, frames, frame_is_u16, rowspec, RB,
-                                              ip_chunk(pl, kRowThreads), magic_div((unsigned)pl.n),
-                                              magic_div((unsigned)g.w));
-  return hipGetLastError();
+  return for_frame_chunks(g.B, [&](int b0, int nb) {
+    Geometry gc = g;
+    gc.b0 = b0;
+    kern<<<dim3((g.H + RB - 1) / RB, g.P, nb), kRowThreads, lds, s>>>(gc, pl, twW, frames, frame_is_u16, rowspec,
+                                                                      RB, ip_chunk(pl, kRowThreads),
+                                                                      magic_div((unsigned)pl.n),
+                                                                      magic_div((unsigned)g.w));
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_col_window_iy(const Geometry& g, const FFTPlan& pc, const FFTPlan& py, const float2* rowspec,
@@ -848,11 +867,14 @@ hipError_t launch_col_window_iy(const Geometry& g, const FFTPlan& pc, const FFTP
   }
   hipError_t e = set_lds(kern, lds);
   if (e != hipSuccess) return e;
-  dim3 grid((g.w + CB - 1) / CB, g.P, g.B);
-  kern<<<grid, kColThreads, lds, s>>>(g, pc, py, rowspec, factor, win, CB,
-                                                 ip_chunk(pc, kColThreads, kColV), ip_chunk(py, kColThreads),
-                                                 cal, peak);
-  return hipGetLastError();
+  return for_frame_chunks(g.B, [&](int b0, int nb) {
+    Geometry gc = g;
+    gc.b0 = b0;
+    kern<<<dim3((g.w + CB - 1) / CB, g.P, nb), kColThreads, lds, s>>>(gc, pc, py, rowspec, factor, win, CB,
+                                                                      ip_chunk(pc, kColThreads, kColV),
+                                                                      ip_chunk(py, kColThreads), cal, peak);
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_ifft_rows_tilt(const Geometry& g, const FFTPlan& pl, const float2* win, const float2* ref,
@@ -862,10 +884,14 @@ hipError_t launch_ifft_rows_tilt(const Geometry& g, const FFTPlan& pl, const flo
   const size_t lds = sizeof(float2) * ((size_t)pl.n + (size_t)pad_stride(pl.n) * RB);
   hipError_t e = set_lds(k_ifft_rows_tilt, lds);
   if (e != hipSuccess) return e;
-  dim3 grid((g.h + RB - 1) / RB, g.P, g.B);
-  k_ifft_rows_tilt<<<grid, kThreads, lds, s>>>(g, pl, win, ref, fields, RB, ip_chunk(pl, kThreads),
-                                               ref ? peak : nullptr);
-  return hipGetLastError();
+  return for_frame_chunks(g.B, [&](int b0, int nb) {
+    Geometry gc = g;
+    gc.b0 = b0;
+    k_ifft_rows_tilt<<<dim3((g.h + RB - 1) / RB, g.P, nb), kThreads, lds, s>>>(gc, pl, win, ref, fields, RB,
+                                                                              ip_chunk(pl, kThreads),
+                                                                              ref ? peak : nullptr);
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_ifft_cols_tilt(const Geometry& g, const FFTPlan& pl, const float2* win, const float2* ref,
@@ -874,9 +900,12 @@ hipError_t launch_ifft_cols_tilt(const Geometry& g, const FFTPlan& pl, const flo
   const size_t lds = 2 * (size_t)CB * pl.n * sizeof(float2);
   hipError_t e = set_lds(k_ifft_cols_tilt, lds);
   if (e != hipSuccess) return e;
-  dim3 grid((g.w + CB - 1) / CB, g.P, g.B);
-  k_ifft_cols_tilt<<<grid, kThreads, lds, s>>>(g, pl, win, ref, fields, CB);
-  return hipGetLastError();
+  return for_frame_chunks(g.B, [&](int b0, int nb) {
+    Geometry gc = g;
+    gc.b0 = b0;
+    k_ifft_cols_tilt<<<dim3((g.w + CB - 1) / CB, g.P, nb), kThreads, lds, s>>>(gc, pl, win, ref, fields, CB);
+    return hipGetLastError();
+  });
 }
 
 }  // namespace dh

@@ -148,14 +148,15 @@ __global__ __launch_bounds__(256) void k_mdl_image(const float2* __restrict__ co
 __global__ __launch_bounds__(256) void k_mdl_outer(float2* __restrict__ g, int N, int M, int rowStride, int s,
                                                    const double2* __restrict__ u, const double2* __restrict__ v,
                                                    double coef) {
-  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)N * M) return;
-  const int b = (int)(e / M), j = (int)(e - (size_t)b * M);
-  const double2 ub = u[b], vj = v[j];
-  const double re = coef * (ub.x * vj.x + ub.y * vj.y), im = coef * (ub.x * vj.y - ub.y * vj.x);
-  float2& o = g[(size_t)b * rowStride + (size_t)s * M + j];
-  o.x += (float)re;
-  o.y += (float)im;
+  const size_t total = (size_t)N * M;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / M), j = (int)(e - (size_t)b * M);
+    const double2 ub = u[b], vj = v[j];
+    const double re = coef * (ub.x * vj.x + ub.y * vj.y), im = coef * (ub.x * vj.y - ub.y * vj.x);
+    float2& o = g[(size_t)b * rowStride + (size_t)s * M + j];
+    o.x += (float)re;
+    o.y += (float)im;
+  }
 }
 
 hipError_t launch_metric_rows(const float2* coefs, int N, int M, int Mpol, int rowStride, int S, double* out,
@@ -182,7 +183,7 @@ hipError_t launch_mdl_seed_pair(const float2* coefs, int N, int M, int rowStride
   const double2* u = rowsAreVectors ? x : y;
   const double2* v = rowsAreVectors ? y : x;
   const size_t n = (size_t)N * M;
-  k_mdl_outer<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(g, N, M, rowStride, s, u, v, coef);
+  k_mdl_outer<<<stride_blocks(n, 256), 256, 0, st>>>(g, N, M, rowStride, s, u, v, coef);
   return hipGetLastError();
 }
 

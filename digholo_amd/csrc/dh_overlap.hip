@@ -364,7 +364,7 @@ __global__ __launch_bounds__(64 * NT2, split_min_blocks(KS1, NT2)) void k_overla
   extern __shared__ __align__(16) unsigned char os_smem[];
   __shared__ float s_red[NT2];
   constexpr int NT = 64 * NT2;
-  const int p = blockIdx.x, b = blockIdx.y;
+  const int p = blockIdx.x, b = blockIdx.y + g.b0;
   const int G = g.G, Gp = (G + 15) & ~15, w = g.w, h = g.h;
   const int tid = threadIdx.x, lane = tid & 63;
   const int mt = __builtin_amdgcn_readfirstlane(tid >> 6);   // this wave's m-tile
@@ -549,10 +549,13 @@ static hipError_t split_launch(const Geometry& g, const float2* fields, const vo
                                float2* coefs, hipStream_t s, const unsigned* peak) {
   if (8 * g.w > split_pairs_per_thread(KS1, NT2) * 64 * NT2) return hipErrorNotSupported;
   const size_t lds = (size_t)2 * 4 * 16 * split_row_halfs(g.w, KS1) * sizeof(_Float16);
-  dim3 grid(g.P, g.B);
-  k_overlap_split<KS1, NT2><<<grid, 64 * NT2, lds, s>>>(g, fields, static_cast<const _Float16*>(hgx16),
-                                                         static_cast<const _Float16*>(hgy16), coefs, peak);
-  return hipGetLastError();
+  return for_frame_chunks(g.B, [&](int b0, int nb) {
+    Geometry gc = g;
+    gc.b0 = b0;
+    k_overlap_split<KS1, NT2><<<dim3(g.P, nb), 64 * NT2, lds, s>>>(gc, fields, static_cast<const _Float16*>(hgx16),
+                                                                   static_cast<const _Float16*>(hgy16), coefs, peak);
+    return hipGetLastError();
+  });
 }
 
 template <int KS1>
@@ -570,8 +573,7 @@ static hipError_t split_dispatch_nt(const Geometry& g, const float2* fields, con
 }
 
 bool overlap_split_supported(const Geometry& g) {
-  return g.G > 0 && g.G <= 96 && g.w % 16 == 0 && g.h % 16 == 0 && g.w >= 16 && g.w <= 160 && g.h >= 16 &&
-         (size_t)g.B <= 65535;
+  return g.G > 0 && g.G <= 96 && g.w % 16 == 0 && g.h % 16 == 0 && g.w >= 16 && g.w <= 160 && g.h >= 16;
 }
 
 hipError_t launch_overlap_split(const Geometry& g, const float2* fields, const void* hgx16, const void* hgy16,

@@ -342,14 +342,54 @@ def test_full_resolution_steps_and_averaging(lib):
     lib.digHoloDestroy(h)
 
 
+@pytest.mark.parametrize("fused,precision", [(0, api.DIGHOLO_PRECISION_FP32), (0, api.DIGHOLO_PRECISION_SPLIT16),
+                                               (1, api.DIGHOLO_PRECISION_SPLIT16), (2, api.DIGHOLO_PRECISION_FP32),
+                                               (3, api.DIGHOLO_PRECISION_SPLIT16)])
+def test_batch_beyond_grid_frame_extent(lib, fused, precision):
+    """More frames than a grid's y / z extent (65535): the kernels that take
+    the frame from blockIdx.y / z are launched over the batch in chunks.  Frame
+    b of the batch is unique frame b mod 4; the frames on both sides of the
+    chunk boundary and the last one match the oracle."""
+    frames, cfg = make_case("small", 4, seed=3)
+    B = 65536 + 3
+    print(f"B={B} fused={fused} precision={precision}", flush=True)
+    big = np.ascontiguousarray(frames[np.arange(B) % 4])
+    h = lib.digHoloCreate()
+    api.configure(lib, h, **cfg)
+    assert lib.digHoloConfigSetFusedWindow(h, fused) == 0
+    assert lib.digHoloConfigSetOverlapPrecision(h, precision) == 0
+    assert lib.digHoloSetBatchUint16(h, B, big.ctypes.data_as(C.c_void_p), 0) == 0
+    b, m, p = C.c_int(), C.c_int(), C.c_int()
+    cptr = lib.digHoloProcessBatch(h, C.byref(b), C.byref(m), C.byref(p))
+    assert cptr and b.value == B
+    assert lib.digHoloGetWindowPath(h) == fused
+    nc = m.value * p.value
+    coefs = np.ctypeslib.as_array(C.cast(cptr, api.P_float), shape=(2 * B * nc,)).view(np.complex64)
+    fb, fp, w, hh = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    xp, yp = api.P_float(), api.P_float()
+    fptr = lib.digHoloGetFields(h, C.byref(fb), C.byref(fp), C.byref(xp), C.byref(yp), C.byref(w), C.byref(hh))
+    assert fptr and fb.value == B
+    nf = fp.value * hh.value * w.value
+    fields = np.ctypeslib.as_array(C.cast(fptr, api.P_float), shape=(2 * B * nf,)).view(np.complex64)
+    f_ref, c_ref, _ = O.process_batch(frames, oracle_for(cfg))
+    for fr in (0, 65534, 65535, 65536, B - 1):
+        u = fr % 4
+        ef = O.relative_l2(fields[fr * nf:(fr + 1) * nf].reshape(f_ref[u].shape), f_ref[u])
+        ec = O.relative_l2(coefs[fr * nc:(fr + 1) * nc].reshape(c_ref[u].shape), c_ref[u])
+        print(f"frame {fr}: fields {ef:.2e} coefs {ec:.2e}", flush=True)
+        assert ef <= 5 * TOL and ec <= 5 * TOL
+    lib.digHoloDestroy(h)
+    del big
+
+
 def test_batch_frame_limit(lib):
-    """A batch beyond the grid's frame extent (65535) is refused before any
+    """A batch beyond the library's frame limit (2^24) is refused before any
     frame is read, and the handle still processes a valid batch afterwards."""
     frames, cfg = make_case("small", 2, seed=3)
     h = lib.digHoloCreate()
     api.configure(lib, h, **cfg)
     buf = np.ascontiguousarray(frames)
-    assert lib.digHoloSetBatchUint16(h, 70000, buf.ctypes.data_as(C.c_void_p), 0) == 0
+    assert lib.digHoloSetBatchUint16(h, (1 << 24) + 1, buf.ctypes.data_as(C.c_void_p), 0) == 0
     b, m, p = C.c_int(), C.c_int(), C.c_int()
     assert not lib.digHoloProcessBatch(h, C.byref(b), C.byref(m), C.byref(p))
     coefs, fields, _, _, h = run_gpu(lib, frames, cfg, h=h)
