@@ -1911,6 +1911,79 @@ size_t window_v3_table_elems(const Geometry& g) {
   return (size_t)g.H + (size_t)L * (M + 1) + M;
 }
 
+// Staged row transform of 512-wide FFT windows (256 complex points of packed
+// real pairs, digHolo.cpp:1855-1872 for the r2c layout), in registers: 16
+// threads per row, each taking points t + 16 m (m = 0..15) straight from the
+// frame, a 16-point DFT over m, twiddles w_256^(t k1), a transpose through LDS
+// within the row's 16 lanes (one wave holds four rows: no block barrier), a
+// 16-point DFT over t giving Z[k1 + 16 k2], then the r2c unpack of the w
+// window columns c0 .. c0 + w - 1 from Z[k] and Z[256 - k].  grid
+// (ceil(H / 16), P, frames), 256 threads.
+constexpr int kR256Rows = 16;
+__global__ __launch_bounds__(256) void k_row_fft256(Geometry g, const float2* __restrict__ tw,
+                                                    const float2* __restrict__ twW, const void* __restrict__ frames,
+                                                    int u16, float2* __restrict__ rowspec) {
+  constexpr int N = 256, RP = 16 * 17;   // LDS floats2 per row: 16 x 16 with a pad of one per 16
+  __shared__ float2 stw[N];
+  __shared__ float2 ex[kR256Rows * RP];
+  const int b = blockIdx.z + g.b0, p = blockIdx.y;
+  const int r = threadIdx.x >> 4, t = threadIdx.x & 15;
+  const int y = blockIdx.x * kR256Rows + r;
+  stw[threadIdx.x] = tw[threadIdx.x];
+  const bool live = y < g.H;
+  const size_t rowOff = (size_t)b * g.frameH * g.frameW + (size_t)(g.cy[p] + (live ? y : 0)) * g.frameW + g.cx[p];
+  float2 z[16];
+  if (u16) {
+    const unsigned short* f = static_cast<const unsigned short*>(frames) + rowOff;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const ushort2 v = *reinterpret_cast<const ushort2*>(f + 2 * (t + 16 * m));
+      z[m] = make_float2((float)v.x, (float)v.y);
+    }
+  } else {
+    const float* f = static_cast<const float*>(frames) + rowOff;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) z[m] = *reinterpret_cast<const float2*>(f + 2 * (t + 16 * m));
+  }
+  dft_any<16, false>(z);
+  __syncthreads();   // twiddles staged
+  float2* xr = ex + r * RP;
+#pragma unroll
+  for (int k1 = 0; k1 < 16; ++k1) xr[17 * k1 + t] = k1 ? cmul(z[k1], stw[t * k1]) : z[0];
+  wave_lds_sync();
+#pragma unroll
+  for (int tt = 0; tt < 16; ++tt) z[tt] = xr[17 * t + tt];
+  dft_any<16, false>(z);   // z[k2] = Z[t + 16 k2]
+  wave_lds_sync();
+#pragma unroll
+  for (int k2 = 0; k2 < 16; ++k2) xr[17 * k2 + t] = z[k2];   // Z[k] at 17 (k >> 4) + (k & 15)
+  wave_lds_sync();
+  if (!live) return;
+  auto Zat = [&](int k) { return xr[17 * (k >> 4) + (k & 15)]; };
+  const int c0 = g.c0[p], w = g.w;
+  float2* out = rowspec + ((size_t)(b * g.P + p) * g.H + y) * w;
+  for (int kk = t; kk < w; kk += 16) {
+    const int k = c0 + kk;   // k <= N (the Nyquist column)
+    const float2 zk = Zat(k == N ? 0 : k);
+    float2 zc = Zat((k == 0 || k == N) ? 0 : N - k);
+    zc.y = -zc.y;
+    const float2 ev = cscale(cadd(zk, zc), 0.5f);
+    const float2 od = mul_mi<false>(cscale(csub(zk, zc), 0.5f));   // (zk - zc)/(2i)
+    out[kk] = cadd(ev, cmul(twW[c0 + kk], od));
+  }
+}
+
+hipError_t launch_row_fft256(const Geometry& g, const float2* tw, const float2* twW, const void* frames,
+                             int frame_is_u16, float2* rowspec, hipStream_t s) {
+  return for_frame_chunks(g.B, [&](int b0, int nb) {
+    Geometry gc = g;
+    gc.b0 = b0;
+    k_row_fft256<<<dim3((g.H + kR256Rows - 1) / kR256Rows, g.P, nb), 256, 0, s>>>(gc, tw, twW, frames, frame_is_u16,
+                                                                                   rowspec);
+    return hipGetLastError();
+  });
+}
+
 hipError_t launch_window_v3_tables(const Geometry& g, float2* tab, hipStream_t s) {
   int L, M;
   if (!w3_lm(g.W / 2, L, M)) return hipErrorNotSupported;

@@ -137,6 +137,10 @@ hipError_t launch_ifft_rows(const Geometry& g, const FFTPlan& pl, float2* win, h
 // + reference wave.  staged_ip_supported says whether the three apply.
 bool staged_ip_supported(const Geometry& g, const FFTPlan& row, const FFTPlan& col, const FFTPlan& ix,
                          const FFTPlan& iy);
+// 256-point rows (512-wide FFT windows) in registers; the staged row kernel's
+// output.  tw: the 256 transform twiddles.
+hipError_t launch_row_fft256(const Geometry& g, const float2* tw, const float2* twW, const void* frames,
+                             int frame_is_u16, float2* rowspec, hipStream_t s);
 hipError_t launch_row_fft_ip(const Geometry& g, const FFTPlan& pl, const float2* twW, const void* frames,
                              int frame_is_u16, float2* rowspec, hipStream_t s);
 // peak (optional, [B][P] float bits): the column kernel clears it, the row

@@ -825,6 +825,7 @@ bool staged_ip_supported(const Geometry& g, const FFTPlan& row, const FFTPlan& c
 
 hipError_t launch_row_fft_ip(const Geometry& g, const FFTPlan& pl, const float2* twW, const void* frames,
                              int frame_is_u16, float2* rowspec, hipStream_t s) {
+  if (pl.n == 256) return launch_row_fft256(g, pl.tw, twW, frames, frame_is_u16, rowspec, s);
   const int RB = ip_rows_per_block(pl, g.H, kRowThreads);
   if (RB < 1) return hipErrorNotSupported;
   const size_t lds = sizeof(float2) * ((size_t)pl.n + g.w + (size_t)pad_stride(pl.n) * RB);
