@@ -608,6 +608,7 @@ __device__ __forceinline__ void dft_any(float2* v) {
   else if constexpr (R == 24) fft_pq<3, 8, INV>(v);
   else if constexpr (R == 12) fft_pq<3, 4, INV>(v);
   else if constexpr (R == 20) fft_pq<4, 5, INV>(v);
+  else if constexpr (R == 32) fft_pq<4, 8, INV>(v);
   else if constexpr (R == 1) (void)v;
   else dftR<R, INV>(v);
 }
@@ -1980,6 +1981,121 @@ hipError_t launch_row_fft256(const Geometry& g, const float2* tw, const float2* 
     gc.b0 = b0;
     k_row_fft256<<<dim3((g.H + kR256Rows - 1) / kR256Rows, g.P, nb), 256, 0, s>>>(gc, tw, twW, frames, frame_is_u16,
                                                                                    rowspec);
+    return hipGetLastError();
+  });
+}
+
+// Staged column stage of 512-row FFT windows, the column transform in
+// registers: 16 lanes per column (lane c + 16 t), each taking band rows
+// t + 16 m (m = 0..31), a 32-point DFT over m, the twiddles w_512^(t k1), a
+// transpose through LDS in two halves (k1 < 16, then the rest), and two
+// 16-point DFTs over t giving Z[k1 + 32 k2] for k1 = t and t + 16.  Only the h
+// window rows go to the plane (window row j is spectrum row (j + SY + H/2) mod
+// H), which takes the exchange's LDS; then as k_col_window_iy: times the
+// window factor (loaded while the transform runs), the inverse transform along
+// y, the window plane out.  grid (ceil(w / 16), P, frames), 256 threads,
+// h <= 256; about 40 KB of LDS.
+constexpr int kC512Cols = 16, kC512Sel = 16;
+constexpr int kC512XS = 16 * 17 + 1;          // exchange floats2 per column: [16 k1][17]
+__global__ __launch_bounds__(256) void k_col512_window_iy(Geometry g, FFTPlan py, const float2* __restrict__ twc_g,
+                                                          const float2* __restrict__ rowspec,
+                                                          const float2* __restrict__ factor,
+                                                          float2* __restrict__ win, int chunkY, BatchCal cal,
+                                                          unsigned* __restrict__ peak) {
+  constexpr int H = 512;
+  extern __shared__ float2 smem[];
+  const int h = g.h, w = g.w, LD = pad_stride(h);
+  float2* twc = smem;       // [H] column-transform twiddles
+  float2* twy = smem + H;   // [h] inverse twiddles
+  float2* A = twy + h;      // the exchange, then window row j of column c at c LD + pad32(j)
+  const int b = blockIdx.z + g.b0, p = blockIdx.y, i0 = blockIdx.x * kC512Cols;
+  const int cols = min(kC512Cols, w - i0);
+  const int c = threadIdx.x & 15, t = threadIdx.x >> 4;
+  if (peak && blockIdx.x == 0 && threadIdx.x == 0) peak[out_pos(g, b) * g.P + p] = 0u;
+  for (int e = threadIdx.x; e < H; e += 256) twc[e] = twc_g[e];
+  for (int e = threadIdx.x; e < h; e += 256) twy[e] = py.tw[e];
+  const float2* in = rowspec + (size_t)(b * g.P + p) * H * w + i0;
+  float2 z[32];
+  if (c < cols) {
+#pragma unroll
+    for (int m = 0; m < 32; ++m) z[m] = in[(size_t)(t + 16 * m) * w + c];
+  } else {
+#pragma unroll
+    for (int m = 0; m < 32; ++m) z[m] = make_float2(0.f, 0.f);
+  }
+  // window row j is spectrum row (j + SY + H/2) mod H
+  int sy = (g.SY[p] + H / 2) % H;
+  if (sy < 0) sy += H;
+  const float2* fac = factor + (size_t)(lam_in(g, b) * g.P + p) * h * w + i0;
+  const unsigned magC = magic_div((unsigned)cols);
+  float2 sel[kC512Sel];
+#pragma unroll
+  for (int s = 0; s < kC512Sel; ++s) {
+    const int e = threadIdx.x + s * 256;
+    if (e < h * cols) {
+      const int j = fdiv(e, magC), cc = e - j * cols;
+      sel[s] = fac[(size_t)j * w + cc];
+    }
+  }
+  dft_any<32, false>(z);
+  __syncthreads();   // twiddles staged
+  float2* xc = A + c * kC512XS;
+  float2 u[2][16];
+#pragma unroll
+  for (int hh = 0; hh < 2; ++hh) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int k1 = 16 * hh + k;
+      xc[17 * k + t] = k1 ? cmul(z[k1], twc[t * k1]) : z[0];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int tt = 0; tt < 16; ++tt) u[hh][tt] = xc[17 * t + tt];
+    __syncthreads();   // read before the next half (or the plane) is written
+    dft_any<16, false>(u[hh]);   // u[hh][k2] = Z[t + 16 hh + 32 k2]
+  }
+#pragma unroll
+  for (int hh = 0; hh < 2; ++hh)
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) {
+      int j = t + 16 * hh + 32 * k2 - sy;
+      if (j < 0) j += H;
+      if (j < h) A[c * LD + pad32(j)] = u[hh][k2];
+    }
+  __syncthreads();
+  const float2 cb = cal.at(p, b);
+#pragma unroll
+  for (int s = 0; s < kC512Sel; ++s) {   // each element read and written by one thread
+    const int e = threadIdx.x + s * 256;
+    if (e < h * cols) {
+      const int j = fdiv(e, magC), cc = e - j * cols;
+      float2* a = A + cc * LD + pad32(j);
+      *a = cmul(cmul(*a, sel[s]), cb);
+    }
+  }
+  __syncthreads();
+  lds_fft_any<true, 256>(A, cols, LD, py, twy, chunkY);
+  float2* out = win + (size_t)(out_pos(g, b) * g.P + p) * h * w + i0;
+  for (int e = threadIdx.x; e < h * cols; e += 256) {
+    const int j = fdiv(e, magC), cc = e - j * cols;
+    out[(size_t)j * w + cc] = A[cc * LD + pad32(j)];
+  }
+}
+
+hipError_t launch_col512_window_iy(const Geometry& g, const FFTPlan& pc, const FFTPlan& py, const float2* rowspec,
+                                   const float2* factor, float2* win, const BatchCal& cal, hipStream_t s,
+                                   unsigned* peak) {
+  const int chunkY = ip_chunk(py, 256);
+  if (g.H != 512 || pc.n != 512 || py.n != g.h || g.h > 256 || chunkY < 1) return hipErrorNotSupported;
+  const size_t lds = sizeof(float2) * (512 + (size_t)g.h + (size_t)kC512Cols * std::max(kC512XS, pad_stride(g.h)));
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_col512_window_iy),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  return for_frame_chunks(g.B, [&](int b0, int nb) {
+    Geometry gc = g;
+    gc.b0 = b0;
+    k_col512_window_iy<<<dim3((g.w + kC512Cols - 1) / kC512Cols, g.P, nb), 256, lds, s>>>(
+        gc, py, pc.tw, rowspec, factor, win, chunkY, cal, peak);
     return hipGetLastError();
   });
 }

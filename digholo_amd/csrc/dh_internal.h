@@ -141,6 +141,11 @@ bool staged_ip_supported(const Geometry& g, const FFTPlan& row, const FFTPlan& c
 // output.  tw: the 256 transform twiddles.
 hipError_t launch_row_fft256(const Geometry& g, const float2* tw, const float2* twW, const void* frames,
                              int frame_is_u16, float2* rowspec, hipStream_t s);
+// Column stage of 512-row FFT windows with the column transform in
+// registers (h <= 256); hipErrorNotSupported for other shapes.
+hipError_t launch_col512_window_iy(const Geometry& g, const FFTPlan& pc, const FFTPlan& py, const float2* rowspec,
+                                   const float2* factor, float2* win, const BatchCal& cal, hipStream_t s,
+                                   unsigned* peak);
 hipError_t launch_row_fft_ip(const Geometry& g, const FFTPlan& pl, const float2* twW, const void* frames,
                              int frame_is_u16, float2* rowspec, hipStream_t s);
 // peak (optional, [B][P] float bits): the column kernel clears it, the row

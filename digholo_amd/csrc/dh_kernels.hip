@@ -854,6 +854,8 @@ hipError_t launch_row_fft_ip(const Geometry& g, const FFTPlan& pl, const float2*
 hipError_t launch_col_window_iy(const Geometry& g, const FFTPlan& pc, const FFTPlan& py, const float2* rowspec,
                                 const float2* factor, float2* win, const BatchCal& cal, hipStream_t s,
                                 unsigned* peak) {
+  if (pc.n == 512 && g.H == 512 && py.n == g.h && g.h <= 256 && ip_chunk(py, 256) >= 1)
+    return launch_col512_window_iy(g, pc, py, rowspec, factor, win, cal, s, peak);
   const int CB = ip_cols_per_block(g, pc, py);
   if (CB < 1 || py.n != g.h) return hipErrorNotSupported;
   const size_t lds = sizeof(float2) * ((size_t)g.H + g.h + (size_t)pad_stride(g.H) * CB);
