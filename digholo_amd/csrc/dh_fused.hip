@@ -609,6 +609,7 @@ __device__ __forceinline__ void dft_any(float2* v) {
   else if constexpr (R == 12) fft_pq<3, 4, INV>(v);
   else if constexpr (R == 20) fft_pq<4, 5, INV>(v);
   else if constexpr (R == 32) fft_pq<4, 8, INV>(v);
+  else if constexpr (R == 10) fft_pq<2, 5, INV>(v);
   else if constexpr (R == 1) (void)v;
   else dftR<R, INV>(v);
 }
@@ -2096,6 +2097,86 @@ hipError_t launch_col512_window_iy(const Geometry& g, const FFTPlan& pc, const F
     gc.b0 = b0;
     k_col512_window_iy<<<dim3((g.w + kC512Cols - 1) / kC512Cols, g.P, nb), 256, lds, s>>>(
         gc, py, pc.tw, rowspec, factor, win, chunkY, cal, peak);
+    return hipGetLastError();
+  });
+}
+
+// Staged inverse transform along x of w = 16 M point rows in registers, then
+// the reference wave: 16 lanes per row (four rows per wave), lane t taking
+// points t + 16 m (m < M), an M-point inverse DFT over m, the twiddles
+// conj(w_w^(t k1)), a transpose through LDS inside the row's lanes, and on
+// lanes t < M a 16-point inverse DFT giving X[t + M k2]; then V = -conj(X ref)
+// (X itself when ref is null) out, and the field's largest |Re|, |Im| into
+// peak (one atomicMax of the float bits per wave).  grid (ceil(h / 16), P,
+// frames), 256 threads.
+template <int M>
+__global__ __launch_bounds__(256) void k_ifft_rows_reg(Geometry g, const float2* __restrict__ tw,
+                                                       const float2* __restrict__ win, const float2* __restrict__ ref,
+                                                       float2* __restrict__ fields, unsigned* __restrict__ peak) {
+  constexpr int N = 16 * M, RP = 17 * M;   // a row's LDS: [M][17] exchange, then X[k] at k + k / 16
+  __shared__ float2 stw[N];
+  __shared__ float2 ex[16 * RP];
+  const int b = blockIdx.z + g.b0, p = blockIdx.y;
+  const int r = threadIdx.x >> 4, t = threadIdx.x & 15;
+  const int y = blockIdx.x * 16 + r;
+  for (int e = threadIdx.x; e < N; e += 256) stw[e] = tw[e];
+  const bool live = y < g.h;
+  const size_t base = ((size_t)(b * g.P + p) * g.h + (live ? y : 0)) * N;
+  float2 z[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) z[m] = win[base + t + 16 * m];
+  dft_any<M, true>(z);
+  __syncthreads();   // twiddles staged
+  float2* xr = ex + r * RP;
+#pragma unroll
+  for (int k1 = 0; k1 < M; ++k1) xr[17 * k1 + t] = k1 ? cmul(z[k1], twiddle<true>(stw, t * k1)) : z[0];
+  wave_lds_sync();
+  float2 u[16];
+  if (t < M) {
+#pragma unroll
+    for (int tt = 0; tt < 16; ++tt) u[tt] = xr[17 * t + tt];
+  }
+  wave_lds_sync();
+  if (t < M) {
+    dft_any<16, true>(u);
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) {
+      const int k = t + M * k2;
+      xr[k + (k >> 4)] = u[k2];
+    }
+  }
+  wave_lds_sync();
+  float mx = 0.f;
+  if (live) {
+    const float2* rp = ref ? ref + ((size_t)(lam_out(g, b) * g.P + p) * g.h + y) * N : nullptr;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      const int k = t + 16 * j;
+      float2 v = xr[k + (k >> 4)];
+      if (rp) {
+        const float2 tt = cmul(v, rp[k]);
+        v = make_float2(-tt.x, tt.y);
+        mx = fmaxf(mx, fmaxf(fabsf(tt.x), fabsf(tt.y)));
+      }
+      fields[base + k] = v;
+    }
+  }
+  if (peak && ref) {   // non-negative floats order as their bit patterns
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(peak + b * g.P + p, __float_as_uint(mx));
+  }
+}
+
+hipError_t launch_ifft_rows_reg(const Geometry& g, const FFTPlan& pl, const float2* win, const float2* ref,
+                                float2* fields, hipStream_t s, unsigned* peak) {
+  if (pl.n != g.w || (g.w != 80 && g.w != 160)) return hipErrorNotSupported;
+  return for_frame_chunks(g.B, [&](int b0, int nb) {
+    Geometry gc = g;
+    gc.b0 = b0;
+    const dim3 grid((g.h + 15) / 16, g.P, nb);
+    if (g.w == 160) k_ifft_rows_reg<10><<<grid, 256, 0, s>>>(gc, pl.tw, win, ref, fields, peak);
+    else k_ifft_rows_reg<5><<<grid, 256, 0, s>>>(gc, pl.tw, win, ref, fields, peak);
     return hipGetLastError();
   });
 }

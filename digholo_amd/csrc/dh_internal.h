@@ -146,6 +146,10 @@ hipError_t launch_row_fft256(const Geometry& g, const float2* tw, const float2* 
 hipError_t launch_col512_window_iy(const Geometry& g, const FFTPlan& pc, const FFTPlan& py, const float2* rowspec,
                                    const float2* factor, float2* win, const BatchCal& cal, hipStream_t s,
                                    unsigned* peak);
+// Inverse transform along x in registers for w = 80 or 160, then the
+// reference wave; hipErrorNotSupported for other widths.
+hipError_t launch_ifft_rows_reg(const Geometry& g, const FFTPlan& pl, const float2* win, const float2* ref,
+                                float2* fields, hipStream_t s, unsigned* peak);
 hipError_t launch_row_fft_ip(const Geometry& g, const FFTPlan& pl, const float2* twW, const void* frames,
                              int frame_is_u16, float2* rowspec, hipStream_t s);
 // peak (optional, [B][P] float bits): the column kernel clears it, the row
