@@ -264,12 +264,16 @@ def test_large_sensor_float32_frames(lib, precision):
 @pytest.mark.parametrize("name,B,quantise,over", [
     ("radix7", 3, True, {}), ("small", 3, True, {}), ("small", 3, False, {}),
     ("dual640", 2, False, {"fft_window": (256, 192), "beam_centre_y": (30e-6, -50e-6)}),
-    ("dual640w192", 2, True, {"fft_window": (192, 256)})])
+    ("dual640w192", 2, True, {"fft_window": (192, 256)}),
+    ("large1024", 1, True, {"fft_window": (512, 384)}), ("large1024", 1, False, {"fft_window": (384, 512)})])
 def test_staged_kernels_any_radix(lib, name, B, quantise, over):
     """The staged kernels: a 56-point row transform (radix 7) has no in-place
     butterfly and takes the ping-pong kernels, the others the in-place kernels
     (uint16 and float32 frames, rectangular FFT windows, ragged column strips);
-    all against the oracle."""
+    all against the oracle.  The 1024 x 1024 cases put the register row
+    transform (512-wide window, 128 x 160 fields) beside the LDS column
+    transform, and the LDS row transform beside the register column transform
+    (512-row window, 160 x 128 fields)."""
     frames, cfg = make_case(name, B, seed=4, quantise=quantise, **over)
     print(f"case {name} B={B} quantise={quantise} {over}", flush=True)
     h = lib.digHoloCreate()
