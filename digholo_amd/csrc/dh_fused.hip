@@ -2107,7 +2107,7 @@ hipError_t launch_col512_window_iy(const Geometry& g, const FFTPlan& pc, const F
 // conj(w_w^(t k1)), a transpose through LDS inside the row's lanes, and on
 // lanes t < M a 16-point inverse DFT giving X[t + M k2]; then V = -conj(X ref)
 // (X itself when ref is null) out, and the field's largest |Re|, |Im| into
-// peak (one atomicMax of the float bits per wave).  grid (ceil(h / 16), P,
+// peak (one atomicMax of the float bits per block: one per wave contended).  grid (ceil(h / 16), P,
 // frames), 256 threads.
 template <int M>
 __global__ __launch_bounds__(256) void k_ifft_rows_reg(Geometry g, const float2* __restrict__ tw,
@@ -2161,10 +2161,14 @@ __global__ __launch_bounds__(256) void k_ifft_rows_reg(Geometry g, const float2*
       fields[base + k] = v;
     }
   }
-  if (peak && ref) {   // non-negative floats order as their bit patterns
+  if (peak && ref) {   // one atomicMax per block; non-negative floats order as their bit patterns
+    __shared__ float s_mx[4];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-    if ((threadIdx.x & 63) == 0) atomicMax(peak + b * g.P + p, __float_as_uint(mx));
+    if ((threadIdx.x & 63) == 0) s_mx[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      atomicMax(peak + b * g.P + p, __float_as_uint(fmaxf(fmaxf(s_mx[0], s_mx[1]), fmaxf(s_mx[2], s_mx[3]))));
   }
 }
 

@@ -455,9 +455,15 @@ __global__ __launch_bounds__(kThreads) void k_ifft_rows_tilt(Geometry g, FFTPlan
     fields[base + e] = make_float2(-t.x, t.y);
     mx = fmaxf(mx, fmaxf(fabsf(t.x), fabsf(t.y)));
   });
-  if (peak) {   // non-negative floats order as their bit patterns
+  if (peak) {   // one atomicMax per block; non-negative floats order as their bit patterns
+    __shared__ float s_mx[kThreads / 64];
     for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-    if ((threadIdx.x & 63) == 0) atomicMax(peak + b * g.P + p, __float_as_uint(mx));
+    if ((threadIdx.x & 63) == 0) s_mx[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int i = 1; i < kThreads / 64; ++i) mx = fmaxf(mx, s_mx[i]);
+      atomicMax(peak + b * g.P + p, __float_as_uint(mx));
+    }
   }
 }
 
