@@ -1913,25 +1913,26 @@ size_t window_v3_table_elems(const Geometry& g) {
   return (size_t)g.H + (size_t)L * (M + 1) + M;
 }
 
-// Staged row transform of 512-wide FFT windows (256 complex points of packed
-// real pairs, digHolo.cpp:1855-1872 for the r2c layout), in registers: 16
-// threads per row, each taking points t + 16 m (m = 0..15) straight from the
-// frame, a 16-point DFT over m, twiddles w_256^(t k1), a transpose through LDS
-// within the row's 16 lanes (one wave holds four rows: no block barrier), a
-// 16-point DFT over t giving Z[k1 + 16 k2], then the r2c unpack of the w
-// window columns c0 .. c0 + w - 1 from Z[k] and Z[256 - k].  grid
-// (ceil(H / 16), P, frames), 256 threads.
-constexpr int kR256Rows = 16;
-__global__ __launch_bounds__(256) void k_row_fft256(Geometry g, const float2* __restrict__ tw,
-                                                    const float2* __restrict__ twW, const void* __restrict__ frames,
-                                                    int u16, float2* __restrict__ rowspec) {
-  constexpr int N = 256, RP = 16 * 17;   // LDS floats2 per row: 16 x 16 with a pad of one per 16
+// Staged row transform of 256- and 512-wide FFT windows (N = 16 T complex
+// points of packed real pairs, digHolo.cpp:1855-1872 for the r2c layout), in
+// registers: T lanes per row, each taking points t + T m (m = 0..15) straight
+// from the frame, a 16-point DFT over m, twiddles w_N^(t k1), a transpose
+// through LDS within the row's lanes (a wave holds 64 / T rows: no block
+// barrier), T-point DFTs over t for k1 = t, t + T, .. giving Z[k1 + 16 k2],
+// then the r2c unpack of the w window columns c0 .. c0 + w - 1 from Z[k] and
+// Z[N - k].  grid (ceil(H / (256 / T)), P, frames), 256 threads.
+template <int T>
+__global__ __launch_bounds__(256) void k_row_fft_reg(Geometry g, const float2* __restrict__ tw,
+                                                     const float2* __restrict__ twW, const void* __restrict__ frames,
+                                                     int u16, float2* __restrict__ rowspec) {
+  constexpr int N = 16 * T, ROWS = 256 / T;
+  constexpr int RP = 16 * (T + 1) + 1;   // a row's LDS: [16 k1][T + 1], then Z[k] at 17 (k / 16) + k % 16
   __shared__ float2 stw[N];
-  __shared__ float2 ex[kR256Rows * RP];
+  __shared__ float2 ex[ROWS * RP];
   const int b = blockIdx.z + g.b0, p = blockIdx.y;
-  const int r = threadIdx.x >> 4, t = threadIdx.x & 15;
-  const int y = blockIdx.x * kR256Rows + r;
-  stw[threadIdx.x] = tw[threadIdx.x];
+  const int r = threadIdx.x / T, t = threadIdx.x % T;
+  const int y = blockIdx.x * ROWS + r;
+  for (int e = threadIdx.x; e < N; e += 256) stw[e] = tw[e];
   const bool live = y < g.H;
   const size_t rowOff = (size_t)b * g.frameH * g.frameW + (size_t)(g.cy[p] + (live ? y : 0)) * g.frameW + g.cx[p];
   float2 z[16];
@@ -1939,32 +1940,40 @@ __global__ __launch_bounds__(256) void k_row_fft256(Geometry g, const float2* __
     const unsigned short* f = static_cast<const unsigned short*>(frames) + rowOff;
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
-      const ushort2 v = *reinterpret_cast<const ushort2*>(f + 2 * (t + 16 * m));
+      const ushort2 v = *reinterpret_cast<const ushort2*>(f + 2 * (t + T * m));
       z[m] = make_float2((float)v.x, (float)v.y);
     }
   } else {
     const float* f = static_cast<const float*>(frames) + rowOff;
 #pragma unroll
-    for (int m = 0; m < 16; ++m) z[m] = *reinterpret_cast<const float2*>(f + 2 * (t + 16 * m));
+    for (int m = 0; m < 16; ++m) z[m] = *reinterpret_cast<const float2*>(f + 2 * (t + T * m));
   }
   dft_any<16, false>(z);
   __syncthreads();   // twiddles staged
   float2* xr = ex + r * RP;
 #pragma unroll
-  for (int k1 = 0; k1 < 16; ++k1) xr[17 * k1 + t] = k1 ? cmul(z[k1], stw[t * k1]) : z[0];
+  for (int k1 = 0; k1 < 16; ++k1) xr[(T + 1) * k1 + t] = k1 ? cmul(z[k1], stw[t * k1]) : z[0];
+  wave_lds_sync();
+  constexpr int NK = 16 / T;   // k1 values per lane
+  float2 u[NK][T];
+#pragma unroll
+  for (int i = 0; i < NK; ++i) {
+#pragma unroll
+    for (int tt = 0; tt < T; ++tt) u[i][tt] = xr[(T + 1) * (t + T * i) + tt];
+  }
   wave_lds_sync();
 #pragma unroll
-  for (int tt = 0; tt < 16; ++tt) z[tt] = xr[17 * t + tt];
-  dft_any<16, false>(z);   // z[k2] = Z[t + 16 k2]
-  wave_lds_sync();
+  for (int i = 0; i < NK; ++i) {
+    dft_any<T, false>(u[i]);   // u[i][k2] = Z[t + T i + 16 k2]
 #pragma unroll
-  for (int k2 = 0; k2 < 16; ++k2) xr[17 * k2 + t] = z[k2];   // Z[k] at 17 (k >> 4) + (k & 15)
+    for (int k2 = 0; k2 < T; ++k2) xr[17 * k2 + t + T * i] = u[i][k2];
+  }
   wave_lds_sync();
   if (!live) return;
   auto Zat = [&](int k) { return xr[17 * (k >> 4) + (k & 15)]; };
   const int c0 = g.c0[p], w = g.w;
   float2* out = rowspec + ((size_t)(b * g.P + p) * g.H + y) * w;
-  for (int kk = t; kk < w; kk += 16) {
+  for (int kk = t; kk < w; kk += T) {
     const int k = c0 + kk;   // k <= N (the Nyquist column)
     const float2 zk = Zat(k == N ? 0 : k);
     float2 zc = Zat((k == 0 || k == N) ? 0 : N - k);
@@ -1975,22 +1984,25 @@ __global__ __launch_bounds__(256) void k_row_fft256(Geometry g, const float2* __
   }
 }
 
-hipError_t launch_row_fft256(const Geometry& g, const float2* tw, const float2* twW, const void* frames,
-                             int frame_is_u16, float2* rowspec, hipStream_t s) {
+hipError_t launch_row_fft_reg(const Geometry& g, const FFTPlan& pl, const float2* twW, const void* frames,
+                              int frame_is_u16, float2* rowspec, hipStream_t s) {
+  if (pl.n != 128 && pl.n != 256) return hipErrorNotSupported;
   return for_frame_chunks(g.B, [&](int b0, int nb) {
     Geometry gc = g;
     gc.b0 = b0;
-    k_row_fft256<<<dim3((g.H + kR256Rows - 1) / kR256Rows, g.P, nb), 256, 0, s>>>(gc, tw, twW, frames, frame_is_u16,
-                                                                                   rowspec);
+    if (pl.n == 256)
+      k_row_fft_reg<16><<<dim3((g.H + 15) / 16, g.P, nb), 256, 0, s>>>(gc, pl.tw, twW, frames, frame_is_u16, rowspec);
+    else
+      k_row_fft_reg<8><<<dim3((g.H + 31) / 32, g.P, nb), 256, 0, s>>>(gc, pl.tw, twW, frames, frame_is_u16, rowspec);
     return hipGetLastError();
   });
 }
 
-// Staged column stage of 512-row FFT windows, the column transform in
-// registers: 16 lanes per column (lane c + 16 t), each taking band rows
-// t + 16 m (m = 0..31), a 32-point DFT over m, the twiddles w_512^(t k1), a
-// transpose through LDS in two halves (k1 < 16, then the rest), and two
-// 16-point DFTs over t giving Z[k1 + 32 k2] for k1 = t and t + 16.  Only the h
+// Staged column stage of 256- and 512-row FFT windows, the column transform
+// in registers: 16 lanes per column (lane c + 16 t), each taking band rows
+// t + 16 m (m < H / 16), an H/16-point DFT over m, the twiddles w_H^(t k1), a
+// transpose through LDS in halves of 16 k1, and 16-point DFTs over t giving
+// Z[k1 + (H / 16) k2] for k1 = t, t + 16, ...  Only the h
 // window rows go to the plane (window row j is spectrum row (j + SY + H/2) mod
 // H), which takes the exchange's LDS; then as k_col_window_iy: times the
 // window factor (loaded while the transform runs), the inverse transform along
@@ -1998,12 +2010,13 @@ hipError_t launch_row_fft256(const Geometry& g, const float2* tw, const float2* 
 // h <= 256; about 40 KB of LDS.
 constexpr int kC512Cols = 16, kC512Sel = 16;
 constexpr int kC512XS = 16 * 17 + 1;          // exchange floats2 per column: [16 k1][17]
-__global__ __launch_bounds__(256) void k_col512_window_iy(Geometry g, FFTPlan py, const float2* __restrict__ twc_g,
+template <int H>
+__global__ __launch_bounds__(256) void k_col_reg_window_iy(Geometry g, FFTPlan py, const float2* __restrict__ twc_g,
                                                           const float2* __restrict__ rowspec,
                                                           const float2* __restrict__ factor,
                                                           float2* __restrict__ win, int chunkY, BatchCal cal,
                                                           unsigned* __restrict__ peak) {
-  constexpr int H = 512;
+  constexpr int NM = H / 16, NH = H / 256;   // points per lane, halves of the exchange
   extern __shared__ float2 smem[];
   const int h = g.h, w = g.w, LD = pad_stride(h);
   float2* twc = smem;       // [H] column-transform twiddles
@@ -2016,13 +2029,13 @@ __global__ __launch_bounds__(256) void k_col512_window_iy(Geometry g, FFTPlan py
   for (int e = threadIdx.x; e < H; e += 256) twc[e] = twc_g[e];
   for (int e = threadIdx.x; e < h; e += 256) twy[e] = py.tw[e];
   const float2* in = rowspec + (size_t)(b * g.P + p) * H * w + i0;
-  float2 z[32];
+  float2 z[NM];
   if (c < cols) {
 #pragma unroll
-    for (int m = 0; m < 32; ++m) z[m] = in[(size_t)(t + 16 * m) * w + c];
+    for (int m = 0; m < NM; ++m) z[m] = in[(size_t)(t + 16 * m) * w + c];
   } else {
 #pragma unroll
-    for (int m = 0; m < 32; ++m) z[m] = make_float2(0.f, 0.f);
+    for (int m = 0; m < NM; ++m) z[m] = make_float2(0.f, 0.f);
   }
   // window row j is spectrum row (j + SY + H/2) mod H
   int sy = (g.SY[p] + H / 2) % H;
@@ -2038,12 +2051,12 @@ __global__ __launch_bounds__(256) void k_col512_window_iy(Geometry g, FFTPlan py
       sel[s] = fac[(size_t)j * w + cc];
     }
   }
-  dft_any<32, false>(z);
+  dft_any<NM, false>(z);
   __syncthreads();   // twiddles staged
   float2* xc = A + c * kC512XS;
-  float2 u[2][16];
+  float2 u[NH][16];
 #pragma unroll
-  for (int hh = 0; hh < 2; ++hh) {
+  for (int hh = 0; hh < NH; ++hh) {
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       const int k1 = 16 * hh + k;
@@ -2053,13 +2066,13 @@ __global__ __launch_bounds__(256) void k_col512_window_iy(Geometry g, FFTPlan py
 #pragma unroll
     for (int tt = 0; tt < 16; ++tt) u[hh][tt] = xc[17 * t + tt];
     __syncthreads();   // read before the next half (or the plane) is written
-    dft_any<16, false>(u[hh]);   // u[hh][k2] = Z[t + 16 hh + 32 k2]
+    dft_any<16, false>(u[hh]);   // u[hh][k2] = Z[t + 16 hh + NM k2]
   }
 #pragma unroll
-  for (int hh = 0; hh < 2; ++hh)
+  for (int hh = 0; hh < NH; ++hh)
 #pragma unroll
     for (int k2 = 0; k2 < 16; ++k2) {
-      int j = t + 16 * hh + 32 * k2 - sy;
+      int j = t + 16 * hh + NM * k2 - sy;
       if (j < 0) j += H;
       if (j < h) A[c * LD + pad32(j)] = u[hh][k2];
     }
@@ -2083,20 +2096,25 @@ __global__ __launch_bounds__(256) void k_col512_window_iy(Geometry g, FFTPlan py
   }
 }
 
-hipError_t launch_col512_window_iy(const Geometry& g, const FFTPlan& pc, const FFTPlan& py, const float2* rowspec,
-                                   const float2* factor, float2* win, const BatchCal& cal, hipStream_t s,
-                                   unsigned* peak) {
+hipError_t launch_col_reg_window_iy(const Geometry& g, const FFTPlan& pc, const FFTPlan& py, const float2* rowspec,
+                                    const float2* factor, float2* win, const BatchCal& cal, hipStream_t s,
+                                    unsigned* peak) {
   const int chunkY = ip_chunk(py, 256);
-  if (g.H != 512 || pc.n != 512 || py.n != g.h || g.h > 256 || chunkY < 1) return hipErrorNotSupported;
-  const size_t lds = sizeof(float2) * (512 + (size_t)g.h + (size_t)kC512Cols * std::max(kC512XS, pad_stride(g.h)));
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_col512_window_iy),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if ((g.H != 256 && g.H != 512) || pc.n != g.H || py.n != g.h || g.h > 256 || chunkY < 1)
+    return hipErrorNotSupported;
+  const size_t lds = sizeof(float2) * ((size_t)g.H + g.h + (size_t)kC512Cols * std::max(kC512XS, pad_stride(g.h)));
+  const void* kern = g.H == 512 ? reinterpret_cast<const void*>(k_col_reg_window_iy<512>)
+                                : reinterpret_cast<const void*>(k_col_reg_window_iy<256>);
+  hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   return for_frame_chunks(g.B, [&](int b0, int nb) {
     Geometry gc = g;
     gc.b0 = b0;
-    k_col512_window_iy<<<dim3((g.w + kC512Cols - 1) / kC512Cols, g.P, nb), 256, lds, s>>>(
-        gc, py, pc.tw, rowspec, factor, win, chunkY, cal, peak);
+    const dim3 grid((g.w + kC512Cols - 1) / kC512Cols, g.P, nb);
+    if (g.H == 512)
+      k_col_reg_window_iy<512><<<grid, 256, lds, s>>>(gc, py, pc.tw, rowspec, factor, win, chunkY, cal, peak);
+    else
+      k_col_reg_window_iy<256><<<grid, 256, lds, s>>>(gc, py, pc.tw, rowspec, factor, win, chunkY, cal, peak);
     return hipGetLastError();
   });
 }

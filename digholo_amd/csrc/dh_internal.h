@@ -137,13 +137,13 @@ hipError_t launch_ifft_rows(const Geometry& g, const FFTPlan& pl, float2* win, h
 // + reference wave.  staged_ip_supported says whether the three apply.
 bool staged_ip_supported(const Geometry& g, const FFTPlan& row, const FFTPlan& col, const FFTPlan& ix,
                          const FFTPlan& iy);
-// 256-point rows (512-wide FFT windows) in registers; the staged row kernel's
-// output.  tw: the 256 transform twiddles.
-hipError_t launch_row_fft256(const Geometry& g, const float2* tw, const float2* twW, const void* frames,
-                             int frame_is_u16, float2* rowspec, hipStream_t s);
-// Column stage of 512-row FFT windows with the column transform in
+// 128- and 256-point rows (256- and 512-wide FFT windows) in registers; the
+// staged row kernel's output.  hipErrorNotSupported for other lengths.
+hipError_t launch_row_fft_reg(const Geometry& g, const FFTPlan& pl, const float2* twW, const void* frames,
+                              int frame_is_u16, float2* rowspec, hipStream_t s);
+// Column stage of 256- and 512-row FFT windows with the column transform in
 // registers (h <= 256); hipErrorNotSupported for other shapes.
-hipError_t launch_col512_window_iy(const Geometry& g, const FFTPlan& pc, const FFTPlan& py, const float2* rowspec,
+hipError_t launch_col_reg_window_iy(const Geometry& g, const FFTPlan& pc, const FFTPlan& py, const float2* rowspec,
                                    const float2* factor, float2* win, const BatchCal& cal, hipStream_t s,
                                    unsigned* peak);
 // Inverse transform along x in registers for w = 80 or 160, then the

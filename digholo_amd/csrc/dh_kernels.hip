@@ -831,7 +831,7 @@ bool staged_ip_supported(const Geometry& g, const FFTPlan& row, const FFTPlan& c
 
 hipError_t launch_row_fft_ip(const Geometry& g, const FFTPlan& pl, const float2* twW, const void* frames,
                              int frame_is_u16, float2* rowspec, hipStream_t s) {
-  if (pl.n == 256) return launch_row_fft256(g, pl.tw, twW, frames, frame_is_u16, rowspec, s);
+  if (pl.n == 128 || pl.n == 256) return launch_row_fft_reg(g, pl, twW, frames, frame_is_u16, rowspec, s);
   const int RB = ip_rows_per_block(pl, g.H, kRowThreads);
   if (RB < 1) return hipErrorNotSupported;
   const size_t lds = sizeof(float2) * ((size_t)pl.n + g.w + (size_t)pad_stride(pl.n) * RB);
@@ -860,8 +860,8 @@ hipError_t launch_row_fft_ip(const Geometry& g, const FFTPlan& pl, const float2*
 hipError_t launch_col_window_iy(const Geometry& g, const FFTPlan& pc, const FFTPlan& py, const float2* rowspec,
                                 const float2* factor, float2* win, const BatchCal& cal, hipStream_t s,
                                 unsigned* peak) {
-  if (pc.n == 512 && g.H == 512 && py.n == g.h && g.h <= 256 && ip_chunk(py, 256) >= 1)
-    return launch_col512_window_iy(g, pc, py, rowspec, factor, win, cal, s, peak);
+  if ((g.H == 256 || g.H == 512) && pc.n == g.H && py.n == g.h && g.h <= 256 && ip_chunk(py, 256) >= 1)
+    return launch_col_reg_window_iy(g, pc, py, rowspec, factor, win, cal, s, peak);
   const int CB = ip_cols_per_block(g, pc, py);
   if (CB < 1 || py.n != g.h) return hipErrorNotSupported;
   const size_t lds = sizeof(float2) * ((size_t)g.H + g.h + (size_t)pad_stride(g.H) * CB);
