@@ -2192,13 +2192,17 @@ __global__ __launch_bounds__(256) void k_ifft_rows_reg(Geometry g, const float2*
 
 hipError_t launch_ifft_rows_reg(const Geometry& g, const FFTPlan& pl, const float2* win, const float2* ref,
                                 float2* fields, hipStream_t s, unsigned* peak) {
-  if (pl.n != g.w || (g.w != 80 && g.w != 160)) return hipErrorNotSupported;
+  if (pl.n != g.w || !ifft_rows_reg_width(g.w)) return hipErrorNotSupported;
   return for_frame_chunks(g.B, [&](int b0, int nb) {
     Geometry gc = g;
     gc.b0 = b0;
     const dim3 grid((g.h + 15) / 16, g.P, nb);
-    if (g.w == 160) k_ifft_rows_reg<10><<<grid, 256, 0, s>>>(gc, pl.tw, win, ref, fields, peak);
-    else k_ifft_rows_reg<5><<<grid, 256, 0, s>>>(gc, pl.tw, win, ref, fields, peak);
+    switch (g.w) {
+      case 48: k_ifft_rows_reg<3><<<grid, 256, 0, s>>>(gc, pl.tw, win, ref, fields, peak); break;
+      case 64: k_ifft_rows_reg<4><<<grid, 256, 0, s>>>(gc, pl.tw, win, ref, fields, peak); break;
+      case 80: k_ifft_rows_reg<5><<<grid, 256, 0, s>>>(gc, pl.tw, win, ref, fields, peak); break;
+      default: k_ifft_rows_reg<10><<<grid, 256, 0, s>>>(gc, pl.tw, win, ref, fields, peak); break;
+    }
     return hipGetLastError();
   });
 }

@@ -146,8 +146,9 @@ hipError_t launch_row_fft_reg(const Geometry& g, const FFTPlan& pl, const float2
 hipError_t launch_col_reg_window_iy(const Geometry& g, const FFTPlan& pc, const FFTPlan& py, const float2* rowspec,
                                    const float2* factor, float2* win, const BatchCal& cal, hipStream_t s,
                                    unsigned* peak);
-// Inverse transform along x in registers for w = 80 or 160, then the
+// Inverse transform along x in registers for w = 48, 64, 80 or 160, then the
 // reference wave; hipErrorNotSupported for other widths.
+inline bool ifft_rows_reg_width(int w) { return w == 48 || w == 64 || w == 80 || w == 160; }
 hipError_t launch_ifft_rows_reg(const Geometry& g, const FFTPlan& pl, const float2* win, const float2* ref,
                                 float2* fields, hipStream_t s, unsigned* peak);
 hipError_t launch_row_fft_ip(const Geometry& g, const FFTPlan& pl, const float2* twW, const void* frames,

@@ -888,7 +888,7 @@ hipError_t launch_col_window_iy(const Geometry& g, const FFTPlan& pc, const FFTP
 
 hipError_t launch_ifft_rows_tilt(const Geometry& g, const FFTPlan& pl, const float2* win, const float2* ref,
                                  float2* fields, hipStream_t s, unsigned* peak) {
-  if (pl.n == g.w && (g.w == 80 || g.w == 160)) return launch_ifft_rows_reg(g, pl, win, ref, fields, s, peak);
+  if (pl.n == g.w && ifft_rows_reg_width(g.w)) return launch_ifft_rows_reg(g, pl, win, ref, fields, s, peak);
   const int RB = ip_rows_per_block(pl, g.h);
   if (RB < 1 || pl.n != g.w) return hipErrorNotSupported;
   const size_t lds = sizeof(float2) * ((size_t)pl.n + (size_t)pad_stride(pl.n) * RB);
